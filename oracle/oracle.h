@@ -1,0 +1,161 @@
+/*
+ * oracle.h -- CPU restatement of mobydiff's time-step hot path (TEST INFRASTRUCTURE).
+ *
+ * This header and oracle.c are the correctness oracle: a plain-C, FP64,
+ * OpenMP restatement of the reference algorithm.  Only the test suite, the
+ * smoke check and the CPU-baseline leg of the benchmark may load it, and only
+ * as the checker.  The product (the HIP library in mobydiff_amd/csrc) never
+ * links, loads or calls anything here.
+ *
+ * Storage convention (same as the reference, column-major Fortran):
+ *   ghosted array q(0:nx+1, 0:ny+1, 0:nz+1)  ->  q[i + (nx+2)*(j + (ny+2)*k)]
+ *   interior array r(1:nx, 1:ny, 1:nz)       ->  r[(i-1) + nx*((j-1) + ny*(k-1))]
+ *   FFT rhs(1:nx, 1:nz, 1:ny)               ->  rhs[(i-1) + nx*((k-1) + nz*(j-1))]
+ *   half spectrum hat(1:nxh, 1:nz, 1:ny)     ->  interleaved complex, kx fastest
+ *   BC table bcType(1:3, 0:1, 0:3)           ->  t[(dir-1) + 3*side + 6*var]
+ *
+ * Pinning status of each function is listed in oracle.c's header.
+ */
+#ifndef MOBYDIFF_ORACLE_H
+#define MOBYDIFF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nx, ny, nz;
+    double lx, ly, lz;
+    double dx, dy, dz;
+    double re;
+    double forcing[3];
+    int periodic[3];          /* 0/1 per direction x,y,z */
+    int bc_type[24];          /* bcType(dir,side,var), 0 Dirichlet, 1 Neumann */
+    double bc_value[24];
+    /* stretched wall-normal grid (NULL => uniform dy, the reference grid) */
+    const double *ihc;        /* 1/(cell height), j = 0..ny+1 */
+    const double *ihv;        /* 1/(centre distance across v-face j), j = 0..ny+1 */
+} mdo_grid;
+
+typedef struct {
+    int kind;                 /* 0: reference bottom sine bump; 1: two-sided wavy channel */
+    int second_order;         /* 1: USE_IBM_SECONDORDER distance correction */
+    /* kind 0 (ibm.f90:45-50) */
+    int n_wave_x;
+    double amp_x, phase_x;
+    /* kind 1: walls at y_c +- (1 - A cos(2 pi x / lambda)), y_c = ly/2 */
+    double A, lambda, half_height;
+} mdo_geometry;
+
+typedef struct {
+    int iterations;           /* full red+black iterations performed */
+    int converged;
+    double div_inf;           /* final ||div||_inf */
+    double div0_inf;          /* initial ||div||_inf */
+} mdo_sor_stats;
+
+#define MDO_BCI(dir, side, var) (((dir) - 1) + 3 * (side) + 6 * (var))
+
+/* grid helpers */
+void mdo_finalize_grid(mdo_grid *g);
+void mdo_stretched_metrics(int ny, double ly, double beta,
+                           double *yc, double *yv, double *ihc, double *ihv);
+
+/* RK3 and momentum (step.f90) */
+extern const double mdo_rk_alpha[3], mdo_rk_beta[3], mdo_rk_gamma[3];
+void mdo_momentum(const mdo_grid *g,
+                  const double *un, const double *vn, const double *wn, const double *pn,
+                  double *us, double *vs, double *ws,
+                  double *oru, double *orv, double *orw,
+                  const double *cu, const double *cv, const double *cw,
+                  double dta, double dtb, double dtg);
+double mdo_get_cfl(const mdo_grid *g, const double *un, const double *vn, const double *wn);
+
+/* boundary.f90 */
+void mdo_apply_bc_fft(const mdo_grid *g, double *un, double *vn, double *wn,
+                      double *us, double *vs, double *ws, double *pn);
+void mdo_apply_bc_redblack(const mdo_grid *g, double *un, double *vn, double *wn, double *pn);
+
+/* FFT + tridiagonal path (pressure_solver.f90, pressure_fft.f90, pressure_backend_fftw.f90) */
+void mdo_build_fft_rhs(const mdo_grid *g, const double *us, const double *vs, const double *ws,
+                       double dtg, double *rhs);
+void mdo_fft_r2c_xz(const mdo_grid *g, const double *rhs, double *hat);
+void mdo_fft_c2r_xz(const mdo_grid *g, const double *hat, double *rhs);
+void mdo_tridiag_coefficients(const mdo_grid *g, double *cp, double *den_inv, double *a_row);
+void mdo_tridiag_solve(const mdo_grid *g, const double *cp, const double *den_inv,
+                       const double *a_row, double *hat, double *dp);
+void mdo_fft_pressure_correction(const mdo_grid *g, const double *phi,
+                                 const double *us, const double *vs, const double *ws,
+                                 const double *cu, const double *cv, const double *cw, double dtg,
+                                 double *un, double *vn, double *wn, double *pn);
+void mdo_projection_fft(const mdo_grid *g, double dtg,
+                        const double *us, const double *vs, const double *ws,
+                        const double *cu, const double *cv, const double *cw,
+                        double *un, double *vn, double *wn, double *pn);
+
+/* red-black SOLA iteration (pressure_redblack.f90) */
+void mdo_sola_redblack(const mdo_grid *g, int niter, double sor, double dtg,
+                       const double *cu, const double *cv, const double *cw,
+                       const double *us, const double *vs, const double *ws,
+                       double *un, double *vn, double *wn, double *pn);
+
+/* pressure-correction form of the same SOR, run to a divergence tolerance */
+void mdo_sor_div_rhs(const mdo_grid *g, const double *un, const double *vn, const double *wn,
+                     double *rhs);
+void mdo_sor_sweep(const mdo_grid *g, int color, double omega, double dtg,
+                   const double *cu, const double *cv, const double *cw,
+                   const double *rhs, double *phi);
+double mdo_sor_residual_inf(const mdo_grid *g, double dtg,
+                            const double *cu, const double *cv, const double *cw,
+                            const double *rhs, const double *phi);
+void mdo_sor_apply(const mdo_grid *g, double dtg,
+                   const double *cu, const double *cv, const double *cw,
+                   const double *phi, double *un, double *vn, double *wn, double *pn);
+void mdo_projection_sor(const mdo_grid *g, double omega, double tol, int max_iter, int check_every,
+                        double dtg, const double *cu, const double *cv, const double *cw,
+                        const double *us, const double *vs, const double *ws,
+                        double *un, double *vn, double *wn, double *pn,
+                        double *phi_work, double *rhs_work, mdo_sor_stats *st);
+
+/* divergence norm, bulk velocity */
+double mdo_div_inf(const mdo_grid *g, const double *un, const double *vn, const double *wn);
+double mdo_bulk_u(const mdo_grid *g, const double *u, double height);
+double mdo_flowrate_shift(const mdo_grid *g, double *us, const double *cu, double dtg,
+                          double target, double height);
+
+/* immersed boundary (ibm.f90) */
+int mdo_is_in_body(const mdo_grid *g, const mdo_geometry *geo, const double x[3]);
+void mdo_bisection(const mdo_grid *g, const mdo_geometry *geo, const double xa_in[3], double xb[3]);
+void mdo_set_ibm_coeff(const mdo_grid *g, const mdo_geometry *geo, int dix, int diy, int diz,
+                       double *coeff);
+
+/* initial condition: Poiseuille + counter-based uniform perturbation */
+uint64_t mdo_splitmix64(uint64_t x);
+double mdo_uniform01(uint32_t seed, uint32_t comp, uint64_t index);
+void mdo_initial_field(const mdo_grid *g, const mdo_geometry *geo, int use_geometry,
+                       const double *yc_box, const double *yv_box,
+                       uint32_t seed, double amp, double *us, double *vs, double *ws);
+
+/* complete RK3 steps */
+typedef struct {
+    int solver;               /* 0 FFT, 1 SOLA fixed niter, 2 SOR to tolerance */
+    int niter;
+    double omega, tol;
+    int max_iter, check_every;
+    int flowrate;             /* 1: constant bulk velocity */
+    double ub_target, height;
+} mdo_step_opts;
+
+void mdo_rk3_step(const mdo_grid *g, const mdo_step_opts *o, double dt,
+                  double *un, double *vn, double *wn, double *pn,
+                  double *us, double *vs, double *ws,
+                  double *oru, double *orv, double *orw,
+                  const double *cu, const double *cv, const double *cw,
+                  double *phi_work, double *rhs_work, mdo_sor_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
