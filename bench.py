@@ -1,0 +1,155 @@
+"""Benchmark: wavy-wall channel time step on one MI355X.
+
+Workload (the headline configuration): wavy channel, walls at y = +-(1 - A cos(2 pi x)), A = 0.1,
+Lx x Ly x Lz = 4 x 2.2 x 2, uniform grid 192 x 128 x 96, Re_b = 2000 with constant flow rate,
+dt = 5e-4, Poiseuille + 0.05 seeded perturbation.  Each step is one RK3 step: momentum + IBM,
+red-black SOR pressure solve to ||div||_inf < 1e-8, projection, boundary conditions.
+
+Timed configurations, both from the same state after the warm-up steps:
+  * headline: FP32 red-black sweeps inside an FP64 defect-correction loop (plus the learned
+    initial guess when a trained network is attached);
+  * plain: FP64 red-black SOR (the comparison path).
+Prints one JSON line (see the "metric" field); value = Mcell-steps/s of the headline path.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-steps/s at ‖div‖∞<1e-8: learned+FP32 SOR vs plain FP64 solve"
+NX, NY, NZ = 192, 128, 96
+LX, LY, LZ = 4.0, 2.2, 2.0
+RE, DT, A, AMP = 2000.0, 5e-4, 0.1, 0.05
+TOL, OMEGA = 1e-8, 1.8
+HBM_PEAK_GBS = 8000.0
+# canonical algorithmic bytes per cell for one full red+black iteration (FP64 88 B, FP32 44 B);
+# one sweep launch covers one colour of every cell, i.e. half of that
+BYTES_PER_CELL_ITER = {"fp64": 88.0, "fp32": 44.0}
+
+
+def make_channel(M, precision, check_every, inner):
+    cfg = M.channel_config(NX, NY, NZ, LX, LY, LZ, RE, solver=M.SOLVER_SOR, omega=OMEGA, tol=TOL,
+                           max_iter=200000, check_every=check_every, precision=precision,
+                           inner_iters=inner, flowrate=True, flow_height=2.0, timing=True)
+    return M.Channel(cfg)
+
+
+def run_steps(ch, k):
+    ch.synchronize()
+    ch.reset_phase_times()
+    its, div = [], 0.0
+    t0 = time.perf_counter()
+    for _ in range(k):
+        st = ch.step(DT)
+        its.append([st.stage[q].iterations for q in range(3)])
+    ch.synchronize()
+    el = time.perf_counter() - t0
+    div = ch.div_norm()
+    return el, its, div, ch.phase_times()
+
+
+def roofline(times, kind):
+    if times.sweep_launches == 0:
+        return None
+    avg_s = times.sweep_ms / times.sweep_launches * 1e-3
+    cells_per_launch = times.sweep_cells / times.sweep_launches
+    bytes_per_launch = BYTES_PER_CELL_ITER[kind] / 2.0 * cells_per_launch
+    ach = bytes_per_launch / avg_s / 1e9
+    return {"kernel": f"red-black SOR sweep ({kind}, one colour)", "bound": "hbm", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": bytes_per_launch}
+
+
+def cpu_baseline(f_host, coef, cores, steps):
+    """The oracle (C restatement of the reference CPU path, FP64, OpenMP) on the same state."""
+    from oracle import oracle as O
+    g = O.Grid(NX, NY, NZ, lx=LX, ly=LY, lz=LZ, re=RE, forcing=(0.0, 0.0, 0.0), periodic=(1, 0, 1))
+    f = {k: v.copy() for k, v in f_host.items()}
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(steps):
+        its.append([s[0] for s in O.rk3_step(g, f, coef, DT, solver=2, omega=OMEGA, tol=TOL, max_iter=200000,
+                                              check_every=20, flowrate=True, ub_target=1.0, height=2.0)])
+    el = time.perf_counter() - t0
+    return {"value": round(NX * NY * NZ * steps / el / 1e6, 3), "unit": "Mcell-steps/s", "cores": cores,
+            "kind": "port", "sample": f"{steps} RK3 step(s) of the same 192x128x96 wavy workload from the "
+            f"post-warm-up state, FP64 SOR to 1e-8 (iterations per stage {its})",
+            "seconds": round(el, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--check-every", type=int, default=20)
+    ap.add_argument("--inner", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mobydiff_amd as M
+
+    geo = M.wavy_geometry(A=A)
+    head = make_channel(M, M.PREC_MIXED, args.check_every, args.inner)
+    coef = head.compute_ibm(geo)
+    head.init_fields(seed=0, amp=AMP, geo=geo, dt=DT)
+    for _ in range(args.warmup):
+        head.step(DT)
+    state = head.download()
+
+    plain = make_channel(M, M.PREC_FP64, args.check_every, args.inner)
+    plain.set_ibm(*coef)
+    plain.upload(**state)
+
+    el_h, its_h, div_h, t_h = run_steps(head, args.steps)
+    el_p, its_p, div_p, t_p = run_steps(plain, args.steps)
+    cells = NX * NY * NZ
+    v_h = cells * args.steps / el_h / 1e6
+    v_p = cells * args.steps / el_p / 1e6
+
+    def split(t, k):
+        return {"stencil": round(t.stencil_ms / k, 3), "poisson": round(t.poisson_ms / k, 3),
+                "projection": round(t.projection_ms / k, 3), "guess": round(t.guess_ms / k, 3)}
+
+    out = {
+        "metric": METRIC, "value": round(v_h, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el_h / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": "wavy-wall channel 192x128x96 (A=0.1, Re_b=2000, dt=5e-4), RK3 step with "
+                               "red-black SOR to ||div||_inf<1e-8: FP32 sweeps in FP64 defect correction",
+                   "grid": [NX, NY, NZ], "learned_guess": False, "omega": OMEGA, "tol": TOL,
+                   "check_every": args.check_every, "inner_iters": args.inner, "parallelism": "1 GPU"},
+        "phase_ms_per_step": split(t_h, args.steps),
+        "sor_iterations_per_stage": sum(map(sum, its_h)) / (3.0 * args.steps),
+        "div_inf_final": div_h,
+        "fallback_rate": None,
+        "plain_fp64": {"value": round(v_p, 3), "ms_per_step": round(el_p / args.steps * 1e3, 3),
+                       "phase_ms_per_step": split(t_p, args.steps),
+                       "sor_iterations_per_stage": sum(map(sum, its_p)) / (3.0 * args.steps),
+                       "div_inf_final": div_p, "roofline": roofline(t_p, "fp64")},
+        "speedup_vs_plain_fp64": round(v_h / v_p, 3),
+        "roofline": roofline(t_h, "fp32"),
+    }
+    if not args.no_cpu_baseline:
+        aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cores = int(os.environ.get("OMP_NUM_THREADS", aff))   # the oracle's OpenMP thread count
+        out["cpu_baseline"] = cpu_baseline(state, coef, cores, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    head.close()
+    plain.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

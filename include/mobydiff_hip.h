@@ -1,0 +1,160 @@
+/*
+ * mobydiff_hip.h -- C ABI of the MI355X time-step library (libmobydiff_hip.so).
+ *
+ * This is the drop-in boundary.  Each entry point replaces one module procedure that the
+ * reference driver (main.f90) calls, so a thin Fortran wrapper with bind(C) interfaces can keep
+ * the reference call surface unchanged.  All functions return 0 on success and a non-zero code
+ * on failure; mdx_last_error() then returns a message (the Fortran wrappers turn it into
+ * `error stop`, as cufft_bindings.f90:56-74 does for cuFFT/CUDA codes).
+ *
+ * Arrays cross the boundary as plain host pointers in the reference's layout:
+ *   ghosted field   q(0:nx+1,0:ny+1,0:nz+1)  -> q[i + (nx+2)*(j + (ny+2)*k)]   (FP64)
+ *   interior field  r(1:nx,1:ny,1:nz)        -> r[(i-1) + nx*((j-1) + ny*(k-1))] (FP64)
+ *   BC table bcType(1:3,0:1,0:3) / bcValue   -> t[(dir-1) + 3*side + 6*var]
+ * The library owns all device memory from mdx_create to mdx_destroy (the persistent OpenMP
+ * mapping of gpu_runtime.f90:22-48 in the reference).  Calls are ordered on one HIP stream; any
+ * call that returns host-visible data synchronises that stream first.
+ */
+#ifndef MOBYDIFF_HIP_H
+#define MOBYDIFF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdx_handle mdx_handle;
+
+enum { MDX_OK = 0, MDX_ERR_ARG = 1, MDX_ERR_HIP = 2, MDX_ERR_UNSUPPORTED = 3, MDX_ERR_STATE = 4,
+       MDX_ERR_NOT_CONVERGED = 5 };
+
+enum { MDX_SOLVER_FFT = 0,   /* FFT(x,z) + Thomas(y), pressure_solver.f90:48-63 (FFT build)      */
+       MDX_SOLVER_SOLA = 1,  /* red-black SOLA, fixed niter, pressure_redblack.f90:27-254        */
+       MDX_SOLVER_SOR = 2 }; /* same red-black SOR run to ||div||_inf < tol (pressure form)      */
+
+enum { MDX_PREC_FP64 = 0, MDX_PREC_MIXED = 1 };   /* MIXED: FP32 sweeps in FP64 defect correction */
+
+enum { MDX_GUESS_PREVIOUS = 0, MDX_GUESS_LEARNED = 1 };
+
+typedef struct {
+    /* grid_type (init.f90:5-12) */
+    int32_t nx, ny, nz;
+    double lx, ly, lz;
+    double re;
+    double forcing[3];
+    /* boundary_type (boundary.f90:6-10); periodic as 0/1 */
+    int32_t periodic[3];
+    int32_t bc_type[24];
+    double bc_value[24];
+    /* wall-normal grid: 0 = uniform dy (reference); >0 = tanh stretching parameter beta */
+    double stretch_beta;
+    /* pressure solver (pressure_workspace.f90:5-24) */
+    int32_t solver;          /* MDX_SOLVER_* */
+    int32_t niter;           /* SOLA iterations (reference default 3) */
+    double omega;            /* SOR relaxation (reference default 1.5) */
+    double tol;              /* MDX_SOLVER_SOR: ||div||_inf tolerance */
+    int32_t max_iter;        /* MDX_SOLVER_SOR: iteration cap (then MDX_ERR_NOT_CONVERGED unless 0) */
+    int32_t check_every;     /* MDX_SOLVER_SOR: iterations between residual checks */
+    int32_t precision;       /* MDX_PREC_* */
+    int32_t inner_iters;     /* MDX_PREC_MIXED: FP32 iterations per defect-correction cycle */
+    int32_t guess;           /* MDX_GUESS_* */
+    /* constant flow rate: bulk velocity held at ub_target over a channel of height flow_height */
+    int32_t flowrate;
+    double ub_target, flow_height;
+    /* collect per-phase GPU times (stencil / Poisson / projection) with HIP events */
+    int32_t timing;
+} mdx_config;
+
+typedef struct {
+    int32_t kind;            /* 0: reference bottom sine bump (ibm.f90:84-99); 1: two-sided wavy channel */
+    int32_t second_order;    /* USE_IBM_SECONDORDER (CMakeLists.txt:11, default ON) */
+    int32_t n_wave_x;        /* kind 0 */
+    double amp_x, phase_x;   /* kind 0; amp_x <= 0 selects the reference's 5*dy (ibm.f90:47) */
+    double A, lambda, half_height;   /* kind 1: walls at ly/2 +- (half_height - A cos(2 pi x / lambda)) */
+} mdx_geometry;
+
+typedef struct {
+    int32_t iterations;      /* full red+black iterations (SOLA: niter) */
+    int32_t converged;
+    int32_t fallback;        /* learned guess rejected (initial residual worse than the previous-step guess) */
+    int32_t checks;          /* residual evaluations */
+    double div0_inf;         /* ||div||_inf of the initial guess */
+    double div_inf;          /* ||div||_inf after the solve (from the final residual) */
+    double guess_div_inf;    /* ||div||_inf of the learned guess (when used) */
+} mdx_solve_stats;
+
+typedef struct {
+    mdx_solve_stats stage[3];
+    double ub;               /* bulk velocity after the step (flowrate on) */
+} mdx_step_stats;
+
+typedef struct {
+    double stencil_ms;       /* momentum + flow-rate shift + boundary conditions */
+    double poisson_ms;       /* divergence RHS + pressure solve (FFT/tridiag or SOR sweeps + checks) */
+    double projection_ms;    /* velocity/pressure correction */
+    double guess_ms;         /* learned initial guess (network inference) */
+    int64_t stages;
+    double sweep_ms;         /* red-black sweep kernels alone (event-bracketed batches) */
+    int64_t sweep_launches;  /* colour sweeps in those batches (one colour over all cells each) */
+    int64_t sweep_cells;     /* cells covered by those batches: launches x nx ny nz */
+} mdx_phase_times;
+
+/* error text of the last failing call on this thread */
+const char *mdx_last_error(void);
+
+/* lifecycle: enter_field_data / init_pressure_solver (gpu_runtime.f90:22-34, pressure_solver.f90:35-46) */
+int mdx_create(const mdx_config *cfg, mdx_handle **out);
+/* exit_field_data / destroy_pressure_solver (gpu_runtime.f90:36-48, pressure_solver.f90:65-74) */
+int mdx_destroy(mdx_handle *h);
+int mdx_synchronize(mdx_handle *h);
+
+/* `target update to/from` of the field_type arrays (io.f90:124-127,161-164); NULL skips an array */
+int mdx_upload_fields(mdx_handle *h, const double *un, const double *vn, const double *wn,
+                      const double *pn, const double *us, const double *vs, const double *ws,
+                      const double *oldrhsu, const double *oldrhsv, const double *oldrhsw);
+int mdx_download_fields(mdx_handle *h, double *un, double *vn, double *wn, double *pn,
+                        double *us, double *vs, double *ws,
+                        double *oldrhsu, double *oldrhsv, double *oldrhsw);
+
+/* enter_ibm_data (ibm.f90:57-68): upload coef_u/v/w (ghosted); all NULL disables the IBM */
+int mdx_set_ibm(mdx_handle *h, const double *coef_u, const double *coef_v, const double *coef_w);
+/* init_ibm + set_ibm_coeff x3 + enter_ibm_data (ibm.f90:41-55,133-187; main.f90:57-61),
+ * computed by the library on the host and uploaded; the arrays are also returned when non-NULL */
+int mdx_compute_ibm(mdx_handle *h, const mdx_geometry *geo, double *coef_u, double *coef_v,
+                    double *coef_w);
+
+/* step.f90:27 momentum(f, g, dt_alpha, dt_beta, dt_gamma, ibm, bc) */
+int mdx_momentum(mdx_handle *h, double dt_alpha, double dt_beta, double dt_gamma);
+/* boundary.f90:23 apply_bc(f, g) -- FFT build (n- and *-fields) */
+int mdx_apply_bc(mdx_handle *h);
+/* boundary.f90:173 apply_bc_redblack(f, g, bc) */
+int mdx_apply_bc_redblack(mdx_handle *h);
+/* pressure_solver.f90:48 pressure_projection(ps, f, g, dt_gamma, ibm, bc); stats may be NULL */
+int mdx_pressure_projection(mdx_handle *h, double dt_gamma, mdx_solve_stats *stats);
+/* step.f90:210 get_cfl(f, g) */
+int mdx_get_cfl(mdx_handle *h, double *cfl);
+
+/* ||div u||_inf over interior cells (FP64) */
+int mdx_div_norm(mdx_handle *h, double *div_inf);
+/* constant flow rate: bulk-velocity correction of u* for the stage with dt_gamma; returns du */
+int mdx_flowrate_shift(mdx_handle *h, double dt_gamma, double *du);
+/* bulk velocity of u over the channel height */
+int mdx_bulk_velocity(mdx_handle *h, double *ub);
+/* Poiseuille + seeded uniform perturbation of amplitude amp, projected divergence-free, p = 0.
+ * geo selects the wall geometry for the profile and masks (NULL: flat channel). */
+int mdx_init_fields(mdx_handle *h, uint32_t seed, double amp, const mdx_geometry *geo, double dt);
+/* one RK3 step (main.f90:66-89): 3 x {momentum, [flow rate], [apply_bc], projection, bc} */
+int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats);
+
+/* per-phase GPU time accumulated since the last reset (cfg.timing must be set) */
+int mdx_get_phase_times(mdx_handle *h, mdx_phase_times *t);
+int mdx_reset_phase_times(mdx_handle *h);
+
+/* wall-normal coordinates of cell centres and v-faces (box coordinates, length ny+2 each) */
+int mdx_get_y_coordinates(mdx_handle *h, double *yc, double *yv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,257 @@
+"""mobydiff_amd -- MI355X (gfx950) time-step hot path of the mobydiff channel-flow solver.
+
+The compute path is ``csrc/libmobydiff_hip.so`` (hand-written HIP), reached through its C ABI
+(``include/mobydiff_hip.h``).  This module is the Python front end of that ABI: ctypes
+structures mirroring the header and a ``Channel`` object whose methods mirror the reference's
+module procedures (``momentum``, ``apply_bc``, ``apply_bc_redblack``, ``pressure_projection``,
+``get_cfl``; step.f90, boundary.f90, pressure_solver.f90).  There is no CPU fallback: if the
+library is not built, or no HIP device is present, loading fails with an error.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "csrc", "libmobydiff_hip.so")
+
+SOLVER_FFT, SOLVER_SOLA, SOLVER_SOR = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+GUESS_PREVIOUS, GUESS_LEARNED = 0, 1
+
+_PD = C.POINTER(C.c_double)
+
+
+class Config(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("re", C.c_double), ("forcing", C.c_double * 3),
+                ("periodic", C.c_int32 * 3), ("bc_type", C.c_int32 * 24),
+                ("bc_value", C.c_double * 24), ("stretch_beta", C.c_double),
+                ("solver", C.c_int32), ("niter", C.c_int32), ("omega", C.c_double),
+                ("tol", C.c_double), ("max_iter", C.c_int32), ("check_every", C.c_int32),
+                ("precision", C.c_int32), ("inner_iters", C.c_int32), ("guess", C.c_int32),
+                ("flowrate", C.c_int32), ("ub_target", C.c_double), ("flow_height", C.c_double),
+                ("timing", C.c_int32)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("second_order", C.c_int32), ("n_wave_x", C.c_int32),
+                ("amp_x", C.c_double), ("phase_x", C.c_double),
+                ("A", C.c_double), ("lam", C.c_double), ("half_height", C.c_double)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("fallback", C.c_int32),
+                ("checks", C.c_int32), ("div0_inf", C.c_double), ("div_inf", C.c_double),
+                ("guess_div_inf", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StepStats(C.Structure):
+    _fields_ = [("stage", SolveStats * 3), ("ub", C.c_double)]
+
+
+class PhaseTimes(C.Structure):
+    _fields_ = [("stencil_ms", C.c_double), ("poisson_ms", C.c_double),
+                ("projection_ms", C.c_double), ("guess_ms", C.c_double), ("stages", C.c_int64)]
+
+
+def bci(d, side, var):
+    """Flat index of bcType(d, side, var) (boundary.f90:8), d = 1..3."""
+    return (d - 1) + 3 * side + 6 * var
+
+
+_lib = None
+
+
+class MdxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mobydiff_hip error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libmobydiff_hip.so; raise if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is not built: run __graft_entry__.build() "
+                               "or make -C mobydiff_amd/csrc")
+        _lib = C.CDLL(LIBPATH)
+        _lib.mdx_last_error.restype = C.c_char_p
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MdxError(rc, lib().mdx_last_error().decode())
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_PD)
+
+
+def channel_config(nx, ny, nz, lx, ly, lz, re, forcing=(0.0, 0.0, 0.0), periodic=(1, 0, 1),
+                   solver=SOLVER_SOR, niter=3, omega=1.8, tol=1e-8, max_iter=200000,
+                   check_every=20, precision=PREC_FP64, inner_iters=20, guess=GUESS_PREVIOUS,
+                   flowrate=False, ub_target=1.0, flow_height=2.0, stretch_beta=0.0,
+                   p_neumann_y=True, timing=False):
+    """Config for a channel: x/z periodic, no-slip y walls (Dirichlet u,v,w = 0), Neumann p on the
+    walls, as tutorials/wavychannel/input.ini sets up."""
+    c = Config()
+    c.nx, c.ny, c.nz = nx, ny, nz
+    c.lx, c.ly, c.lz = lx, ly, lz
+    c.re = re
+    for q in range(3):
+        c.forcing[q] = forcing[q]
+        c.periodic[q] = int(periodic[q])
+    if p_neumann_y:
+        c.bc_type[bci(2, 0, 0)] = 1
+        c.bc_type[bci(2, 1, 0)] = 1
+    c.stretch_beta = stretch_beta
+    c.solver, c.niter, c.omega, c.tol = solver, niter, omega, tol
+    c.max_iter, c.check_every = max_iter, check_every
+    c.precision, c.inner_iters, c.guess = precision, inner_iters, guess
+    c.flowrate, c.ub_target, c.flow_height = int(flowrate), ub_target, flow_height
+    c.timing = int(timing)
+    return c
+
+
+def wavy_geometry(A=0.1, lam=1.0, half_height=1.0, second_order=True):
+    g = Geometry()
+    g.kind, g.second_order, g.A, g.lam, g.half_height = 1, int(second_order), A, lam, half_height
+    return g
+
+
+def bump_geometry(amp_x=0.0, phase_x=0.0, n_wave_x=1, second_order=True):
+    g = Geometry()
+    g.kind, g.second_order, g.n_wave_x, g.amp_x, g.phase_x = 0, int(second_order), n_wave_x, amp_x, phase_x
+    return g
+
+
+RK_ALPHA = (64.0 / 120.0, 50.0 / 120.0, 90.0 / 120.0)   # step.f90:21-23
+RK_BETA = (0.0, -34.0 / 120.0, -50.0 / 120.0)
+RK_GAMMA = (64.0 / 120.0, 16.0 / 120.0, 40.0 / 120.0)
+
+FIELD_NAMES = ("un", "vn", "wn", "pn", "us", "vs", "ws", "oru", "orv", "orw")
+
+
+class Channel:
+    """One device-resident flow state (field_type + ibm_type + pressure_solver_type)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.h = C.c_void_p()
+        _check(lib().mdx_create(C.byref(cfg), C.byref(self.h)))
+        self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
+
+    # shapes (C order, x fastest)
+    @property
+    def gshape(self):
+        return (self.nz + 2, self.ny + 2, self.nx + 2)
+
+    @property
+    def ishape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def close(self):
+        if self.h:
+            lib().mdx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, **f):
+        args = [_p(f.get(k)) for k in FIELD_NAMES]
+        _check(lib().mdx_upload_fields(self.h, *args))
+
+    def download(self, names=FIELD_NAMES):
+        out = {}
+        args = []
+        for k in FIELD_NAMES:
+            if k in names:
+                out[k] = np.zeros(self.ishape if k.startswith("or") else self.gshape)
+                args.append(_p(out[k]))
+            else:
+                args.append(None)
+        _check(lib().mdx_download_fields(self.h, *args))
+        return out
+
+    def set_ibm(self, cu, cv, cw):
+        _check(lib().mdx_set_ibm(self.h, _p(cu), _p(cv), _p(cw)))
+
+    def compute_ibm(self, geo):
+        cu, cv, cw = (np.zeros(self.gshape) for _ in range(3))
+        _check(lib().mdx_compute_ibm(self.h, C.byref(geo), _p(cu), _p(cv), _p(cw)))
+        return cu, cv, cw
+
+    def momentum(self, dta, dtb, dtg):
+        _check(lib().mdx_momentum(self.h, C.c_double(dta), C.c_double(dtb), C.c_double(dtg)))
+
+    def apply_bc(self):
+        _check(lib().mdx_apply_bc(self.h))
+
+    def apply_bc_redblack(self):
+        _check(lib().mdx_apply_bc_redblack(self.h))
+
+    def pressure_projection(self, dtg):
+        st = SolveStats()
+        _check(lib().mdx_pressure_projection(self.h, C.c_double(dtg), C.byref(st)))
+        return st
+
+    def get_cfl(self):
+        v = C.c_double()
+        _check(lib().mdx_get_cfl(self.h, C.byref(v)))
+        return v.value
+
+    def div_norm(self):
+        v = C.c_double()
+        _check(lib().mdx_div_norm(self.h, C.byref(v)))
+        return v.value
+
+    def flowrate_shift(self, dtg):
+        v = C.c_double()
+        _check(lib().mdx_flowrate_shift(self.h, C.c_double(dtg), C.byref(v)))
+        return v.value
+
+    def bulk_velocity(self):
+        v = C.c_double()
+        _check(lib().mdx_bulk_velocity(self.h, C.byref(v)))
+        return v.value
+
+    def init_fields(self, seed, amp, geo=None, dt=1e-3):
+        _check(lib().mdx_init_fields(self.h, C.c_uint32(seed), C.c_double(amp),
+                                     C.byref(geo) if geo is not None else None, C.c_double(dt)))
+
+    def step(self, dt):
+        st = StepStats()
+        _check(lib().mdx_step(self.h, C.c_double(dt), C.byref(st)))
+        return st
+
+    def synchronize(self):
+        _check(lib().mdx_synchronize(self.h))
+
+    def phase_times(self):
+        t = PhaseTimes()
+        _check(lib().mdx_get_phase_times(self.h, C.byref(t)))
+        return t
+
+    def reset_phase_times(self):
+        _check(lib().mdx_reset_phase_times(self.h))
+
+    def y_coordinates(self):
+        yc = np.zeros(self.ny + 2)
+        yv = np.zeros(self.ny + 2)
+        _check(lib().mdx_get_y_coordinates(self.h, _p(yc), _p(yv)))
+        return yc, yv

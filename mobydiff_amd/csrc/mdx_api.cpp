@@ -1,0 +1,839 @@
+// mdx_api.cpp -- the C ABI of libmobydiff_hip: device state, solver dispatch and host-side setup.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "mdx_internal.h"
+
+static thread_local std::string g_last_error;
+
+void mdx_set_error(const std::string &msg) { g_last_error = msg; }
+
+extern "C" const char *mdx_last_error(void) { return g_last_error.c_str(); }
+
+namespace mdx {
+
+GridK grid_k(const mdx_handle *h) {
+    GridK g;
+    g.d = h->d;
+    g.dx = h->dx;
+    g.dy = h->dy;
+    g.dz = h->dz;
+    g.re = h->cfg.re;
+    g.fx = h->cfg.forcing[0];
+    g.fy = h->cfg.forcing[1];
+    g.fz = h->cfg.forcing[2];
+    for (int q = 0; q < 3; ++q) g.per[q] = h->cfg.periodic[q] ? 1 : 0;
+    g.ihc = h->stretched ? h->d_ihc : nullptr;
+    g.ihv = h->stretched ? h->d_ihv : nullptr;
+    return g;
+}
+
+BcK bc_k(const mdx_handle *h) {
+    BcK b;
+    for (int q = 0; q < 24; ++q) {
+        b.type[q] = h->cfg.bc_type[q];
+        b.value[q] = h->cfg.bc_value[q];
+    }
+    return b;
+}
+
+}  // namespace mdx
+
+using namespace mdx;
+
+static const double PI = 3.141592653589793;
+
+#define MDX_CHECK(cond, code, msg)  \
+    do {                            \
+        if (!(cond)) {              \
+            mdx_set_error(msg);     \
+            return code;            \
+        }                           \
+    } while (0)
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// pressure_fft.f90:63,79 divide one_c by a complex denominator; compiled with the scaled C99 Annex G
+// algorithm this rounds as c/(c*c) after scaling c by 2^-logb(c).  Kept identical so the Thomas
+// coefficients match the reference bit for bit.
+static double recip_complex_div(double c) {
+    double lb = std::logb(std::fabs(c));
+    int e = 0;
+    if (std::isfinite(lb)) {
+        e = (int)lb;
+        c = std::scalbn(c, -e);
+    }
+    double denom = c * c;
+    return std::scalbn(c / denom, -e);
+}
+
+// Modified-wavenumber Thomas factorisation per (kx,kz) line (pressure_fft.f90:20-100), generalised
+// to a stretched wall-normal grid.  cp/den_inv layout (nxh, ny, nz), a_row(ny).
+static void tridiag_tables(const mdx_handle *h, std::vector<double> &cp, std::vector<double> &di,
+                           std::vector<double> &ar) {
+    const int nx = h->d.nx, ny = h->d.ny, nz = h->d.nz, nxh = nx / 2 + 1;
+    const double dx = h->dx, dy = h->dy, dz = h->dz;
+    const double dyi2 = 1.0 / (dy * dy);
+    cp.assign((size_t)nxh * ny * nz, 0.0);
+    di.assign((size_t)nxh * ny * nz, 0.0);
+    ar.assign(ny, 0.0);
+    for (int j = 1; j <= ny; ++j) ar[j - 1] = h->stretched ? h->ihv[j] * h->ihc[j] : dyi2;
+    for (int ikx = 1; ikx <= nxh; ++ikx)
+        for (int ikz = 1; ikz <= nz; ++ikz) {
+            const int kx = ikx - 1;
+            const double s = std::sin(PI * (double)kx / (double)nx);
+            const double kx_ph = (4.0 / (dx * dx)) * (s * s);
+            const int kz = (ikz <= nz / 2 + 1) ? ikz - 1 : ikz - nz - 1;
+            const double sz = std::sin(PI * (double)kz / (double)nz);
+            const double kz_ph = (4.0 / (dz * dz)) * (sz * sz);
+            const double k_tot = kx_ph + kz_ph;
+            const bool zero_mode = (kx == 0 && kz == 0);
+            auto CI = [&](int jj) { return (size_t)(ikx - 1) + (size_t)nxh * ((size_t)(jj - 1) + (size_t)ny * (ikz - 1)); };
+            if (!h->stretched) {
+                if (zero_mode) {
+                    di[CI(1)] = 1.0;
+                    cp[CI(1)] = 0.0;
+                } else {
+                    double b = -dyi2 - k_tot;
+                    di[CI(1)] = recip_complex_div(b);
+                    cp[CI(1)] = dyi2 * di[CI(1)];
+                }
+                for (int jj = 2; jj <= ny; ++jj) {
+                    double b, c;
+                    if (jj < ny) { b = -2.0 * dyi2 - k_tot; c = dyi2; }
+                    else { b = -dyi2 - k_tot; c = 0.0; }
+                    double denom = b - dyi2 * cp[CI(jj - 1)];
+                    di[CI(jj)] = recip_complex_div(denom);
+                    cp[CI(jj)] = (jj < ny) ? c * di[CI(jj)] : 0.0;
+                }
+            } else {
+                const double *ihc = h->ihc.data(), *ihv = h->ihv.data();
+                if (zero_mode) {
+                    di[CI(1)] = 1.0;
+                    cp[CI(1)] = 0.0;
+                } else {
+                    double c1 = ihv[2] * ihc[1];
+                    double b = -c1 - k_tot;
+                    di[CI(1)] = recip_complex_div(b);
+                    cp[CI(1)] = c1 * di[CI(1)];
+                }
+                for (int jj = 2; jj <= ny; ++jj) {
+                    double aj = ihv[jj] * ihc[jj];
+                    double cj = (jj < ny) ? ihv[jj + 1] * ihc[jj] : 0.0;
+                    double b = -(aj + cj) - k_tot;
+                    double denom = b - aj * cp[CI(jj - 1)];
+                    di[CI(jj)] = recip_complex_div(denom);
+                    cp[CI(jj)] = (jj < ny) ? cj * di[CI(jj)] : 0.0;
+                }
+            }
+        }
+}
+
+static void twiddles(int n, std::vector<double> &tw) {
+    tw.assign((size_t)n, 0.0);   // n/2 complex
+    for (int m = 0; m < n / 2; ++m) {
+        double ang = -2.0 * PI * (double)m / (double)n;
+        tw[2 * m] = std::cos(ang);
+        tw[2 * m + 1] = std::sin(ang);
+    }
+}
+
+// tanh-stretched wall-normal grid on [0, ly]: faces yv(j), j=1..ny+1, centres midway, ghost
+// centres mirrored about the walls; uniform grid uses the IBM sample-position expressions.
+static void y_metrics(mdx_handle *h) {
+    const int ny = h->d.ny;
+    h->yc.assign(ny + 2, 0.0);
+    h->yv.assign(ny + 2, 0.0);
+    h->ihc.assign(ny + 2, 0.0);
+    h->ihv.assign(ny + 2, 0.0);
+    const double ly = h->cfg.ly, beta = h->cfg.stretch_beta;
+    if (!h->stretched) {
+        for (int j = 0; j <= ny + 1; ++j) {
+            h->yc[j] = ((double)(j + 1) - 1.5) * h->dy;
+            h->yv[j] = ((double)(j + 1) - 0.5 - 1.5) * h->dy;
+            h->ihc[j] = 1.0 / h->dy;
+            h->ihv[j] = 1.0 / h->dy;
+        }
+        return;
+    }
+    std::vector<double> yv(ny + 2);
+    const double tb = std::tanh(beta);
+    for (int j = 1; j <= ny + 1; ++j) {
+        double eta = (double)(j - 1) / (double)ny;
+        yv[j] = 0.5 * ly * (1.0 + std::tanh(beta * (2.0 * eta - 1.0)) / tb);
+    }
+    yv[1] = 0.0;
+    yv[ny + 1] = ly;
+    yv[0] = -yv[2];
+    for (int j = 1; j <= ny; ++j) h->yc[j] = 0.5 * (yv[j] + yv[j + 1]);
+    h->yc[0] = 2.0 * yv[1] - h->yc[1];
+    h->yc[ny + 1] = 2.0 * yv[ny + 1] - h->yc[ny];
+    for (int j = 1; j <= ny; ++j) h->ihc[j] = 1.0 / (yv[j + 1] - yv[j]);
+    h->ihc[0] = h->ihc[1];
+    h->ihc[ny + 1] = h->ihc[ny];
+    for (int j = 1; j <= ny + 1; ++j) h->ihv[j] = 1.0 / (h->yc[j] - h->yc[j - 1]);
+    h->ihv[0] = h->ihv[1];
+    for (int j = 0; j <= ny + 1; ++j) h->yv[j] = yv[j];
+}
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t n) {
+    return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+static void free_all(mdx_handle *h) {
+    void *ptrs[] = {h->un, h->vn, h->wn, h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw,
+                    h->cu, h->cv, h->cw, h->d_ihc, h->d_ihv, h->d_yc, h->phi, h->phi2, h->rhs,
+                    h->e32, h->r32, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
+                    h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
+                    h->red_part, h->red_max};
+    for (auto &s : h->graphs)
+        if (s.exec) (void)hipGraphExecDestroy(s.exec);
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+    if (h->timing)
+        for (auto &e : h->ev) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
+    MDX_CHECK(cfg && out, MDX_ERR_ARG, "mdx_create: null argument");
+    *out = nullptr;
+    const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+    MDX_CHECK(nx >= 2 && ny >= 2 && nz >= 2, MDX_ERR_ARG, "mdx_create: nx, ny, nz must be >= 2");
+    MDX_CHECK((double)(nx + 2) * (ny + 2) * (nz + 2) < 2.0e9, MDX_ERR_ARG, "mdx_create: grid too large");
+    MDX_CHECK(cfg->lx > 0 && cfg->ly > 0 && cfg->lz > 0 && cfg->re > 0, MDX_ERR_ARG,
+              "mdx_create: lengths and re must be positive");
+    if (cfg->solver == MDX_SOLVER_FFT) {
+        MDX_CHECK(is_pow2(nx) && is_pow2(nz) && nx <= 1024 && nz <= 1024, MDX_ERR_UNSUPPORTED,
+                  "FFT pressure solver: nx and nz must be powers of two <= 1024");
+    } else if (cfg->solver == MDX_SOLVER_SOLA || cfg->solver == MDX_SOLVER_SOR) {
+        MDX_CHECK(nx % 2 == 0 && nz % 2 == 0, MDX_ERR_ARG,
+                  "red-black pressure solver requires even nx and nz for periodic coloring");
+        MDX_CHECK(cfg->omega > 0.0 && cfg->omega < 2.0, MDX_ERR_ARG, "red-black pressure solver: 0 < omega < 2");
+        if (cfg->solver == MDX_SOLVER_SOLA) MDX_CHECK(cfg->niter > 0, MDX_ERR_ARG, "pressure nIter must be positive");
+        if (cfg->solver == MDX_SOLVER_SOR) {
+            MDX_CHECK(cfg->periodic[0] && cfg->periodic[2], MDX_ERR_UNSUPPORTED,
+                      "SOR to tolerance: x and z must be periodic");
+            if (!cfg->periodic[1]) {
+                MDX_CHECK(cfg->bc_type[bci(2, 0, 2)] == 0 && cfg->bc_type[bci(2, 1, 2)] == 0 &&
+                              cfg->bc_type[bci(2, 0, 0)] == 1 && cfg->bc_type[bci(2, 1, 0)] == 1,
+                          MDX_ERR_UNSUPPORTED,
+                          "SOR to tolerance: y walls need Dirichlet v and Neumann p (or periodic y)");
+            }
+            MDX_CHECK(cfg->tol > 0.0 && cfg->check_every > 0, MDX_ERR_ARG,
+                      "SOR to tolerance: tol > 0 and check_every > 0 required");
+            MDX_CHECK(cfg->precision != MDX_PREC_MIXED || cfg->inner_iters > 0, MDX_ERR_ARG,
+                      "mixed precision: inner_iters > 0 required");
+        }
+    } else {
+        MDX_CHECK(false, MDX_ERR_ARG, "mdx_create: unknown solver");
+    }
+    if (cfg->flowrate)
+        MDX_CHECK(cfg->periodic[0] && cfg->flow_height > 0.0, MDX_ERR_UNSUPPORTED,
+                  "constant flow rate needs periodic x and flow_height > 0");
+    MDX_CHECK(cfg->stretch_beta >= 0.0, MDX_ERR_ARG, "stretch_beta must be >= 0");
+
+    mdx_handle *h = new mdx_handle();
+    h->cfg = *cfg;
+    h->d = Dims{nx, ny, nz, nx + 2, ny + 2};
+    h->dx = cfg->lx / (double)nx;   // init.f90:48-55
+    h->dy = cfg->ly / (double)ny;
+    h->dz = cfg->lz / (double)nz;
+    h->ng = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    h->ni = (size_t)nx * ny * nz;
+    h->stretched = cfg->stretch_beta > 0.0;
+    if (const char *ng = std::getenv("MOBYDIFF_NO_GRAPHS")) h->use_graphs = !(ng[0] == '1');
+    y_metrics(h);
+
+    auto fail = [&](hipError_t e, const char *what) {
+        mdx_set_error(std::string("mdx_create: ") + what + ": " + hipGetErrorString(e));
+        free_all(h);
+        delete h;
+        return MDX_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&h->device)) != hipSuccess) return fail(e, "hipGetDevice");
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
+    double **fields[] = {&h->un, &h->vn, &h->wn, &h->pn, &h->us, &h->vs, &h->ws};
+    for (double **f : fields)
+        if ((e = dalloc(f, h->ng)) != hipSuccess || (e = hipMemset(*f, 0, h->ng * 8)) != hipSuccess)
+            return fail(e, "field allocation");
+    double **olds[] = {&h->oru, &h->orv, &h->orw};
+    for (double **f : olds)
+        if ((e = dalloc(f, h->ni)) != hipSuccess || (e = hipMemset(*f, 0, h->ni * 8)) != hipSuccess)
+            return fail(e, "oldrhs allocation");
+    if ((e = dalloc(&h->d_ihc, ny + 2)) != hipSuccess || (e = dalloc(&h->d_ihv, ny + 2)) != hipSuccess ||
+        (e = dalloc(&h->d_yc, ny + 2)) != hipSuccess)
+        return fail(e, "metric allocation");
+    if ((e = hipMemcpy(h->d_ihc, h->ihc.data(), (ny + 2) * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->d_ihv, h->ihv.data(), (ny + 2) * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->d_yc, h->yc.data(), (ny + 2) * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "metric upload");
+    if ((e = dalloc(&h->red_part, 4096)) != hipSuccess) return fail(e, "reduction buffer");
+    if ((e = dalloc(&h->red_max, 4)) != hipSuccess) return fail(e, "reduction buffer");
+    if ((e = hipHostMalloc((void **)&h->h_pinned, 64 * sizeof(double))) != hipSuccess) return fail(e, "pinned buffer");
+
+    if (cfg->solver == MDX_SOLVER_SOR) {
+        if ((e = dalloc(&h->phi, h->ni)) != hipSuccess || (e = dalloc(&h->phi2, h->ni)) != hipSuccess ||
+            (e = dalloc(&h->rhs, h->ni)) != hipSuccess)
+            return fail(e, "SOR buffers");
+        if (cfg->precision == MDX_PREC_MIXED)
+            if ((e = dalloc(&h->e32, h->ni)) != hipSuccess || (e = dalloc(&h->r32, h->ni)) != hipSuccess)
+                return fail(e, "SOR FP32 buffers");
+        if ((e = hipMemset(h->phi, 0, h->ni * 8)) != hipSuccess) return fail(e, "SOR buffers");
+    }
+    if (cfg->solver == MDX_SOLVER_FFT) {
+        const int nxh = nx / 2 + 1;
+        const size_t nh = (size_t)nxh * ny * nz;
+        std::vector<double> cp, di, ar, twx, twz;
+        tridiag_tables(h, cp, di, ar);
+        twiddles(nx, twx);
+        twiddles(nz, twz);
+        if ((e = dalloc(&h->fft_rhs, h->ni)) != hipSuccess || (e = dalloc(&h->hat, 2 * nh)) != hipSuccess ||
+            (e = dalloc(&h->cp, nh)) != hipSuccess || (e = dalloc(&h->den_inv, nh)) != hipSuccess ||
+            (e = dalloc(&h->arow, ny)) != hipSuccess || (e = dalloc(&h->tw_x, nx)) != hipSuccess ||
+            (e = dalloc(&h->tw_z, nz)) != hipSuccess)
+            return fail(e, "FFT workspace");
+        if ((e = hipMemcpy(h->cp, cp.data(), nh * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->den_inv, di.data(), nh * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->arow, ar.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->tw_x, twx.data(), nx * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->tw_z, twz.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "FFT table upload");
+    }
+    if (cfg->timing) {
+        h->timing = true;
+        for (auto &ev : h->ev)
+            if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "events");
+    }
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "synchronize");
+    *out = h;
+    return MDX_OK;
+}
+
+extern "C" int mdx_destroy(mdx_handle *h) {
+    if (!h) return MDX_OK;
+    (void)hipStreamSynchronize(h->stream);
+    free_all(h);
+    delete h;
+    return MDX_OK;
+}
+
+extern "C" int mdx_synchronize(mdx_handle *h) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    return MDX_OK;
+}
+
+static int copy_in(mdx_handle *h, double *dst, const double *src, size_t n) {
+    if (!src) return MDX_OK;
+    MDX_HIP(hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, h->stream));
+    return MDX_OK;
+}
+static int copy_out(mdx_handle *h, double *dst, const double *src, size_t n) {
+    if (!dst) return MDX_OK;
+    MDX_HIP(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, h->stream));
+    return MDX_OK;
+}
+
+extern "C" int mdx_upload_fields(mdx_handle *h, const double *un, const double *vn, const double *wn,
+                                 const double *pn, const double *us, const double *vs, const double *ws,
+                                 const double *oru, const double *orv, const double *orw) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    int r;
+    if ((r = copy_in(h, h->un, un, h->ng)) || (r = copy_in(h, h->vn, vn, h->ng)) || (r = copy_in(h, h->wn, wn, h->ng)) ||
+        (r = copy_in(h, h->pn, pn, h->ng)) || (r = copy_in(h, h->us, us, h->ng)) || (r = copy_in(h, h->vs, vs, h->ng)) ||
+        (r = copy_in(h, h->ws, ws, h->ng)) || (r = copy_in(h, h->oru, oru, h->ni)) ||
+        (r = copy_in(h, h->orv, orv, h->ni)) || (r = copy_in(h, h->orw, orw, h->ni)))
+        return r;
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    return MDX_OK;
+}
+
+extern "C" int mdx_download_fields(mdx_handle *h, double *un, double *vn, double *wn, double *pn, double *us,
+                                   double *vs, double *ws, double *oru, double *orv, double *orw) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    int r;
+    if ((r = copy_out(h, un, h->un, h->ng)) || (r = copy_out(h, vn, h->vn, h->ng)) || (r = copy_out(h, wn, h->wn, h->ng)) ||
+        (r = copy_out(h, pn, h->pn, h->ng)) || (r = copy_out(h, us, h->us, h->ng)) || (r = copy_out(h, vs, h->vs, h->ng)) ||
+        (r = copy_out(h, ws, h->ws, h->ng)) || (r = copy_out(h, oru, h->oru, h->ni)) ||
+        (r = copy_out(h, orv, h->orv, h->ni)) || (r = copy_out(h, orw, h->orw, h->ni)))
+        return r;
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    return MDX_OK;
+}
+
+// ----------------------------------------------------------------------------------- IBM
+// captured sweep batches hold the mu-table pointers: drop them when the tables change
+static void invalidate_graphs(mdx_handle *h) {
+    for (auto &s : h->graphs) {
+        if (s.exec) (void)hipGraphExecDestroy(s.exec);
+        s = mdx_handle::GraphSlot{};
+    }
+}
+
+static int upload_ibm(mdx_handle *h) {
+    const size_t ng = h->ng;
+    const int sx = h->d.sx, sy = h->d.sy;
+    const size_t plane = (size_t)sx * sy;
+    // z-invariant geometry: every k plane equals the first
+    bool inv = true;
+    for (int k = 1; k < h->d.nz + 2 && inv; ++k)
+        inv = std::memcmp(h->h_cu.data(), h->h_cu.data() + k * plane, plane * 8) == 0 &&
+              std::memcmp(h->h_cv.data(), h->h_cv.data() + k * plane, plane * 8) == 0 &&
+              std::memcmp(h->h_cw.data(), h->h_cw.data() + k * plane, plane * 8) == 0;
+    h->coef_plane = inv;
+    if (!h->cu) {
+        MDX_HIP(dalloc(&h->cu, ng));
+        MDX_HIP(dalloc(&h->cv, ng));
+        MDX_HIP(dalloc(&h->cw, ng));
+    }
+    MDX_HIP(hipMemcpy(h->cu, h->h_cu.data(), ng * 8, hipMemcpyHostToDevice));
+    MDX_HIP(hipMemcpy(h->cv, h->h_cv.data(), ng * 8, hipMemcpyHostToDevice));
+    MDX_HIP(hipMemcpy(h->cw, h->h_cw.data(), ng * 8, hipMemcpyHostToDevice));
+    if (h->cfg.solver == MDX_SOLVER_SOR) {
+        for (int q = 0; q < 3; ++q) {
+            if (h->mu64[q]) { (void)hipFree(h->mu64[q]); h->mu64[q] = nullptr; }
+            if (h->mu32[q]) { (void)hipFree(h->mu32[q]); h->mu32[q] = nullptr; }
+            size_t n = inv ? plane : ng;
+            MDX_HIP(dalloc(&h->mu64[q], n));
+            MDX_HIP(dalloc(&h->mu32[q], n));
+        }
+    }
+    h->mu_dtg = -1.0;
+    h->ibm_on = true;
+    invalidate_graphs(h);
+    return MDX_OK;
+}
+
+extern "C" int mdx_set_ibm(mdx_handle *h, const double *cu, const double *cv, const double *cw) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    if (!cu && !cv && !cw) {
+        h->ibm_on = false;
+        h->mu_dtg = -1.0;
+        invalidate_graphs(h);
+        return MDX_OK;
+    }
+    MDX_CHECK(cu && cv && cw, MDX_ERR_ARG, "mdx_set_ibm: give all three coefficient arrays or none");
+    h->h_cu.assign(cu, cu + h->ng);
+    h->h_cv.assign(cv, cv + h->ng);
+    h->h_cw.assign(cw, cw + h->ng);
+    return upload_ibm(h);
+}
+
+// ibm.f90:84-99 (kind 0) and the two-sided wavy channel (kind 1)
+static bool in_body(const mdx_handle *h, const mdx_geometry *geo, const double x[3]) {
+    if (geo->kind == 0) {
+        double amp = geo->amp_x > 0.0 ? geo->amp_x : 5 * h->dy;
+        int nw = geo->n_wave_x > 0 ? geo->n_wave_x : 1;
+        double y_body = amp * 0.5 * (1.0 + std::sin(2.0 * PI * (double)nw * x[0] / h->cfg.lx + geo->phase_x)) + 2 * h->dy;
+        return x[1] < y_body;
+    }
+    double hh = geo->half_height - geo->A * std::cos(2.0 * PI * x[0] / geo->lambda);
+    return std::fabs(x[1] - 0.5 * h->cfg.ly) > hh;
+}
+
+// ibm.f90:101-130: bisection to the interface along the segment xa -> xb
+static void bisection(const mdx_handle *h, const mdx_geometry *geo, const double xa_in[3], double xb[3]) {
+    double xa[3] = {xa_in[0], xa_in[1], xa_in[2]}, xm[3] = {0, 0, 0};
+    for (int it = 1; it <= 200; ++it) {
+        for (int q = 0; q < 3; ++q) xm[q] = 0.5 * (xa[q] + xb[q]);
+        double e0 = xb[0] - xa[0], e1 = xb[1] - xa[1], e2 = xb[2] - xa[2];
+        if (std::sqrt(e0 * e0 + e1 * e1 + e2 * e2) < 1.0e-10) break;
+        bool la = in_body(h, geo, xa), lm = in_body(h, geo, xm);
+        if (lm == la) std::memcpy(xa, xm, sizeof xa);
+        else std::memcpy(xb, xm, sizeof xa);
+    }
+    std::memcpy(xb, xm, sizeof xm);
+}
+
+// ibm.f90:133-187 over the ghosted array; a z-invariant geometry is evaluated on one k plane
+static void ibm_coeff(const mdx_handle *h, const mdx_geometry *geo, int dix, int diy, int diz,
+                      std::vector<double> &coeff) {
+    const int nx = h->d.nx, ny = h->d.ny, nz = h->d.nz;
+    static const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    coeff.assign(h->ng, 0.0);
+    const bool zinv = true;   // both geometries depend on x and y only
+    const int kmax = zinv ? 1 : nz + 2;
+    for (int iz = 1; iz <= kmax; ++iz)
+
+        for (int iy = 1; iy <= ny + 2; ++iy)
+            for (int ix = 1; ix <= nx + 2; ++ix) {
+                double xa[3] = {((double)ix - (double)dix * 0.5 - 1.5) * h->dx,
+                                ((double)iy - (double)diy * 0.5 - 1.5) * h->dy,
+                                ((double)iz - (double)diz * 0.5 - 1.5) * h->dz};
+                double acc = 0.0;
+                if (in_body(h, geo, xa)) {
+                    acc = 1.0e30;
+                } else if (geo->second_order) {
+                    for (int q = 0; q < 6; ++q) {
+                        double xb[3] = {((double)(ix + nb[q][0]) - (double)dix * 0.5 - 1.5) * h->dx,
+                                        ((double)(iy + nb[q][1]) - (double)diy * 0.5 - 1.5) * h->dy,
+                                        ((double)(iz + nb[q][2]) - (double)diz * 0.5 - 1.5) * h->dz};
+                        double e0 = xb[0] - xa[0], e1 = xb[1] - xa[1], e2 = xb[2] - xa[2];
+                        double d0 = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+                        if (in_body(h, geo, xb)) {
+                            bisection(h, geo, xa, xb);
+                            double f0 = xb[0] - xa[0], f1 = xb[1] - xa[1], f2 = xb[2] - xa[2];
+                            double dd = std::sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+                            acc = acc + ((d0 - dd) / dd) / (d0 * d0);
+                        }
+                    }
+                }
+                coeff[(size_t)(ix - 1) + (size_t)(nx + 2) * ((size_t)(iy - 1) + (size_t)(ny + 2) * (iz - 1))] =
+                    acc / h->cfg.re;
+            }
+    if (zinv) {
+        const size_t plane = (size_t)(nx + 2) * (ny + 2);
+        for (int k = 1; k < nz + 2; ++k) std::copy(coeff.begin(), coeff.begin() + plane, coeff.begin() + k * plane);
+    }
+}
+
+extern "C" int mdx_compute_ibm(mdx_handle *h, const mdx_geometry *geo, double *cu, double *cv, double *cw) {
+    MDX_CHECK(h && geo, MDX_ERR_ARG, "mdx_compute_ibm: null argument");
+    MDX_CHECK(geo->kind == 0 || geo->kind == 1, MDX_ERR_ARG, "mdx_compute_ibm: unknown geometry kind");
+    MDX_CHECK(!h->stretched, MDX_ERR_UNSUPPORTED, "IBM geometry requires the uniform wall-normal grid");
+    if (geo->kind == 1)
+        MDX_CHECK(geo->lambda > 0.0 && geo->half_height > 0.0, MDX_ERR_ARG, "wavy geometry: lambda, half_height > 0");
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    ibm_coeff(h, geo, 1, 0, 0, h->h_cu);
+    ibm_coeff(h, geo, 0, 1, 0, h->h_cv);
+    ibm_coeff(h, geo, 0, 0, 1, h->h_cw);
+    if (cu) std::memcpy(cu, h->h_cu.data(), h->ng * 8);
+    if (cv) std::memcpy(cv, h->h_cv.data(), h->ng * 8);
+    if (cw) std::memcpy(cw, h->h_cw.data(), h->ng * 8);
+    return upload_ibm(h);
+}
+
+// ----------------------------------------------------------------------------------- operators
+extern "C" int mdx_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_HIP(launch_momentum(h, dta, dtb, dtg));
+    return MDX_OK;
+}
+
+extern "C" int mdx_apply_bc(mdx_handle *h) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_HIP(launch_apply_bc_fft(h));
+    return MDX_OK;
+}
+
+extern "C" int mdx_apply_bc_redblack(mdx_handle *h) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_HIP(launch_apply_bc_redblack(h));
+    return MDX_OK;
+}
+
+static int read_max(mdx_handle *h, double *out) {
+    MDX_HIP(hipMemcpyAsync(h->h_pinned, h->red_max, 8, hipMemcpyDeviceToHost, h->stream));
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    *out = h->h_pinned[0];
+    return MDX_OK;
+}
+
+extern "C" int mdx_div_norm(mdx_handle *h, double *div_inf) {
+    MDX_CHECK(h && div_inf, MDX_ERR_ARG, "null argument");
+    MDX_HIP(launch_reset_max(h));
+    MDX_HIP(launch_div_max(h, h->un, h->vn, h->wn));
+    return read_max(h, div_inf);
+}
+
+extern "C" int mdx_get_cfl(mdx_handle *h, double *cfl) {
+    MDX_CHECK(h && cfl, MDX_ERR_ARG, "null argument");
+    MDX_HIP(launch_reset_max(h));
+    MDX_HIP(launch_cfl_max(h));
+    return read_max(h, cfl);
+}
+
+static int ensure_mu(mdx_handle *h, double dtg) {
+    if (h->ibm_on && h->mu_dtg != dtg) {
+        MDX_HIP(launch_build_mu(h, dtg));
+        h->mu_dtg = dtg;
+    }
+    return MDX_OK;
+}
+
+static void ev_record(mdx_handle *h, int q) {
+    if (h->timing) (void)hipEventRecord(h->ev[q], h->stream);
+}
+
+// A batch of m red+black iterations.  The launch sequence of a batch never changes for a given
+// (precision, m), so it is captured once into a HIP graph and replayed; kernel arguments are
+// device pointers that stay fixed for the life of the handle (mu tables are rewritten in place).
+static hipError_t sweep_launches(mdx_handle *h, bool fp32, int m) {
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < m && e == hipSuccess; ++q) {
+        if (fp32) {
+            if ((e = launch_sor_sweep32(h, 0, (float)h->cfg.omega)) == hipSuccess)
+                e = launch_sor_sweep32(h, 1, (float)h->cfg.omega);
+        } else {
+            if ((e = launch_sor_sweep64(h, 0, h->cfg.omega)) == hipSuccess)
+                e = launch_sor_sweep64(h, 1, h->cfg.omega);
+        }
+    }
+    return e;
+}
+
+static int sweep_batch(mdx_handle *h, bool fp32, int m) {
+    ev_record(h, 5);
+    auto &slot = h->graphs[fp32 ? 1 : 0];
+    if (h->use_graphs && m >= 2) {
+        if (!slot.exec || slot.m != m || slot.ibm != h->ibm_on) {
+            if (slot.exec) { (void)hipGraphExecDestroy(slot.exec); slot.exec = nullptr; }
+            hipGraph_t graph = nullptr;
+            MDX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = sweep_launches(h, fp32, m);
+            hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+            MDX_HIP(e);
+            MDX_HIP(e2);
+            hipError_t e3 = hipGraphInstantiate(&slot.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            MDX_HIP(e3);
+            slot.m = m;
+            slot.ibm = h->ibm_on;
+        }
+        MDX_HIP(hipGraphLaunch(slot.exec, h->stream));
+    } else {
+        MDX_HIP(sweep_launches(h, fp32, m));
+    }
+    ev_record(h, 6);
+    if (h->timing) h->sweep_pending += m;
+    return MDX_OK;
+}
+
+// after a stream synchronisation: add the last batch's event time to the sweep counters
+static void sweep_account(mdx_handle *h) {
+    if (!h->timing || h->sweep_pending == 0) return;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev[5], h->ev[6]) == hipSuccess) {
+        h->times.sweep_ms += ms;
+        h->times.sweep_launches += 2 * (int64_t)h->sweep_pending;
+        h->times.sweep_cells += 2 * (int64_t)h->sweep_pending * (int64_t)h->ni;
+    }
+    h->sweep_pending = 0;
+}
+
+// Plain FP64 red-black SOR to ||div||_inf < tol; Phi starts at zero (the previous-step pressure).
+static int sor_solve64(mdx_handle *h, mdx_solve_stats *st) {
+    const mdx_config &c = h->cfg;
+    double r = 0.0;
+    int it = 0, checks = 0;
+    for (;;) {
+        MDX_HIP(launch_reset_max(h));
+        MDX_HIP(launch_sor_residual64(h, h->phi));
+        int rc = read_max(h, &r);
+        if (rc) return rc;
+        sweep_account(h);
+        if (checks == 0) st->div0_inf = r;
+        ++checks;
+        if (!(r >= c.tol) || it >= c.max_iter) break;   // NaN stops too and is reported below
+        const int m = std::min(c.check_every, c.max_iter - it);
+        if ((rc = sweep_batch(h, false, m))) return rc;
+        it += m;
+    }
+    st->iterations = it;
+    st->checks = checks;
+    st->div_inf = r;
+    st->converged = r < c.tol;
+    return MDX_OK;
+}
+
+// FP32 red-black sweeps on the defect equation inside an FP64 defect-correction loop.
+static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
+    const mdx_config &c = h->cfg;
+    double r = 0.0;
+    int it = 0, checks = 0;
+    MDX_HIP(launch_zero32(h, h->e32, h->ni));
+    for (;;) {
+        MDX_HIP(launch_reset_max(h));
+        MDX_HIP(launch_defect64(h, h->phi, h->e32, h->phi2, h->r32));
+        std::swap(h->phi, h->phi2);
+        int rc = read_max(h, &r);
+        if (rc) return rc;
+        sweep_account(h);
+        if (checks == 0) st->div0_inf = r;
+        ++checks;
+        if (!(r >= c.tol) || it >= c.max_iter) break;
+        MDX_HIP(launch_zero32(h, h->e32, h->ni));
+        const int m = std::min(c.inner_iters, c.max_iter - it);
+        if ((rc = sweep_batch(h, true, m))) return rc;
+        it += m;
+    }
+    st->iterations = it;
+    st->checks = checks;
+    st->div_inf = r;
+    st->converged = r < c.tol;
+    return MDX_OK;
+}
+
+extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stats *stats) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_CHECK(dtg > 0.0, MDX_ERR_ARG, "pressure_projection: dt_gamma must be positive");
+    mdx_solve_stats st{};
+    const mdx_config &c = h->cfg;
+    if (c.solver == MDX_SOLVER_FFT) {
+        ev_record(h, 1);
+        MDX_HIP(launch_fft_forward(h, dtg));
+        MDX_HIP(launch_tridiag(h));
+        ev_record(h, 2);
+        MDX_HIP(launch_fft_backward_correct(h, dtg));
+        ev_record(h, 3);
+        st.converged = 1;
+    } else if (c.solver == MDX_SOLVER_SOLA) {
+        ev_record(h, 1);
+        const size_t bytes = h->ng * 8;
+        MDX_HIP(hipMemcpyAsync(h->un, h->us, bytes, hipMemcpyDeviceToDevice, h->stream));
+        MDX_HIP(hipMemcpyAsync(h->vn, h->vs, bytes, hipMemcpyDeviceToDevice, h->stream));
+        MDX_HIP(hipMemcpyAsync(h->wn, h->ws, bytes, hipMemcpyDeviceToDevice, h->stream));
+        for (int it = 0; it < c.niter; ++it)
+            for (int parity = 0; parity <= 1; ++parity) MDX_HIP(launch_sola_sweep(h, parity, c.omega, dtg));
+        ev_record(h, 2);
+        ev_record(h, 3);
+        st.iterations = c.niter;
+    } else {
+        ev_record(h, 1);
+        int rc = ensure_mu(h, dtg);
+        if (rc) return rc;
+        MDX_HIP(launch_sor_prepare(h));
+        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st) : sor_solve64(h, &st);
+        if (rc) return rc;
+        ev_record(h, 2);
+        MDX_HIP(launch_sor_apply(h, h->phi, dtg));
+        ev_record(h, 3);
+        // Reaching max_iter is reported through stats.converged (the caller decides); a
+        // non-finite residual is an error.
+        if (!(st.div_inf < 1e300)) {
+            mdx_set_error("pressure_projection: non-finite divergence in the SOR solve");
+            if (stats) *stats = st;
+            return MDX_ERR_NOT_CONVERGED;
+        }
+    }
+    if (stats) *stats = st;
+    return MDX_OK;
+}
+
+extern "C" int mdx_flowrate_shift(mdx_handle *h, double dtg, double *du) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_CHECK(h->cfg.flow_height > 0.0, MDX_ERR_ARG, "flow_height must be positive");
+    double *dev = h->red_part + 2048;
+    MDX_HIP(launch_bulk_sum(h, h->us, dtg, dev));
+    MDX_HIP(launch_flow_shift(h, dtg, dev));
+    if (du) {
+        MDX_HIP(hipMemcpyAsync(h->h_pinned, dev, 3 * 8, hipMemcpyDeviceToHost, h->stream));
+        MDX_HIP(hipStreamSynchronize(h->stream));
+        *du = h->h_pinned[2];
+    }
+    return MDX_OK;
+}
+
+extern "C" int mdx_bulk_velocity(mdx_handle *h, double *ub) {
+    MDX_CHECK(h && ub, MDX_ERR_ARG, "null argument");
+    MDX_CHECK(h->cfg.flow_height > 0.0, MDX_ERR_ARG, "flow_height must be positive");
+    double *dev = h->red_part + 2048;
+    MDX_HIP(launch_bulk_sum(h, h->un, 1.0, dev));
+    MDX_HIP(hipMemcpyAsync(h->h_pinned, dev, 3 * 8, hipMemcpyDeviceToHost, h->stream));
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    *ub = h->h_pinned[0];
+    return MDX_OK;
+}
+
+static const double RK_ALPHA[3] = {64.0 / 120.0, 50.0 / 120.0, 90.0 / 120.0};   // step.f90:21-23
+static const double RK_BETA[3] = {0.0, -34.0 / 120.0, -50.0 / 120.0};
+static const double RK_GAMMA[3] = {64.0 / 120.0, 16.0 / 120.0, 40.0 / 120.0};
+
+static int post_bc(mdx_handle *h) {
+    return h->cfg.solver == MDX_SOLVER_FFT ? mdx_apply_bc(h) : mdx_apply_bc_redblack(h);
+}
+
+extern "C" int mdx_init_fields(mdx_handle *h, uint32_t seed, double amp, const mdx_geometry *geo, double dt) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_CHECK(dt > 0.0, MDX_ERR_ARG, "mdx_init_fields: dt must be positive");
+    const int use_geo = geo != nullptr;
+    if (use_geo) MDX_CHECK(h->ibm_on, MDX_ERR_STATE, "mdx_init_fields: geometry given but no IBM coefficients set");
+    double *fields[] = {h->un, h->vn, h->wn, h->pn, h->us, h->vs, h->ws};
+    for (double *f : fields) MDX_HIP(launch_zero64(h, f, h->ng));
+    double *olds[] = {h->oru, h->orv, h->orw};
+    for (double *f : olds) MDX_HIP(launch_zero64(h, f, h->ni));
+    std::vector<double> hx(h->d.nx + 2, 1.0);
+    if (use_geo)
+        for (int i = 0; i <= h->d.nx + 1; ++i) {
+            double x = ((double)(i + 1) - 0.5 - 1.5) * h->dx;
+            hx[i] = geo->half_height - geo->A * std::cos(2.0 * PI * x / geo->lambda);
+        }
+    double *d_hx = h->red_part + 3072;
+    MDX_CHECK(h->d.nx + 2 <= 1024, MDX_ERR_UNSUPPORTED, "mdx_init_fields: nx <= 1022");
+    MDX_HIP(hipMemcpyAsync(d_hx, hx.data(), hx.size() * 8, hipMemcpyHostToDevice, h->stream));
+    MDX_HIP(launch_initial_field(h, seed, amp, use_geo, d_hx, 0.5 * h->cfg.ly));
+    int rc;
+    if (h->cfg.solver == MDX_SOLVER_FFT && (rc = mdx_apply_bc(h))) return rc;
+    mdx_solve_stats st;
+    if ((rc = mdx_pressure_projection(h, dt * RK_GAMMA[0], &st))) return rc;
+    if ((rc = post_bc(h))) return rc;
+    MDX_HIP(launch_zero64(h, h->pn, h->ng));
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    return MDX_OK;
+}
+
+extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_CHECK(dt > 0.0, MDX_ERR_ARG, "mdx_step: dt must be positive");
+    int rc;
+    for (int s = 0; s < 3; ++s) {
+        const double dta = dt * RK_ALPHA[s], dtb = dt * RK_BETA[s], dtg = dt * RK_GAMMA[s];
+        ev_record(h, 0);
+        if ((rc = mdx_momentum(h, dta, dtb, dtg))) return rc;
+        if (h->cfg.flowrate && (rc = mdx_flowrate_shift(h, dtg, nullptr))) return rc;
+        if (h->cfg.solver == MDX_SOLVER_FFT && (rc = mdx_apply_bc(h))) return rc;
+        mdx_solve_stats st{};
+        if ((rc = mdx_pressure_projection(h, dtg, &st))) {
+            if (stats) stats->stage[s] = st;
+            return rc;
+        }
+        if ((rc = post_bc(h))) return rc;
+        ev_record(h, 4);
+        if (stats) stats->stage[s] = st;
+        if (h->timing) {
+            MDX_HIP(hipEventSynchronize(h->ev[4]));
+            float a = 0, b = 0, c = 0, d = 0;
+            (void)hipEventElapsedTime(&a, h->ev[0], h->ev[1]);
+            (void)hipEventElapsedTime(&b, h->ev[1], h->ev[2]);
+            (void)hipEventElapsedTime(&c, h->ev[2], h->ev[3]);
+            (void)hipEventElapsedTime(&d, h->ev[3], h->ev[4]);
+            h->times.stencil_ms += a + d;
+            h->times.poisson_ms += b;
+            h->times.projection_ms += c;
+            h->times.stages += 1;
+        }
+    }
+    if (stats && h->cfg.flowrate) {
+        if ((rc = mdx_bulk_velocity(h, &stats->ub))) return rc;
+    }
+    return MDX_OK;
+}
+
+extern "C" int mdx_get_phase_times(mdx_handle *h, mdx_phase_times *t) {
+    MDX_CHECK(h && t, MDX_ERR_ARG, "null argument");
+    *t = h->times;
+    return MDX_OK;
+}
+
+extern "C" int mdx_reset_phase_times(mdx_handle *h) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    h->times = mdx_phase_times{};
+    return MDX_OK;
+}
+
+extern "C" int mdx_get_y_coordinates(mdx_handle *h, double *yc, double *yv) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    if (yc) std::memcpy(yc, h->yc.data(), h->yc.size() * 8);
+    if (yv) std::memcpy(yv, h->yv.data(), h->yv.size() * 8);
+    return MDX_OK;
+}

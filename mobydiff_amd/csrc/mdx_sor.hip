@@ -1,0 +1,456 @@
+// mdx_sor.hip -- red-black pressure solvers for gfx950.
+//
+//  * sola_sweep: the reference's SOLA pressure-velocity sweep (pressure_redblack.f90:86-250),
+//    statement for statement, updating u, v, w, p in place; used for the fixed-nIter mode.
+//  * sor_*: the same iteration in pressure-correction form.  Each sweep updates the accumulated
+//    correction Phi only; u = u* + mu (Phi_west - Phi_east)/h is formed once at the end.  The
+//    residual of a cell is div(u*) - L_mu(Phi), the relaxation denominator is the reference's
+//    (wall faces included), so the iterates equal the SOLA iterates.  FP64 for the plain solve;
+//    FP32 sweeps on the defect equation inside an FP64 defect-correction loop for the mixed solve.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+static constexpr int SX = 32;   // active cells of one colour along x per block row
+static constexpr int SY = 8;
+
+__device__ inline double nanmax_d(double a, double b) { return (a != a || a > b) ? a : b; }
+
+__device__ inline void block_max_atomic(double v, unsigned long long *out) {
+    __shared__ double part[16];
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    for (int off = 32; off > 0; off >>= 1) v = nanmax_d(v, __shfl_xor(v, off, 64));
+    if ((t & 63) == 0) part[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+        double m = part[0];
+        for (int w = 1; w < (nt + 63) / 64; ++w) m = nanmax_d(m, part[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// ------------------------------------------------------------------------------- SOLA (reference)
+__global__ __launch_bounds__(SX * SY) void sola_sweep_kernel(
+    GridK g, BcK b, int parity, double sor, double dtg, double idx, double idy, double idz,
+    double idx2, double idy2, double idz2, double idt, const double *__restrict__ cu,
+    const double *__restrict__ cv, const double *__restrict__ cw, double *un, double *vn, double *wn,
+    double *pn) {
+    const Dims d = g.d;
+    const int ni = blockIdx.x * SX + threadIdx.x + 1;
+    const int j = blockIdx.y * SY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (ni > d.nx / 2 || j > d.ny) return;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const bool per1 = g.per[0], per2 = g.per[1], per3 = g.per[2];
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    int jp = j + 1;
+    if (per2 && jp > ny) jp = 1;
+    int kp = k + 1;
+    if (per3 && kp > nz) kp = 1;
+    const int iLo = 1 + ((j + k + parity) % 2);
+    const int i = iLo + 2 * (ni - 1);
+    int ip = i + 1;
+    if (per1 && ip > nx) ip = 1;
+    const size_t c = d.g(i, j, k), cip = d.g(ip, j, k), cjp = d.g(i, jp, k), ckp = d.g(i, j, kp);
+#define MU(a, q) (1.0 / (1.0 + dtg * ((a) ? (a)[q] : 0.0)))
+#define T(dd, s, q) b.type[bci(dd, s, q)]
+#define V(dd, s, q) b.value[bci(dd, s, q)]
+    const double mu_u_i = MU(cu, c), mu_u_ip = MU(cu, cip);
+    const double mu_v_j = MU(cv, c), mu_v_jp = MU(cv, cjp);
+    const double mu_w_k = MU(cw, c), mu_w_kp = MU(cw, ckp);
+    const double denom = (mu_u_i + mu_u_ip) * idx2 + (mu_w_k + mu_w_kp) * idz2 + (mu_v_j + mu_v_jp) * idy2;
+    if (!per1) {
+        if (i < 2) un[c] = T(1, 0, 1) == 0 ? V(1, 0, 1) : un[cip] - V(1, 0, 1) * dx;
+        if (i >= nx) un[cip] = T(1, 1, 1) == 0 ? V(1, 1, 1) : un[d.g(nx, j, k)] + V(1, 1, 1) * dx;
+    }
+    if (!per2) {
+        if (j < 2) vn[c] = T(2, 0, 2) == 0 ? V(2, 0, 2) : vn[cjp] - V(2, 0, 2) * dy;
+        if (j >= ny) vn[cjp] = T(2, 1, 2) == 0 ? V(2, 1, 2) : vn[c] + V(2, 1, 2) * dy;
+    }
+    if (!per3) {
+        if (k < 2) wn[c] = T(3, 0, 3) == 0 ? V(3, 0, 3) : wn[ckp] - V(3, 0, 3) * dz;
+        if (k >= nz) wn[ckp] = T(3, 1, 3) == 0 ? V(3, 1, 3) : wn[c] + V(3, 1, 3) * dz;
+    }
+    const double phi = ((un[cip] - un[c]) * idx + (vn[cjp] - vn[c]) * idy + (wn[ckp] - wn[c]) * idz) *
+                       (-sor / denom);
+    pn[c] = pn[c] + phi * idt;
+    un[c] = un[c] - phi * idx * mu_u_i;
+    if (!per1 && i == 1) {
+        if (T(1, 0, 0) == 0) un[c] = un[c] + V(1, 0, 0) * idx * mu_u_i;
+        else un[c] = un[c] + (phi * idx - V(1, 0, 0)) * mu_u_i;
+    }
+    un[cip] = un[cip] + phi * idx * mu_u_ip;
+    if (!per1 && i == nx) {
+        if (T(1, 1, 0) == 0) un[cip] = un[cip] - V(1, 1, 0) * idx * mu_u_ip;
+        else un[cip] = un[cip] + (V(1, 1, 0) - phi * idx) * mu_u_ip;
+    }
+    vn[c] = vn[c] - phi * idy * mu_v_j;
+    if (!per2 && j == 1) {
+        if (T(2, 0, 0) == 0) vn[c] = vn[c] + V(2, 0, 0) * idy * mu_v_j;
+        else vn[c] = vn[c] + (phi * idy - V(2, 0, 0)) * mu_v_j;
+    }
+    vn[cjp] = vn[cjp] + phi * idy * mu_v_jp;
+    if (!per2 && j == ny) {
+        if (T(2, 1, 0) == 0) vn[cjp] = vn[cjp] - V(2, 1, 0) * idy * mu_v_jp;
+        else vn[cjp] = vn[cjp] + (V(2, 1, 0) - phi * idy) * mu_v_jp;
+    }
+    wn[c] = wn[c] - phi * idz * mu_w_k;
+    if (!per3 && k == 1) {
+        if (T(3, 0, 0) == 0) wn[c] = wn[c] + V(3, 0, 0) * idz * mu_w_k;
+        else wn[c] = wn[c] + (phi * idz - V(3, 0, 0)) * mu_w_k;
+    }
+    wn[ckp] = wn[ckp] + phi * idz * mu_w_kp;
+    if (!per3 && k == nz) {
+        // pressure_redblack.f90:229-230 carries no mu factor on this Dirichlet branch
+        if (T(3, 1, 0) == 0) wn[ckp] = wn[ckp] - V(3, 1, 0) * idz;
+        else wn[ckp] = wn[ckp] + (V(3, 1, 0) - phi * idz) * mu_w_kp;
+    }
+#undef MU
+#undef T
+#undef V
+}
+
+hipError_t launch_sola_sweep(mdx_handle *h, int parity, double sor, double dtg) {
+    GridK g = grid_k(h);
+    BcK b = bc_k(h);
+    const double idx = 1.0 / h->dx, idy = 1.0 / h->dy, idz = 1.0 / h->dz;
+    dim3 blk(SX, SY), grd((h->d.nx / 2 + SX - 1) / SX, (h->d.ny + SY - 1) / SY, h->d.nz);
+    const double *cu = h->ibm_on ? h->cu : nullptr, *cv = h->ibm_on ? h->cv : nullptr,
+                 *cw = h->ibm_on ? h->cw : nullptr;
+    hipLaunchKernelGGL(sola_sweep_kernel, grd, blk, 0, h->stream, g, b, parity, sor, dtg, idx, idy, idz,
+                       idx * idx, idy * idy, idz * idz, 1.0 / dtg, cu, cv, cw, h->un, h->vn, h->wn, h->pn);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- face coefficients
+__global__ void build_mu_kernel(size_t n, double dtg, const double *__restrict__ cu,
+                                const double *__restrict__ cv, const double *__restrict__ cw,
+                                double *mu, double *mv, double *mw, float *fu, float *fv, float *fw) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double a = 1.0 / (1.0 + dtg * cu[q]), b = 1.0 / (1.0 + dtg * cv[q]), c = 1.0 / (1.0 + dtg * cw[q]);
+    mu[q] = a;
+    mv[q] = b;
+    mw[q] = c;
+    fu[q] = (float)a;
+    fv[q] = (float)b;
+    fw[q] = (float)c;
+}
+
+hipError_t launch_build_mu(mdx_handle *h, double dtg) {
+    if (!h->ibm_on) return hipSuccess;
+    size_t n = h->coef_plane ? (size_t)h->d.sx * h->d.sy : h->ng;
+    hipLaunchKernelGGL(build_mu_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, n, dtg,
+                       h->cu, h->cv, h->cw, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
+                       h->mu32[2]);
+    return hipGetLastError();
+}
+
+template <typename T>
+static MuK<T> mu_k(const mdx_handle *h, T *const *tabs) {
+    MuK<T> m;
+    m.u = h->ibm_on ? tabs[0] : nullptr;
+    m.v = h->ibm_on ? tabs[1] : nullptr;
+    m.w = h->ibm_on ? tabs[2] : nullptr;
+    m.plane = h->coef_plane ? 1 : 0;
+    m.sx = h->d.sx;
+    m.sy = h->d.sy;
+    return m;
+}
+
+// ------------------------------------------------------------------------------- residual stencil
+template <typename T>
+struct SorConst {
+    T idx2, idy2, idz2;
+};
+
+// residual r = rhs - L_mu(Phi) of cell (i,j,k) and the reference's relaxation denominator
+template <typename T, bool STRETCH, bool IBM, typename PhiAt>
+__device__ inline T cell_residual(const GridK &g, const MuK<T> &mu, const SorConst<T> &s, PhiAt P,
+                                  T rhs_c, int i, int j, int k, T &denom) {
+    const Dims d = g.d;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const bool per2 = g.per[1];
+    const int ip = i == nx ? 1 : i + 1, im = i == 1 ? nx : i - 1;
+    const int kp = k == nz ? 1 : k + 1, km = k == 1 ? nz : k - 1;
+    int jp = j + 1, jm = j - 1;
+    if (per2) {
+        if (jp > ny) jp = 1;
+        if (jm < 1) jm = ny;
+    }
+    T mui = 1, muip = 1, mvj = 1, mvjp = 1, mwk = 1, mwkp = 1;
+    if (IBM) {
+        mui = mu.u[mu.at(i, j, k)];
+        muip = mu.u[mu.at(ip, j, k)];
+        mvj = mu.v[mu.at(i, j, k)];
+        mvjp = mu.v[mu.at(i, jp, k)];
+        mwk = mu.w[mu.at(i, j, k)];
+        mwkp = mu.w[mu.at(i, j, kp)];
+    }
+    const T p0 = P(i, j, k);
+    const bool wn_ = !per2 && j == ny, ws_ = !per2 && j == 1;
+    const T sn = wn_ ? T(0) : mvjp * (P(i, wn_ ? j : jp, k) - p0);
+    const T ss = ws_ ? T(0) : mvj * (p0 - P(i, ws_ ? j : jm, k));
+    const T lx = (muip * (P(ip, j, k) - p0) - mui * (p0 - P(im, j, k))) * s.idx2;
+    const T lz = (mwkp * (P(i, j, kp) - p0) - mwk * (p0 - P(i, j, km))) * s.idz2;
+    T ly;
+    if (!STRETCH) {
+        ly = (sn - ss) * s.idy2;
+        denom = (mui + muip) * s.idx2 + (mwk + mwkp) * s.idz2 + (mvj + mvjp) * s.idy2;
+    } else {
+        const T an = (T)(g.ihv[j + 1] * g.ihc[j]), as = (T)(g.ihv[j] * g.ihc[j]);
+        ly = sn * an - ss * as;
+        denom = (mui + muip) * s.idx2 + (mwk + mwkp) * s.idz2 + (mvj * as + mvjp * an);
+    }
+    return rhs_c - (lx + ly + lz);
+}
+
+template <typename T>
+struct PlainPhi {
+    const T *p;
+    Dims d;
+    __device__ inline T operator()(int i, int j, int k) const { return p[d.in(i, j, k)]; }
+};
+
+// Phi64 + e32, the corrected iterate of the defect-correction loop
+struct SumPhi {
+    const double *p;
+    const float *e;
+    Dims d;
+    __device__ inline double operator()(int i, int j, int k) const {
+        size_t q = d.in(i, j, k);
+        return p[q] + (double)e[q];
+    }
+};
+
+// ------------------------------------------------------------------------------- prepare / apply
+// After u = u* (pressure_redblack.f90:64-84): Dirichlet v on the walls, rhs = div(u), Phi = 0.
+template <bool STRETCH>
+__global__ __launch_bounds__(256) void sor_prepare_kernel(GridK g, double vbot, double vtop,
+                                                          double *vn, const double *__restrict__ un,
+                                                          const double *__restrict__ wn, double *rhs,
+                                                          double *phi, double idx, double idy, double idz) {
+    const Dims d = g.d;
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nx || j > d.ny) return;
+    const bool per2 = g.per[1];
+    if (!per2) {
+        if (j == 1) vn[d.g(i, 1, k)] = vbot;
+        if (j == d.ny) vn[d.g(i, d.ny + 1, k)] = vtop;
+    }
+    const int ip = i == d.nx ? 1 : i + 1, kp = k == d.nz ? 1 : k + 1;
+    const int jp = (per2 && j == d.ny) ? 1 : j + 1;
+    const size_t c = d.g(i, j, k);
+    const double dvy = STRETCH ? (vn[d.g(i, jp, k)] - vn[c]) * g.ihc[j] : (vn[d.g(i, jp, k)] - vn[c]) * idy;
+    rhs[d.in(i, j, k)] = (un[d.g(ip, j, k)] - un[c]) * idx + dvy + (wn[d.g(i, j, kp)] - wn[c]) * idz;
+    phi[d.in(i, j, k)] = 0.0;
+}
+
+hipError_t launch_sor_prepare(mdx_handle *h) {
+    GridK g = grid_k(h);
+    const size_t bytes = h->ng * sizeof(double);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(h->un, h->us, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(h->vn, h->vs, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(h->wn, h->ws, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
+    dim3 blk(64, 4), grd((h->d.nx + 63) / 64, (h->d.ny + 3) / 4, h->d.nz);
+    const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
+    if (h->stretched)
+        hipLaunchKernelGGL(sor_prepare_kernel<true>, grd, blk, 0, h->stream, g, vb, vt, h->vn, h->un, h->wn,
+                           h->rhs, h->phi, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz);
+    else
+        hipLaunchKernelGGL(sor_prepare_kernel<false>, grd, blk, 0, h->stream, g, vb, vt, h->vn, h->un, h->wn,
+                           h->rhs, h->phi, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz);
+    return hipGetLastError();
+}
+
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu, const double *__restrict__ phi,
+                                                        double *un, double *vn, double *wn, double *pn,
+                                                        double idx, double idy, double idz, double idt) {
+    const Dims d = g.d;
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nx || j > d.ny) return;
+    const bool per2 = g.per[1];
+    const int im = i == 1 ? d.nx : i - 1, km = k == 1 ? d.nz : k - 1;
+    const size_t c = d.g(i, j, k);
+    const double p0 = phi[d.in(i, j, k)];
+    const double mu_u = IBM ? mu.u[mu.at(i, j, k)] : 1.0;
+    const double mu_v = IBM ? mu.v[mu.at(i, j, k)] : 1.0;
+    const double mu_w = IBM ? mu.w[mu.at(i, j, k)] : 1.0;
+    un[c] = un[c] + mu_u * (phi[d.in(im, j, k)] - p0) * idx;
+    wn[c] = wn[c] + mu_w * (phi[d.in(i, j, km)] - p0) * idz;
+    if (j >= 2 || per2) {
+        const int jm = j == 1 ? d.ny : j - 1;
+        const double gy = STRETCH ? (phi[d.in(i, jm, k)] - p0) * g.ihv[j] : (phi[d.in(i, jm, k)] - p0) * idy;
+        vn[c] = vn[c] + mu_v * gy;
+    }
+    pn[c] = pn[c] + p0 * idt;
+}
+
+hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg) {
+    GridK g = grid_k(h);
+    MuK<double> mu = mu_k<double>(h, h->mu64);
+    dim3 blk(64, 4), grd((h->d.nx + 63) / 64, (h->d.ny + 3) / 4, h->d.nz);
+    const double a = 1.0 / h->dx, b = 1.0 / h->dy, c = 1.0 / h->dz, t = 1.0 / dtg;
+#define MDX_APPLY(S, I) \
+    hipLaunchKernelGGL((sor_apply_kernel<S, I>), grd, blk, 0, h->stream, g, mu, phi, h->un, h->vn, h->wn, h->pn, a, b, c, t)
+    if (h->stretched) { if (h->ibm_on) MDX_APPLY(true, true); else MDX_APPLY(true, false); }
+    else { if (h->ibm_on) MDX_APPLY(false, true); else MDX_APPLY(false, false); }
+#undef MDX_APPLY
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- FP64 sweeps
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(SX * SY) void sor_sweep64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
+                                                              int color, double omega,
+                                                              const double *__restrict__ rhs, double *phi) {
+    const Dims d = g.d;
+    const int ni = blockIdx.x * SX + threadIdx.x + 1;
+    const int j = blockIdx.y * SY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (ni > d.nx / 2 || j > d.ny) return;
+    const int i = 1 + ((j + k + color) & 1) + 2 * (ni - 1);
+    PlainPhi<double> P{phi, d};
+    double denom;
+    const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[d.in(i, j, k)], i, j, k, denom);
+    const size_t q = d.in(i, j, k);
+    phi[q] = phi[q] + r * (-omega / denom);
+}
+
+hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega) {
+    GridK g = grid_k(h);
+    MuK<double> mu = mu_k<double>(h, h->mu64);
+    SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    dim3 blk(SX, SY), grd((h->d.nx / 2 + SX - 1) / SX, (h->d.ny + SY - 1) / SY, h->d.nz);
+#define MDX_SW(S, I) \
+    hipLaunchKernelGGL((sor_sweep64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, color, omega, h->rhs, h->phi)
+    if (h->stretched) { if (h->ibm_on) MDX_SW(true, true); else MDX_SW(true, false); }
+    else { if (h->ibm_on) MDX_SW(false, true); else MDX_SW(false, false); }
+#undef MDX_SW
+    return hipGetLastError();
+}
+
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(256) void sor_residual64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
+                                                             const double *__restrict__ rhs,
+                                                             const double *__restrict__ phi,
+                                                             unsigned long long *out) {
+    const Dims d = g.d;
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double m = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        PlainPhi<double> P{phi, d};
+        double denom;
+        m = fabs(cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[d.in(i, j, k)], i, j, k, denom));
+    }
+    block_max_atomic(m, out);
+}
+
+hipError_t launch_sor_residual64(mdx_handle *h, const double *phi) {
+    GridK g = grid_k(h);
+    MuK<double> mu = mu_k<double>(h, h->mu64);
+    SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    dim3 blk(64, 4), grd((h->d.nx + 63) / 64, (h->d.ny + 3) / 4, h->d.nz);
+#define MDX_RS(S, I) \
+    hipLaunchKernelGGL((sor_residual64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi, h->red_max)
+    if (h->stretched) { if (h->ibm_on) MDX_RS(true, true); else MDX_RS(true, false); }
+    else { if (h->ibm_on) MDX_RS(false, true); else MDX_RS(false, false); }
+#undef MDX_RS
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- mixed precision
+// One FP64 defect-correction step: Phi_out = Phi_in + e; r = rhs - L_mu(Phi_out) in FP64;
+// r32 = (float) r for the next FP32 inner solve; max |r| for the stopping test.
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
+                                                       const double *__restrict__ rhs,
+                                                       const double *__restrict__ phi_in,
+                                                       const float *__restrict__ e, double *phi_out,
+                                                       float *r32, unsigned long long *out) {
+    const Dims d = g.d;
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double m = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        SumPhi P{phi_in, e, d};
+        double denom;
+        const size_t q = d.in(i, j, k);
+        const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[q], i, j, k, denom);
+        phi_out[q] = P(i, j, k);
+        r32[q] = (float)r;
+        m = fabs(r);
+    }
+    block_max_atomic(m, out);
+}
+
+hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32) {
+    GridK g = grid_k(h);
+    MuK<double> mu = mu_k<double>(h, h->mu64);
+    SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    dim3 blk(64, 4), grd((h->d.nx + 63) / 64, (h->d.ny + 3) / 4, h->d.nz);
+#define MDX_DF(S, I)                                                                                       \
+    hipLaunchKernelGGL((defect64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi_in, e, phi_out, \
+                       r32, h->red_max)
+    if (h->stretched) { if (h->ibm_on) MDX_DF(true, true); else MDX_DF(true, false); }
+    else { if (h->ibm_on) MDX_DF(false, true); else MDX_DF(false, false); }
+#undef MDX_DF
+    return hipGetLastError();
+}
+
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(SX * SY) void sor_sweep32_kernel(GridK g, MuK<float> mu, SorConst<float> s,
+                                                              int color, float omega,
+                                                              const float *__restrict__ rhs, float *e) {
+    const Dims d = g.d;
+    const int ni = blockIdx.x * SX + threadIdx.x + 1;
+    const int j = blockIdx.y * SY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (ni > d.nx / 2 || j > d.ny) return;
+    const int i = 1 + ((j + k + color) & 1) + 2 * (ni - 1);
+    PlainPhi<float> P{e, d};
+    float denom;
+    const float r = cell_residual<float, STRETCH, IBM>(g, mu, s, P, rhs[d.in(i, j, k)], i, j, k, denom);
+    const size_t q = d.in(i, j, k);
+    e[q] = e[q] + r * (-omega / denom);
+}
+
+hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega) {
+    GridK g = grid_k(h);
+    MuK<float> mu = mu_k<float>(h, h->mu32);
+    SorConst<float> s{(float)(1.0 / (h->dx * h->dx)), (float)(1.0 / (h->dy * h->dy)), (float)(1.0 / (h->dz * h->dz))};
+    dim3 blk(SX, SY), grd((h->d.nx / 2 + SX - 1) / SX, (h->d.ny + SY - 1) / SY, h->d.nz);
+#define MDX_SW(S, I) \
+    hipLaunchKernelGGL((sor_sweep32_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, color, omega, h->r32, h->e32)
+    if (h->stretched) { if (h->ibm_on) MDX_SW(true, true); else MDX_SW(true, false); }
+    else { if (h->ibm_on) MDX_SW(false, true); else MDX_SW(false, false); }
+#undef MDX_SW
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ void zero_kernel(T *p, size_t n) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = T(0);
+}
+
+hipError_t launch_zero32(mdx_handle *h, float *p, size_t n) {
+    hipLaunchKernelGGL(zero_kernel<float>, dim3(2048), dim3(256), 0, h->stream, p, n);
+    return hipGetLastError();
+}
+hipError_t launch_zero64(mdx_handle *h, double *p, size_t n) {
+    hipLaunchKernelGGL(zero_kernel<double>, dim3(2048), dim3(256), 0, h->stream, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace mdx

@@ -1,0 +1,548 @@
+// mdx_stencil.hip -- momentum predictor, ghost-cell kernels, reductions, flow-rate control and
+// the initial field, for gfx950.  FP64 throughout; compiled with -ffp-contract=off so every
+// expression rounds exactly as the reference's (step.f90, boundary.f90) does on the CPU.
+#include "mdx_internal.h"
+
+namespace mdx {
+
+static constexpr int BX = 64;   // one wavefront along x: coalesced rows of the ghosted arrays
+static constexpr int BY = 4;
+
+// ------------------------------------------------------------------------------- momentum
+// step.f90:27-207.  One thread per interior cell updates u*, v*, w* and oldrhs.  Reads 10 arrays,
+// writes 6 (128 B/cell with IBM coefficients, 104 B without).
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(BX * BY) void momentum_kernel(
+    GridK g, const double *__restrict__ un, const double *__restrict__ vn,
+    const double *__restrict__ wn, const double *__restrict__ pn, double *__restrict__ us,
+    double *__restrict__ vs, double *__restrict__ ws, double *__restrict__ oru,
+    double *__restrict__ orv, double *__restrict__ orw, const double *__restrict__ cu,
+    const double *__restrict__ cv, const double *__restrict__ cw, double dta, double dtb,
+    double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
+    double ire) {
+    const Dims d = g.d;
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nx || j > d.ny) return;
+    const int nx0 = g.per[0] ? 1 : 2, ny0 = g.per[1] ? 1 : 2, nz0 = g.per[2] ? 1 : 2;
+    const int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
+    const size_t c = d.g(i, j, k);
+    const size_t o = d.in(i, j, k);
+    const double *ihc = g.ihc, *ihv = g.ihv;
+
+    if (i >= nx0) {
+        const double u0 = un[c], uip = un[d.g(ip, j, k)], uim = un[d.g(im, j, k)];
+        const double ujp = un[d.g(i, jp, k)], ujm = un[d.g(i, jm, k)];
+        const double ukp = un[d.g(i, j, kp)], ukm = un[d.g(i, j, km)];
+        double uu_p = 0.25 * ((uip + u0) * (uip + u0));
+        double uu_m = 0.25 * ((u0 + uim) * (u0 + uim));
+        double uv_p = 0.25 * (ujp + u0) * (vn[d.g(i, jp, k)] + vn[d.g(im, jp, k)]);
+        double uv_m = 0.25 * (u0 + ujm) * (vn[c] + vn[d.g(im, j, k)]);
+        double uw_p = 0.25 * (u0 + ukp) * (wn[d.g(i, j, kp)] + wn[d.g(im, j, kp)]);
+        double uw_m = 0.25 * (u0 + ukm) * (wn[c] + wn[d.g(im, j, k)]);
+        double diff_ux = (uim - 2.0 * u0 + uip) * idx2;
+        double diff_uy, cy;
+        if (!STRETCH) {
+            diff_uy = (ujm - 2.0 * u0 + ujp) * idy2;
+            cy = (uv_p - uv_m) * idy;
+        } else {
+            diff_uy = ((ujp - u0) * ihv[jp] - (u0 - ujm) * ihv[j]) * ihc[j];
+            cy = (uv_p - uv_m) * ihc[j];
+        }
+        double diff_uz = (ukm - 2.0 * u0 + ukp) * idz2;
+        double dpx = (pn[c] - pn[d.g(im, j, k)]) * idx;
+        double rhsu = -(uu_p - uu_m) * idx - cy - (uw_p - uw_m) * idz + g.fx +
+                      ire * (diff_ux + diff_uy + diff_uz);
+        double s = u0 + dta * rhsu + dtb * oru[o] - dtg * dpx;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cu[c]));
+        us[c] = s;
+        oru[o] = rhsu;
+    }
+    if (j >= ny0) {
+        const double v0 = vn[c], vip = vn[d.g(ip, j, k)], vim = vn[d.g(im, j, k)];
+        const double vjp = vn[d.g(i, jp, k)], vjm = vn[d.g(i, jm, k)];
+        const double vkp = vn[d.g(i, j, kp)], vkm = vn[d.g(i, j, km)];
+        double vu_p = 0.25 * (v0 + vip) * (un[d.g(ip, j, k)] + un[d.g(ip, jm, k)]);
+        double vu_m = 0.25 * (v0 + vim) * (un[c] + un[d.g(i, jm, k)]);
+        double vv_p = 0.25 * ((v0 + vjp) * (v0 + vjp));
+        double vv_m = 0.25 * ((v0 + vjm) * (v0 + vjm));
+        double vw_p = 0.25 * (vkp + v0) * (wn[d.g(i, j, kp)] + wn[d.g(i, jm, kp)]);
+        double vw_m = 0.25 * (vkm + v0) * (wn[c] + wn[d.g(i, jm, k)]);
+        double diff_vx = (vim - 2.0 * v0 + vip) * idx2;
+        double diff_vy, cy, dpy;
+        if (!STRETCH) {
+            diff_vy = (vjm - 2.0 * v0 + vjp) * idy2;
+            cy = (vv_p - vv_m) * idy;
+            dpy = (pn[c] - pn[d.g(i, jm, k)]) * idy;
+        } else {
+            diff_vy = ((vjp - v0) * ihc[j] - (v0 - vjm) * ihc[jm]) * ihv[j];
+            cy = (vv_p - vv_m) * ihv[j];
+            dpy = (pn[c] - pn[d.g(i, jm, k)]) * ihv[j];
+        }
+        double diff_vz = (vkm - 2.0 * v0 + vkp) * idz2;
+        double rhsv = -(vu_p - vu_m) * idx - cy - (vw_p - vw_m) * idz + g.fy +
+                      ire * (diff_vx + diff_vy + diff_vz);
+        double s = v0 + dta * rhsv + dtb * orv[o] - dtg * dpy;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cv[c]));
+        vs[c] = s;
+        orv[o] = rhsv;
+    }
+    if (k >= nz0) {
+        const double w0 = wn[c], wip = wn[d.g(ip, j, k)], wim = wn[d.g(im, j, k)];
+        const double wjp = wn[d.g(i, jp, k)], wjm = wn[d.g(i, jm, k)];
+        const double wkp = wn[d.g(i, j, kp)], wkm = wn[d.g(i, j, km)];
+        double wu_p = 0.25 * (w0 + wip) * (un[d.g(ip, j, k)] + un[d.g(ip, j, km)]);
+        double wu_m = 0.25 * (w0 + wim) * (un[c] + un[d.g(i, j, km)]);
+        double ww_p = 0.25 * ((w0 + wkp) * (w0 + wkp));
+        double ww_m = 0.25 * ((w0 + wkm) * (w0 + wkm));
+        double wv_p = 0.25 * (w0 + wjp) * (vn[d.g(i, jp, k)] + vn[d.g(i, jp, km)]);
+        double wv_m = 0.25 * (w0 + wjm) * (vn[c] + vn[d.g(i, j, km)]);
+        double diff_wx = (wim - 2.0 * w0 + wip) * idx2;
+        double diff_wy, cy;
+        if (!STRETCH) {
+            diff_wy = (wjm - 2.0 * w0 + wjp) * idy2;
+            cy = (wv_p - wv_m) * idy;
+        } else {
+            diff_wy = ((wjp - w0) * ihv[jp] - (w0 - wjm) * ihv[j]) * ihc[j];
+            cy = (wv_p - wv_m) * ihc[j];
+        }
+        double diff_wz = (wkm - 2.0 * w0 + wkp) * idz2;
+        double dpz = (pn[c] - pn[d.g(i, j, km)]) * idz;
+        double rhsw = -(wu_p - wu_m) * idx - cy - (ww_p - ww_m) * idz + g.fz +
+                      ire * (diff_wx + diff_wy + diff_wz);
+        double s = w0 + dta * rhsw + dtb * orw[o] - dtg * dpz;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cw[c]));
+        ws[c] = s;
+        orw[o] = rhsw;
+    }
+}
+
+hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
+    GridK g = grid_k(h);
+    const double dx = h->dx, dy = h->dy, dz = h->dz;
+    const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+    dim3 blk(BX, BY, 1), grd((h->d.nx + BX - 1) / BX, (h->d.ny + BY - 1) / BY, h->d.nz);
+#define MDX_MOM(S, I)                                                                          \
+    hipLaunchKernelGGL((momentum_kernel<S, I>), grd, blk, 0, h->stream, g, h->un, h->vn, h->wn, \
+                       h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw, h->cu, h->cv, h->cw,  \
+                       dta, dtb, dtg, 1.0 / dx, 1.0 / dy, 1.0 / dz, 1.0 / dx2, 1.0 / dy2,      \
+                       1.0 / dz2, 1.0 / h->cfg.re)
+    if (h->stretched) {
+        if (h->ibm_on) MDX_MOM(true, true); else MDX_MOM(true, false);
+    } else {
+        if (h->ibm_on) MDX_MOM(false, true); else MDX_MOM(false, false);
+    }
+#undef MDX_MOM
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- apply_bc (FFT build)
+// boundary.f90:23-170: y no-slip walls (edges included), then x- and z-periodic copies.
+__global__ void bc_fft_y(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.y;
+    if (i > d.nx + 1 || k > d.nz + 1) return;
+    const int ny = d.ny;
+    un[d.g(i, ny + 1, k)] = -un[d.g(i, ny, k)];
+    vn[d.g(i, ny + 1, k)] = 0.0;
+    wn[d.g(i, ny + 1, k)] = -wn[d.g(i, ny, k)];
+    un[d.g(i, 0, k)] = -un[d.g(i, 1, k)];
+    vn[d.g(i, 1, k)] = 0.0;
+    wn[d.g(i, 0, k)] = -wn[d.g(i, 1, k)];
+    us[d.g(i, ny + 1, k)] = -us[d.g(i, ny, k)];
+    vs[d.g(i, ny + 1, k)] = 0.0;
+    ws[d.g(i, ny + 1, k)] = -ws[d.g(i, ny, k)];
+    us[d.g(i, 0, k)] = -us[d.g(i, 1, k)];
+    vs[d.g(i, 1, k)] = 0.0;
+    ws[d.g(i, 0, k)] = -ws[d.g(i, 1, k)];
+}
+
+__global__ void bc_fft_x(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws,
+                         double *pn) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.y;
+    if (j > d.ny + 1 || k > d.nz + 1) return;
+    const int nx = d.nx;
+    un[d.g(0, j, k)] = un[d.g(nx, j, k)];
+    wn[d.g(0, j, k)] = wn[d.g(nx, j, k)];
+    un[d.g(nx + 1, j, k)] = un[d.g(1, j, k)];
+    wn[d.g(nx + 1, j, k)] = wn[d.g(1, j, k)];
+    us[d.g(0, j, k)] = us[d.g(nx, j, k)];
+    ws[d.g(0, j, k)] = ws[d.g(nx, j, k)];
+    us[d.g(nx + 1, j, k)] = us[d.g(1, j, k)];
+    ws[d.g(nx + 1, j, k)] = ws[d.g(1, j, k)];
+    if (j >= 1) {
+        vn[d.g(0, j, k)] = vn[d.g(nx, j, k)];
+        vn[d.g(nx + 1, j, k)] = vn[d.g(1, j, k)];
+        vs[d.g(0, j, k)] = vs[d.g(nx, j, k)];
+        vs[d.g(nx + 1, j, k)] = vs[d.g(1, j, k)];
+    }
+    if (j >= 1 && j <= d.ny) {
+        pn[d.g(0, j, k)] = pn[d.g(nx, j, k)];
+        pn[d.g(nx + 1, j, k)] = pn[d.g(1, j, k)];
+    }
+}
+
+__global__ void bc_fft_z(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws,
+                         double *pn) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i > d.nx + 1 || j > d.ny + 1) return;
+    const int nz = d.nz;
+    un[d.g(i, j, 0)] = un[d.g(i, j, nz)];
+    wn[d.g(i, j, 0)] = wn[d.g(i, j, nz)];
+    un[d.g(i, j, nz + 1)] = un[d.g(i, j, 1)];
+    wn[d.g(i, j, nz + 1)] = wn[d.g(i, j, 1)];
+    us[d.g(i, j, 0)] = us[d.g(i, j, nz)];
+    ws[d.g(i, j, 0)] = ws[d.g(i, j, nz)];
+    us[d.g(i, j, nz + 1)] = us[d.g(i, j, 1)];
+    ws[d.g(i, j, nz + 1)] = ws[d.g(i, j, 1)];
+    if (j >= 1) {
+        vn[d.g(i, j, 0)] = vn[d.g(i, j, nz)];
+        vn[d.g(i, j, nz + 1)] = vn[d.g(i, j, 1)];
+        vs[d.g(i, j, 0)] = vs[d.g(i, j, nz)];
+        vs[d.g(i, j, nz + 1)] = vs[d.g(i, j, 1)];
+    }
+    if (j >= 1 && j <= d.ny) {
+        pn[d.g(i, j, 0)] = pn[d.g(i, j, nz)];
+        pn[d.g(i, j, nz + 1)] = pn[d.g(i, j, 1)];
+    }
+}
+
+hipError_t launch_apply_bc_fft(mdx_handle *h) {
+    const Dims d = h->d;
+    hipLaunchKernelGGL(bc_fft_y, dim3((d.nx + 2 + 63) / 64, d.nz + 2), dim3(64), 0, h->stream, d,
+                       h->un, h->vn, h->wn, h->us, h->vs, h->ws);
+    hipLaunchKernelGGL(bc_fft_x, dim3((d.ny + 2 + 63) / 64, d.nz + 2), dim3(64), 0, h->stream, d,
+                       h->un, h->vn, h->wn, h->us, h->vs, h->ws, h->pn);
+    hipLaunchKernelGGL(bc_fft_z, dim3((d.nx + 2 + 63) / 64, d.ny + 2), dim3(64), 0, h->stream, d,
+                       h->un, h->vn, h->wn, h->us, h->vs, h->ws, h->pn);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- apply_bc_redblack
+// boundary.f90:173-368.  The reference runs the six faces in one parallel loop; the faces are
+// applied here in its sequential order (y, then x, then z), which is also the only order in which
+// the faces' shared reads (e.g. vn(nx,1,k) after the bottom wall) are well defined.
+__global__ void bc_rb_y(GridK g, BcK b, double *un, double *vn, double *wn, double *pn) {
+    const Dims d = g.d;
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int k = blockIdx.y + 1;
+    if (i > d.nx || k > d.nz) return;
+    const int ny = d.ny;
+    const double dy = g.dy;
+#define T(dd, s, q) b.type[bci(dd, s, q)]
+#define V(dd, s, q) b.value[bci(dd, s, q)]
+    if (g.per[1]) {
+        un[d.g(i, 0, k)] = un[d.g(i, ny, k)];
+        vn[d.g(i, 0, k)] = vn[d.g(i, ny, k)];
+        wn[d.g(i, 0, k)] = wn[d.g(i, ny, k)];
+        pn[d.g(i, 0, k)] = pn[d.g(i, ny, k)];
+        un[d.g(i, ny + 1, k)] = un[d.g(i, 1, k)];
+        vn[d.g(i, ny + 1, k)] = vn[d.g(i, 1, k)];
+        wn[d.g(i, ny + 1, k)] = wn[d.g(i, 1, k)];
+        pn[d.g(i, ny + 1, k)] = pn[d.g(i, 1, k)];
+    } else {
+        un[d.g(i, 0, k)] = T(2, 0, 1) == 0 ? 2 * V(2, 0, 1) - un[d.g(i, 1, k)] : un[d.g(i, 1, k)] - V(2, 0, 1) * dy;
+        vn[d.g(i, 1, k)] = T(2, 0, 2) == 0 ? V(2, 0, 2) : vn[d.g(i, 2, k)] - V(2, 0, 2) * dy;
+        wn[d.g(i, 0, k)] = T(2, 0, 3) == 0 ? 2 * V(2, 0, 3) - wn[d.g(i, 1, k)] : wn[d.g(i, 1, k)] - V(2, 0, 3) * dy;
+        un[d.g(i, ny + 1, k)] = T(2, 1, 1) == 0 ? 2 * V(2, 1, 1) - un[d.g(i, ny, k)] : un[d.g(i, ny, k)] + V(2, 1, 1) * dy;
+        vn[d.g(i, ny + 1, k)] = T(2, 1, 2) == 0 ? V(2, 1, 2) : vn[d.g(i, ny, k)] + V(2, 1, 2) * dy;
+        wn[d.g(i, ny + 1, k)] = T(2, 1, 3) == 0 ? 2 * V(2, 1, 3) - wn[d.g(i, ny, k)] : wn[d.g(i, ny, k)] + V(2, 1, 3) * dy;
+    }
+}
+
+__global__ void bc_rb_x(GridK g, BcK b, double *un, double *vn, double *wn, double *pn) {
+    const Dims d = g.d;
+    int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int k = blockIdx.y + 1;
+    if (j > d.ny || k > d.nz) return;
+    const int nx = d.nx;
+    const double dx = g.dx;
+    if (g.per[0]) {
+        un[d.g(0, j, k)] = un[d.g(nx, j, k)];
+        vn[d.g(0, j, k)] = vn[d.g(nx, j, k)];
+        wn[d.g(0, j, k)] = wn[d.g(nx, j, k)];
+        pn[d.g(0, j, k)] = pn[d.g(nx, j, k)];
+        un[d.g(nx + 1, j, k)] = un[d.g(1, j, k)];
+        vn[d.g(nx + 1, j, k)] = vn[d.g(1, j, k)];
+        wn[d.g(nx + 1, j, k)] = wn[d.g(1, j, k)];
+        pn[d.g(nx + 1, j, k)] = pn[d.g(1, j, k)];
+    } else {
+        un[d.g(1, j, k)] = T(1, 0, 1) == 0 ? V(1, 0, 1) : un[d.g(2, j, k)] - V(1, 0, 1) * dx;
+        vn[d.g(0, j, k)] = T(1, 0, 2) == 0 ? 2 * V(1, 0, 2) - vn[d.g(1, j, k)] : vn[d.g(1, j, k)] - V(1, 0, 2) * dx;
+        wn[d.g(0, j, k)] = T(1, 0, 3) == 0 ? 2 * V(1, 0, 3) - wn[d.g(1, j, k)] : wn[d.g(1, j, k)] - V(1, 0, 3) * dx;
+        un[d.g(nx + 1, j, k)] = T(1, 1, 1) == 0 ? V(1, 1, 1) : un[d.g(nx, j, k)] + V(1, 1, 1) * dx;
+        vn[d.g(nx + 1, j, k)] = T(1, 1, 2) == 0 ? 2 * V(1, 1, 2) - vn[d.g(nx, j, k)] : vn[d.g(nx, j, k)] + V(1, 1, 2) * dx;
+        wn[d.g(nx + 1, j, k)] = T(1, 1, 3) == 0 ? 2 * V(1, 1, 3) - wn[d.g(nx, j, k)] : wn[d.g(nx, j, k)] + V(1, 1, 3) * dx;
+    }
+}
+
+__global__ void bc_rb_z(GridK g, BcK b, double *un, double *vn, double *wn, double *pn) {
+    const Dims d = g.d;
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y + 1;
+    if (i > d.nx || j > d.ny) return;
+    const int nz = d.nz;
+    const double dz = g.dz;
+    if (g.per[2]) {
+        un[d.g(i, j, 0)] = un[d.g(i, j, nz)];
+        vn[d.g(i, j, 0)] = vn[d.g(i, j, nz)];
+        wn[d.g(i, j, 0)] = wn[d.g(i, j, nz)];
+        pn[d.g(i, j, 0)] = pn[d.g(i, j, nz)];
+        un[d.g(i, j, nz + 1)] = un[d.g(i, j, 1)];
+        vn[d.g(i, j, nz + 1)] = vn[d.g(i, j, 1)];
+        wn[d.g(i, j, nz + 1)] = wn[d.g(i, j, 1)];
+        pn[d.g(i, j, nz + 1)] = pn[d.g(i, j, 1)];
+    } else {
+        un[d.g(i, j, 0)] = T(3, 0, 1) == 0 ? 2 * V(3, 0, 1) - un[d.g(i, j, 1)] : un[d.g(i, j, 1)] - V(3, 0, 1) * dz;
+        vn[d.g(i, j, 0)] = T(3, 0, 2) == 0 ? 2 * V(3, 0, 2) - vn[d.g(i, j, 1)] : vn[d.g(i, j, 1)] - V(3, 0, 2) * dz;
+        wn[d.g(i, j, 1)] = T(3, 0, 3) == 0 ? V(3, 0, 3) : wn[d.g(i, j, 2)] - V(3, 0, 3) * dz;
+        un[d.g(i, j, nz + 1)] = T(3, 1, 1) == 0 ? 2 * V(3, 1, 1) - un[d.g(i, j, nz)] : un[d.g(i, j, nz)] + V(3, 1, 1) * dz;
+        vn[d.g(i, j, nz + 1)] = T(3, 1, 2) == 0 ? 2 * V(3, 1, 2) - vn[d.g(i, j, nz)] : vn[d.g(i, j, nz)] + V(3, 1, 2) * dz;
+        wn[d.g(i, j, nz + 1)] = T(3, 1, 3) == 0 ? V(3, 1, 3) : wn[d.g(i, j, nz)] + V(3, 1, 3) * dz;
+    }
+#undef T
+#undef V
+}
+
+hipError_t launch_apply_bc_redblack(mdx_handle *h) {
+    GridK g = grid_k(h);
+    BcK b = bc_k(h);
+    const Dims d = h->d;
+    hipLaunchKernelGGL(bc_rb_y, dim3((d.nx + 63) / 64, d.nz), dim3(64), 0, h->stream, g, b, h->un,
+                       h->vn, h->wn, h->pn);
+    hipLaunchKernelGGL(bc_rb_x, dim3((d.ny + 63) / 64, d.nz), dim3(64), 0, h->stream, g, b, h->un,
+                       h->vn, h->wn, h->pn);
+    hipLaunchKernelGGL(bc_rb_z, dim3((d.nx + 63) / 64, d.ny), dim3(64), 0, h->stream, g, b, h->un,
+                       h->vn, h->wn, h->pn);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- reductions
+// Maximum of non-negative doubles through their bit patterns (monotone as unsigned integers);
+// a NaN (positive quiet NaN from fabs) orders above +inf and therefore propagates.
+__device__ inline double nanmax(double a, double b) { return (a != a || a > b) ? a : b; }
+
+__device__ inline void block_max_to(double v, unsigned long long *out) {
+    __shared__ double part[16];
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
+    if ((t & 63) == 0) part[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+        double m = part[0];
+        for (int w = 1; w < (nt + 63) / 64; ++w) m = nanmax(m, part[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void reset_max_kernel(unsigned long long *p) { *p = 0ull; }
+
+hipError_t launch_reset_max(mdx_handle *h) {
+    hipLaunchKernelGGL(reset_max_kernel, dim3(1), dim3(1), 0, h->stream, h->red_max);
+    return hipGetLastError();
+}
+
+// ||div u||_inf over interior cells (periodic wrap as in the red-black solver)
+template <bool STRETCH>
+__global__ __launch_bounds__(BX * BY) void div_max_kernel(GridK g, const double *__restrict__ un,
+                                                          const double *__restrict__ vn,
+                                                          const double *__restrict__ wn, double idx,
+                                                          double idy, double idz,
+                                                          unsigned long long *out) {
+    const Dims d = g.d;
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double m = 0.0;
+    if (i <= d.nx && j <= d.ny) {
+        int ip = (g.per[0] && i == d.nx) ? 1 : i + 1;
+        int kp = (g.per[2] && k == d.nz) ? 1 : k + 1;
+        int jp = (g.per[1] && j == d.ny) ? 1 : j + 1;
+        size_t c = d.g(i, j, k);
+        double dvy = STRETCH ? (vn[d.g(i, jp, k)] - vn[c]) * g.ihc[j] : (vn[d.g(i, jp, k)] - vn[c]) * idy;
+        m = fabs((un[d.g(ip, j, k)] - un[c]) * idx + dvy + (wn[d.g(i, j, kp)] - wn[c]) * idz);
+    }
+    block_max_to(m, out);
+}
+
+hipError_t launch_div_max(mdx_handle *h, const double *u, const double *v, const double *w) {
+    GridK g = grid_k(h);
+    dim3 blk(BX, BY, 1), grd((h->d.nx + BX - 1) / BX, (h->d.ny + BY - 1) / BY, h->d.nz);
+    if (h->stretched)
+        hipLaunchKernelGGL(div_max_kernel<true>, grd, blk, 0, h->stream, g, u, v, w, 1.0 / h->dx,
+                           1.0 / h->dy, 1.0 / h->dz, h->red_max);
+    else
+        hipLaunchKernelGGL(div_max_kernel<false>, grd, blk, 0, h->stream, g, u, v, w, 1.0 / h->dx,
+                           1.0 / h->dy, 1.0 / h->dz, h->red_max);
+    return hipGetLastError();
+}
+
+// step.f90:210-258: max over every ghosted entry of |u|/dx, |v|/dy, |w|/dz
+__global__ void cfl_kernel(GridK g, const double *__restrict__ un, const double *__restrict__ vn,
+                           const double *__restrict__ wn, size_t n, unsigned long long *out) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    for (; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        size_t j = (q / (size_t)g.d.sx) % (size_t)g.d.sy;
+        double a = fabs(un[q] / g.dx);
+        double b = g.ihv ? fabs(vn[q] * g.ihv[j]) : fabs(vn[q] / g.dy);
+        double c = fabs(wn[q] / g.dz);
+        m = nanmax(m, nanmax(a, nanmax(b, c)));
+    }
+    block_max_to(m, out);
+}
+
+hipError_t launch_cfl_max(mdx_handle *h) {
+    GridK g = grid_k(h);
+    hipLaunchKernelGGL(cfl_kernel, dim3(1024), dim3(256), 0, h->stream, g, h->un, h->vn, h->wn, h->ng,
+                       h->red_max);
+    return hipGetLastError();
+}
+
+// Deterministic two-pass sums over interior u points: S1 = sum u*h_j, S2 = sum mu_u*h_j.
+static constexpr int SUM_BLOCKS = 512;
+static constexpr int SUM_THREADS = 256;
+
+__global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, const double *__restrict__ u,
+                                                                   const double *__restrict__ cu, double dtg,
+                                                                   double *part) {
+    const Dims d = g.d;
+    __shared__ double s1[SUM_THREADS], s2[SUM_THREADS];
+    double a = 0.0, b = 0.0;
+    const size_t n = (size_t)d.nx * d.ny * d.nz;
+    for (size_t q = (size_t)blockIdx.x * SUM_THREADS + threadIdx.x; q < n;
+         q += (size_t)SUM_BLOCKS * SUM_THREADS) {
+        int i = (int)(q % d.nx) + 1;
+        int j = (int)((q / d.nx) % d.ny) + 1;
+        int k = (int)(q / ((size_t)d.nx * d.ny)) + 1;
+        double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
+        size_t c = d.g(i, j, k);
+        a += u[c] * hj;
+        b += (cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0) * hj;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = SUM_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1[0];
+        part[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+// out[0] = bulk velocity, out[1] = mean mu_u, out[2] = du = (target - ub)/mean_mu
+__global__ __launch_bounds__(SUM_THREADS) void bulk_final_kernel(const double *part, double norm,
+                                                                 double target, double *out) {
+    __shared__ double s1[SUM_THREADS], s2[SUM_THREADS];
+    double a = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < SUM_BLOCKS; q += SUM_THREADS) {
+        a += part[2 * q];
+        b += part[2 * q + 1];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = SUM_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double ub = s1[0] / norm;
+        double sm = s2[0] / norm;
+        out[0] = ub;
+        out[1] = sm;
+        out[2] = (target - ub) / sm;
+    }
+}
+
+hipError_t launch_bulk_sum(mdx_handle *h, const double *u, double dtg, double *out_dev) {
+    GridK g = grid_k(h);
+    hipLaunchKernelGGL(bulk_partial_kernel, dim3(SUM_BLOCKS), dim3(SUM_THREADS), 0, h->stream, g, u,
+                       h->ibm_on ? h->cu : nullptr, dtg, h->red_part);
+    double norm = (double)h->d.nx * (double)h->d.nz * h->cfg.flow_height;
+    hipLaunchKernelGGL(bulk_final_kernel, dim3(1), dim3(SUM_THREADS), 0, h->stream, h->red_part, norm,
+                       h->cfg.ub_target, out_dev);
+    return hipGetLastError();
+}
+
+// u* += du * mu_u on interior u points; du read from device memory (no host round trip)
+__global__ __launch_bounds__(BX * BY) void flow_shift_kernel(GridK g, double *us, const double *cu,
+                                                             double dtg, const double *du_dev) {
+    const Dims d = g.d;
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nx || j > d.ny) return;
+    const double du = du_dev[2];
+    size_t c = d.g(i, j, k);
+    double mu = cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0;
+    us[c] = us[c] + du * mu;
+}
+
+hipError_t launch_flow_shift(mdx_handle *h, double dtg, const double *du_dev) {
+    GridK g = grid_k(h);
+    dim3 blk(BX, BY, 1), grd((h->d.nx + BX - 1) / BX, (h->d.ny + BY - 1) / BY, h->d.nz);
+    hipLaunchKernelGGL(flow_shift_kernel, grd, blk, 0, h->stream, g, h->us, h->ibm_on ? h->cu : nullptr,
+                       dtg, du_dev);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- initial field
+__device__ inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ inline double uniform01(uint32_t seed, uint32_t comp, uint64_t index) {
+    uint64_t key = ((uint64_t)seed << 34) + ((uint64_t)comp << 32) + index;
+    return (double)(splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Poiseuille u = 1.5(1-(y/h)^2) + amp(2U-1), v,w = amp(2U-1) on interior faces; with a geometry
+// (use_geo), samples inside the body (coefficient == solid) are zero and h = hx_u[i].
+__global__ __launch_bounds__(BX * BY) void initial_field_kernel(
+    GridK g, const double *__restrict__ yc, const double *__restrict__ hx_u, double ymid, uint32_t seed,
+    double amp, int use_geo, const double *cu, const double *cv, const double *cw, double solid,
+    double *us, double *vs, double *ws) {
+    const Dims d = g.d;
+    const int i = blockIdx.x * BX + threadIdx.x + 1;
+    const int j = blockIdx.y * BY + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nx || j > d.ny) return;
+    const int ny0 = g.per[1] ? 1 : 2;
+    size_t c = d.g(i, j, k);
+    double h = use_geo ? hx_u[i] : 1.0;
+    bool fu = !use_geo || cu[c] != solid;
+    bool fv = !use_geo || cv[c] != solid;
+    bool fw = !use_geo || cw[c] != solid;
+    double q = (yc[j] - ymid) / h;
+    double base = 1.5 * (1.0 - q * q);
+    us[c] = fu ? base + amp * (2.0 * uniform01(seed, 0, c) - 1.0) : 0.0;
+    if (j >= ny0) vs[c] = fv ? amp * (2.0 * uniform01(seed, 1, c) - 1.0) : 0.0;
+    ws[c] = fw ? amp * (2.0 * uniform01(seed, 2, c) - 1.0) : 0.0;
+}
+
+hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int use_geo,
+                                const double *hx_u_dev, double ymid) {
+    GridK g = grid_k(h);
+    dim3 blk(BX, BY, 1), grd((h->d.nx + BX - 1) / BX, (h->d.ny + BY - 1) / BY, h->d.nz);
+    hipLaunchKernelGGL(initial_field_kernel, grd, blk, 0, h->stream, g, h->d_yc, hx_u_dev, ymid, seed, amp,
+                       use_geo, h->cu, h->cv, h->cw, 1.0e30 / h->cfg.re, h->us, h->vs, h->ws);
+    return hipGetLastError();
+}
+
+}  // namespace mdx

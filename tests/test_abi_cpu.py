@@ -1,0 +1,66 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point declared in
+include/mobydiff_hip.h, and rejects invalid configurations before touching the device."""
+import os
+import re
+
+import pytest
+
+import mobydiff_amd as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "mobydiff_hip.h")).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(mdx_\w+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = M.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def _expect_error(cfg, code, text):
+    h = M.C.c_void_p()
+    rc = M.lib().mdx_create(M.C.byref(cfg), M.C.byref(h))
+    assert rc == code
+    assert text in M.lib().mdx_last_error().decode()
+    assert not h.value
+
+
+def test_redblack_rejects_odd_nx():
+    # pressure_redblack.f90:16-18
+    cfg = M.channel_config(9, 8, 8, 1, 1, 1, 100, solver=M.SOLVER_SOLA)
+    _expect_error(cfg, 1, "even nx and nz")
+
+
+def test_fft_requires_powers_of_two():
+    cfg = M.channel_config(12, 8, 8, 1, 1, 1, 100, solver=M.SOLVER_FFT)
+    _expect_error(cfg, 3, "powers of two")
+
+
+def test_sor_rejects_unsupported_bc():
+    cfg = M.channel_config(8, 8, 8, 1, 1, 1, 100, solver=M.SOLVER_SOR, p_neumann_y=False)
+    _expect_error(cfg, 3, "Neumann p")
+
+
+def test_sor_requires_periodic_xz():
+    cfg = M.channel_config(8, 8, 8, 1, 1, 1, 100, solver=M.SOLVER_SOR, periodic=(0, 0, 1))
+    _expect_error(cfg, 3, "periodic")
+
+
+def test_bad_omega_and_sizes():
+    _expect_error(M.channel_config(8, 8, 8, 1, 1, 1, 100, omega=2.5), 1, "omega")
+    _expect_error(M.channel_config(1, 8, 8, 1, 1, 1, 100), 1, ">= 2")
+    _expect_error(M.channel_config(8, 8, 8, 1, 1, 1, -1), 1, "positive")
+    _expect_error(M.channel_config(8, 8, 8, 1, 1, 1, 100, precision=M.PREC_MIXED, inner_iters=0), 1,
+                  "inner_iters")
+
+
+def test_null_arguments():
+    assert M.lib().mdx_create(None, None) == 1
+    assert M.lib().mdx_momentum(None, M.C.c_double(0), M.C.c_double(0), M.C.c_double(0)) == 1
+    assert M.lib().mdx_destroy(None) == 0

@@ -1,0 +1,260 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on the same inputs.
+
+Bar: bit-exact for the FP64 operators whose expression order is shared (momentum, boundary
+conditions, SOLA sweeps, pressure-correction SOR, norms, IBM coefficients, the initial field);
+FFT-based projections agree to 1e-12 relative (different, equally exact, FFT orderings); the
+mixed-precision solve reaches the same ||div||_inf tolerance and agrees to 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+import mobydiff_amd as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("un", "vn", "wn", "pn", "us", "vs", "ws", "oru", "orv", "orw")
+
+
+def _cfg_from(g, **kw):
+    c = M.channel_config(g.nx, g.ny, g.nz, g.lx, g.ly, g.lz, g.re, forcing=g.forcing,
+                         periodic=g.periodic, p_neumann_y=False, stretch_beta=g.beta or 0.0, **kw)
+    for q in range(24):
+        c.bc_type[q] = int(g.bc_type[q])
+        c.bc_value[q] = float(g.bc_value[q])
+    return c
+
+
+def _channel(g, f, coef=None, **kw):
+    ch = M.Channel(_cfg_from(g, **kw))
+    ch.upload(**f)
+    if coef is not None:
+        ch.set_ibm(*coef)
+    return ch
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _bc_table(g, rng):
+    for d in (1, 2, 3):
+        for s in (0, 1):
+            for v in range(4):
+                g.bc_type[O.bci(d, s, v)] = int(rng.random() < 0.5)
+                g.bc_value[O.bci(d, s, v)] = rng.standard_normal()
+
+
+# ------------------------------------------------------------------------------ operators
+@pytest.mark.parametrize("shape", [(16, 12, 8), (40, 20, 6)])
+@pytest.mark.parametrize("per", [(1, 0, 1), (0, 1, 0)])
+@pytest.mark.parametrize("ibm", [False, True])
+@pytest.mark.parametrize("beta", [None, 1.5])
+def test_momentum_bit_exact(shape, per, ibm, beta):
+    rng = np.random.default_rng(1)
+    g = O.Grid(*shape, lx=1.3, ly=2.0, lz=0.7, re=300.0, forcing=(1.0, 0.2, -0.1), periodic=per, beta=beta)
+    f = O.random_fields(g, rng)
+    coef = tuple(np.abs(rng.standard_normal(g.gshape)) * 30 for _ in range(3)) if ibm else None
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOLA)
+    ch.momentum(0.1, -0.03, 0.05)
+    out = ch.download()
+    O.momentum(g, f, coef, 0.1, -0.03, 0.05)
+    for k in KEYS:
+        assert np.array_equal(out[k], f[k]), k
+
+
+def test_momentum_bit_exact_wavy_production_shape():
+    g = O.Grid(192, 128, 96, lx=4.0, ly=2.2, lz=2.0, re=2000.0, periodic=(1, 0, 1))
+    geo = O.Geometry(kind=1, A=0.1, lam=1.0, half_height=1.0)
+    coef = O.ibm_coefs(g, geo)
+    rng = np.random.default_rng(2)
+    f = O.random_fields(g, rng, amp=0.5)
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOR)
+    ch.momentum(5e-4 * 8 / 15, 0.0, 5e-4 * 8 / 15)
+    out = ch.download(("us", "vs", "ws", "oru"))
+    O.momentum(g, f, coef, 5e-4 * 8 / 15, 0.0, 5e-4 * 8 / 15)
+    for k in ("us", "vs", "ws", "oru"):
+        assert np.array_equal(out[k], f[k]), k
+
+
+def test_apply_bc_fft_bit_exact():
+    rng = np.random.default_rng(3)
+    g = O.Grid(16, 10, 8)
+    f = O.random_fields(g, rng)
+    ch = _channel(g, f, solver=M.SOLVER_FFT)
+    ch.apply_bc()
+    out = ch.download()
+    O.apply_bc_fft(g, f)
+    for k in KEYS:
+        assert np.array_equal(out[k], f[k]), k
+
+
+@pytest.mark.parametrize("per", [(1, 0, 1), (1, 1, 1), (0, 0, 0), (0, 1, 1)])
+def test_apply_bc_redblack_bit_exact(per):
+    rng = np.random.default_rng(4)
+    g = O.Grid(12, 10, 6, lx=1.0, ly=2.0, lz=0.5, periodic=per)
+    _bc_table(g, rng)
+    f = O.random_fields(g, rng)
+    ch = _channel(g, f, solver=M.SOLVER_SOLA)
+    ch.apply_bc_redblack()
+    out = ch.download()
+    O.apply_bc_redblack(g, f)
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(out[k], f[k]), k
+
+
+@pytest.mark.parametrize("per", [(1, 0, 1), (1, 1, 1), (0, 0, 0), (0, 1, 1)])
+@pytest.mark.parametrize("niter", [1, 3])
+def test_sola_projection_bit_exact(per, niter):
+    rng = np.random.default_rng(5)
+    g = O.Grid(12, 10, 6, lx=1.0, ly=1.5, lz=0.5, periodic=per)
+    _bc_table(g, rng)
+    f = O.random_fields(g, rng)
+    coef = tuple(np.abs(rng.standard_normal(g.gshape)) * 20 for _ in range(3))
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOLA, niter=niter, omega=1.5)
+    st = ch.pressure_projection(0.03)
+    assert st.iterations == niter
+    out = ch.download()
+    O.sola_redblack(g, f, coef, niter, 1.5, 0.03)
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(out[k], f[k]), k
+
+
+@pytest.mark.parametrize("per_y", [0, 1])
+@pytest.mark.parametrize("ibm", [False, True])
+@pytest.mark.parametrize("beta", [None, 1.5])
+def test_sor_fixed_iterations_bit_exact(per_y, ibm, beta):
+    if beta and per_y:
+        pytest.skip("stretched grid is a wall-bounded grid")
+    rng = np.random.default_rng(6)
+    g = O.Grid(16, 12, 8, lx=1.0, ly=1.5, lz=0.5, periodic=(1, per_y, 1), beta=beta)
+    g.bc_type[O.bci(2, 0, 0)] = 1
+    g.bc_type[O.bci(2, 1, 0)] = 1
+    f = O.random_fields(g, rng)
+    coef = tuple(np.abs(rng.standard_normal(g.gshape)) * 20 for _ in range(3)) if ibm else None
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-300, max_iter=7, check_every=7)
+    st = ch.pressure_projection(0.03)
+    assert st.iterations == 7 and st.converged == 0
+    out = ch.download()
+    ost, phi, rhs = O.projection_sor(g, f, coef, 1.8, 0.0, 7, 7, 0.03)
+    assert ost.iterations == 7
+    # the reported residual is the residual of the final iterate
+    assert st.div_inf == O.sor_residual_inf(g, coef, 0.03, rhs, phi)
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(out[k], f[k]), k
+
+
+def test_sor_to_tolerance_bit_exact_wavy():
+    g = O.Grid(48, 32, 24, lx=4.0, ly=2.2, lz=2.0, re=2000.0, periodic=(1, 0, 1))
+    geo = O.Geometry(kind=1, A=0.1, lam=1.0, half_height=1.0)
+    coef = O.ibm_coefs(g, geo)
+    rng = np.random.default_rng(7)
+    f = O.random_fields(g, rng, amp=0.05)
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, max_iter=100000, check_every=10)
+    st = ch.pressure_projection(5e-4 * 64 / 120)
+    out = ch.download()
+    ost, _, _ = O.projection_sor(g, f, coef, 1.8, 1e-8, 100000, 10, 5e-4 * 64 / 120)
+    assert st.converged == 1 and ost.converged == 1
+    assert st.iterations == ost.iterations
+    assert st.div_inf < 1e-8
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(out[k], f[k]), k
+    assert ch.div_norm() == O.div_inf(g, f)
+
+
+def test_mixed_precision_sor_reaches_tolerance():
+    g = O.Grid(48, 32, 24, lx=4.0, ly=2.2, lz=2.0, re=2000.0, periodic=(1, 0, 1))
+    geo = O.Geometry(kind=1, A=0.1, lam=1.0, half_height=1.0)
+    coef = O.ibm_coefs(g, geo)
+    rng = np.random.default_rng(8)
+    f = O.random_fields(g, rng, amp=0.05)
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, max_iter=100000, check_every=10,
+                  precision=M.PREC_MIXED, inner_iters=10)
+    st = ch.pressure_projection(5e-4 * 64 / 120)
+    out = ch.download()
+    ost, _, _ = O.projection_sor(g, f, coef, 1.8, 1e-8, 100000, 10, 5e-4 * 64 / 120)
+    assert st.converged == 1 and st.div_inf < 1e-8
+    assert ch.div_norm() < 1.5e-8
+    assert st.iterations <= 1.5 * ost.iterations + 10
+    for k in ("un", "vn", "wn"):
+        assert _rel(out[k][1:-1, 1:-1, 1:-1], f[k][1:-1, 1:-1, 1:-1]) < 1e-6, k
+    assert _rel(out["pn"][1:-1, 1:-1, 1:-1], f["pn"][1:-1, 1:-1, 1:-1]) < 1e-4
+
+
+@pytest.mark.parametrize("beta", [None, 1.5])
+@pytest.mark.parametrize("shape", [(16, 12, 8), (64, 24, 32)])
+def test_fft_projection_matches_oracle(beta, shape):
+    rng = np.random.default_rng(9)
+    g = O.Grid(*shape, lx=2.0, ly=2.0, lz=1.0, periodic=(1, 0, 1), beta=beta)
+    f = O.random_fields(g, rng)
+    ch = _channel(g, f, solver=M.SOLVER_FFT)
+    ch.apply_bc()
+    ch.pressure_projection(0.07)
+    ch.apply_bc()
+    out = ch.download()
+    O.apply_bc_fft(g, f)
+    O.projection_fft(g, f, None, 0.07)
+    O.apply_bc_fft(g, f)
+    for k in ("un", "vn", "wn", "pn"):
+        assert _rel(out[k], f[k]) < 1e-12, k
+    assert ch.div_norm() < 1e-11
+
+
+def test_cfl_and_div_norm_bit_exact():
+    rng = np.random.default_rng(10)
+    g = O.Grid(20, 14, 10, lx=2.0, ly=1.0, lz=0.5)
+    f = O.random_fields(g, rng)
+    ch = _channel(g, f, solver=M.SOLVER_SOLA)
+    assert ch.get_cfl() == O.get_cfl(g, f)
+    assert ch.div_norm() == O.div_inf(g, f)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_ibm_coefficients_bit_exact(kind):
+    if kind == 0:
+        g = O.Grid(32, 24, 6, lx=1.0, ly=1.0, lz=0.25, re=100.0)
+        geo = O.Geometry(kind=0, second_order=1, n_wave_x=1, amp_x=5 * g.dy)
+        mgeo = M.bump_geometry()
+    else:
+        g = O.Grid(48, 40, 6, lx=4.0, ly=2.2, lz=0.5, re=2000.0)
+        geo = O.Geometry(kind=1, second_order=1, A=0.13, lam=1.0, half_height=1.0)
+        mgeo = M.wavy_geometry(A=0.13)
+    ch = M.Channel(_cfg_from(g, solver=M.SOLVER_SOR))
+    mine = ch.compute_ibm(mgeo)
+    ref = O.ibm_coefs(g, geo)
+    for a, b in zip(mine, ref):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("solver", [M.SOLVER_FFT, M.SOLVER_SOR])
+def test_initial_field_matches_oracle(solver):
+    dt = 1e-3
+    if solver == M.SOLVER_FFT:
+        g = O.Grid(32, 24, 16, lx=4 * np.pi, ly=2.0, lz=2 * np.pi, re=2800.0, periodic=(1, 0, 1), beta=1.5)
+        geo = None
+    else:
+        g = O.Grid(48, 32, 24, lx=4.0, ly=2.2, lz=2.0, re=2000.0, periodic=(1, 0, 1))
+        geo = O.Geometry(kind=1, A=0.1, lam=1.0, half_height=1.0)
+    g.bc_type[O.bci(2, 0, 0)] = 1
+    g.bc_type[O.bci(2, 1, 0)] = 1
+    ch = M.Channel(_cfg_from(g, solver=solver, omega=1.8, tol=1e-8, check_every=10))
+    coef = None
+    if geo is not None:
+        ch.compute_ibm(M.wavy_geometry(A=0.1))
+        coef = O.ibm_coefs(g, geo)
+    ch.init_fields(seed=3, amp=0.1, geo=M.wavy_geometry(A=0.1) if geo is not None else None, dt=dt)
+    out = ch.download()
+    f = O.new_fields(g)
+    O.initial_field(g, f, 3, 0.1, geo)
+    if solver == M.SOLVER_FFT:
+        O.apply_bc_fft(g, f)
+        O.projection_fft(g, f, None, dt * 64 / 120)
+        O.apply_bc_fft(g, f)
+        tol = 1e-13
+    else:
+        O.projection_sor(g, f, coef, 1.8, 1e-8, 1000000, 10, dt * 64 / 120)
+        O.apply_bc_redblack(g, f)
+        tol = 0.0
+    for k in ("us", "vs", "ws", "un", "vn", "wn"):
+        assert _rel(out[k], f[k]) <= tol, k
+    assert not out["pn"].any()
