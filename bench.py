@@ -128,7 +128,8 @@ def main():
         "config": {"workload": "wavy-wall channel 192x128x96 (A=0.1, Re_b=2000, dt=5e-4), RK3 step with "
                                "red-black SOR to ||div||_inf<1e-8: FP32 sweeps in FP64 defect correction",
                    "grid": [NX, NY, NZ], "learned_guess": False, "omega": OMEGA, "tol": TOL,
-                   "check_every": args.check_every, "inner_iters": args.inner, "parallelism": "1 GPU"},
+                   "check_every": args.check_every, "inner_iters": args.inner, "parallelism": "1 GPU",
+                   "sor_kernel": head.info()},
         "phase_ms_per_step": split(t_h, args.steps),
         "sor_iterations_per_stage": sum(map(sum, its_h)) / (3.0 * args.steps),
         "div_inf_final": div_h,

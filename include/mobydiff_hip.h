@@ -82,6 +82,7 @@ typedef struct {
     double div0_inf;         /* ||div||_inf of the initial guess */
     double div_inf;          /* ||div||_inf after the solve (from the final residual) */
     double guess_div_inf;    /* ||div||_inf of the learned guess (when used) */
+    int32_t lds_sweeps;      /* 1: sweeps ran in the LDS-resident kernel, 0: streaming kernels */
 } mdx_solve_stats;
 
 typedef struct {
@@ -153,6 +154,41 @@ int mdx_reset_phase_times(mdx_handle *h);
 
 /* wall-normal coordinates of cell centres and v-faces (box coordinates, length ny+2 each) */
 int mdx_get_y_coordinates(mdx_handle *h, double *yc, double *yv);
+
+/* one line of text describing the active kernels (which red-black sweep kernel, its decomposition,
+ * or why the streaming sweeps are used); truncated to len-1 characters */
+int mdx_get_info(mdx_handle *h, char *buf, int32_t len);
+
+/* ---------------------------------------------------------------- learned initial guess
+ * A 3-level 3-D U-Net (16/32/64 channels, 3x3x3 convolutions, BF16 weights and activations,
+ * FP32 accumulation on MFMA) maps (div u*, previous pressure) to a pressure-correction guess for
+ * the red-black SOR.  No counterpart exists in the reference solver.  Parameters are FP32 masters
+ * in a flat layout: per layer W(tap, cin, cout) then b(cout), layers e1a e1b e2a e2b b3a b3b d2a
+ * d2b d1a d1b head. */
+typedef struct mdx_net mdx_net;
+int64_t mdx_net_param_count(void);
+int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, mdx_net **out);
+int mdx_net_destroy(mdx_net *net);
+/* c0: target scale (Phi = output * rms(div u*) * c0); get may pass NULL, set ignores c0 <= 0 */
+int mdx_net_get_params(mdx_net *net, float *params, double *c0);
+int mdx_net_set_params(mdx_net *net, const float *params, double c0);
+int mdx_net_get_grads(mdx_net *net, float *grads);
+int mdx_net_zero_grads(mdx_net *net);
+/* explicit inputs x2[v][2] (the two input channels, already scaled), output out[v] */
+int mdx_net_forward_host(mdx_net *net, const float *x2, float *out);
+/* MSE against target[v]; adds this sample's gradients / batch to the accumulated gradients */
+int mdx_net_accumulate_grad_host(mdx_net *net, const float *x2, const float *target, int32_t batch,
+                                 float *loss);
+/* Adam (lr, beta1 0.9, beta2 0.999, eps 1e-8) on the accumulated gradients, which it then zeroes */
+int mdx_net_adam_step(mdx_net *net, float lr);
+/* replay buffer of solver-generated (input, converged Phi) pairs, filled by attached handles */
+int mdx_net_replay_reserve(mdx_net *net, int32_t capacity);
+int mdx_net_replay_count(mdx_net *net, int32_t *count);
+int mdx_net_train(mdx_net *net, int32_t steps, int32_t batch, float lr, uint64_t seed, float *mean_loss);
+int mdx_net_eval_replay(mdx_net *net, int32_t count, float *mean_loss);
+/* attach (or detach with net = NULL) a network to a solver handle: cfg.guess = MDX_GUESS_LEARNED
+ * uses it for the initial guess; capture != 0 appends a pair per converged SOR solve */
+int mdx_attach_net(mdx_handle *h, mdx_net *net, int32_t capture);
 
 #ifdef __cplusplus
 }

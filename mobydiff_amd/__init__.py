@@ -44,7 +44,7 @@ class Geometry(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("fallback", C.c_int32),
                 ("checks", C.c_int32), ("div0_inf", C.c_double), ("div_inf", C.c_double),
-                ("guess_div_inf", C.c_double)]
+                ("guess_div_inf", C.c_double), ("lds_sweeps", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -56,7 +56,8 @@ class StepStats(C.Structure):
 
 class PhaseTimes(C.Structure):
     _fields_ = [("stencil_ms", C.c_double), ("poisson_ms", C.c_double),
-                ("projection_ms", C.c_double), ("guess_ms", C.c_double), ("stages", C.c_int64)]
+                ("projection_ms", C.c_double), ("guess_ms", C.c_double), ("stages", C.c_int64),
+                ("sweep_ms", C.c_double), ("sweep_launches", C.c_int64), ("sweep_cells", C.c_int64)]
 
 
 def bci(d, side, var):
@@ -250,8 +251,106 @@ class Channel:
     def reset_phase_times(self):
         _check(lib().mdx_reset_phase_times(self.h))
 
+    def attach_net(self, net, capture=False):
+        _check(lib().mdx_attach_net(self.h, net.h if net is not None else None, C.c_int32(1 if capture else 0)))
+        self._net = net
+
+    def info(self):
+        buf = C.create_string_buffer(1024)
+        _check(lib().mdx_get_info(self.h, buf, C.c_int32(1024)))
+        return buf.value.decode()
+
     def y_coordinates(self):
         yc = np.zeros(self.ny + 2)
         yv = np.zeros(self.ny + 2)
         _check(lib().mdx_get_y_coordinates(self.h, _p(yc), _p(yv)))
         return yc, yv
+
+
+_PF = C.POINTER(C.c_float)
+
+
+def _pf(a):
+    if a is None:
+        return None
+    if a.dtype != np.float32 or not a.flags.c_contiguous:
+        raise TypeError("arrays must be C-contiguous float32")
+    return a.ctypes.data_as(_PF)
+
+
+class Net:
+    """The pressure-guess U-Net (BF16/MFMA HIP kernels); see mdx_net_* in include/mobydiff_hip.h."""
+
+    def __init__(self, nx, ny, nz, seed=0):
+        self.h = C.c_void_p()
+        _check(lib().mdx_net_create(C.c_int32(nx), C.c_int32(ny), C.c_int32(nz), C.c_uint64(seed),
+                                    C.byref(self.h)))
+        self.shape = (nz, ny, nx)
+        self.nparam = lib().mdx_net_param_count() if False else None
+        lib().mdx_net_param_count.restype = C.c_int64
+        self.nparam = int(lib().mdx_net_param_count())
+
+    def close(self):
+        if self.h:
+            lib().mdx_net_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparam, np.float32)
+        c0 = C.c_double()
+        _check(lib().mdx_net_get_params(self.h, _pf(p), C.byref(c0)))
+        return p, c0.value
+
+    def set_params(self, p, c0=0.0):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().mdx_net_set_params(self.h, _pf(p), C.c_double(c0)))
+
+    def get_grads(self):
+        g = np.zeros(self.nparam, np.float32)
+        _check(lib().mdx_net_get_grads(self.h, _pf(g)))
+        return g
+
+    def zero_grads(self):
+        _check(lib().mdx_net_zero_grads(self.h))
+
+    def forward(self, x2):
+        """x2: (nz, ny, nx, 2) float32 -> (nz, ny, nx) float32"""
+        x2 = np.ascontiguousarray(x2, dtype=np.float32)
+        out = np.zeros(self.shape, np.float32)
+        _check(lib().mdx_net_forward_host(self.h, _pf(x2), _pf(out)))
+        return out
+
+    def accumulate_grad(self, x2, target, batch=1):
+        x2 = np.ascontiguousarray(x2, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().mdx_net_accumulate_grad_host(self.h, _pf(x2), _pf(target), C.c_int32(batch), C.byref(loss)))
+        return loss.value
+
+    def adam_step(self, lr=1e-3):
+        _check(lib().mdx_net_adam_step(self.h, C.c_float(lr)))
+
+    def replay_reserve(self, capacity):
+        _check(lib().mdx_net_replay_reserve(self.h, C.c_int32(capacity)))
+
+    def replay_count(self):
+        n = C.c_int32()
+        _check(lib().mdx_net_replay_count(self.h, C.byref(n)))
+        return n.value
+
+    def train(self, steps, batch=4, lr=1e-3, seed=0):
+        loss = C.c_float()
+        _check(lib().mdx_net_train(self.h, C.c_int32(steps), C.c_int32(batch), C.c_float(lr), C.c_uint64(seed),
+                                   C.byref(loss)))
+        return loss.value
+
+    def eval_replay(self, count):
+        loss = C.c_float()
+        _check(lib().mdx_net_eval_replay(self.h, C.c_int32(count), C.byref(loss)))
+        return loss.value

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "mdx_internal.h"
+#include "mdx_unet.h"
 
 static thread_local std::string g_last_error;
 
@@ -191,7 +192,7 @@ static void free_all(mdx_handle *h) {
                     h->cu, h->cv, h->cw, h->d_ihc, h->d_ihv, h->d_yc, h->phi, h->phi2, h->rhs,
                     h->e32, h->r32, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
-                    h->red_part, h->red_max};
+                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -207,7 +208,8 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     *out = nullptr;
     const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
     MDX_CHECK(nx >= 2 && ny >= 2 && nz >= 2, MDX_ERR_ARG, "mdx_create: nx, ny, nz must be >= 2");
-    MDX_CHECK((double)(nx + 2) * (ny + 2) * (nz + 2) < 2.0e9, MDX_ERR_ARG, "mdx_create: grid too large");
+    MDX_CHECK((double)(nx + 2) * (ny + 2) * (nz + 2) < 2.0e9 && xcd_grid(nx, ny, nz, 64, 4) <= MAX_PARTS,
+              MDX_ERR_ARG, "mdx_create: grid too large");
     MDX_CHECK(cfg->lx > 0 && cfg->ly > 0 && cfg->lz > 0 && cfg->re > 0, MDX_ERR_ARG,
               "mdx_create: lengths and re must be positive");
     if (cfg->solver == MDX_SOLVER_FFT) {
@@ -276,8 +278,12 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         (e = hipMemcpy(h->d_ihv, h->ihv.data(), (ny + 2) * 8, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(h->d_yc, h->yc.data(), (ny + 2) * 8, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "metric upload");
-    if ((e = dalloc(&h->red_part, 4096)) != hipSuccess) return fail(e, "reduction buffer");
-    if ((e = dalloc(&h->red_max, 4)) != hipSuccess) return fail(e, "reduction buffer");
+    if ((e = dalloc(&h->red_part, 4096)) != hipSuccess || (e = dalloc(&h->max_part, MAX_PARTS)) != hipSuccess)
+        return fail(e, "reduction buffer");
+    if ((e = dalloc(&h->red_max, 4)) != hipSuccess || (e = hipMemset(h->red_max, 0, 32)) != hipSuccess)
+        return fail(e, "reduction buffer");
+    h->rb_err = reinterpret_cast<int *>(h->red_max + 1);
+    if (const char *nl = std::getenv("MOBYDIFF_NO_LDS_SOR")) h->use_lds_sor = !(nl[0] == '1');
     if ((e = hipHostMalloc((void **)&h->h_pinned, 64 * sizeof(double))) != hipSuccess) return fail(e, "pinned buffer");
 
     if (cfg->solver == MDX_SOLVER_SOR) {
@@ -377,6 +383,7 @@ static void invalidate_graphs(mdx_handle *h) {
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
         s = mdx_handle::GraphSlot{};
     }
+    h->rb_state = 0;   // the LDS solver's plan depends on the IBM state too
 }
 
 static int upload_ibm(mdx_handle *h) {
@@ -532,10 +539,57 @@ extern "C" int mdx_apply_bc_redblack(mdx_handle *h) {
     return MDX_OK;
 }
 
+// Reads the max reduction (red_max[0]) and the LDS solver's wait-timeout word (red_max[1]) in
+// one copy; a timeout is an error (the kernel drained, but its result is not trustworthy).
 static int read_max(mdx_handle *h, double *out) {
-    MDX_HIP(hipMemcpyAsync(h->h_pinned, h->red_max, 8, hipMemcpyDeviceToHost, h->stream));
+    MDX_HIP(hipMemcpyAsync(h->h_pinned, h->red_max, 16, hipMemcpyDeviceToHost, h->stream));
     MDX_HIP(hipStreamSynchronize(h->stream));
     *out = h->h_pinned[0];
+    int err;
+    std::memcpy(&err, &h->h_pinned[1], sizeof err);
+    if (err) {
+        mdx_set_error("LDS red-black solver: a neighbour wait exceeded its spin limit");
+        return MDX_ERR_HIP;
+    }
+    return MDX_OK;
+}
+
+// Plan and allocate the LDS-resident solver once the IBM state is known (uniform grid, z-invariant
+// or absent IBM coefficients); rb_state = -1 leaves the streaming sweeps in charge.
+static int rb_prepare(mdx_handle *h) {
+    if (h->rb_state != 0) return MDX_OK;
+    h->rb_state = -1;
+    if (!h->use_lds_sor) { h->rb_note = "streaming sweeps: disabled by MOBYDIFF_NO_LDS_SOR"; return MDX_OK; }
+    if (h->stretched) { h->rb_note = "streaming sweeps: stretched grid"; return MDX_OK; }
+    if (h->ibm_on && !h->coef_plane) { h->rb_note = "streaming sweeps: IBM coefficients vary in z"; return MDX_OK; }
+    RbPlanOut p64{}, p32{};
+    std::string why;
+    if (!rb_plan(h, false, p64, why) || !rb_plan(h, true, p32, why)) {
+        h->rb_note = "streaming sweeps: " + why;
+        return MDX_OK;
+    }
+    auto desc = [](const RbPlanOut &p) {
+        return std::to_string(p.px) + "x" + std::to_string(p.py) + "x" + std::to_string(p.pz) + " workgroups of " +
+               std::to_string(p.bx) + "x" + std::to_string(p.by) + "x" + std::to_string(p.bz) + " cells, LDS " +
+               std::to_string(p.lds) + " B";
+    };
+    h->rb_note = "LDS-resident sweeps: fp64 " + desc(p64) + "; fp32 " + desc(p32);
+    size_t need = 0;
+    for (const RbPlanOut *p : {&p64, &p32}) {
+        const size_t fmax = std::max((size_t)p->by * p->bz, std::max((size_t)p->bx * p->bz, (size_t)p->bx * p->by));
+        const size_t nwg = (size_t)p->px * p->py * p->pz;
+        need = std::max(need, nwg * 12 * fmax * (p == &p64 ? 8 : 4));
+    }
+    const size_t nflags = std::max((size_t)p64.px * p64.py * p64.pz, (size_t)p32.px * p32.py * p32.pz);
+    if (h->rb_faces) { (void)hipFree(h->rb_faces); h->rb_faces = nullptr; }
+    if (h->rb_flags) { (void)hipFree(h->rb_flags); h->rb_flags = nullptr; }
+    MDX_HIP(hipMalloc(&h->rb_faces, need));
+    MDX_HIP(hipMalloc((void **)&h->rb_flags, nflags * sizeof(unsigned long long)));
+    MDX_HIP(hipMemset(h->rb_flags, 0, nflags * sizeof(unsigned long long)));
+    h->rb_epoch = 0;
+    h->rb_plan64 = p64;
+    h->rb_plan32 = p32;
+    h->rb_state = 1;
     return MDX_OK;
 }
 
@@ -585,7 +639,19 @@ static hipError_t sweep_launches(mdx_handle *h, bool fp32, int m) {
 static int sweep_batch(mdx_handle *h, bool fp32, int m) {
     ev_record(h, 5);
     auto &slot = h->graphs[fp32 ? 1 : 0];
-    if (h->use_graphs && m >= 2) {
+    if (h->rb_state == 1) {
+        // LDS-resident sweeps; the FP32 defect solve starts from zero inside the kernel
+        hipError_t e = launch_rb_lds(h, fp32, m, fp32 ? 1 : 0);
+        if (e != hipSuccess) {
+            // the runtime refused the cooperative launch (nothing ran): use the streaming sweeps
+            (void)hipGetLastError();
+            h->rb_state = -1;
+            h->rb_note = std::string("streaming sweeps: cooperative launch refused (") + hipGetErrorString(e) + ")";
+            if (fp32) MDX_HIP(launch_zero32(h, h->e32, h->ni));   // the streaming sweeps start from e = 0
+        }
+    }
+    if (h->rb_state == 1) {
+    } else if (h->use_graphs && m >= 2) {
         if (!slot.exec || slot.m != m || slot.ibm != h->ibm_on) {
             if (slot.exec) { (void)hipGraphExecDestroy(slot.exec); slot.exec = nullptr; }
             hipGraph_t graph = nullptr;
@@ -622,15 +688,20 @@ static void sweep_account(mdx_handle *h) {
 }
 
 // Plain FP64 red-black SOR to ||div||_inf < tol; Phi starts at zero (the previous-step pressure).
-static int sor_solve64(mdx_handle *h, mdx_solve_stats *st) {
+// r_first >= 0: the residual of the initial iterate is already known (learned-guess selection).
+static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
     const mdx_config &c = h->cfg;
     double r = 0.0;
     int it = 0, checks = 0;
     for (;;) {
-        MDX_HIP(launch_reset_max(h));
-        MDX_HIP(launch_sor_residual64(h, h->phi));
-        int rc = read_max(h, &r);
-        if (rc) return rc;
+        int rc;
+        if (checks == 0 && r_first >= 0.0) {
+            r = r_first;
+        } else {
+            MDX_HIP(launch_reset_max(h));
+            MDX_HIP(launch_sor_residual64(h, h->phi));
+            if ((rc = read_max(h, &r))) return rc;
+        }
         sweep_account(h);
         if (checks == 0) st->div0_inf = r;
         ++checks;
@@ -643,6 +714,7 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st) {
     st->checks = checks;
     st->div_inf = r;
     st->converged = r < c.tol;
+    st->lds_sweeps = h->rb_state == 1;
     return MDX_OK;
 }
 
@@ -662,7 +734,7 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
         if (checks == 0) st->div0_inf = r;
         ++checks;
         if (!(r >= c.tol) || it >= c.max_iter) break;
-        MDX_HIP(launch_zero32(h, h->e32, h->ni));
+        if (h->rb_state != 1) MDX_HIP(launch_zero32(h, h->e32, h->ni));
         const int m = std::min(c.inner_iters, c.max_iter - it);
         if ((rc = sweep_batch(h, true, m))) return rc;
         it += m;
@@ -671,6 +743,68 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     st->checks = checks;
     st->div_inf = r;
     st->converged = r < c.tol;
+    st->lds_sweeps = h->rb_state == 1;
+    return MDX_OK;
+}
+
+// Learned initial guess: network inference on (div u*, p_prev), then the guess and the
+// previous-step guess (Phi = 0) are compared by their initial ||div||_inf and the smaller one
+// starts the SOR; the solve itself, and its stopping rule, are the same either way.
+static int learned_guess(mdx_handle *h, mdx_solve_stats *st, double *r_first) {
+    mdx_net *n = h->net;
+    const bool want_guess = n && h->cfg.guess == MDX_GUESS_LEARNED && n->c0 > 0.0;
+    if (!n || !(want_guess || h->net_capture)) return MDX_OK;
+    hipStream_t saved = n->stream;
+    n->stream = h->stream;
+    ev_record(h, 8);
+    hipError_t e = unet::net_input_from_solver(n, h, h->rhs, h->pn);
+    if (e == hipSuccess && want_guess) e = unet::net_guess_to_phi(n, h->phi2);
+    ev_record(h, 9);
+    n->stream = saved;
+    MDX_HIP(e);
+    if (!want_guess) return MDX_OK;
+    double r0 = 0.0, rg = 0.0;
+    int rc;
+    MDX_HIP(launch_reset_max(h));
+    MDX_HIP(launch_sor_residual64(h, h->phi2));
+    if ((rc = read_max(h, &rg))) return rc;
+    MDX_HIP(launch_reset_max(h));
+    MDX_HIP(launch_sor_residual64(h, h->phi));   // Phi = 0: the previous-step guess
+    if ((rc = read_max(h, &r0))) return rc;
+    if (h->timing) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
+    }
+    st->guess_div_inf = rg;
+    if (rg < r0) {
+        std::swap(h->phi, h->phi2);
+        *r_first = rg;
+    } else {
+        st->fallback = 1;
+        *r_first = r0;
+    }
+    return MDX_OK;
+}
+
+// Append (network input, converged Phi) to the attached network's replay buffer.  The target
+// scale c0 = rms(Phi) / rms(div u*) is fixed by the first captured pair.
+static int capture_pair(mdx_handle *h) {
+    mdx_net *n = h->net;
+    hipStream_t saved = n->stream;
+    n->stream = h->stream;
+    hipError_t e = hipSuccess;
+    if (n->c0 <= 0.0) {
+        e = unet::net_rms_to_stats3(n, h, h->phi);
+        if (e == hipSuccess) e = hipMemcpyAsync(h->h_pinned + 8, n->stats, 4 * 8, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) {
+            const double sr = h->h_pinned[8], sphi = h->h_pinned[11];
+            n->c0 = (sphi > 0.0 && sr > 0.0) ? sphi / sr : 1.0;
+        }
+    }
+    if (e == hipSuccess) e = unet::net_capture(n, h->phi);
+    n->stream = saved;
+    MDX_HIP(e);
     return MDX_OK;
 }
 
@@ -702,9 +836,14 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         ev_record(h, 1);
         int rc = ensure_mu(h, dtg);
         if (rc) return rc;
+        if ((rc = rb_prepare(h))) return rc;
         MDX_HIP(launch_sor_prepare(h));
-        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st) : sor_solve64(h, &st);
+        double r_first = -1.0;
+        if ((rc = learned_guess(h, &st, &r_first))) return rc;
+        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st) : sor_solve64(h, &st, r_first);
         if (rc) return rc;
+        if (st.fallback == 0 && r_first >= 0.0) st.div0_inf = r_first;
+        if (h->net_capture && st.converged && (rc = capture_pair(h))) return rc;
         ev_record(h, 2);
         MDX_HIP(launch_sor_apply(h, h->phi, dtg));
         ev_record(h, 3);
@@ -835,5 +974,14 @@ extern "C" int mdx_get_y_coordinates(mdx_handle *h, double *yc, double *yv) {
     MDX_CHECK(h, MDX_ERR_ARG, "null handle");
     if (yc) std::memcpy(yc, h->yc.data(), h->yc.size() * 8);
     if (yv) std::memcpy(yv, h->yv.data(), h->yv.size() * 8);
+    return MDX_OK;
+}
+
+extern "C" int mdx_get_info(mdx_handle *h, char *buf, int32_t len) {
+    MDX_CHECK(h && buf && len > 0, MDX_ERR_ARG, "mdx_get_info: bad argument");
+    std::string s = h->cfg.solver == MDX_SOLVER_FFT ? "FFT/tridiagonal pressure solve"
+                    : h->cfg.solver == MDX_SOLVER_SOLA ? "SOLA red-black sweeps (fixed niter)"
+                    : (h->rb_note.empty() ? "red-black SOR (not yet planned)" : h->rb_note);
+    std::snprintf(buf, (size_t)len, "%s", s.c_str());
     return MDX_OK;
 }

@@ -54,8 +54,46 @@ struct MuK {
     }
 };
 
-struct Timer {
-    hipEvent_t ev[8];
+// XCD-aware tile order for the 3-D stencil kernels.  Workgroups are dispatched round-robin over the
+// 8 XCDs (blockIdx.x % 8); XCD x is given the contiguous tile range [x*per, (x+1)*per), tiles ordered
+// x fastest, then y, then z, so the neighbouring z planes a stencil reads stay in that XCD's L2.
+// Launch with xcd_grid(); returns false for the padding workgroups.
+__host__ inline unsigned xcd_grid(int items_x, int items_y, int nz, int bxx, int byy) {
+    const long nt = (long)((items_x + bxx - 1) / bxx) * ((items_y + byy - 1) / byy) * nz;
+    return (unsigned)(8 * ((nt + 7) / 8));
+}
+__device__ inline bool xcd_tile(int items_x, int items_y, int nz, int bxx, int byy, int &tx, int &ty, int &tz) {
+    const int ntx = (items_x + bxx - 1) / bxx, nty = (items_y + byy - 1) / byy;
+    const int nt = ntx * nty * nz;
+    const int per = (nt + 7) / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (t >= nt) return false;
+    tx = t % ntx;
+    ty = (t / ntx) % nty;
+    tz = t / (ntx * nty);
+    return true;
+}
+
+// per-workgroup maximum of non-negative doubles into part[workgroup]; NaN propagates
+__device__ inline double nanmax_any(double a, double b) { return (a != a || a > b) ? a : b; }
+__device__ inline void block_max_part(double v, double *part) {
+    __shared__ double wmax[16];
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    for (int off = 32; off > 0; off >>= 1) v = nanmax_any(v, __shfl_xor(v, off, 64));
+    if ((t & 63) == 0) wmax[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+        double m = wmax[0];
+        for (int w = 1; w < (nt + 63) / 64; ++w) m = nanmax_any(m, wmax[w]);
+        part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = m;
+    }
+}
+
+// decomposition of the LDS-resident red-black solver (mdx_sor_lds.hip)
+struct RbPlanOut {
+    int px, py, pz, bx, by, bz, maxp;
+    size_t lds;
 };
 
 }  // namespace mdx
@@ -96,13 +134,14 @@ struct mdx_handle {
 
     // reductions
     double *red_part = nullptr;    // per-block partial results
-    unsigned long long *red_max = nullptr;   // atomic max of non-negative doubles (bit patterns)
+    unsigned long long *red_max = nullptr;   // [0] max result (double bits), [1] LDS-solver error word
+    double *max_part = nullptr;    // per-workgroup maxima (first pass of the max reductions)
     double *h_pinned = nullptr;    // pinned host staging (8 doubles)
     int red_blocks = 0;
 
     // timing
     bool timing = false;
-    hipEvent_t ev[8];
+    hipEvent_t ev[10];
     mdx_phase_times times{};
     int sweep_pending = 0;         // iterations of the last (unaccounted) sweep batch
 
@@ -114,8 +153,19 @@ struct mdx_handle {
     } graphs[2];
     bool use_graphs = true;
 
-    // learned guess (attached network), opaque
-    void *net = nullptr;
+    // LDS-resident red-black solver: plans per precision, face buffers, progress counters
+    bool use_lds_sor = true;
+    int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
+    mdx::RbPlanOut rb_plan64{}, rb_plan32{};
+    void *rb_faces = nullptr;
+    unsigned long long *rb_flags = nullptr;
+    int *rb_err = nullptr;         // points into red_max[1]
+    unsigned long long rb_epoch = 0;
+    std::string rb_note;           // why the streaming sweeps are in use, or the chosen plan
+
+    // learned guess (attached network) and training-pair capture
+    struct mdx_net *net = nullptr;
+    bool net_capture = false;
 };
 
 // kernel launchers (defined in the .hip files)
@@ -133,6 +183,8 @@ hipError_t launch_flow_shift(mdx_handle *h, double dtg, const double *du_dev);
 hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int use_geo,
                                 const double *hx_u, double ymid);
 hipError_t launch_reset_max(mdx_handle *h);
+hipError_t launch_max_final(mdx_handle *h, unsigned nparts);   // max_part[0..n) -> red_max[0]
+static constexpr unsigned MAX_PARTS = 1u << 17;
 
 // red-black solver
 hipError_t launch_sola_sweep(mdx_handle *h, int parity, double sor, double dtg);
@@ -145,6 +197,8 @@ hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, 
 hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega);
 hipError_t launch_zero32(mdx_handle *h, float *p, size_t n);
 hipError_t launch_zero64(mdx_handle *h, double *p, size_t n);
+bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);
+hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init);
 
 // FFT path
 hipError_t launch_fft_forward(mdx_handle *h, double dtg);

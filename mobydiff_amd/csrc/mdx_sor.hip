@@ -7,28 +7,12 @@
 //    residual of a cell is div(u*) - L_mu(Phi), the relaxation denominator is the reference's
 //    (wall faces included), so the iterates equal the SOLA iterates.  FP64 for the plain solve;
 //    FP32 sweeps on the defect equation inside an FP64 defect-correction loop for the mixed solve.
-#include "mdx_internal.h"
+#include "mdx_sor_core.h"
 
 namespace mdx {
 
 static constexpr int SX = 32;   // active cells of one colour along x per block row
 static constexpr int SY = 8;
-
-__device__ inline double nanmax_d(double a, double b) { return (a != a || a > b) ? a : b; }
-
-__device__ inline void block_max_atomic(double v, unsigned long long *out) {
-    __shared__ double part[16];
-    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    const int nt = blockDim.x * blockDim.y * blockDim.z;
-    for (int off = 32; off > 0; off >>= 1) v = nanmax_d(v, __shfl_xor(v, off, 64));
-    if ((t & 63) == 0) part[t >> 6] = v;
-    __syncthreads();
-    if (t == 0) {
-        double m = part[0];
-        for (int w = 1; w < (nt + 63) / 64; ++w) m = nanmax_d(m, part[w]);
-        atomicMax(out, (unsigned long long)__double_as_longlong(m));
-    }
-}
 
 // ------------------------------------------------------------------------------- SOLA (reference)
 __global__ __launch_bounds__(SX * SY) void sola_sweep_kernel(
@@ -37,9 +21,11 @@ __global__ __launch_bounds__(SX * SY) void sola_sweep_kernel(
     const double *__restrict__ cv, const double *__restrict__ cw, double *un, double *vn, double *wn,
     double *pn) {
     const Dims d = g.d;
-    const int ni = blockIdx.x * SX + threadIdx.x + 1;
-    const int j = blockIdx.y * SY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx_, ty_, tz_;
+    if (!xcd_tile(d.nx / 2, d.ny, d.nz, SX, SY, tx_, ty_, tz_)) return;
+    const int ni = tx_ * SX + threadIdx.x + 1;
+    const int j = ty_ * SY + threadIdx.y + 1;
+    const int k = tz_ + 1;
     if (ni > d.nx / 2 || j > d.ny) return;
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const bool per1 = g.per[0], per2 = g.per[1], per3 = g.per[2];
@@ -115,7 +101,7 @@ hipError_t launch_sola_sweep(mdx_handle *h, int parity, double sor, double dtg) 
     GridK g = grid_k(h);
     BcK b = bc_k(h);
     const double idx = 1.0 / h->dx, idy = 1.0 / h->dy, idz = 1.0 / h->dz;
-    dim3 blk(SX, SY), grd((h->d.nx / 2 + SX - 1) / SX, (h->d.ny + SY - 1) / SY, h->d.nz);
+    dim3 blk(SX, SY), grd(xcd_grid(h->d.nx / 2, h->d.ny, h->d.nz, SX, SY));
     const double *cu = h->ibm_on ? h->cu : nullptr, *cv = h->ibm_on ? h->cv : nullptr,
                  *cw = h->ibm_on ? h->cw : nullptr;
     hipLaunchKernelGGL(sola_sweep_kernel, grd, blk, 0, h->stream, g, b, parity, sor, dtg, idx, idy, idz,
@@ -160,11 +146,6 @@ static MuK<T> mu_k(const mdx_handle *h, T *const *tabs) {
 }
 
 // ------------------------------------------------------------------------------- residual stencil
-template <typename T>
-struct SorConst {
-    T idx2, idy2, idz2;
-};
-
 // residual r = rhs - L_mu(Phi) of cell (i,j,k) and the reference's relaxation denominator
 template <typename T, bool STRETCH, bool IBM, typename PhiAt>
 __device__ inline T cell_residual(const GridK &g, const MuK<T> &mu, const SorConst<T> &s, PhiAt P,
@@ -188,22 +169,16 @@ __device__ inline T cell_residual(const GridK &g, const MuK<T> &mu, const SorCon
         mwk = mu.w[mu.at(i, j, k)];
         mwkp = mu.w[mu.at(i, j, kp)];
     }
-    const T p0 = P(i, j, k);
     const bool wn_ = !per2 && j == ny, ws_ = !per2 && j == 1;
-    const T sn = wn_ ? T(0) : mvjp * (P(i, wn_ ? j : jp, k) - p0);
-    const T ss = ws_ ? T(0) : mvj * (p0 - P(i, ws_ ? j : jm, k));
-    const T lx = (muip * (P(ip, j, k) - p0) - mui * (p0 - P(im, j, k))) * s.idx2;
-    const T lz = (mwkp * (P(i, j, kp) - p0) - mwk * (p0 - P(i, j, km))) * s.idz2;
-    T ly;
-    if (!STRETCH) {
-        ly = (sn - ss) * s.idy2;
-        denom = (mui + muip) * s.idx2 + (mwk + mwkp) * s.idz2 + (mvj + mvjp) * s.idy2;
-    } else {
-        const T an = (T)(g.ihv[j + 1] * g.ihc[j]), as = (T)(g.ihv[j] * g.ihc[j]);
-        ly = sn * an - ss * as;
-        denom = (mui + muip) * s.idx2 + (mwk + mwkp) * s.idz2 + (mvj * as + mvjp * an);
+    const T p0 = P(i, j, k);
+    const T pyp = P(i, wn_ ? j : jp, k), pym = P(i, ws_ ? j : jm, k);
+    T an = 0, as = 0;
+    if (STRETCH) {
+        an = (T)(g.ihv[j + 1] * g.ihc[j]);
+        as = (T)(g.ihv[j] * g.ihc[j]);
     }
-    return rhs_c - (lx + ly + lz);
+    return sor_residual_core<T, STRETCH>(p0, P(ip, j, k), P(im, j, k), pyp, pym, P(i, j, kp), P(i, j, km), mui,
+                                         muip, mvj, mvjp, mwk, mwkp, wn_, ws_, rhs_c, s, an, as, denom);
 }
 
 template <typename T>
@@ -232,9 +207,11 @@ __global__ __launch_bounds__(256) void sor_prepare_kernel(GridK g, double vbot, 
                                                           const double *__restrict__ wn, double *rhs,
                                                           double *phi, double idx, double idy, double idz) {
     const Dims d = g.d;
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx_, ty_, tz_;
+    if (!xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_)) return;
+    const int i = tx_ * 64 + threadIdx.x + 1;
+    const int j = ty_ * 4 + threadIdx.y + 1;
+    const int k = tz_ + 1;
     if (i > d.nx || j > d.ny) return;
     const bool per2 = g.per[1];
     if (!per2) {
@@ -256,7 +233,7 @@ hipError_t launch_sor_prepare(mdx_handle *h) {
     if ((e = hipMemcpyAsync(h->un, h->us, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(h->vn, h->vs, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(h->wn, h->ws, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
-    dim3 blk(64, 4), grd((h->d.nx + 63) / 64, (h->d.ny + 3) / 4, h->d.nz);
+    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
     const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
     if (h->stretched)
         hipLaunchKernelGGL(sor_prepare_kernel<true>, grd, blk, 0, h->stream, g, vb, vt, h->vn, h->un, h->wn,
@@ -272,9 +249,11 @@ __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu,
                                                         double *un, double *vn, double *wn, double *pn,
                                                         double idx, double idy, double idz, double idt) {
     const Dims d = g.d;
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx_, ty_, tz_;
+    if (!xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_)) return;
+    const int i = tx_ * 64 + threadIdx.x + 1;
+    const int j = ty_ * 4 + threadIdx.y + 1;
+    const int k = tz_ + 1;
     if (i > d.nx || j > d.ny) return;
     const bool per2 = g.per[1];
     const int im = i == 1 ? d.nx : i - 1, km = k == 1 ? d.nz : k - 1;
@@ -296,7 +275,7 @@ __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu,
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
-    dim3 blk(64, 4), grd((h->d.nx + 63) / 64, (h->d.ny + 3) / 4, h->d.nz);
+    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
     const double a = 1.0 / h->dx, b = 1.0 / h->dy, c = 1.0 / h->dz, t = 1.0 / dtg;
 #define MDX_APPLY(S, I) \
     hipLaunchKernelGGL((sor_apply_kernel<S, I>), grd, blk, 0, h->stream, g, mu, phi, h->un, h->vn, h->wn, h->pn, a, b, c, t)
@@ -312,9 +291,11 @@ __global__ __launch_bounds__(SX * SY) void sor_sweep64_kernel(GridK g, MuK<doubl
                                                               int color, double omega,
                                                               const double *__restrict__ rhs, double *phi) {
     const Dims d = g.d;
-    const int ni = blockIdx.x * SX + threadIdx.x + 1;
-    const int j = blockIdx.y * SY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx_, ty_, tz_;
+    if (!xcd_tile(d.nx / 2, d.ny, d.nz, SX, SY, tx_, ty_, tz_)) return;
+    const int ni = tx_ * SX + threadIdx.x + 1;
+    const int j = ty_ * SY + threadIdx.y + 1;
+    const int k = tz_ + 1;
     if (ni > d.nx / 2 || j > d.ny) return;
     const int i = 1 + ((j + k + color) & 1) + 2 * (ni - 1);
     PlainPhi<double> P{phi, d};
@@ -328,7 +309,7 @@ hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
-    dim3 blk(SX, SY), grd((h->d.nx / 2 + SX - 1) / SX, (h->d.ny + SY - 1) / SY, h->d.nz);
+    dim3 blk(SX, SY), grd(xcd_grid(h->d.nx / 2, h->d.ny, h->d.nz, SX, SY));
 #define MDX_SW(S, I) \
     hipLaunchKernelGGL((sor_sweep64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, color, omega, h->rhs, h->phi)
     if (h->stretched) { if (h->ibm_on) MDX_SW(true, true); else MDX_SW(true, false); }
@@ -340,32 +321,34 @@ hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega) {
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(256) void sor_residual64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
                                                              const double *__restrict__ rhs,
-                                                             const double *__restrict__ phi,
-                                                             unsigned long long *out) {
+                                                             const double *__restrict__ phi, double *out) {
     const Dims d = g.d;
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx_ = 0, ty_ = 0, tz_ = 0;
+    const bool ok = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);   // padding groups report 0
+    const int i = tx_ * 64 + threadIdx.x + 1;
+    const int j = ty_ * 4 + threadIdx.y + 1;
+    const int k = tz_ + 1;
     double m = 0.0;
-    if (i <= d.nx && j <= d.ny) {
+    if (ok && i <= d.nx && j <= d.ny) {
         PlainPhi<double> P{phi, d};
         double denom;
         m = fabs(cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[d.in(i, j, k)], i, j, k, denom));
     }
-    block_max_atomic(m, out);
+    block_max_part(m, out);
 }
 
 hipError_t launch_sor_residual64(mdx_handle *h, const double *phi) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
-    dim3 blk(64, 4), grd((h->d.nx + 63) / 64, (h->d.ny + 3) / 4, h->d.nz);
+    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
 #define MDX_RS(S, I) \
-    hipLaunchKernelGGL((sor_residual64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi, h->red_max)
+    hipLaunchKernelGGL((sor_residual64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi, h->max_part)
     if (h->stretched) { if (h->ibm_on) MDX_RS(true, true); else MDX_RS(true, false); }
     else { if (h->ibm_on) MDX_RS(false, true); else MDX_RS(false, false); }
 #undef MDX_RS
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_max_final(h, grd.x);
 }
 
 // ------------------------------------------------------------------------------- mixed precision
@@ -376,13 +359,15 @@ __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, 
                                                        const double *__restrict__ rhs,
                                                        const double *__restrict__ phi_in,
                                                        const float *__restrict__ e, double *phi_out,
-                                                       float *r32, unsigned long long *out) {
+                                                       float *r32, double *out) {
     const Dims d = g.d;
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx_ = 0, ty_ = 0, tz_ = 0;
+    const bool ok = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);   // padding groups report 0
+    const int i = tx_ * 64 + threadIdx.x + 1;
+    const int j = ty_ * 4 + threadIdx.y + 1;
+    const int k = tz_ + 1;
     double m = 0.0;
-    if (i <= d.nx && j <= d.ny) {
+    if (ok && i <= d.nx && j <= d.ny) {
         SumPhi P{phi_in, e, d};
         double denom;
         const size_t q = d.in(i, j, k);
@@ -391,21 +376,22 @@ __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, 
         r32[q] = (float)r;
         m = fabs(r);
     }
-    block_max_atomic(m, out);
+    block_max_part(m, out);
 }
 
 hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
-    dim3 blk(64, 4), grd((h->d.nx + 63) / 64, (h->d.ny + 3) / 4, h->d.nz);
+    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
 #define MDX_DF(S, I)                                                                                       \
     hipLaunchKernelGGL((defect64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi_in, e, phi_out, \
-                       r32, h->red_max)
+                       r32, h->max_part)
     if (h->stretched) { if (h->ibm_on) MDX_DF(true, true); else MDX_DF(true, false); }
     else { if (h->ibm_on) MDX_DF(false, true); else MDX_DF(false, false); }
 #undef MDX_DF
-    return hipGetLastError();
+    hipError_t err = hipGetLastError();
+    return err != hipSuccess ? err : launch_max_final(h, grd.x);
 }
 
 template <bool STRETCH, bool IBM>
@@ -413,9 +399,11 @@ __global__ __launch_bounds__(SX * SY) void sor_sweep32_kernel(GridK g, MuK<float
                                                               int color, float omega,
                                                               const float *__restrict__ rhs, float *e) {
     const Dims d = g.d;
-    const int ni = blockIdx.x * SX + threadIdx.x + 1;
-    const int j = blockIdx.y * SY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx_, ty_, tz_;
+    if (!xcd_tile(d.nx / 2, d.ny, d.nz, SX, SY, tx_, ty_, tz_)) return;
+    const int ni = tx_ * SX + threadIdx.x + 1;
+    const int j = ty_ * SY + threadIdx.y + 1;
+    const int k = tz_ + 1;
     if (ni > d.nx / 2 || j > d.ny) return;
     const int i = 1 + ((j + k + color) & 1) + 2 * (ni - 1);
     PlainPhi<float> P{e, d};
@@ -429,7 +417,7 @@ hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega) {
     GridK g = grid_k(h);
     MuK<float> mu = mu_k<float>(h, h->mu32);
     SorConst<float> s{(float)(1.0 / (h->dx * h->dx)), (float)(1.0 / (h->dy * h->dy)), (float)(1.0 / (h->dz * h->dz))};
-    dim3 blk(SX, SY), grd((h->d.nx / 2 + SX - 1) / SX, (h->d.ny + SY - 1) / SY, h->d.nz);
+    dim3 blk(SX, SY), grd(xcd_grid(h->d.nx / 2, h->d.ny, h->d.nz, SX, SY));
 #define MDX_SW(S, I) \
     hipLaunchKernelGGL((sor_sweep32_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, color, omega, h->r32, h->e32)
     if (h->stretched) { if (h->ibm_on) MDX_SW(true, true); else MDX_SW(true, false); }

@@ -21,9 +21,11 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
     double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
     double ire) {
     const Dims d = g.d;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx, ty, tz;
+    if (!xcd_tile(d.nx, d.ny, d.nz, BX, BY, tx, ty, tz)) return;
+    const int i = tx * BX + threadIdx.x + 1;
+    const int j = ty * BY + threadIdx.y + 1;
+    const int k = tz + 1;
     if (i > d.nx || j > d.ny) return;
     const int nx0 = g.per[0] ? 1 : 2, ny0 = g.per[1] ? 1 : 2, nz0 = g.per[2] ? 1 : 2;
     const int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
@@ -122,8 +124,8 @@ hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
     GridK g = grid_k(h);
     const double dx = h->dx, dy = h->dy, dz = h->dz;
     const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
-    dim3 blk(BX, BY, 1), grd((h->d.nx + BX - 1) / BX, (h->d.ny + BY - 1) / BY, h->d.nz);
-#define MDX_MOM(S, I)                                                                          \
+    dim3 blk(BX, BY, 1), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, BX, BY));
+#define MDX_MOM(S, I)                                                                        \
     hipLaunchKernelGGL((momentum_kernel<S, I>), grd, blk, 0, h->stream, g, h->un, h->vn, h->wn, \
                        h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw, h->cu, h->cv, h->cw,  \
                        dta, dtb, dtg, 1.0 / dx, 1.0 / dy, 1.0 / dz, 1.0 / dx2, 1.0 / dy2,      \
@@ -321,29 +323,35 @@ hipError_t launch_apply_bc_redblack(mdx_handle *h) {
 }
 
 // ------------------------------------------------------------------------------- reductions
-// Maximum of non-negative doubles through their bit patterns (monotone as unsigned integers);
-// a NaN (positive quiet NaN from fabs) orders above +inf and therefore propagates.
+// Two-pass maximum: every workgroup writes its maximum to max_part (block_max_part), one
+// workgroup folds them.  (A single-address atomicMax per workgroup serialises ~10^4 atomics.)
 __device__ inline double nanmax(double a, double b) { return (a != a || a > b) ? a : b; }
 
-__device__ inline void block_max_to(double v, unsigned long long *out) {
-    __shared__ double part[16];
-    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    const int nt = blockDim.x * blockDim.y * blockDim.z;
-    for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
-    if ((t & 63) == 0) part[t >> 6] = v;
+__global__ __launch_bounds__(1024) void max_final_kernel(const double *__restrict__ part, unsigned n,
+                                                         unsigned long long *out) {
+    __shared__ double w[16];
+    double m = 0.0;
+    for (unsigned q = threadIdx.x; q < n; q += 1024) m = nanmax(m, part[q]);
+    for (int off = 32; off > 0; off >>= 1) m = nanmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (t == 0) {
-        double m = part[0];
-        for (int w = 1; w < (nt + 63) / 64; ++w) m = nanmax(m, part[w]);
-        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 16; ++q) m = nanmax(m, w[q]);
+        m = nanmax(w[0], m);
+        out[0] = (unsigned long long)__double_as_longlong(m);
     }
 }
 
-__global__ void reset_max_kernel(unsigned long long *p) { *p = 0ull; }
-
-hipError_t launch_reset_max(mdx_handle *h) {
-    hipLaunchKernelGGL(reset_max_kernel, dim3(1), dim3(1), 0, h->stream, h->red_max);
+hipError_t launch_max_final(mdx_handle *h, unsigned nparts) {
+    if (nparts > MAX_PARTS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(1024), 0, h->stream, h->max_part, nparts, h->red_max);
     return hipGetLastError();
+}
+
+// kept for callers that still clear the result word first (a no-op for the two-pass reductions)
+hipError_t launch_reset_max(mdx_handle *h) {
+    (void)h;
+    return hipSuccess;
 }
 
 // ||div u||_inf over interior cells (periodic wrap as in the red-black solver)
@@ -351,14 +359,15 @@ template <bool STRETCH>
 __global__ __launch_bounds__(BX * BY) void div_max_kernel(GridK g, const double *__restrict__ un,
                                                           const double *__restrict__ vn,
                                                           const double *__restrict__ wn, double idx,
-                                                          double idy, double idz,
-                                                          unsigned long long *out) {
+                                                          double idy, double idz, double *part) {
     const Dims d = g.d;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx = 0, ty = 0, tz = 0;
+    const bool ok = xcd_tile(d.nx, d.ny, d.nz, BX, BY, tx, ty, tz);   // padding groups still report 0
+    const int i = tx * BX + threadIdx.x + 1;
+    const int j = ty * BY + threadIdx.y + 1;
+    const int k = tz + 1;
     double m = 0.0;
-    if (i <= d.nx && j <= d.ny) {
+    if (ok && i <= d.nx && j <= d.ny) {
         int ip = (g.per[0] && i == d.nx) ? 1 : i + 1;
         int kp = (g.per[2] && k == d.nz) ? 1 : k + 1;
         int jp = (g.per[1] && j == d.ny) ? 1 : j + 1;
@@ -366,24 +375,26 @@ __global__ __launch_bounds__(BX * BY) void div_max_kernel(GridK g, const double 
         double dvy = STRETCH ? (vn[d.g(i, jp, k)] - vn[c]) * g.ihc[j] : (vn[d.g(i, jp, k)] - vn[c]) * idy;
         m = fabs((un[d.g(ip, j, k)] - un[c]) * idx + dvy + (wn[d.g(i, j, kp)] - wn[c]) * idz);
     }
-    block_max_to(m, out);
+    block_max_part(m, part);
 }
 
 hipError_t launch_div_max(mdx_handle *h, const double *u, const double *v, const double *w) {
     GridK g = grid_k(h);
-    dim3 blk(BX, BY, 1), grd((h->d.nx + BX - 1) / BX, (h->d.ny + BY - 1) / BY, h->d.nz);
+    const unsigned nb = xcd_grid(h->d.nx, h->d.ny, h->d.nz, BX, BY);
+    dim3 blk(BX, BY, 1);
     if (h->stretched)
-        hipLaunchKernelGGL(div_max_kernel<true>, grd, blk, 0, h->stream, g, u, v, w, 1.0 / h->dx,
-                           1.0 / h->dy, 1.0 / h->dz, h->red_max);
+        hipLaunchKernelGGL(div_max_kernel<true>, dim3(nb), blk, 0, h->stream, g, u, v, w, 1.0 / h->dx,
+                           1.0 / h->dy, 1.0 / h->dz, h->max_part);
     else
-        hipLaunchKernelGGL(div_max_kernel<false>, grd, blk, 0, h->stream, g, u, v, w, 1.0 / h->dx,
-                           1.0 / h->dy, 1.0 / h->dz, h->red_max);
-    return hipGetLastError();
+        hipLaunchKernelGGL(div_max_kernel<false>, dim3(nb), blk, 0, h->stream, g, u, v, w, 1.0 / h->dx,
+                           1.0 / h->dy, 1.0 / h->dz, h->max_part);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_max_final(h, nb);
 }
 
 // step.f90:210-258: max over every ghosted entry of |u|/dx, |v|/dy, |w|/dz
 __global__ void cfl_kernel(GridK g, const double *__restrict__ un, const double *__restrict__ vn,
-                           const double *__restrict__ wn, size_t n, unsigned long long *out) {
+                           const double *__restrict__ wn, size_t n, double *part) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
     for (; q < n; q += (size_t)gridDim.x * blockDim.x) {
@@ -393,14 +404,15 @@ __global__ void cfl_kernel(GridK g, const double *__restrict__ un, const double 
         double c = fabs(wn[q] / g.dz);
         m = nanmax(m, nanmax(a, nanmax(b, c)));
     }
-    block_max_to(m, out);
+    block_max_part(m, part);
 }
 
 hipError_t launch_cfl_max(mdx_handle *h) {
     GridK g = grid_k(h);
     hipLaunchKernelGGL(cfl_kernel, dim3(1024), dim3(256), 0, h->stream, g, h->un, h->vn, h->wn, h->ng,
-                       h->red_max);
-    return hipGetLastError();
+                       h->max_part);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_max_final(h, 1024);
 }
 
 // Deterministic two-pass sums over interior u points: S1 = sum u*h_j, S2 = sum mu_u*h_j.

@@ -1,0 +1,62 @@
+// mdx_unet.h -- state of the pressure-guess U-Net and its launch interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+typedef __bf16 bf16;
+
+struct mdx_net {
+    int W[3], H[3], D[3];          // per level: full, /2, /4
+    long V[3];
+    int64_t nparam = 0;
+    int64_t woff[11], boff[11];    // flat FP32 parameter layout: per layer W(tap,cin,cout) then b(cout)
+    hipStream_t stream = nullptr;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    bf16 *wp[11] = {}, *wd[11] = {};   // forward / input-gradient weights in MFMA fragment order
+    bf16 *x_in = nullptr, *a[10] = {}, *p1 = nullptr, *p2 = nullptr, *cat1 = nullptr, *cat2 = nullptr;
+    bf16 *g[10] = {}, *gp1 = nullptr, *gp2 = nullptr, *gcat1 = nullptr, *gcat2 = nullptr;
+    float *out = nullptr, *dout = nullptr, *target = nullptr, *loss = nullptr, *x2_stage = nullptr;
+    double *stats = nullptr, *part = nullptr, *h_pinned = nullptr;
+    std::vector<void *> allocs;
+    double c0 = 0.0;               // target scale: Phi = out * rms(div u*) * c0
+    int adam_t = 0;
+    // replay buffer of captured (input, target) pairs
+    int cap = 0, count = 0, next = 0;
+    bf16 *rb_x = nullptr;          // [cap][V][2]
+    float *rb_t = nullptr;         // [cap][V]
+    uint64_t sample_state = 0;
+    // inference timing
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace mdx {
+namespace unet {
+
+struct LayerDesc {
+    const char *name;
+    int cin, cout, ks, level;   // level 0 = full resolution
+};
+static const LayerDesc LAYERS[] = {
+    {"e1a", 16, 16, 3, 0}, {"e1b", 16, 16, 3, 0}, {"e2a", 16, 32, 3, 1}, {"e2b", 32, 32, 3, 1},
+    {"b3a", 32, 64, 3, 2}, {"b3b", 64, 64, 3, 2}, {"d2a", 96, 32, 3, 1}, {"d2b", 32, 32, 3, 1},
+    {"d1a", 48, 16, 3, 0}, {"d1b", 16, 16, 3, 0}, {"head", 16, 1, 1, 0},
+};
+static constexpr int NL = 11;
+
+hipError_t repack_all(mdx_net *n);
+hipError_t upload_input2(mdx_net *n, const float *x2);
+hipError_t net_forward(mdx_net *n);
+hipError_t net_backward(mdx_net *n, float scale);
+hipError_t net_adam(mdx_net *n, float lr);
+// solver coupling: input from the solver's rhs / pressure, output to an FP64 Phi guess
+hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *rhs, const double *pn);
+hipError_t net_guess_to_phi(mdx_net *n, double *phi);
+hipError_t net_capture(mdx_net *n, const double *phi);
+hipError_t net_load_replay(mdx_net *n, int slot);
+hipError_t net_mse_only(mdx_net *n);
+hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi);
+}  // namespace unet
+}  // namespace mdx

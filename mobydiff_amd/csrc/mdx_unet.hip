@@ -1,0 +1,649 @@
+// mdx_unet.hip -- the pressure-guess U-Net on gfx950: BF16 weights and activations, FP32
+// accumulation on MFMA (v_mfma_f32_16x16x32_bf16), with forward, backward and Adam kernels.
+//
+// Network (3 levels, 16/32/64 channels, 3x3x3 convs, zero padding, ReLU):
+//   e1a 16->16 (input channels 0,1 carry div(u*) and p; the rest are zero), e1b 16->16,
+//   avg-pool 2, e2a 16->32, e2b 32->32, avg-pool 2, b3a 32->64, b3b 64->64,
+//   nearest x2 + concat(e2b) -> d2a 96->32, d2b 32->32,
+//   nearest x2 + concat(e1b) -> d1a 48->16, d1b 16->16, head 1x1 16->1 (linear, FP32 out).
+// Activations are channels-last, voxel v = x + W (y + H z), element v*C + c.
+//
+// Convolution = implicit GEMM: out[v][n] = sum_k A[v][k] B[k][n], k = tap*CIN + cin, A gathered
+// from the input (8 consecutive channels of one tap per lane: one 16-byte load), B from weights
+// repacked into MFMA fragment order.  The input gradient is the same GEMM with the weights
+// flipped in space and transposed in channels.  The weight gradient reduces over voxels:
+// dW[k][n] = sum_v A[v][k] dY[v][n], computed per tap on 32-voxel chunks staged (transposed)
+// in LDS, partial sums added atomically in FP32.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mdx_internal.h"
+#include "mdx_unet.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace mdx {
+namespace unet {
+
+// ------------------------------------------------------------------------------- conv forward
+template <int CIN, int COUT, bool RELU>
+__global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__ in, const bf16x8 *__restrict__ wp,
+                                                        const float *__restrict__ bias, bf16 *__restrict__ out,
+                                                        int W, int H, int D) {
+    constexpr int K = 27 * CIN, KC = (K + 31) / 32, NT = COUT / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long V = (long)W * H * D;
+    const long v0 = ((long)blockIdx.x * 4 + wave) * 16;
+    if (v0 >= V) return;
+    const int r = lane & 15, g = lane >> 4;
+    const long v = v0 + r;
+    const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
+    f32x4 acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero8 = {};
+    for (int kc = 0; kc < KC; ++kc) {
+        const int k0 = kc * 32 + g * 8;
+        bf16x8 a = zero8;
+        if (k0 < K) {
+            const int tap = k0 / CIN, c0 = k0 - tap * CIN;
+            const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
+            if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
+                a = *reinterpret_cast<const bf16x8 *>(in + (((long)zz * H + yy) * W + xx) * CIN + c0);
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wp[((long)kc * NT + nt) * 64 + lane], acc[nt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int ch = nt * 16 + (lane & 15);
+        const float b = bias ? bias[ch] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long vo = v0 + 4 * g + i;
+            float val = acc[nt][i] + b;
+            if (RELU) val = val > 0.f ? val : 0.f;
+            out[vo * COUT + ch] = (bf16)val;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------- weight gradient
+// One workgroup = one tap x one slice of voxels.  Each 32-voxel chunk: dY (32 x COUT) and the
+// tap-shifted input (32 x CIN) are staged transposed in LDS ([c][v], so 8 consecutive voxels of one
+// channel are one 16-byte read); each wave owns a subset of the (CIN/16 x COUT/16) output tiles.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void conv3_wgrad_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ dy,
+                                                          float *__restrict__ dw, int W, int H, int D, int slices) {
+    constexpr int MT = CIN / 16, NT = COUT / 16, TILES = MT * NT;
+    constexpr int TPW = (TILES + 3) / 4;   // tiles per wave
+    __shared__ __attribute__((aligned(16))) bf16 sx[CIN * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sd[COUT * 32];
+    const int tap = blockIdx.x % 27, slice = blockIdx.x / 27;
+    const int dxo = tap % 3 - 1, dyo = (tap / 3) % 3 - 1, dzo = tap / 9 - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long V = (long)W * H * D;
+    const long nchunks = V / 32;
+    const long c_begin = nchunks * slice / slices, c_end = nchunks * (slice + 1) / slices;
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long ch = c_begin; ch < c_end; ++ch) {
+        const long vb = ch * 32;
+        __syncthreads();
+        // stage: thread t covers (voxel t/8... ) -- CIN/8 and COUT/8 vector loads per voxel
+        for (int q = threadIdx.x; q < 32 * (CIN / 8); q += 256) {
+            const int vv = q / (CIN / 8), c0 = (q % (CIN / 8)) * 8;
+            const long v = vb + vv;
+            const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
+            const int xx = x + dxo, yy = y + dyo, zz = z + dzo;
+            bf16x8 a = {};
+            if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
+                a = *reinterpret_cast<const bf16x8 *>(in + (((long)zz * H + yy) * W + xx) * CIN + c0);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sx[(c0 + e) * 32 + vv] = a[e];
+        }
+        for (int q = threadIdx.x; q < 32 * (COUT / 8); q += 256) {
+            const int vv = q / (COUT / 8), c0 = (q % (COUT / 8)) * 8;
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(dy + (vb + vv) * COUT + c0);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sd[(c0 + e) * 32 + vv] = a[e];
+        }
+        __syncthreads();
+        // D[m = cin][n = cout] += sum_{v in chunk} X[v][m] dY[v][n]:  A[m][k=v], B[k=v][n]
+        const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) {
+            const int tile = wave * TPW + q;
+            if (tile < TILES) {
+                const int mt = tile / NT, nt = tile % NT;
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sx[(mt * 16 + r) * 32 + 8 * g]);
+                const bf16x8 bb = *reinterpret_cast<const bf16x8 *>(&sd[(nt * 16 + r) * 32 + 8 * g]);
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[q], 0, 0, 0);
+            }
+        }
+    }
+    // accumulate: lane holds D[4g+i][lane%16] -> dw[(tap*CIN + m)*COUT + n]
+    const int g = lane >> 4;
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+        const int tile = wave * TPW + q;
+        if (tile < TILES) {
+            const int mt = tile / NT, nt = tile % NT;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = mt * 16 + 4 * g + i, n = nt * 16 + (lane & 15);
+                atomicAdd(&dw[((long)tap * CIN + m) * COUT + n], acc[q][i]);
+            }
+        }
+    }
+}
+
+// bias gradient: db[n] = sum_v dY[v][n]
+template <int COUT>
+__global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__ dy, float *__restrict__ db, long V) {
+    float s[COUT];
+#pragma unroll
+    for (int n = 0; n < COUT; ++n) s[n] = 0.f;
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long)gridDim.x * 256)
+#pragma unroll
+        for (int n = 0; n < COUT; ++n) s[n] += (float)dy[v * COUT + n];
+#pragma unroll
+    for (int n = 0; n < COUT; ++n) {
+        float t = s[n];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&db[n], t);
+    }
+}
+
+// ------------------------------------------------------------------------------- elementwise
+// avg-pool 2x2x2 (fine W,H,D -> coarse W/2,H/2,D/2)
+__global__ void pool_fwd_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int C, int W, int H, int D) {
+    const int Wc = W / 2, Hc = H / 2, Dc = D / 2;
+    const long n = (long)Wc * Hc * Dc * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long v = q / C;
+        const int x = (int)(v % Wc), y = (int)((v / Wc) % Hc), z = (int)(v / ((long)Wc * Hc));
+        float s = 0.f;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx)
+                    s += (float)in[(((long)(2 * z + dz) * H + 2 * y + dy) * W + 2 * x + dx) * C + c];
+        out[q] = (bf16)(s * 0.125f);
+    }
+}
+
+// nearest up-sampling of `coarse` into channels [0, Cu) of `cat` and a copy of `skip` into
+// channels [Cu, Cu+Cs) (the concat that feeds the decoder conv)
+__global__ void upcat_kernel(const bf16 *__restrict__ coarse, const bf16 *__restrict__ skip, bf16 *__restrict__ cat,
+                             int Cu, int Cs, int W, int H, int D) {
+    const int C = Cu + Cs;
+    const long n = (long)W * H * D * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long v = q / C;
+        if (c < Cu) {
+            const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
+            cat[q] = coarse[(((long)(z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2) * Cu + c];
+        } else {
+            cat[q] = skip[v * Cs + (c - Cu)];
+        }
+    }
+}
+
+// backward of upcat: d_coarse = sum over the 8 children of dcat[:, :Cu]; d_skip += dcat[:, Cu:]
+__global__ void upcat_bwd_coarse_kernel(const bf16 *__restrict__ dcat, bf16 *__restrict__ dcoarse, int Cu, int Cs,
+                                        int W, int H, int D) {
+    const int C = Cu + Cs, Wc = W / 2, Hc = H / 2, Dc = D / 2;
+    const long n = (long)Wc * Hc * Dc * Cu;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % Cu);
+        const long v = q / Cu;
+        const int x = (int)(v % Wc), y = (int)((v / Wc) % Hc), z = (int)(v / ((long)Wc * Hc));
+        float s = 0.f;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx)
+                    s += (float)dcat[(((long)(2 * z + dz) * H + 2 * y + dy) * W + 2 * x + dx) * C + c];
+        dcoarse[q] = (bf16)s;
+    }
+}
+
+// d_out (gradient arriving at a ReLU layer's output) = (d_from_next + d_skip) * (y > 0)
+__global__ void relu_bwd_kernel(bf16 *__restrict__ g, const bf16 *__restrict__ extra, int Ce, int coff, int C,
+                                const bf16 *__restrict__ y, long n) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        float d = (float)g[q];
+        if (extra) {
+            const long v = q / C;
+            const int c = (int)(q % C);
+            d += (float)extra[v * Ce + coff + c];
+        }
+        g[q] = (bf16)((float)y[q] > 0.f ? d : 0.f);
+    }
+}
+
+// avg-pool backward: d_fine[v] = d_coarse[parent(v)] / 8
+__global__ void pool_bwd_kernel(const bf16 *__restrict__ dcoarse, bf16 *__restrict__ dfine, int C, int W, int H, int D) {
+    const long n = (long)W * H * D * C;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(q % C);
+        const long v = q / C;
+        const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
+        dfine[q] = (bf16)((float)dcoarse[(((long)(z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c] * 0.125f);
+    }
+}
+
+// head: out[v] = sum_c w[c] x[v][c] + b (FP32)
+__global__ void head_fwd_kernel(const bf16 *__restrict__ x, const float *__restrict__ w, const float *__restrict__ b,
+                                float *__restrict__ out, long V) {
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        float s = 0.f;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += w[c] * (float)x[v * 16 + c];
+        out[v] = s + b[0];
+    }
+}
+
+// MSE loss against the target: dout = 2 (out - t) / (V * batch); loss partial sums
+__global__ void mse_kernel(const float *__restrict__ out, const float *__restrict__ t, float *__restrict__ dout,
+                           float scale, long V, float *__restrict__ loss) {
+    float s = 0.f;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        const float e = out[v] - t[v];
+        dout[v] = 2.f * e * scale;
+        s += e * e;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, s * scale);
+}
+
+// head backward: dx[v][c] = dout[v] w[c]; dw[c] += sum_v dout[v] x[v][c]; db += sum dout
+__global__ __launch_bounds__(256) void head_bwd_kernel(const bf16 *__restrict__ x, const float *__restrict__ w,
+                                                       const float *__restrict__ dout, bf16 *__restrict__ dx,
+                                                       float *__restrict__ dw, float *__restrict__ db, long V) {
+    float sw[16], sb = 0.f;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) sw[c] = 0.f;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        const float d = dout[v];
+        sb += d;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            sw[c] += d * (float)x[v * 16 + c];
+            dx[v * 16 + c] = (bf16)(d * w[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        float t = sw[c];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&dw[c], t);
+    }
+    for (int off = 32; off > 0; off >>= 1) sb += __shfl_xor(sb, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(db, sb);
+}
+
+// ------------------------------------------------------------------------------- weights
+// fragment repack: wp[(kc*NT+nt)*64 + lane][e] = W[k][n], k = 32 kc + 8 (lane/16) + e, n = 16 nt + lane%16;
+// flip != 0 builds the input-gradient weights W'[tap'][cout][cin] = W[26-tap'][cin][cout]
+__global__ void repack_kernel(const float *__restrict__ w, bf16 *__restrict__ wp, int cin, int cout, int flip) {
+    const int ci = flip ? cout : cin, co = flip ? cin : cout;   // GEMM input / output channels
+    const int K = 27 * ci, KC = (K + 31) / 32, NT = co / 16;
+    const long n = (long)KC * NT * 64 * 8;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int e = (int)(q % 8);
+        const int lane = (int)((q / 8) % 64);
+        const long blk = q / 512;
+        const int nt = (int)(blk % NT), kc = (int)(blk / NT);
+        const int k = 32 * kc + 8 * (lane / 16) + e, nn = 16 * nt + lane % 16;
+        float val = 0.f;
+        if (k < K) {
+            const int tap = k / ci, c = k % ci;
+            if (!flip) val = w[((long)tap * cin + c) * cout + nn];
+            else val = w[((long)(26 - tap) * cin + nn) * cout + c];
+        }
+        wp[q] = (bf16)val;
+    }
+}
+
+// Adam (Kingma & Ba 2015, Alg. 1) on FP32 master weights; grads are averaged over the batch
+__global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, long n, float lr, float b1, float b2, float eps, float c1,
+                            float c2, float gscale) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const float gg = g[q] * gscale;
+        const float mm = b1 * m[q] + (1.f - b1) * gg;
+        const float vv = b2 * v[q] + (1.f - b2) * gg * gg;
+        m[q] = mm;
+        v[q] = vv;
+        p[q] = p[q] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    }
+}
+
+// ------------------------------------------------------------------------------- solver coupling
+// network input from the solver's FP64 state: ch0 = div(u*) / s_r, ch1 = (p - mean p) / s_p
+__global__ void make_input_kernel(Dims d, const double *__restrict__ rhs, const double *__restrict__ pn,
+                                  const double *__restrict__ stats, bf16 *__restrict__ x) {
+    const long V = (long)d.nx * d.ny * d.nz;
+    const double sr = stats[0], pm = stats[1], sp = stats[2];
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(v % d.nx) + 1, j = (int)((v / d.nx) % d.ny) + 1, k = (int)(v / ((long)d.nx * d.ny)) + 1;
+        bf16x8 a = {}, b = {};
+        a[0] = (bf16)(float)(rhs[v] / sr);
+        a[1] = (bf16)(float)((pn[d.g(i, j, k)] - pm) / sp);
+        *reinterpret_cast<bf16x8 *>(x + v * 16) = a;
+        *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = b;
+    }
+}
+
+// sums over interior cells for the input scaling: [sum r^2, sum p, sum p^2] (deterministic, 2 pass)
+__global__ __launch_bounds__(256) void input_stats_partial(Dims d, const double *__restrict__ rhs,
+                                                           const double *__restrict__ pn, double *part) {
+    __shared__ double s[3][256];
+    double a = 0, b = 0, c = 0;
+    const long V = (long)d.nx * d.ny * d.nz;
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long)gridDim.x * 256) {
+        const int i = (int)(v % d.nx) + 1, j = (int)((v / d.nx) % d.ny) + 1, k = (int)(v / ((long)d.nx * d.ny)) + 1;
+        const double r = rhs[v], p = pn[d.g(i, j, k)];
+        a += r * r;
+        b += p;
+        c += p * p;
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 3; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) part[3 * blockIdx.x + q] = s[q][0];
+}
+
+__global__ void input_stats_final(const double *part, int nb, double V, double *stats) {
+    if (threadIdx.x != 0) return;
+    double a = 0, b = 0, c = 0;
+    for (int q = 0; q < nb; ++q) {
+        a += part[3 * q];
+        b += part[3 * q + 1];
+        c += part[3 * q + 2];
+    }
+    const double sr = sqrt(a / V), pm = b / V, var = c / V - pm * pm;
+    stats[0] = sr > 1e-300 ? sr : 1.0;
+    stats[1] = pm;
+    stats[2] = var > 1e-300 ? sqrt(var) : 1.0;
+}
+
+// Phi guess (FP64, interior layout) = out * s_r * c0
+__global__ void guess_out_kernel(const float *__restrict__ out, const double *__restrict__ stats, double c0,
+                                 double *__restrict__ phi, long V) {
+    const double s = stats[0] * c0;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
+        phi[v] = (double)out[v] * s;
+}
+
+// training target = Phi / (s_r c0) in FP32
+__global__ void make_target_kernel(const double *__restrict__ phi, const double *__restrict__ stats, double c0,
+                                   float *__restrict__ t, long V) {
+    const double s = 1.0 / (stats[0] * c0);
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
+        t[v] = (float)(phi[v] * s);
+}
+
+// replay buffer: store / load the 2 used input channels and the target
+__global__ void pack2_kernel(const bf16 *__restrict__ x, bf16 *__restrict__ x2, long V) {
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        x2[2 * v] = x[v * 16];
+        x2[2 * v + 1] = x[v * 16 + 1];
+    }
+}
+__global__ void unpack2_kernel(const bf16 *__restrict__ x2, bf16 *__restrict__ x, long V) {
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        bf16x8 a = {}, b = {};
+        a[0] = x2[2 * v];
+        a[1] = x2[2 * v + 1];
+        *reinterpret_cast<bf16x8 *>(x + v * 16) = a;
+        *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = b;
+    }
+}
+__global__ void input_from_f32_kernel(const float *__restrict__ x2, bf16 *__restrict__ x, long V) {
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        bf16x8 a = {}, b = {};
+        a[0] = (bf16)x2[2 * v];
+        a[1] = (bf16)x2[2 * v + 1];
+        *reinterpret_cast<bf16x8 *>(x + v * 16) = a;
+        *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = b;
+    }
+}
+
+// ------------------------------------------------------------------------------- launchers
+static inline unsigned nblk(long n, int t = 256) {
+    long b = (n + t - 1) / t;
+    return (unsigned)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+
+static float *Wt(mdx_net *n, int l) { return n->params + n->woff[l]; }
+static float *Bs(mdx_net *n, int l) { return n->params + n->boff[l]; }
+static float *dWt(mdx_net *n, int l) { return n->grads + n->woff[l]; }
+static float *dBs(mdx_net *n, int l) { return n->grads + n->boff[l]; }
+
+// 3x3x3 conv (implicit GEMM on MFMA): forward (relu, bias) or input gradient (plain)
+static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in, const bf16 *wp, const float *bias,
+                        bf16 *out, int lvl) {
+    const int W = n->W[lvl], H = n->H[lvl], D = n->D[lvl];
+    const unsigned blocks = (unsigned)(n->V[lvl] / 64);
+#define CONV_CASE(CI, CO, R)                                                                                       \
+    if (cin == CI && cout == CO && relu == R) {                                                                    \
+        hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, R>), dim3(blocks), dim3(256), 0, n->stream, in,                \
+                           reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);                              \
+        return hipGetLastError();                                                                                  \
+    }
+    CONV_CASE(16, 16, true) CONV_CASE(16, 32, true) CONV_CASE(32, 32, true) CONV_CASE(32, 64, true)
+    CONV_CASE(64, 64, true) CONV_CASE(96, 32, true) CONV_CASE(48, 16, true)
+    CONV_CASE(16, 16, false) CONV_CASE(32, 16, false) CONV_CASE(32, 32, false) CONV_CASE(64, 32, false)
+    CONV_CASE(64, 64, false) CONV_CASE(32, 96, false) CONV_CASE(16, 48, false)
+#undef CONV_CASE
+    return hipErrorInvalidValue;
+}
+
+static hipError_t wgrad(mdx_net *n, int l, const bf16 *in, const bf16 *dy) {
+    const LayerDesc &L = LAYERS[l];
+    const int lvl = L.level;
+    const int W = n->W[lvl], H = n->H[lvl], D = n->D[lvl];
+    const long chunks = n->V[lvl] / 32;
+    int slices = (int)std::min<long>(48, std::max<long>(1, chunks / 16));
+#define WG_CASE(CI, CO)                                                                                       \
+    if (L.cin == CI && L.cout == CO) {                                                                       \
+        hipLaunchKernelGGL((conv3_wgrad_kernel<CI, CO>), dim3(27 * slices), dim3(256), 0, n->stream, in, dy,    \
+                           dWt(n, l), W, H, D, slices);                                                      \
+    } else
+    WG_CASE(16, 16) WG_CASE(16, 32) WG_CASE(32, 32) WG_CASE(32, 64) WG_CASE(64, 64) WG_CASE(96, 32) WG_CASE(48, 16)
+    return hipErrorInvalidValue;
+#undef WG_CASE
+    const long V = n->V[lvl];
+    if (L.cout == 16) hipLaunchKernelGGL(bias_grad_kernel<16>, dim3(nblk(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    else if (L.cout == 32) hipLaunchKernelGGL(bias_grad_kernel<32>, dim3(nblk(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    else hipLaunchKernelGGL(bias_grad_kernel<64>, dim3(nblk(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    return hipGetLastError();
+}
+
+static hipError_t relu_bwd(mdx_net *n, bf16 *g, const bf16 *extra, int Ce, int coff, int C, const bf16 *y, int lvl) {
+    const long cnt = n->V[lvl] * C;
+    hipLaunchKernelGGL(relu_bwd_kernel, dim3(nblk(cnt)), dim3(256), 0, n->stream, g, extra, Ce, coff, C, y, cnt);
+    return hipGetLastError();
+}
+
+hipError_t repack_all(mdx_net *n) {
+    for (int l = 0; l < NL; ++l) {
+        const LayerDesc &L = LAYERS[l];
+        if (L.ks != 3) continue;
+        const long cf = (long)((27 * L.cin + 31) / 32) * (L.cout / 16) * 512;
+        hipLaunchKernelGGL(repack_kernel, dim3(nblk(cf)), dim3(256), 0, n->stream, Wt(n, l), n->wp[l], L.cin, L.cout, 0);
+        if (n->wd[l]) {
+            const long cd = (long)((27 * L.cout + 31) / 32) * (L.cin / 16) * 512;
+            hipLaunchKernelGGL(repack_kernel, dim3(nblk(cd)), dim3(256), 0, n->stream, Wt(n, l), n->wd[l], L.cin, L.cout, 1);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t upload_input2(mdx_net *n, const float *x2) {
+    const long V = n->V[0];
+    hipError_t e = hipMemcpyAsync(n->x2_stage, x2, (size_t)V * 2 * 4, hipMemcpyHostToDevice, n->stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(input_from_f32_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, n->x2_stage, n->x_in, V);
+    return hipGetLastError();
+}
+
+hipError_t net_forward(mdx_net *n) {
+    hipError_t e;
+    const int W0 = n->W[0], H0 = n->H[0], D0 = n->D[0], W1 = n->W[1], H1 = n->H[1], D1 = n->D[1];
+#define TRY(x) if ((e = (x)) != hipSuccess) return e
+    TRY(conv3(n, 16, 16, true, n->x_in, n->wp[0], Bs(n, 0), n->a[0], 0));
+    TRY(conv3(n, 16, 16, true, n->a[0], n->wp[1], Bs(n, 1), n->a[1], 0));
+    hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[1] * 16)), dim3(256), 0, n->stream, n->a[1], n->p1, 16, W0, H0, D0);
+    TRY(conv3(n, 16, 32, true, n->p1, n->wp[2], Bs(n, 2), n->a[2], 1));
+    TRY(conv3(n, 32, 32, true, n->a[2], n->wp[3], Bs(n, 3), n->a[3], 1));
+    hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[2] * 32)), dim3(256), 0, n->stream, n->a[3], n->p2, 32, W1, H1, D1);
+    TRY(conv3(n, 32, 64, true, n->p2, n->wp[4], Bs(n, 4), n->a[4], 2));
+    TRY(conv3(n, 64, 64, true, n->a[4], n->wp[5], Bs(n, 5), n->a[5], 2));
+    hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[1] * 96)), dim3(256), 0, n->stream, n->a[5], n->a[3], n->cat2, 64, 32,
+                       W1, H1, D1);
+    TRY(conv3(n, 96, 32, true, n->cat2, n->wp[6], Bs(n, 6), n->a[6], 1));
+    TRY(conv3(n, 32, 32, true, n->a[6], n->wp[7], Bs(n, 7), n->a[7], 1));
+    hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[0] * 48)), dim3(256), 0, n->stream, n->a[7], n->a[1], n->cat1, 32, 16,
+                       W0, H0, D0);
+    TRY(conv3(n, 48, 16, true, n->cat1, n->wp[8], Bs(n, 8), n->a[8], 0));
+    TRY(conv3(n, 16, 16, true, n->a[8], n->wp[9], Bs(n, 9), n->a[9], 0));
+    hipLaunchKernelGGL(head_fwd_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->a[9], Wt(n, 10), Bs(n, 10),
+                       n->out, n->V[0]);
+    return hipGetLastError();
+}
+
+// loss = mean (out - target)^2 of this sample; parameter gradients scaled by `scale` are added
+hipError_t net_backward(mdx_net *n, float scale) {
+    hipError_t e;
+    const long V0 = n->V[0];
+    const int W0 = n->W[0], H0 = n->H[0], D0 = n->D[0], W1 = n->W[1], H1 = n->H[1], D1 = n->D[1];
+    TRY(hipMemsetAsync(n->loss, 0, 4, n->stream));
+    hipLaunchKernelGGL(mse_kernel, dim3(nblk(V0)), dim3(256), 0, n->stream, n->out, n->target, n->dout,
+                       scale / (float)V0, V0, n->loss);
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(nblk(V0)), dim3(256), 0, n->stream, n->a[9], Wt(n, 10), n->dout, n->g[9],
+                       dWt(n, 10), dBs(n, 10), V0);
+    // decoder level 0
+    TRY(relu_bwd(n, n->g[9], nullptr, 0, 0, 16, n->a[9], 0));
+    TRY(wgrad(n, 9, n->a[8], n->g[9]));
+    TRY(conv3(n, 16, 16, false, n->g[9], n->wd[9], nullptr, n->g[8], 0));
+    TRY(relu_bwd(n, n->g[8], nullptr, 0, 0, 16, n->a[8], 0));
+    TRY(wgrad(n, 8, n->cat1, n->g[8]));
+    TRY(conv3(n, 16, 48, false, n->g[8], n->wd[8], nullptr, n->gcat1, 0));
+    // decoder level 1
+    hipLaunchKernelGGL(upcat_bwd_coarse_kernel, dim3(nblk(n->V[1] * 32)), dim3(256), 0, n->stream, n->gcat1, n->g[7], 32,
+                       16, W0, H0, D0);
+    TRY(relu_bwd(n, n->g[7], nullptr, 0, 0, 32, n->a[7], 1));
+    TRY(wgrad(n, 7, n->a[6], n->g[7]));
+    TRY(conv3(n, 32, 32, false, n->g[7], n->wd[7], nullptr, n->g[6], 1));
+    TRY(relu_bwd(n, n->g[6], nullptr, 0, 0, 32, n->a[6], 1));
+    TRY(wgrad(n, 6, n->cat2, n->g[6]));
+    TRY(conv3(n, 32, 96, false, n->g[6], n->wd[6], nullptr, n->gcat2, 1));
+    // bottleneck level 2
+    hipLaunchKernelGGL(upcat_bwd_coarse_kernel, dim3(nblk(n->V[2] * 64)), dim3(256), 0, n->stream, n->gcat2, n->g[5], 64,
+                       32, W1, H1, D1);
+    TRY(relu_bwd(n, n->g[5], nullptr, 0, 0, 64, n->a[5], 2));
+    TRY(wgrad(n, 5, n->a[4], n->g[5]));
+    TRY(conv3(n, 64, 64, false, n->g[5], n->wd[5], nullptr, n->g[4], 2));
+    TRY(relu_bwd(n, n->g[4], nullptr, 0, 0, 64, n->a[4], 2));
+    TRY(wgrad(n, 4, n->p2, n->g[4]));
+    TRY(conv3(n, 64, 32, false, n->g[4], n->wd[4], nullptr, n->gp2, 2));
+    // encoder level 1 (pool backward + skip gradient from the concat)
+    hipLaunchKernelGGL(pool_bwd_kernel, dim3(nblk(n->V[1] * 32)), dim3(256), 0, n->stream, n->gp2, n->g[3], 32, W1, H1, D1);
+    TRY(relu_bwd(n, n->g[3], n->gcat2, 96, 64, 32, n->a[3], 1));
+    TRY(wgrad(n, 3, n->a[2], n->g[3]));
+    TRY(conv3(n, 32, 32, false, n->g[3], n->wd[3], nullptr, n->g[2], 1));
+    TRY(relu_bwd(n, n->g[2], nullptr, 0, 0, 32, n->a[2], 1));
+    TRY(wgrad(n, 2, n->p1, n->g[2]));
+    TRY(conv3(n, 32, 16, false, n->g[2], n->wd[2], nullptr, n->gp1, 1));
+    // encoder level 0
+    hipLaunchKernelGGL(pool_bwd_kernel, dim3(nblk(n->V[0] * 16)), dim3(256), 0, n->stream, n->gp1, n->g[1], 16, W0, H0, D0);
+    TRY(relu_bwd(n, n->g[1], n->gcat1, 48, 32, 16, n->a[1], 0));
+    TRY(wgrad(n, 1, n->a[0], n->g[1]));
+    TRY(conv3(n, 16, 16, false, n->g[1], n->wd[1], nullptr, n->g[0], 0));
+    TRY(relu_bwd(n, n->g[0], nullptr, 0, 0, 16, n->a[0], 0));
+    TRY(wgrad(n, 0, n->x_in, n->g[0]));
+    return hipGetLastError();
+}
+
+hipError_t net_adam(mdx_net *n, float lr) {
+    n->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)n->adam_t), c2 = 1.f - powf(b2, (float)n->adam_t);
+    hipLaunchKernelGGL(adam_kernel, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->params, n->grads, n->adam_m,
+                       n->adam_v, (long)n->nparam, lr, b1, b2, 1e-8f, c1, c2, 1.0f);
+    hipError_t e;
+    TRY(hipMemsetAsync(n->grads, 0, n->nparam * 4, n->stream));
+    return repack_all(n);
+}
+
+// ------------------------------------------------------------------------------- solver coupling
+hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *rhs, const double *pn) {
+    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, h->d, rhs, pn, n->part);
+    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)n->V[0], n->stats);
+    hipLaunchKernelGGL(make_input_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, h->d, rhs, pn, n->stats, n->x_in);
+    return hipGetLastError();
+}
+
+hipError_t net_guess_to_phi(mdx_net *n, double *phi) {
+    hipError_t e;
+    TRY(net_forward(n));
+    hipLaunchKernelGGL(guess_out_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->stats, n->c0, phi,
+                       n->V[0]);
+    return hipGetLastError();
+}
+
+// append (current input, Phi) to the replay ring buffer
+hipError_t net_capture(mdx_net *n, const double *phi) {
+    const long V = n->V[0];
+    const int slot = n->next;
+    hipLaunchKernelGGL(pack2_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, n->x_in, n->rb_x + (size_t)slot * 2 * V, V);
+    hipLaunchKernelGGL(make_target_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, phi, n->stats, n->c0,
+                       n->rb_t + (size_t)slot * V, V);
+    n->next = (n->next + 1) % n->cap;
+    if (n->count < n->cap) n->count += 1;
+    return hipGetLastError();
+}
+
+// loss of the current output against the current target (no gradient accumulation)
+hipError_t net_mse_only(mdx_net *n) {
+    hipError_t e = hipMemsetAsync(n->loss, 0, 4, n->stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mse_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->target, n->dout,
+                       1.0f / (float)n->V[0], n->V[0], n->loss);
+    return hipGetLastError();
+}
+
+// root mean square of an interior FP64 field (deterministic), into stats[3]
+hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi) {
+    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, h->d, phi, h->pn, n->part);
+    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)n->V[0], n->stats + 4);
+    return hipMemcpyAsync(n->stats + 3, n->stats + 4, 8, hipMemcpyDeviceToDevice, n->stream);
+}
+
+hipError_t net_load_replay(mdx_net *n, int slot) {
+    const long V = n->V[0];
+    hipLaunchKernelGGL(unpack2_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, n->rb_x + (size_t)slot * 2 * V, n->x_in, V);
+    return hipMemcpyAsync(n->target, n->rb_t + (size_t)slot * V, (size_t)V * 4, hipMemcpyDeviceToDevice, n->stream);
+}
+#undef TRY
+
+}  // namespace unet
+}  // namespace mdx

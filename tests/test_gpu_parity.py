@@ -258,3 +258,49 @@ def test_initial_field_matches_oracle(solver):
     for k in ("us", "vs", "ws", "un", "vn", "wn"):
         assert _rel(out[k], f[k]) <= tol, k
     assert not out["pn"].any()
+
+
+# ------------------------------------------------------------------------------ LDS-resident sweeps
+def _wavy(nx, ny, nz, per_y=0, ibm=True):
+    g = O.Grid(nx, ny, nz, lx=4.0, ly=2.2, lz=2.0, re=2000.0, periodic=(1, per_y, 1))
+    g.bc_type[O.bci(2, 0, 0)] = 1
+    g.bc_type[O.bci(2, 1, 0)] = 1
+    coef = O.ibm_coefs(g, O.Geometry(kind=1, A=0.1, lam=1.0, half_height=1.0)) if ibm else None
+    return g, coef
+
+
+@pytest.mark.parametrize("shape,per_y,ibm", [((48, 32, 24), 0, True), ((32, 16, 16), 1, False),
+                                             ((192, 128, 96), 0, True)])
+def test_lds_sweeps_bit_exact_fixed_iterations(shape, per_y, ibm, monkeypatch):
+    monkeypatch.delenv("MOBYDIFF_NO_LDS_SOR", raising=False)
+    g, coef = _wavy(*shape, per_y=per_y, ibm=ibm)
+    rng = np.random.default_rng(21)
+    f = O.random_fields(g, rng, amp=0.05)
+    n = 9
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-300, max_iter=n, check_every=n)
+    st = ch.pressure_projection(0.02)
+    assert st.lds_sweeps == 1 and st.iterations == n, ch.info()
+    out = ch.download()
+    O.projection_sor(g, f, coef, 1.8, 0.0, n, n, 0.02)
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(out[k], f[k]), k
+
+
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
+def test_lds_sweeps_equal_streaming_sweeps(precision, monkeypatch):
+    g, coef = _wavy(48, 32, 24)
+    rng = np.random.default_rng(22)
+    f = O.random_fields(g, rng, amp=0.05)
+    outs, stats, infos = [], [], []
+    for off in ("0", "1"):
+        monkeypatch.setenv("MOBYDIFF_NO_LDS_SOR", off)
+        ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=10,
+                      precision=precision, inner_iters=10)
+        stats.append(ch.pressure_projection(5e-4 * 64 / 120))
+        outs.append(ch.download())
+        infos.append(ch.info())
+        ch.close()
+    assert stats[0].lds_sweeps == 1 and stats[1].lds_sweeps == 0, infos
+    assert stats[0].iterations == stats[1].iterations and stats[0].converged == 1
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k

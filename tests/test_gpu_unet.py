@@ -1,0 +1,175 @@
+"""GPU checks of the pressure-guess U-Net (BF16/MFMA HIP kernels) against the torch-CPU fp32
+reference of the same network (oracle/unet_ref.py).
+
+Bar (written per test): forward within 1e-2 relative of the bf16-emulating reference; parameter
+gradients within 5e-2 relative (L2, per layer) of fp32 autograd on the same bf16-rounded inputs;
+Adam bit-for-bit-close (1e-6) to the closed form applied to the GPU's own gradients.
+"""
+import numpy as np
+import pytest
+import torch
+
+import mobydiff_amd as M
+from oracle import unet_ref as R
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 64, 16, 16
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-30))
+
+
+def _inputs(seed):
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    smooth = np.sin(2 * np.pi * x / NX) * np.cos(np.pi * y / NY) + 0.5 * np.cos(2 * np.pi * z / NZ)
+    x2 = np.stack([smooth + 0.3 * rng.standard_normal(smooth.shape),
+                   np.cos(2 * np.pi * x / NX + 0.3) + 0.3 * rng.standard_normal(smooth.shape)], axis=-1)
+    target = (0.5 * smooth + 0.1 * rng.standard_normal(smooth.shape)).astype(np.float32)
+    return x2.astype(np.float32), target
+
+
+def _torch_x(x2):
+    t = torch.zeros(1, 16, NZ, NY, NX)
+    t[0, 0] = torch.as_tensor(x2[..., 0])
+    t[0, 1] = torch.as_tensor(x2[..., 1])
+    return t
+
+
+def test_param_layout_matches_reference():
+    net = M.Net(NX, NY, NZ, seed=3)
+    p, _ = net.get_params()
+    assert p.size == sum(k ** 3 * ci * co + co for _, ci, co, k in R.LAYERS)
+    params = R.flat_to_params(p)
+    assert np.array_equal(R.params_to_flat(params), p)
+    # He-normal initialisation: per-layer std close to sqrt(2 / fan_in)
+    for name, ci, co, k in R.LAYERS:
+        w = params[name + ".w"].numpy()
+        assert abs(w.std() / np.sqrt(2.0 / (k ** 3 * ci)) - 1) < 0.15, name
+
+
+def test_forward_matches_bf16_reference():
+    net = M.Net(NX, NY, NZ, seed=1)
+    p, _ = net.get_params()
+    params = R.flat_to_params(p)
+    x2, _ = _inputs(0)
+    out = net.forward(x2)
+    ref = R.forward_bf16(params, _torch_x(x2))[0, 0].numpy()
+    assert np.isfinite(out).all()
+    assert _rel(out, ref) < 1e-2, _rel(out, ref)
+    ref32 = R.forward(params, _torch_x(x2))[0, 0].detach().numpy()
+    assert _rel(out, ref32) < 5e-2
+
+
+def test_gradients_match_autograd():
+    net = M.Net(NX, NY, NZ, seed=2)
+    p, _ = net.get_params()
+    params = R.flat_to_params(p)
+    x2, target = _inputs(1)
+    net.zero_grads()
+    loss = net.accumulate_grad(x2, target, batch=1)
+    g = R.flat_to_params(net.get_grads())
+    bparams = {k: (R._bf(v) if k.endswith(".w") and not k.startswith("head") else v) for k, v in params.items()}
+    rl, rg = R.loss_and_grads(bparams, R._bf(_torch_x(x2)), torch.as_tensor(target)[None, None])
+    assert abs(loss - float(rl)) < 2e-2 * float(rl)
+    for name, ci, co, k in R.LAYERS:
+        for suffix in (".w", ".b"):
+            a, b = g[name + suffix].numpy(), rg[name + suffix].numpy()
+            assert _rel(a, b) < 5e-2, (name + suffix, _rel(a, b))
+
+
+def test_gradient_directional_finite_difference():
+    """g . d against (L(p + eps d) - L(p - eps d)) / (2 eps) on the GPU network itself."""
+    net = M.Net(NX, NY, NZ, seed=4)
+    p, _ = net.get_params()
+    x2, target = _inputs(2)
+    net.zero_grads()
+    net.accumulate_grad(x2, target)
+    g = net.get_grads().astype(np.float64)
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal(p.size).astype(np.float32) * np.abs(p).mean()
+    eps = 0.05
+    losses = []
+    for s in (+1, -1):
+        net.set_params(p + s * eps * d)
+        net.zero_grads()
+        losses.append(net.accumulate_grad(x2, target))
+    fd = (losses[0] - losses[1]) / (2 * eps)
+    an = float(np.dot(g, d.astype(np.float64)))
+    assert abs(fd - an) < 0.1 * abs(an) + 1e-4, (fd, an)
+
+
+def test_adam_matches_closed_form():
+    net = M.Net(NX, NY, NZ, seed=6)
+    p, _ = net.get_params()
+    x2, target = _inputs(3)
+    net.zero_grads()
+    net.accumulate_grad(x2, target)
+    g = net.get_grads()
+    net.adam_step(1e-3)
+    p1, _ = net.get_params()
+    P = {"w": torch.as_tensor(p)}
+    G = {"w": torch.as_tensor(g)}
+    Z = {"w": torch.zeros_like(P["w"])}
+    ref, _, _ = R.adam_step(P, G, Z, Z, 1, lr=1e-3)
+    assert np.max(np.abs(p1 - ref["w"].numpy())) < 1e-6
+    assert np.array_equal(net.get_grads(), np.zeros_like(g))
+
+
+def test_training_reduces_loss():
+    net = M.Net(NX, NY, NZ, seed=7)
+    data = [_inputs(10 + q) for q in range(4)]
+    first = None
+    for step in range(60):
+        loss = 0.0
+        for x2, t in data:
+            loss += net.accumulate_grad(x2, t, batch=len(data))
+        net.adam_step(3e-3)
+        first = loss if first is None else first
+    assert loss < 0.5 * first, (first, loss)
+
+
+def test_learned_guess_in_the_solver():
+    """Capture (rhs, p_prev) -> Phi pairs from the wavy-channel solver, train the network on them,
+    then solve with the learned guess: the tolerance is still met, the guess is closer to the
+    converged correction than the previous-step guess, and the fallback is reported."""
+    nx, ny, nz, dt = 64, 32, 32, 5e-4
+    geo = M.wavy_geometry(A=0.1)
+
+    def channel(guess):
+        cfg = M.channel_config(nx, ny, nz, 4.0, 2.2, 2.0, 2000.0, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8,
+                               check_every=10, precision=M.PREC_MIXED, inner_iters=10, guess=guess,
+                               flowrate=True, flow_height=2.0)
+        ch = M.Channel(cfg)
+        ch.compute_ibm(geo)
+        ch.init_fields(seed=0, amp=0.05, geo=geo, dt=dt)
+        return ch
+
+    net = M.Net(nx, ny, nz, seed=0)
+    net.replay_reserve(64)
+    ch = channel(M.GUESS_PREVIOUS)
+    ch.attach_net(net, capture=True)
+    for _ in range(12):
+        ch.step(dt)
+    assert net.replay_count() == 36
+    l0 = net.eval_replay(8)
+    net.train(150, batch=4, lr=1e-3)
+    l1 = net.eval_replay(8)
+    assert l1 < 0.5 * l0, (l0, l1)
+    # continue the same flow with the learned guess
+    state = ch.download()
+    ch2 = channel(M.GUESS_LEARNED)
+    ch2.upload(**state)
+    ch2.attach_net(net, capture=False)
+    its, fallbacks = [], 0
+    for _ in range(3):
+        st = ch2.step(dt)
+        for q in range(3):
+            s = st.stage[q]
+            assert s.converged == 1 and s.div_inf < 1e-8
+            its.append(s.iterations)
+            fallbacks += s.fallback
+    assert ch2.div_norm() < 1.5e-8
+    assert fallbacks <= 9
