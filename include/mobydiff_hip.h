@@ -124,6 +124,10 @@ int mdx_set_ibm(mdx_handle *h, const double *coef_u, const double *coef_v, const
  * computed by the library on the host and uploaded; the arrays are also returned when non-NULL */
 int mdx_compute_ibm(mdx_handle *h, const mdx_geometry *geo, double *coef_u, double *coef_v,
                     double *coef_w);
+/* set_ibm_coeff(g, ibm, coeff, dix, diy, diz) (ibm.f90:133): one ghosted coefficient array for the
+ * staggering (dix, diy, diz), computed on the host into coeff; nothing is uploaded */
+int mdx_ibm_coefficient(mdx_handle *h, const mdx_geometry *geo, int32_t dix, int32_t diy, int32_t diz,
+                        double *coeff);
 
 /* step.f90:27 momentum(f, g, dt_alpha, dt_beta, dt_gamma, ibm, bc) */
 int mdx_momentum(mdx_handle *h, double dt_alpha, double dt_beta, double dt_gamma);

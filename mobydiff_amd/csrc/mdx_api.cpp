@@ -504,6 +504,17 @@ static void ibm_coeff(const mdx_handle *h, const mdx_geometry *geo, int dix, int
     }
 }
 
+extern "C" int mdx_ibm_coefficient(mdx_handle *h, const mdx_geometry *geo, int32_t dix, int32_t diy, int32_t diz,
+                                   double *coeff) {
+    MDX_CHECK(h && geo && coeff, MDX_ERR_ARG, "mdx_ibm_coefficient: null argument");
+    MDX_CHECK(geo->kind == 0 || geo->kind == 1, MDX_ERR_ARG, "mdx_ibm_coefficient: unknown geometry kind");
+    MDX_CHECK(!h->stretched, MDX_ERR_UNSUPPORTED, "IBM geometry requires the uniform wall-normal grid");
+    std::vector<double> c;
+    ibm_coeff(h, geo, dix, diy, diz, c);
+    std::memcpy(coeff, c.data(), h->ng * 8);
+    return MDX_OK;
+}
+
 extern "C" int mdx_compute_ibm(mdx_handle *h, const mdx_geometry *geo, double *cu, double *cv, double *cw) {
     MDX_CHECK(h && geo, MDX_ERR_ARG, "mdx_compute_ibm: null argument");
     MDX_CHECK(geo->kind == 0 || geo->kind == 1, MDX_ERR_ARG, "mdx_compute_ibm: unknown geometry kind");

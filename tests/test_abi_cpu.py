@@ -12,7 +12,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     text = open(os.path.join(ROOT, "include", "mobydiff_hip.h")).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(mdx_\w+)\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t|const char \*)\s*(mdx_\w+)\(", text, re.M)))
+
+
+def test_declaration_scan_sees_every_entry_point():
+    text = open(os.path.join(ROOT, "include", "mobydiff_hip.h")).read()
+    every = set(re.findall(r"\b(mdx_\w+)\(", text))
+    assert every <= set(_declared()), sorted(every - set(_declared()))
 
 
 def test_library_exports_every_declared_symbol():
