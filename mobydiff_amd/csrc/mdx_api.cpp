@@ -284,6 +284,16 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         return fail(e, "reduction buffer");
     h->rb_err = reinterpret_cast<int *>(h->red_max + 1);
     if (const char *nl = std::getenv("MOBYDIFF_NO_LDS_SOR")) h->use_lds_sor = !(nl[0] == '1');
+    if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
+        const int v = std::atoi(kz);
+        if (!(v == 0 || v == 1 || v == 2 || v == 4 || v == 8)) {
+            mdx_set_error("mdx_create: MOBYDIFF_SWEEP_KZ must be 0, 1, 2, 4 or 8");
+            free_all(h);
+            delete h;
+            return MDX_ERR_ARG;
+        }
+        h->sweep_kz = v;
+    }
     if ((e = hipHostMalloc((void **)&h->h_pinned, 64 * sizeof(double))) != hipSuccess) return fail(e, "pinned buffer");
 
     if (cfg->solver == MDX_SOLVER_SOR) {

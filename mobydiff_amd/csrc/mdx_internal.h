@@ -152,6 +152,8 @@ struct mdx_handle {
         bool ibm = false;
     } graphs[2];
     bool use_graphs = true;
+    // streaming red-black sweeps: z planes marched per thread (0: one cell per thread)
+    int sweep_kz = 4;
 
     // LDS-resident red-black solver: plans per precision, face buffers, progress counters
     bool use_lds_sor = true;

@@ -285,6 +285,140 @@ hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg) {
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------- z-marching sweeps
+// One thread owns the x-pair of cells (i0, i0+1), i0 = 2p+1, in row j and marches KZ planes in z.
+// In every plane exactly one cell of the pair has the sweep's colour, and it alternates with k, so
+// the pair is loaded once per plane as one 2-element vector and kept in registers for the z
+// neighbours of the next plane; the x neighbour inside the pair comes from the same vector.  Per
+// updated cell this leaves one vector load, three scalar loads (outer x neighbour, y neighbours),
+// rhs and one vector store of the pair (its other cell is rewritten with its own, unchanged bits:
+// cells of the other colour are not written by this sweep).  z-invariant mu tables are loaded
+// once per thread.  The arithmetic is sor_residual_core with the operands of cell_residual, so the
+// iterates are bit-identical to the one-cell-per-thread sweep.
+template <typename T> struct Pair;
+template <> struct Pair<float> { using V = float2; };
+template <> struct Pair<double> { using V = double2; };
+
+static constexpr int ZX = 32;   // pairs per block row
+static constexpr int ZY = 8;    // rows per block
+
+template <typename T, bool STRETCH, bool IBM, int KZ>
+__global__ __launch_bounds__(ZX * ZY) void sor_sweep_zm_kernel(GridK g, MuK<T> mu, SorConst<T> s, int color,
+                                                               T omega, const T *__restrict__ rhs, T *phi) {
+    using V = typename Pair<T>::V;
+    const Dims d = g.d;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int np = nx / 2, nseg = (nz + KZ - 1) / KZ;
+    int tx_, ty_, tz_;
+    if (!xcd_tile(np, ny, nseg, ZX, ZY, tx_, ty_, tz_)) return;
+    const int p = tx_ * ZX + threadIdx.x;
+    const int j = ty_ * ZY + threadIdx.y + 1;
+    if (p >= np || j > ny) return;
+    const int k0 = tz_ * KZ + 1;
+    const int k1 = k0 + KZ - 1 < nz ? k0 + KZ - 1 : nz;
+    const int i0 = 2 * p + 1;
+    const int iw = i0 == 1 ? nx : i0 - 1;          // west neighbour of i0
+    const int ie = i0 + 1 == nx ? 1 : i0 + 2;      // east neighbour of i0+1
+    const bool per2 = g.per[1];
+    int jp = j + 1, jm = j - 1;
+    if (per2) {
+        if (jp > ny) jp = 1;
+        if (jm < 1) jm = ny;
+    }
+    const bool wn_ = !per2 && j == ny, ws_ = !per2 && j == 1;
+    const int jn = wn_ ? j : jp, js = ws_ ? j : jm;
+    T an = 0, as = 0;
+    if (STRETCH) {
+        an = (T)(g.ihv[j + 1] * g.ihc[j]);
+        as = (T)(g.ihv[j] * g.ihc[j]);
+    }
+    // mu of the two cells: u faces at i0, i0+1, ie; v faces at j, jp; w faces at k, kp
+    T mu_u0 = 1, mu_u1 = 1, mu_u2 = 1, mv0 = 1, mv1 = 1, mvp0 = 1, mvp1 = 1;
+    T mw0 = 1, mw1 = 1, mwp0 = 1, mwp1 = 1;
+    if (IBM && mu.plane) {
+        mu_u0 = mu.u[mu.at(i0, j, 0)];
+        mu_u1 = mu.u[mu.at(i0 + 1, j, 0)];
+        mu_u2 = mu.u[mu.at(ie, j, 0)];
+        mv0 = mu.v[mu.at(i0, j, 0)];
+        mv1 = mu.v[mu.at(i0 + 1, j, 0)];
+        mvp0 = mu.v[mu.at(i0, jp, 0)];
+        mvp1 = mu.v[mu.at(i0 + 1, jp, 0)];
+        mw0 = mwp0 = mu.w[mu.at(i0, j, 0)];
+        mw1 = mwp1 = mu.w[mu.at(i0 + 1, j, 0)];
+    }
+    const size_t rs = (size_t)nx, ps = (size_t)nx * ny;
+    const size_t rowj = (size_t)(i0 - 1) + rs * (size_t)(j - 1);   // pair offset within a plane
+    const size_t rown = rs * (size_t)(jn - 1);
+    const size_t rows = rs * (size_t)(js - 1);
+    const V *P2 = reinterpret_cast<const V *>(phi);
+    V *W2 = reinterpret_cast<V *>(phi);
+    const int km0 = k0 == 1 ? nz : k0 - 1;
+    V prv = P2[(rowj + ps * (size_t)(km0 - 1)) / 2];
+    V cur = P2[(rowj + ps * (size_t)(k0 - 1)) / 2];
+    for (int k = k0; k <= k1; ++k) {
+        const int kp = k == nz ? 1 : k + 1;
+        const size_t pk = ps * (size_t)(k - 1);
+        const V nxt = P2[(rowj + ps * (size_t)(kp - 1)) / 2];
+        const int c = (j + k + color) & 1;
+        const int i = i0 + c;
+        const T pout = phi[(size_t)((c ? ie : iw) - 1) + rs * (size_t)(j - 1) + pk];
+        const T pyp = phi[(size_t)(i - 1) + rown + pk];
+        const T pym = phi[(size_t)(i - 1) + rows + pk];
+        const T rc = rhs[rowj + (size_t)c + pk];
+        if (IBM && !mu.plane) {
+            mu_u0 = mu.u[mu.at(i0, j, k)];
+            mu_u1 = mu.u[mu.at(i0 + 1, j, k)];
+            mu_u2 = mu.u[mu.at(ie, j, k)];
+            mv0 = mu.v[mu.at(i0, j, k)];
+            mv1 = mu.v[mu.at(i0 + 1, j, k)];
+            mvp0 = mu.v[mu.at(i0, jp, k)];
+            mvp1 = mu.v[mu.at(i0 + 1, jp, k)];
+            mw0 = mu.w[mu.at(i0, j, k)];
+            mw1 = mu.w[mu.at(i0 + 1, j, k)];
+            mwp0 = mu.w[mu.at(i0, j, kp)];
+            mwp1 = mu.w[mu.at(i0 + 1, j, kp)];
+        }
+        const T p0 = c ? cur.y : cur.x;
+        const T pin = c ? cur.x : cur.y;
+        const T pxp = c ? pout : pin, pxm = c ? pin : pout;
+        const T pzp = c ? nxt.y : nxt.x, pzm = c ? prv.y : prv.x;
+        const T mui = c ? mu_u1 : mu_u0, muip = c ? mu_u2 : mu_u1;
+        const T mvj = c ? mv1 : mv0, mvjp = c ? mvp1 : mvp0;
+        const T mwk = c ? mw1 : mw0, mwkp = c ? mwp1 : mwp0;
+        T denom;
+        const T r = sor_residual_core<T, STRETCH>(p0, pxp, pxm, pyp, pym, pzp, pzm, mui, muip, mvj, mvjp, mwk, mwkp,
+                                                  wn_, ws_, rc, s, an, as, denom);
+        const T nv = p0 + r * (-omega / denom);
+        if (c) cur.y = nv;
+        else cur.x = nv;
+        W2[(rowj + pk) / 2] = cur;
+        prv = cur;
+        cur = nxt;
+    }
+}
+
+template <typename T>
+static hipError_t launch_sweep_zm(mdx_handle *h, int color, T omega, const MuK<T> &mu, const SorConst<T> &s,
+                                  const T *rhs, T *phi) {
+    GridK g = grid_k(h);
+    const int kz = h->sweep_kz;
+    dim3 blk(ZX, ZY), grd(xcd_grid(h->d.nx / 2, h->d.ny, (h->d.nz + kz - 1) / kz, ZX, ZY));
+#define MDX_ZM(S, I, K) \
+    hipLaunchKernelGGL((sor_sweep_zm_kernel<T, S, I, K>), grd, blk, 0, h->stream, g, mu, s, color, omega, rhs, phi)
+#define MDX_ZMK(S, I)                      \
+    switch (kz) {                          \
+    case 1: MDX_ZM(S, I, 1); break;        \
+    case 2: MDX_ZM(S, I, 2); break;        \
+    case 8: MDX_ZM(S, I, 8); break;        \
+    default: MDX_ZM(S, I, 4); break;       \
+    }
+    if (h->stretched) { if (h->ibm_on) { MDX_ZMK(true, true) } else { MDX_ZMK(true, false) } }
+    else { if (h->ibm_on) { MDX_ZMK(false, true) } else { MDX_ZMK(false, false) } }
+#undef MDX_ZMK
+#undef MDX_ZM
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------------------- FP64 sweeps
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(SX * SY) void sor_sweep64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
@@ -309,6 +443,7 @@ hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    if (h->sweep_kz > 0) return launch_sweep_zm<double>(h, color, omega, mu, s, h->rhs, h->phi);
     dim3 blk(SX, SY), grd(xcd_grid(h->d.nx / 2, h->d.ny, h->d.nz, SX, SY));
 #define MDX_SW(S, I) \
     hipLaunchKernelGGL((sor_sweep64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, color, omega, h->rhs, h->phi)
@@ -417,6 +552,7 @@ hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega) {
     GridK g = grid_k(h);
     MuK<float> mu = mu_k<float>(h, h->mu32);
     SorConst<float> s{(float)(1.0 / (h->dx * h->dx)), (float)(1.0 / (h->dy * h->dy)), (float)(1.0 / (h->dz * h->dz))};
+    if (h->sweep_kz > 0) return launch_sweep_zm<float>(h, color, omega, mu, s, h->r32, h->e32);
     dim3 blk(SX, SY), grd(xcd_grid(h->d.nx / 2, h->d.ny, h->d.nz, SX, SY));
 #define MDX_SW(S, I) \
     hipLaunchKernelGGL((sor_sweep32_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, color, omega, h->r32, h->e32)

@@ -286,6 +286,56 @@ def test_lds_sweeps_bit_exact_fixed_iterations(shape, per_y, ibm, monkeypatch):
         assert np.array_equal(out[k], f[k]), k
 
 
+@pytest.mark.parametrize("kz", [1, 2, 4, 8])
+@pytest.mark.parametrize("case", ["wavy", "stretched_random_mu", "periodic_y"])
+def test_zmarch_sweeps_bit_exact_fixed_iterations(kz, case, monkeypatch):
+    """Streaming sweeps with a thread marching kz planes (nz = 22: the last segment is short)
+    against the oracle, bit for bit, for z-invariant (plane) and general 3-D mu tables."""
+    monkeypatch.setenv("MOBYDIFF_NO_LDS_SOR", "1")
+    monkeypatch.setenv("MOBYDIFF_SWEEP_KZ", str(kz))
+    rng = np.random.default_rng(23)
+    if case == "wavy":
+        g, coef = _wavy(48, 32, 22)
+    elif case == "periodic_y":
+        g, coef = _wavy(40, 16, 22, per_y=1, ibm=False)
+    else:
+        g = O.Grid(48, 24, 22, lx=4.0, ly=2.0, lz=2.0, re=2800.0, periodic=(1, 0, 1), beta=1.5)
+        g.bc_type[O.bci(2, 0, 0)] = 1
+        g.bc_type[O.bci(2, 1, 0)] = 1
+        coef = tuple(np.where(rng.random(g.gshape) < 0.2, 0.0, 50.0 * rng.random(g.gshape)) for _ in range(3))
+    f = O.random_fields(g, rng, amp=0.05)
+    n = 7
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-300, max_iter=n, check_every=n)
+    st = ch.pressure_projection(0.02)
+    assert st.lds_sweeps == 0 and st.iterations == n, ch.info()
+    out = ch.download()
+    ch.close()
+    O.projection_sor(g, f, coef, 1.8, 0.0, n, n, 0.02)
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(out[k], f[k]), (k, case, kz)
+
+
+@pytest.mark.parametrize("kz", [2, 8])
+def test_mixed_solve_same_for_sweep_kernels(kz, monkeypatch):
+    """The FP32 defect-correction solve gives the same iterates with the one-cell-per-thread sweep
+    (kz = 0) and the z-marching sweep."""
+    g, coef = _wavy(48, 32, 24)
+    rng = np.random.default_rng(24)
+    f = O.random_fields(g, rng, amp=0.05)
+    monkeypatch.setenv("MOBYDIFF_NO_LDS_SOR", "1")
+    outs = []
+    for v in ("0", str(kz)):
+        monkeypatch.setenv("MOBYDIFF_SWEEP_KZ", v)
+        ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=10,
+                      precision=M.PREC_MIXED, inner_iters=10)
+        st = ch.pressure_projection(5e-4 * 64 / 120)
+        assert st.converged == 1
+        outs.append(ch.download())
+        ch.close()
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
 def test_lds_sweeps_equal_streaming_sweeps(precision, monkeypatch):
     g, coef = _wavy(48, 32, 24)
