@@ -5,10 +5,13 @@ Lx x Ly x Lz = 4 x 2.2 x 2, uniform grid 192 x 128 x 96, Re_b = 2000 with consta
 dt = 5e-4, Poiseuille + 0.05 seeded perturbation.  Each step is one RK3 step: momentum + IBM,
 red-black SOR pressure solve to ||div||_inf < 1e-8, projection, boundary conditions.
 
-Timed configurations, both from the same state after the warm-up steps:
-  * headline: FP32 red-black sweeps inside an FP64 defect-correction loop (plus the learned
-    initial guess when a trained network is attached);
-  * plain: FP64 red-black SOR (the comparison path).
+During the warm-up steps the solver's converged solves are captured as training pairs for the
+U-Net initial guess, which is then trained on the GPU (Adam, batch 4) before the timed steps.
+Timed configurations, all from the same state after the warm-up steps:
+  * headline: learned initial guess + FP32 red-black sweeps inside an FP64 defect-correction loop;
+  * plain: FP64 red-black SOR from the previous pressure (the comparison path);
+  * mixed_no_guess: the headline's solver without the guess;
+  * mixed_previous_phi_guess: the previous step's Phi of each stage as the guess, no network.
 Prints one JSON line (see the "metric" field); value = Mcell-steps/s of the headline path.
 """
 import argparse
@@ -41,15 +44,16 @@ def make_channel(M, precision, check_every, inner):
 def run_steps(ch, k):
     ch.synchronize()
     ch.reset_phase_times()
-    its, div = [], 0.0
+    its, fb = [], 0
     t0 = time.perf_counter()
     for _ in range(k):
         st = ch.step(DT)
         its.append([st.stage[q].iterations for q in range(3)])
+        fb += sum(st.stage[q].fallback for q in range(3))
     ch.synchronize()
     el = time.perf_counter() - t0
     div = ch.div_norm()
-    return el, its, div, ch.phase_times()
+    return el, its, div, ch.phase_times(), fb
 
 
 def roofline(times, kind):
@@ -90,6 +94,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-every", type=int, default=20)
     ap.add_argument("--inner", type=int, default=20)
+    ap.add_argument("--train-steps", type=int, default=200)
+    ap.add_argument("--no-learned", action="store_true", help="headline without the learned guess")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -102,42 +108,86 @@ def main():
     head = make_channel(M, M.PREC_MIXED, args.check_every, args.inner)
     coef = head.compute_ibm(geo)
     head.init_fields(seed=0, amp=AMP, geo=geo, dt=DT)
+    learned = not args.no_learned
+    training = None
+    net = None
+    if learned:
+        # the network learns from the solver's own converged solves during the warm-up steps
+        net = M.Net(NX, NY, NZ, seed=0)
+        net.replay_reserve(3 * args.warmup)
+        head.attach_net(net, capture=True)
     for _ in range(args.warmup):
         head.step(DT)
     state = head.download()
+    if learned:
+        t0 = time.perf_counter()
+        pairs = net.replay_count()
+        l0 = net.eval_replay(pairs)
+        net.train(args.train_steps, batch=4, lr=1e-3)
+        l1 = net.eval_replay(pairs)
+        head.synchronize()
+        training = {"pairs": pairs, "adam_steps": args.train_steps, "batch": 4, "lr": 1e-3,
+                    "mse_before": l0, "mse_after": l1, "seconds": round(time.perf_counter() - t0, 2),
+                    "source": f"(div u*, previous-step Phi) -> Phi pairs captured from the {args.warmup} "
+                              "warm-up steps of this workload"}
+        head.attach_net(net, capture=False)
+        head.set_guess(M.GUESS_LEARNED)
 
-    plain = make_channel(M, M.PREC_FP64, args.check_every, args.inner)
-    plain.set_ibm(*coef)
-    plain.upload(**state)
+    def fresh(precision, guess=M.GUESS_PREVIOUS, prime=False):
+        ch = make_channel(M, precision, args.check_every, args.inner)
+        ch.set_ibm(*coef)
+        ch.upload(**state)
+        ch.set_guess(guess)
+        if prime:
+            ch.step(DT)   # fills the per-stage history the guess uses
+        return ch
 
-    el_h, its_h, div_h, t_h = run_steps(head, args.steps)
-    el_p, its_p, div_p, t_p = run_steps(plain, args.steps)
+    el_h, its_h, div_h, t_h, fb_h = run_steps(head, args.steps)
+    plain = fresh(M.PREC_FP64)
+    el_p, its_p, div_p, t_p, _ = run_steps(plain, args.steps)
+    plain.close()
+    mixed = fresh(M.PREC_MIXED)
+    el_m, its_m, div_m, t_m, _ = run_steps(mixed, args.steps)
+    mixed.close()
+    prevphi = fresh(M.PREC_MIXED, M.GUESS_PHI_PREV, prime=True)
+    el_q, its_q, div_q, t_q, fb_q = run_steps(prevphi, args.steps)
+    prevphi.close()
     cells = NX * NY * NZ
-    v_h = cells * args.steps / el_h / 1e6
-    v_p = cells * args.steps / el_p / 1e6
+    rate = lambda el: cells * args.steps / el / 1e6   # noqa: E731
+    per_stage = lambda its: sum(map(sum, its)) / (3.0 * args.steps)   # noqa: E731
 
     def split(t, k):
         return {"stencil": round(t.stencil_ms / k, 3), "poisson": round(t.poisson_ms / k, 3),
                 "projection": round(t.projection_ms / k, 3), "guess": round(t.guess_ms / k, 3)}
 
+    def side(el, its, div, t):
+        return {"value": round(rate(el), 3), "ms_per_step": round(el / args.steps * 1e3, 3),
+                "phase_ms_per_step": split(t, args.steps), "sor_iterations_per_stage": per_stage(its),
+                "div_inf_final": div}
+
+    v_h, v_p = rate(el_h), rate(el_p)
     out = {
         "metric": METRIC, "value": round(v_h, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el_h / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
         "data": "synthetic",
         "config": {"workload": "wavy-wall channel 192x128x96 (A=0.1, Re_b=2000, dt=5e-4), RK3 step with "
-                               "red-black SOR to ||div||_inf<1e-8: FP32 sweeps in FP64 defect correction",
-                   "grid": [NX, NY, NZ], "learned_guess": False, "omega": OMEGA, "tol": TOL,
+                               "red-black SOR to ||div||_inf<1e-8: " +
+                               ("learned initial guess + " if learned else "") +
+                               "FP32 sweeps in FP64 defect correction",
+                   "grid": [NX, NY, NZ], "learned_guess": learned, "omega": OMEGA, "tol": TOL,
                    "check_every": args.check_every, "inner_iters": args.inner, "parallelism": "1 GPU",
                    "sor_kernel": head.info()},
         "phase_ms_per_step": split(t_h, args.steps),
-        "sor_iterations_per_stage": sum(map(sum, its_h)) / (3.0 * args.steps),
+        "sor_iterations_per_stage": per_stage(its_h),
         "div_inf_final": div_h,
-        "fallback_rate": None,
-        "plain_fp64": {"value": round(v_p, 3), "ms_per_step": round(el_p / args.steps * 1e3, 3),
-                       "phase_ms_per_step": split(t_p, args.steps),
-                       "sor_iterations_per_stage": sum(map(sum, its_p)) / (3.0 * args.steps),
-                       "div_inf_final": div_p, "roofline": roofline(t_p, "fp64")},
+        "fallback_rate": round(fb_h / (3.0 * args.steps), 4) if learned else None,
+        "training": training,
+        "plain_fp64": dict(side(el_p, its_p, div_p, t_p), roofline=roofline(t_p, "fp64")),
+        "mixed_no_guess": side(el_m, its_m, div_m, t_m),
+        "mixed_previous_phi_guess": dict(side(el_q, its_q, div_q, t_q),
+                                         fallback_rate=round(fb_q / (3.0 * args.steps), 4)),
+        "sweeps_saved_vs_no_guess": round(1.0 - per_stage(its_h) / per_stage(its_m), 4),
         "speedup_vs_plain_fp64": round(v_h / v_p, 3),
         "roofline": roofline(t_h, "fp32"),
     }
@@ -148,7 +198,6 @@ def main():
     else:
         out["cpu_baseline"] = None
     head.close()
-    plain.close()
     print(json.dumps(out))
 
 

@@ -35,7 +35,11 @@ enum { MDX_SOLVER_FFT = 0,   /* FFT(x,z) + Thomas(y), pressure_solver.f90:48-63 
 
 enum { MDX_PREC_FP64 = 0, MDX_PREC_MIXED = 1 };   /* MIXED: FP32 sweeps in FP64 defect correction */
 
-enum { MDX_GUESS_PREVIOUS = 0, MDX_GUESS_LEARNED = 1 };
+/* initial guess of the SOR solve: PREVIOUS = Phi 0 (start from the previous pressure, the plain
+ * method); LEARNED = the attached network on (div u*, Phi of the same RK stage at the previous
+ * step); PHI_PREV = that previous Phi alone.  LEARNED and PHI_PREV fall back to PREVIOUS whenever
+ * their initial residual is not smaller. */
+enum { MDX_GUESS_PREVIOUS = 0, MDX_GUESS_LEARNED = 1, MDX_GUESS_PHI_PREV = 2 };
 
 typedef struct {
     /* grid_type (init.f90:5-12) */
@@ -149,8 +153,14 @@ int mdx_bulk_velocity(mdx_handle *h, double *ub);
 /* Poiseuille + seeded uniform perturbation of amplitude amp, projected divergence-free, p = 0.
  * geo selects the wall geometry for the profile and masks (NULL: flat channel). */
 int mdx_init_fields(mdx_handle *h, uint32_t seed, double amp, const mdx_geometry *geo, double dt);
+/* select the initial guess of later SOR solves (MDX_GUESS_*); the per-stage history is kept */
+int mdx_set_guess(mdx_handle *h, int32_t guess);
 /* one RK3 step (main.f90:66-89): 3 x {momentum, [flow rate], [apply_bc], projection, bc} */
 int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats);
+
+/* one RK3 step of n independent channels, driven concurrently (one host thread and stream per
+ * handle); stats may be NULL or hold n entries.  No counterpart in the reference. */
+int mdx_step_batch(mdx_handle *const *hs, int32_t n, double dt, mdx_step_stats *stats);
 
 /* per-phase GPU time accumulated since the last reset (cfg.timing must be set) */
 int mdx_get_phase_times(mdx_handle *h, mdx_phase_times *t);

@@ -17,7 +17,7 @@ LIBPATH = os.path.join(HERE, "csrc", "libmobydiff_hip.so")
 
 SOLVER_FFT, SOLVER_SOLA, SOLVER_SOR = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
-GUESS_PREVIOUS, GUESS_LEARNED = 0, 1
+GUESS_PREVIOUS, GUESS_LEARNED, GUESS_PHI_PREV = 0, 1, 2
 
 _PD = C.POINTER(C.c_double)
 
@@ -137,6 +137,16 @@ def bump_geometry(amp_x=0.0, phase_x=0.0, n_wave_x=1, second_order=True):
     return g
 
 
+def step_batch(channels, dt):
+    """One RK3 step of independent channels, run concurrently by the library (mdx_step_batch);
+    returns the StepStats of each channel."""
+    n = len(channels)
+    hs = (C.c_void_p * n)(*[ch.h.value for ch in channels])
+    st = (StepStats * n)()
+    _check(lib().mdx_step_batch(hs, C.c_int32(n), C.c_double(dt), st))
+    return list(st)
+
+
 RK_ALPHA = (64.0 / 120.0, 50.0 / 120.0, 90.0 / 120.0)   # step.f90:21-23
 RK_BETA = (0.0, -34.0 / 120.0, -50.0 / 120.0)
 RK_GAMMA = (64.0 / 120.0, 16.0 / 120.0, 40.0 / 120.0)
@@ -250,6 +260,9 @@ class Channel:
 
     def reset_phase_times(self):
         _check(lib().mdx_reset_phase_times(self.h))
+
+    def set_guess(self, guess):
+        _check(lib().mdx_set_guess(self.h, C.c_int32(guess)))
 
     def attach_net(self, net, capture=False):
         _check(lib().mdx_attach_net(self.h, net.h if net is not None else None, C.c_int32(1 if capture else 0)))

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mdx_internal.h"
@@ -192,7 +193,7 @@ static void free_all(mdx_handle *h) {
                     h->cu, h->cv, h->cw, h->d_ihc, h->d_ihv, h->d_yc, h->phi, h->phi2, h->rhs,
                     h->e32, h->r32, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
-                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags};
+                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags, h->phi_hist};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -237,6 +238,9 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     } else {
         MDX_CHECK(false, MDX_ERR_ARG, "mdx_create: unknown solver");
     }
+    MDX_CHECK(cfg->guess == MDX_GUESS_PREVIOUS || (cfg->solver == MDX_SOLVER_SOR &&
+              (cfg->guess == MDX_GUESS_LEARNED || cfg->guess == MDX_GUESS_PHI_PREV)),
+              MDX_ERR_ARG, "mdx_create: unknown guess mode, or a guess for a solver other than SOR");
     if (cfg->flowrate)
         MDX_CHECK(cfg->periodic[0] && cfg->flow_height > 0.0, MDX_ERR_UNSUPPORTED,
                   "constant flow rate needs periodic x and flow_height > 0");
@@ -304,6 +308,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             if ((e = dalloc(&h->e32, h->ni)) != hipSuccess || (e = dalloc(&h->r32, h->ni)) != hipSuccess)
                 return fail(e, "SOR FP32 buffers");
         if ((e = hipMemset(h->phi, 0, h->ni * 8)) != hipSuccess) return fail(e, "SOR buffers");
+        if ((e = dalloc(&h->phi_hist, 3 * h->ni)) != hipSuccess) return fail(e, "SOR guess history");
     }
     if (cfg->solver == MDX_SOLVER_FFT) {
         const int nxh = nx / 2 + 1;
@@ -768,22 +773,37 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     return MDX_OK;
 }
 
-// Learned initial guess: network inference on (div u*, p_prev), then the guess and the
-// previous-step guess (Phi = 0) are compared by their initial ||div||_inf and the smaller one
-// starts the SOR; the solve itself, and its stopping rule, are the same either way.
-static int learned_guess(mdx_handle *h, mdx_solve_stats *st, double *r_first) {
+// Initial guess of the SOR solve for stage s; the solve itself, and its stopping rule, are the
+// same whatever starts it.  The plain method starts from Phi = 0 (the previous
+// pressure).  MDX_GUESS_LEARNED evaluates the network on (div u*, Phi of stage s at the previous
+// step) and MDX_GUESS_PHI_PREV takes that previous Phi alone (scaled by the dt_gamma ratio); either
+// candidate is kept only when its FP64 residual max-norm is below that of Phi = 0, otherwise the
+// solve falls back to Phi = 0 (stats.fallback).  With capture on, the network input is built for
+// every solve so the converged Phi can be stored as a training pair.
+static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, double *r_first) {
     mdx_net *n = h->net;
-    const bool want_guess = n && h->cfg.guess == MDX_GUESS_LEARNED && n->c0 > 0.0;
-    if (!n || !(want_guess || h->net_capture)) return MDX_OK;
-    hipStream_t saved = n->stream;
-    n->stream = h->stream;
-    ev_record(h, 8);
-    hipError_t e = unet::net_input_from_solver(n, h, h->rhs, h->pn);
-    if (e == hipSuccess && want_guess) e = unet::net_guess_to_phi(n, h->phi2);
-    ev_record(h, 9);
-    n->stream = saved;
-    MDX_HIP(e);
-    if (!want_guess) return MDX_OK;
+    const bool have_prev = h->hist_dtg[s] > 0.0;
+    const double *prev = have_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
+    const double fac = have_prev ? dtg / h->hist_dtg[s] : 0.0;
+    const bool learned = n && h->cfg.guess == MDX_GUESS_LEARNED && n->c0 > 0.0;
+    const bool phi_prev = h->cfg.guess == MDX_GUESS_PHI_PREV && have_prev;
+    if (n && (learned || h->net_capture)) {
+        hipStream_t saved = n->stream;
+        n->stream = h->stream;
+        ev_record(h, 8);
+        hipError_t e = unet::net_input_from_solver(n, h, h->rhs, prev, fac);
+        if (e == hipSuccess && learned) e = unet::net_guess_to_phi(n, prev, fac, h->phi2);
+        ev_record(h, 9);
+        n->stream = saved;
+        MDX_HIP(e);
+        if (h->timing && learned) {
+            float ms = 0.f;
+            MDX_HIP(hipEventSynchronize(h->ev[9]));
+            if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
+        }
+    }
+    if (!learned && !phi_prev) return MDX_OK;
+    if (phi_prev) MDX_HIP(launch_scale_copy64(h, h->phi2, prev, fac, h->ni));
     double r0 = 0.0, rg = 0.0;
     int rc;
     MDX_HIP(launch_reset_max(h));
@@ -792,10 +812,6 @@ static int learned_guess(mdx_handle *h, mdx_solve_stats *st, double *r_first) {
     MDX_HIP(launch_reset_max(h));
     MDX_HIP(launch_sor_residual64(h, h->phi));   // Phi = 0: the previous-step guess
     if ((rc = read_max(h, &r0))) return rc;
-    if (h->timing) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
-    }
     st->guess_div_inf = rg;
     if (rg < r0) {
         std::swap(h->phi, h->phi2);
@@ -807,10 +823,13 @@ static int learned_guess(mdx_handle *h, mdx_solve_stats *st, double *r_first) {
     return MDX_OK;
 }
 
-// Append (network input, converged Phi) to the attached network's replay buffer.  The target
-// scale c0 = rms(Phi) / rms(div u*) is fixed by the first captured pair.
-static int capture_pair(mdx_handle *h) {
+// Append (network input, converged Phi) of stage s to the attached network's replay buffer.  The
+// target scale c0 = rms(Phi) / rms(div u*) is fixed by the first captured pair.
+static int capture_pair(mdx_handle *h, int s, double dtg) {
     mdx_net *n = h->net;
+    const bool have_prev = h->hist_dtg[s] > 0.0;
+    const double *prev = have_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
+    const double fac = have_prev ? dtg / h->hist_dtg[s] : 0.0;
     hipStream_t saved = n->stream;
     n->stream = h->stream;
     hipError_t e = hipSuccess;
@@ -823,7 +842,7 @@ static int capture_pair(mdx_handle *h) {
             n->c0 = (sphi > 0.0 && sr > 0.0) ? sphi / sr : 1.0;
         }
     }
-    if (e == hipSuccess) e = unet::net_capture(n, h->phi);
+    if (e == hipSuccess) e = unet::net_capture(n, h->phi, prev, fac);
     n->stream = saved;
     MDX_HIP(e);
     return MDX_OK;
@@ -860,11 +879,17 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         if ((rc = rb_prepare(h))) return rc;
         MDX_HIP(launch_sor_prepare(h));
         double r_first = -1.0;
-        if ((rc = learned_guess(h, &st, &r_first))) return rc;
+        const int stage = (int)(h->proj_count % 3);
+        if ((rc = initial_guess(h, stage, dtg, &st, &r_first))) return rc;
         rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st) : sor_solve64(h, &st, r_first);
         if (rc) return rc;
         if (st.fallback == 0 && r_first >= 0.0) st.div0_inf = r_first;
-        if (h->net_capture && st.converged && (rc = capture_pair(h))) return rc;
+        if (h->net && h->net_capture && st.converged && (rc = capture_pair(h, stage, dtg))) return rc;
+        // the converged Phi becomes the history of this stage for the next step
+        MDX_HIP(hipMemcpyAsync(h->phi_hist + (size_t)stage * h->ni, h->phi, h->ni * 8, hipMemcpyDeviceToDevice,
+                               h->stream));
+        h->hist_dtg[stage] = dtg;
+        h->proj_count += 1;
         ev_record(h, 2);
         MDX_HIP(launch_sor_apply(h, h->phi, dtg));
         ev_record(h, 3);
@@ -877,6 +902,16 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         }
     }
     if (stats) *stats = st;
+    return MDX_OK;
+}
+
+extern "C" int mdx_set_guess(mdx_handle *h, int32_t guess) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    MDX_CHECK(guess == MDX_GUESS_PREVIOUS || guess == MDX_GUESS_LEARNED || guess == MDX_GUESS_PHI_PREV,
+              MDX_ERR_ARG, "mdx_set_guess: unknown guess mode");
+    MDX_CHECK(guess == MDX_GUESS_PREVIOUS || h->cfg.solver == MDX_SOLVER_SOR, MDX_ERR_UNSUPPORTED,
+              "mdx_set_guess: initial guesses serve the SOR solver");
+    h->cfg.guess = guess;
     return MDX_OK;
 }
 
@@ -939,6 +974,9 @@ extern "C" int mdx_init_fields(mdx_handle *h, uint32_t seed, double amp, const m
     if ((rc = post_bc(h))) return rc;
     MDX_HIP(launch_zero64(h, h->pn, h->ng));
     MDX_HIP(hipStreamSynchronize(h->stream));
+    // the projection of the initial field is not an RK stage: start the guess history afresh
+    h->proj_count = 0;
+    for (double &t : h->hist_dtg) t = 0.0;
     return MDX_OK;
 }
 
@@ -953,6 +991,7 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
         if (h->cfg.flowrate && (rc = mdx_flowrate_shift(h, dtg, nullptr))) return rc;
         if (h->cfg.solver == MDX_SOLVER_FFT && (rc = mdx_apply_bc(h))) return rc;
         mdx_solve_stats st{};
+        h->proj_count = s;   // RK stage of this projection (initial-guess history)
         if ((rc = mdx_pressure_projection(h, dtg, &st))) {
             if (stats) stats->stage[s] = st;
             return rc;
@@ -976,6 +1015,48 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
     if (stats && h->cfg.flowrate) {
         if ((rc = mdx_bulk_velocity(h, &stats->ub))) return rc;
     }
+    return MDX_OK;
+}
+
+// A batch of independent channels (for example wavy channels with different amplitudes and
+// seeds) stepped concurrently: one host thread per handle drives that handle's own stream, so the
+// GPU overlaps the instances' kernels and each instance's residual checks wait only on its own
+// work.  Each instance computes exactly what mdx_step computes for it alone.  The LDS-resident
+// sweeps need the whole GPU for one cooperative launch and are switched off for batched handles.
+extern "C" int mdx_step_batch(mdx_handle *const *hs, int32_t n, double dt, mdx_step_stats *stats) {
+    MDX_CHECK(hs && n > 0, MDX_ERR_ARG, "mdx_step_batch: need n > 0 handles");
+    for (int q = 0; q < n; ++q) {
+        MDX_CHECK(hs[q], MDX_ERR_ARG, "mdx_step_batch: null handle");
+        for (int r = 0; r < q; ++r) MDX_CHECK(hs[r] != hs[q], MDX_ERR_ARG, "mdx_step_batch: a handle appears twice");
+    }
+    for (int q = 0; q < n; ++q) {
+        mdx_handle *h = hs[q];
+        if (h->use_lds_sor) {
+            h->use_lds_sor = false;
+            h->rb_state = -1;
+            h->rb_note = "streaming sweeps: concurrent batch";
+        }
+    }
+    std::vector<int> rc(n, MDX_OK);
+    std::vector<std::string> err(n);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int q = 0; q < n; ++q)
+        th.emplace_back([&, q] {
+            if (hipSetDevice(hs[q]->device) != hipSuccess) {
+                rc[q] = MDX_ERR_HIP;
+                err[q] = "hipSetDevice failed";
+                return;
+            }
+            rc[q] = mdx_step(hs[q], dt, stats ? &stats[q] : nullptr);
+            if (rc[q]) err[q] = mdx_last_error();
+        });
+    for (auto &t : th) t.join();
+    for (int q = 0; q < n; ++q)
+        if (rc[q]) {
+            mdx_set_error("mdx_step_batch: instance " + std::to_string(q) + ": " + err[q]);
+            return rc[q];
+        }
     return MDX_OK;
 }
 

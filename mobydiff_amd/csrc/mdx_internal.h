@@ -168,6 +168,11 @@ struct mdx_handle {
     // learned guess (attached network) and training-pair capture
     struct mdx_net *net = nullptr;
     bool net_capture = false;
+    // converged Phi of each RK stage at the previous step (initial-guess history), its dt_gamma
+    // (0: no history yet) and the count of SOR projections (stage = count mod 3)
+    double *phi_hist = nullptr;
+    double hist_dtg[3] = {0.0, 0.0, 0.0};
+    long proj_count = 0;
 };
 
 // kernel launchers (defined in the .hip files)
@@ -199,6 +204,7 @@ hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, 
 hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega);
 hipError_t launch_zero32(mdx_handle *h, float *p, size_t n);
 hipError_t launch_zero64(mdx_handle *h, double *p, size_t n);
+hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, double fac, size_t n);
 bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init);
 

@@ -568,6 +568,17 @@ __global__ void zero_kernel(T *p, size_t n) {
     for (; q < n; q += (size_t)gridDim.x * blockDim.x) p[q] = T(0);
 }
 
+__global__ void scale_copy_kernel(double *__restrict__ dst, const double *__restrict__ src, double fac, size_t n) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q < n; q += (size_t)gridDim.x * blockDim.x) dst[q] = fac * src[q];
+}
+
+// dst = fac * src over n values
+hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, double fac, size_t n) {
+    hipLaunchKernelGGL(scale_copy_kernel, dim3(2048), dim3(256), 0, h->stream, dst, src, fac, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero32(mdx_handle *h, float *p, size_t n) {
     hipLaunchKernelGGL(zero_kernel<float>, dim3(2048), dim3(256), 0, h->stream, p, n);
     return hipGetLastError();

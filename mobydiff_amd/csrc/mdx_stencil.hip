@@ -17,7 +17,7 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
     const double *__restrict__ wn, const double *__restrict__ pn, double *__restrict__ us,
     double *__restrict__ vs, double *__restrict__ ws, double *__restrict__ oru,
     double *__restrict__ orv, double *__restrict__ orw, const double *__restrict__ cu,
-    const double *__restrict__ cv, const double *__restrict__ cw, double dta, double dtb,
+    const double *__restrict__ cv, const double *__restrict__ cw, int coef_kstride, double dta, double dtb,
     double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
     double ire) {
     const Dims d = g.d;
@@ -31,6 +31,8 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
     const int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
     const size_t c = d.g(i, j, k);
     const size_t o = d.in(i, j, k);
+    // coefficient index: z-invariant geometries keep one x-y plane (coef_kstride 0)
+    const size_t cc = d.g(i, j, k * coef_kstride);
     const double *ihc = g.ihc, *ihv = g.ihv;
 
     if (i >= nx0) {
@@ -57,7 +59,7 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
         double rhsu = -(uu_p - uu_m) * idx - cy - (uw_p - uw_m) * idz + g.fx +
                       ire * (diff_ux + diff_uy + diff_uz);
         double s = u0 + dta * rhsu + dtb * oru[o] - dtg * dpx;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cu[c]));
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cu[cc]));
         us[c] = s;
         oru[o] = rhsu;
     }
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
         double rhsv = -(vu_p - vu_m) * idx - cy - (vw_p - vw_m) * idz + g.fy +
                       ire * (diff_vx + diff_vy + diff_vz);
         double s = v0 + dta * rhsv + dtb * orv[o] - dtg * dpy;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cv[c]));
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cv[cc]));
         vs[c] = s;
         orv[o] = rhsv;
     }
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
         double rhsw = -(wu_p - wu_m) * idx - cy - (ww_p - ww_m) * idz + g.fz +
                       ire * (diff_wx + diff_wy + diff_wz);
         double s = w0 + dta * rhsw + dtb * orw[o] - dtg * dpz;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cw[c]));
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cw[cc]));
         ws[c] = s;
         orw[o] = rhsw;
     }
@@ -128,7 +130,7 @@ hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
 #define MDX_MOM(S, I)                                                                        \
     hipLaunchKernelGGL((momentum_kernel<S, I>), grd, blk, 0, h->stream, g, h->un, h->vn, h->wn, \
                        h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw, h->cu, h->cv, h->cw,  \
-                       dta, dtb, dtg, 1.0 / dx, 1.0 / dy, 1.0 / dz, 1.0 / dx2, 1.0 / dy2,      \
+                       h->coef_plane ? 0 : 1, dta, dtb, dtg, 1.0 / dx, 1.0 / dy, 1.0 / dz, 1.0 / dx2, 1.0 / dy2,      \
                        1.0 / dz2, 1.0 / h->cfg.re)
     if (h->stretched) {
         if (h->ibm_on) MDX_MOM(true, true); else MDX_MOM(true, false);

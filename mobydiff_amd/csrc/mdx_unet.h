@@ -51,10 +51,14 @@ hipError_t upload_input2(mdx_net *n, const float *x2);
 hipError_t net_forward(mdx_net *n);
 hipError_t net_backward(mdx_net *n, float scale);
 hipError_t net_adam(mdx_net *n, float lr);
-// solver coupling: input from the solver's rhs / pressure, output to an FP64 Phi guess
-hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *rhs, const double *pn);
-hipError_t net_guess_to_phi(mdx_net *n, double *phi);
-hipError_t net_capture(mdx_net *n, const double *phi);
+// solver coupling.  Inputs: ch0 = div(u*) / s_r, ch1 = fac * Phi_prev / (s_r c0), where Phi_prev
+// is the converged correction of the same RK stage at the previous step (nullptr: none yet) and
+// fac = dt_gamma / dt_gamma_prev.  The network predicts the change: Phi = fac * Phi_prev +
+// out * s_r * c0, so its training target is (Phi - fac * Phi_prev) / (s_r c0).
+hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *rhs, const double *phi_prev,
+                                 double fac);
+hipError_t net_guess_to_phi(mdx_net *n, const double *phi_prev, double fac, double *phi);
+hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, double fac);
 hipError_t net_load_replay(mdx_net *n, int slot);
 hipError_t net_mse_only(mdx_net *n);
 hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi);

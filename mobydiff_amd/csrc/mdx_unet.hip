@@ -329,75 +329,57 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
 }
 
 // ------------------------------------------------------------------------------- solver coupling
-// network input from the solver's FP64 state: ch0 = div(u*) / s_r, ch1 = (p - mean p) / s_p
-__global__ void make_input_kernel(Dims d, const double *__restrict__ rhs, const double *__restrict__ pn,
-                                  const double *__restrict__ stats, bf16 *__restrict__ x) {
-    const long V = (long)d.nx * d.ny * d.nz;
-    const double sr = stats[0], pm = stats[1], sp = stats[2];
+// network input from the solver's FP64 state (interior layout, voxel = cell):
+// ch0 = div(u*) / s_r, ch1 = fac * Phi_prev / (s_r c0) (0 without history or before c0 is known)
+__global__ void make_input_kernel(long V, const double *__restrict__ rhs, const double *__restrict__ phi_prev,
+                                  double fac, double c0, const double *__restrict__ stats, bf16 *__restrict__ x) {
+    const double sr = stats[0];
+    const double sp = (phi_prev && c0 > 0.0) ? fac / (sr * c0) : 0.0;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(v % d.nx) + 1, j = (int)((v / d.nx) % d.ny) + 1, k = (int)(v / ((long)d.nx * d.ny)) + 1;
         bf16x8 a = {}, b = {};
         a[0] = (bf16)(float)(rhs[v] / sr);
-        a[1] = (bf16)(float)((pn[d.g(i, j, k)] - pm) / sp);
+        a[1] = (bf16)(float)(phi_prev ? phi_prev[v] * sp : 0.0);
         *reinterpret_cast<bf16x8 *>(x + v * 16) = a;
         *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = b;
     }
 }
 
-// sums over interior cells for the input scaling: [sum r^2, sum p, sum p^2] (deterministic, 2 pass)
-__global__ __launch_bounds__(256) void input_stats_partial(Dims d, const double *__restrict__ rhs,
-                                                           const double *__restrict__ pn, double *part) {
-    __shared__ double s[3][256];
-    double a = 0, b = 0, c = 0;
-    const long V = (long)d.nx * d.ny * d.nz;
-    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long)gridDim.x * 256) {
-        const int i = (int)(v % d.nx) + 1, j = (int)((v / d.nx) % d.ny) + 1, k = (int)(v / ((long)d.nx * d.ny)) + 1;
-        const double r = rhs[v], p = pn[d.g(i, j, k)];
-        a += r * r;
-        b += p;
-        c += p * p;
-    }
-    s[0][threadIdx.x] = a;
-    s[1][threadIdx.x] = b;
-    s[2][threadIdx.x] = c;
+// root-mean-square scaling of an interior field: partial sums of squares (deterministic, 2 pass)
+__global__ __launch_bounds__(256) void input_stats_partial(long V, const double *__restrict__ r, double *part) {
+    __shared__ double s[256];
+    double a = 0;
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long)gridDim.x * 256) a += r[v] * r[v];
+    s[threadIdx.x] = a;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int q = 0; q < 3; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + w];
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 3; ++q) part[3 * blockIdx.x + q] = s[q][0];
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
 __global__ void input_stats_final(const double *part, int nb, double V, double *stats) {
     if (threadIdx.x != 0) return;
-    double a = 0, b = 0, c = 0;
-    for (int q = 0; q < nb; ++q) {
-        a += part[3 * q];
-        b += part[3 * q + 1];
-        c += part[3 * q + 2];
-    }
-    const double sr = sqrt(a / V), pm = b / V, var = c / V - pm * pm;
+    double a = 0;
+    for (int q = 0; q < nb; ++q) a += part[q];
+    const double sr = sqrt(a / V);
     stats[0] = sr > 1e-300 ? sr : 1.0;
-    stats[1] = pm;
-    stats[2] = var > 1e-300 ? sqrt(var) : 1.0;
 }
 
-// Phi guess (FP64, interior layout) = out * s_r * c0
+// Phi guess (FP64, interior layout) = fac * Phi_prev + out * s_r * c0
 __global__ void guess_out_kernel(const float *__restrict__ out, const double *__restrict__ stats, double c0,
-                                 double *__restrict__ phi, long V) {
+                                 const double *__restrict__ phi_prev, double fac, double *__restrict__ phi, long V) {
     const double s = stats[0] * c0;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
-        phi[v] = (double)out[v] * s;
+        phi[v] = (phi_prev ? fac * phi_prev[v] : 0.0) + (double)out[v] * s;
 }
 
-// training target = Phi / (s_r c0) in FP32
-__global__ void make_target_kernel(const double *__restrict__ phi, const double *__restrict__ stats, double c0,
-                                   float *__restrict__ t, long V) {
+// training target = (Phi - fac * Phi_prev) / (s_r c0) in FP32
+__global__ void make_target_kernel(const double *__restrict__ phi, const double *__restrict__ phi_prev, double fac,
+                                   const double *__restrict__ stats, double c0, float *__restrict__ t, long V) {
     const double s = 1.0 / (stats[0] * c0);
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
-        t[v] = (float)(phi[v] * s);
+        t[v] = (float)((phi[v] - (phi_prev ? fac * phi_prev[v] : 0.0)) * s);
 }
 
 // replay buffer: store / load the 2 used input channels and the target
@@ -595,28 +577,32 @@ hipError_t net_adam(mdx_net *n, float lr) {
 }
 
 // ------------------------------------------------------------------------------- solver coupling
-hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *rhs, const double *pn) {
-    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, h->d, rhs, pn, n->part);
-    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)n->V[0], n->stats);
-    hipLaunchKernelGGL(make_input_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, h->d, rhs, pn, n->stats, n->x_in);
+hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *rhs, const double *phi_prev,
+                                 double fac) {
+    (void)h;
+    const long V = n->V[0];
+    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, V, rhs, n->part);
+    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)V, n->stats);
+    hipLaunchKernelGGL(make_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, V, rhs, phi_prev, fac, n->c0,
+                       n->stats, n->x_in);
     return hipGetLastError();
 }
 
-hipError_t net_guess_to_phi(mdx_net *n, double *phi) {
+hipError_t net_guess_to_phi(mdx_net *n, const double *phi_prev, double fac, double *phi) {
     hipError_t e;
     TRY(net_forward(n));
-    hipLaunchKernelGGL(guess_out_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->stats, n->c0, phi,
-                       n->V[0]);
+    hipLaunchKernelGGL(guess_out_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->stats, n->c0,
+                       phi_prev, fac, phi, n->V[0]);
     return hipGetLastError();
 }
 
-// append (current input, Phi) to the replay ring buffer
-hipError_t net_capture(mdx_net *n, const double *phi) {
+// append (current input, target of the converged Phi) to the replay ring buffer
+hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, double fac) {
     const long V = n->V[0];
     const int slot = n->next;
     hipLaunchKernelGGL(pack2_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, n->x_in, n->rb_x + (size_t)slot * 2 * V, V);
-    hipLaunchKernelGGL(make_target_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, phi, n->stats, n->c0,
-                       n->rb_t + (size_t)slot * V, V);
+    hipLaunchKernelGGL(make_target_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, phi, phi_prev, fac, n->stats,
+                       n->c0, n->rb_t + (size_t)slot * V, V);
     n->next = (n->next + 1) % n->cap;
     if (n->count < n->cap) n->count += 1;
     return hipGetLastError();
@@ -633,9 +619,11 @@ hipError_t net_mse_only(mdx_net *n) {
 
 // root mean square of an interior FP64 field (deterministic), into stats[3]
 hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi) {
-    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, h->d, phi, h->pn, n->part);
-    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)n->V[0], n->stats + 4);
-    return hipMemcpyAsync(n->stats + 3, n->stats + 4, 8, hipMemcpyDeviceToDevice, n->stream);
+    (void)h;
+    const long V = n->V[0];
+    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, V, phi, n->part);
+    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)V, n->stats + 3);
+    return hipGetLastError();
 }
 
 hipError_t net_load_replay(mdx_net *n, int slot) {

@@ -228,7 +228,8 @@ def test_driver_tutorial_setup_matches_oracle(tmp_path):
     got = _read_dump(tmp_path / "final.bin")
 
     g = O.Grid(32, 24, 16, lx=1.0, ly=1.0, lz=1.0, re=100.0, forcing=(1.0, 0.0, 0.0), periodic=(1, 0, 1))
-    coef = O.ibm_coefs(g, O.Geometry(kind=0))
+    # init_ibm (ibm.f90:47): bump amplitude 5 dy
+    coef = O.ibm_coefs(g, O.Geometry(kind=0, amp_x=5 * (1.0 / 24)))
     f = O.new_fields(g)
     for _ in range(3):
         O.rk3_step(g, f, coef, 1e-4, solver=1, niter=3, omega=1.5)
