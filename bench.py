@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--inner", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=200)
     ap.add_argument("--no-learned", action="store_true", help="headline without the learned guess")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device once (the known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -103,6 +106,16 @@ def main():
         sys.exit(2)
 
     import mobydiff_amd as M
+
+    if args.calibration_copy_mib > 0:
+        import torch
+        n = args.calibration_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     geo = M.wavy_geometry(A=A)
     head = make_channel(M, M.PREC_MIXED, args.check_every, args.inner)

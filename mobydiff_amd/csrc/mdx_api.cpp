@@ -193,7 +193,7 @@ static void free_all(mdx_handle *h) {
                     h->cu, h->cv, h->cw, h->d_ihc, h->d_ihv, h->d_yc, h->phi, h->phi2, h->rhs,
                     h->e32, h->r32, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
-                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags, h->phi_hist};
+                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags, h->phi_hist, h->res64};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -287,7 +287,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if ((e = dalloc(&h->red_max, 4)) != hipSuccess || (e = hipMemset(h->red_max, 0, 32)) != hipSuccess)
         return fail(e, "reduction buffer");
     h->rb_err = reinterpret_cast<int *>(h->red_max + 1);
-    if (const char *nl = std::getenv("MOBYDIFF_NO_LDS_SOR")) h->use_lds_sor = !(nl[0] == '1');
+    if (const char *ls = std::getenv("MOBYDIFF_LDS_SOR")) h->use_lds_sor = ls[0] == '1';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
         const int v = std::atoi(kz);
         if (!(v == 0 || v == 1 || v == 2 || v == 4 || v == 8)) {
@@ -308,7 +308,8 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             if ((e = dalloc(&h->e32, h->ni)) != hipSuccess || (e = dalloc(&h->r32, h->ni)) != hipSuccess)
                 return fail(e, "SOR FP32 buffers");
         if ((e = hipMemset(h->phi, 0, h->ni * 8)) != hipSuccess) return fail(e, "SOR buffers");
-        if ((e = dalloc(&h->phi_hist, 3 * h->ni)) != hipSuccess) return fail(e, "SOR guess history");
+        if ((e = dalloc(&h->phi_hist, 3 * h->ni)) != hipSuccess || (e = dalloc(&h->res64, h->ni)) != hipSuccess)
+            return fail(e, "SOR guess history");
     }
     if (cfg->solver == MDX_SOLVER_FFT) {
         const int nxh = nx / 2 + 1;
@@ -585,7 +586,11 @@ static int read_max(mdx_handle *h, double *out) {
 static int rb_prepare(mdx_handle *h) {
     if (h->rb_state != 0) return MDX_OK;
     h->rb_state = -1;
-    if (!h->use_lds_sor) { h->rb_note = "streaming sweeps: disabled by MOBYDIFF_NO_LDS_SOR"; return MDX_OK; }
+    if (!h->use_lds_sor) {
+        h->rb_note = "streaming sweeps: z-marching pairs, " + std::to_string(h->sweep_kz) + " planes per thread";
+        if (h->sweep_kz == 0) h->rb_note = "streaming sweeps: one cell per thread";
+        return MDX_OK;
+    }
     if (h->stretched) { h->rb_note = "streaming sweeps: stretched grid"; return MDX_OK; }
     if (h->ibm_on && !h->coef_plane) { h->rb_note = "streaming sweeps: IBM coefficients vary in z"; return MDX_OK; }
     RbPlanOut p64{}, p32{};
@@ -774,57 +779,84 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
 }
 
 // Initial guess of the SOR solve for stage s; the solve itself, and its stopping rule, are the
-// same whatever starts it.  The plain method starts from Phi = 0 (the previous
-// pressure).  MDX_GUESS_LEARNED evaluates the network on (div u*, Phi of stage s at the previous
-// step) and MDX_GUESS_PHI_PREV takes that previous Phi alone (scaled by the dt_gamma ratio); either
-// candidate is kept only when its FP64 residual max-norm is below that of Phi = 0, otherwise the
-// solve falls back to Phi = 0 (stats.fallback).  With capture on, the network input is built for
-// every solve so the converged Phi can be stored as a training pair.
+// same whatever starts it.  Candidates, each judged by the max-norm of its FP64 residual:
+//   Phi = 0 (the previous pressure: the plain method);
+//   Phi_0 = fac * Phi of stage s at the previous step (MDX_GUESS_PHI_PREV and MDX_GUESS_LEARNED);
+//   Phi_0 + the network's correction, predicted from (rhs - L_mu Phi_0, Phi_0) (MDX_GUESS_LEARNED).
+// The smallest residual starts the solve.  In learned mode a start other than the network's is a
+// fallback (stats.fallback).  With capture on, the network input is built for every solve so the
+// converged Phi can be stored as a training pair.
 static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, double *r_first) {
     mdx_net *n = h->net;
     const bool have_prev = h->hist_dtg[s] > 0.0;
     const double *prev = have_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
     const double fac = have_prev ? dtg / h->hist_dtg[s] : 0.0;
-    const bool learned = n && h->cfg.guess == MDX_GUESS_LEARNED && n->c0 > 0.0;
-    const bool phi_prev = h->cfg.guess == MDX_GUESS_PHI_PREV && have_prev;
-    if (n && (learned || h->net_capture)) {
+    const int mode = h->cfg.guess;
+    const bool learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0;
+    const bool use_prev = have_prev && (mode == MDX_GUESS_PHI_PREV || mode == MDX_GUESS_LEARNED);
+    const bool build_input = n && (learned || h->net_capture);
+    if (!learned && !use_prev && !build_input) return MDX_OK;
+    int rc;
+    ev_record(h, 8);
+    // Phi_0 in phi2 and, for the network, its residual field
+    if (have_prev) MDX_HIP(launch_scale_copy64(h, h->phi2, prev, fac, h->ni));
+    double rp = -1.0;
+    if (build_input || use_prev) {
+        MDX_HIP(launch_reset_max(h));
+        MDX_HIP(launch_residual_field64(h, have_prev ? h->phi2 : h->phi, build_input ? h->res64 : nullptr));
+        if ((rc = read_max(h, &rp))) return rc;
+    }
+    if (build_input) {
         hipStream_t saved = n->stream;
         n->stream = h->stream;
-        ev_record(h, 8);
-        hipError_t e = unet::net_input_from_solver(n, h, h->rhs, prev, fac);
-        if (e == hipSuccess && learned) e = unet::net_guess_to_phi(n, prev, fac, h->phi2);
-        ev_record(h, 9);
+        hipError_t e = unet::net_input_from_solver(n, h, h->res64, have_prev ? h->phi2 : nullptr);
+        // the learned guess is formed in phi2 (in place over Phi_0, or from zero)
+        if (e == hipSuccess && learned) e = unet::net_guess_to_phi(n, have_prev ? h->phi2 : nullptr, h->phi2);
         n->stream = saved;
         MDX_HIP(e);
-        if (h->timing && learned) {
-            float ms = 0.f;
-            MDX_HIP(hipEventSynchronize(h->ev[9]));
-            if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
-        }
     }
-    if (!learned && !phi_prev) return MDX_OK;
-    if (phi_prev) MDX_HIP(launch_scale_copy64(h, h->phi2, prev, fac, h->ni));
-    double r0 = 0.0, rg = 0.0;
-    int rc;
-    MDX_HIP(launch_reset_max(h));
-    MDX_HIP(launch_sor_residual64(h, h->phi2));
-    if ((rc = read_max(h, &rg))) return rc;
-    MDX_HIP(launch_reset_max(h));
-    MDX_HIP(launch_sor_residual64(h, h->phi));   // Phi = 0: the previous-step guess
-    if ((rc = read_max(h, &r0))) return rc;
-    st->guess_div_inf = rg;
-    if (rg < r0) {
-        std::swap(h->phi, h->phi2);
-        *r_first = rg;
+    double rg = -1.0;
+    if (learned) {
+        MDX_HIP(launch_reset_max(h));
+        MDX_HIP(launch_sor_residual64(h, h->phi2));
+        if ((rc = read_max(h, &rg))) return rc;
+    }
+    ev_record(h, 9);
+    if (h->timing && learned) {
+        float ms = 0.f;
+        MDX_HIP(hipEventSynchronize(h->ev[9]));
+        if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
+    }
+    if (!learned && !use_prev) return MDX_OK;   // capture only: the plain start
+    double r0 = 0.0;
+    if (have_prev) {
+        MDX_HIP(launch_reset_max(h));
+        MDX_HIP(launch_sor_residual64(h, h->phi));   // Phi = 0
+        if ((rc = read_max(h, &r0))) return rc;
     } else {
-        st->fallback = 1;
+        r0 = rp;                                     // Phi_0 = 0 was evaluated above
+    }
+    st->guess_div_inf = learned ? rg : rp;
+    // the chosen start is copied into phi (captured sweep graphs hold that buffer's address)
+    if (learned && rg < r0 && (!use_prev || rg < rp)) {
+        MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
+        *r_first = rg;
+        return MDX_OK;
+    }
+    if (learned) st->fallback = 1;
+    if (use_prev && rp < r0) {
+        if (learned) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));   // phi2 holds the network's
+        else MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
+        *r_first = rp;
+    } else {
+        if (!learned) st->fallback = 1;
         *r_first = r0;
     }
     return MDX_OK;
 }
 
 // Append (network input, converged Phi) of stage s to the attached network's replay buffer.  The
-// target scale c0 = rms(Phi) / rms(div u*) is fixed by the first captured pair.
+// target scale c0 = rms(Phi - Phi_0) / rms(r0) is fixed by the first captured pair.
 static int capture_pair(mdx_handle *h, int s, double dtg) {
     mdx_net *n = h->net;
     const bool have_prev = h->hist_dtg[s] > 0.0;
@@ -834,12 +866,12 @@ static int capture_pair(mdx_handle *h, int s, double dtg) {
     n->stream = h->stream;
     hipError_t e = hipSuccess;
     if (n->c0 <= 0.0) {
-        e = unet::net_rms_to_stats3(n, h, h->phi);
+        e = unet::net_rms_to_stats3(n, h, h->phi, prev, fac);
         if (e == hipSuccess) e = hipMemcpyAsync(h->h_pinned + 8, n->stats, 4 * 8, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e == hipSuccess) {
-            const double sr = h->h_pinned[8], sphi = h->h_pinned[11];
-            n->c0 = (sphi > 0.0 && sr > 0.0) ? sphi / sr : 1.0;
+            const double sr = h->h_pinned[8], se = h->h_pinned[11];
+            n->c0 = (se > 0.0 && sr > 0.0) ? se / sr : 1.0;
         }
     }
     if (e == hipSuccess) e = unet::net_capture(n, h->phi, prev, fac);

@@ -155,8 +155,9 @@ struct mdx_handle {
     // streaming red-black sweeps: z planes marched per thread (0: one cell per thread)
     int sweep_kz = 4;
 
-    // LDS-resident red-black solver: plans per precision, face buffers, progress counters
-    bool use_lds_sor = true;
+    // LDS-resident red-black solver (opt-in, MOBYDIFF_LDS_SOR=1): plans per precision, face buffers,
+    // progress counters
+    bool use_lds_sor = false;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
     mdx::RbPlanOut rb_plan64{}, rb_plan32{};
     void *rb_faces = nullptr;
@@ -171,6 +172,7 @@ struct mdx_handle {
     // converged Phi of each RK stage at the previous step (initial-guess history), its dt_gamma
     // (0: no history yet) and the count of SOR projections (stage = count mod 3)
     double *phi_hist = nullptr;
+    double *res64 = nullptr;       // residual field of the previous-Phi guess (network input)
     double hist_dtg[3] = {0.0, 0.0, 0.0};
     long proj_count = 0;
 };
@@ -205,6 +207,7 @@ hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega);
 hipError_t launch_zero32(mdx_handle *h, float *p, size_t n);
 hipError_t launch_zero64(mdx_handle *h, double *p, size_t n);
 hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, double fac, size_t n);
+hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *field);
 bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init);
 

@@ -456,7 +456,8 @@ hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega) {
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(256) void sor_residual64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
                                                              const double *__restrict__ rhs,
-                                                             const double *__restrict__ phi, double *out) {
+                                                             const double *__restrict__ phi, double *out,
+                                                             double *__restrict__ field) {
     const Dims d = g.d;
     int tx_ = 0, ty_ = 0, tz_ = 0;
     const bool ok = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);   // padding groups report 0
@@ -467,24 +468,29 @@ __global__ __launch_bounds__(256) void sor_residual64_kernel(GridK g, MuK<double
     if (ok && i <= d.nx && j <= d.ny) {
         PlainPhi<double> P{phi, d};
         double denom;
-        m = fabs(cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[d.in(i, j, k)], i, j, k, denom));
+        const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[d.in(i, j, k)], i, j, k, denom);
+        if (field) field[d.in(i, j, k)] = r;
+        m = fabs(r);
     }
     block_max_part(m, out);
 }
 
-hipError_t launch_sor_residual64(mdx_handle *h, const double *phi) {
+// max |rhs - L_mu(phi)| into red_max; with field != nullptr the residual itself is stored too
+hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *field) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
     dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
 #define MDX_RS(S, I) \
-    hipLaunchKernelGGL((sor_residual64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi, h->max_part)
+    hipLaunchKernelGGL((sor_residual64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi, h->max_part, field)
     if (h->stretched) { if (h->ibm_on) MDX_RS(true, true); else MDX_RS(true, false); }
     else { if (h->ibm_on) MDX_RS(false, true); else MDX_RS(false, false); }
 #undef MDX_RS
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_max_final(h, grd.x);
 }
+
+hipError_t launch_sor_residual64(mdx_handle *h, const double *phi) { return launch_residual_field64(h, phi, nullptr); }
 
 // ------------------------------------------------------------------------------- mixed precision
 // One FP64 defect-correction step: Phi_out = Phi_in + e; r = rhs - L_mu(Phi_out) in FP64;

@@ -30,6 +30,7 @@ struct mdx_net {
     uint64_t sample_state = 0;
     // inference timing
     hipEvent_t ev[2] = {nullptr, nullptr};
+    bool tiled_conv = true;        // LDS-tiled convolutions where the grid allows (bit-identical)
 };
 
 namespace mdx {
@@ -51,16 +52,17 @@ hipError_t upload_input2(mdx_net *n, const float *x2);
 hipError_t net_forward(mdx_net *n);
 hipError_t net_backward(mdx_net *n, float scale);
 hipError_t net_adam(mdx_net *n, float lr);
-// solver coupling.  Inputs: ch0 = div(u*) / s_r, ch1 = fac * Phi_prev / (s_r c0), where Phi_prev
-// is the converged correction of the same RK stage at the previous step (nullptr: none yet) and
-// fac = dt_gamma / dt_gamma_prev.  The network predicts the change: Phi = fac * Phi_prev +
-// out * s_r * c0, so its training target is (Phi - fac * Phi_prev) / (s_r c0).
-hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *rhs, const double *phi_prev,
-                                 double fac);
-hipError_t net_guess_to_phi(mdx_net *n, const double *phi_prev, double fac, double *phi);
+// solver coupling.  The solve starts from Phi_0 = fac * Phi_prev (the converged correction of the
+// same RK stage at the previous step, fac = dt_gamma / dt_gamma_prev; Phi_0 = 0 without history).
+// Inputs: ch0 = r0 / s_r with r0 = rhs - L_mu(Phi_0) computed in FP64 by the solver, ch1 = Phi_0 /
+// s_p.  The network predicts the remaining correction: Phi = Phi_0 + out * s_r * c0, so the
+// training target is (Phi - Phi_0) / (s_r c0).
+hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0);
+hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi);
 hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, double fac);
 hipError_t net_load_replay(mdx_net *n, int slot);
 hipError_t net_mse_only(mdx_net *n);
-hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi);
+hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi, const double *phi_prev,
+                             double fac);
 }  // namespace unet
 }  // namespace mdx

@@ -74,6 +74,77 @@ __global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__
     }
 }
 
+// ------------------------------------------------------------------------------- conv forward, LDS-tiled
+// Same GEMM, same weight fragments and the same K order and chunking as conv3_fwd_kernel (so the
+// outputs are bit-identical), but a workgroup computes a 16 x 4 x 4 output tile and first stages
+// its 18 x 6 x 6 input halo (all CIN channels) in LDS: every input voxel is fetched from global
+// memory about 2.5 times instead of once per tap (27), and each wave reuses one weight fragment
+// for 4 output rows.  Wave w owns output row y = y0 + w, its 4 M-tiles are z = z0 .. z0+3, and a
+// tile row is 16 consecutive x voxels.  Needs W % 16 == 0, H % 4 == 0, D % 4 == 0.
+static constexpr int CTX = 16, CTY = 4, CTZ = 4;
+static constexpr int CHX = CTX + 2, CHY = CTY + 2, CHZ = CTZ + 2, CHV = CHX * CHY * CHZ;
+
+template <int CIN, int COUT, bool RELU>
+__global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict__ in, const bf16x8 *__restrict__ wp,
+                                                          const float *__restrict__ bias, bf16 *__restrict__ out,
+                                                          int W, int H, int D) {
+    constexpr int K = 27 * CIN, KC = (K + 31) / 32, NT = COUT / 16, U = CIN / 8;
+    __shared__ __attribute__((aligned(16))) bf16 tile[CHV * CIN];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ntx = W / CTX, nty = H / CTY;
+    const int x0 = (int)(blockIdx.x % ntx) * CTX, y0 = (int)((blockIdx.x / ntx) % nty) * CTY;
+    const int z0 = (int)(blockIdx.x / (ntx * nty)) * CTZ;
+    for (int q = threadIdx.x; q < CHV * U; q += 256) {
+        const int v = q / U, u = q - v * U;
+        const int hx = v % CHX, hy = (v / CHX) % CHY, hz = v / (CHX * CHY);
+        const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+        bf16x8 a = {};
+        if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D)
+            a = *reinterpret_cast<const bf16x8 *>(in + (((long)z * H + y) * W + x) * CIN + u * 8);
+        *reinterpret_cast<bf16x8 *>(&tile[v * CIN + u * 8]) = a;
+    }
+    __syncthreads();
+    const int r = lane & 15, g = lane >> 4;
+    f32x4 acc[CTZ][NT];
+#pragma unroll
+    for (int m = 0; m < CTZ; ++m)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[m][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero8 = {};
+    for (int kc = 0; kc < KC; ++kc) {
+        const int k0 = kc * 32 + g * 8;
+        const bool valid = k0 < K;
+        const int tap = valid ? k0 / CIN : 0, c0 = k0 - tap * CIN;
+        const int base = ((tap / 9) * CHY + wave + (tap / 3) % 3) * CHX + r + tap % 3;
+        bf16x8 b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) b[nt] = wp[((long)kc * NT + nt) * 64 + lane];
+#pragma unroll
+        for (int m = 0; m < CTZ; ++m) {
+            const bf16x8 a = valid ? *reinterpret_cast<const bf16x8 *>(&tile[(base + m * CHX * CHY) * CIN + c0]) : zero8;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[m][nt], 0, 0, 0);
+        }
+    }
+    const int y = y0 + wave;
+#pragma unroll
+    for (int m = 0; m < CTZ; ++m) {
+        const long row = ((long)(z0 + m) * H + y) * W + x0;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int ch = nt * 16 + (lane & 15);
+            const float bb = bias ? bias[ch] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float val = acc[m][nt][i] + bb;
+                if (RELU) val = val > 0.f ? val : 0.f;
+                out[(row + 4 * g + i) * COUT + ch] = (bf16)val;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------- weight gradient
 // One workgroup = one tap x one slice of voxels.  Each 32-voxel chunk: dY (32 x COUT) and the
 // tap-shifted input (32 x CIN) are staged transposed in LDS ([c][v], so 8 consecutive voxels of one
@@ -330,26 +401,31 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
 
 // ------------------------------------------------------------------------------- solver coupling
 // network input from the solver's FP64 state (interior layout, voxel = cell):
-// ch0 = div(u*) / s_r, ch1 = fac * Phi_prev / (s_r c0) (0 without history or before c0 is known)
-__global__ void make_input_kernel(long V, const double *__restrict__ rhs, const double *__restrict__ phi_prev,
-                                  double fac, double c0, const double *__restrict__ stats, bf16 *__restrict__ x) {
-    const double sr = stats[0];
-    const double sp = (phi_prev && c0 > 0.0) ? fac / (sr * c0) : 0.0;
+// ch0 = r0 / s_r, the residual rhs - L_mu(Phi_0) of the previous-step start Phi_0 = fac * Phi_prev;
+// ch1 = Phi_0 / s_p (0 without history).  s_r = rms(r0) = stats[0], s_p = rms(Phi_0) = stats[1].
+__global__ void make_input_kernel(long V, const double *__restrict__ r0, const double *__restrict__ phi0,
+                                  const double *__restrict__ stats, bf16 *__restrict__ x) {
+    const double ir = 1.0 / stats[0], ip = phi0 ? 1.0 / stats[1] : 0.0;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
         bf16x8 a = {}, b = {};
-        a[0] = (bf16)(float)(rhs[v] / sr);
-        a[1] = (bf16)(float)(phi_prev ? phi_prev[v] * sp : 0.0);
+        a[0] = (bf16)(float)(r0[v] * ir);
+        a[1] = (bf16)(float)(phi0 ? phi0[v] * ip : 0.0);
         *reinterpret_cast<bf16x8 *>(x + v * 16) = a;
         *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = b;
     }
 }
 
-// root-mean-square scaling of an interior field: partial sums of squares (deterministic, 2 pass)
-__global__ __launch_bounds__(256) void input_stats_partial(long V, const double *__restrict__ r, double *part) {
+// root mean square of a - fac * b over interior cells (b may be null): partial sums of squares,
+// then one final pass (deterministic order)
+__global__ __launch_bounds__(256) void input_stats_partial(long V, const double *__restrict__ a,
+                                                           const double *__restrict__ b, double fac, double *part) {
     __shared__ double s[256];
-    double a = 0;
-    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long)gridDim.x * 256) a += r[v] * r[v];
-    s[threadIdx.x] = a;
+    double acc = 0;
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long)gridDim.x * 256) {
+        const double t = a[v] - (b ? fac * b[v] : 0.0);
+        acc += t * t;
+    }
+    s[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
@@ -358,20 +434,20 @@ __global__ __launch_bounds__(256) void input_stats_partial(long V, const double 
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-__global__ void input_stats_final(const double *part, int nb, double V, double *stats) {
+__global__ void input_stats_final(const double *part, int nb, double V, double *stat) {
     if (threadIdx.x != 0) return;
     double a = 0;
     for (int q = 0; q < nb; ++q) a += part[q];
-    const double sr = sqrt(a / V);
-    stats[0] = sr > 1e-300 ? sr : 1.0;
+    const double r = sqrt(a / V);
+    stat[0] = r > 1e-300 ? r : 1.0;
 }
 
-// Phi guess (FP64, interior layout) = fac * Phi_prev + out * s_r * c0
+// Phi guess (FP64, interior layout) = Phi_0 + out * s_r * c0 (in place when phi == phi0)
 __global__ void guess_out_kernel(const float *__restrict__ out, const double *__restrict__ stats, double c0,
-                                 const double *__restrict__ phi_prev, double fac, double *__restrict__ phi, long V) {
+                                 const double *phi0, double *phi, long V) {
     const double s = stats[0] * c0;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
-        phi[v] = (phi_prev ? fac * phi_prev[v] : 0.0) + (double)out[v] * s;
+        phi[v] = (phi0 ? phi0[v] : 0.0) + (double)out[v] * s;
 }
 
 // training target = (Phi - fac * Phi_prev) / (s_r c0) in FP32
@@ -424,6 +500,18 @@ static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in
                         bf16 *out, int lvl) {
     const int W = n->W[lvl], H = n->H[lvl], D = n->D[lvl];
     const unsigned blocks = (unsigned)(n->V[lvl] / 64);
+    const bool tiled = n->tiled_conv && W % CTX == 0 && H % CTY == 0 && D % CTZ == 0;
+    const unsigned tiles = (unsigned)((W / CTX) * (H / CTY) * (D / CTZ));
+#define TILED_CASE(CI, CO, R)                                                                                      \
+    if (tiled && cin == CI && cout == CO && relu == R) {                                                           \
+        hipLaunchKernelGGL((conv3_tiled_kernel<CI, CO, R>), dim3(tiles), dim3(256), 0, n->stream, in,               \
+                           reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);                              \
+        return hipGetLastError();                                                                                  \
+    }
+    TILED_CASE(16, 16, true) TILED_CASE(16, 32, true) TILED_CASE(32, 32, true) TILED_CASE(32, 64, true)
+    TILED_CASE(48, 16, true) TILED_CASE(16, 16, false) TILED_CASE(32, 16, false) TILED_CASE(32, 32, false)
+    TILED_CASE(32, 96, false) TILED_CASE(16, 48, false)
+#undef TILED_CASE
 #define CONV_CASE(CI, CO, R)                                                                                       \
     if (cin == CI && cout == CO && relu == R) {                                                                    \
         hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, R>), dim3(blocks), dim3(256), 0, n->stream, in,                \
@@ -577,22 +665,28 @@ hipError_t net_adam(mdx_net *n, float lr) {
 }
 
 // ------------------------------------------------------------------------------- solver coupling
-hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *rhs, const double *phi_prev,
-                                 double fac) {
-    (void)h;
+static hipError_t rms_into(mdx_net *n, const double *a, const double *b, double fac, double *stat) {
     const long V = n->V[0];
-    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, V, rhs, n->part);
-    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)V, n->stats);
-    hipLaunchKernelGGL(make_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, V, rhs, phi_prev, fac, n->c0,
-                       n->stats, n->x_in);
+    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, V, a, b, fac, n->part);
+    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)V, stat);
     return hipGetLastError();
 }
 
-hipError_t net_guess_to_phi(mdx_net *n, const double *phi_prev, double fac, double *phi) {
+hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0) {
+    (void)h;
+    hipError_t e;
+    const long V = n->V[0];
+    TRY(rms_into(n, r0, nullptr, 0.0, n->stats));
+    if (phi0) TRY(rms_into(n, phi0, nullptr, 0.0, n->stats + 1));
+    hipLaunchKernelGGL(make_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, V, r0, phi0, n->stats, n->x_in);
+    return hipGetLastError();
+}
+
+hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi) {
     hipError_t e;
     TRY(net_forward(n));
     hipLaunchKernelGGL(guess_out_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->stats, n->c0,
-                       phi_prev, fac, phi, n->V[0]);
+                       phi0, phi, n->V[0]);
     return hipGetLastError();
 }
 
@@ -618,12 +712,10 @@ hipError_t net_mse_only(mdx_net *n) {
 }
 
 // root mean square of an interior FP64 field (deterministic), into stats[3]
-hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi) {
+// rms of the first training target's numerator, Phi - fac * Phi_prev, into stats[3]
+hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi, const double *phi_prev, double fac) {
     (void)h;
-    const long V = n->V[0];
-    hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, V, phi, n->part);
-    hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)V, n->stats + 3);
-    return hipGetLastError();
+    return rms_into(n, phi, phi_prev, fac, n->stats + 3);
 }
 
 hipError_t net_load_replay(mdx_net *n, int slot) {

@@ -2,6 +2,7 @@
 // U-Net (kernels in mdx_unet.hip).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -52,6 +53,7 @@ extern "C" int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed,
               "mdx_net_create: needs nx a multiple of 64, ny and nz multiples of 4, and (nx ny nz / 64) a "
               "multiple of 64");
     mdx_net *n = new mdx_net();
+    if (const char *t = std::getenv("MOBYDIFF_NO_TILED_CONV")) n->tiled_conv = !(t[0] == '1');
     n->W[0] = nx; n->H[0] = ny; n->D[0] = nz;
     for (int q = 1; q < 3; ++q) { n->W[q] = n->W[q - 1] / 2; n->H[q] = n->H[q - 1] / 2; n->D[q] = n->D[q - 1] / 2; }
     for (int q = 0; q < 3; ++q) n->V[q] = (long)n->W[q] * n->H[q] * n->D[q];

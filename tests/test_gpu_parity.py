@@ -272,7 +272,7 @@ def _wavy(nx, ny, nz, per_y=0, ibm=True):
 @pytest.mark.parametrize("shape,per_y,ibm", [((48, 32, 24), 0, True), ((32, 16, 16), 1, False),
                                              ((192, 128, 96), 0, True)])
 def test_lds_sweeps_bit_exact_fixed_iterations(shape, per_y, ibm, monkeypatch):
-    monkeypatch.delenv("MOBYDIFF_NO_LDS_SOR", raising=False)
+    monkeypatch.setenv("MOBYDIFF_LDS_SOR", "1")
     g, coef = _wavy(*shape, per_y=per_y, ibm=ibm)
     rng = np.random.default_rng(21)
     f = O.random_fields(g, rng, amp=0.05)
@@ -291,7 +291,7 @@ def test_lds_sweeps_bit_exact_fixed_iterations(shape, per_y, ibm, monkeypatch):
 def test_zmarch_sweeps_bit_exact_fixed_iterations(kz, case, monkeypatch):
     """Streaming sweeps with a thread marching kz planes (nz = 22: the last segment is short)
     against the oracle, bit for bit, for z-invariant (plane) and general 3-D mu tables."""
-    monkeypatch.setenv("MOBYDIFF_NO_LDS_SOR", "1")
+    monkeypatch.setenv("MOBYDIFF_LDS_SOR", "0")
     monkeypatch.setenv("MOBYDIFF_SWEEP_KZ", str(kz))
     rng = np.random.default_rng(23)
     if case == "wavy":
@@ -322,7 +322,7 @@ def test_mixed_solve_same_for_sweep_kernels(kz, monkeypatch):
     g, coef = _wavy(48, 32, 24)
     rng = np.random.default_rng(24)
     f = O.random_fields(g, rng, amp=0.05)
-    monkeypatch.setenv("MOBYDIFF_NO_LDS_SOR", "1")
+    monkeypatch.setenv("MOBYDIFF_LDS_SOR", "0")
     outs = []
     for v in ("0", str(kz)):
         monkeypatch.setenv("MOBYDIFF_SWEEP_KZ", v)
@@ -342,8 +342,8 @@ def test_lds_sweeps_equal_streaming_sweeps(precision, monkeypatch):
     rng = np.random.default_rng(22)
     f = O.random_fields(g, rng, amp=0.05)
     outs, stats, infos = [], [], []
-    for off in ("0", "1"):
-        monkeypatch.setenv("MOBYDIFF_NO_LDS_SOR", off)
+    for on in ("1", "0"):
+        monkeypatch.setenv("MOBYDIFF_LDS_SOR", on)
         ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=10,
                       precision=precision, inner_iters=10)
         stats.append(ch.pressure_projection(5e-4 * 64 / 120))

@@ -63,6 +63,23 @@ def test_forward_matches_bf16_reference():
     assert _rel(out, ref32) < 5e-2
 
 
+def test_tiled_convolutions_match_direct_kernels(monkeypatch):
+    """The LDS-tiled convolutions use the same fragments, K order and accumulation sequence as the
+    direct kernels: forward outputs are bit-identical; gradients (FP32 atomics in the weight
+    gradient) agree to rounding."""
+    x2, target = _inputs(5)
+    outs, grads = [], []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MOBYDIFF_NO_TILED_CONV", flag)
+        net = M.Net(NX, NY, NZ, seed=11)
+        outs.append(net.forward(x2))
+        net.zero_grads()
+        net.accumulate_grad(x2, target)
+        grads.append(net.get_grads())
+    assert np.array_equal(outs[0], outs[1])
+    assert _rel(grads[1], grads[0]) < 1e-5
+
+
 def test_gradients_match_autograd():
     net = M.Net(NX, NY, NZ, seed=2)
     p, _ = net.get_params()
@@ -194,11 +211,21 @@ def test_previous_phi_guess_in_the_solver():
     ch.compute_ibm(geo)
     ch.init_fields(seed=0, amp=0.05, geo=geo, dt=dt)
     first = ch.step(dt)
+    # no history yet: the plain start, nothing evaluated
     assert all(first.stage[q].fallback == 0 and first.stage[q].guess_div_inf == 0.0 for q in range(3))
-    for _ in range(3):
+    kept = 0
+    for _ in range(4):
         st = ch.step(dt)
         for q in range(3):
             s = st.stage[q]
             assert s.converged == 1 and s.div_inf < 1e-8
-            assert s.fallback == 0 and 0.0 < s.guess_div_inf == s.div0_inf
+            assert s.guess_div_inf > 0.0
+            if s.fallback:
+                # rejected: the solve started from Phi = 0, whose residual was not larger
+                assert s.div0_inf <= s.guess_div_inf
+            else:
+                kept += 1
+                assert s.guess_div_inf == s.div0_inf
+    # after the start-up transient the previous step's Phi is the better start
+    assert kept >= 6, kept
     ch.close()
