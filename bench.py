@@ -68,6 +68,42 @@ def roofline(times, kind):
             "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": bytes_per_launch}
 
 
+def flat_fft_production(M, steps):
+    """Secondary line: the flat channel at production size, 256 x 192 x 256, tanh-stretched y
+    (beta 1.5), 4 pi x 2 x 2 pi, Re_b = 5600, constant flow rate, dt = 5e-4, FFT + tridiagonal
+    Poisson solve.  Canonical algorithmic traffic 768 B per cell-step (3 stages x 256 B)."""
+    import math
+    nx, ny, nz = 256, 192, 256
+    cfg = M.channel_config(nx, ny, nz, 4 * math.pi, 2.0, 2 * math.pi, 5600.0, solver=M.SOLVER_FFT,
+                           flowrate=True, ub_target=1.0, flow_height=2.0, stretch_beta=1.5, p_neumann_y=False,
+                           timing=True)
+    ch = M.Channel(cfg)
+    ch.init_fields(seed=0, amp=0.1, geo=None, dt=5e-4)
+    for _ in range(2):
+        ch.step(5e-4)
+    ch.synchronize()
+    ch.reset_phase_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ch.step(5e-4)
+    ch.synchronize()
+    el = time.perf_counter() - t0
+    t = ch.phase_times()
+    div = ch.div_norm()
+    ch.close()
+    cells = nx * ny * nz
+    gbs = 768.0 * cells * steps / el / 1e9
+    return {"workload": "flat channel 256x192x256 (beta=1.5, Re_b=5600, dt=5e-4), FFT+tridiagonal Poisson, FP64",
+            "value": round(cells * steps / el / 1e6, 3), "unit": "Mcell-steps/s", "steps": steps,
+            "ms_per_step": round(el / steps * 1e3, 3),
+            "phase_ms_per_step": {"stencil": round(t.stencil_ms / steps, 3), "poisson": round(t.poisson_ms / steps, 3),
+                                  "projection": round(t.projection_ms / steps, 3)},
+            "div_inf_final": div,
+            "step_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(gbs / HBM_PEAK_GBS, 4),
+                              "bytes_per_step": 768.0 * cells}}
+
+
 def cpu_baseline(f_host, coef, cores, steps):
     """The oracle (C restatement of the reference CPU path, FP64, OpenMP) on the same state."""
     from oracle import oracle as O
@@ -96,6 +132,7 @@ def main():
     ap.add_argument("--inner", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=200)
     ap.add_argument("--no-learned", action="store_true", help="headline without the learned guess")
+    ap.add_argument("--flat-steps", type=int, default=10, help="steps of the flat FFT production line (0: skip)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once (the known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -204,6 +241,8 @@ def main():
         "speedup_vs_plain_fp64": round(v_h / v_p, 3),
         "roofline": roofline(t_h, "fp32"),
     }
+    if args.flat_steps > 0:
+        out["flat_fft_production"] = flat_fft_production(M, args.flat_steps)
     if not args.no_cpu_baseline:
         aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         cores = int(os.environ.get("OMP_NUM_THREADS", aff))   # the oracle's OpenMP thread count

@@ -87,6 +87,8 @@ typedef struct {
     double div_inf;          /* ||div||_inf after the solve (from the final residual) */
     double guess_div_inf;    /* ||div||_inf of the learned guess (when used) */
     int32_t lds_sweeps;      /* 1: sweeps ran in the LDS-resident kernel, 0: streaming kernels */
+    int32_t guess_skipped;   /* learned mode: the network was not evaluated (its start lost the
+                                previous solves of this stage; it is re-tried after 2..16 solves) */
 } mdx_solve_stats;
 
 typedef struct {

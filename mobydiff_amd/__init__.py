@@ -44,7 +44,7 @@ class Geometry(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("fallback", C.c_int32),
                 ("checks", C.c_int32), ("div0_inf", C.c_double), ("div_inf", C.c_double),
-                ("guess_div_inf", C.c_double), ("lds_sweeps", C.c_int32)]
+                ("guess_div_inf", C.c_double), ("lds_sweeps", C.c_int32), ("guess_skipped", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -288,6 +288,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         return fail(e, "reduction buffer");
     h->rb_err = reinterpret_cast<int *>(h->red_max + 1);
     if (const char *ls = std::getenv("MOBYDIFF_LDS_SOR")) h->use_lds_sor = ls[0] == '1';
+    if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
         const int v = std::atoi(kz);
         if (!(v == 0 || v == 1 || v == 2 || v == 4 || v == 8)) {
@@ -792,7 +793,15 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
     const double *prev = have_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
     const double fac = have_prev ? dtg / h->hist_dtg[s] : 0.0;
     const int mode = h->cfg.guess;
-    const bool learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0;
+    bool learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0;
+    // gate: after the network's start has lost twice in a row for this stage, it is evaluated
+    // again only after 2, 4, ..., at most 16 skipped solves (the previous-Phi start is used then)
+    if (learned && h->net_skip[s] > 0) {
+        h->net_skip[s] -= 1;
+        learned = false;
+        st->guess_skipped = 1;
+        st->fallback = 1;
+    }
     const bool use_prev = have_prev && (mode == MDX_GUESS_PHI_PREV || mode == MDX_GUESS_LEARNED);
     const bool build_input = n && (learned || h->net_capture);
     if (!learned && !use_prev && !build_input) return MDX_OK;
@@ -841,9 +850,14 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
     if (learned && rg < r0 && (!use_prev || rg < rp)) {
         MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
         *r_first = rg;
+        h->net_losses[s] = 0;
         return MDX_OK;
     }
-    if (learned) st->fallback = 1;
+    if (learned) {
+        st->fallback = 1;
+        h->net_losses[s] += 1;
+        if (h->net_losses[s] >= 2) h->net_skip[s] = std::min(16, 1 << std::min(h->net_losses[s] - 1, 4));
+    }
     if (use_prev && rp < r0) {
         if (learned) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));   // phi2 holds the network's
         else MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
@@ -939,6 +953,7 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
 
 extern "C" int mdx_set_guess(mdx_handle *h, int32_t guess) {
     MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    for (int q = 0; q < 3; ++q) h->net_losses[q] = h->net_skip[q] = 0;
     MDX_CHECK(guess == MDX_GUESS_PREVIOUS || guess == MDX_GUESS_LEARNED || guess == MDX_GUESS_PHI_PREV,
               MDX_ERR_ARG, "mdx_set_guess: unknown guess mode");
     MDX_CHECK(guess == MDX_GUESS_PREVIOUS || h->cfg.solver == MDX_SOLVER_SOR, MDX_ERR_UNSUPPORTED,

@@ -152,6 +152,8 @@ struct mdx_handle {
         bool ibm = false;
     } graphs[2];
     bool use_graphs = true;
+    // momentum through the LDS-tiled kernel (MOBYDIFF_MOMENTUM_DIRECT=1: one thread per cell)
+    bool tiled_momentum = true;
     // streaming red-black sweeps: z planes marched per thread (0: one cell per thread)
     int sweep_kz = 4;
 
@@ -174,6 +176,8 @@ struct mdx_handle {
     double *phi_hist = nullptr;
     double *res64 = nullptr;       // residual field of the previous-Phi guess (network input)
     double hist_dtg[3] = {0.0, 0.0, 0.0};
+    // learned-guess gate per stage: consecutive solves the network's start lost, solves left to skip
+    int net_losses[3] = {0, 0, 0}, net_skip[3] = {0, 0, 0};
     long proj_count = 0;
 };
 

@@ -302,7 +302,8 @@ template <> struct Pair<double> { using V = double2; };
 static constexpr int ZX = 32;   // pairs per block row
 static constexpr int ZY = 8;    // rows per block
 
-template <typename T, bool STRETCH, bool IBM, int KZ>
+// MU: 0 no immersed boundary (mu = 1), 1 z-invariant mu planes, 2 mu varying in z
+template <typename T, bool STRETCH, int MU, int KZ>
 __global__ __launch_bounds__(ZX * ZY) void sor_sweep_zm_kernel(GridK g, MuK<T> mu, SorConst<T> s, int color,
                                                                T omega, const T *__restrict__ rhs, T *phi) {
     using V = typename Pair<T>::V;
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(ZX * ZY) void sor_sweep_zm_kernel(GridK g, MuK<T> m
     // mu of the two cells: u faces at i0, i0+1, ie; v faces at j, jp; w faces at k, kp
     T mu_u0 = 1, mu_u1 = 1, mu_u2 = 1, mv0 = 1, mv1 = 1, mvp0 = 1, mvp1 = 1;
     T mw0 = 1, mw1 = 1, mwp0 = 1, mwp1 = 1;
-    if (IBM && mu.plane) {
+    if (MU == 1) {
         mu_u0 = mu.u[mu.at(i0, j, 0)];
         mu_u1 = mu.u[mu.at(i0 + 1, j, 0)];
         mu_u2 = mu.u[mu.at(ie, j, 0)];
@@ -352,48 +353,86 @@ __global__ __launch_bounds__(ZX * ZY) void sor_sweep_zm_kernel(GridK g, MuK<T> m
     const size_t rows = rs * (size_t)(js - 1);
     const V *P2 = reinterpret_cast<const V *>(phi);
     V *W2 = reinterpret_cast<V *>(phi);
-    const int km0 = k0 == 1 ? nz : k0 - 1;
-    V prv = P2[(rowj + ps * (size_t)(km0 - 1)) / 2];
-    V cur = P2[(rowj + ps * (size_t)(k0 - 1)) / 2];
-    for (int k = k0; k <= k1; ++k) {
-        const int kp = k == nz ? 1 : k + 1;
-        const size_t pk = ps * (size_t)(k - 1);
-        const V nxt = P2[(rowj + ps * (size_t)(kp - 1)) / 2];
-        const int c = (j + k + color) & 1;
-        const int i = i0 + c;
-        const T pout = phi[(size_t)((c ? ie : iw) - 1) + rs * (size_t)(j - 1) + pk];
-        const T pyp = phi[(size_t)(i - 1) + rown + pk];
-        const T pym = phi[(size_t)(i - 1) + rows + pk];
-        const T rc = rhs[rowj + (size_t)c + pk];
-        if (IBM && !mu.plane) {
-            mu_u0 = mu.u[mu.at(i0, j, k)];
-            mu_u1 = mu.u[mu.at(i0 + 1, j, k)];
-            mu_u2 = mu.u[mu.at(ie, j, k)];
-            mv0 = mu.v[mu.at(i0, j, k)];
-            mv1 = mu.v[mu.at(i0 + 1, j, k)];
-            mvp0 = mu.v[mu.at(i0, jp, k)];
-            mvp1 = mu.v[mu.at(i0 + 1, jp, k)];
-            mw0 = mu.w[mu.at(i0, j, k)];
-            mw1 = mu.w[mu.at(i0 + 1, j, k)];
-            mwp0 = mu.w[mu.at(i0, j, kp)];
-            mwp1 = mu.w[mu.at(i0 + 1, j, kp)];
+    const int nk = k1 - k0 + 1;
+    // Every value the segment reads is fetched before the first store: loads issued after a store
+    // through the same pointer cannot be reordered above it, which would serialise the planes.
+    // None of these values is written by this sweep (other-colour cells, or this thread's own
+    // pairs before their update), so the prefetch reads exactly what the plane-by-plane order reads.
+    V pr[KZ + 2];
+    T pout[KZ], pyp[KZ], pym[KZ], rcv[KZ];
+    T nmu_u0[KZ], nmu_u1[KZ], nmu_u2[KZ], nmv0[KZ], nmv1[KZ], nmvp0[KZ], nmvp1[KZ];
+    T nmw0[KZ], nmw1[KZ], nmwp0[KZ], nmwp1[KZ];
+#pragma unroll
+    for (int t = 0; t < KZ + 2; ++t) {
+        if (t <= nk + 1) {
+            int kk = k0 - 1 + t;
+            if (kk < 1) kk = nz;
+            if (kk > nz) kk = 1;
+            pr[t] = P2[(rowj + ps * (size_t)(kk - 1)) / 2];
         }
-        const T p0 = c ? cur.y : cur.x;
-        const T pin = c ? cur.x : cur.y;
-        const T pxp = c ? pout : pin, pxm = c ? pin : pout;
-        const T pzp = c ? nxt.y : nxt.x, pzm = c ? prv.y : prv.x;
-        const T mui = c ? mu_u1 : mu_u0, muip = c ? mu_u2 : mu_u1;
-        const T mvj = c ? mv1 : mv0, mvjp = c ? mvp1 : mvp0;
-        const T mwk = c ? mw1 : mw0, mwkp = c ? mwp1 : mwp0;
-        T denom;
-        const T r = sor_residual_core<T, STRETCH>(p0, pxp, pxm, pyp, pym, pzp, pzm, mui, muip, mvj, mvjp, mwk, mwkp,
-                                                  wn_, ws_, rc, s, an, as, denom);
-        const T nv = p0 + r * (-omega / denom);
-        if (c) cur.y = nv;
-        else cur.x = nv;
-        W2[(rowj + pk) / 2] = cur;
-        prv = cur;
-        cur = nxt;
+    }
+#pragma unroll
+    for (int t = 0; t < KZ; ++t) {
+        if (t < nk) {
+            const int k = k0 + t;
+            const int c = (j + k + color) & 1;
+            const int i = i0 + c;
+            const size_t pk = ps * (size_t)(k - 1);
+            pout[t] = phi[(size_t)((c ? ie : iw) - 1) + rs * (size_t)(j - 1) + pk];
+            pyp[t] = phi[(size_t)(i - 1) + rown + pk];
+            pym[t] = phi[(size_t)(i - 1) + rows + pk];
+            rcv[t] = rhs[rowj + (size_t)c + pk];
+            if (MU == 2) {
+                const int kp = k == nz ? 1 : k + 1;
+                nmu_u0[t] = mu.u[mu.at(i0, j, k)];
+                nmu_u1[t] = mu.u[mu.at(i0 + 1, j, k)];
+                nmu_u2[t] = mu.u[mu.at(ie, j, k)];
+                nmv0[t] = mu.v[mu.at(i0, j, k)];
+                nmv1[t] = mu.v[mu.at(i0 + 1, j, k)];
+                nmvp0[t] = mu.v[mu.at(i0, jp, k)];
+                nmvp1[t] = mu.v[mu.at(i0 + 1, jp, k)];
+                nmw0[t] = mu.w[mu.at(i0, j, k)];
+                nmw1[t] = mu.w[mu.at(i0 + 1, j, k)];
+                nmwp0[t] = mu.w[mu.at(i0, j, kp)];
+                nmwp1[t] = mu.w[mu.at(i0 + 1, j, kp)];
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < KZ; ++t) {
+        if (t < nk) {
+            const int k = k0 + t;
+            const int c = (j + k + color) & 1;
+            if (MU == 2) {
+                mu_u0 = nmu_u0[t];
+                mu_u1 = nmu_u1[t];
+                mu_u2 = nmu_u2[t];
+                mv0 = nmv0[t];
+                mv1 = nmv1[t];
+                mvp0 = nmvp0[t];
+                mvp1 = nmvp1[t];
+                mw0 = nmw0[t];
+                mw1 = nmw1[t];
+                mwp0 = nmwp0[t];
+                mwp1 = nmwp1[t];
+            }
+            V cur = pr[t + 1];
+            const V prv = pr[t], nxt = pr[t + 2];
+            const T p0 = c ? cur.y : cur.x;
+            const T pin = c ? cur.x : cur.y;
+            const T pxp = c ? pout[t] : pin, pxm = c ? pin : pout[t];
+            const T pzp = c ? nxt.y : nxt.x, pzm = c ? prv.y : prv.x;
+            const T mui = c ? mu_u1 : mu_u0, muip = c ? mu_u2 : mu_u1;
+            const T mvj = c ? mv1 : mv0, mvjp = c ? mvp1 : mvp0;
+            const T mwk = c ? mw1 : mw0, mwkp = c ? mwp1 : mwp0;
+            T denom;
+            const T r = sor_residual_core<T, STRETCH>(p0, pxp, pxm, pyp[t], pym[t], pzp, pzm, mui, muip, mvj, mvjp,
+                                                      mwk, mwkp, wn_, ws_, rcv[t], s, an, as, denom);
+            const T nv = p0 + r * (-omega / denom);
+            if (c) cur.y = nv;
+            else cur.x = nv;
+            W2[(rowj + ps * (size_t)(k - 1)) / 2] = cur;
+        }
     }
 }
 
@@ -405,6 +444,7 @@ static hipError_t launch_sweep_zm(mdx_handle *h, int color, T omega, const MuK<T
     dim3 blk(ZX, ZY), grd(xcd_grid(h->d.nx / 2, h->d.ny, (h->d.nz + kz - 1) / kz, ZX, ZY));
 #define MDX_ZM(S, I, K) \
     hipLaunchKernelGGL((sor_sweep_zm_kernel<T, S, I, K>), grd, blk, 0, h->stream, g, mu, s, color, omega, rhs, phi)
+    const int mumode = h->ibm_on ? (h->coef_plane ? 1 : 2) : 0;
 #define MDX_ZMK(S, I)                      \
     switch (kz) {                          \
     case 1: MDX_ZM(S, I, 1); break;        \
@@ -412,8 +452,11 @@ static hipError_t launch_sweep_zm(mdx_handle *h, int color, T omega, const MuK<T
     case 8: MDX_ZM(S, I, 8); break;        \
     default: MDX_ZM(S, I, 4); break;       \
     }
-    if (h->stretched) { if (h->ibm_on) { MDX_ZMK(true, true) } else { MDX_ZMK(true, false) } }
-    else { if (h->ibm_on) { MDX_ZMK(false, true) } else { MDX_ZMK(false, false) } }
+    if (h->stretched) {
+        if (mumode == 1) { MDX_ZMK(true, 1) } else if (mumode == 2) { MDX_ZMK(true, 2) } else { MDX_ZMK(true, 0) }
+    } else {
+        if (mumode == 1) { MDX_ZMK(false, 1) } else if (mumode == 2) { MDX_ZMK(false, 2) } else { MDX_ZMK(false, 0) }
+    }
 #undef MDX_ZMK
 #undef MDX_ZM
     return hipGetLastError();

@@ -9,8 +9,119 @@ static constexpr int BX = 64;   // one wavefront along x: coalesced rows of the 
 static constexpr int BY = 4;
 
 // ------------------------------------------------------------------------------- momentum
-// step.f90:27-207.  One thread per interior cell updates u*, v*, w* and oldrhs.  Reads 10 arrays,
-// writes 6 (128 B/cell with IBM coefficients, 104 B without).
+// step.f90:27-207.  Updates u*, v*, w* and oldrhs of one interior cell.  Reads 10 arrays, writes 6
+// (128 B/cell with IBM coefficients, 104 B without).  The stencil values come through an accessor
+// L (L.u(a, b, c) = un(i+a, j+b, k+c), likewise v, w, p), so the direct and the LDS-tiled kernels
+// evaluate the same expressions and round identically.
+template <bool STRETCH, bool IBM, class Acc>
+__device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j, int k, size_t c, size_t o,
+                                     size_t cc, double *__restrict__ us, double *__restrict__ vs,
+                                     double *__restrict__ ws, double *__restrict__ oru, double *__restrict__ orv,
+                                     double *__restrict__ orw, const double *__restrict__ cu,
+                                     const double *__restrict__ cv, const double *__restrict__ cw, double dta,
+                                     double dtb, double dtg, double idx, double idy, double idz, double idx2,
+                                     double idy2, double idz2, double ire) {
+    const int nx0 = g.per[0] ? 1 : 2, ny0 = g.per[1] ? 1 : 2, nz0 = g.per[2] ? 1 : 2;
+    const int jp = j + 1, jm = j - 1;
+    const double *ihc = g.ihc, *ihv = g.ihv;
+    if (i >= nx0) {
+        const double u0 = L.u(0, 0, 0), uip = L.u(1, 0, 0), uim = L.u(-1, 0, 0);
+        const double ujp = L.u(0, 1, 0), ujm = L.u(0, -1, 0);
+        const double ukp = L.u(0, 0, 1), ukm = L.u(0, 0, -1);
+        double uu_p = 0.25 * ((uip + u0) * (uip + u0));
+        double uu_m = 0.25 * ((u0 + uim) * (u0 + uim));
+        double uv_p = 0.25 * (ujp + u0) * (L.v(0, 1, 0) + L.v(-1, 1, 0));
+        double uv_m = 0.25 * (u0 + ujm) * (L.v(0, 0, 0) + L.v(-1, 0, 0));
+        double uw_p = 0.25 * (u0 + ukp) * (L.w(0, 0, 1) + L.w(-1, 0, 1));
+        double uw_m = 0.25 * (u0 + ukm) * (L.w(0, 0, 0) + L.w(-1, 0, 0));
+        double diff_ux = (uim - 2.0 * u0 + uip) * idx2;
+        double diff_uy, cy;
+        if (!STRETCH) {
+            diff_uy = (ujm - 2.0 * u0 + ujp) * idy2;
+            cy = (uv_p - uv_m) * idy;
+        } else {
+            diff_uy = ((ujp - u0) * ihv[jp] - (u0 - ujm) * ihv[j]) * ihc[j];
+            cy = (uv_p - uv_m) * ihc[j];
+        }
+        double diff_uz = (ukm - 2.0 * u0 + ukp) * idz2;
+        double dpx = (L.p(0, 0, 0) - L.p(-1, 0, 0)) * idx;
+        double rhsu = -(uu_p - uu_m) * idx - cy - (uw_p - uw_m) * idz + g.fx +
+                      ire * (diff_ux + diff_uy + diff_uz);
+        double s = u0 + dta * rhsu + dtb * oru[o] - dtg * dpx;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cu[cc]));
+        us[c] = s;
+        oru[o] = rhsu;
+    }
+    if (j >= ny0) {
+        const double v0 = L.v(0, 0, 0), vip = L.v(1, 0, 0), vim = L.v(-1, 0, 0);
+        const double vjp = L.v(0, 1, 0), vjm = L.v(0, -1, 0);
+        const double vkp = L.v(0, 0, 1), vkm = L.v(0, 0, -1);
+        double vu_p = 0.25 * (v0 + vip) * (L.u(1, 0, 0) + L.u(1, -1, 0));
+        double vu_m = 0.25 * (v0 + vim) * (L.u(0, 0, 0) + L.u(0, -1, 0));
+        double vv_p = 0.25 * ((v0 + vjp) * (v0 + vjp));
+        double vv_m = 0.25 * ((v0 + vjm) * (v0 + vjm));
+        double vw_p = 0.25 * (vkp + v0) * (L.w(0, 0, 1) + L.w(0, -1, 1));
+        double vw_m = 0.25 * (vkm + v0) * (L.w(0, 0, 0) + L.w(0, -1, 0));
+        double diff_vx = (vim - 2.0 * v0 + vip) * idx2;
+        double diff_vy, cy, dpy;
+        if (!STRETCH) {
+            diff_vy = (vjm - 2.0 * v0 + vjp) * idy2;
+            cy = (vv_p - vv_m) * idy;
+            dpy = (L.p(0, 0, 0) - L.p(0, -1, 0)) * idy;
+        } else {
+            diff_vy = ((vjp - v0) * ihc[j] - (v0 - vjm) * ihc[jm]) * ihv[j];
+            cy = (vv_p - vv_m) * ihv[j];
+            dpy = (L.p(0, 0, 0) - L.p(0, -1, 0)) * ihv[j];
+        }
+        double diff_vz = (vkm - 2.0 * v0 + vkp) * idz2;
+        double rhsv = -(vu_p - vu_m) * idx - cy - (vw_p - vw_m) * idz + g.fy +
+                      ire * (diff_vx + diff_vy + diff_vz);
+        double s = v0 + dta * rhsv + dtb * orv[o] - dtg * dpy;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cv[cc]));
+        vs[c] = s;
+        orv[o] = rhsv;
+    }
+    if (k >= nz0) {
+        const double w0 = L.w(0, 0, 0), wip = L.w(1, 0, 0), wim = L.w(-1, 0, 0);
+        const double wjp = L.w(0, 1, 0), wjm = L.w(0, -1, 0);
+        const double wkp = L.w(0, 0, 1), wkm = L.w(0, 0, -1);
+        double wu_p = 0.25 * (w0 + wip) * (L.u(1, 0, 0) + L.u(1, 0, -1));
+        double wu_m = 0.25 * (w0 + wim) * (L.u(0, 0, 0) + L.u(0, 0, -1));
+        double ww_p = 0.25 * ((w0 + wkp) * (w0 + wkp));
+        double ww_m = 0.25 * ((w0 + wkm) * (w0 + wkm));
+        double wv_p = 0.25 * (w0 + wjp) * (L.v(0, 1, 0) + L.v(0, 1, -1));
+        double wv_m = 0.25 * (w0 + wjm) * (L.v(0, 0, 0) + L.v(0, 0, -1));
+        double diff_wx = (wim - 2.0 * w0 + wip) * idx2;
+        double diff_wy, cy;
+        if (!STRETCH) {
+            diff_wy = (wjm - 2.0 * w0 + wjp) * idy2;
+            cy = (wv_p - wv_m) * idy;
+        } else {
+            diff_wy = ((wjp - w0) * ihv[jp] - (w0 - wjm) * ihv[j]) * ihc[j];
+            cy = (wv_p - wv_m) * ihc[j];
+        }
+        double diff_wz = (wkm - 2.0 * w0 + wkp) * idz2;
+        double dpz = (L.p(0, 0, 0) - L.p(0, 0, -1)) * idz;
+        double rhsw = -(wu_p - wu_m) * idx - cy - (ww_p - ww_m) * idz + g.fz +
+                      ire * (diff_wx + diff_wy + diff_wz);
+        double s = w0 + dta * rhsw + dtb * orw[o] - dtg * dpz;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * cw[cc]));
+        ws[c] = s;
+        orw[o] = rhsw;
+    }
+}
+
+// global-memory operands of the stencil
+struct MomGlobal {
+    const double *un, *vn, *wn, *pn;
+    Dims d;
+    int i, j, k;
+    __device__ inline double u(int a, int b, int c) const { return un[d.g(i + a, j + b, k + c)]; }
+    __device__ inline double v(int a, int b, int c) const { return vn[d.g(i + a, j + b, k + c)]; }
+    __device__ inline double w(int a, int b, int c) const { return wn[d.g(i + a, j + b, k + c)]; }
+    __device__ inline double p(int a, int b, int c) const { return pn[d.g(i + a, j + b, k + c)]; }
+};
+
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(BX * BY) void momentum_kernel(
     GridK g, const double *__restrict__ un, const double *__restrict__ vn,
@@ -27,98 +138,81 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
     const int j = ty * BY + threadIdx.y + 1;
     const int k = tz + 1;
     if (i > d.nx || j > d.ny) return;
-    const int nx0 = g.per[0] ? 1 : 2, ny0 = g.per[1] ? 1 : 2, nz0 = g.per[2] ? 1 : 2;
-    const int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
-    const size_t c = d.g(i, j, k);
-    const size_t o = d.in(i, j, k);
+    const MomGlobal L{un, vn, wn, pn, d, i, j, k};
     // coefficient index: z-invariant geometries keep one x-y plane (coef_kstride 0)
-    const size_t cc = d.g(i, j, k * coef_kstride);
-    const double *ihc = g.ihc, *ihv = g.ihv;
+    momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), d.in(i, j, k), d.g(i, j, k * coef_kstride), us, vs, ws,
+                                oru, orv, orw, cu, cv, cw, dta, dtb, dtg, idx, idy, idz, idx2, idy2, idz2, ire);
+}
 
-    if (i >= nx0) {
-        const double u0 = un[c], uip = un[d.g(ip, j, k)], uim = un[d.g(im, j, k)];
-        const double ujp = un[d.g(i, jp, k)], ujm = un[d.g(i, jm, k)];
-        const double ukp = un[d.g(i, j, kp)], ukm = un[d.g(i, j, km)];
-        double uu_p = 0.25 * ((uip + u0) * (uip + u0));
-        double uu_m = 0.25 * ((u0 + uim) * (u0 + uim));
-        double uv_p = 0.25 * (ujp + u0) * (vn[d.g(i, jp, k)] + vn[d.g(im, jp, k)]);
-        double uv_m = 0.25 * (u0 + ujm) * (vn[c] + vn[d.g(im, j, k)]);
-        double uw_p = 0.25 * (u0 + ukp) * (wn[d.g(i, j, kp)] + wn[d.g(im, j, kp)]);
-        double uw_m = 0.25 * (u0 + ukm) * (wn[c] + wn[d.g(im, j, k)]);
-        double diff_ux = (uim - 2.0 * u0 + uip) * idx2;
-        double diff_uy, cy;
-        if (!STRETCH) {
-            diff_uy = (ujm - 2.0 * u0 + ujp) * idy2;
-            cy = (uv_p - uv_m) * idy;
-        } else {
-            diff_uy = ((ujp - u0) * ihv[jp] - (u0 - ujm) * ihv[j]) * ihc[j];
-            cy = (uv_p - uv_m) * ihc[j];
+// LDS-tiled momentum: a 64 x 8 workgroup owns a 64 x 8 column of cells and marches MKZ planes in z.
+// The u, v, w planes k-1, k, k+1 (with a one-cell x-y halo) and the p plane k live in LDS as a
+// rolling window, so each value is fetched from global memory about 1.3 (x-y halo) x (MKZ+2)/MKZ
+// (z halo) times instead of once per stencil use (about 30 loads per cell in the direct kernel).
+static constexpr int MTX = 64, MTY = 8, MTW = MTX + 2, MTH = MTY + 2, MPL = MTW * MTH, MKZ = 8;
+
+struct MomTile {
+    const double *su, *sv, *sw, *sp;   // LDS planes
+    int off[3];                        // plane offsets of k-1, k, k+1
+    int t;                             // index of the cell in a plane
+    double pkm;                        // p(i, j, k-1)
+    __device__ inline double u(int a, int b, int c) const { return su[off[c + 1] + t + a + MTW * b]; }
+    __device__ inline double v(int a, int b, int c) const { return sv[off[c + 1] + t + a + MTW * b]; }
+    __device__ inline double w(int a, int b, int c) const { return sw[off[c + 1] + t + a + MTW * b]; }
+    __device__ inline double p(int a, int b, int c) const { return c == 0 ? sp[t + a + MTW * b] : pkm; }
+};
+
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
+    GridK g, const double *__restrict__ un, const double *__restrict__ vn,
+    const double *__restrict__ wn, const double *__restrict__ pn, double *__restrict__ us,
+    double *__restrict__ vs, double *__restrict__ ws, double *__restrict__ oru,
+    double *__restrict__ orv, double *__restrict__ orw, const double *__restrict__ cu,
+    const double *__restrict__ cv, const double *__restrict__ cw, int coef_kstride, double dta, double dtb,
+    double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
+    double ire) {
+    __shared__ double su[3 * MPL], sv[3 * MPL], sw[3 * MPL], sp[MPL];
+    const Dims d = g.d;
+    const int nseg = (d.nz + MKZ - 1) / MKZ;
+    int tx, ty, tz;
+    if (!xcd_tile(d.nx, d.ny, nseg, MTX, MTY, tx, ty, tz)) return;   // uniform per workgroup
+    const int i0 = tx * MTX + 1, j0 = ty * MTY + 1, k0 = tz * MKZ + 1;
+    const int k1 = k0 + MKZ - 1 < d.nz ? k0 + MKZ - 1 : d.nz;
+    const int tid = threadIdx.x + MTX * threadIdx.y;
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool active = i <= d.nx && j <= d.ny;
+    const int t = (threadIdx.x + 1) + MTW * (threadIdx.y + 1);
+    // plane k of u, v, w into window slot `slot`; halo cells beyond the ghost layer are not read
+    auto load_uvw = [&](int slot, int k) {
+        for (int q = tid; q < MPL; q += MTX * MTY) {
+            const int gi = i0 - 1 + q % MTW, gj = j0 - 1 + q / MTW;
+            if (gi <= d.nx + 1 && gj <= d.ny + 1) {
+                const size_t x = d.g(gi, gj, k);
+                su[slot * MPL + q] = un[x];
+                sv[slot * MPL + q] = vn[x];
+                sw[slot * MPL + q] = wn[x];
+            }
         }
-        double diff_uz = (ukm - 2.0 * u0 + ukp) * idz2;
-        double dpx = (pn[c] - pn[d.g(im, j, k)]) * idx;
-        double rhsu = -(uu_p - uu_m) * idx - cy - (uw_p - uw_m) * idz + g.fx +
-                      ire * (diff_ux + diff_uy + diff_uz);
-        double s = u0 + dta * rhsu + dtb * oru[o] - dtg * dpx;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cu[cc]));
-        us[c] = s;
-        oru[o] = rhsu;
-    }
-    if (j >= ny0) {
-        const double v0 = vn[c], vip = vn[d.g(ip, j, k)], vim = vn[d.g(im, j, k)];
-        const double vjp = vn[d.g(i, jp, k)], vjm = vn[d.g(i, jm, k)];
-        const double vkp = vn[d.g(i, j, kp)], vkm = vn[d.g(i, j, km)];
-        double vu_p = 0.25 * (v0 + vip) * (un[d.g(ip, j, k)] + un[d.g(ip, jm, k)]);
-        double vu_m = 0.25 * (v0 + vim) * (un[c] + un[d.g(i, jm, k)]);
-        double vv_p = 0.25 * ((v0 + vjp) * (v0 + vjp));
-        double vv_m = 0.25 * ((v0 + vjm) * (v0 + vjm));
-        double vw_p = 0.25 * (vkp + v0) * (wn[d.g(i, j, kp)] + wn[d.g(i, jm, kp)]);
-        double vw_m = 0.25 * (vkm + v0) * (wn[c] + wn[d.g(i, jm, k)]);
-        double diff_vx = (vim - 2.0 * v0 + vip) * idx2;
-        double diff_vy, cy, dpy;
-        if (!STRETCH) {
-            diff_vy = (vjm - 2.0 * v0 + vjp) * idy2;
-            cy = (vv_p - vv_m) * idy;
-            dpy = (pn[c] - pn[d.g(i, jm, k)]) * idy;
-        } else {
-            diff_vy = ((vjp - v0) * ihc[j] - (v0 - vjm) * ihc[jm]) * ihv[j];
-            cy = (vv_p - vv_m) * ihv[j];
-            dpy = (pn[c] - pn[d.g(i, jm, k)]) * ihv[j];
+    };
+    load_uvw(0, k0 - 1);
+    load_uvw(1, k0);
+    load_uvw(2, k0 + 1);
+    double pkm = active ? pn[d.g(i, j, k0 - 1)] : 0.0;
+    for (int k = k0; k <= k1; ++k) {
+        for (int q = tid; q < MPL; q += MTX * MTY) {
+            const int gi = i0 - 1 + q % MTW, gj = j0 - 1 + q / MTW;
+            if (gi <= d.nx + 1 && gj <= d.ny + 1) sp[q] = pn[d.g(gi, gj, k)];
         }
-        double diff_vz = (vkm - 2.0 * v0 + vkp) * idz2;
-        double rhsv = -(vu_p - vu_m) * idx - cy - (vw_p - vw_m) * idz + g.fy +
-                      ire * (diff_vx + diff_vy + diff_vz);
-        double s = v0 + dta * rhsv + dtb * orv[o] - dtg * dpy;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cv[cc]));
-        vs[c] = s;
-        orv[o] = rhsv;
-    }
-    if (k >= nz0) {
-        const double w0 = wn[c], wip = wn[d.g(ip, j, k)], wim = wn[d.g(im, j, k)];
-        const double wjp = wn[d.g(i, jp, k)], wjm = wn[d.g(i, jm, k)];
-        const double wkp = wn[d.g(i, j, kp)], wkm = wn[d.g(i, j, km)];
-        double wu_p = 0.25 * (w0 + wip) * (un[d.g(ip, j, k)] + un[d.g(ip, j, km)]);
-        double wu_m = 0.25 * (w0 + wim) * (un[c] + un[d.g(i, j, km)]);
-        double ww_p = 0.25 * ((w0 + wkp) * (w0 + wkp));
-        double ww_m = 0.25 * ((w0 + wkm) * (w0 + wkm));
-        double wv_p = 0.25 * (w0 + wjp) * (vn[d.g(i, jp, k)] + vn[d.g(i, jp, km)]);
-        double wv_m = 0.25 * (w0 + wjm) * (vn[c] + vn[d.g(i, j, km)]);
-        double diff_wx = (wim - 2.0 * w0 + wip) * idx2;
-        double diff_wy, cy;
-        if (!STRETCH) {
-            diff_wy = (wjm - 2.0 * w0 + wjp) * idy2;
-            cy = (wv_p - wv_m) * idy;
-        } else {
-            diff_wy = ((wjp - w0) * ihv[jp] - (w0 - wjm) * ihv[j]) * ihc[j];
-            cy = (wv_p - wv_m) * ihc[j];
+        __syncthreads();
+        const int s0 = (k - k0) % 3;   // slot of plane k-1
+        if (active) {
+            const MomTile L{su, sv, sw, sp, {s0 * MPL, ((s0 + 1) % 3) * MPL, ((s0 + 2) % 3) * MPL}, t, pkm};
+            momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), d.in(i, j, k), d.g(i, j, k * coef_kstride), us,
+                                        vs, ws, oru, orv, orw, cu, cv, cw, dta, dtb, dtg, idx, idy, idz, idx2, idy2,
+                                        idz2, ire);
+            pkm = sp[t];
         }
-        double diff_wz = (wkm - 2.0 * w0 + wkp) * idz2;
-        double dpz = (pn[c] - pn[d.g(i, j, km)]) * idz;
-        double rhsw = -(wu_p - wu_m) * idx - cy - (ww_p - ww_m) * idz + g.fz +
-                      ire * (diff_wx + diff_wy + diff_wz);
-        double s = w0 + dta * rhsw + dtb * orw[o] - dtg * dpz;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cw[cc]));
-        ws[c] = s;
-        orw[o] = rhsw;
+        __syncthreads();
+        if (k < k1) load_uvw(s0, k + 2);   // plane k-1 leaves the window, plane k+2 enters
     }
 }
 
@@ -126,9 +220,13 @@ hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
     GridK g = grid_k(h);
     const double dx = h->dx, dy = h->dy, dz = h->dz;
     const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
-    dim3 blk(BX, BY, 1), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, BX, BY));
+    const bool tiled = h->tiled_momentum;
+    dim3 blk = tiled ? dim3(MTX, MTY, 1) : dim3(BX, BY, 1);
+    dim3 grd(tiled ? xcd_grid(h->d.nx, h->d.ny, (h->d.nz + MKZ - 1) / MKZ, MTX, MTY)
+                   : xcd_grid(h->d.nx, h->d.ny, h->d.nz, BX, BY));
 #define MDX_MOM(S, I)                                                                        \
-    hipLaunchKernelGGL((momentum_kernel<S, I>), grd, blk, 0, h->stream, g, h->un, h->vn, h->wn, \
+    hipLaunchKernelGGL((tiled ? momentum_tiled_kernel<S, I> : momentum_kernel<S, I>), grd, blk, 0, h->stream, \
+                       g, h->un, h->vn, h->wn,                                                   \
                        h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw, h->cu, h->cv, h->cw,  \
                        h->coef_plane ? 0 : 1, dta, dtb, dtg, 1.0 / dx, 1.0 / dy, 1.0 / dz, 1.0 / dx2, 1.0 / dy2,      \
                        1.0 / dz2, 1.0 / h->cfg.re)

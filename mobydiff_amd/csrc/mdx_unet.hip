@@ -76,38 +76,43 @@ __global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__
 
 // ------------------------------------------------------------------------------- conv forward, LDS-tiled
 // Same GEMM, same weight fragments and the same K order and chunking as conv3_fwd_kernel (so the
-// outputs are bit-identical), but a workgroup computes a 16 x 4 x 4 output tile and first stages
-// its 18 x 6 x 6 input halo (all CIN channels) in LDS: every input voxel is fetched from global
+// outputs are bit-identical), but a workgroup computes a 16 x 4 x TZ output tile and first stages
+// its 18 x 6 x (TZ+2) input halo (all CIN channels) in LDS: every input voxel is fetched from global
 // memory about 2.5 times instead of once per tap (27), and each wave reuses one weight fragment
-// for 4 output rows.  Wave w owns output row y = y0 + w, its 4 M-tiles are z = z0 .. z0+3, and a
-// tile row is 16 consecutive x voxels.  Needs W % 16 == 0, H % 4 == 0, D % 4 == 0.
-static constexpr int CTX = 16, CTY = 4, CTZ = 4;
-static constexpr int CHX = CTX + 2, CHY = CTY + 2, CHZ = CTZ + 2, CHV = CHX * CHY * CHZ;
+// for TZ output rows.  Wave w owns output row y = y0 + w, its TZ M-tiles are z = z0 .. z0+TZ-1, and
+// a tile row is 16 consecutive x voxels.  A voxel's channels sit CIN + 8 elements apart in LDS so
+// the 16 lanes of a row read disjoint bank groups.  TZ = 4 (2 from 48 channels up, to stay within
+// 64 KB of LDS).  Needs W % 16 == 0, H % 4 == 0, D % TZ == 0.
+static constexpr int CTX = 16, CTY = 4, CHX = CTX + 2, CHY = CTY + 2;
+template <int CIN> struct ConvTile {
+    static constexpr int TZ = CIN >= 48 ? 2 : 4, HV = CHX * CHY * (TZ + 2), CS = CIN + 8;
+};
 
 template <int CIN, int COUT, bool RELU>
 __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict__ in, const bf16x8 *__restrict__ wp,
                                                           const float *__restrict__ bias, bf16 *__restrict__ out,
                                                           int W, int H, int D) {
     constexpr int K = 27 * CIN, KC = (K + 31) / 32, NT = COUT / 16, U = CIN / 8;
-    __shared__ __attribute__((aligned(16))) bf16 tile[CHV * CIN];
+    constexpr int TZ = ConvTile<CIN>::TZ, HV = ConvTile<CIN>::HV, CS = ConvTile<CIN>::CS;
+    __shared__ __attribute__((aligned(16))) bf16 tile[HV * CS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ntx = W / CTX, nty = H / CTY;
     const int x0 = (int)(blockIdx.x % ntx) * CTX, y0 = (int)((blockIdx.x / ntx) % nty) * CTY;
-    const int z0 = (int)(blockIdx.x / (ntx * nty)) * CTZ;
-    for (int q = threadIdx.x; q < CHV * U; q += 256) {
+    const int z0 = (int)(blockIdx.x / (ntx * nty)) * TZ;
+    for (int q = threadIdx.x; q < HV * U; q += 256) {
         const int v = q / U, u = q - v * U;
         const int hx = v % CHX, hy = (v / CHX) % CHY, hz = v / (CHX * CHY);
         const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
         bf16x8 a = {};
         if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D)
             a = *reinterpret_cast<const bf16x8 *>(in + (((long)z * H + y) * W + x) * CIN + u * 8);
-        *reinterpret_cast<bf16x8 *>(&tile[v * CIN + u * 8]) = a;
+        *reinterpret_cast<bf16x8 *>(&tile[v * CS + u * 8]) = a;
     }
     __syncthreads();
     const int r = lane & 15, g = lane >> 4;
-    f32x4 acc[CTZ][NT];
+    f32x4 acc[TZ][NT];
 #pragma unroll
-    for (int m = 0; m < CTZ; ++m)
+    for (int m = 0; m < TZ; ++m)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[m][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero8 = {};
@@ -120,8 +125,8 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) b[nt] = wp[((long)kc * NT + nt) * 64 + lane];
 #pragma unroll
-        for (int m = 0; m < CTZ; ++m) {
-            const bf16x8 a = valid ? *reinterpret_cast<const bf16x8 *>(&tile[(base + m * CHX * CHY) * CIN + c0]) : zero8;
+        for (int m = 0; m < TZ; ++m) {
+            const bf16x8 a = valid ? *reinterpret_cast<const bf16x8 *>(&tile[(base + m * CHX * CHY) * CS + c0]) : zero8;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
                 acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[m][nt], 0, 0, 0);
@@ -129,7 +134,7 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
     }
     const int y = y0 + wave;
 #pragma unroll
-    for (int m = 0; m < CTZ; ++m) {
+    for (int m = 0; m < TZ; ++m) {
         const long row = ((long)(z0 + m) * H + y) * W + x0;
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
@@ -236,36 +241,47 @@ __global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__
 // ------------------------------------------------------------------------------- elementwise
 // avg-pool 2x2x2 (fine W,H,D -> coarse W/2,H/2,D/2)
 __global__ void pool_fwd_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int C, int W, int H, int D) {
-    const int Wc = W / 2, Hc = H / 2, Dc = D / 2;
-    const long n = (long)Wc * Hc * Dc * C;
+    // one thread per coarse voxel and 8 channels (one 16-byte vector)
+    const int Wc = W / 2, Hc = H / 2, Dc = D / 2, G = C / 8;
+    const long n = (long)Wc * Hc * Dc * G;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C);
-        const long v = q / C;
+        const int c0 = (int)(q % G) * 8;
+        const long v = q / G;
         const int x = (int)(v % Wc), y = (int)((v / Wc) % Hc), z = (int)(v / ((long)Wc * Hc));
-        float s = 0.f;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int dz = 0; dz < 2; ++dz)
             for (int dy = 0; dy < 2; ++dy)
-                for (int dx = 0; dx < 2; ++dx)
-                    s += (float)in[(((long)(2 * z + dz) * H + 2 * y + dy) * W + 2 * x + dx) * C + c];
-        out[q] = (bf16)(s * 0.125f);
+                for (int dx = 0; dx < 2; ++dx) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
+                        in + (((long)(2 * z + dz) * H + 2 * y + dy) * W + 2 * x + dx) * C + c0);
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) s[e] += (float)a[e];
+                }
+        bf16x8 o;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) o[e] = (bf16)(s[e] * 0.125f);
+        *reinterpret_cast<bf16x8 *>(out + v * C + c0) = o;
     }
 }
 
 // nearest up-sampling of `coarse` into channels [0, Cu) of `cat` and a copy of `skip` into
-// channels [Cu, Cu+Cs) (the concat that feeds the decoder conv)
+// channels [Cu, Cu+Cs) (the concat that feeds the decoder conv); one thread per voxel and
+// 8 channels
 __global__ void upcat_kernel(const bf16 *__restrict__ coarse, const bf16 *__restrict__ skip, bf16 *__restrict__ cat,
                              int Cu, int Cs, int W, int H, int D) {
-    const int C = Cu + Cs;
-    const long n = (long)W * H * D * C;
+    const int C = Cu + Cs, G = C / 8;
+    const long n = (long)W * H * D * G;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int c = (int)(q % C);
-        const long v = q / C;
+        const int c = (int)(q % G) * 8;
+        const long v = q / G;
+        bf16x8 a;
         if (c < Cu) {
             const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
-            cat[q] = coarse[(((long)(z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2) * Cu + c];
+            a = *reinterpret_cast<const bf16x8 *>(coarse + (((long)(z / 2) * (H / 2) + y / 2) * (W / 2) + x / 2) * Cu + c);
         } else {
-            cat[q] = skip[v * Cs + (c - Cu)];
+            a = *reinterpret_cast<const bf16x8 *>(skip + v * Cs + (c - Cu));
         }
+        *reinterpret_cast<bf16x8 *>(cat + v * C + c) = a;
     }
 }
 
@@ -485,6 +501,11 @@ __global__ void input_from_f32_kernel(const float *__restrict__ x2, bf16 *__rest
 }
 
 // ------------------------------------------------------------------------------- launchers
+// reductions into a few accumulators (bias and head gradients, the loss): a bounded grid, so each
+// thread sums many voxels and the atomic additions stay few (one per wave and accumulator)
+static constexpr unsigned RED_BLOCKS = 512;
+static inline unsigned nred(long n) { return (unsigned)std::min<long>((n + 255) / 256, RED_BLOCKS); }
+
 static inline unsigned nblk(long n, int t = 256) {
     long b = (n + t - 1) / t;
     return (unsigned)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
@@ -500,17 +521,18 @@ static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in
                         bf16 *out, int lvl) {
     const int W = n->W[lvl], H = n->H[lvl], D = n->D[lvl];
     const unsigned blocks = (unsigned)(n->V[lvl] / 64);
-    const bool tiled = n->tiled_conv && W % CTX == 0 && H % CTY == 0 && D % CTZ == 0;
-    const unsigned tiles = (unsigned)((W / CTX) * (H / CTY) * (D / CTZ));
 #define TILED_CASE(CI, CO, R)                                                                                      \
-    if (tiled && cin == CI && cout == CO && relu == R) {                                                           \
+    if (n->tiled_conv && cin == CI && cout == CO && relu == R && W % CTX == 0 && H % CTY == 0 &&                   \
+        D % ConvTile<CI>::TZ == 0) {                                                                               \
+        const unsigned tiles = (unsigned)((W / CTX) * (H / CTY) * (D / ConvTile<CI>::TZ));                          \
         hipLaunchKernelGGL((conv3_tiled_kernel<CI, CO, R>), dim3(tiles), dim3(256), 0, n->stream, in,               \
                            reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);                              \
         return hipGetLastError();                                                                                  \
     }
     TILED_CASE(16, 16, true) TILED_CASE(16, 32, true) TILED_CASE(32, 32, true) TILED_CASE(32, 64, true)
-    TILED_CASE(48, 16, true) TILED_CASE(16, 16, false) TILED_CASE(32, 16, false) TILED_CASE(32, 32, false)
-    TILED_CASE(32, 96, false) TILED_CASE(16, 48, false)
+    TILED_CASE(48, 16, true) TILED_CASE(64, 64, true) TILED_CASE(16, 16, false) TILED_CASE(32, 16, false)
+    TILED_CASE(32, 32, false) TILED_CASE(32, 96, false) TILED_CASE(16, 48, false) TILED_CASE(64, 32, false)
+    TILED_CASE(64, 64, false)
 #undef TILED_CASE
 #define CONV_CASE(CI, CO, R)                                                                                       \
     if (cin == CI && cout == CO && relu == R) {                                                                    \
@@ -541,9 +563,9 @@ static hipError_t wgrad(mdx_net *n, int l, const bf16 *in, const bf16 *dy) {
     return hipErrorInvalidValue;
 #undef WG_CASE
     const long V = n->V[lvl];
-    if (L.cout == 16) hipLaunchKernelGGL(bias_grad_kernel<16>, dim3(nblk(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
-    else if (L.cout == 32) hipLaunchKernelGGL(bias_grad_kernel<32>, dim3(nblk(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
-    else hipLaunchKernelGGL(bias_grad_kernel<64>, dim3(nblk(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    if (L.cout == 16) hipLaunchKernelGGL(bias_grad_kernel<16>, dim3(nred(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    else if (L.cout == 32) hipLaunchKernelGGL(bias_grad_kernel<32>, dim3(nred(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    else hipLaunchKernelGGL(bias_grad_kernel<64>, dim3(nred(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
     return hipGetLastError();
 }
 
@@ -581,17 +603,17 @@ hipError_t net_forward(mdx_net *n) {
 #define TRY(x) if ((e = (x)) != hipSuccess) return e
     TRY(conv3(n, 16, 16, true, n->x_in, n->wp[0], Bs(n, 0), n->a[0], 0));
     TRY(conv3(n, 16, 16, true, n->a[0], n->wp[1], Bs(n, 1), n->a[1], 0));
-    hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[1] * 16)), dim3(256), 0, n->stream, n->a[1], n->p1, 16, W0, H0, D0);
+    hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[1] * 2)), dim3(256), 0, n->stream, n->a[1], n->p1, 16, W0, H0, D0);
     TRY(conv3(n, 16, 32, true, n->p1, n->wp[2], Bs(n, 2), n->a[2], 1));
     TRY(conv3(n, 32, 32, true, n->a[2], n->wp[3], Bs(n, 3), n->a[3], 1));
-    hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[2] * 32)), dim3(256), 0, n->stream, n->a[3], n->p2, 32, W1, H1, D1);
+    hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[2] * 4)), dim3(256), 0, n->stream, n->a[3], n->p2, 32, W1, H1, D1);
     TRY(conv3(n, 32, 64, true, n->p2, n->wp[4], Bs(n, 4), n->a[4], 2));
     TRY(conv3(n, 64, 64, true, n->a[4], n->wp[5], Bs(n, 5), n->a[5], 2));
-    hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[1] * 96)), dim3(256), 0, n->stream, n->a[5], n->a[3], n->cat2, 64, 32,
+    hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[1] * 12)), dim3(256), 0, n->stream, n->a[5], n->a[3], n->cat2, 64, 32,
                        W1, H1, D1);
     TRY(conv3(n, 96, 32, true, n->cat2, n->wp[6], Bs(n, 6), n->a[6], 1));
     TRY(conv3(n, 32, 32, true, n->a[6], n->wp[7], Bs(n, 7), n->a[7], 1));
-    hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[0] * 48)), dim3(256), 0, n->stream, n->a[7], n->a[1], n->cat1, 32, 16,
+    hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[0] * 6)), dim3(256), 0, n->stream, n->a[7], n->a[1], n->cat1, 32, 16,
                        W0, H0, D0);
     TRY(conv3(n, 48, 16, true, n->cat1, n->wp[8], Bs(n, 8), n->a[8], 0));
     TRY(conv3(n, 16, 16, true, n->a[8], n->wp[9], Bs(n, 9), n->a[9], 0));
@@ -606,9 +628,9 @@ hipError_t net_backward(mdx_net *n, float scale) {
     const long V0 = n->V[0];
     const int W0 = n->W[0], H0 = n->H[0], D0 = n->D[0], W1 = n->W[1], H1 = n->H[1], D1 = n->D[1];
     TRY(hipMemsetAsync(n->loss, 0, 4, n->stream));
-    hipLaunchKernelGGL(mse_kernel, dim3(nblk(V0)), dim3(256), 0, n->stream, n->out, n->target, n->dout,
+    hipLaunchKernelGGL(mse_kernel, dim3(nred(V0)), dim3(256), 0, n->stream, n->out, n->target, n->dout,
                        scale / (float)V0, V0, n->loss);
-    hipLaunchKernelGGL(head_bwd_kernel, dim3(nblk(V0)), dim3(256), 0, n->stream, n->a[9], Wt(n, 10), n->dout, n->g[9],
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(nred(V0)), dim3(256), 0, n->stream, n->a[9], Wt(n, 10), n->dout, n->g[9],
                        dWt(n, 10), dBs(n, 10), V0);
     // decoder level 0
     TRY(relu_bwd(n, n->g[9], nullptr, 0, 0, 16, n->a[9], 0));
@@ -706,7 +728,7 @@ hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, do
 hipError_t net_mse_only(mdx_net *n) {
     hipError_t e = hipMemsetAsync(n->loss, 0, 4, n->stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(mse_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->target, n->dout,
+    hipLaunchKernelGGL(mse_kernel, dim3(nred(n->V[0])), dim3(256), 0, n->stream, n->out, n->target, n->dout,
                        1.0f / (float)n->V[0], n->V[0], n->loss);
     return hipGetLastError();
 }
