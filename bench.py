@@ -68,7 +68,7 @@ def roofline(times, kind):
             "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": bytes_per_launch}
 
 
-def flat_fft_production(M, steps):
+def flat_fft_production(M, steps, precision=None):
     """Secondary line: the flat channel at production size, 256 x 192 x 256, tanh-stretched y
     (beta 1.5), 4 pi x 2 x 2 pi, Re_b = 5600, constant flow rate, dt = 5e-4, FFT + tridiagonal
     Poisson solve.  Canonical algorithmic traffic 768 B per cell-step (3 stages x 256 B)."""
@@ -77,6 +77,8 @@ def flat_fft_production(M, steps):
     cfg = M.channel_config(nx, ny, nz, 4 * math.pi, 2.0, 2 * math.pi, 5600.0, solver=M.SOLVER_FFT,
                            flowrate=True, ub_target=1.0, flow_height=2.0, stretch_beta=1.5, p_neumann_y=False,
                            timing=True)
+    if precision is not None:
+        cfg.precision = precision   # mixed: FP32 transforms/Thomas in FP64 defect correction to tol
     ch = M.Channel(cfg)
     ch.init_fields(seed=0, amp=0.1, geo=None, dt=5e-4)
     for _ in range(2):
@@ -93,7 +95,8 @@ def flat_fft_production(M, steps):
     ch.close()
     cells = nx * ny * nz
     gbs = 768.0 * cells * steps / el / 1e9
-    return {"workload": "flat channel 256x192x256 (beta=1.5, Re_b=5600, dt=5e-4), FFT+tridiagonal Poisson, FP64",
+    label = "FP64" if precision in (None, M.PREC_FP64) else "FP32 transforms/Thomas in FP64 defect correction to 1e-8"
+    return {"workload": "flat channel 256x192x256 (beta=1.5, Re_b=5600, dt=5e-4), FFT+tridiagonal Poisson, " + label,
             "value": round(cells * steps / el / 1e6, 3), "unit": "Mcell-steps/s", "steps": steps,
             "ms_per_step": round(el / steps * 1e3, 3),
             "phase_ms_per_step": {"stencil": round(t.stencil_ms / steps, 3), "poisson": round(t.poisson_ms / steps, 3),
@@ -243,6 +246,7 @@ def main():
     }
     if args.flat_steps > 0:
         out["flat_fft_production"] = flat_fft_production(M, args.flat_steps)
+        out["flat_fft_production_mixed"] = flat_fft_production(M, args.flat_steps, M.PREC_MIXED)
     if not args.no_cpu_baseline:
         aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         cores = int(os.environ.get("OMP_NUM_THREADS", aff))   # the oracle's OpenMP thread count

@@ -193,7 +193,9 @@ static void free_all(mdx_handle *h) {
                     h->cu, h->cv, h->cw, h->d_ihc, h->d_ihv, h->d_yc, h->phi, h->phi2, h->rhs,
                     h->e32, h->r32, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
-                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags, h->phi_hist, h->res64};
+                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags, h->phi_hist, h->res64,
+                    h->fft_phi, h->fft_res, h->fft_mean, h->hat32, h->cp32, h->den32, h->arow32, h->tw_x32,
+                    h->tw_z32};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -216,6 +218,8 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if (cfg->solver == MDX_SOLVER_FFT) {
         MDX_CHECK(is_pow2(nx) && is_pow2(nz) && nx <= 1024 && nz <= 1024, MDX_ERR_UNSUPPORTED,
                   "FFT pressure solver: nx and nz must be powers of two <= 1024");
+        MDX_CHECK(cfg->precision != MDX_PREC_MIXED || cfg->tol > 0.0, MDX_ERR_ARG,
+                  "mixed-precision FFT solve: tol > 0 required (||div||_inf target of the defect correction)");
     } else if (cfg->solver == MDX_SOLVER_SOLA || cfg->solver == MDX_SOLVER_SOR) {
         MDX_CHECK(nx % 2 == 0 && nz % 2 == 0, MDX_ERR_ARG,
                   "red-black pressure solver requires even nx and nz for periodic coloring");
@@ -330,6 +334,23 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             (e = hipMemcpy(h->tw_x, twx.data(), nx * 8, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(h->tw_z, twz.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess)
             return fail(e, "FFT table upload");
+        if (cfg->precision == MDX_PREC_MIXED) {
+            auto to32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
+            const std::vector<float> cp32 = to32(cp), di32 = to32(di), ar32 = to32(ar), twx32 = to32(twx),
+                                     twz32 = to32(twz);
+            if ((e = dalloc(&h->fft_phi, h->ni)) != hipSuccess || (e = dalloc(&h->fft_res, h->ni)) != hipSuccess ||
+                (e = dalloc(&h->fft_mean, 2 * (size_t)ny)) != hipSuccess ||
+                (e = dalloc(&h->hat32, 2 * nh)) != hipSuccess || (e = dalloc(&h->cp32, nh)) != hipSuccess ||
+                (e = dalloc(&h->den32, nh)) != hipSuccess || (e = dalloc(&h->arow32, ny)) != hipSuccess ||
+                (e = dalloc(&h->tw_x32, nx)) != hipSuccess || (e = dalloc(&h->tw_z32, nz)) != hipSuccess)
+                return fail(e, "mixed FFT workspace");
+            if ((e = hipMemcpy(h->cp32, cp32.data(), nh * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(h->den32, di32.data(), nh * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(h->arow32, ar32.data(), ny * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(h->tw_x32, twx32.data(), nx * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(h->tw_z32, twz32.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess)
+                return fail(e, "mixed FFT table upload");
+        }
     }
     if (cfg->timing) {
         h->timing = true;
@@ -899,7 +920,32 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
     MDX_CHECK(dtg > 0.0, MDX_ERR_ARG, "pressure_projection: dt_gamma must be positive");
     mdx_solve_stats st{};
     const mdx_config &c = h->cfg;
-    if (c.solver == MDX_SOLVER_FFT) {
+    if (c.solver == MDX_SOLVER_FFT && c.precision == MDX_PREC_MIXED) {
+        // FP32 transforms and Thomas solves in an FP64 defect correction: phi starts from the FP64
+        // mean mode; each pass solves L e = rhs - L phi in FP32 (mean mode excluded) and adds e, until
+        // ||div||_inf = dt_gamma ||rhs - L phi||_inf < tol (at most MAX_FFT_PASSES passes)
+        constexpr int MAX_FFT_PASSES = 4;
+        ev_record(h, 1);
+        MDX_HIP(launch_fft_mixed_setup(h, dtg));
+        int passes = 0, rc;
+        double r = 0.0;
+        for (;;) {
+            MDX_HIP(launch_reset_max(h));
+            MDX_HIP(launch_fft_residual(h));
+            if ((rc = read_max(h, &r))) return rc;
+            if (passes == 0) st.div0_inf = r * dtg;
+            ++st.checks;
+            if (!(r * dtg >= c.tol) || passes >= MAX_FFT_PASSES) break;
+            MDX_HIP(launch_fft32_solve_add(h));
+            ++passes;
+        }
+        st.iterations = passes;
+        st.div_inf = r * dtg;
+        st.converged = r * dtg < c.tol;
+        ev_record(h, 2);
+        MDX_HIP(launch_fft_correct(h, h->fft_phi, dtg));
+        ev_record(h, 3);
+    } else if (c.solver == MDX_SOLVER_FFT) {
         ev_record(h, 1);
         MDX_HIP(launch_fft_forward(h, dtg));
         MDX_HIP(launch_tridiag(h));

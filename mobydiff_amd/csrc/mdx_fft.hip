@@ -15,26 +15,39 @@
 
 namespace mdx {
 
+// complex element types of the transforms: FP64 for the plain solve, FP32 for the defect solves
+template <typename C> struct Cx;
+template <> struct Cx<double2> {
+    using R = double;
+    static __device__ inline double2 mk(double a, double b) { return make_double2(a, b); }
+};
+template <> struct Cx<float2> {
+    using R = float;
+    static __device__ inline float2 mk(float a, float b) { return make_float2(a, b); }
+};
+
 // One radix-2 Stockham pass sequence over LINES lines of length n held in LDS (a, b ping-pong).
 // Returns the buffer holding the result.  tw[m] = exp(-2 pi i m / n), m < n/2.
-__device__ inline double2 *stockham(double2 *a, double2 *b, int n, int lines, const double2 *__restrict__ tw,
-                                    bool inverse, int tid, int nthr) {
+template <typename C>
+__device__ inline C *stockham(C *a, C *b, int n, int lines, const C *__restrict__ tw, bool inverse, int tid,
+                              int nthr) {
+    using R = typename Cx<C>::R;
     const int half = n >> 1;
     for (int ns = 1; ns < n; ns <<= 1) {
         const int stride = n / (2 * ns);
         for (int t = tid; t < half * lines; t += nthr) {
             const int line = t / half, jj = t - line * half;
             const int kk = jj & (ns - 1);
-            const double2 x0 = a[line * n + jj], x1 = a[line * n + jj + half];
-            double2 w = tw[kk * stride];
+            const C x0 = a[line * n + jj], x1 = a[line * n + jj + half];
+            C w = tw[kk * stride];
             if (inverse) w.y = -w.y;
-            const double tr = x1.x * w.x - x1.y * w.y, ti = x1.x * w.y + x1.y * w.x;
+            const R tr = x1.x * w.x - x1.y * w.y, ti = x1.x * w.y + x1.y * w.x;
             const int o = ((jj - kk) << 1) + kk;
-            b[line * n + o] = make_double2(x0.x + tr, x0.y + ti);
-            b[line * n + o + ns] = make_double2(x0.x - tr, x0.y - ti);
+            b[line * n + o] = Cx<C>::mk(x0.x + tr, x0.y + ti);
+            b[line * n + o + ns] = Cx<C>::mk(x0.x - tr, x0.y - ti);
         }
         __syncthreads();
-        double2 *t2 = a;
+        C *t2 = a;
         a = b;
         b = t2;
     }
@@ -51,7 +64,8 @@ __global__ __launch_bounds__(FFT_THREADS) void rhs_xfft_kernel(GridK g, const do
                                                               const double *__restrict__ ws, double dtg,
                                                               const double2 *__restrict__ tw, int lines,
                                                               double2 *hat) {
-    extern __shared__ double2 lds[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double2 *lds = reinterpret_cast<double2 *>(lds_raw);
     const Dims d = g.d;
     const int n = d.nx, nxh = n / 2 + 1;
     const int nrows = d.ny * d.nz;
@@ -79,21 +93,23 @@ __global__ __launch_bounds__(FFT_THREADS) void rhs_xfft_kernel(GridK g, const do
 }
 
 // 2./4. z-transform of TX consecutive kx columns of one j plane.
-__global__ __launch_bounds__(FFT_THREADS) void zfft_kernel(Dims d, double2 *hat, const double2 *__restrict__ tw,
-                                                           int tx, int inverse) {
-    extern __shared__ double2 lds[];
+template <typename C>
+__global__ __launch_bounds__(FFT_THREADS) void zfft_kernel(Dims d, C *hat, const C *__restrict__ tw, int tx,
+                                                           int inverse) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *lds = reinterpret_cast<C *>(lds_raw);
     const int nz = d.nz, nxh = d.nx / 2 + 1;
     const int tiles = (nxh + tx - 1) / tx;
     const int j = blockIdx.x / tiles, kx0 = (blockIdx.x % tiles) * tx;
-    double2 *a = lds, *b = lds + (size_t)tx * nz;
-    double2 *plane = hat + (size_t)nxh * nz * j;
+    C *a = lds, *b = lds + (size_t)tx * nz;
+    C *plane = hat + (size_t)nxh * nz * j;
     for (int t = threadIdx.x; t < tx * nz; t += FFT_THREADS) {
         const int c = t % tx, kz = t / tx;
         const int kx = kx0 + c;
-        a[c * nz + kz] = kx < nxh ? plane[(size_t)kx + (size_t)nxh * kz] : make_double2(0.0, 0.0);
+        a[c * nz + kz] = kx < nxh ? plane[(size_t)kx + (size_t)nxh * kz] : Cx<C>::mk(0, 0);
     }
     __syncthreads();
-    double2 *res = stockham(a, b, nz, tx, tw, inverse != 0, threadIdx.x, FFT_THREADS);
+    C *res = stockham(a, b, nz, tx, tw, inverse != 0, threadIdx.x, FFT_THREADS);
     for (int t = threadIdx.x; t < tx * nz; t += FFT_THREADS) {
         const int c = t % tx, kz = t / tx;
         const int kx = kx0 + c;
@@ -102,10 +118,15 @@ __global__ __launch_bounds__(FFT_THREADS) void zfft_kernel(Dims d, double2 *hat,
 }
 
 // 3. pressure_fft.f90:102-148 with precomputed cp/den_inv (pressure_fft.f90:20-100); one thread
-//    per (kx,kz) line, consecutive kx in consecutive lanes (coalesced), in place.
-__global__ __launch_bounds__(256) void tridiag_kernel(Dims d, double2 *hat, const double *__restrict__ cp,
-                                                      const double *__restrict__ den_inv,
-                                                      const double *__restrict__ arow) {
+//    per (kx,kz) line, consecutive kx in consecutive lanes (coalesced), in place.  The recurrences
+//    run in blocks of TB planes whose loads are all issued before the block's stores (a load after
+//    a store through the same pointer cannot be hoisted above it), so TB loads are in flight at a
+//    time instead of one; the arithmetic and its order are those of the plane-by-plane loops.
+static constexpr int TB = 8;
+
+__global__ __launch_bounds__(64) void tridiag_kernel(Dims d, double2 *hat, const double *__restrict__ cp,
+                                                     const double *__restrict__ den_inv,
+                                                     const double *__restrict__ arow) {
     const int nxh = d.nx / 2 + 1, ny = d.ny, nz = d.nz;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nxh * nz) return;
@@ -123,18 +144,49 @@ __global__ __launch_bounds__(256) void tridiag_kernel(Dims d, double2 *hat, cons
         dp = make_double2(r.x * di, r.y * di);
     }
     hat[h0] = dp;
-    for (int jj = 2; jj <= ny; ++jj) {
-        const double2 r = hat[h0 + hs * (jj - 1)];
-        const double a = arow[jj - 1], di = den_inv[c0 + cs * (jj - 1)];
-        dp = make_double2((r.x - a * dp.x) * di, (r.y - a * dp.y) * di);
-        hat[h0 + hs * (jj - 1)] = dp;
+    // forward elimination, planes 2..ny
+    for (int jb = 2; jb <= ny; jb += TB) {
+        double2 rr[TB];
+        double aa[TB], dd[TB];
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int jj = jb + q;
+            if (jj <= ny) {
+                rr[q] = hat[h0 + hs * (jj - 1)];
+                aa[q] = arow[jj - 1];
+                dd[q] = den_inv[c0 + cs * (jj - 1)];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int jj = jb + q;
+            if (jj <= ny) {
+                dp = make_double2((rr[q].x - aa[q] * dp.x) * dd[q], (rr[q].y - aa[q] * dp.y) * dd[q]);
+                hat[h0 + hs * (jj - 1)] = dp;
+            }
+        }
     }
-    double2 nxt = dp;   // hat(ny) = dp(ny)
-    for (int jj = ny - 1; jj >= 1; --jj) {
-        const double2 q = hat[h0 + hs * (jj - 1)];
-        const double c = cp[c0 + cs * (jj - 1)];
-        nxt = make_double2(q.x - c * nxt.x, q.y - c * nxt.y);
-        hat[h0 + hs * (jj - 1)] = nxt;
+    // back substitution, planes ny-1..1 (hat(ny) = dp(ny))
+    double2 nxt = dp;
+    for (int jb = ny - 1; jb >= 1; jb -= TB) {
+        double2 qq[TB];
+        double cc[TB];
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int jj = jb - q;
+            if (jj >= 1) {
+                qq[q] = hat[h0 + hs * (jj - 1)];
+                cc[q] = cp[c0 + cs * (jj - 1)];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int jj = jb - q;
+            if (jj >= 1) {
+                nxt = make_double2(qq[q].x - cc[q] * nxt.x, qq[q].y - cc[q] * nxt.y);
+                hat[h0 + hs * (jj - 1)] = nxt;
+            }
+        }
     }
 }
 
@@ -143,7 +195,8 @@ __global__ __launch_bounds__(256) void tridiag_kernel(Dims d, double2 *hat, cons
 __global__ __launch_bounds__(FFT_THREADS) void ixfft_kernel(Dims d, const double2 *__restrict__ hat,
                                                            const double2 *__restrict__ tw, int lines,
                                                            double scale, double *phi) {
-    extern __shared__ double2 lds[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double2 *lds = reinterpret_cast<double2 *>(lds_raw);
     const int n = d.nx, nxh = n / 2 + 1;
     const int nrows = d.ny * d.nz;
     double2 *a = lds, *b = lds + (size_t)lines * n;
@@ -230,7 +283,7 @@ hipError_t launch_fft_forward(mdx_handle *h, double dtg) {
                            (double2 *)h->hat);
     const int tx = z_tile(d.nz), nxh = d.nx / 2 + 1;
     const size_t shz = 2 * (size_t)tx * d.nz * sizeof(double2);
-    hipLaunchKernelGGL(zfft_kernel, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
+    hipLaunchKernelGGL(zfft_kernel<double2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
                        (double2 *)h->hat, (const double2 *)h->tw_z, tx, 0);
     return hipGetLastError();
 }
@@ -238,7 +291,7 @@ hipError_t launch_fft_forward(mdx_handle *h, double dtg) {
 hipError_t launch_tridiag(mdx_handle *h) {
     const Dims d = h->d;
     const int nlines = (d.nx / 2 + 1) * d.nz;
-    hipLaunchKernelGGL(tridiag_kernel, dim3((nlines + 255) / 256), dim3(256), 0, h->stream, d, (double2 *)h->hat,
+    hipLaunchKernelGGL(tridiag_kernel, dim3((nlines + 63) / 64), dim3(64), 0, h->stream, d, (double2 *)h->hat,
                        h->cp, h->den_inv, h->arow);
     return hipGetLastError();
 }
@@ -248,7 +301,7 @@ hipError_t launch_fft_backward_correct(mdx_handle *h, double dtg) {
     const Dims d = h->d;
     const int tx = z_tile(d.nz), nxh = d.nx / 2 + 1;
     const size_t shz = 2 * (size_t)tx * d.nz * sizeof(double2);
-    hipLaunchKernelGGL(zfft_kernel, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
+    hipLaunchKernelGGL(zfft_kernel<double2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
                        (double2 *)h->hat, (const double2 *)h->tw_z, tx, 1);
     const int lines = rows_per_block(d.nx);
     const int nrows = d.ny * d.nz;
@@ -260,6 +313,298 @@ hipError_t launch_fft_backward_correct(mdx_handle *h, double dtg) {
 #define MDX_CORR(S, I)                                                                                     \
     hipLaunchKernelGGL((fft_correct_kernel<S, I>), grd, blk, 0, h->stream, g, h->fft_rhs, h->us, h->vs, h->ws, \
                        h->cu, h->cv, h->cw, dtg, h->un, h->vn, h->wn, h->pn)
+    if (h->stretched) { if (h->ibm_on) MDX_CORR(true, true); else MDX_CORR(true, false); }
+    else { if (h->ibm_on) MDX_CORR(false, true); else MDX_CORR(false, false); }
+#undef MDX_CORR
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- mixed precision
+// FP32 transforms and Thomas solves inside an FP64 defect correction.  The mean mode (kx = kz = 0),
+// whose system is singular and pinned by phi_hat(y_1) = 0 (pressure_fft.f90:92-96, 119-121), is
+// solved once in FP64 from the plane sums of the FP64 right-hand side; the FP32 solves carry the
+// other modes of the residual r = rhs - L phi, formed in FP64 with the discrete operator the
+// transforms diagonalise (3-point second differences in x and z, the tridiagonal rows in y).
+
+// rhs(i,k,j) of rhs_xfft_kernel, in FP64, layout (nx, nz, ny)
+template <bool STRETCH>
+__global__ __launch_bounds__(256) void fft_div_rhs_kernel(GridK g, const double *__restrict__ us,
+                                                          const double *__restrict__ vs,
+                                                          const double *__restrict__ ws, double dtg,
+                                                          double *__restrict__ rhs) {
+    const Dims d = g.d;
+    const long n = (long)d.nx * d.ny * d.nz;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % d.nx) + 1;
+        const long r = q / d.nx;
+        const int k = (int)(r % d.nz) + 1, j = (int)(r / d.nz) + 1;
+        const double dv = STRETCH ? (vs[d.g(i, j + 1, k)] - vs[d.g(i, j, k)]) * g.ihc[j]
+                                  : (vs[d.g(i, j + 1, k)] - vs[d.g(i, j, k)]) / g.dy;
+        rhs[q] = ((us[d.g(i + 1, j, k)] - us[d.g(i, j, k)]) / g.dx + dv +
+                  (ws[d.g(i, j, k + 1)] - ws[d.g(i, j, k)]) / g.dz) / dtg;
+    }
+}
+
+// sums[j-1] = sum over the (x, z) plane j of rhs (one workgroup per plane, fixed order)
+__global__ __launch_bounds__(256) void fft_plane_sum_kernel(Dims d, const double *__restrict__ rhs, double *sums) {
+    __shared__ double s[256];
+    const long plane = (long)d.nx * d.nz;
+    const double *p = rhs + plane * blockIdx.x;
+    double a = 0.0;
+    for (long q = threadIdx.x; q < plane; q += 256) a += p[q];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
+}
+
+// the kx = kz = 0 line of tridiag_kernel in FP64 on the plane sums; mean[j-1] = x_j / (nx nz)
+__global__ void fft_mean_mode_kernel(Dims d, const double *__restrict__ sums, const double *__restrict__ cp,
+                                     const double *__restrict__ den_inv, const double *__restrict__ arow,
+                                     double scale, double *mean) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int ny = d.ny;
+    const size_t cs = (size_t)(d.nx / 2 + 1);   // stride between j of line (0, 0) in cp / den_inv
+    double dp = 0.0;
+    mean[0] = 0.0;
+    for (int jj = 2; jj <= ny; ++jj) {
+        dp = (sums[jj - 1] - arow[jj - 1] * dp) * den_inv[cs * (jj - 1)];
+        mean[jj - 1] = dp;
+    }
+    double nxt = dp;
+    mean[ny - 1] = nxt * scale;
+    for (int jj = ny - 1; jj >= 1; --jj) {
+        nxt = mean[jj - 1] - cp[cs * (jj - 1)] * nxt;
+        mean[jj - 1] = nxt;
+    }
+    for (int jj = 1; jj < ny; ++jj) mean[jj - 1] *= scale;
+}
+
+// phi = mean[j] everywhere in plane j
+__global__ void fft_fill_mean_kernel(Dims d, const double *__restrict__ mean, double *__restrict__ phi) {
+    const long n = (long)d.nx * d.ny * d.nz;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        phi[q] = mean[q / ((long)d.nx * d.nz)];
+}
+
+// res = rhs - L phi (FP64), and max |res| per workgroup; layout (nx, nz, ny), x and z periodic
+template <bool STRETCH>
+__global__ __launch_bounds__(256) void fft_residual_kernel(GridK g, const double *__restrict__ rhs,
+                                                           const double *__restrict__ phi, const double *__restrict__ arow,
+                                                           double *__restrict__ res, double *part) {
+    const Dims d = g.d;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const long n = (long)nx * ny * nz;
+    const double idx2 = 1.0 / (g.dx * g.dx), idz2 = 1.0 / (g.dz * g.dz);
+    double m = 0.0;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % nx);
+        const long r = q / nx;
+        const int k = (int)(r % nz), j = (int)(r / nz);   // 0-based
+        const long row = q - i;
+        const double p0 = phi[q];
+        const double pxp = phi[row + (i + 1 == nx ? 0 : i + 1)], pxm = phi[row + (i == 0 ? nx - 1 : i - 1)];
+        const long plane = (long)nx * nz * j;
+        const long kp = k + 1 == nz ? 0 : k + 1, km = k == 0 ? nz - 1 : k - 1;
+        const double pzp = phi[plane + (long)nx * kp + i], pzm = phi[plane + (long)nx * km + i];
+        double ly = 0.0;
+        if (j > 0) ly += arow[j] * (phi[q - (long)nx * nz] - p0);
+        if (j < ny - 1) {
+            const double c = STRETCH ? g.ihv[j + 2] * g.ihc[j + 1] : arow[j + 1];
+            ly += c * (phi[q + (long)nx * nz] - p0);
+        }
+        const double lap = (pxp - 2.0 * p0 + pxm) * idx2 + (pzp - 2.0 * p0 + pzm) * idz2 + ly;
+        const double rr = rhs[q] - lap;
+        res[q] = rr;
+        const double a = fabs(rr);
+        m = nanmax_any(m, a);
+    }
+    block_max_part(m, part);
+}
+
+// 1. (FP32) r2c x-transform of a real FP64 field in (nx, nz, ny) layout
+__global__ __launch_bounds__(FFT_THREADS) void xfft32_kernel(Dims d, const double *__restrict__ in,
+                                                             const float2 *__restrict__ tw, int lines, float2 *hat) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float2 *lds = reinterpret_cast<float2 *>(lds_raw);
+    const int n = d.nx, nxh = n / 2 + 1;
+    const int nrows = d.ny * d.nz;
+    float2 *a = lds, *b = lds + (size_t)lines * n;
+    for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
+        const int line = t / n, i = t - line * n;
+        const int r = blockIdx.x * lines + line;
+        a[t] = make_float2(r < nrows ? (float)in[(size_t)n * r + i] : 0.f, 0.f);
+    }
+    __syncthreads();
+    float2 *res = stockham(a, b, n, lines, tw, false, threadIdx.x, FFT_THREADS);
+    for (int t = threadIdx.x; t < lines * nxh; t += FFT_THREADS) {
+        const int line = t / nxh, m = t - line * nxh;
+        const int r = blockIdx.x * lines + line;
+        if (r < nrows) hat[(size_t)nxh * r + m] = res[line * n + m];
+    }
+}
+
+// 3. (FP32) Thomas per (kx, kz) line with FP32 tables; the mean mode line is set to zero (it is
+//    solved in FP64 separately)
+__global__ __launch_bounds__(64) void tridiag32_kernel(Dims d, float2 *hat, const float *__restrict__ cp,
+                                                       const float *__restrict__ den_inv,
+                                                       const float *__restrict__ arow) {
+    const int nxh = d.nx / 2 + 1, ny = d.ny, nz = d.nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nxh * nz) return;
+    const int kx = t % nxh, kz = t / nxh;
+    const size_t hs = (size_t)nxh * nz, cs = (size_t)nxh;
+    const size_t h0 = (size_t)kx + (size_t)nxh * kz;
+    const size_t c0 = (size_t)kx + (size_t)nxh * (size_t)ny * kz;
+    if (kx == 0 && kz == 0) {
+        for (int jj = 1; jj <= ny; ++jj) hat[h0 + hs * (jj - 1)] = make_float2(0.f, 0.f);
+        return;
+    }
+    float2 dp;
+    {
+        const float2 r = hat[h0];
+        const float di = den_inv[c0];
+        dp = make_float2(r.x * di, r.y * di);
+    }
+    hat[h0] = dp;
+    for (int jb = 2; jb <= ny; jb += TB) {
+        float2 rr[TB];
+        float aa[TB], dd[TB];
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int jj = jb + q;
+            if (jj <= ny) {
+                rr[q] = hat[h0 + hs * (jj - 1)];
+                aa[q] = arow[jj - 1];
+                dd[q] = den_inv[c0 + cs * (jj - 1)];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int jj = jb + q;
+            if (jj <= ny) {
+                dp = make_float2((rr[q].x - aa[q] * dp.x) * dd[q], (rr[q].y - aa[q] * dp.y) * dd[q]);
+                hat[h0 + hs * (jj - 1)] = dp;
+            }
+        }
+    }
+    float2 nxt = dp;
+    for (int jb = ny - 1; jb >= 1; jb -= TB) {
+        float2 qq[TB];
+        float cc[TB];
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int jj = jb - q;
+            if (jj >= 1) {
+                qq[q] = hat[h0 + hs * (jj - 1)];
+                cc[q] = cp[c0 + cs * (jj - 1)];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int jj = jb - q;
+            if (jj >= 1) {
+                nxt = make_float2(qq[q].x - cc[q] * nxt.x, qq[q].y - cc[q] * nxt.y);
+                hat[h0 + hs * (jj - 1)] = nxt;
+            }
+        }
+    }
+}
+
+// 5. (FP32) c2r x-transform, scaled, added to the FP64 phi
+__global__ __launch_bounds__(FFT_THREADS) void ixfft32_add_kernel(Dims d, const float2 *__restrict__ hat,
+                                                                  const float2 *__restrict__ tw, int lines, float scale,
+                                                                  double *phi) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float2 *lds = reinterpret_cast<float2 *>(lds_raw);
+    const int n = d.nx, nxh = n / 2 + 1;
+    const int nrows = d.ny * d.nz;
+    float2 *a = lds, *b = lds + (size_t)lines * n;
+    for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
+        const int line = t / n, m = t - line * n;
+        const int r = blockIdx.x * lines + line;
+        float2 v = make_float2(0.f, 0.f);
+        if (r < nrows) {
+            const float2 *row = hat + (size_t)nxh * r;
+            if (m == 0 || 2 * m == n) v = make_float2(row[m].x, 0.f);
+            else if (m < nxh) v = row[m];
+            else v = make_float2(row[n - m].x, -row[n - m].y);
+        }
+        a[t] = v;
+    }
+    __syncthreads();
+    float2 *res = stockham(a, b, n, lines, tw, true, threadIdx.x, FFT_THREADS);
+    for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
+        const int line = t / n, i = t - line * n;
+        const int r = blockIdx.x * lines + line;
+        if (r < nrows) phi[(size_t)n * r + i] += (double)(res[t].x * scale);
+    }
+}
+
+// the mean mode and the FP64 right-hand side: rhs -> h->fft_rhs, mean-mode phi -> h->fft_phi
+hipError_t launch_fft_mixed_setup(mdx_handle *h, double dtg) {
+    GridK g = grid_k(h);
+    const Dims d = h->d;
+    if (h->stretched)
+        hipLaunchKernelGGL(fft_div_rhs_kernel<true>, dim3(4096), dim3(256), 0, h->stream, g, h->us, h->vs, h->ws, dtg,
+                           h->fft_rhs);
+    else
+        hipLaunchKernelGGL(fft_div_rhs_kernel<false>, dim3(4096), dim3(256), 0, h->stream, g, h->us, h->vs, h->ws,
+                           dtg, h->fft_rhs);
+    hipLaunchKernelGGL(fft_plane_sum_kernel, dim3(d.ny), dim3(256), 0, h->stream, d, h->fft_rhs, h->fft_mean);
+    hipLaunchKernelGGL(fft_mean_mode_kernel, dim3(1), dim3(64), 0, h->stream, d, h->fft_mean, h->cp, h->den_inv,
+                       h->arow, 1.0 / (double)(d.nx * d.nz), h->fft_mean + d.ny);
+    hipLaunchKernelGGL(fft_fill_mean_kernel, dim3(4096), dim3(256), 0, h->stream, d, h->fft_mean + d.ny, h->fft_phi);
+    return hipGetLastError();
+}
+
+// res = rhs - L phi into h->fft_res; its max |.| into the max reduction
+hipError_t launch_fft_residual(mdx_handle *h) {
+    GridK g = grid_k(h);
+    const unsigned blocks = 2048;
+    if (h->stretched)
+        hipLaunchKernelGGL(fft_residual_kernel<true>, dim3(blocks), dim3(256), 0, h->stream, g, h->fft_rhs, h->fft_phi,
+                           h->arow, h->fft_res, h->max_part);
+    else
+        hipLaunchKernelGGL(fft_residual_kernel<false>, dim3(blocks), dim3(256), 0, h->stream, g, h->fft_rhs,
+                           h->fft_phi, h->arow, h->fft_res, h->max_part);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_max_final(h, blocks);
+}
+
+// one FP32 solve of L e = res (mean mode excluded), e added to h->fft_phi
+hipError_t launch_fft32_solve_add(mdx_handle *h) {
+    const Dims d = h->d;
+    const int lines = rows_per_block(d.nx), nrows = d.ny * d.nz, nxh = d.nx / 2 + 1;
+    const size_t shx = 2 * (size_t)lines * d.nx * sizeof(float2);
+    hipLaunchKernelGGL(xfft32_kernel, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx, h->stream, d,
+                       h->fft_res, (const float2 *)h->tw_x32, lines, (float2 *)h->hat32);
+    const int tx = z_tile(d.nz);
+    const size_t shz = 2 * (size_t)tx * d.nz * sizeof(float2);
+    hipLaunchKernelGGL(zfft_kernel<float2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
+                       (float2 *)h->hat32, (const float2 *)h->tw_z32, tx, 0);
+    const int nlines = nxh * d.nz;
+    hipLaunchKernelGGL(tridiag32_kernel, dim3((nlines + 63) / 64), dim3(64), 0, h->stream, d, (float2 *)h->hat32,
+                       h->cp32, h->den32, h->arow32);
+    hipLaunchKernelGGL(zfft_kernel<float2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
+                       (float2 *)h->hat32, (const float2 *)h->tw_z32, tx, 1);
+    hipLaunchKernelGGL(ixfft32_add_kernel, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx, h->stream, d,
+                       (const float2 *)h->hat32, (const float2 *)h->tw_x32, lines, 1.0f / (float)(d.nx * d.nz),
+                       h->fft_phi);
+    return hipGetLastError();
+}
+
+// the projection with a given FP64 phi in (nx, nz, ny) layout (step 6)
+hipError_t launch_fft_correct(mdx_handle *h, const double *phi, double dtg) {
+    GridK g = grid_k(h);
+    const Dims d = h->d;
+    dim3 blk(64, 4), grd((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz);
+#define MDX_CORR(S, I)                                                                                     \
+    hipLaunchKernelGGL((fft_correct_kernel<S, I>), grd, blk, 0, h->stream, g, phi, h->us, h->vs, h->ws, h->cu, \
+                       h->cv, h->cw, dtg, h->un, h->vn, h->wn, h->pn)
     if (h->stretched) { if (h->ibm_on) MDX_CORR(true, true); else MDX_CORR(true, false); }
     else { if (h->ibm_on) MDX_CORR(false, true); else MDX_CORR(false, false); }
 #undef MDX_CORR

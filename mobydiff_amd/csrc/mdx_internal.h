@@ -131,6 +131,9 @@ struct mdx_handle {
     // FFT path
     double *fft_rhs = nullptr, *hat = nullptr, *cp = nullptr, *den_inv = nullptr, *arow = nullptr;
     double *tw_x = nullptr, *tw_z = nullptr;
+    // mixed-precision FFT solve: FP64 rhs in fft_rhs, FP64 phi and residual, FP32 spectrum and tables
+    double *fft_phi = nullptr, *fft_res = nullptr, *fft_mean = nullptr;   // fft_mean: ny sums + ny mean phi
+    float *hat32 = nullptr, *cp32 = nullptr, *den32 = nullptr, *arow32 = nullptr, *tw_x32 = nullptr, *tw_z32 = nullptr;
 
     // reductions
     double *red_part = nullptr;    // per-block partial results
@@ -219,6 +222,10 @@ hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init);
 hipError_t launch_fft_forward(mdx_handle *h, double dtg);
 hipError_t launch_tridiag(mdx_handle *h);
 hipError_t launch_fft_backward_correct(mdx_handle *h, double dtg);
+hipError_t launch_fft_mixed_setup(mdx_handle *h, double dtg);
+hipError_t launch_fft_residual(mdx_handle *h);
+hipError_t launch_fft32_solve_add(mdx_handle *h);
+hipError_t launch_fft_correct(mdx_handle *h, const double *phi, double dtg);
 }  // namespace mdx
 
 // error helpers
