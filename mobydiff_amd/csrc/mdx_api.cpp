@@ -921,8 +921,8 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
     mdx_solve_stats st{};
     const mdx_config &c = h->cfg;
     if (c.solver == MDX_SOLVER_FFT && c.precision == MDX_PREC_MIXED) {
-        // FP32 transforms and Thomas solves in an FP64 defect correction: phi starts from the FP64
-        // mean mode; each pass solves L e = rhs - L phi in FP32 (mean mode excluded) and adds e, until
+        // FP32 transforms and Thomas solves in an FP64 defect correction: from phi = 0, each pass
+        // solves L e = rhs - L phi (mean mode in FP64, the rest in FP32) and adds e, until
         // ||div||_inf = dt_gamma ||rhs - L phi||_inf < tol (at most MAX_FFT_PASSES passes)
         constexpr int MAX_FFT_PASSES = 4;
         ev_record(h, 1);

@@ -320,11 +320,11 @@ hipError_t launch_fft_backward_correct(mdx_handle *h, double dtg) {
 }
 
 // ------------------------------------------------------------------------------- mixed precision
-// FP32 transforms and Thomas solves inside an FP64 defect correction.  The mean mode (kx = kz = 0),
-// whose system is singular and pinned by phi_hat(y_1) = 0 (pressure_fft.f90:92-96, 119-121), is
-// solved once in FP64 from the plane sums of the FP64 right-hand side; the FP32 solves carry the
-// other modes of the residual r = rhs - L phi, formed in FP64 with the discrete operator the
-// transforms diagonalise (3-point second differences in x and z, the tridiagonal rows in y).
+// FP32 transforms and Thomas solves inside an FP64 defect correction.  Each pass takes the residual
+// r = rhs - L phi, formed in FP64 with the discrete operator the transforms diagonalise (3-point
+// second differences in x and z, the tridiagonal rows in y).  Its mean mode (kx = kz = 0), whose
+// system is singular and pinned by phi_hat(y_1) = 0 (pressure_fft.f90:92-96, 119-121), is solved in
+// FP64 from the plane sums of r; the other modes go through the FP32 transforms and Thomas solves.
 
 // rhs(i,k,j) of rhs_xfft_kernel, in FP64, layout (nx, nz, ny)
 template <bool STRETCH>
@@ -383,11 +383,11 @@ __global__ void fft_mean_mode_kernel(Dims d, const double *__restrict__ sums, co
     for (int jj = 1; jj < ny; ++jj) mean[jj - 1] *= scale;
 }
 
-// phi = mean[j] everywhere in plane j
-__global__ void fft_fill_mean_kernel(Dims d, const double *__restrict__ mean, double *__restrict__ phi) {
+// phi += mean[j] everywhere in plane j
+__global__ void fft_add_mean_kernel(Dims d, const double *__restrict__ mean, double *__restrict__ phi) {
     const long n = (long)d.nx * d.ny * d.nz;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
-        phi[q] = mean[q / ((long)d.nx * d.nz)];
+        phi[q] += mean[q / ((long)d.nx * d.nz)];
 }
 
 // res = rhs - L phi (FP64), and max |res| per workgroup; layout (nx, nz, ny), x and z periodic
@@ -544,21 +544,17 @@ __global__ __launch_bounds__(FFT_THREADS) void ixfft32_add_kernel(Dims d, const 
     }
 }
 
-// the mean mode and the FP64 right-hand side: rhs -> h->fft_rhs, mean-mode phi -> h->fft_phi
+// the FP64 right-hand side -> h->fft_rhs, phi = 0
 hipError_t launch_fft_mixed_setup(mdx_handle *h, double dtg) {
     GridK g = grid_k(h);
-    const Dims d = h->d;
     if (h->stretched)
         hipLaunchKernelGGL(fft_div_rhs_kernel<true>, dim3(4096), dim3(256), 0, h->stream, g, h->us, h->vs, h->ws, dtg,
                            h->fft_rhs);
     else
         hipLaunchKernelGGL(fft_div_rhs_kernel<false>, dim3(4096), dim3(256), 0, h->stream, g, h->us, h->vs, h->ws,
                            dtg, h->fft_rhs);
-    hipLaunchKernelGGL(fft_plane_sum_kernel, dim3(d.ny), dim3(256), 0, h->stream, d, h->fft_rhs, h->fft_mean);
-    hipLaunchKernelGGL(fft_mean_mode_kernel, dim3(1), dim3(64), 0, h->stream, d, h->fft_mean, h->cp, h->den_inv,
-                       h->arow, 1.0 / (double)(d.nx * d.nz), h->fft_mean + d.ny);
-    hipLaunchKernelGGL(fft_fill_mean_kernel, dim3(4096), dim3(256), 0, h->stream, d, h->fft_mean + d.ny, h->fft_phi);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_zero64(h, h->fft_phi, h->ni);
 }
 
 // res = rhs - L phi into h->fft_res; its max |.| into the max reduction
@@ -575,9 +571,14 @@ hipError_t launch_fft_residual(mdx_handle *h) {
     return e != hipSuccess ? e : launch_max_final(h, blocks);
 }
 
-// one FP32 solve of L e = res (mean mode excluded), e added to h->fft_phi
+// one defect-correction pass on res = rhs - L phi: the mean mode in FP64 (plane sums, pinned
+// Thomas), the other modes by FP32 transforms and Thomas; both corrections added to h->fft_phi
 hipError_t launch_fft32_solve_add(mdx_handle *h) {
     const Dims d = h->d;
+    hipLaunchKernelGGL(fft_plane_sum_kernel, dim3(d.ny), dim3(256), 0, h->stream, d, h->fft_res, h->fft_mean);
+    hipLaunchKernelGGL(fft_mean_mode_kernel, dim3(1), dim3(64), 0, h->stream, d, h->fft_mean, h->cp, h->den_inv,
+                       h->arow, 1.0 / (double)(d.nx * d.nz), h->fft_mean + d.ny);
+    hipLaunchKernelGGL(fft_add_mean_kernel, dim3(4096), dim3(256), 0, h->stream, d, h->fft_mean + d.ny, h->fft_phi);
     const int lines = rows_per_block(d.nx), nrows = d.ny * d.nz, nxh = d.nx / 2 + 1;
     const size_t shx = 2 * (size_t)lines * d.nx * sizeof(float2);
     hipLaunchKernelGGL(xfft32_kernel, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx, h->stream, d,

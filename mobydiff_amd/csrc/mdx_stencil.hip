@@ -196,23 +196,56 @@ __global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
     load_uvw(0, k0 - 1);
     load_uvw(1, k0);
     load_uvw(2, k0 + 1);
+    constexpr int NT_ = MTX * MTY, PER = (MPL + NT_ - 1) / NT_;   // window values per thread
+    for (int q = tid; q < MPL; q += NT_) {
+        const int gi = i0 - 1 + q % MTW, gj = j0 - 1 + q / MTW;
+        if (gi <= d.nx + 1 && gj <= d.ny + 1) sp[q] = pn[d.g(gi, gj, k0)];
+    }
     double pkm = active ? pn[d.g(i, j, k0 - 1)] : 0.0;
+    __syncthreads();
     for (int k = k0; k <= k1; ++k) {
-        for (int q = tid; q < MPL; q += MTX * MTY) {
-            const int gi = i0 - 1 + q % MTW, gj = j0 - 1 + q / MTW;
-            if (gi <= d.nx + 1 && gj <= d.ny + 1) sp[q] = pn[d.g(gi, gj, k)];
+        // the next planes (u, v, w at k+2, p at k+1) are fetched into registers before plane k is
+        // computed, so their latency overlaps the arithmetic; they enter LDS after it
+        double ru[PER], rv[PER], rw[PER], rp[PER];
+        if (k < k1) {
+#pragma unroll
+            for (int c = 0; c < PER; ++c) {
+                const int q = tid + c * NT_;
+                const int gi = i0 - 1 + q % MTW, gj = j0 - 1 + q / MTW;
+                if (q < MPL && gi <= d.nx + 1 && gj <= d.ny + 1) {
+                    const size_t x2 = d.g(gi, gj, k + 2), x1 = d.g(gi, gj, k + 1);
+                    ru[c] = un[x2];
+                    rv[c] = vn[x2];
+                    rw[c] = wn[x2];
+                    rp[c] = pn[x1];
+                }
+            }
         }
-        __syncthreads();
         const int s0 = (k - k0) % 3;   // slot of plane k-1
+        double pnext = 0.0;
         if (active) {
             const MomTile L{su, sv, sw, sp, {s0 * MPL, ((s0 + 1) % 3) * MPL, ((s0 + 2) % 3) * MPL}, t, pkm};
             momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), d.in(i, j, k), d.g(i, j, k * coef_kstride), us,
                                         vs, ws, oru, orv, orw, cu, cv, cw, dta, dtb, dtg, idx, idy, idz, idx2, idy2,
                                         idz2, ire);
-            pkm = sp[t];
+            pnext = sp[t];
         }
         __syncthreads();
-        if (k < k1) load_uvw(s0, k + 2);   // plane k-1 leaves the window, plane k+2 enters
+        if (k < k1) {   // plane k-1 leaves the window, plane k+2 enters; p plane k+1 replaces plane k
+#pragma unroll
+            for (int c = 0; c < PER; ++c) {
+                const int q = tid + c * NT_;
+                const int gi = i0 - 1 + q % MTW, gj = j0 - 1 + q / MTW;
+                if (q < MPL && gi <= d.nx + 1 && gj <= d.ny + 1) {
+                    su[s0 * MPL + q] = ru[c];
+                    sv[s0 * MPL + q] = rv[c];
+                    sw[s0 * MPL + q] = rw[c];
+                    sp[q] = rp[c];
+                }
+            }
+        }
+        pkm = pnext;
+        __syncthreads();
     }
 }
 
