@@ -56,6 +56,21 @@ def run_steps(ch, k):
     return el, its, div, ch.phase_times(), fb
 
 
+# HBM bytes per dispatch of the sweep kernels, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+# workload scaled by the counters' reading of a known-size copy (profiles/sweep_traffic_wavy_192x128x96.json)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep_traffic_wavy_192x128x96.json")
+
+
+def measured_traffic(kind, kz):
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            table = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    entry = table.get("kernels", {}).get(f"{kind}_kz{kz}")
+    return None if entry is None else entry.get("bytes_per_dispatch")
+
+
 def roofline(times, kind):
     if times.sweep_launches == 0:
         return None
@@ -63,9 +78,11 @@ def roofline(times, kind):
     cells_per_launch = times.sweep_cells / times.sweep_launches
     bytes_per_launch = BYTES_PER_CELL_ITER[kind] / 2.0 * cells_per_launch
     ach = bytes_per_launch / avg_s / 1e9
+    kz = int(os.environ.get("MOBYDIFF_SWEEP_KZ", "4"))
     return {"kernel": f"red-black SOR sweep ({kind}, one colour)", "bound": "hbm", "achieved": round(ach, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": bytes_per_launch}
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(kind, kz), "avg_launch_us": round(avg_s * 1e6, 2),
+            "bytes_per_launch": bytes_per_launch}
 
 
 def flat_fft_production(M, steps, precision=None):

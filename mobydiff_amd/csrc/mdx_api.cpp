@@ -868,17 +868,25 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
     }
     st->guess_div_inf = learned ? rg : rp;
     // the chosen start is copied into phi (captured sweep graphs hold that buffer's address)
+    if (learned) {
+        // for the gate, the network's start wins only when it clearly beats the best other start
+        // (residual below NET_MARGIN times it): a marginally smaller start residual saves no
+        // sweeps but costs an inference per solve
+        constexpr double NET_MARGIN = 0.8;
+        const double best_other = use_prev ? std::min(rp, r0) : r0;
+        if (rg < NET_MARGIN * best_other) {
+            h->net_losses[s] = 0;
+        } else {
+            h->net_losses[s] += 1;
+            if (h->net_losses[s] >= 2) h->net_skip[s] = std::min(16, 1 << std::min(h->net_losses[s] - 1, 4));
+        }
+    }
     if (learned && rg < r0 && (!use_prev || rg < rp)) {
         MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
         *r_first = rg;
-        h->net_losses[s] = 0;
         return MDX_OK;
     }
-    if (learned) {
-        st->fallback = 1;
-        h->net_losses[s] += 1;
-        if (h->net_losses[s] >= 2) h->net_skip[s] = std::min(16, 1 << std::min(h->net_losses[s] - 1, 4));
-    }
+    if (learned) st->fallback = 1;
     if (use_prev && rp < r0) {
         if (learned) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));   // phi2 holds the network's
         else MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
@@ -942,6 +950,7 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         st.iterations = passes;
         st.div_inf = r * dtg;
         st.converged = r * dtg < c.tol;
+        MDX_HIP(launch_fft_pin(h));
         ev_record(h, 2);
         MDX_HIP(launch_fft_correct(h, h->fft_phi, dtg));
         ev_record(h, 3);

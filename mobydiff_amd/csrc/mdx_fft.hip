@@ -390,6 +390,15 @@ __global__ void fft_add_mean_kernel(Dims d, const double *__restrict__ mean, dou
         phi[q] += mean[q / ((long)d.nx * d.nz)];
 }
 
+// phi -= (plane-1 sum of phi) / (nx nz): re-imposes the pin phi_hat(y_1) = 0 exactly (the FP32
+// inverse transforms leave plane means of the order of their rounding, and a constant is invisible
+// to the residual); velocities see only differences of phi and do not change
+__global__ void fft_pin_kernel(Dims d, const double *__restrict__ sums, double *__restrict__ phi) {
+    const long n = (long)d.nx * d.ny * d.nz;
+    const double c = sums[0] / (double)((long)d.nx * d.nz);
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) phi[q] -= c;
+}
+
 // res = rhs - L phi (FP64), and max |res| per workgroup; layout (nx, nz, ny), x and z periodic
 template <bool STRETCH>
 __global__ __launch_bounds__(256) void fft_residual_kernel(GridK g, const double *__restrict__ rhs,
@@ -595,6 +604,13 @@ hipError_t launch_fft32_solve_add(mdx_handle *h) {
     hipLaunchKernelGGL(ixfft32_add_kernel, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx, h->stream, d,
                        (const float2 *)h->hat32, (const float2 *)h->tw_x32, lines, 1.0f / (float)(d.nx * d.nz),
                        h->fft_phi);
+    return hipGetLastError();
+}
+
+hipError_t launch_fft_pin(mdx_handle *h) {
+    const Dims d = h->d;
+    hipLaunchKernelGGL(fft_plane_sum_kernel, dim3(1), dim3(256), 0, h->stream, d, h->fft_phi, h->fft_mean);
+    hipLaunchKernelGGL(fft_pin_kernel, dim3(4096), dim3(256), 0, h->stream, d, h->fft_mean, h->fft_phi);
     return hipGetLastError();
 }
 

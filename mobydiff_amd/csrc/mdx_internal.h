@@ -225,6 +225,7 @@ hipError_t launch_fft_backward_correct(mdx_handle *h, double dtg);
 hipError_t launch_fft_mixed_setup(mdx_handle *h, double dtg);
 hipError_t launch_fft_residual(mdx_handle *h);
 hipError_t launch_fft32_solve_add(mdx_handle *h);
+hipError_t launch_fft_pin(mdx_handle *h);
 hipError_t launch_fft_correct(mdx_handle *h, const double *phi, double dtg);
 }  // namespace mdx
 
