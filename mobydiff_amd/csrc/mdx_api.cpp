@@ -293,6 +293,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     h->rb_err = reinterpret_cast<int *>(h->red_max + 1);
     if (const char *ls = std::getenv("MOBYDIFF_LDS_SOR")) h->use_lds_sor = ls[0] == '1';
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
+    if (const char *sf = std::getenv("MOBYDIFF_SWEEP_FUSED")) h->sweep_fused = sf[0] != '0';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
         const int v = std::atoi(kz);
         if (!(v == 0 || v == 1 || v == 2 || v == 4 || v == 8)) {
@@ -611,6 +612,7 @@ static int rb_prepare(mdx_handle *h) {
     if (!h->use_lds_sor) {
         h->rb_note = "streaming sweeps: z-marching pairs, " + std::to_string(h->sweep_kz) + " planes per thread";
         if (h->sweep_kz == 0) h->rb_note = "streaming sweeps: one cell per thread";
+        if (h->sweep_fused) h->rb_note = "fused red+black iterations: 32x8x8 tiles with a two-cell halo in LDS";
         return MDX_OK;
     }
     if (h->stretched) { h->rb_note = "streaming sweeps: stretched grid"; return MDX_OK; }
@@ -678,7 +680,9 @@ static void ev_record(mdx_handle *h, int q) {
 static hipError_t sweep_launches(mdx_handle *h, bool fp32, int m) {
     hipError_t e = hipSuccess;
     for (int q = 0; q < m && e == hipSuccess; ++q) {
-        if (fp32) {
+        if (h->sweep_fused) {
+            e = fp32 ? launch_sor_iter32(h, (float)h->cfg.omega) : launch_sor_iter64(h, h->cfg.omega);
+        } else if (fp32) {
             if ((e = launch_sor_sweep32(h, 0, (float)h->cfg.omega)) == hipSuccess)
                 e = launch_sor_sweep32(h, 1, (float)h->cfg.omega);
         } else {
@@ -734,7 +738,7 @@ static void sweep_account(mdx_handle *h) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, h->ev[5], h->ev[6]) == hipSuccess) {
         h->times.sweep_ms += ms;
-        h->times.sweep_launches += 2 * (int64_t)h->sweep_pending;
+        h->times.sweep_launches += (h->sweep_fused ? 1 : 2) * (int64_t)h->sweep_pending;
         h->times.sweep_cells += 2 * (int64_t)h->sweep_pending * (int64_t)h->ni;
     }
     h->sweep_pending = 0;

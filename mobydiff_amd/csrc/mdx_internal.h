@@ -159,6 +159,8 @@ struct mdx_handle {
     bool tiled_momentum = true;
     // streaming red-black sweeps: z planes marched per thread (0: one cell per thread)
     int sweep_kz = 4;
+    // one launch per red+black iteration (halo-recomputing tiles) instead of one per colour
+    bool sweep_fused = true;
 
     // LDS-resident red-black solver (opt-in, MOBYDIFF_LDS_SOR=1): plans per precision, face buffers,
     // progress counters
@@ -211,6 +213,8 @@ hipError_t launch_sor_residual64(mdx_handle *h, const double *phi);
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg);
 hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32);
 hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega);
+hipError_t launch_sor_iter64(mdx_handle *h, double omega);
+hipError_t launch_sor_iter32(mdx_handle *h, float omega);
 hipError_t launch_zero32(mdx_handle *h, float *p, size_t n);
 hipError_t launch_zero64(mdx_handle *h, double *p, size_t n);
 hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, double fac, size_t n);

@@ -462,6 +462,116 @@ static hipError_t launch_sweep_zm(mdx_handle *h, int color, T omega, const MuK<T
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------- fused iteration
+// One full red-black iteration (colour 0, then colour 1) per launch.  A workgroup owns a
+// FTX x FTY x FTZ tile, stages the tile plus a two-cell halo of Phi in LDS, updates the colour-0
+// cells of the tile plus a one-cell ring (the ring's new values are also computed by the
+// neighbouring workgroups, from the same inputs, so they are identical), then the colour-1 cells
+// of the tile from those values, and writes the tile back.  No workgroup reads another's results
+// within a launch, so no grid-wide synchronisation is needed, and the iteration costs one launch
+// instead of two.  The per-cell arithmetic is sor_residual_core with the operands of cell_residual,
+// so the iterates are bit-identical to two one-colour sweeps.  x and z are periodic (indices
+// wrap); y is periodic or bounded by the walls (cells outside 1..ny are neither loaded nor updated).
+static constexpr int FTX = 32, FTY = 8, FTZ = 8;
+static constexpr int FRX = FTX + 4, FRY = FTY + 4, FRZ = FTZ + 4, FRV = FRX * FRY * FRZ;
+
+template <typename T, bool STRETCH, int MU>
+__global__ __launch_bounds__(256) void sor_iter_fused_kernel(GridK g, MuK<T> mu, SorConst<T> s, T omega,
+                                                             const T *__restrict__ rhs, T *__restrict__ phi) {
+    __shared__ T tile[FRV];
+    const Dims d = g.d;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    int tx_, ty_, tz_;
+    if (!xcd_tile(nx, ny, (nz + FTZ - 1) / FTZ, FTX, FTY, tx_, ty_, tz_)) return;   // uniform per workgroup
+    const int i0 = tx_ * FTX + 1, j0 = ty_ * FTY + 1, k0 = tz_ * FTZ + 1;
+    const bool per2 = g.per[1];
+    // region cell (a, b, c) <-> (i, j, k) = (i0 - 2 + a, j0 - 2 + b, k0 - 2 + c), x and z wrapped
+    auto wrap = [](int v, int n) { return ((v - 1) % n + n) % n + 1; };   // any number of periods
+    auto jmap = [&](int b) {   // y index of region row b, or 0 when outside a walled domain
+        int j = j0 - 2 + b;
+        if (per2) return wrap(j, ny);
+        return (j < 1 || j > ny) ? 0 : j;
+    };
+    for (int q = threadIdx.x; q < FRV; q += 256) {
+        const int a = q % FRX, b = (q / FRX) % FRY, c = q / (FRX * FRY);
+        const int j = jmap(b);
+        if (j) tile[q] = phi[d.in(wrap(i0 - 2 + a, nx), j, wrap(k0 - 2 + c, nz))];
+    }
+    __syncthreads();
+    // one update of region cell (a, b, c); neighbours come from LDS
+    auto update = [&](int a, int b, int c) -> T {
+        const int i = wrap(i0 - 2 + a, nx), j = jmap(b), k = wrap(k0 - 2 + c, nz);
+        const int ip = i == nx ? 1 : i + 1, kp = k == nz ? 1 : k + 1;
+        int jp = j + 1;
+        if (per2 && jp > ny) jp = 1;
+        const bool wn_ = !per2 && j == ny, ws_ = !per2 && j == 1;
+        const int q = a + FRX * (b + FRY * c);
+        T mui = 1, muip = 1, mvj = 1, mvjp = 1, mwk = 1, mwkp = 1;
+        if (MU != 0) {
+            mui = mu.u[mu.at(i, j, k)];
+            muip = mu.u[mu.at(ip, j, k)];
+            mvj = mu.v[mu.at(i, j, k)];
+            mvjp = mu.v[mu.at(i, jp, k)];
+            mwk = mu.w[mu.at(i, j, k)];
+            mwkp = mu.w[mu.at(i, j, kp)];
+        }
+        T an = 0, as = 0;
+        if (STRETCH) {
+            an = (T)(g.ihv[j + 1] * g.ihc[j]);
+            as = (T)(g.ihv[j] * g.ihc[j]);
+        }
+        const T p0 = tile[q];
+        const T pyp = wn_ ? p0 : tile[q + FRX], pym = ws_ ? p0 : tile[q - FRX];
+        T denom;
+        const T r = sor_residual_core<T, STRETCH>(p0, tile[q + 1], tile[q - 1], pyp, pym, tile[q + FRX * FRY],
+                                                  tile[q - FRX * FRY], mui, muip, mvj, mvjp, mwk, mwkp, wn_, ws_,
+                                                  rhs[d.in(i, j, k)], s, an, as, denom);
+        return p0 + r * (-omega / denom);
+    };
+    // colour 0 ((i + j + k) odd) on the tile and a one-cell ring
+    for (int q = threadIdx.x; q < (FRX - 2) * (FRY - 2) * (FRZ - 2); q += 256) {
+        const int a = 1 + q % (FRX - 2), b = 1 + (q / (FRX - 2)) % (FRY - 2), c = 1 + q / ((FRX - 2) * (FRY - 2));
+        const int j = jmap(b);
+        if (!j) continue;
+        const int i = wrap(i0 - 2 + a, nx), k = wrap(k0 - 2 + c, nz);
+        if (((i + j + k) & 1) == 0) continue;
+        tile[a + FRX * (b + FRY * c)] = update(a, b, c);
+    }
+    __syncthreads();
+    // colour 1 on the tile
+    for (int q = threadIdx.x; q < FTX * FTY * FTZ; q += 256) {
+        const int a = 2 + q % FTX, b = 2 + (q / FTX) % FTY, c = 2 + q / (FTX * FTY);
+        const int i = i0 - 2 + a, j = j0 - 2 + b, k = k0 - 2 + c;
+        if (i > nx || j > ny || k > nz) continue;
+        if (((i + j + k) & 1) == 1) continue;
+        tile[a + FRX * (b + FRY * c)] = update(a, b, c);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < FTX * FTY * FTZ; q += 256) {
+        const int a = 2 + q % FTX, b = 2 + (q / FTX) % FTY, c = 2 + q / (FTX * FTY);
+        const int i = i0 - 2 + a, j = j0 - 2 + b, k = k0 - 2 + c;
+        if (i > nx || j > ny || k > nz) continue;
+        phi[d.in(i, j, k)] = tile[a + FRX * (b + FRY * c)];
+    }
+}
+
+template <typename T>
+static hipError_t launch_iter_fused(mdx_handle *h, T omega, const MuK<T> &mu, const SorConst<T> &s, const T *rhs,
+                                    T *phi) {
+    GridK g = grid_k(h);
+    dim3 blk(256), grd(xcd_grid(h->d.nx, h->d.ny, (h->d.nz + FTZ - 1) / FTZ, FTX, FTY));
+    const int mumode = h->ibm_on ? (h->coef_plane ? 1 : 2) : 0;
+#define MDX_FU(S, M) \
+    hipLaunchKernelGGL((sor_iter_fused_kernel<T, S, M>), grd, blk, 0, h->stream, g, mu, s, omega, rhs, phi)
+    if (h->stretched) {
+        if (mumode == 1) MDX_FU(true, 1); else if (mumode == 2) MDX_FU(true, 2); else MDX_FU(true, 0);
+    } else {
+        if (mumode == 1) MDX_FU(false, 1); else if (mumode == 2) MDX_FU(false, 2); else MDX_FU(false, 0);
+    }
+#undef MDX_FU
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------------------- FP64 sweeps
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(SX * SY) void sor_sweep64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
@@ -609,6 +719,19 @@ hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega) {
     else { if (h->ibm_on) MDX_SW(false, true); else MDX_SW(false, false); }
 #undef MDX_SW
     return hipGetLastError();
+}
+
+// one full red+black iteration in one launch (sor_iter_fused_kernel)
+hipError_t launch_sor_iter64(mdx_handle *h, double omega) {
+    MuK<double> mu = mu_k<double>(h, h->mu64);
+    SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    return launch_iter_fused<double>(h, omega, mu, s, h->rhs, h->phi);
+}
+
+hipError_t launch_sor_iter32(mdx_handle *h, float omega) {
+    MuK<float> mu = mu_k<float>(h, h->mu32);
+    SorConst<float> s{(float)(1.0 / (h->dx * h->dx)), (float)(1.0 / (h->dy * h->dy)), (float)(1.0 / (h->dz * h->dz))};
+    return launch_iter_fused<float>(h, omega, mu, s, h->r32, h->e32);
 }
 
 template <typename T>

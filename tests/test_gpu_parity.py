@@ -292,6 +292,7 @@ def test_zmarch_sweeps_bit_exact_fixed_iterations(kz, case, monkeypatch):
     """Streaming sweeps with a thread marching kz planes (nz = 22: the last segment is short)
     against the oracle, bit for bit, for z-invariant (plane) and general 3-D mu tables."""
     monkeypatch.setenv("MOBYDIFF_LDS_SOR", "0")
+    monkeypatch.setenv("MOBYDIFF_SWEEP_FUSED", "0")
     monkeypatch.setenv("MOBYDIFF_SWEEP_KZ", str(kz))
     rng = np.random.default_rng(23)
     if case == "wavy":
@@ -323,9 +324,60 @@ def test_mixed_solve_same_for_sweep_kernels(kz, monkeypatch):
     rng = np.random.default_rng(24)
     f = O.random_fields(g, rng, amp=0.05)
     monkeypatch.setenv("MOBYDIFF_LDS_SOR", "0")
+    monkeypatch.setenv("MOBYDIFF_SWEEP_FUSED", "0")
     outs = []
     for v in ("0", str(kz)):
         monkeypatch.setenv("MOBYDIFF_SWEEP_KZ", v)
+        ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=10,
+                      precision=M.PREC_MIXED, inner_iters=10)
+        st = ch.pressure_projection(5e-4 * 64 / 120)
+        assert st.converged == 1
+        outs.append(ch.download())
+        ch.close()
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+@pytest.mark.parametrize("shape,case", [((48, 32, 22), "wavy"), ((48, 20, 22), "stretched_random_mu"),
+                                        ((40, 16, 22), "periodic_y"), ((16, 12, 8), "wavy"),
+                                        ((192, 128, 96), "wavy")])
+def test_fused_iterations_bit_exact_fixed_iterations(shape, case, monkeypatch):
+    """One launch per red+black iteration (tiles with a recomputed halo) against the oracle, bit
+    for bit: partial tiles in x and z, walls and periodic y, z-invariant and 3-D mu, a grid smaller
+    than one tile's halo region (16 x 12 x 8: the region wraps onto itself)."""
+    monkeypatch.setenv("MOBYDIFF_LDS_SOR", "0")
+    monkeypatch.setenv("MOBYDIFF_SWEEP_FUSED", "1")
+    rng = np.random.default_rng(25)
+    nx, ny, nz = shape
+    if case == "wavy":
+        g, coef = _wavy(nx, ny, nz)
+    elif case == "periodic_y":
+        g, coef = _wavy(nx, ny, nz, per_y=1, ibm=False)
+    else:
+        g = O.Grid(nx, ny, nz, lx=4.0, ly=2.0, lz=2.0, re=2800.0, periodic=(1, 0, 1), beta=1.5)
+        g.bc_type[O.bci(2, 0, 0)] = 1
+        g.bc_type[O.bci(2, 1, 0)] = 1
+        coef = tuple(np.where(rng.random(g.gshape) < 0.2, 0.0, 50.0 * rng.random(g.gshape)) for _ in range(3))
+    f = O.random_fields(g, rng, amp=0.05)
+    n = 7
+    ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-300, max_iter=n, check_every=n)
+    st = ch.pressure_projection(0.02)
+    assert st.iterations == n and "fused" in ch.info(), ch.info()
+    out = ch.download()
+    ch.close()
+    O.projection_sor(g, f, coef, 1.8, 0.0, n, n, 0.02)
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(out[k], f[k]), (k, case, shape)
+
+
+def test_fused_mixed_solve_equals_one_colour_sweeps(monkeypatch):
+    g, coef = _wavy(48, 32, 24)
+    rng = np.random.default_rng(26)
+    f = O.random_fields(g, rng, amp=0.05)
+    monkeypatch.setenv("MOBYDIFF_LDS_SOR", "0")
+    outs = []
+    for fused in ("0", "1"):
+        monkeypatch.setenv("MOBYDIFF_SWEEP_FUSED", fused)
         ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=10,
                       precision=M.PREC_MIXED, inner_iters=10)
         st = ch.pressure_projection(5e-4 * 64 / 120)
