@@ -191,7 +191,7 @@ static hipError_t dalloc(T **p, size_t n) {
 static void free_all(mdx_handle *h) {
     void *ptrs[] = {h->un, h->vn, h->wn, h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw,
                     h->cu, h->cv, h->cw, h->d_ihc, h->d_ihv, h->d_yc, h->phi, h->phi2, h->rhs,
-                    h->e32, h->r32, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
+                    h->e32, h->r32, h->e32b, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
                     h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags, h->phi_hist, h->res64,
                     h->fft_phi, h->fft_res, h->fft_mean, h->hat32, h->cp32, h->den32, h->arow32, h->tw_x32,
@@ -293,7 +293,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     h->rb_err = reinterpret_cast<int *>(h->red_max + 1);
     if (const char *ls = std::getenv("MOBYDIFF_LDS_SOR")) h->use_lds_sor = ls[0] == '1';
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
-    if (const char *sf = std::getenv("MOBYDIFF_SWEEP_FUSED")) h->sweep_fused = sf[0] != '0';
+    if (const char *sf = std::getenv("MOBYDIFF_SWEEP_FUSED")) h->sweep_fused = sf[0] == '1';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
         const int v = std::atoi(kz);
         if (!(v == 0 || v == 1 || v == 2 || v == 4 || v == 8)) {
@@ -311,7 +311,8 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             (e = dalloc(&h->rhs, h->ni)) != hipSuccess)
             return fail(e, "SOR buffers");
         if (cfg->precision == MDX_PREC_MIXED)
-            if ((e = dalloc(&h->e32, h->ni)) != hipSuccess || (e = dalloc(&h->r32, h->ni)) != hipSuccess)
+            if ((e = dalloc(&h->e32, h->ni)) != hipSuccess || (e = dalloc(&h->r32, h->ni)) != hipSuccess ||
+                (e = dalloc(&h->e32b, h->ni)) != hipSuccess)
                 return fail(e, "SOR FP32 buffers");
         if ((e = hipMemset(h->phi, 0, h->ni * 8)) != hipSuccess) return fail(e, "SOR buffers");
         if ((e = dalloc(&h->phi_hist, 3 * h->ni)) != hipSuccess || (e = dalloc(&h->res64, h->ni)) != hipSuccess)
@@ -675,14 +676,13 @@ static void ev_record(mdx_handle *h, int q) {
 }
 
 // A batch of m red+black iterations.  The launch sequence of a batch never changes for a given
-// (precision, m), so it is captured once into a HIP graph and replayed; kernel arguments are
-// device pointers that stay fixed for the life of the handle (mu tables are rewritten in place).
+// (precision, m, iterate buffer), so it is captured once into a HIP graph and replayed; the mu
+// tables are rewritten in place, and a batch is captured again if the iterate buffer changed.
 static hipError_t sweep_launches(mdx_handle *h, bool fp32, int m) {
     hipError_t e = hipSuccess;
+    if (h->sweep_fused) return fp32 ? launch_sor_iters32(h, m, (float)h->cfg.omega) : launch_sor_iters64(h, m, h->cfg.omega);
     for (int q = 0; q < m && e == hipSuccess; ++q) {
-        if (h->sweep_fused) {
-            e = fp32 ? launch_sor_iter32(h, (float)h->cfg.omega) : launch_sor_iter64(h, h->cfg.omega);
-        } else if (fp32) {
+        if (fp32) {
             if ((e = launch_sor_sweep32(h, 0, (float)h->cfg.omega)) == hipSuccess)
                 e = launch_sor_sweep32(h, 1, (float)h->cfg.omega);
         } else {
@@ -709,7 +709,8 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m) {
     }
     if (h->rb_state == 1) {
     } else if (h->use_graphs && m >= 2) {
-        if (!slot.exec || slot.m != m || slot.ibm != h->ibm_on) {
+        const void *iterate = fp32 ? (const void *)h->e32 : (const void *)h->phi;
+        if (!slot.exec || slot.m != m || slot.ibm != h->ibm_on || slot.iterate != iterate) {
             if (slot.exec) { (void)hipGraphExecDestroy(slot.exec); slot.exec = nullptr; }
             hipGraph_t graph = nullptr;
             MDX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -722,6 +723,7 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m) {
             MDX_HIP(e3);
             slot.m = m;
             slot.ibm = h->ibm_on;
+            slot.iterate = iterate;
         }
         MDX_HIP(hipGraphLaunch(slot.exec, h->stream));
     } else {

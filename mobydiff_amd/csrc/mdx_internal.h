@@ -124,6 +124,7 @@ struct mdx_handle {
     // red-black SOR in pressure-correction form
     double *phi = nullptr, *phi2 = nullptr, *rhs = nullptr;   // FP64 interior
     float *e32 = nullptr, *r32 = nullptr;                     // FP32 interior
+    float *e32b = nullptr;                                    // second FP32 iterate (fused iterations)
     double *mu64[3] = {nullptr, nullptr, nullptr};
     float *mu32[3] = {nullptr, nullptr, nullptr};
     double mu_dtg = -1.0;          // dt_gamma the mu tables were built for
@@ -153,6 +154,7 @@ struct mdx_handle {
         hipGraphExec_t exec = nullptr;
         int m = 0;
         bool ibm = false;
+        const void *iterate = nullptr;   // the iterate buffer the batch was captured on
     } graphs[2];
     bool use_graphs = true;
     // momentum through the LDS-tiled kernel (MOBYDIFF_MOMENTUM_DIRECT=1: one thread per cell)
@@ -160,7 +162,7 @@ struct mdx_handle {
     // streaming red-black sweeps: z planes marched per thread (0: one cell per thread)
     int sweep_kz = 4;
     // one launch per red+black iteration (halo-recomputing tiles) instead of one per colour
-    bool sweep_fused = true;
+    bool sweep_fused = false;
 
     // LDS-resident red-black solver (opt-in, MOBYDIFF_LDS_SOR=1): plans per precision, face buffers,
     // progress counters
@@ -213,8 +215,8 @@ hipError_t launch_sor_residual64(mdx_handle *h, const double *phi);
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg);
 hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32);
 hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega);
-hipError_t launch_sor_iter64(mdx_handle *h, double omega);
-hipError_t launch_sor_iter32(mdx_handle *h, float omega);
+hipError_t launch_sor_iters64(mdx_handle *h, int m, double omega);
+hipError_t launch_sor_iters32(mdx_handle *h, int m, float omega);
 hipError_t launch_zero32(mdx_handle *h, float *p, size_t n);
 hipError_t launch_zero64(mdx_handle *h, double *p, size_t n);
 hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, double fac, size_t n);

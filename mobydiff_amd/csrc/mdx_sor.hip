@@ -463,113 +463,196 @@ static hipError_t launch_sweep_zm(mdx_handle *h, int color, T omega, const MuK<T
 }
 
 // ------------------------------------------------------------------------------- fused iteration
-// One full red-black iteration (colour 0, then colour 1) per launch.  A workgroup owns a
-// FTX x FTY x FTZ tile, stages the tile plus a two-cell halo of Phi in LDS, updates the colour-0
-// cells of the tile plus a one-cell ring (the ring's new values are also computed by the
-// neighbouring workgroups, from the same inputs, so they are identical), then the colour-1 cells
-// of the tile from those values, and writes the tile back.  No workgroup reads another's results
-// within a launch, so no grid-wide synchronisation is needed, and the iteration costs one launch
-// instead of two.  The per-cell arithmetic is sor_residual_core with the operands of cell_residual,
-// so the iterates are bit-identical to two one-colour sweeps.  x and z are periodic (indices
-// wrap); y is periodic or bounded by the walls (cells outside 1..ny are neither loaded nor updated).
-static constexpr int FTX = 32, FTY = 8, FTZ = 8;
-static constexpr int FRX = FTX + 4, FRY = FTY + 4, FRZ = FTZ + 4, FRV = FRX * FRY * FRZ;
+// One full red-black iteration (colour 0, then colour 1) per launch, reading the old iterate from
+// src and writing the new one to dst (no workgroup reads what another writes in the same launch).
+// A workgroup owns FPX x-pairs x FPY rows x KZF planes and is (FPX+2) x (FPY+2) threads: thread
+// (tx, ty) owns the x-pair p0-1+tx in row j0-1+ty; the outer ring of threads only computes
+// colour-0 values, for the x/y neighbours of the tile's colour-1 cells.  Every thread first loads
+// all old values its segment needs (as the one-colour sweep does, so the loads overlap), then
+// computes the colour-0 cells of its column, planes k0-1 .. k1+1, publishes planes k0 .. k1 in LDS,
+// and after one barrier updates the colour-1 cells of planes k0 .. k1: x and y neighbours from LDS,
+// z neighbours from its own colour-0 values.  Colour-0 values of the ring and of planes k0-1, k1+1
+// are recomputed by every workgroup that needs them, from the same inputs and with the same
+// arithmetic (sor_residual_core with the operands of cell_residual), so the iterates are
+// bit-identical to two one-colour sweeps.
+static constexpr int FPX = 16, FPY = 8;
+static constexpr int FWX = FPX + 2, FWY = FPY + 2;
 
-template <typename T, bool STRETCH, int MU>
-__global__ __launch_bounds__(256) void sor_iter_fused_kernel(GridK g, MuK<T> mu, SorConst<T> s, T omega,
-                                                             const T *__restrict__ rhs, T *__restrict__ phi) {
-    __shared__ T tile[FRV];
+template <typename T, bool STRETCH, int MU, int KZF>
+__global__ __launch_bounds__(FWX * FWY) void sor_iter_fused_kernel(GridK g, MuK<T> mu, SorConst<T> s, T omega,
+                                                                   const T *__restrict__ rhs,
+                                                                   const T *__restrict__ src, T *__restrict__ dst) {
+    using V = typename Pair<T>::V;
+    __shared__ T red[KZF][FWY][FWX];
     const Dims d = g.d;
-    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int nx = d.nx, ny = d.ny, nz = d.nz, np = nx / 2;
     int tx_, ty_, tz_;
-    if (!xcd_tile(nx, ny, (nz + FTZ - 1) / FTZ, FTX, FTY, tx_, ty_, tz_)) return;   // uniform per workgroup
-    const int i0 = tx_ * FTX + 1, j0 = ty_ * FTY + 1, k0 = tz_ * FTZ + 1;
+    if (!xcd_tile(np, ny, (nz + KZF - 1) / KZF, FPX, FPY, tx_, ty_, tz_)) return;   // uniform per workgroup
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    // pair index and row of this thread (x periodic: any number of periods; y periodic or walled)
+    const int pr = tx_ * FPX - 1 + tx;
+    const int p = ((pr % np) + np) % np;
+    int j = ty_ * FPY + ty;   // = j0 - 1 + ty with j0 = ty_ * FPY + 1
     const bool per2 = g.per[1];
-    // region cell (a, b, c) <-> (i, j, k) = (i0 - 2 + a, j0 - 2 + b, k0 - 2 + c), x and z wrapped
-    auto wrap = [](int v, int n) { return ((v - 1) % n + n) % n + 1; };   // any number of periods
-    auto jmap = [&](int b) {   // y index of region row b, or 0 when outside a walled domain
-        int j = j0 - 2 + b;
-        if (per2) return wrap(j, ny);
-        return (j < 1 || j > ny) ? 0 : j;
-    };
-    for (int q = threadIdx.x; q < FRV; q += 256) {
-        const int a = q % FRX, b = (q / FRX) % FRY, c = q / (FRX * FRY);
-        const int j = jmap(b);
-        if (j) tile[q] = phi[d.in(wrap(i0 - 2 + a, nx), j, wrap(k0 - 2 + c, nz))];
+    bool valid = true;
+    if (per2) j = ((j - 1) % ny + ny) % ny + 1;
+    else valid = j >= 1 && j <= ny;
+    const bool inner = tx >= 1 && tx <= FPX && ty >= 1 && ty <= FPY && pr < np && ty_ * FPY + ty <= ny;
+    const int k0 = tz_ * KZF + 1;
+    const int k1 = k0 + KZF - 1 < nz ? k0 + KZF - 1 : nz;
+    const int nk = k1 - k0 + 1;
+    const int i0 = 2 * p + 1;
+    const int iw = i0 == 1 ? nx : i0 - 1, ie = i0 + 1 == nx ? 1 : i0 + 2;
+    const int jj = valid ? j : 1;   // a safe row for address arithmetic of invalid (never used) rows
+    int jp = jj + 1, jm = jj - 1;
+    if (per2) {
+        if (jp > ny) jp = 1;
+        if (jm < 1) jm = ny;
     }
-    __syncthreads();
-    // one update of region cell (a, b, c); neighbours come from LDS
-    auto update = [&](int a, int b, int c) -> T {
-        const int i = wrap(i0 - 2 + a, nx), j = jmap(b), k = wrap(k0 - 2 + c, nz);
-        const int ip = i == nx ? 1 : i + 1, kp = k == nz ? 1 : k + 1;
-        int jp = j + 1;
-        if (per2 && jp > ny) jp = 1;
-        const bool wn_ = !per2 && j == ny, ws_ = !per2 && j == 1;
-        const int q = a + FRX * (b + FRY * c);
-        T mui = 1, muip = 1, mvj = 1, mvjp = 1, mwk = 1, mwkp = 1;
-        if (MU != 0) {
-            mui = mu.u[mu.at(i, j, k)];
-            muip = mu.u[mu.at(ip, j, k)];
-            mvj = mu.v[mu.at(i, j, k)];
-            mvjp = mu.v[mu.at(i, jp, k)];
-            mwk = mu.w[mu.at(i, j, k)];
-            mwkp = mu.w[mu.at(i, j, kp)];
+    const bool wn_ = !per2 && jj == ny, ws_ = !per2 && jj == 1;
+    const int jn = wn_ ? jj : jp, js = ws_ ? jj : jm;
+    T an = 0, as = 0;
+    if (STRETCH) {
+        an = (T)(g.ihv[jj + 1] * g.ihc[jj]);
+        as = (T)(g.ihv[jj] * g.ihc[jj]);
+    }
+    T mu_u0 = 1, mu_u1 = 1, mu_u2 = 1, mv0 = 1, mv1 = 1, mvp0 = 1, mvp1 = 1, mw0 = 1, mw1 = 1, mwp0 = 1, mwp1 = 1;
+    if (MU == 1) {
+        mu_u0 = mu.u[mu.at(i0, jj, 0)];
+        mu_u1 = mu.u[mu.at(i0 + 1, jj, 0)];
+        mu_u2 = mu.u[mu.at(ie, jj, 0)];
+        mv0 = mu.v[mu.at(i0, jj, 0)];
+        mv1 = mu.v[mu.at(i0 + 1, jj, 0)];
+        mvp0 = mu.v[mu.at(i0, jp, 0)];
+        mvp1 = mu.v[mu.at(i0 + 1, jp, 0)];
+        mw0 = mwp0 = mu.w[mu.at(i0, jj, 0)];
+        mw1 = mwp1 = mu.w[mu.at(i0 + 1, jj, 0)];
+    }
+    const size_t rs = (size_t)nx, ps = (size_t)nx * ny;
+    const size_t rowj = (size_t)(i0 - 1) + rs * (size_t)(jj - 1);
+    const size_t rown = rs * (size_t)(jn - 1), rows = rs * (size_t)(js - 1);
+    const V *S2 = reinterpret_cast<const V *>(src);
+    V *D2 = reinterpret_cast<V *>(dst);
+    auto kw = [&](int k) { return ((k - 1) % nz + nz) % nz + 1; };   // periodic z, any number of periods
+    // the update of cell (i0 + c, j, kk) from its 7 values and the face coefficients
+    auto relax = [&](int kk, int c, T p0, T pin, T pout, T pyp, T pym, T pzp, T pzm, T rc) -> T {
+        if (MU == 2) {
+            const int kp = kk == nz ? 1 : kk + 1;
+            mu_u0 = mu.u[mu.at(i0, jj, kk)];
+            mu_u1 = mu.u[mu.at(i0 + 1, jj, kk)];
+            mu_u2 = mu.u[mu.at(ie, jj, kk)];
+            mv0 = mu.v[mu.at(i0, jj, kk)];
+            mv1 = mu.v[mu.at(i0 + 1, jj, kk)];
+            mvp0 = mu.v[mu.at(i0, jp, kk)];
+            mvp1 = mu.v[mu.at(i0 + 1, jp, kk)];
+            mw0 = mu.w[mu.at(i0, jj, kk)];
+            mw1 = mu.w[mu.at(i0 + 1, jj, kk)];
+            mwp0 = mu.w[mu.at(i0, jj, kp)];
+            mwp1 = mu.w[mu.at(i0 + 1, jj, kp)];
         }
-        T an = 0, as = 0;
-        if (STRETCH) {
-            an = (T)(g.ihv[j + 1] * g.ihc[j]);
-            as = (T)(g.ihv[j] * g.ihc[j]);
-        }
-        const T p0 = tile[q];
-        const T pyp = wn_ ? p0 : tile[q + FRX], pym = ws_ ? p0 : tile[q - FRX];
+        const T pxp = c ? pout : pin, pxm = c ? pin : pout;
+        const T mui = c ? mu_u1 : mu_u0, muip = c ? mu_u2 : mu_u1;
+        const T mvj = c ? mv1 : mv0, mvjp = c ? mvp1 : mvp0;
+        const T mwk = c ? mw1 : mw0, mwkp = c ? mwp1 : mwp0;
         T denom;
-        const T r = sor_residual_core<T, STRETCH>(p0, tile[q + 1], tile[q - 1], pyp, pym, tile[q + FRX * FRY],
-                                                  tile[q - FRX * FRY], mui, muip, mvj, mvjp, mwk, mwkp, wn_, ws_,
-                                                  rhs[d.in(i, j, k)], s, an, as, denom);
+        const T r = sor_residual_core<T, STRETCH>(p0, pxp, pxm, pyp, pym, pzp, pzm, mui, muip, mvj, mvjp, mwk, mwkp,
+                                                  wn_, ws_, rc, s, an, as, denom);
         return p0 + r * (-omega / denom);
     };
-    // colour 0 ((i + j + k) odd) on the tile and a one-cell ring
-    for (int q = threadIdx.x; q < (FRX - 2) * (FRY - 2) * (FRZ - 2); q += 256) {
-        const int a = 1 + q % (FRX - 2), b = 1 + (q / (FRX - 2)) % (FRY - 2), c = 1 + q / ((FRX - 2) * (FRY - 2));
-        const int j = jmap(b);
-        if (!j) continue;
-        const int i = wrap(i0 - 2 + a, nx), k = wrap(k0 - 2 + c, nz);
-        if (((i + j + k) & 1) == 0) continue;
-        tile[a + FRX * (b + FRY * c)] = update(a, b, c);
+    // Colour-0 planes t = 0 .. nk+1 are k = k0-1+t; pair planes q = 0 .. nk+3 are k = k0-2+q.
+    // The ring threads need colour 0 of planes k0 .. k1 only (t = 1 .. nk, pairs q = 1 .. nk+2).
+    V pr2[KZF + 4];
+    T pout[KZF + 2], pyp[KZF + 2], pym[KZF + 2], rc0[KZF + 2], rc1[KZF];
+#pragma unroll
+    for (int q = 0; q < KZF + 4; ++q) {
+        const bool need = valid && q <= nk + 3 && (inner || (q >= 1 && q <= nk + 2));
+        if (need) pr2[q] = S2[(rowj + ps * (size_t)(kw(k0 - 2 + q) - 1)) / 2];
     }
-    __syncthreads();
-    // colour 1 on the tile
-    for (int q = threadIdx.x; q < FTX * FTY * FTZ; q += 256) {
-        const int a = 2 + q % FTX, b = 2 + (q / FTX) % FTY, c = 2 + q / (FTX * FTY);
-        const int i = i0 - 2 + a, j = j0 - 2 + b, k = k0 - 2 + c;
-        if (i > nx || j > ny || k > nz) continue;
-        if (((i + j + k) & 1) == 1) continue;
-        tile[a + FRX * (b + FRY * c)] = update(a, b, c);
+#pragma unroll
+    for (int t = 0; t < KZF + 2; ++t) {
+        const bool need = valid && t <= nk + 1 && (inner || (t >= 1 && t <= nk));
+        if (need) {
+            const int kk = kw(k0 - 1 + t);
+            const int c0 = (jj + kk) & 1, ic = i0 + c0;
+            const size_t pk = ps * (size_t)(kk - 1);
+            pout[t] = src[(size_t)((c0 ? ie : iw) - 1) + rs * (size_t)(jj - 1) + pk];
+            pyp[t] = src[(size_t)(ic - 1) + rown + pk];
+            pym[t] = src[(size_t)(ic - 1) + rows + pk];
+            rc0[t] = rhs[rowj + (size_t)c0 + pk];
+        }
     }
+#pragma unroll
+    for (int u = 0; u < KZF; ++u) {
+        if (inner && u < nk) {
+            const int kk = k0 + u;   // k0 .. k1 lie in 1 .. nz
+            rc1[u] = rhs[rowj + (size_t)((jj + kk + 1) & 1) + ps * (size_t)(kk - 1)];
+        }
+    }
+    // colour 0 of planes k0-1 .. k1+1 (own column): every neighbour is colour 1, so old values
+    T r0[KZF + 2];
+#pragma unroll
+    for (int t = 0; t < KZF + 2; ++t) {
+        r0[t] = 0;
+        const bool need = valid && t <= nk + 1 && (inner || (t >= 1 && t <= nk));
+        if (need) {
+            const int kk = kw(k0 - 1 + t);
+            const int c0 = (jj + kk) & 1;
+            const V cur = pr2[t + 1], nxt = pr2[t + 2], prv = pr2[t];
+            r0[t] = relax(kk, c0, c0 ? cur.y : cur.x, c0 ? cur.x : cur.y, pout[t], pyp[t], pym[t], c0 ? nxt.y : nxt.x,
+                          c0 ? prv.y : prv.x, rc0[t]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < KZF; ++u) red[u][ty][tx] = r0[u + 1];
     __syncthreads();
-    for (int q = threadIdx.x; q < FTX * FTY * FTZ; q += 256) {
-        const int a = 2 + q % FTX, b = 2 + (q / FTX) % FTY, c = 2 + q / (FTX * FTY);
-        const int i = i0 - 2 + a, j = j0 - 2 + b, k = k0 - 2 + c;
-        if (i > nx || j > ny || k > nz) continue;
-        phi[d.in(i, j, k)] = tile[a + FRX * (b + FRY * c)];
+    // colour 1 of planes k0 .. k1 (tile threads)
+    if (!inner) return;
+#pragma unroll
+    for (int u = 0; u < KZF; ++u) {
+        if (u < nk) {
+            const int kk = k0 + u;
+            const int c1 = (jj + kk + 1) & 1;
+            const T pin = r0[u + 1];   // own pair's colour-0 cell
+            const T po = c1 ? red[u][ty][tx + 1] : red[u][ty][tx - 1];
+            const T yp = wn_ ? T(0) : red[u][ty + 1][tx], ym = ws_ ? T(0) : red[u][ty - 1][tx];
+            const V cur = pr2[u + 2];
+            const T bnew = relax(kk, c1, c1 ? cur.y : cur.x, pin, po, yp, ym, r0[u + 2], r0[u], rc1[u]);
+            V out;
+            if (c1) { out.x = pin; out.y = bnew; } else { out.x = bnew; out.y = pin; }
+            D2[(rowj + ps * (size_t)(kk - 1)) / 2] = out;
+        }
     }
 }
 
 template <typename T>
 static hipError_t launch_iter_fused(mdx_handle *h, T omega, const MuK<T> &mu, const SorConst<T> &s, const T *rhs,
-                                    T *phi) {
+                                    const T *src, T *dst) {
     GridK g = grid_k(h);
-    dim3 blk(256), grd(xcd_grid(h->d.nx, h->d.ny, (h->d.nz + FTZ - 1) / FTZ, FTX, FTY));
+    const int kzf = h->sweep_kz == 2 || h->sweep_kz == 8 ? h->sweep_kz : 4;
+    dim3 blk(FWX, FWY), grd(xcd_grid(h->d.nx / 2, h->d.ny, (h->d.nz + kzf - 1) / kzf, FPX, FPY));
     const int mumode = h->ibm_on ? (h->coef_plane ? 1 : 2) : 0;
-#define MDX_FU(S, M) \
-    hipLaunchKernelGGL((sor_iter_fused_kernel<T, S, M>), grd, blk, 0, h->stream, g, mu, s, omega, rhs, phi)
-    if (h->stretched) {
-        if (mumode == 1) MDX_FU(true, 1); else if (mumode == 2) MDX_FU(true, 2); else MDX_FU(true, 0);
-    } else {
-        if (mumode == 1) MDX_FU(false, 1); else if (mumode == 2) MDX_FU(false, 2); else MDX_FU(false, 0);
+#define MDX_FU(S, M, K) \
+    hipLaunchKernelGGL((sor_iter_fused_kernel<T, S, M, K>), grd, blk, 0, h->stream, g, mu, s, omega, rhs, src, dst)
+#define MDX_FUK(S, M)                   \
+    switch (kzf) {                      \
+    case 2: MDX_FU(S, M, 2); break;     \
+    case 8: MDX_FU(S, M, 8); break;     \
+    default: MDX_FU(S, M, 4); break;    \
     }
+    if (h->stretched) {
+        if (mumode == 1) { MDX_FUK(true, 1) } else if (mumode == 2) { MDX_FUK(true, 2) } else { MDX_FUK(true, 0) }
+    } else {
+        if (mumode == 1) { MDX_FUK(false, 1) } else if (mumode == 2) { MDX_FUK(false, 2) } else { MDX_FUK(false, 0) }
+    }
+#undef MDX_FUK
 #undef MDX_FU
     return hipGetLastError();
+}
+
+template <typename T>
+__global__ void copy_kernel(T *__restrict__ dst, const T *__restrict__ src, size_t n) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q < n; q += (size_t)gridDim.x * blockDim.x) dst[q] = src[q];
 }
 
 // ------------------------------------------------------------------------------- FP64 sweeps
@@ -721,17 +804,34 @@ hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega) {
     return hipGetLastError();
 }
 
-// one full red+black iteration in one launch (sor_iter_fused_kernel)
-hipError_t launch_sor_iter64(mdx_handle *h, double omega) {
+// m full red+black iterations, one launch each (sor_iter_fused_kernel), alternating between the
+// iterate and a second buffer; an odd m ends with a copy back, so the result is in the iterate
+hipError_t launch_sor_iters64(mdx_handle *h, int m, double omega) {
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
-    return launch_iter_fused<double>(h, omega, mu, s, h->rhs, h->phi);
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < m && e == hipSuccess; ++q)
+        e = (q & 1) ? launch_iter_fused<double>(h, omega, mu, s, h->rhs, h->phi2, h->phi)
+                    : launch_iter_fused<double>(h, omega, mu, s, h->rhs, h->phi, h->phi2);
+    if (e == hipSuccess && (m & 1)) {
+        hipLaunchKernelGGL(copy_kernel<double>, dim3(2048), dim3(256), 0, h->stream, h->phi, h->phi2, h->ni);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
-hipError_t launch_sor_iter32(mdx_handle *h, float omega) {
+hipError_t launch_sor_iters32(mdx_handle *h, int m, float omega) {
     MuK<float> mu = mu_k<float>(h, h->mu32);
     SorConst<float> s{(float)(1.0 / (h->dx * h->dx)), (float)(1.0 / (h->dy * h->dy)), (float)(1.0 / (h->dz * h->dz))};
-    return launch_iter_fused<float>(h, omega, mu, s, h->r32, h->e32);
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < m && e == hipSuccess; ++q)
+        e = (q & 1) ? launch_iter_fused<float>(h, omega, mu, s, h->r32, h->e32b, h->e32)
+                    : launch_iter_fused<float>(h, omega, mu, s, h->r32, h->e32, h->e32b);
+    if (e == hipSuccess && (m & 1)) {
+        hipLaunchKernelGGL(copy_kernel<float>, dim3(2048), dim3(256), 0, h->stream, h->e32, h->e32b, h->ni);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 template <typename T>
