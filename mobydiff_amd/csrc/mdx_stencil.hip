@@ -13,14 +13,31 @@ static constexpr int BY = 4;
 // (128 B/cell with IBM coefficients, 104 B without).  The stencil values come through an accessor
 // L (L.u(a, b, c) = un(i+a, j+b, k+c), likewise v, w, p), so the direct and the LDS-tiled kernels
 // evaluate the same expressions and round identically.
+// Per-cell operands that are not stencil values: the previous stage's right-hand sides and the
+// immersed-boundary coefficients of the three faces (read by the caller, so they can be prefetched).
+struct MomOld {
+    double ru, rv, rw, cu, cv, cw;
+};
+
+template <bool IBM>
+__device__ inline MomOld mom_old(const double *__restrict__ oru, const double *__restrict__ orv,
+                                 const double *__restrict__ orw, const double *__restrict__ cu,
+                                 const double *__restrict__ cv, const double *__restrict__ cw, size_t o, size_t cc) {
+    MomOld m{oru[o], orv[o], orw[o], 0.0, 0.0, 0.0};
+    if (IBM) {
+        m.cu = cu[cc];
+        m.cv = cv[cc];
+        m.cw = cw[cc];
+    }
+    return m;
+}
+
 template <bool STRETCH, bool IBM, class Acc>
 __device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j, int k, size_t c, size_t o,
-                                     size_t cc, double *__restrict__ us, double *__restrict__ vs,
+                                     const MomOld &m, double *__restrict__ us, double *__restrict__ vs,
                                      double *__restrict__ ws, double *__restrict__ oru, double *__restrict__ orv,
-                                     double *__restrict__ orw, const double *__restrict__ cu,
-                                     const double *__restrict__ cv, const double *__restrict__ cw, double dta,
-                                     double dtb, double dtg, double idx, double idy, double idz, double idx2,
-                                     double idy2, double idz2, double ire) {
+                                     double *__restrict__ orw, double dta, double dtb, double dtg, double idx,
+                                     double idy, double idz, double idx2, double idy2, double idz2, double ire) {
     const int nx0 = g.per[0] ? 1 : 2, ny0 = g.per[1] ? 1 : 2, nz0 = g.per[2] ? 1 : 2;
     const int jp = j + 1, jm = j - 1;
     const double *ihc = g.ihc, *ihv = g.ihv;
@@ -47,8 +64,8 @@ __device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j,
         double dpx = (L.p(0, 0, 0) - L.p(-1, 0, 0)) * idx;
         double rhsu = -(uu_p - uu_m) * idx - cy - (uw_p - uw_m) * idz + g.fx +
                       ire * (diff_ux + diff_uy + diff_uz);
-        double s = u0 + dta * rhsu + dtb * oru[o] - dtg * dpx;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cu[cc]));
+        double s = u0 + dta * rhsu + dtb * m.ru - dtg * dpx;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * m.cu));
         us[c] = s;
         oru[o] = rhsu;
     }
@@ -76,8 +93,8 @@ __device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j,
         double diff_vz = (vkm - 2.0 * v0 + vkp) * idz2;
         double rhsv = -(vu_p - vu_m) * idx - cy - (vw_p - vw_m) * idz + g.fy +
                       ire * (diff_vx + diff_vy + diff_vz);
-        double s = v0 + dta * rhsv + dtb * orv[o] - dtg * dpy;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cv[cc]));
+        double s = v0 + dta * rhsv + dtb * m.rv - dtg * dpy;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * m.cv));
         vs[c] = s;
         orv[o] = rhsv;
     }
@@ -104,8 +121,8 @@ __device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j,
         double dpz = (L.p(0, 0, 0) - L.p(0, 0, -1)) * idz;
         double rhsw = -(wu_p - wu_m) * idx - cy - (ww_p - ww_m) * idz + g.fz +
                       ire * (diff_wx + diff_wy + diff_wz);
-        double s = w0 + dta * rhsw + dtb * orw[o] - dtg * dpz;
-        if (IBM) s = s * (1.0 / (1.0 + dtg * cw[cc]));
+        double s = w0 + dta * rhsw + dtb * m.rw - dtg * dpz;
+        if (IBM) s = s * (1.0 / (1.0 + dtg * m.cw));
         ws[c] = s;
         orw[o] = rhsw;
     }
@@ -140,8 +157,10 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
     if (i > d.nx || j > d.ny) return;
     const MomGlobal L{un, vn, wn, pn, d, i, j, k};
     // coefficient index: z-invariant geometries keep one x-y plane (coef_kstride 0)
-    momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), d.in(i, j, k), d.g(i, j, k * coef_kstride), us, vs, ws,
-                                oru, orv, orw, cu, cv, cw, dta, dtb, dtg, idx, idy, idz, idx2, idy2, idz2, ire);
+    const size_t o = d.in(i, j, k);
+    const MomOld m = mom_old<IBM>(oru, orv, orw, cu, cv, cw, o, d.g(i, j, k * coef_kstride));
+    momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), o, m, us, vs, ws, oru, orv, orw, dta, dtb, dtg, idx, idy,
+                                idz, idx2, idy2, idz2, ire);
 }
 
 // LDS-tiled momentum: a 64 x 8 workgroup owns a 64 x 8 column of cells and marches MKZ planes in z.
@@ -202,11 +221,17 @@ __global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
         if (gi <= d.nx + 1 && gj <= d.ny + 1) sp[q] = pn[d.g(gi, gj, k0)];
     }
     double pkm = active ? pn[d.g(i, j, k0 - 1)] : 0.0;
+    // the per-cell operands of plane k0 (the next plane's are fetched while a plane is computed)
+    MomOld m{};
+    if (active) m = mom_old<IBM>(oru, orv, orw, cu, cv, cw, d.in(i, j, k0), d.g(i, j, k0 * coef_kstride));
     __syncthreads();
     for (int k = k0; k <= k1; ++k) {
-        // the next planes (u, v, w at k+2, p at k+1) are fetched into registers before plane k is
-        // computed, so their latency overlaps the arithmetic; they enter LDS after it
+        // the next planes (u, v, w at k+2, p at k+1, the per-cell operands at k+1) are fetched into
+        // registers before plane k is computed, so their latency overlaps the arithmetic; the window
+        // values enter LDS after it
         double ru[PER], rv[PER], rw[PER], rp[PER];
+        MomOld mn{};
+        if (active && k < k1) mn = mom_old<IBM>(oru, orv, orw, cu, cv, cw, d.in(i, j, k + 1), d.g(i, j, (k + 1) * coef_kstride));
         if (k < k1) {
 #pragma unroll
             for (int c = 0; c < PER; ++c) {
@@ -225,9 +250,8 @@ __global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
         double pnext = 0.0;
         if (active) {
             const MomTile L{su, sv, sw, sp, {s0 * MPL, ((s0 + 1) % 3) * MPL, ((s0 + 2) % 3) * MPL}, t, pkm};
-            momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), d.in(i, j, k), d.g(i, j, k * coef_kstride), us,
-                                        vs, ws, oru, orv, orw, cu, cv, cw, dta, dtb, dtg, idx, idy, idz, idx2, idy2,
-                                        idz2, ire);
+            momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), d.in(i, j, k), m, us, vs, ws, oru, orv, orw, dta,
+                                        dtb, dtg, idx, idy, idz, idx2, idy2, idz2, ire);
             pnext = sp[t];
         }
         __syncthreads();
@@ -245,6 +269,7 @@ __global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
             }
         }
         pkm = pnext;
+        m = mn;
         __syncthreads();
     }
 }
