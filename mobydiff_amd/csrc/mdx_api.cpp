@@ -193,7 +193,7 @@ static void free_all(mdx_handle *h) {
                     h->cu, h->cv, h->cw, h->d_ihc, h->d_ihv, h->d_yc, h->phi, h->phi2, h->rhs,
                     h->e32, h->r32, h->e32b, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
-                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->rb_flags, h->phi_hist, h->res64,
+                    h->red_part, h->red_max, h->max_part, h->rb_faces, h->phi_hist, h->res64,
                     h->fft_phi, h->fft_res, h->fft_mean, h->hat32, h->cp32, h->den32, h->arow32, h->tw_x32,
                     h->tw_z32};
     for (auto &s : h->graphs)
@@ -291,7 +291,16 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if ((e = dalloc(&h->red_max, 4)) != hipSuccess || (e = hipMemset(h->red_max, 0, 32)) != hipSuccess)
         return fail(e, "reduction buffer");
     h->rb_err = reinterpret_cast<int *>(h->red_max + 1);
-    if (const char *ls = std::getenv("MOBYDIFF_LDS_SOR")) h->use_lds_sor = ls[0] == '1';
+    if (const char *ls = std::getenv("MOBYDIFF_LDS_SOR")) {
+        const int v = std::atoi(ls);
+        if (v < 0 || v > 2) {
+            mdx_set_error("mdx_create: MOBYDIFF_LDS_SOR must be 0, 1 or 2");
+            free_all(h);
+            delete h;
+            return MDX_ERR_ARG;
+        }
+        h->lds_sor_mode = v;
+    }
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
     if (const char *sf = std::getenv("MOBYDIFF_SWEEP_FUSED")) h->sweep_fused = sf[0] == '1';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
@@ -610,7 +619,7 @@ static int read_max(mdx_handle *h, double *out) {
 static int rb_prepare(mdx_handle *h) {
     if (h->rb_state != 0) return MDX_OK;
     h->rb_state = -1;
-    if (!h->use_lds_sor) {
+    if (h->lds_sor_mode == 0) {
         h->rb_note = "streaming sweeps: z-marching pairs, " + std::to_string(h->sweep_kz) + " planes per thread";
         if (h->sweep_kz == 0) h->rb_note = "streaming sweeps: one cell per thread";
         if (h->sweep_fused) h->rb_note = "fused red+black iterations: 32x8x8 tiles with a two-cell halo in LDS";
@@ -620,29 +629,31 @@ static int rb_prepare(mdx_handle *h) {
     if (h->ibm_on && !h->coef_plane) { h->rb_note = "streaming sweeps: IBM coefficients vary in z"; return MDX_OK; }
     RbPlanOut p64{}, p32{};
     std::string why;
-    if (!rb_plan(h, false, p64, why) || !rb_plan(h, true, p32, why)) {
+    if (!rb_plan(h, true, p32, why)) {
         h->rb_note = "streaming sweeps: " + why;
         return MDX_OK;
+    }
+    std::string note64;
+    if (h->lds_sor_mode == 2 && !rb_plan(h, false, p64, why)) {
+        h->lds_sor_mode = 1;   // FP64 batches stay on the streaming sweeps
+        note64 = "; fp64: streaming (" + why + ")";
     }
     auto desc = [](const RbPlanOut &p) {
         return std::to_string(p.px) + "x" + std::to_string(p.py) + "x" + std::to_string(p.pz) + " workgroups of " +
                std::to_string(p.bx) + "x" + std::to_string(p.by) + "x" + std::to_string(p.bz) + " cells, LDS " +
                std::to_string(p.lds) + " B";
     };
-    h->rb_note = "LDS-resident sweeps: fp64 " + desc(p64) + "; fp32 " + desc(p32);
-    size_t need = 0;
-    for (const RbPlanOut *p : {&p64, &p32}) {
-        const size_t fmax = std::max((size_t)p->by * p->bz, std::max((size_t)p->bx * p->bz, (size_t)p->bx * p->by));
-        const size_t nwg = (size_t)p->px * p->py * p->pz;
-        need = std::max(need, nwg * 12 * fmax * (p == &p64 ? 8 : 4));
-    }
-    const size_t nflags = std::max((size_t)p64.px * p64.py * p64.pz, (size_t)p32.px * p32.py * p32.pz);
+    h->rb_note = "LDS-resident sweeps (two-deep halo, one exchange per iteration): fp32 " + desc(p32);
+    if (h->lds_sor_mode == 2) h->rb_note += "; fp64 " + desc(p64);
+    else if (!note64.empty()) h->rb_note += note64;
+    else h->rb_note += "; fp64: streaming z-marching pairs, " + std::to_string(h->sweep_kz) + " planes per thread";
+    size_t need = rb_faces_bytes(p32, true);
+    if (h->lds_sor_mode == 2) need = std::max(need, rb_faces_bytes(p64, false));
     if (h->rb_faces) { (void)hipFree(h->rb_faces); h->rb_faces = nullptr; }
-    if (h->rb_flags) { (void)hipFree(h->rb_flags); h->rb_flags = nullptr; }
     MDX_HIP(hipMalloc(&h->rb_faces, need));
-    MDX_HIP(hipMalloc((void **)&h->rb_flags, nflags * sizeof(unsigned long long)));
-    MDX_HIP(hipMemset(h->rb_flags, 0, nflags * sizeof(unsigned long long)));
-    h->rb_epoch = 0;
+    MDX_HIP(hipMemset(h->rb_faces, 0, need));   // tag 0 is never published
+    h->rb_faces_bytes = need;
+    h->rb_tag = 0;
     h->rb_plan64 = p64;
     h->rb_plan32 = p32;
     h->rb_state = 1;
@@ -693,10 +704,13 @@ static hipError_t sweep_launches(mdx_handle *h, bool fp32, int m) {
     return e;
 }
 
+// whether batches of this precision run on the LDS-resident solver
+static bool lds_for(const mdx_handle *h, bool fp32) { return h->rb_state == 1 && (fp32 || h->lds_sor_mode == 2); }
+
 static int sweep_batch(mdx_handle *h, bool fp32, int m) {
     ev_record(h, 5);
     auto &slot = h->graphs[fp32 ? 1 : 0];
-    if (h->rb_state == 1) {
+    if (lds_for(h, fp32)) {
         // LDS-resident sweeps; the FP32 defect solve starts from zero inside the kernel
         hipError_t e = launch_rb_lds(h, fp32, m, fp32 ? 1 : 0);
         if (e != hipSuccess) {
@@ -707,7 +721,7 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m) {
             if (fp32) MDX_HIP(launch_zero32(h, h->e32, h->ni));   // the streaming sweeps start from e = 0
         }
     }
-    if (h->rb_state == 1) {
+    if (lds_for(h, fp32)) {
     } else if (h->use_graphs && m >= 2) {
         const void *iterate = fp32 ? (const void *)h->e32 : (const void *)h->phi;
         if (!slot.exec || slot.m != m || slot.ibm != h->ibm_on || slot.iterate != iterate) {
@@ -773,7 +787,7 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
     st->checks = checks;
     st->div_inf = r;
     st->converged = r < c.tol;
-    st->lds_sweeps = h->rb_state == 1;
+    st->lds_sweeps = lds_for(h, false);
     return MDX_OK;
 }
 
@@ -793,7 +807,7 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
         if (checks == 0) st->div0_inf = r;
         ++checks;
         if (!(r >= c.tol) || it >= c.max_iter) break;
-        if (h->rb_state != 1) MDX_HIP(launch_zero32(h, h->e32, h->ni));
+        if (!lds_for(h, true)) MDX_HIP(launch_zero32(h, h->e32, h->ni));
         const int m = std::min(c.inner_iters, c.max_iter - it);
         if ((rc = sweep_batch(h, true, m))) return rc;
         it += m;
@@ -802,7 +816,7 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     st->checks = checks;
     st->div_inf = r;
     st->converged = r < c.tol;
-    st->lds_sweeps = h->rb_state == 1;
+    st->lds_sweeps = lds_for(h, true);
     return MDX_OK;
 }
 
@@ -1139,8 +1153,8 @@ extern "C" int mdx_step_batch(mdx_handle *const *hs, int32_t n, double dt, mdx_s
     }
     for (int q = 0; q < n; ++q) {
         mdx_handle *h = hs[q];
-        if (h->use_lds_sor) {
-            h->use_lds_sor = false;
+        if (h->lds_sor_mode != 0) {
+            h->lds_sor_mode = 0;
             h->rb_state = -1;
             h->rb_note = "streaming sweeps: concurrent batch";
         }

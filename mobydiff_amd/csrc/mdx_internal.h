@@ -94,6 +94,7 @@ __device__ inline void block_max_part(double v, double *part) {
 struct RbPlanOut {
     int px, py, pz, bx, by, bz, maxp;
     size_t lds;
+    int total;   // halo values exchanged per workgroup and iteration
 };
 
 }  // namespace mdx
@@ -164,15 +165,15 @@ struct mdx_handle {
     // one launch per red+black iteration (halo-recomputing tiles) instead of one per colour
     bool sweep_fused = false;
 
-    // LDS-resident red-black solver (opt-in, MOBYDIFF_LDS_SOR=1): plans per precision, face buffers,
-    // progress counters
-    bool use_lds_sor = false;
+    // LDS-resident red-black solver (MOBYDIFF_LDS_SOR): 0 off, 1 FP32 sweep batches (the default),
+    // 2 FP32 and FP64 batches; plans per precision and the tagged face buffers
+    int lds_sor_mode = 1;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
     mdx::RbPlanOut rb_plan64{}, rb_plan32{};
-    void *rb_faces = nullptr;
-    unsigned long long *rb_flags = nullptr;
+    void *rb_faces = nullptr;      // tagged face words of the LDS-resident sweeps
+    size_t rb_faces_bytes = 0;
     int *rb_err = nullptr;         // points into red_max[1]
-    unsigned long long rb_epoch = 0;
+    unsigned long long rb_tag = 0; // half-sweeps published so far (the tag of the last one)
     std::string rb_note;           // why the streaming sweeps are in use, or the chosen plan
 
     // learned guess (attached network) and training-pair capture
@@ -223,6 +224,7 @@ hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, do
 hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *field);
 bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init);
+size_t rb_faces_bytes(const RbPlanOut &p, bool fp32);
 
 // FFT path
 hipError_t launch_fft_forward(mdx_handle *h, double dtg);

@@ -1,15 +1,26 @@
-// mdx_sor_lds.hip -- LDS-resident red-black SOR for gfx950 (multi-sweep temporal blocking).
+// mdx_sor_lds.hip -- LDS-resident red-black SOR for gfx950 with multi-sweep temporal blocking.
 //
 // The streaming sweeps (mdx_sor.hip) read and write the whole grid once per colour; on the wavy
-// channel (2.4 M cells) that is a few microseconds of HBM/MALL traffic but each launch also pays
-// dispatch latency, so an iteration costs ~35 us.  Here every workgroup owns one subdomain for a
-// whole batch of iterations: its Phi tile (with a one-cell halo) lives in LDS, its right-hand side
-// in registers, and its face coefficients in an LDS table (the IBM geometry is z-invariant, so
-// one x-y plane of mu suffices).  After each half-sweep a workgroup publishes its six boundary
-// layers to global memory, raises a per-workgroup counter, waits for its (at most six)
-// neighbours' counters and pulls their layers into its halo.  All workgroups must be co-resident:
-// the launch is cooperative (the runtime refuses a grid that does not fit) and every wait is
-// bounded (a spin limit sets an error flag and the kernel runs to completion).
+// channel (2.4 M cells) that is a few microseconds of HBM/MALL traffic, but every launch also pays
+// dispatch latency.  Here every workgroup owns one subdomain for a whole batch of iterations: its
+// Phi tile lives in LDS with a halo two cells deep, its right-hand side in registers, and its face
+// coefficients in an LDS table (the IBM geometry is z-invariant, so one x-y plane of mu suffices).
+//
+// Temporal blocking: an iteration relaxes colour 0 on the subdomain AND on the one-cell ring of
+// face cells around it (recomputing what the neighbours compute for those cells, from the same old
+// values, so the results are identical), then colour 1 on the subdomain, whose neighbours are then
+// all up to date.  One halo exchange per iteration therefore suffices instead of one per colour.
+// The ring cells read the second halo layer and the halo edge lines, so a workgroup exchanges with
+// up to 18 neighbours: 6 faces two layers deep and 12 edges one cell thick.
+//
+// Exchange: every published word carries the number of the iteration it belongs to (a 32-bit
+// payload and a 32-bit tag in one 64-bit agent-scope atomic store), and a consumer polls the face
+// data itself until every word it needs shows the expected tag: one store and one load per
+// hand-off, with no acknowledgement wait and no separate counter.  Faces are double-buffered by
+// iteration parity; a producer cannot overwrite a buffer before its consumers have read it,
+// because it first needs their next layers.  All workgroups must be co-resident: the launch is
+// cooperative (the runtime refuses a grid that does not fit) and every wait is bounded (a spin
+// limit sets an error flag and the kernel runs to completion).
 //
 // The arithmetic per cell is sor_residual_core(), the same expressions the streaming sweeps use,
 // and the relaxation factor -omega/denom is formed from the same values, so the iterates are
@@ -19,12 +30,15 @@
 namespace mdx {
 
 static constexpr int RB_THREADS = 512;
+static constexpr int RB_HE = 12;     // exchanged values per thread (a subdomain's halo <= RB_HE * RB_THREADS)
+static constexpr int RB_MAXR = 4;    // colour-0 ring cells per thread
+static constexpr int RB_H = 2;       // halo depth
 
 template <typename T>
 struct RbArgs {
     Dims d;
     int px, py, pz;          // workgroups along x, y, z
-    int bx, by, bz;          // subdomain size (bx even)
+    int bx, by, bz;          // subdomain size (all even)
     int per_y;
     SorConst<T> s;
     T omega;
@@ -33,179 +47,375 @@ struct RbArgs {
     const T *rhs;
     int zero_init;           // x starts at zero (the defect equation)
     int iters;               // full red+black iterations
-    T *faces;                // [wg][face][buffer][fmax]
-    int fmax;
-    unsigned long long *flags;   // per-workgroup progress counters
-    unsigned long long epoch;    // counters of this launch are epoch*2^32 + half-sweeps done
+    unsigned long long *faces;   // [wg][buffer][exchange entry][words per value], tagged words
+    int total;               // exchange entries per workgroup (faces two deep, then edges)
+    unsigned tag0;           // iteration s of this launch publishes tag tag0 + s + 1 (never 0)
     int *err;
 };
+
+// a value as tagged 64-bit words: the tag in the high half, 32 payload bits in the low half
+template <typename T> struct Tagged;
+template <> struct Tagged<float> {
+    static constexpr int NW = 1;
+    __device__ static void pack(float v, unsigned tag, unsigned long long *o) {
+        o[0] = ((unsigned long long)tag << 32) | __float_as_uint(v);
+    }
+    __device__ static float unpack(const unsigned long long *o) { return __uint_as_float((unsigned)o[0]); }
+};
+template <> struct Tagged<double> {
+    static constexpr int NW = 2;
+    __device__ static void pack(double v, unsigned tag, unsigned long long *o) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        o[0] = ((unsigned long long)tag << 32) | (b & 0xffffffffull);
+        o[1] = ((unsigned long long)tag << 32) | (b >> 32);
+    }
+    __device__ static double unpack(const unsigned long long *o) {
+        return __longlong_as_double((long long)((o[1] << 32) | (o[0] & 0xffffffffull)));
+    }
+};
+
+// Exchange direction q: 0-5 faces (x-, x+, y-, y+, z-, z+), two layers deep; 6-17 edges, one cell.
+__host__ __device__ inline void rb_dir(int q, int &dx, int &dy, int &dz) {
+    dx = dy = dz = 0;
+    if (q < 6) {
+        const int sgn = (q & 1) ? 1 : -1;
+        if (q < 2) dx = sgn; else if (q < 4) dy = sgn; else dz = sgn;
+        return;
+    }
+    const int e = q - 6, s1 = (e & 1) ? 1 : -1, s2 = (e & 2) ? 1 : -1;
+    if (e < 4) { dx = s1; dy = s2; }          // x-y edges (lines along z)
+    else if (e < 8) { dx = s1; dz = s2; }     // x-z edges (lines along y)
+    else { dy = s1; dz = s2; }                // y-z edges (lines along x)
+}
+
+// extents of direction q's region: the subdomain size along axes where the direction is 0, else
+// the depth (2 for faces, 1 for edges)
+__host__ __device__ inline void rb_extent(int q, int bx, int by, int bz, int &ex, int &ey, int &ez) {
+    int dx, dy, dz;
+    rb_dir(q, dx, dy, dz);
+    const int h = q < 6 ? RB_H : 1;
+    ex = dx ? h : bx;
+    ey = dy ? h : by;
+    ez = dz ? h : bz;
+}
+
+__host__ __device__ inline int rb_xchg_total(int bx, int by, int bz) {
+    int t = 0;
+    for (int q = 0; q < 18; ++q) {
+        int ex, ey, ez;
+        rb_extent(q, bx, by, bz, ex, ey, ez);
+        t += ex * ey * ez;
+    }
+    return t;
+}
+
+// colour-0 ring cells: on each face ring, half of by*bz (x faces), bx*bz (y) or bx*by (z) cells
+__host__ __device__ inline int rb_ring_total(int bx, int by, int bz) { return by * bz + bx * bz + bx * by; }
 
 template <typename T, bool IBM, int MAXP>
 __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     extern __shared__ unsigned char smem_raw[];
+    using TW = Tagged<T>;
+    constexpr int NW = TW::NW;
     const Dims d = a.d;
     const int bx = a.bx, by = a.by, bz = a.bz;
-    const int tx = bx + 2, ty = by + 2;
+    const int tx = bx + 2 * RB_H, ty = by + 2 * RB_H, tz = bz + 2 * RB_H;
+    const int cw = bx + 2, cpl = (bx + 2) * (by + 2);   // coefficient table: local i, j in [0, b+1]
     T *X = reinterpret_cast<T *>(smem_raw);
-    T *C = X + (size_t)tx * ty * (bz + 2);          // 6 values per plane cell when IBM
+    T *C = X + (size_t)tx * ty * tz;   // IBM: 6 planes of cpl values (planes: threads read at pair stride)
     const int nwg = a.px * a.py * a.pz;
     // XCD-aware placement: workgroups are dispatched round-robin over the 8 XCDs, so consecutive
-    // subdomain ids (neighbours along z first) are given to workgroups on the same XCD.
+    // subdomain ids (neighbours along x first) are given to workgroups on the same XCD.
     const int per_xcd = nwg / 8;
     const int b = blockIdx.x;
     const int w = (nwg % 8 == 0) ? (b % 8) * per_xcd + b / 8 : b;
     const int wx = w % a.px, wy = (w / a.px) % a.py, wz = w / (a.px * a.py);
     const int ox = wx * bx, oy = wy * by, oz = wz * bz;
     const int t = threadIdx.x;
-    auto L = [&](int i, int j, int k) { return i + tx * (j + ty * k); };
+    const bool per_y = a.per_y != 0;
+    // LDS index of local cell (i, j, k); the subdomain is 1..b, the halo -1..0 and b+1..b+2
+    auto L = [&](int i, int j, int k) { return (i + 1) + tx * ((j + 1) + ty * (k + 1)); };
+    auto wrap = [](int g, int n) { return ((g - 1) % n + n) % n + 1; };
+    // the workgroup in direction (dx, dy, dz), or -1 beyond a wall
+    auto neighbour = [&](int dx, int dy, int dz) {
+        int ny_ = wy + dy;
+        if (ny_ < 0 || ny_ >= a.py) {
+            if (!per_y) return -1;
+            ny_ = (ny_ + a.py) % a.py;
+        }
+        return (wx + dx + a.px) % a.px + a.px * (ny_ + a.py * ((wz + dz + a.pz) % a.pz));
+    };
+    auto wall_bits = [&](int gj) { return (!per_y && gj == d.ny ? 1 : 0) | (!per_y && gj == 1 ? 2 : 0); };
 
-    // neighbours: 0 x-, 1 x+, 2 y-, 3 y+, 4 z-, 5 z+ (periodic in x and z; y periodic or walls)
-    auto wid = [&](int ix, int iy, int iz) { return ix + a.px * (iy + a.py * iz); };
-    int nb[6];
-    nb[0] = wid((wx + a.px - 1) % a.px, wy, wz);
-    nb[1] = wid((wx + 1) % a.px, wy, wz);
-    nb[2] = (wy > 0 || a.per_y) ? wid(wx, (wy + a.py - 1) % a.py, wz) : -1;
-    nb[3] = (wy < a.py - 1 || a.per_y) ? wid(wx, (wy + 1) % a.py, wz) : -1;
-    nb[4] = wid(wx, wy, (wz + a.pz - 1) % a.pz);
-    nb[5] = wid(wx, wy, (wz + 1) % a.pz);
-
-    // ---- load the tile (interior and halo) from the global solution
-    const int tiles = tx * ty * (bz + 2);
-    for (int q = t; q < tiles; q += RB_THREADS) {
-        const int i = q % tx, j = (q / tx) % ty, k = q / (tx * ty);
+    // ---- load the tile (subdomain and halo) from the global solution
+    for (int q = t; q < tx * ty * tz; q += RB_THREADS) {
+        const int li = q % tx - 1, lj = (q / tx) % ty - 1, lk = q / (tx * ty) - 1;
         T v = T(0);
         if (!a.zero_init) {
-            int gi = ox + i, gj = oy + j, gk = oz + k;
-            gi = gi < 1 ? d.nx : (gi > d.nx ? 1 : gi);
-            gk = gk < 1 ? d.nz : (gk > d.nz ? 1 : gk);
-            if (a.per_y) gj = gj < 1 ? d.ny : (gj > d.ny ? 1 : gj);
+            const int gi = wrap(ox + li, d.nx), gk = wrap(oz + lk, d.nz);
+            int gj = oy + lj;
+            if (per_y) gj = wrap(gj, d.ny);
             if (gj >= 1 && gj <= d.ny) v = a.x[d.in(gi, gj, gk)];
         }
         X[q] = v;
     }
-    // ---- face coefficient table (one x-y plane): mui, muip, mvj, mvjp, mw, -omega/denom
+    // ---- face coefficient table (one x-y plane, ring included): mui, muip, mvj, mvjp, mw, -omega/denom
     T qconst = T(0);
     if (IBM) {
         const int sx = d.sx;
-        for (int q = t; q < bx * by; q += RB_THREADS) {
-            const int i = q % bx + 1, j = q / bx + 1;
-            const int gi = ox + i, gj = oy + j;
-            const int gip = gi == d.nx ? 1 : gi + 1;
-            const int gjp = (a.per_y && gj == d.ny) ? 1 : gj + 1;
-            const T mui = a.mu_u[(size_t)gi + (size_t)sx * gj], muip = a.mu_u[(size_t)gip + (size_t)sx * gj];
-            const T mvj = a.mu_v[(size_t)gi + (size_t)sx * gj], mvjp = a.mu_v[(size_t)gi + (size_t)sx * gjp];
-            const T mw = a.mu_w[(size_t)gi + (size_t)sx * gj];
-            const T denom = (mui + muip) * a.s.idx2 + (mw + mw) * a.s.idz2 + (mvj + mvjp) * a.s.idy2;
-            T *c = C + 6 * q;
-            c[0] = mui; c[1] = muip; c[2] = mvj; c[3] = mvjp; c[4] = mw; c[5] = -a.omega / denom;
+        for (int q = t; q < cpl; q += RB_THREADS) {
+            const int li = q % cw, lj = q / cw;
+            const int gi = wrap(ox + li, d.nx);
+            int gj = oy + lj;
+            if (per_y) gj = wrap(gj, d.ny);
+            T mui = 1, muip = 1, mvj = 1, mvjp = 1, mw = 1, qq = 0;
+            if (gj >= 1 && gj <= d.ny) {   // rows beyond a wall are never relaxed
+                const int gip = gi == d.nx ? 1 : gi + 1;
+                const int gjp = (per_y && gj == d.ny) ? 1 : gj + 1;
+                mui = a.mu_u[(size_t)gi + (size_t)sx * gj];
+                muip = a.mu_u[(size_t)gip + (size_t)sx * gj];
+                mvj = a.mu_v[(size_t)gi + (size_t)sx * gj];
+                mvjp = a.mu_v[(size_t)gi + (size_t)sx * gjp];
+                mw = a.mu_w[(size_t)gi + (size_t)sx * gj];
+                const T denom = (mui + muip) * a.s.idx2 + (mw + mw) * a.s.idz2 + (mvj + mvjp) * a.s.idy2;
+                qq = -a.omega / denom;
+            }
+            C[q] = mui; C[q + cpl] = muip; C[q + 2 * cpl] = mvj; C[q + 3 * cpl] = mvjp; C[q + 4 * cpl] = mw;
+            C[q + 5 * cpl] = qq;
         }
     } else {
         const T one = T(1);
         const T denom = (one + one) * a.s.idx2 + (one + one) * a.s.idz2 + (one + one) * a.s.idy2;
         qconst = -a.omega / denom;
     }
-    // the right-hand side is read from global memory at each update: a subdomain's share
-    // (bx*by*bz values) stays resident in its XCD's L2 for the whole batch
+
+    // ---- subdomain pairs of this thread, packed in one word: LDS index of the colour-0 cell
+    // (bits 0-15), its coefficient-table index (bits 16-27), wall flags (bits 28-29) and whether
+    // the pair's colour-1 cell is at -1 (bit 30); bit 31 clear marks a valid entry (word >= 0)
     const int hbx = bx / 2, npairs = hbx * by * bz;
-    const T *__restrict__ rhs_g = a.rhs;
-    int my_nb = -1;                                  // the neighbour this thread waits for (t < 6)
-    if (t < 6) my_nb = nb[0] * (t == 0) + nb[1] * (t == 1) + nb[2] * (t == 2) + nb[3] * (t == 3) +
-                       nb[4] * (t == 4) + nb[5] * (t == 5);
+    auto pack = [](int c, int ti, int walls, int neg) { return c | (ti << 16) | (walls << 28) | (neg << 30); };
+    int pw[MAXP];
+    T rh[MAXP][2];
+#pragma unroll
+    for (int m = 0; m < MAXP; ++m) {
+        const int p = t + m * RB_THREADS;
+        pw[m] = -1;
+        rh[m][0] = rh[m][1] = T(0);
+        if (p < npairs) {
+            const int i0 = 2 * (p % hbx) + 1, j = (p / hbx) % by + 1, k = p / (hbx * by) + 1;
+            const int gj = oy + j;
+            // colour c relaxes the cell with (gi + gj + gk + c) odd
+            const int off0 = ((ox + i0 + gj + oz + k) & 1) ? 0 : 1;
+            pw[m] = pack(L(i0 + off0, j, k), (i0 + off0) + cw * j, wall_bits(gj), off0);
+            rh[m][0] = a.rhs[d.in(ox + i0 + off0, gj, oz + k)];
+            rh[m][1] = a.rhs[d.in(ox + i0 + 1 - off0, gj, oz + k)];
+        }
+    }
+    // ---- colour-0 cells of the face ring owned by neighbours, recomputed here.  Face f has fixed
+    // coordinate 0 or b+1 along its axis A and spans 1..bU x 1..bV (U = y for x faces, else x);
+    // with bU even each U-line holds bU/2 colour-0 cells.
+    int rw[RB_MAXR];   // packed as the pair words
+    T rr[RB_MAXR];
+    {
+        const int nr = rb_ring_total(bx, by, bz);
+#pragma unroll
+        for (int m = 0; m < RB_MAXR; ++m) {
+            int e = t + m * RB_THREADS;
+            rw[m] = -1;
+            rr[m] = T(0);
+            if (e < nr) {
+                int f;   // 0 x faces, 1 y faces, 2 z faces; two faces per axis
+                if (e < by * bz) f = 0;
+                else if ((e -= by * bz) < bx * bz) f = 1;
+                else { e -= bx * bz; f = 2; }
+                const int bU = f == 0 ? by : bx, bV = f == 2 ? by : bz;
+                const int half = bU / 2 * bV, side = e / half;
+                const int r = e - side * half, v = r / (bU / 2) + 1, uq = r % (bU / 2);
+                const int dxn = f == 0 ? (side ? 1 : -1) : 0, dyn = f == 1 ? (side ? 1 : -1) : 0,
+                          dzn = f == 2 ? (side ? 1 : -1) : 0;
+                if (neighbour(dxn, dyn, dzn) >= 0) {
+                    // local coordinates with u = 2 uq + 1 + s (s picks the colour-0 cell)
+                    int li, lj, lk;
+                    const int cA = side ? (f == 0 ? bx + 1 : (f == 1 ? by + 1 : bz + 1)) : 0;
+                    if (f == 0) { li = cA; lj = 2 * uq + 1; lk = v; }
+                    else if (f == 1) { li = 2 * uq + 1; lj = cA; lk = v; }
+                    else { li = 2 * uq + 1; lj = v; lk = cA; }
+                    const int s = ((ox + li + oy + lj + oz + lk) & 1) ? 0 : 1;
+                    if (f == 0) lj += s; else li += s;
+                    const int gi = wrap(ox + li, d.nx), gk = wrap(oz + lk, d.nz);
+                    int gj = oy + lj;
+                    if (per_y) gj = wrap(gj, d.ny);
+                    rw[m] = pack(L(li, lj, lk), li + cw * lj, wall_bits(gj), 0);
+                    rr[m] = a.rhs[d.in(gi, gj, gk)];
+                }
+            }
+        }
+    }
+    // ---- exchange lists: entry e = t + jj * RB_THREADS of the 18 regions.  Publish: the LDS cell
+    // this workgroup sends for its neighbour in direction D.  Pull: the region that the neighbour
+    // in direction -D sent for this workgroup: LDS destination (bits 0-19) and D (bits 20-24),
+    // the sender's id coming from the neighbour table.
+    __shared__ int nbr_from[18];   // the workgroup that sends direction q's region to this one
+    if (t < 18) {
+        int dx, dy, dz;
+        rb_dir(t, dx, dy, dz);
+        nbr_from[t] = neighbour(-dx, -dy, -dz);
+    }
+    const int total = a.total;
+    int xpub[RB_HE], xdst[RB_HE];   // < 0: nothing to do
+#pragma unroll
+    for (int jj = 0; jj < RB_HE; ++jj) {
+        const int e = t + jj * RB_THREADS;
+        xpub[jj] = -1; xdst[jj] = -1;
+        if (e < total) {
+            int q = 0, off = 0, ex = 0, ey = 0, ez = 0;
+            for (; q < 18; ++q) {
+                rb_extent(q, bx, by, bz, ex, ey, ez);
+                if (e < off + ex * ey * ez) break;
+                off += ex * ey * ez;
+            }
+            int dx, dy, dz;
+            rb_dir(q, dx, dy, dz);
+            const int h = q < 6 ? RB_H : 1, r = e - off;
+            const int rx = r % ex, ry = (r / ex) % ey, rz = r / (ex * ey);
+            // sender side: the layer next to its face D
+            auto snd = [&](int dd, int rr_, int bb) { return dd > 0 ? bb - h + 1 + rr_ : 1 + rr_; };
+            xpub[jj] = L(snd(dx, rx, bx), snd(dy, ry, by), snd(dz, rz, bz));
+            // receiver side: the sender is the neighbour in direction -D; its layer lands in the halo
+            if (neighbour(-dx, -dy, -dz) >= 0) {
+                auto rcv = [&](int dd, int rr_, int bb) {
+                    return dd < 0 ? bb + 1 + rr_ : (dd > 0 ? 1 - h + rr_ : 1 + rr_);
+                };
+                xdst[jj] = L(rcv(dx, rx, bx), rcv(dy, ry, by), rcv(dz, rz, bz)) | (q << 20);
+            }
+        }
+    }
+    bool failed = false;
     __syncthreads();
 
-    const int fmax = a.fmax;
-    const unsigned long long base = a.epoch << 32;
-    const int nhalf = 2 * a.iters;
-    for (int s = 0; s < nhalf; ++s) {
-        const int color = s & 1;
-        // ---- relax the cells of this colour: (gi + gj + gk + color) odd
-#pragma unroll
-        for (int m = 0; m < MAXP; ++m) {
-            const int p = t + m * RB_THREADS;
-            if (p < npairs) {
-                const int i0 = 2 * (p % hbx) + 1, j = (p / hbx) % by + 1, k = p / (hbx * by) + 1;
-                const int gj = oy + j;
-                const int odd = (ox + i0 + gj + oz + k + color) & 1;
-                const int i = odd ? i0 : i0 + 1;
-                const T rhs = rhs_g[d.in(ox + i, gj, oz + k)];
-                const bool wall_n = !a.per_y && gj == d.ny, wall_s = !a.per_y && gj == 1;
-                const int c = L(i, j, k);
-                T mui = 1, muip = 1, mvj = 1, mvjp = 1, mw = 1, q = qconst;
-                if (IBM) {
-                    const T *cc = C + 6 * ((i - 1) + bx * (j - 1));
-                    mui = cc[0]; muip = cc[1]; mvj = cc[2]; mvjp = cc[3]; mw = cc[4]; q = cc[5];
-                }
-                T denom;
-                const T p0 = X[c];
-                const T r = sor_residual_core<T, false>(p0, X[c + 1], X[c - 1], X[c + tx], X[c - tx],
-                                                        X[c + tx * ty], X[c - tx * ty], mui, muip, mvj, mvjp,
-                                                        mw, mw, wall_n, wall_s, rhs, a.s, T(0), T(0), denom);
-                X[c] = p0 + r * q;
-            }
+    // relax the cell of packed word pw shifted by dlt (0: the colour-0 cell, +-1: its pair partner)
+    auto relax = [&](int pwd, int dlt, T rhs) -> T {
+        const int c = (pwd & 0xffff) + dlt, ti = ((pwd >> 16) & 0xfff) + dlt;
+        T mui = 1, muip = 1, mvj = 1, mvjp = 1, mw = 1, q = qconst;
+        if (IBM) {
+            mui = C[ti]; muip = C[ti + cpl]; mvj = C[ti + 2 * cpl]; mvjp = C[ti + 3 * cpl]; mw = C[ti + 4 * cpl];
+            q = C[ti + 5 * cpl];
         }
-        __syncthreads();
-        if (s == nhalf - 1) break;
+        T denom;
+        const T p0 = X[c];
+        const T r = sor_residual_core<T, false>(p0, X[c + 1], X[c - 1], X[c + tx], X[c - tx], X[c + tx * ty],
+                                                X[c - tx * ty], mui, muip, mvj, mvjp, mw, mw, (pwd >> 28) & 1,
+                                                (pwd >> 29) & 1, rhs, a.s, T(0), T(0), denom);
+        return p0 + r * q;
+    };
+    auto partner = [](int pwd) { return ((pwd >> 30) & 1) ? -1 : 1; };
 
-        // ---- publish the six boundary layers of this subdomain.  Face values and counters are
-        // agent-scope atomic stores and loads: they are coherent across XCDs by themselves, so the
-        // hand-off needs no L2 write-back or invalidate, only the completion of this workgroup's
-        // stores (s_waitcnt vmcnt(0) in every wave, then the barrier) before the counter is raised.
+    unsigned long long *F = a.faces;
+    for (int s = 0; s < a.iters; ++s) {
+        // The packed words are loop-invariant; left visible, the compiler would keep every address
+        // derived from them (13 per cell and colour) live across the loop and run out of registers.
+        // An empty asm that claims to redefine them makes the few address adds happen per use.
+#pragma unroll
+        for (int m = 0; m < MAXP; ++m) asm volatile("" : "+v"(pw[m]));
+#pragma unroll
+        for (int m = 0; m < RB_MAXR; ++m) asm volatile("" : "+v"(rw[m]));
+#pragma unroll
+        for (int jj = 0; jj < RB_HE; ++jj) asm volatile("" : "+v"(xpub[jj]), "+v"(xdst[jj]));
+        // ---- colour 0 on the subdomain and the face ring.  Cells of one colour never neighbour
+        // each other, so each is stored as soon as it is formed; groups of four pairs with a
+        // scheduling barrier between them bound the loads in flight (registers).
+#pragma unroll
+        for (int g = 0; g < MAXP; g += 4) {
+            T nv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (pw[g + u] >= 0) nv[u] = relax(pw[g + u], 0, rh[g + u][0]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (pw[g + u] >= 0) X[pw[g + u] & 0xffff] = nv[u];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        {
+            T nv[RB_MAXR];
+#pragma unroll
+            for (int m = 0; m < RB_MAXR; ++m)
+                if (rw[m] >= 0) nv[m] = relax(rw[m], 0, rr[m]);
+#pragma unroll
+            for (int m = 0; m < RB_MAXR; ++m)
+                if (rw[m] >= 0) X[rw[m] & 0xffff] = nv[m];
+        }
+        __syncthreads();
+        // ---- colour 1 on the subdomain: its neighbours are subdomain or ring cells, all updated
+#pragma unroll
+        for (int g = 0; g < MAXP; g += 4) {
+            T nv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (pw[g + u] >= 0) nv[u] = relax(pw[g + u], partner(pw[g + u]), rh[g + u][1]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (pw[g + u] >= 0) X[(pw[g + u] & 0xffff) + partner(pw[g + u])] = nv[u];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+        if (s == a.iters - 1) break;
+
+        // ---- publish this subdomain's layers, tagged with this iteration
         const int buf = s & 1;
-        T *myf = a.faces + (size_t)w * 12 * fmax;
-        for (int q = t; q < 2 * by * bz; q += RB_THREADS) {          // x faces (j,k)
-            const int side = q / (by * bz), r = q % (by * bz), j = r % by + 1, k = r / by + 1;
-            __hip_atomic_store(&myf[(size_t)((side) * 2 + buf) * fmax + r], X[L(side ? bx : 1, j, k)],
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        for (int q = t; q < 2 * bx * bz; q += RB_THREADS) {          // y faces (i,k)
-            const int side = q / (bx * bz), r = q % (bx * bz), i = r % bx + 1, k = r / bx + 1;
-            __hip_atomic_store(&myf[(size_t)((2 + side) * 2 + buf) * fmax + r], X[L(i, side ? by : 1, k)],
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        for (int q = t; q < 2 * bx * by; q += RB_THREADS) {          // z faces (i,j)
-            const int side = q / (bx * by), r = q % (bx * by), i = r % bx + 1, j = r / bx + 1;
-            __hip_atomic_store(&myf[(size_t)((4 + side) * 2 + buf) * fmax + r], X[L(i, j, side ? bz : 1)],
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        const unsigned long long target = base + (unsigned long long)(s + 1);
-        if (t == 0) __hip_atomic_store(&a.flags[w], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (t < 6 && my_nb >= 0) {
-            long spins = 0;
-            while (__hip_atomic_load(&a.flags[my_nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                if (++spins > (1l << 24)) {
-                    __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+        const unsigned tag = a.tag0 + (unsigned)s + 1u;
+        const size_t mybase = ((size_t)(w * 2 + buf) * total) * NW;
+#pragma unroll
+        for (int jj = 0; jj < RB_HE; ++jj) {
+            if (xpub[jj] >= 0) {
+                unsigned long long wv[NW];
+                TW::pack(X[xpub[jj]], tag, wv);
+                unsigned long long *dst = F + mybase + (size_t)(t + jj * RB_THREADS) * NW;
+#pragma unroll
+                for (int q = 0; q < NW; ++q)
+                    __hip_atomic_store(dst + q, wv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        __syncthreads();
-        // ---- pull the neighbours' layers into the halo
-        for (int q = t; q < 2 * by * bz; q += RB_THREADS) {
-            const int side = q / (by * bz), r = q % (by * bz), j = r % by + 1, k = r / by + 1;
-            const int n = nb[side];                      // side 0: x- neighbour gives its x+ face
-            T *nf = a.faces + (size_t)n * 12 * fmax + (size_t)((1 - side) * 2 + buf) * fmax;
-            X[L(side ? bx + 1 : 0, j, k)] = __hip_atomic_load(&nf[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        for (int q = t; q < 2 * bx * bz; q += RB_THREADS) {
-            const int side = q / (bx * bz), r = q % (bx * bz), i = r % bx + 1, k = r / bx + 1;
-            const int n = nb[2 + side];
-            if (n >= 0) {
-                T *nf = a.faces + (size_t)n * 12 * fmax + (size_t)((2 + 1 - side) * 2 + buf) * fmax;
-                X[L(i, side ? by + 1 : 0, k)] = __hip_atomic_load(&nf[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // ---- pull the neighbours' layers: poll the tagged words until all carry this tag
+        // source word of entry jj: the sender's buffer `buf`, entry e, in words
+        auto src_of = [&](int jj) {
+            const int sender = nbr_from[(xdst[jj] >> 20) & 31];
+            return ((size_t)(sender * 2 + buf) * total + (size_t)(t + jj * RB_THREADS)) * NW;
+        };
+        unsigned long long got[RB_HE][NW];
+#pragma unroll
+        for (int jj = 0; jj < RB_HE; ++jj)
+#pragma unroll
+            for (int q = 0; q < NW; ++q)
+                got[jj][q] = xdst[jj] >= 0 ? __hip_atomic_load(F + src_of(jj) + q, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT)
+                                           : ((unsigned long long)tag << 32);
+        for (int spins = 0; !failed; ++spins) {
+            bool ready = true;
+#pragma unroll
+            for (int jj = 0; jj < RB_HE; ++jj)
+#pragma unroll
+                for (int q = 0; q < NW; ++q) ready = ready && (unsigned)(got[jj][q] >> 32) == tag;
+            if (ready) break;
+            if (spins >= (1 << 20)) {   // a neighbour never published: flag it and run to the end
+                __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                failed = true;
+                break;
             }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int jj = 0; jj < RB_HE; ++jj)
+#pragma unroll
+                for (int q = 0; q < NW; ++q)
+                    if ((unsigned)(got[jj][q] >> 32) != tag)
+                        got[jj][q] = __hip_atomic_load(F + src_of(jj) + q, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
         }
-        for (int q = t; q < 2 * bx * by; q += RB_THREADS) {
-            const int side = q / (bx * by), r = q % (bx * by), i = r % bx + 1, j = r / bx + 1;
-            const int n = nb[4 + side];
-            T *nf = a.faces + (size_t)n * 12 * fmax + (size_t)((4 + 1 - side) * 2 + buf) * fmax;
-            X[L(i, j, side ? bz + 1 : 0)] = __hip_atomic_load(&nf[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+#pragma unroll
+        for (int jj = 0; jj < RB_HE; ++jj)
+            if (xdst[jj] >= 0) X[xdst[jj] & 0xfffff] = TW::unpack(got[jj]);
         __syncthreads();
     }
-    // ---- write the interior back
+    // ---- write the subdomain back
     for (int q = t; q < bx * by * bz; q += RB_THREADS) {
         const int i = q % bx + 1, j = (q / bx) % by + 1, k = q / (bx * by) + 1;
         a.x[d.in(ox + i, oy + j, oz + k)] = X[L(i, j, k)];
@@ -214,7 +424,8 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
 
 // ------------------------------------------------------------------------------- host side
 static size_t rb_lds_bytes(int bx, int by, int bz, size_t tsize, bool ibm) {
-    return ((size_t)(bx + 2) * (by + 2) * (bz + 2) + (ibm ? 6 * (size_t)bx * by : 0)) * tsize;
+    return ((size_t)(bx + 2 * RB_H) * (by + 2 * RB_H) * (bz + 2 * RB_H) + (ibm ? 6 * (size_t)(bx + 2) * (by + 2) : 0)) *
+           tsize;
 }
 
 template <typename T, bool IBM, int MAXP>
@@ -226,10 +437,13 @@ template <typename T, bool IBM>
 static const void *rb_fn_for(int maxp) {
     if (maxp <= 4) return rb_fn<T, IBM, 4>();
     if (maxp <= 8) return rb_fn<T, IBM, 8>();
+    if (maxp <= 12) return rb_fn<T, IBM, 12>();
     return rb_fn<T, IBM, 16>();
 }
 
-static int rb_maxp_class(int need) { return need <= 4 ? 4 : (need <= 8 ? 8 : (need <= 16 ? 16 : 0)); }
+static int rb_maxp_class(int need) {
+    return need <= 4 ? 4 : (need <= 8 ? 8 : (need <= 12 ? 12 : (need <= 16 ? 16 : 0)));
+}
 
 // workgroups of this kernel variant that fit on one CU (registers, LDS, waves), from the runtime
 static int rb_occupancy(bool fp32, bool ibm, int maxp, size_t lds) {
@@ -248,7 +462,7 @@ static int rb_occupancy(bool fp32, bool ibm, int maxp, size_t lds) {
 }
 
 // Choose the decomposition: every workgroup co-resident (runtime occupancy), the fewest cells
-// per CU, then the smallest halo surface.  `why` receives the reason when nothing fits.
+// per CU, then the smallest exchange.  `why` receives the reason when nothing fits.
 bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
     const int nx = h->d.nx, ny = h->d.ny, nz = h->d.nz;
     hipDeviceProp_t prop;
@@ -259,37 +473,47 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
                                ? prop.maxSharedMemoryPerMultiProcessor : prop.sharedMemPerBlock;
     const size_t ts = fp32 ? 4 : 8;
     bool found = false;
-    double best_cells = 1e300, best_surf = 1e300;
+    double best_cells = 1e300, best_x = 1e300;
     why = "no decomposition fits (cus=" + std::to_string(cus) + ", lds/block=" + std::to_string(prop.sharedMemPerBlock) +
           ", lds/cu=" + std::to_string(prop.maxSharedMemoryPerMultiProcessor) + ")";
     for (int px = 1; px <= nx; ++px) {
         if (nx % px || (nx / px) % 2) continue;
         for (int py = 1; py <= ny; ++py) {
-            if (ny % py) continue;
+            if (ny % py || (ny / py) % 2) continue;
             for (int pz = 1; pz <= nz; ++pz) {
-                if (nz % pz) continue;
+                if (nz % pz || (nz / pz) % 2) continue;
                 const int bx = nx / px, by = ny / py, bz = nz / pz;
                 const int nwg = px * py * pz;
                 const size_t lds = rb_lds_bytes(bx, by, bz, ts, h->ibm_on);
                 if (lds > lds_max) continue;
+                // packed per-cell words: LDS index < 2^16, coefficient-table index < 2^12
+                if ((size_t)(bx + 2 * RB_H) * (by + 2 * RB_H) * (bz + 2 * RB_H) >= 65536 ||
+                    (bx + 2) * (by + 2) >= 4096)
+                    continue;
                 const int maxp = rb_maxp_class((bx / 2 * by * bz + RB_THREADS - 1) / RB_THREADS);
                 if (!maxp) continue;
+                if (rb_ring_total(bx, by, bz) > RB_MAXR * RB_THREADS) continue;
+                const int total = rb_xchg_total(bx, by, bz);
+                if (total > RB_HE * RB_THREADS) continue;
                 if (nwg > 4 * cus) continue;
                 const int per_cu = rb_occupancy(fp32, h->ibm_on, maxp, lds);
                 if (per_cu < 1 || nwg > per_cu * cus) continue;
                 const double rounds = std::ceil((double)nwg / cus);
                 const double cells = rounds * bx * by * bz;
-                const double surf = 2.0 * ((double)by * bz + (double)bx * bz + (double)bx * by);
-                if (cells < best_cells - 0.5 || (cells < best_cells + 0.5 && surf < best_surf)) {
+                if (cells < best_cells - 0.5 || (cells < best_cells + 0.5 && total < best_x)) {
                     best_cells = cells;
-                    best_surf = surf;
-                    out = RbPlanOut{px, py, pz, bx, by, bz, maxp, lds};
+                    best_x = total;
+                    out = RbPlanOut{px, py, pz, bx, by, bz, maxp, lds, total};
                     found = true;
                 }
             }
         }
     }
     return found;
+}
+
+size_t rb_faces_bytes(const RbPlanOut &p, bool fp32) {
+    return (size_t)p.px * p.py * p.pz * 2 * (size_t)p.total * 8 * (fp32 ? 1 : 2);
 }
 
 template <typename T>
@@ -313,10 +537,17 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     a.rhs = rhs;
     a.zero_init = zero_init;
     a.iters = iters;
-    a.faces = reinterpret_cast<T *>(h->rb_faces);
-    a.fmax = std::max(pl.by * pl.bz, std::max(pl.bx * pl.bz, pl.bx * pl.by));
-    a.flags = h->rb_flags;
-    a.epoch = ++h->rb_epoch;
+    a.faces = reinterpret_cast<unsigned long long *>(h->rb_faces);
+    a.total = pl.total;
+    // tags of this launch: rb_tag + 1 .. rb_tag + iters; before the 32-bit tag would wrap to 0 the
+    // face buffer is cleared and the count restarts
+    if (h->rb_tag + (unsigned long long)iters + 1ull > 0xffffffffull) {
+        hipError_t e = hipMemsetAsync(h->rb_faces, 0, h->rb_faces_bytes, h->stream);
+        if (e != hipSuccess) return e;
+        h->rb_tag = 0;
+    }
+    a.tag0 = (unsigned)h->rb_tag;
+    h->rb_tag += (unsigned long long)iters;
     a.err = h->rb_err;
     const void *fn = h->ibm_on ? rb_fn_for<T, true>(pl.maxp) : rb_fn_for<T, false>(pl.maxp);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds);

@@ -272,7 +272,7 @@ def _wavy(nx, ny, nz, per_y=0, ibm=True):
 @pytest.mark.parametrize("shape,per_y,ibm", [((48, 32, 24), 0, True), ((32, 16, 16), 1, False),
                                              ((192, 128, 96), 0, True)])
 def test_lds_sweeps_bit_exact_fixed_iterations(shape, per_y, ibm, monkeypatch):
-    monkeypatch.setenv("MOBYDIFF_LDS_SOR", "1")
+    monkeypatch.setenv("MOBYDIFF_LDS_SOR", "2")   # FP64 batches on the LDS-resident solver too
     g, coef = _wavy(*shape, per_y=per_y, ibm=ibm)
     rng = np.random.default_rng(21)
     f = O.random_fields(g, rng, amp=0.05)
@@ -394,7 +394,7 @@ def test_lds_sweeps_equal_streaming_sweeps(precision, monkeypatch):
     rng = np.random.default_rng(22)
     f = O.random_fields(g, rng, amp=0.05)
     outs, stats, infos = [], [], []
-    for on in ("1", "0"):
+    for on in ("2", "0"):
         monkeypatch.setenv("MOBYDIFF_LDS_SOR", on)
         ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=10,
                       precision=precision, inner_iters=10)
