@@ -31,8 +31,10 @@ namespace mdx {
 
 static constexpr int RB_THREADS = 512;
 static constexpr int RB_HE = 12;     // exchanged values per thread (a subdomain's halo <= RB_HE * RB_THREADS)
-static constexpr int RB_MAXR = 4;    // colour-0 ring cells per thread
+static constexpr int RB_MAXR = 3;    // colour-0 ring cells per thread
+static constexpr int RB_G = 3;       // cells relaxed per group (loads issued together)
 static constexpr int RB_H = 2;       // halo depth
+static constexpr int RB_XSTRIDE = RB_HE * RB_THREADS;   // exchange entries per workgroup and buffer
 
 template <typename T>
 struct RbArgs {
@@ -48,7 +50,7 @@ struct RbArgs {
     int zero_init;           // x starts at zero (the defect equation)
     int iters;               // full red+black iterations
     unsigned long long *faces;   // [wg][buffer][exchange entry][words per value], tagged words
-    int total;               // exchange entries per workgroup (faces two deep, then edges)
+    int total;               // exchange entries in use per workgroup (faces two deep, then edges)
     unsigned tag0;           // iteration s of this launch publishes tag tag0 + s + 1 (never 0)
     int *err;
 };
@@ -122,7 +124,11 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     const int tx = bx + 2 * RB_H, ty = by + 2 * RB_H, tz = bz + 2 * RB_H;
     const int cw = bx + 2, cpl = (bx + 2) * (by + 2);   // coefficient table: local i, j in [0, b+1]
     T *X = reinterpret_cast<T *>(smem_raw);
-    T *C = X + (size_t)tx * ty * tz;   // IBM: 6 planes of cpl values (planes: threads read at pair stride)
+    // unused slots of the per-thread cell lists point at a dummy cell past the tile, whose stencil
+    // reads and writes stay inside a padding of 2*tx*ty+3 values: the relax loops need no branches
+    const int dummy = tx * ty * tz + tx * ty + 1;
+    T *C = X + (size_t)tx * ty * tz + 2 * tx * ty + 3;   // IBM: 6 planes of cpl values
+                                                        // (planes: threads read at pair stride)
     const int nwg = a.px * a.py * a.pz;
     // XCD-aware placement: workgroups are dispatched round-robin over the 8 XCDs, so consecutive
     // subdomain ids (neighbours along x first) are given to workgroups on the same XCD.
@@ -191,7 +197,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
 
     // ---- subdomain pairs of this thread, packed in one word: LDS index of the colour-0 cell
     // (bits 0-15), its coefficient-table index (bits 16-27), wall flags (bits 28-29) and whether
-    // the pair's colour-1 cell is at -1 (bit 30); bit 31 clear marks a valid entry (word >= 0)
+    // the pair's colour-1 cell is at -1 (bit 30)
     const int hbx = bx / 2, npairs = hbx * by * bz;
     auto pack = [](int c, int ti, int walls, int neg) { return c | (ti << 16) | (walls << 28) | (neg << 30); };
     int pw[MAXP];
@@ -199,7 +205,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
 #pragma unroll
     for (int m = 0; m < MAXP; ++m) {
         const int p = t + m * RB_THREADS;
-        pw[m] = -1;
+        pw[m] = pack(dummy, 0, 0, 0);
         rh[m][0] = rh[m][1] = T(0);
         if (p < npairs) {
             const int i0 = 2 * (p % hbx) + 1, j = (p / hbx) % by + 1, k = p / (hbx * by) + 1;
@@ -221,7 +227,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
 #pragma unroll
         for (int m = 0; m < RB_MAXR; ++m) {
             int e = t + m * RB_THREADS;
-            rw[m] = -1;
+            rw[m] = pack(dummy, 0, 0, 0);
             rr[m] = T(0);
             if (e < nr) {
                 int f;   // 0 x faces, 1 y faces, 2 z faces; two faces per axis
@@ -255,18 +261,22 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     // this workgroup sends for its neighbour in direction D.  Pull: the region that the neighbour
     // in direction -D sent for this workgroup: LDS destination (bits 0-19) and D (bits 20-24),
     // the sender's id coming from the neighbour table.
-    __shared__ int nbr_from[18];   // the workgroup that sends direction q's region to this one
+    __shared__ int nbr_from[19];   // the workgroup that sends direction q's region to this one
     if (t < 18) {
         int dx, dy, dz;
         rb_dir(t, dx, dy, dz);
         nbr_from[t] = neighbour(-dx, -dy, -dz);
     }
+    if (t == 18) nbr_from[18] = w;
     const int total = a.total;
-    int xpub[RB_HE], xdst[RB_HE];   // < 0: nothing to do
+    // Unused entries publish the dummy cell; an entry with no sender (beyond a wall, or past the
+    // end of the list) polls this thread's own just-published word, which carries the tag, and
+    // lands in the dummy cell: the exchange needs no branches either.  Direction code 18 = self.
+    int xpub[RB_HE], xdst[RB_HE];
 #pragma unroll
     for (int jj = 0; jj < RB_HE; ++jj) {
         const int e = t + jj * RB_THREADS;
-        xpub[jj] = -1; xdst[jj] = -1;
+        xpub[jj] = dummy; xdst[jj] = dummy | (18 << 20);
         if (e < total) {
             int q = 0, off = 0, ex = 0, ey = 0, ez = 0;
             for (; q < 18; ++q) {
@@ -322,39 +332,33 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
 #pragma unroll
         for (int jj = 0; jj < RB_HE; ++jj) asm volatile("" : "+v"(xpub[jj]), "+v"(xdst[jj]));
         // ---- colour 0 on the subdomain and the face ring.  Cells of one colour never neighbour
-        // each other, so each is stored as soon as it is formed; groups of four pairs with a
+        // each other, so each is stored as soon as it is formed; groups of RB_G cells with a
         // scheduling barrier between them bound the loads in flight (registers).
 #pragma unroll
-        for (int g = 0; g < MAXP; g += 4) {
-            T nv[4];
+        for (int g = 0; g < MAXP; g += RB_G) {
+            T nv[RB_G];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (pw[g + u] >= 0) nv[u] = relax(pw[g + u], 0, rh[g + u][0]);
+            for (int u = 0; u < RB_G; ++u) nv[u] = relax(pw[g + u], 0, rh[g + u][0]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (pw[g + u] >= 0) X[pw[g + u] & 0xffff] = nv[u];
+            for (int u = 0; u < RB_G; ++u) X[pw[g + u] & 0xffff] = nv[u];
             __builtin_amdgcn_sched_barrier(0);
         }
         {
             T nv[RB_MAXR];
 #pragma unroll
-            for (int m = 0; m < RB_MAXR; ++m)
-                if (rw[m] >= 0) nv[m] = relax(rw[m], 0, rr[m]);
+            for (int m = 0; m < RB_MAXR; ++m) nv[m] = relax(rw[m], 0, rr[m]);
 #pragma unroll
-            for (int m = 0; m < RB_MAXR; ++m)
-                if (rw[m] >= 0) X[rw[m] & 0xffff] = nv[m];
+            for (int m = 0; m < RB_MAXR; ++m) X[rw[m] & 0xffff] = nv[m];
         }
         __syncthreads();
         // ---- colour 1 on the subdomain: its neighbours are subdomain or ring cells, all updated
 #pragma unroll
-        for (int g = 0; g < MAXP; g += 4) {
-            T nv[4];
+        for (int g = 0; g < MAXP; g += RB_G) {
+            T nv[RB_G];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (pw[g + u] >= 0) nv[u] = relax(pw[g + u], partner(pw[g + u]), rh[g + u][1]);
+            for (int u = 0; u < RB_G; ++u) nv[u] = relax(pw[g + u], partner(pw[g + u]), rh[g + u][1]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (pw[g + u] >= 0) X[(pw[g + u] & 0xffff) + partner(pw[g + u])] = nv[u];
+            for (int u = 0; u < RB_G; ++u) X[(pw[g + u] & 0xffff) + partner(pw[g + u])] = nv[u];
             __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
@@ -363,32 +367,28 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         // ---- publish this subdomain's layers, tagged with this iteration
         const int buf = s & 1;
         const unsigned tag = a.tag0 + (unsigned)s + 1u;
-        const size_t mybase = ((size_t)(w * 2 + buf) * total) * NW;
+        const size_t mybase = ((size_t)(w * 2 + buf) * RB_XSTRIDE) * NW;
 #pragma unroll
         for (int jj = 0; jj < RB_HE; ++jj) {
-            if (xpub[jj] >= 0) {
-                unsigned long long wv[NW];
-                TW::pack(X[xpub[jj]], tag, wv);
-                unsigned long long *dst = F + mybase + (size_t)(t + jj * RB_THREADS) * NW;
+            unsigned long long wv[NW];
+            TW::pack(X[xpub[jj]], tag, wv);
+            unsigned long long *dst = F + mybase + (size_t)(t + jj * RB_THREADS) * NW;
 #pragma unroll
-                for (int q = 0; q < NW; ++q)
-                    __hip_atomic_store(dst + q, wv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            for (int q = 0; q < NW; ++q)
+                __hip_atomic_store(dst + q, wv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         // ---- pull the neighbours' layers: poll the tagged words until all carry this tag
         // source word of entry jj: the sender's buffer `buf`, entry e, in words
         auto src_of = [&](int jj) {
             const int sender = nbr_from[(xdst[jj] >> 20) & 31];
-            return ((size_t)(sender * 2 + buf) * total + (size_t)(t + jj * RB_THREADS)) * NW;
+            return ((size_t)(sender * 2 + buf) * RB_XSTRIDE + (size_t)(t + jj * RB_THREADS)) * NW;
         };
         unsigned long long got[RB_HE][NW];
 #pragma unroll
         for (int jj = 0; jj < RB_HE; ++jj)
 #pragma unroll
             for (int q = 0; q < NW; ++q)
-                got[jj][q] = xdst[jj] >= 0 ? __hip_atomic_load(F + src_of(jj) + q, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT)
-                                           : ((unsigned long long)tag << 32);
+                got[jj][q] = __hip_atomic_load(F + src_of(jj) + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (int spins = 0; !failed; ++spins) {
             bool ready = true;
 #pragma unroll
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         }
 #pragma unroll
         for (int jj = 0; jj < RB_HE; ++jj)
-            if (xdst[jj] >= 0) X[xdst[jj] & 0xfffff] = TW::unpack(got[jj]);
+            X[xdst[jj] & 0xfffff] = TW::unpack(got[jj]);
         __syncthreads();
     }
     // ---- write the subdomain back
@@ -424,8 +424,8 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
 
 // ------------------------------------------------------------------------------- host side
 static size_t rb_lds_bytes(int bx, int by, int bz, size_t tsize, bool ibm) {
-    return ((size_t)(bx + 2 * RB_H) * (by + 2 * RB_H) * (bz + 2 * RB_H) + (ibm ? 6 * (size_t)(bx + 2) * (by + 2) : 0)) *
-           tsize;
+    const size_t tx = bx + 2 * RB_H, ty = by + 2 * RB_H, tz = bz + 2 * RB_H;
+    return (tx * ty * tz + 2 * tx * ty + 3 + (ibm ? 6 * (size_t)(bx + 2) * (by + 2) : 0)) * tsize;
 }
 
 template <typename T, bool IBM, int MAXP>
@@ -433,16 +433,21 @@ static const void *rb_fn() {
     return reinterpret_cast<const void *>(&rb_lds_kernel<T, IBM, MAXP>);
 }
 
+// pairs per thread: multiples of the group size RB_G
 template <typename T, bool IBM>
 static const void *rb_fn_for(int maxp) {
-    if (maxp <= 4) return rb_fn<T, IBM, 4>();
-    if (maxp <= 8) return rb_fn<T, IBM, 8>();
-    if (maxp <= 12) return rb_fn<T, IBM, 12>();
-    return rb_fn<T, IBM, 16>();
+    switch (maxp) {
+    case 3: return rb_fn<T, IBM, 3>();
+    case 6: return rb_fn<T, IBM, 6>();
+    case 9: return rb_fn<T, IBM, 9>();
+    case 12: return rb_fn<T, IBM, 12>();
+    default: return rb_fn<T, IBM, 15>();
+    }
 }
 
 static int rb_maxp_class(int need) {
-    return need <= 4 ? 4 : (need <= 8 ? 8 : (need <= 12 ? 12 : (need <= 16 ? 16 : 0)));
+    const int c = (need + RB_G - 1) / RB_G * RB_G;
+    return c < RB_G ? RB_G : (c <= 15 ? c : 0);
 }
 
 // workgroups of this kernel variant that fit on one CU (registers, LDS, waves), from the runtime
@@ -486,10 +491,10 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
                 const int nwg = px * py * pz;
                 const size_t lds = rb_lds_bytes(bx, by, bz, ts, h->ibm_on);
                 if (lds > lds_max) continue;
-                // packed per-cell words: LDS index < 2^16, coefficient-table index < 2^12
-                if ((size_t)(bx + 2 * RB_H) * (by + 2 * RB_H) * (bz + 2 * RB_H) >= 65536 ||
-                    (bx + 2) * (by + 2) >= 4096)
-                    continue;
+                // packed per-cell words: LDS index (dummy cell and its padding included) < 2^16,
+                // coefficient-table index < 2^12
+                const size_t txy = (size_t)(bx + 2 * RB_H) * (by + 2 * RB_H);
+                if (txy * (bz + 2 * RB_H) + 2 * txy + 3 >= 65536 || (bx + 2) * (by + 2) >= 4096) continue;
                 const int maxp = rb_maxp_class((bx / 2 * by * bz + RB_THREADS - 1) / RB_THREADS);
                 if (!maxp) continue;
                 if (rb_ring_total(bx, by, bz) > RB_MAXR * RB_THREADS) continue;
@@ -513,7 +518,7 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
 }
 
 size_t rb_faces_bytes(const RbPlanOut &p, bool fp32) {
-    return (size_t)p.px * p.py * p.pz * 2 * (size_t)p.total * 8 * (fp32 ? 1 : 2);
+    return (size_t)p.px * p.py * p.pz * 2 * (size_t)RB_XSTRIDE * 8 * (fp32 ? 1 : 2);
 }
 
 template <typename T>
