@@ -103,11 +103,12 @@ typedef struct {
     double guess_ms;         /* learned initial guess (network inference) */
     int64_t stages;
     double sweep_ms;         /* red-black sweep kernels alone (event-bracketed batches) */
-    int64_t sweep_launches;  /* kernel launches in those batches (one colour each, or one fused
-                                red+black iteration each) */
+    int64_t sweep_launches;  /* kernel launches in those batches: one per colour (streaming), one
+                                per red+black iteration (fused), or one per batch (LDS-resident) */
     int64_t sweep_cells;     /* colour passes over the grid in those batches times nx ny nz, i.e.
                                 2 x iterations x nx ny nz (a one-colour launch covers nx ny nz of it,
-                                a fused launch 2 nx ny nz) */
+                                a fused launch 2 nx ny nz, an LDS-resident batch 2 x its iterations
+                                x nx ny nz) */
 } mdx_phase_times;
 
 /* error text of the last failing call on this thread */

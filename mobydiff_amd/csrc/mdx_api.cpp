@@ -201,8 +201,12 @@ static void free_all(mdx_handle *h) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_pinned) (void)hipHostFree(h->h_pinned);
-    if (h->timing)
+    if (h->timing) {
         for (auto &e : h->ev) (void)hipEventDestroy(e);
+        for (auto &q : h->ev_sw)
+            for (auto &e : q) (void)hipEventDestroy(e);
+    }
+    if (h->ev_max) (void)hipEventDestroy(h->ev_max);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -367,7 +371,11 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         h->timing = true;
         for (auto &ev : h->ev)
             if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "events");
+        for (auto &q : h->ev_sw)
+            for (auto &ev : q)
+                if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "events");
     }
+    if ((e = hipEventCreateWithFlags(&h->ev_max, hipEventDisableTiming)) != hipSuccess) return fail(e, "events");
     if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "synchronize");
     *out = h;
     return MDX_OK;
@@ -707,12 +715,36 @@ static hipError_t sweep_launches(mdx_handle *h, bool fp32, int m) {
 // whether batches of this precision run on the LDS-resident solver
 static bool lds_for(const mdx_handle *h, bool fp32) { return h->rb_state == 1 && (fp32 || h->lds_sor_mode == 2); }
 
+// sweep counters: account finished batches in order (end event complete; `wait`: block for them)
+static void sweep_account(mdx_handle *h, bool wait = false) {
+    while (h->timing && h->sw_count > 0) {
+        const int q = h->sw_head;
+        if (wait) (void)hipEventSynchronize(h->ev_sw[q][1]);
+        else if (hipEventQuery(h->ev_sw[q][1]) != hipSuccess) break;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->ev_sw[q][0], h->ev_sw[q][1]) == hipSuccess) {
+            h->times.sweep_ms += ms;
+            h->times.sweep_launches += h->sw_launches[q];
+            h->times.sweep_cells += 2 * (int64_t)h->sw_its[q] * (int64_t)h->ni;
+        }
+        h->sw_head = (q + 1) % mdx_handle::SW_Q;
+        h->sw_count -= 1;
+    }
+}
+
 static int sweep_batch(mdx_handle *h, bool fp32, int m) {
-    ev_record(h, 5);
+    int qs = -1;   // accounting queue entry of this batch
+    if (h->timing) {
+        if (h->sw_count == mdx_handle::SW_Q) sweep_account(h, true);
+        qs = (h->sw_head + h->sw_count) % mdx_handle::SW_Q;
+        MDX_HIP(hipEventRecord(h->ev_sw[qs][0], h->stream));
+    }
+    int launches = 0;
     auto &slot = h->graphs[fp32 ? 1 : 0];
     if (lds_for(h, fp32)) {
         // LDS-resident sweeps; the FP32 defect solve starts from zero inside the kernel
         hipError_t e = launch_rb_lds(h, fp32, m, fp32 ? 1 : 0);
+        launches = 1;
         if (e != hipSuccess) {
             // the runtime refused the cooperative launch (nothing ran): use the streaming sweeps
             (void)hipGetLastError();
@@ -743,21 +775,14 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m) {
     } else {
         MDX_HIP(sweep_launches(h, fp32, m));
     }
-    ev_record(h, 6);
-    if (h->timing) h->sweep_pending += m;
-    return MDX_OK;
-}
-
-// after a stream synchronisation: add the last batch's event time to the sweep counters
-static void sweep_account(mdx_handle *h) {
-    if (!h->timing || h->sweep_pending == 0) return;
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, h->ev[5], h->ev[6]) == hipSuccess) {
-        h->times.sweep_ms += ms;
-        h->times.sweep_launches += (h->sweep_fused ? 1 : 2) * (int64_t)h->sweep_pending;
-        h->times.sweep_cells += 2 * (int64_t)h->sweep_pending * (int64_t)h->ni;
+    if (!launches) launches = (h->sweep_fused ? 1 : 2) * m;   // streaming: one launch per colour
+    if (qs >= 0) {
+        MDX_HIP(hipEventRecord(h->ev_sw[qs][1], h->stream));
+        h->sw_its[qs] = m;
+        h->sw_launches[qs] = launches;
+        h->sw_count += 1;
     }
-    h->sweep_pending = 0;
+    return MDX_OK;
 }
 
 // Plain FP64 red-black SOR to ||div||_inf < tol; Phi starts at zero (the previous-step pressure).
@@ -792,31 +817,67 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
 }
 
 // FP32 red-black sweeps on the defect equation inside an FP64 defect-correction loop.
+//
+// A check k forms Phi += e and the FP64 residual r_k; the FP64 iterate changes only there, so the
+// next batch of FP32 sweeps can be queued before r_k is read: if r_k turns out converged, that
+// batch's correction is simply never applied, and Phi, the iteration count and every check are
+// what the synchronous loop gives.  The batch is queued early only when the residuals seen so far
+// predict (geometric decay) that check k is still at least SPEC_MARGIN times above the tolerance,
+// so a wasted batch is rare while the GPU does not idle through the host's read of the residual.
 static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
+    constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
     const mdx_config &c = h->cfg;
-    double r = 0.0;
+    const bool lds = lds_for(h, true);
+    double r = 0.0, r1 = -1.0, r2 = -1.0;   // r_k, r_{k-1}, r_{k-2}
     int it = 0, checks = 0;
     MDX_HIP(launch_zero32(h, h->e32, h->ni));
-    for (;;) {
+    auto enqueue_check = [&]() -> int {
         MDX_HIP(launch_reset_max(h));
         MDX_HIP(launch_defect64(h, h->phi, h->e32, h->phi2, h->r32));
         std::swap(h->phi, h->phi2);
-        int rc = read_max(h, &r);
-        if (rc) return rc;
+        MDX_HIP(hipMemcpyAsync(h->h_pinned, h->red_max, 16, hipMemcpyDeviceToHost, h->stream));
+        MDX_HIP(hipEventRecord(h->ev_max, h->stream));
+        return MDX_OK;
+    };
+    int rc;
+    if ((rc = enqueue_check())) return rc;
+    for (;;) {
+        const int m = std::min(c.inner_iters, c.max_iter - it);
+        bool spec = false;
+        if (m > 0 && r1 > 0.0) {
+            const double pred = r2 > 0.0 ? r1 * std::min(1.0, r1 / r2) : r1 / SPEC_FIRST;
+            spec = pred > SPEC_MARGIN * c.tol;
+        }
+        if (spec) {
+            if (!lds) MDX_HIP(launch_zero32(h, h->e32, h->ni));
+            if ((rc = sweep_batch(h, true, m))) return rc;
+        }
+        MDX_HIP(hipEventSynchronize(h->ev_max));
+        r = h->h_pinned[0];
+        int err;
+        std::memcpy(&err, &h->h_pinned[1], sizeof err);
+        if (err) {
+            mdx_set_error("LDS red-black solver: a neighbour wait exceeded its spin limit");
+            return MDX_ERR_HIP;
+        }
         sweep_account(h);
         if (checks == 0) st->div0_inf = r;
         ++checks;
-        if (!(r >= c.tol) || it >= c.max_iter) break;
-        if (!lds_for(h, true)) MDX_HIP(launch_zero32(h, h->e32, h->ni));
-        const int m = std::min(c.inner_iters, c.max_iter - it);
-        if ((rc = sweep_batch(h, true, m))) return rc;
+        if (!(r >= c.tol) || it >= c.max_iter) break;   // a speculative batch is discarded
+        if (!spec) {
+            if (!lds) MDX_HIP(launch_zero32(h, h->e32, h->ni));
+            if ((rc = sweep_batch(h, true, m))) return rc;
+        }
         it += m;
+        r2 = r1;
+        r1 = r;
+        if ((rc = enqueue_check())) return rc;
     }
     st->iterations = it;
     st->checks = checks;
     st->div_inf = r;
     st->converged = r < c.tol;
-    st->lds_sweeps = lds_for(h, true);
+    st->lds_sweeps = lds;
     return MDX_OK;
 }
 
@@ -836,7 +897,8 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
     const int mode = h->cfg.guess;
     bool learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0;
     // gate: after the network's start has lost twice in a row for this stage, it is evaluated
-    // again only after 2, 4, ..., at most 16 skipped solves (the previous-Phi start is used then)
+    // again only after 2, 4, ..., at most NET_SKIP_MAX skipped solves (previous-Phi start then)
+    constexpr int NET_SKIP_MAX = 64;
     if (learned && h->net_skip[s] > 0) {
         h->net_skip[s] -= 1;
         learned = false;
@@ -845,18 +907,21 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
     }
     const bool use_prev = have_prev && (mode == MDX_GUESS_PHI_PREV || mode == MDX_GUESS_LEARNED);
     const bool build_input = n && (learned || h->net_capture);
-    if (!learned && !use_prev && !build_input) return MDX_OK;
+    if (!learned && !build_input) {
+        // previous-Phi start (or the plain one without a history); the solver's first residual
+        // check evaluates it, so no candidate residual is formed here
+        if (use_prev) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));
+        return MDX_OK;
+    }
     int rc;
     ev_record(h, 8);
-    // Phi_0 in phi2 and, for the network, its residual field
+    // Phi_0 in phi2 and its residual field (the network's input)
     if (have_prev) MDX_HIP(launch_scale_copy64(h, h->phi2, prev, fac, h->ni));
     double rp = -1.0;
-    if (build_input || use_prev) {
-        MDX_HIP(launch_reset_max(h));
-        MDX_HIP(launch_residual_field64(h, have_prev ? h->phi2 : h->phi, build_input ? h->res64 : nullptr));
-        if ((rc = read_max(h, &rp))) return rc;
-    }
-    if (build_input) {
+    MDX_HIP(launch_reset_max(h));
+    MDX_HIP(launch_residual_field64(h, have_prev ? h->phi2 : h->phi, h->res64));
+    if ((rc = read_max(h, &rp))) return rc;
+    {
         hipStream_t saved = n->stream;
         n->stream = h->stream;
         hipError_t e = unet::net_input_from_solver(n, h, h->res64, have_prev ? h->phi2 : nullptr);
@@ -877,44 +942,39 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
         MDX_HIP(hipEventSynchronize(h->ev[9]));
         if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
     }
-    if (!learned && !use_prev) return MDX_OK;   // capture only: the plain start
-    double r0 = 0.0;
-    if (have_prev) {
-        MDX_HIP(launch_reset_max(h));
-        MDX_HIP(launch_sor_residual64(h, h->phi));   // Phi = 0
-        if ((rc = read_max(h, &r0))) return rc;
-    } else {
-        r0 = rp;                                     // Phi_0 = 0 was evaluated above
-    }
-    st->guess_div_inf = learned ? rg : rp;
-    // the chosen start is copied into phi (captured sweep graphs hold that buffer's address)
-    if (learned) {
-        // for the gate, the network's start wins only when it clearly beats the best other start
-        // (residual below NET_MARGIN times it): a marginally smaller start residual saves no
-        // sweeps but costs an inference per solve
-        constexpr double NET_MARGIN = 0.8;
-        const double best_other = use_prev ? std::min(rp, r0) : r0;
-        if (rg < NET_MARGIN * best_other) {
-            h->net_losses[s] = 0;
-        } else {
-            h->net_losses[s] += 1;
-            if (h->net_losses[s] >= 2) h->net_skip[s] = std::min(16, 1 << std::min(h->net_losses[s] - 1, 4));
+    // rp is the residual of the start the solver would otherwise take: Phi_0 with a history (in
+    // the previous-Phi and learned modes), else the previous pressure (Phi = 0)
+    const bool prev_start = use_prev;
+    if (!learned) {   // capture only
+        if (prev_start) {
+            MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
+            *r_first = rp;
+        } else if (!have_prev) {
+            *r_first = rp;   // Phi = 0 was evaluated
         }
+        return MDX_OK;
     }
-    if (learned && rg < r0 && (!use_prev || rg < rp)) {
+    st->guess_div_inf = rg;
+    // the network's start wins only when it clearly beats the other start (residual below
+    // NET_MARGIN times it): a marginally smaller start residual saves no sweeps but costs an
+    // inference per solve.  Two losses in a row start the gate's back-off.
+    constexpr double NET_MARGIN = 0.8;
+    const double other = prev_start || !have_prev ? rp : -1.0;
+    if (other >= 0.0 && !(rg < NET_MARGIN * other)) {
+        h->net_losses[s] += 1;
+        if (h->net_losses[s] >= 2) h->net_skip[s] = std::min(NET_SKIP_MAX, 1 << std::min(h->net_losses[s] - 1, 6));
+    } else {
+        h->net_losses[s] = 0;
+    }
+    // the chosen start is copied into phi (captured sweep graphs hold that buffer's address)
+    if (other < 0.0 || rg < other) {
         MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
         *r_first = rg;
         return MDX_OK;
     }
-    if (learned) st->fallback = 1;
-    if (use_prev && rp < r0) {
-        if (learned) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));   // phi2 holds the network's
-        else MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
-        *r_first = rp;
-    } else {
-        if (!learned) st->fallback = 1;
-        *r_first = r0;
-    }
+    st->fallback = 1;
+    if (prev_start) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));   // phi2 holds the network's
+    *r_first = rp;
     return MDX_OK;
 }
 
@@ -1184,12 +1244,14 @@ extern "C" int mdx_step_batch(mdx_handle *const *hs, int32_t n, double dt, mdx_s
 
 extern "C" int mdx_get_phase_times(mdx_handle *h, mdx_phase_times *t) {
     MDX_CHECK(h && t, MDX_ERR_ARG, "null argument");
+    sweep_account(h, true);   // batches still queued (a discarded speculative one included)
     *t = h->times;
     return MDX_OK;
 }
 
 extern "C" int mdx_reset_phase_times(mdx_handle *h) {
     MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    sweep_account(h, true);
     h->times = mdx_phase_times{};
     return MDX_OK;
 }

@@ -148,7 +148,13 @@ struct mdx_handle {
     bool timing = false;
     hipEvent_t ev[10];
     mdx_phase_times times{};
-    int sweep_pending = 0;         // iterations of the last (unaccounted) sweep batch
+    // sweep batches not yet added to the counters: start/end events, iterations and launches
+    // (a speculative batch can be in flight while the previous one is accounted)
+    static constexpr int SW_Q = 4;
+    hipEvent_t ev_sw[SW_Q][2];
+    int sw_its[SW_Q] = {}, sw_launches[SW_Q] = {};
+    int sw_head = 0, sw_count = 0;
+    hipEvent_t ev_max = nullptr;   // completion of the residual-max copy (speculative checks)
 
     // captured sweep batches: [0] FP64, [1] FP32
     struct GraphSlot {

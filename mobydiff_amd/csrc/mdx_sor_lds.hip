@@ -35,6 +35,13 @@ static constexpr int RB_MAXR = 3;    // colour-0 ring cells per thread
 static constexpr int RB_G = 3;       // cells relaxed per group (loads issued together)
 static constexpr int RB_H = 2;       // halo depth
 static constexpr int RB_XSTRIDE = RB_HE * RB_THREADS;   // exchange entries per workgroup and buffer
+// Fixed LDS geometry: a tile row holds RB_TXP values and a plane RB_TYP rows, whatever the
+// subdomain (bx + 4 <= RB_TXP, by + 4 <= RB_TYP), so every stencil and coefficient-plane offset is
+// an immediate of the LDS instruction; the coefficient table uses the same plane shape.  The box
+// is the 24 x 16 tile of the 192 x 128 x 96 channel on 256 workgroups with its halo; its 28-value
+// rows also spread a wave's cells (one colour: every other value) well over the 32 banks (about
+// 1.7-way on average, against 2 for rows of 32 and 3 for rows of 33).
+static constexpr int RB_TXP = 28, RB_TYP = 20, RB_PLANE = RB_TXP * RB_TYP;
 
 template <typename T>
 struct RbArgs {
@@ -121,14 +128,14 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     constexpr int NW = TW::NW;
     const Dims d = a.d;
     const int bx = a.bx, by = a.by, bz = a.bz;
-    const int tx = bx + 2 * RB_H, ty = by + 2 * RB_H, tz = bz + 2 * RB_H;
-    const int cw = bx + 2, cpl = (bx + 2) * (by + 2);   // coefficient table: local i, j in [0, b+1]
+    const int tx = bx + 2 * RB_H, ty = by + 2 * RB_H, tz = bz + 2 * RB_H;   // tile extent in use
+    constexpr int cw = RB_TXP, cpl = RB_PLANE;   // coefficient table: local i, j in [0, b+1]
     T *X = reinterpret_cast<T *>(smem_raw);
     // unused slots of the per-thread cell lists point at a dummy cell past the tile, whose stencil
-    // reads and writes stay inside a padding of 2*tx*ty+3 values: the relax loops need no branches
-    const int dummy = tx * ty * tz + tx * ty + 1;
-    T *C = X + (size_t)tx * ty * tz + 2 * tx * ty + 3;   // IBM: 6 planes of cpl values
-                                                        // (planes: threads read at pair stride)
+    // reads and writes stay inside a padding of 2 planes + 3 values: the relax loops need no branches
+    const int dummy = RB_PLANE * tz + RB_PLANE + 1;
+    T *C = X + (size_t)RB_PLANE * tz + 2 * RB_PLANE + 3;   // IBM: 6 planes of cpl values
+                                                          // (planes: threads read at pair stride)
     const int nwg = a.px * a.py * a.pz;
     // XCD-aware placement: workgroups are dispatched round-robin over the 8 XCDs, so consecutive
     // subdomain ids (neighbours along x first) are given to workgroups on the same XCD.
@@ -140,7 +147,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     const int t = threadIdx.x;
     const bool per_y = a.per_y != 0;
     // LDS index of local cell (i, j, k); the subdomain is 1..b, the halo -1..0 and b+1..b+2
-    auto L = [&](int i, int j, int k) { return (i + 1) + tx * ((j + 1) + ty * (k + 1)); };
+    auto L = [&](int i, int j, int k) { return (i + 1) + RB_TXP * (j + 1) + RB_PLANE * (k + 1); };
     auto wrap = [](int g, int n) { return ((g - 1) % n + n) % n + 1; };
     // the workgroup in direction (dx, dy, dz), or -1 beyond a wall
     auto neighbour = [&](int dx, int dy, int dz) {
@@ -163,14 +170,14 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
             if (per_y) gj = wrap(gj, d.ny);
             if (gj >= 1 && gj <= d.ny) v = a.x[d.in(gi, gj, gk)];
         }
-        X[q] = v;
+        X[L(li, lj, lk)] = v;
     }
     // ---- face coefficient table (one x-y plane, ring included): mui, muip, mvj, mvjp, mw, -omega/denom
     T qconst = T(0);
     if (IBM) {
         const int sx = d.sx;
-        for (int q = t; q < cpl; q += RB_THREADS) {
-            const int li = q % cw, lj = q / cw;
+        for (int q0 = t; q0 < (bx + 2) * (by + 2); q0 += RB_THREADS) {
+            const int li = q0 % (bx + 2), lj = q0 / (bx + 2), q = li + cw * lj;
             const int gi = wrap(ox + li, d.nx);
             int gj = oy + lj;
             if (per_y) gj = wrap(gj, d.ny);
@@ -313,8 +320,9 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         }
         T denom;
         const T p0 = X[c];
-        const T r = sor_residual_core<T, false>(p0, X[c + 1], X[c - 1], X[c + tx], X[c - tx], X[c + tx * ty],
-                                                X[c - tx * ty], mui, muip, mvj, mvjp, mw, mw, (pwd >> 28) & 1,
+        const T r = sor_residual_core<T, false>(p0, X[c + 1], X[c - 1], X[c + RB_TXP], X[c - RB_TXP],
+                                                X[c + RB_PLANE], X[c - RB_PLANE], mui, muip, mvj, mvjp, mw, mw,
+                                                (pwd >> 28) & 1,
                                                 (pwd >> 29) & 1, rhs, a.s, T(0), T(0), denom);
         return p0 + r * q;
     };
@@ -424,8 +432,10 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
 
 // ------------------------------------------------------------------------------- host side
 static size_t rb_lds_bytes(int bx, int by, int bz, size_t tsize, bool ibm) {
-    const size_t tx = bx + 2 * RB_H, ty = by + 2 * RB_H, tz = bz + 2 * RB_H;
-    return (tx * ty * tz + 2 * tx * ty + 3 + (ibm ? 6 * (size_t)(bx + 2) * (by + 2) : 0)) * tsize;
+    (void)bx;
+    (void)by;
+    const size_t tz = bz + 2 * RB_H;
+    return ((size_t)RB_PLANE * tz + 2 * RB_PLANE + 3 + (ibm ? 6 * (size_t)RB_PLANE : 0)) * tsize;
 }
 
 template <typename T, bool IBM, int MAXP>
@@ -491,10 +501,10 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
                 const int nwg = px * py * pz;
                 const size_t lds = rb_lds_bytes(bx, by, bz, ts, h->ibm_on);
                 if (lds > lds_max) continue;
-                // packed per-cell words: LDS index (dummy cell and its padding included) < 2^16,
-                // coefficient-table index < 2^12
-                const size_t txy = (size_t)(bx + 2 * RB_H) * (by + 2 * RB_H);
-                if (txy * (bz + 2 * RB_H) + 2 * txy + 3 >= 65536 || (bx + 2) * (by + 2) >= 4096) continue;
+                // the fixed LDS row and plane; packed per-cell words: LDS index (dummy cell and its
+                // padding included) < 2^16, coefficient-table index < 2^12
+                if (bx + 2 * RB_H > RB_TXP || by + 2 * RB_H > RB_TYP) continue;
+                if ((size_t)RB_PLANE * (bz + 2 * RB_H) + 2 * RB_PLANE + 3 >= 65536 || RB_PLANE >= 4096) continue;
                 const int maxp = rb_maxp_class((bx / 2 * by * bz + RB_THREADS - 1) / RB_THREADS);
                 if (!maxp) continue;
                 if (rb_ring_total(bx, by, bz) > RB_MAXR * RB_THREADS) continue;

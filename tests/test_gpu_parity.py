@@ -270,7 +270,7 @@ def _wavy(nx, ny, nz, per_y=0, ibm=True):
 
 
 @pytest.mark.parametrize("shape,per_y,ibm", [((48, 32, 24), 0, True), ((32, 16, 16), 1, False),
-                                             ((192, 128, 96), 0, True)])
+                                             ((96, 64, 48), 0, True)])
 def test_lds_sweeps_bit_exact_fixed_iterations(shape, per_y, ibm, monkeypatch):
     monkeypatch.setenv("MOBYDIFF_LDS_SOR", "2")   # FP64 batches on the LDS-resident solver too
     g, coef = _wavy(*shape, per_y=per_y, ibm=ibm)
@@ -384,6 +384,27 @@ def test_fused_mixed_solve_equals_one_colour_sweeps(monkeypatch):
         assert st.converged == 1
         outs.append(ch.download())
         ch.close()
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_lds_fp32_batches_equal_streaming_at_production_shape(monkeypatch):
+    """192 x 128 x 96 wavy channel, mixed precision, a fixed 60 iterations (three FP32 batches of
+    20 between FP64 defect corrections): the LDS-resident batches (the default) and the streaming
+    one-colour sweeps give bit-identical fields."""
+    g, coef = _wavy(192, 128, 96)
+    rng = np.random.default_rng(24)
+    f = O.random_fields(g, rng, amp=0.05)
+    outs, stats = [], []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("MOBYDIFF_LDS_SOR", mode)
+        ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-300, max_iter=60, check_every=20,
+                      precision=M.PREC_MIXED, inner_iters=20)
+        stats.append(ch.pressure_projection(0.02))
+        outs.append(ch.download(("un", "vn", "wn", "pn")))
+        ch.close()
+    assert stats[0].lds_sweeps == 1 and stats[1].lds_sweeps == 0
+    assert stats[0].iterations == stats[1].iterations == 60
     for k in ("un", "vn", "wn", "pn"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
 

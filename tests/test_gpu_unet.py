@@ -202,30 +202,29 @@ def test_learned_guess_in_the_solver():
 
 
 def test_previous_phi_guess_in_the_solver():
-    """The non-learned ablation: the previous-step Phi of the same stage as the initial guess."""
+    """The non-learned ablation: the previous-step Phi of the same stage, scaled by the ratio of the
+    stages' dt*gamma, as the initial guess.  The same flow with the plain start (the previous
+    pressure, Phi = 0) needs more sweeps after the first step; both meet the tolerance."""
     nx, ny, nz, dt = 64, 32, 32, 5e-4
     geo = M.wavy_geometry(A=0.1)
-    cfg = M.channel_config(nx, ny, nz, 4.0, 2.2, 2.0, 2000.0, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8,
-                           check_every=10, guess=M.GUESS_PHI_PREV)
-    ch = M.Channel(cfg)
-    ch.compute_ibm(geo)
-    ch.init_fields(seed=0, amp=0.05, geo=geo, dt=dt)
-    first = ch.step(dt)
-    # no history yet: the plain start, nothing evaluated
-    assert all(first.stage[q].fallback == 0 and first.stage[q].guess_div_inf == 0.0 for q in range(3))
-    kept = 0
-    for _ in range(4):
-        st = ch.step(dt)
-        for q in range(3):
-            s = st.stage[q]
-            assert s.converged == 1 and s.div_inf < 1e-8
-            assert s.guess_div_inf > 0.0
-            if s.fallback:
-                # rejected: the solve started from Phi = 0, whose residual was not larger
-                assert s.div0_inf <= s.guess_div_inf
-            else:
-                kept += 1
-                assert s.guess_div_inf == s.div0_inf
-    # after the start-up transient the previous step's Phi is the better start
-    assert kept >= 6, kept
-    ch.close()
+    its = {}
+    for guess in (M.GUESS_PREVIOUS, M.GUESS_PHI_PREV):
+        cfg = M.channel_config(nx, ny, nz, 4.0, 2.2, 2.0, 2000.0, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8,
+                               check_every=10, guess=guess)
+        ch = M.Channel(cfg)
+        ch.compute_ibm(geo)
+        ch.init_fields(seed=0, amp=0.05, geo=geo, dt=dt)
+        first = ch.step(dt)
+        # no history yet: the plain start, nothing rejected
+        assert all(first.stage[q].fallback == 0 for q in range(3))
+        its[guess] = []
+        for _ in range(4):
+            st = ch.step(dt)
+            for q in range(3):
+                s = st.stage[q]
+                assert s.converged == 1 and s.div_inf < 1e-8 and s.fallback == 0
+                its[guess].append(s.iterations)
+        assert ch.div_norm() < 1.5e-8
+        ch.close()
+    print("sweeps per solve: plain start", its[M.GUESS_PREVIOUS], "previous Phi", its[M.GUESS_PHI_PREV])
+    assert sum(its[M.GUESS_PHI_PREV]) < sum(its[M.GUESS_PREVIOUS]), its
