@@ -206,7 +206,8 @@ static void free_all(mdx_handle *h) {
         for (auto &q : h->ev_sw)
             for (auto &e : q) (void)hipEventDestroy(e);
     }
-    if (h->ev_max) (void)hipEventDestroy(h->ev_max);
+    for (auto &e : h->ev_max)
+        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -375,7 +376,8 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             for (auto &ev : q)
                 if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "events");
     }
-    if ((e = hipEventCreateWithFlags(&h->ev_max, hipEventDisableTiming)) != hipSuccess) return fail(e, "events");
+    for (auto &ev : h->ev_max)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return fail(e, "events");
     if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "synchronize");
     *out = h;
     return MDX_OK;
@@ -732,7 +734,7 @@ static void sweep_account(mdx_handle *h, bool wait = false) {
     }
 }
 
-static int sweep_batch(mdx_handle *h, bool fp32, int m) {
+static int sweep_batch(mdx_handle *h, bool fp32, int m, const RbDefect *df = nullptr) {
     int qs = -1;   // accounting queue entry of this batch
     if (h->timing) {
         if (h->sw_count == mdx_handle::SW_Q) sweep_account(h, true);
@@ -743,11 +745,12 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m) {
     auto &slot = h->graphs[fp32 ? 1 : 0];
     if (lds_for(h, fp32)) {
         // LDS-resident sweeps; the FP32 defect solve starts from zero inside the kernel
-        hipError_t e = launch_rb_lds(h, fp32, m, fp32 ? 1 : 0);
+        hipError_t e = launch_rb_lds(h, fp32, m, fp32 ? 1 : 0, df);
         launches = 1;
         if (e != hipSuccess) {
             // the runtime refused the cooperative launch (nothing ran): use the streaming sweeps
             (void)hipGetLastError();
+            launches = 0;
             h->rb_state = -1;
             h->rb_note = std::string("streaming sweeps: cooperative launch refused (") + hipGetErrorString(e) + ")";
             if (fp32) MDX_HIP(launch_zero32(h, h->e32, h->ni));   // the streaming sweeps start from e = 0
@@ -776,6 +779,9 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m) {
         MDX_HIP(sweep_launches(h, fp32, m));
     }
     if (!launches) launches = (h->sweep_fused ? 1 : 2) * m;   // streaming: one launch per colour
+    // a batch that should have ended with the LDS defect epilogue but ran on the streaming sweeps
+    // (cooperative launch refused): the separate defect kernel forms the same check
+    if (df && !lds_for(h, fp32)) MDX_HIP(launch_defect64(h, df->phi_in, h->e32, df->phi_out, h->r32));
     if (qs >= 0) {
         MDX_HIP(hipEventRecord(h->ev_sw[qs][1], h->stream));
         h->sw_its[qs] = m;
@@ -824,23 +830,39 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
 // what the synchronous loop gives.  The batch is queued early only when the residuals seen so far
 // predict (geometric decay) that check k is still at least SPEC_MARGIN times above the tolerance,
 // so a wasted batch is rare while the GPU does not idle through the host's read of the residual.
+// On the LDS-resident solver a batch ends with the next check itself (its defect epilogue).
 static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
+    constexpr int PIN = 16;   // pinned-buffer words of the two check slots: [PIN + 4 * slot]
     const mdx_config &c = h->cfg;
     const bool lds = lds_for(h, true);
     double r = 0.0, r1 = -1.0, r2 = -1.0;   // r_k, r_{k-1}, r_{k-2}
-    int it = 0, checks = 0;
+    int it = 0, checks = 0, slot = 0;
     MDX_HIP(launch_zero32(h, h->e32, h->ni));
-    auto enqueue_check = [&]() -> int {
+    auto post = [&](int sl) -> int {   // the check's max (and the LDS wait flag) to the host
+        MDX_HIP(hipMemcpyAsync(h->h_pinned + PIN + 4 * sl, h->red_max, 16, hipMemcpyDeviceToHost, h->stream));
+        MDX_HIP(hipEventRecord(h->ev_max[sl], h->stream));
+        return MDX_OK;
+    };
+    auto enqueue_check = [&](int sl) -> int {   // separate defect kernel: Phi += e, r32, max
         MDX_HIP(launch_reset_max(h));
         MDX_HIP(launch_defect64(h, h->phi, h->e32, h->phi2, h->r32));
         std::swap(h->phi, h->phi2);
-        MDX_HIP(hipMemcpyAsync(h->h_pinned, h->red_max, 16, hipMemcpyDeviceToHost, h->stream));
-        MDX_HIP(hipEventRecord(h->ev_max, h->stream));
-        return MDX_OK;
+        return post(sl);
+    };
+    auto enqueue_batch = [&](int m, int sl) -> int {   // m iterations; on LDS also the next check
+        if (lds) {
+            const RbDefect df{h->phi, h->phi2};
+            int rc_ = sweep_batch(h, true, m, &df);
+            if (rc_) return rc_;
+            std::swap(h->phi, h->phi2);
+            return post(sl);
+        }
+        MDX_HIP(launch_zero32(h, h->e32, h->ni));
+        return sweep_batch(h, true, m);
     };
     int rc;
-    if ((rc = enqueue_check())) return rc;
+    if ((rc = enqueue_check(slot))) return rc;
     for (;;) {
         const int m = std::min(c.inner_iters, c.max_iter - it);
         bool spec = false;
@@ -848,14 +870,11 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
             const double pred = r2 > 0.0 ? r1 * std::min(1.0, r1 / r2) : r1 / SPEC_FIRST;
             spec = pred > SPEC_MARGIN * c.tol;
         }
-        if (spec) {
-            if (!lds) MDX_HIP(launch_zero32(h, h->e32, h->ni));
-            if ((rc = sweep_batch(h, true, m))) return rc;
-        }
-        MDX_HIP(hipEventSynchronize(h->ev_max));
-        r = h->h_pinned[0];
+        if (spec && (rc = enqueue_batch(m, slot ^ 1))) return rc;
+        MDX_HIP(hipEventSynchronize(h->ev_max[slot]));
+        r = h->h_pinned[PIN + 4 * slot];
         int err;
-        std::memcpy(&err, &h->h_pinned[1], sizeof err);
+        std::memcpy(&err, &h->h_pinned[PIN + 4 * slot + 1], sizeof err);
         if (err) {
             mdx_set_error("LDS red-black solver: a neighbour wait exceeded its spin limit");
             return MDX_ERR_HIP;
@@ -863,15 +882,17 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
         sweep_account(h);
         if (checks == 0) st->div0_inf = r;
         ++checks;
-        if (!(r >= c.tol) || it >= c.max_iter) break;   // a speculative batch is discarded
-        if (!spec) {
-            if (!lds) MDX_HIP(launch_zero32(h, h->e32, h->ni));
-            if ((rc = sweep_batch(h, true, m))) return rc;
+        if (!(r >= c.tol) || it >= c.max_iter) {
+            // a speculative batch is discarded: on LDS its check wrote the other Phi buffer
+            if (spec && lds) std::swap(h->phi, h->phi2);
+            break;
         }
+        if (!spec && (rc = enqueue_batch(m, slot ^ 1))) return rc;
         it += m;
         r2 = r1;
         r1 = r;
-        if ((rc = enqueue_check())) return rc;
+        if (!lds && (rc = enqueue_check(slot ^ 1))) return rc;
+        slot ^= 1;
     }
     st->iterations = it;
     st->checks = checks;
@@ -881,14 +902,36 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     return MDX_OK;
 }
 
+// Learned-guess gate bookkeeping for stage s.  Two losses in a row make the network's start skip
+// 4 solves, then 16, 64 and NET_SKIP_MAX between evaluations; a win resets the count.
+static constexpr int NET_SKIP_MAX = 256;
+static void net_loss(mdx_handle *h, int s) {
+    h->net_losses[s] += 1;
+    if (h->net_losses[s] >= 2) h->net_skip[s] = std::min(NET_SKIP_MAX, 1 << std::min(2 * (h->net_losses[s] - 1), 8));
+}
+
+// After the solve of stage s: a network start wins when its solve needed at most NET_SWEEPS of the
+// sweeps of the last solve of this stage that started otherwise; that solve's count is the
+// yardstick for the next network start.
+static void net_outcome(mdx_handle *h, int s, int iterations) {
+    constexpr double NET_SWEEPS = 0.9;
+    if (!h->net_started[s]) {
+        h->base_its[s] = iterations;
+        return;
+    }
+    if (h->base_its[s] > 0 && iterations <= NET_SWEEPS * h->base_its[s]) h->net_losses[s] = 0;
+    else net_loss(h, s);
+}
+
 // Initial guess of the SOR solve for stage s; the solve itself, and its stopping rule, are the
-// same whatever starts it.  Candidates, each judged by the max-norm of its FP64 residual:
-//   Phi = 0 (the previous pressure: the plain method);
+// same whatever starts it.  Starts:
+//   Phi = 0 (the previous pressure: the plain method, and the start without a history);
 //   Phi_0 = fac * Phi of stage s at the previous step (MDX_GUESS_PHI_PREV and MDX_GUESS_LEARNED);
-//   Phi_0 + the network's correction, predicted from (rhs - L_mu Phi_0, Phi_0) (MDX_GUESS_LEARNED).
-// The smallest residual starts the solve.  In learned mode a start other than the network's is a
-// fallback (stats.fallback).  With capture on, the network input is built for every solve so the
-// converged Phi can be stored as a training pair.
+//   Phi_0 + the network's correction, predicted from (rhs - L_mu Phi_0, Phi_0) (MDX_GUESS_LEARNED),
+//   taken when its FP64 residual max-norm is below NET_MARGIN times that of Phi_0.
+// In learned mode a start other than the network's is a fallback (stats.fallback).  With capture
+// on, the network input is built for every solve so the converged Phi can be stored as a training
+// pair.
 static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, double *r_first) {
     mdx_net *n = h->net;
     const bool have_prev = h->hist_dtg[s] > 0.0;
@@ -896,9 +939,9 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
     const double fac = have_prev ? dtg / h->hist_dtg[s] : 0.0;
     const int mode = h->cfg.guess;
     bool learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0;
+    h->net_started[s] = false;
     // gate: after the network's start has lost twice in a row for this stage, it is evaluated
-    // again only after 2, 4, ..., at most NET_SKIP_MAX skipped solves (previous-Phi start then)
-    constexpr int NET_SKIP_MAX = 64;
+    // again only after 4, 16, 64, then every NET_SKIP_MAX skipped solves (previous-Phi start then)
     if (learned && h->net_skip[s] > 0) {
         h->net_skip[s] -= 1;
         learned = false;
@@ -955,23 +998,20 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
         return MDX_OK;
     }
     st->guess_div_inf = rg;
-    // the network's start wins only when it clearly beats the other start (residual below
+    // the network's start is taken only when it clearly beats the other start (residual below
     // NET_MARGIN times it): a marginally smaller start residual saves no sweeps but costs an
-    // inference per solve.  Two losses in a row start the gate's back-off.
+    // inference per solve.  A start not taken counts as a loss; a start taken is judged by the
+    // sweeps its solve needs (net_outcome).
     constexpr double NET_MARGIN = 0.8;
     const double other = prev_start || !have_prev ? rp : -1.0;
-    if (other >= 0.0 && !(rg < NET_MARGIN * other)) {
-        h->net_losses[s] += 1;
-        if (h->net_losses[s] >= 2) h->net_skip[s] = std::min(NET_SKIP_MAX, 1 << std::min(h->net_losses[s] - 1, 6));
-    } else {
-        h->net_losses[s] = 0;
-    }
     // the chosen start is copied into phi (captured sweep graphs hold that buffer's address)
-    if (other < 0.0 || rg < other) {
+    if (other < 0.0 || rg < NET_MARGIN * other) {
         MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
         *r_first = rg;
+        h->net_started[s] = true;
         return MDX_OK;
     }
+    net_loss(h, s);
     st->fallback = 1;
     if (prev_start) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));   // phi2 holds the network's
     *r_first = rp;
@@ -1065,6 +1105,7 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st) : sor_solve64(h, &st, r_first);
         if (rc) return rc;
         if (st.fallback == 0 && r_first >= 0.0) st.div0_inf = r_first;
+        net_outcome(h, stage, st.iterations);
         if (h->net && h->net_capture && st.converged && (rc = capture_pair(h, stage, dtg))) return rc;
         // the converged Phi becomes the history of this stage for the next step
         MDX_HIP(hipMemcpyAsync(h->phi_hist + (size_t)stage * h->ni, h->phi, h->ni * 8, hipMemcpyDeviceToDevice,
@@ -1088,7 +1129,10 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
 
 extern "C" int mdx_set_guess(mdx_handle *h, int32_t guess) {
     MDX_CHECK(h, MDX_ERR_ARG, "null handle");
-    for (int q = 0; q < 3; ++q) h->net_losses[q] = h->net_skip[q] = 0;
+    for (int q = 0; q < 3; ++q) {
+        h->net_losses[q] = h->net_skip[q] = h->base_its[q] = 0;
+        h->net_started[q] = false;
+    }
     MDX_CHECK(guess == MDX_GUESS_PREVIOUS || guess == MDX_GUESS_LEARNED || guess == MDX_GUESS_PHI_PREV,
               MDX_ERR_ARG, "mdx_set_guess: unknown guess mode");
     MDX_CHECK(guess == MDX_GUESS_PREVIOUS || h->cfg.solver == MDX_SOLVER_SOR, MDX_ERR_UNSUPPORTED,

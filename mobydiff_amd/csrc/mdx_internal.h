@@ -154,7 +154,8 @@ struct mdx_handle {
     hipEvent_t ev_sw[SW_Q][2];
     int sw_its[SW_Q] = {}, sw_launches[SW_Q] = {};
     int sw_head = 0, sw_count = 0;
-    hipEvent_t ev_max = nullptr;   // completion of the residual-max copy (speculative checks)
+    hipEvent_t ev_max[2] = {nullptr, nullptr};   // residual-max copies of the mixed solve's checks
+                                                 // (two in flight with speculative batches)
 
     // captured sweep batches: [0] FP64, [1] FP32
     struct GraphSlot {
@@ -190,8 +191,12 @@ struct mdx_handle {
     double *phi_hist = nullptr;
     double *res64 = nullptr;       // residual field of the previous-Phi guess (network input)
     double hist_dtg[3] = {0.0, 0.0, 0.0};
-    // learned-guess gate per stage: consecutive solves the network's start lost, solves left to skip
+    // learned-guess gate per stage: consecutive solves the network's start lost, solves left to
+    // skip, whether the current solve starts from the network's guess, and the sweeps of the last
+    // solve that did not (the yardstick of a network start)
     int net_losses[3] = {0, 0, 0}, net_skip[3] = {0, 0, 0};
+    bool net_started[3] = {false, false, false};
+    int base_its[3] = {0, 0, 0};
     long proj_count = 0;
 };
 
@@ -229,7 +234,12 @@ hipError_t launch_zero64(mdx_handle *h, double *p, size_t n);
 hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, double fac, size_t n);
 hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *field);
 bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);
-hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init);
+// defect epilogue of an FP32 LDS batch: phi_out = phi_in + e, r32 = defect, max_part = its maxima
+struct RbDefect {
+    const double *phi_in;
+    double *phi_out;
+};
+hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df = nullptr);
 size_t rb_faces_bytes(const RbPlanOut &p, bool fp32);
 
 // FFT path

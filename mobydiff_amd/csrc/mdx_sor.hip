@@ -146,40 +146,7 @@ static MuK<T> mu_k(const mdx_handle *h, T *const *tabs) {
 }
 
 // ------------------------------------------------------------------------------- residual stencil
-// residual r = rhs - L_mu(Phi) of cell (i,j,k) and the reference's relaxation denominator
-template <typename T, bool STRETCH, bool IBM, typename PhiAt>
-__device__ inline T cell_residual(const GridK &g, const MuK<T> &mu, const SorConst<T> &s, PhiAt P,
-                                  T rhs_c, int i, int j, int k, T &denom) {
-    const Dims d = g.d;
-    const int nx = d.nx, ny = d.ny, nz = d.nz;
-    const bool per2 = g.per[1];
-    const int ip = i == nx ? 1 : i + 1, im = i == 1 ? nx : i - 1;
-    const int kp = k == nz ? 1 : k + 1, km = k == 1 ? nz : k - 1;
-    int jp = j + 1, jm = j - 1;
-    if (per2) {
-        if (jp > ny) jp = 1;
-        if (jm < 1) jm = ny;
-    }
-    T mui = 1, muip = 1, mvj = 1, mvjp = 1, mwk = 1, mwkp = 1;
-    if (IBM) {
-        mui = mu.u[mu.at(i, j, k)];
-        muip = mu.u[mu.at(ip, j, k)];
-        mvj = mu.v[mu.at(i, j, k)];
-        mvjp = mu.v[mu.at(i, jp, k)];
-        mwk = mu.w[mu.at(i, j, k)];
-        mwkp = mu.w[mu.at(i, j, kp)];
-    }
-    const bool wn_ = !per2 && j == ny, ws_ = !per2 && j == 1;
-    const T p0 = P(i, j, k);
-    const T pyp = P(i, wn_ ? j : jp, k), pym = P(i, ws_ ? j : jm, k);
-    T an = 0, as = 0;
-    if (STRETCH) {
-        an = (T)(g.ihv[j + 1] * g.ihc[j]);
-        as = (T)(g.ihv[j] * g.ihc[j]);
-    }
-    return sor_residual_core<T, STRETCH>(p0, P(ip, j, k), P(im, j, k), pyp, pym, P(i, j, kp), P(i, j, km), mui,
-                                         muip, mvj, mvjp, mwk, mwkp, wn_, ws_, rhs_c, s, an, as, denom);
-}
+// cell_residual (mdx_sor_core.h): r = rhs - L_mu(Phi) of one cell
 
 template <typename T>
 struct PlainPhi {

@@ -60,6 +60,18 @@ struct RbArgs {
     int total;               // exchange entries in use per workgroup (faces two deep, then edges)
     unsigned tag0;           // iteration s of this launch publishes tag tag0 + s + 1 (never 0)
     int *err;
+    // Defect epilogue (the FP32 batches of the mixed solve): after the last iteration and one more
+    // halo exchange, the corrected iterate phi_out = phi_in + e, the new defect
+    // r_out = (float)(rhs64 - L_mu phi_out) and the workgroup's max |defect| (max_part[wg]) are
+    // formed here, with the arithmetic of the separate defect kernel (cell_residual over
+    // phi_in + (double)e), instead of writing e back for that kernel.
+    int defect;
+    GridK g;
+    MuK<double> mu64;
+    SorConst<double> s64;
+    const double *phi_in, *rhs64;
+    double *phi_out, *max_part;
+    float *r_out;
 };
 
 // a value as tagged 64-bit words: the tag in the high half, 32 payload bits in the low half
@@ -370,7 +382,8 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
             __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
-        if (s == a.iters - 1) break;
+        // the last iteration needs no exchange, unless the defect epilogue reads the final halo
+        if (s == a.iters - 1 && !a.defect) break;
 
         // ---- publish this subdomain's layers, tagged with this iteration
         const int buf = s & 1;
@@ -423,11 +436,67 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
             X[xdst[jj] & 0xfffff] = TW::unpack(got[jj]);
         __syncthreads();
     }
-    // ---- write the subdomain back
-    for (int q = t; q < bx * by * bz; q += RB_THREADS) {
-        const int i = q % bx + 1, j = (q / bx) % by + 1, k = q / (bx * by) + 1;
-        a.x[d.in(ox + i, oy + j, oz + k)] = X[L(i, j, k)];
+    if (!a.defect) {
+        // ---- write the subdomain back
+        for (int q = t; q < bx * by * bz; q += RB_THREADS) {
+            const int i = q % bx + 1, j = (q / bx) % by + 1, k = q / (bx * by) + 1;
+            a.x[d.in(ox + i, oy + j, oz + k)] = X[L(i, j, k)];
+        }
+        return;
     }
+    // ---- defect epilogue: the halo holds the neighbours' final e (exchanged above).  A thread
+    // owns a column (i, j) of the subdomain and marches k, keeping Phi = phi_in + e of planes
+    // k-1, k, k+1 of its column and loading the plane after next ahead; the face coefficients of
+    // a column do not change with k (z-invariant tables).  The operands handed to
+    // sor_residual_core are those cell_residual forms, so the defect equals the separate kernel's.
+    double mx = 0.0;
+    if (t < bx * by) {
+        const int li = t % bx + 1, lj = t / bx + 1;
+        const int i = ox + li, j = oy + lj, nx = d.nx, ny = d.ny, nz = d.nz;
+        const bool per2 = per_y;
+        const int ip = i == nx ? 1 : i + 1, im = i == 1 ? nx : i - 1;
+        int jp = j + 1, jm = j - 1;
+        if (per2) {
+            if (jp > ny) jp = 1;
+            if (jm < 1) jm = ny;
+        }
+        const bool wn_ = !per2 && j == ny, ws_ = !per2 && j == 1;
+        const int jn = wn_ ? j : jp, js = ws_ ? j : jm;
+        const int ljn = wn_ ? lj : lj + 1, ljs = ws_ ? lj : lj - 1;   // LDS rows of the y neighbours
+        double mui = 1, muip = 1, mvj = 1, mvjp = 1, mwk = 1, mwkp = 1;
+        if (IBM) {
+            mui = a.mu64.u[a.mu64.at(i, j, 1)];
+            muip = a.mu64.u[a.mu64.at(ip, j, 1)];
+            mvj = a.mu64.v[a.mu64.at(i, j, 1)];
+            mvjp = a.mu64.v[a.mu64.at(i, jp, 1)];
+            mwk = a.mu64.w[a.mu64.at(i, j, 1)];
+            mwkp = mwk;   // z-invariant table
+        }
+        const double *__restrict__ pin = a.phi_in;
+        auto Pc = [&](int gi, int gj, int gk, int xi, int xj, int xk) {   // phi_in + e of one cell
+            return pin[d.in(gi, gj, gk)] + (double)X[L(xi, xj, xk)];
+        };
+        auto kw = [&](int k) { return k < 1 ? k + nz : (k > nz ? k - nz : k); };
+        // z neighbours of the first plane (k = oz + 1): its lower neighbour wraps periodically
+        double pm = Pc(i, j, kw(oz), li, lj, 0), pc = Pc(i, j, oz + 1, li, lj, 1);
+        for (int lk = 1; lk <= bz; ++lk) {
+            const int k = oz + lk;
+            const double pn = Pc(i, j, kw(k + 1), li, lj, lk + 1);
+            const double pxp = Pc(ip, j, k, li + 1, lj, lk), pxm = Pc(im, j, k, li - 1, lj, lk);
+            const double pyp = Pc(i, jn, k, li, ljn, lk), pym = Pc(i, js, k, li, ljs, lk);
+            const size_t c = d.in(i, j, k);
+            double denom;
+            const double r = sor_residual_core<double, false>(pc, pxp, pxm, pyp, pym, pn, pm, mui, muip, mvj, mvjp,
+                                                              mwk, mwkp, wn_, ws_, a.rhs64[c], a.s64, 0.0, 0.0,
+                                                              denom);
+            a.phi_out[c] = pc;
+            a.r_out[c] = (float)r;
+            mx = nanmax_any(mx, fabs(r));
+            pm = pc;
+            pc = pn;
+        }
+    }
+    block_max_part(mx, a.max_part);
 }
 
 // ------------------------------------------------------------------------------- host side
@@ -533,8 +602,22 @@ size_t rb_faces_bytes(const RbPlanOut &p, bool fp32) {
 
 template <typename T>
 static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T *rhs, int zero_init, int iters,
-                              T *const *mu) {
+                              T *const *mu, const RbDefect *df) {
     RbArgs<T> a;
+    a.defect = df != nullptr;
+    a.g = grid_k(h);
+    a.mu64.u = h->ibm_on ? h->mu64[0] : nullptr;
+    a.mu64.v = h->ibm_on ? h->mu64[1] : nullptr;
+    a.mu64.w = h->ibm_on ? h->mu64[2] : nullptr;
+    a.mu64.plane = h->coef_plane ? 1 : 0;
+    a.mu64.sx = h->d.sx;
+    a.mu64.sy = h->d.sy;
+    a.s64 = SorConst<double>{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    a.phi_in = df ? df->phi_in : nullptr;
+    a.rhs64 = h->rhs;
+    a.phi_out = df ? df->phi_out : nullptr;
+    a.max_part = h->max_part;
+    a.r_out = h->r32;
     a.d = h->d;
     a.px = pl.px; a.py = pl.py; a.pz = pl.pz;
     a.bx = pl.bx; a.by = pl.by; a.bz = pl.bz;
@@ -568,14 +651,17 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds);
     if (e != hipSuccess) return e;
     void *args[] = {&a};
-    return hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args,
-                                      (unsigned)pl.lds, h->stream);
+    e = hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args, (unsigned)pl.lds,
+                                   h->stream);
+    if (e != hipSuccess || !df) return e;
+    return launch_max_final(h, (unsigned)(pl.px * pl.py * pl.pz));
 }
 
-hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init) {
+hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df) {
     const RbPlanOut &pl = fp32 ? h->rb_plan32 : h->rb_plan64;
-    if (fp32) return rb_launch_t<float>(h, pl, h->e32, h->r32, zero_init, iters, h->mu32);
-    return rb_launch_t<double>(h, pl, h->phi, h->rhs, zero_init, iters, h->mu64);
+    if (fp32) return rb_launch_t<float>(h, pl, h->e32, h->r32, zero_init, iters, h->mu32, df);
+    if (df) return hipErrorInvalidValue;   // the defect epilogue serves the FP32 batches only
+    return rb_launch_t<double>(h, pl, h->phi, h->rhs, zero_init, iters, h->mu64, nullptr);
 }
 
 }  // namespace mdx
