@@ -788,6 +788,8 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m, const RbDefect *df = nul
         h->sw_launches[qs] = launches;
         h->sw_count += 1;
     }
+    // the LDS batch's defect epilogue left per-workgroup maxima: fold them (outside the timing)
+    if (df && lds_for(h, fp32)) MDX_HIP(launch_max_final(h, rb_workgroups(h, fp32)));
     return MDX_OK;
 }
 

@@ -240,6 +240,7 @@ struct RbDefect {
     double *phi_out;
 };
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df = nullptr);
+unsigned rb_workgroups(const mdx_handle *h, bool fp32);
 size_t rb_faces_bytes(const RbPlanOut &p, bool fp32);
 
 // FFT path

@@ -651,10 +651,15 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds);
     if (e != hipSuccess) return e;
     void *args[] = {&a};
-    e = hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args, (unsigned)pl.lds,
-                                   h->stream);
-    if (e != hipSuccess || !df) return e;
-    return launch_max_final(h, (unsigned)(pl.px * pl.py * pl.pz));
+    // with the defect epilogue the workgroup maxima are in max_part; the caller folds them
+    // (launch_max_final over rb_workgroups()) after timing the batch
+    return hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args, (unsigned)pl.lds,
+                                      h->stream);
+}
+
+unsigned rb_workgroups(const mdx_handle *h, bool fp32) {
+    const RbPlanOut &pl = fp32 ? h->rb_plan32 : h->rb_plan64;
+    return (unsigned)(pl.px * pl.py * pl.pz);
 }
 
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df) {
