@@ -319,6 +319,8 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         h->sweep_kz = v;
     }
     if ((e = hipHostMalloc((void **)&h->h_pinned, 64 * sizeof(double))) != hipSuccess) return fail(e, "pinned buffer");
+    if ((e = hipHostGetDevicePointer((void **)&h->d_pinned, h->h_pinned, 0)) != hipSuccess)
+        return fail(e, "pinned buffer device address");
 
     if (cfg->solver == MDX_SOLVER_SOR) {
         if ((e = dalloc(&h->phi, h->ni)) != hipSuccess || (e = dalloc(&h->phi2, h->ni)) != hipSuccess ||
@@ -781,15 +783,13 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m, const RbDefect *df = nul
     if (!launches) launches = (h->sweep_fused ? 1 : 2) * m;   // streaming: one launch per colour
     // a batch that should have ended with the LDS defect epilogue but ran on the streaming sweeps
     // (cooperative launch refused): the separate defect kernel forms the same check
-    if (df && !lds_for(h, fp32)) MDX_HIP(launch_defect64(h, df->phi_in, h->e32, df->phi_out, h->r32));
+    if (df && !lds_for(h, fp32)) MDX_HIP(launch_defect64(h, df->phi_in, h->e32, df->phi_out, h->r32, df->host_slot));
     if (qs >= 0) {
         MDX_HIP(hipEventRecord(h->ev_sw[qs][1], h->stream));
         h->sw_its[qs] = m;
         h->sw_launches[qs] = launches;
         h->sw_count += 1;
     }
-    // the LDS batch's defect epilogue left per-workgroup maxima: fold them (outside the timing)
-    if (df && lds_for(h, fp32)) MDX_HIP(launch_max_final(h, rb_workgroups(h, fp32)));
     return MDX_OK;
 }
 
@@ -841,20 +841,22 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     double r = 0.0, r1 = -1.0, r2 = -1.0;   // r_k, r_{k-1}, r_{k-2}
     int it = 0, checks = 0, slot = 0;
     MDX_HIP(launch_zero32(h, h->e32, h->ni));
-    auto post = [&](int sl) -> int {   // the check's max (and the LDS wait flag) to the host
-        MDX_HIP(hipMemcpyAsync(h->h_pinned + PIN + 4 * sl, h->red_max, 16, hipMemcpyDeviceToHost, h->stream));
+    // a check's kernels write [max, LDS error word] straight into pinned slot sl; an event after
+    // them tells the host the words are there
+    auto slot_dev = [&](int sl) { return h->d_pinned + PIN + 4 * sl; };
+    auto post = [&](int sl) -> int {
         MDX_HIP(hipEventRecord(h->ev_max[sl], h->stream));
         return MDX_OK;
     };
     auto enqueue_check = [&](int sl) -> int {   // separate defect kernel: Phi += e, r32, max
         MDX_HIP(launch_reset_max(h));
-        MDX_HIP(launch_defect64(h, h->phi, h->e32, h->phi2, h->r32));
+        MDX_HIP(launch_defect64(h, h->phi, h->e32, h->phi2, h->r32, slot_dev(sl)));
         std::swap(h->phi, h->phi2);
         return post(sl);
     };
     auto enqueue_batch = [&](int m, int sl) -> int {   // m iterations; on LDS also the next check
         if (lds) {
-            const RbDefect df{h->phi, h->phi2};
+            const RbDefect df{h->phi, h->phi2, slot_dev(sl)};
             int rc_ = sweep_batch(h, true, m, &df);
             if (rc_) return rc_;
             std::swap(h->phi, h->phi2);

@@ -141,7 +141,8 @@ struct mdx_handle {
     double *red_part = nullptr;    // per-block partial results
     unsigned long long *red_max = nullptr;   // [0] max result (double bits), [1] LDS-solver error word
     double *max_part = nullptr;    // per-workgroup maxima (first pass of the max reductions)
-    double *h_pinned = nullptr;    // pinned host staging (8 doubles)
+    double *h_pinned = nullptr;    // pinned host staging (64 doubles)
+    double *d_pinned = nullptr;    // the same memory as the device addresses it
     int red_blocks = 0;
 
     // timing
@@ -215,7 +216,9 @@ hipError_t launch_flow_shift(mdx_handle *h, double dtg, const double *du_dev);
 hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int use_geo,
                                 const double *hx_u, double ymid);
 hipError_t launch_reset_max(mdx_handle *h);
-hipError_t launch_max_final(mdx_handle *h, unsigned nparts);   // max_part[0..n) -> red_max[0]
+// max_part[0..n) -> red_max[0]; with host_slot (device address of pinned words) also the max and
+// the LDS-solver error word to host_slot[0..1]
+hipError_t launch_max_final(mdx_handle *h, unsigned nparts, double *host_slot = nullptr);
 static constexpr unsigned MAX_PARTS = 1u << 17;
 
 // red-black solver
@@ -225,7 +228,8 @@ hipError_t launch_sor_prepare(mdx_handle *h);
 hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega);
 hipError_t launch_sor_residual64(mdx_handle *h, const double *phi);
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg);
-hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32);
+hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32,
+                           double *host_slot = nullptr);
 hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega);
 hipError_t launch_sor_iters64(mdx_handle *h, int m, double omega);
 hipError_t launch_sor_iters32(mdx_handle *h, int m, float omega);
@@ -238,9 +242,9 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);
 struct RbDefect {
     const double *phi_in;
     double *phi_out;
+    double *host_slot;   // device address of the pinned words receiving [max, error word]
 };
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df = nullptr);
-unsigned rb_workgroups(const mdx_handle *h, bool fp32);
 size_t rb_faces_bytes(const RbPlanOut &p, bool fp32);
 
 // FFT path

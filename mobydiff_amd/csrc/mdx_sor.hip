@@ -723,7 +723,8 @@ __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, 
     block_max_part(m, out);
 }
 
-hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32) {
+hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32,
+                           double *host_slot) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
@@ -735,7 +736,7 @@ hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, 
     else { if (h->ibm_on) MDX_DF(false, true); else MDX_DF(false, false); }
 #undef MDX_DF
     hipError_t err = hipGetLastError();
-    return err != hipSuccess ? err : launch_max_final(h, grd.x);
+    return err != hipSuccess ? err : launch_max_final(h, grd.x, host_slot);
 }
 
 template <bool STRETCH, bool IBM>

@@ -72,6 +72,10 @@ struct RbArgs {
     const double *phi_in, *rhs64;
     double *phi_out, *max_part;
     float *r_out;
+    // workgroup 0 folds the workgroups' maxima into red_max[0] and the host slot, with the error
+    // word red_max[1]
+    unsigned long long *red_max;
+    double *host_slot;
 };
 
 // a value as tagged 64-bit words: the tag in the high half, 32 payload bits in the low half
@@ -496,7 +500,52 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
             pc = pn;
         }
     }
-    block_max_part(mx, a.max_part);
+    // ---- the maximum over workgroups without fences: every workgroup publishes its partial as two
+    // tagged words (tag0 + iters + 1, unique to this launch) after the face buffers; workgroup 0
+    // polls them all (bounded) and writes the result to red_max[0] and the host slot
+    __shared__ double wred[RB_THREADS / 64];
+    for (int off = 32; off > 0; off >>= 1) mx = nanmax_any(mx, __shfl_xor(mx, off, 64));
+    if ((t & 63) == 0) wred[t >> 6] = mx;
+    __syncthreads();
+    const unsigned mtag = a.tag0 + (unsigned)a.iters + 1u;
+    unsigned long long *MP = a.faces + (size_t)nwg * 2 * RB_XSTRIDE * NW;   // 2 words per workgroup
+    if (t == 0) {
+        double m = wred[0];
+        for (int q = 1; q < RB_THREADS / 64; ++q) m = nanmax_any(m, wred[q]);
+        unsigned long long wv[2];
+        Tagged<double>::pack(m, mtag, wv);
+        __hip_atomic_store(MP + 2 * w, wv[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(MP + 2 * w + 1, wv[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (blockIdx.x != 0) return;
+    double m = 0.0;
+    for (int q = t; q < nwg; q += RB_THREADS) {
+        unsigned long long wv[2];
+        for (int spins = 0;; ++spins) {
+            wv[0] = __hip_atomic_load(MP + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            wv[1] = __hip_atomic_load(MP + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((unsigned)(wv[0] >> 32) == mtag && (unsigned)(wv[1] >> 32) == mtag) break;
+            if (failed || spins >= (1 << 20)) {
+                __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                failed = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        m = nanmax_any(m, Tagged<double>::unpack(wv));
+    }
+    __syncthreads();   // wred reused
+    for (int off = 32; off > 0; off >>= 1) m = nanmax_any(m, __shfl_xor(m, off, 64));
+    if ((t & 63) == 0) wred[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) {
+        for (int q = 1; q < RB_THREADS / 64; ++q) m = nanmax_any(m, wred[q]);
+        m = nanmax_any(wred[0], m);
+        const unsigned long long errw = __hip_atomic_load(&a.red_max[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.red_max[0] = (unsigned long long)__double_as_longlong(m);
+        a.host_slot[0] = m;
+        a.host_slot[1] = __longlong_as_double((long long)errw);
+    }
 }
 
 // ------------------------------------------------------------------------------- host side
@@ -597,7 +646,8 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
 }
 
 size_t rb_faces_bytes(const RbPlanOut &p, bool fp32) {
-    return (size_t)p.px * p.py * p.pz * 2 * (size_t)RB_XSTRIDE * 8 * (fp32 ? 1 : 2);
+    const size_t nwg = (size_t)p.px * p.py * p.pz;
+    return nwg * 2 * (size_t)RB_XSTRIDE * 8 * (fp32 ? 1 : 2) + nwg * 16;   // faces, then the maxima
 }
 
 template <typename T>
@@ -618,6 +668,8 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     a.phi_out = df ? df->phi_out : nullptr;
     a.max_part = h->max_part;
     a.r_out = h->r32;
+    a.red_max = h->red_max;
+    a.host_slot = df ? df->host_slot : nullptr;
     a.d = h->d;
     a.px = pl.px; a.py = pl.py; a.pz = pl.pz;
     a.bx = pl.bx; a.by = pl.by; a.bz = pl.bz;
@@ -637,29 +689,22 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     a.iters = iters;
     a.faces = reinterpret_cast<unsigned long long *>(h->rb_faces);
     a.total = pl.total;
-    // tags of this launch: rb_tag + 1 .. rb_tag + iters; before the 32-bit tag would wrap to 0 the
-    // face buffer is cleared and the count restarts
-    if (h->rb_tag + (unsigned long long)iters + 1ull > 0xffffffffull) {
+    // tags of this launch: rb_tag + 1 .. rb_tag + iters + 1 (the last for the maxima); before the
+    // 32-bit tag would wrap to 0 the face buffer is cleared and the count restarts
+    if (h->rb_tag + (unsigned long long)iters + 2ull > 0xffffffffull) {
         hipError_t e = hipMemsetAsync(h->rb_faces, 0, h->rb_faces_bytes, h->stream);
         if (e != hipSuccess) return e;
         h->rb_tag = 0;
     }
     a.tag0 = (unsigned)h->rb_tag;
-    h->rb_tag += (unsigned long long)iters;
+    h->rb_tag += (unsigned long long)iters + 1ull;
     a.err = h->rb_err;
     const void *fn = h->ibm_on ? rb_fn_for<T, true>(pl.maxp) : rb_fn_for<T, false>(pl.maxp);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds);
     if (e != hipSuccess) return e;
     void *args[] = {&a};
-    // with the defect epilogue the workgroup maxima are in max_part; the caller folds them
-    // (launch_max_final over rb_workgroups()) after timing the batch
     return hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args, (unsigned)pl.lds,
                                       h->stream);
-}
-
-unsigned rb_workgroups(const mdx_handle *h, bool fp32) {
-    const RbPlanOut &pl = fp32 ? h->rb_plan32 : h->rb_plan64;
-    return (unsigned)(pl.px * pl.py * pl.pz);
 }
 
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df) {
