@@ -56,6 +56,7 @@ struct RbArgs {
     const T *rhs;
     int zero_init;           // x starts at zero (the defect equation)
     int iters;               // full red+black iterations
+    int md;                  // leading pair entries of every thread that lie in the deep box
     unsigned long long *faces;   // [wg][buffer][exchange entry][words per value], tagged words
     int total;               // exchange entries in use per workgroup (faces two deep, then edges)
     unsigned tag0;           // iteration s of this launch publishes tag tag0 + s + 1 (never 0)
@@ -221,7 +222,47 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     // ---- subdomain pairs of this thread, packed in one word: LDS index of the colour-0 cell
     // (bits 0-15), its coefficient-table index (bits 16-27), wall flags (bits 28-29) and whether
     // the pair's colour-1 cell is at -1 (bit 30)
+    // Pair order: first the pairs of the deep box (x-pairs 1..hbx-2, rows 2..by-1, planes 2..bz-1),
+    // whose colour-0 cells read no halo, then the shell around it.  Entries m < a.md of every thread
+    // are deep (a.md: whole relax groups), so their colour 0 can run while the halo is in flight.
     const int hbx = bx / 2, npairs = hbx * by * bz;
+    const int dpx = hbx > 2 ? hbx - 2 : 0, dpy = by > 2 ? by - 2 : 0, dpz = bz > 2 ? bz - 2 : 0;
+    const int ndeep = dpx * dpy * dpz;
+    auto pair_of = [&](int p, int &pi, int &j, int &k) {
+        if (p < ndeep) {
+            pi = 1 + p % dpx;
+            j = 2 + (p / dpx) % dpy;
+            k = 2 + p / (dpx * dpy);
+            return;
+        }
+        int q = p - ndeep;
+        if (ndeep == 0) {   // no deep box: natural order
+            pi = q % hbx;
+            j = (q / hbx) % by + 1;
+            k = q / (hbx * by) + 1;
+            return;
+        }
+        const int s1 = 2 * hbx * by;   // planes 1 and bz
+        if (q < s1) {
+            k = q < hbx * by ? 1 : bz;
+            const int r = q % (hbx * by);
+            pi = r % hbx;
+            j = r / hbx + 1;
+            return;
+        }
+        q -= s1;
+        const int rr = 2 * hbx + 2 * (by - 2);   // shell of a middle plane: rows 1, by and the end pairs
+        k = q / rr + 2;
+        int r = q % rr;
+        if (r < 2 * hbx) {
+            j = r < hbx ? 1 : by;
+            pi = r % hbx;
+        } else {
+            r -= 2 * hbx;
+            j = r / 2 + 2;
+            pi = (r & 1) ? hbx - 1 : 0;
+        }
+    };
     auto pack = [](int c, int ti, int walls, int neg) { return c | (ti << 16) | (walls << 28) | (neg << 30); };
     int pw[MAXP];
     T rh[MAXP][2];
@@ -231,7 +272,9 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         pw[m] = pack(dummy, 0, 0, 0);
         rh[m][0] = rh[m][1] = T(0);
         if (p < npairs) {
-            const int i0 = 2 * (p % hbx) + 1, j = (p / hbx) % by + 1, k = p / (hbx * by) + 1;
+            int pi, j, k;
+            pair_of(p, pi, j, k);
+            const int i0 = 2 * pi + 1;
             const int gj = oy + j;
             // colour c relaxes the cell with (gi + gj + gk + c) odd
             const int off0 = ((ox + i0 + gj + oz + k) & 1) ? 0 : 1;
@@ -357,9 +400,11 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         for (int jj = 0; jj < RB_HE; ++jj) asm volatile("" : "+v"(xpub[jj]), "+v"(xdst[jj]));
         // ---- colour 0 on the subdomain and the face ring.  Cells of one colour never neighbour
         // each other, so each is stored as soon as it is formed; groups of RB_G cells with a
-        // scheduling barrier between them bound the loads in flight (registers).
+        // scheduling barrier between them bound the loads in flight (registers).  After the first
+        // iteration the deep groups (g < a.md) were relaxed while the halo was in flight.
 #pragma unroll
         for (int g = 0; g < MAXP; g += RB_G) {
+            if (s > 0 && g < a.md) continue;
             T nv[RB_G];
 #pragma unroll
             for (int u = 0; u < RB_G; ++u) nv[u] = relax(pw[g + u], 0, rh[g + u][0]);
@@ -401,6 +446,22 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
 #pragma unroll
             for (int q = 0; q < NW; ++q)
                 __hip_atomic_store(dst + q, wv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // ---- the next iteration's colour 0 on the deep groups, while the neighbours' layers travel:
+        // those cells read only subdomain cells of colour 1, final for this iteration; the barrier
+        // lets the publish read its layers first
+        if (s < a.iters - 1) {
+            __syncthreads();
+#pragma unroll
+            for (int g = 0; g < MAXP; g += RB_G) {
+                if (g >= a.md) continue;
+                T nv[RB_G];
+#pragma unroll
+                for (int u = 0; u < RB_G; ++u) nv[u] = relax(pw[g + u], 0, rh[g + u][0]);
+#pragma unroll
+                for (int u = 0; u < RB_G; ++u) X[pw[g + u] & 0xffff] = nv[u];
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         // ---- pull the neighbours' layers: poll the tagged words until all carry this tag
         // source word of entry jj: the sender's buffer `buf`, entry e, in words
@@ -687,6 +748,11 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     a.rhs = rhs;
     a.zero_init = zero_init;
     a.iters = iters;
+    {
+        const int hbx = pl.bx / 2, dpx = hbx > 2 ? hbx - 2 : 0, dpy = pl.by > 2 ? pl.by - 2 : 0,
+                  dpz = pl.bz > 2 ? pl.bz - 2 : 0;
+        a.md = (dpx * dpy * dpz / RB_THREADS) / RB_G * RB_G;   // whole relax groups only
+    }
     a.faces = reinterpret_cast<unsigned long long *>(h->rb_faces);
     a.total = pl.total;
     // tags of this launch: rb_tag + 1 .. rb_tag + iters + 1 (the last for the maxima); before the
