@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero8 = {};
-    for (int kc = 0; kc < KC; ++kc) {
+    auto load_a = [&](int kc) {
         const int k0 = kc * 32 + g * 8;
         bf16x8 a = zero8;
         if (k0 < K) {
@@ -56,9 +56,25 @@ __global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__
             if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
                 a = *reinterpret_cast<const bf16x8 *>(in + (((long)zz * H + yy) * W + xx) * CIN + c0);
         }
+        return a;
+    };
+    // operands are software-pipelined one K chunk ahead (the global fetches of chunk kc+1 overlap
+    // the MFMAs of chunk kc); the MFMA sequence is unchanged
+    bf16x8 anext = load_a(0), bnext[NT];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wp[((long)kc * NT + nt) * 64 + lane], acc[nt], 0, 0, 0);
+    for (int nt = 0; nt < NT; ++nt) bnext[nt] = wp[(long)nt * 64 + lane];
+    for (int kc = 0; kc < KC; ++kc) {
+        const bf16x8 a = anext;
+        bf16x8 b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) b[nt] = bnext[nt];
+        if (kc + 1 < KC) {
+            anext = load_a(kc + 1);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) bnext[nt] = wp[((long)(kc + 1) * NT + nt) * 64 + lane];
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[nt], 0, 0, 0);
     }
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -116,6 +132,11 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[m][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero8 = {};
+    // weight fragments are software-pipelined one K chunk ahead: with few output tiles per wave
+    // (COUT 16: one) the global fetch of chunk kc+1 overlaps the MFMAs of chunk kc
+    bf16x8 bnext[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) bnext[nt] = wp[(long)nt * 64 + lane];
     for (int kc = 0; kc < KC; ++kc) {
         const int k0 = kc * 32 + g * 8;
         const bool valid = k0 < K;
@@ -123,7 +144,11 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
         const int base = ((tap / 9) * CHY + wave + (tap / 3) % 3) * CHX + r + tap % 3;
         bf16x8 b[NT];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) b[nt] = wp[((long)kc * NT + nt) * 64 + lane];
+        for (int nt = 0; nt < NT; ++nt) b[nt] = bnext[nt];
+        if (kc + 1 < KC) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) bnext[nt] = wp[((long)(kc + 1) * NT + nt) * 64 + lane];
+        }
 #pragma unroll
         for (int m = 0; m < TZ; ++m) {
             const bf16x8 a = valid ? *reinterpret_cast<const bf16x8 *>(&tile[(base + m * CHX * CHY) * CS + c0]) : zero8;
