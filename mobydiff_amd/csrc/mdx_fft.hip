@@ -26,10 +26,11 @@ template <> struct Cx<float2> {
     static __device__ inline float2 mk(float a, float b) { return make_float2(a, b); }
 };
 
-// One radix-2 Stockham pass sequence over LINES lines of length n held in LDS (a, b ping-pong).
-// Returns the buffer holding the result.  tw[m] = exp(-2 pi i m / n), m < n/2.
+// One radix-2 Stockham pass sequence over LINES lines of length n held in LDS (a, b ping-pong),
+// line q at q * ls.  Returns the buffer holding the result.  tw[m] = exp(-2 pi i m / n), m < n/2,
+// staged in LDS by the caller (stage_twiddles).
 template <typename C>
-__device__ inline C *stockham(C *a, C *b, int n, int lines, const C *__restrict__ tw, bool inverse, int tid,
+__device__ inline C *stockham(C *a, C *b, int n, int lines, int ls, const C *tw, bool inverse, int tid,
                               int nthr) {
     using R = typename Cx<C>::R;
     const int half = n >> 1;
@@ -38,13 +39,13 @@ __device__ inline C *stockham(C *a, C *b, int n, int lines, const C *__restrict_
         for (int t = tid; t < half * lines; t += nthr) {
             const int line = t / half, jj = t - line * half;
             const int kk = jj & (ns - 1);
-            const C x0 = a[line * n + jj], x1 = a[line * n + jj + half];
+            const C x0 = a[line * ls + jj], x1 = a[line * ls + jj + half];
             C w = tw[kk * stride];
             if (inverse) w.y = -w.y;
             const R tr = x1.x * w.x - x1.y * w.y, ti = x1.x * w.y + x1.y * w.x;
             const int o = ((jj - kk) << 1) + kk;
-            b[line * n + o] = Cx<C>::mk(x0.x + tr, x0.y + ti);
-            b[line * n + o + ns] = Cx<C>::mk(x0.x - tr, x0.y - ti);
+            b[line * ls + o] = Cx<C>::mk(x0.x + tr, x0.y + ti);
+            b[line * ls + o + ns] = Cx<C>::mk(x0.x - tr, x0.y - ti);
         }
         __syncthreads();
         C *t2 = a;
@@ -55,6 +56,12 @@ __device__ inline C *stockham(C *a, C *b, int n, int lines, const C *__restrict_
 }
 
 static constexpr int FFT_THREADS = 256;
+
+// the n/2 twiddles into LDS (read by every butterfly of every pass)
+template <typename C>
+__device__ inline void stage_twiddles(C *dst, const C *__restrict__ tw, int n) {
+    for (int t = threadIdx.x; t < n / 2; t += FFT_THREADS) dst[t] = tw[t];
+}
 
 // 1. rhs(i,k,j) = [(u*(i+1)-u*(i))/dx + (v*(j+1)-v*(j))/dy + (w*(k+1)-w*(k))/dz]/dtg, then the
 //    r2c x-transform; rows r = (k-1) + nz (j-1) are contiguous in hat.
@@ -69,7 +76,8 @@ __global__ __launch_bounds__(FFT_THREADS) void rhs_xfft_kernel(GridK g, const do
     const Dims d = g.d;
     const int n = d.nx, nxh = n / 2 + 1;
     const int nrows = d.ny * d.nz;
-    double2 *a = lds, *b = lds + (size_t)lines * n;
+    double2 *a = lds, *b = lds + (size_t)lines * n, *twl = b + (size_t)lines * n;
+    stage_twiddles(twl, tw, n);
     for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
         const int line = t / n, i = t - line * n + 1;
         const int r = blockIdx.x * lines + line;
@@ -84,7 +92,7 @@ __global__ __launch_bounds__(FFT_THREADS) void rhs_xfft_kernel(GridK g, const do
         a[t] = make_double2(v, 0.0);
     }
     __syncthreads();
-    double2 *res = stockham(a, b, n, lines, tw, false, threadIdx.x, FFT_THREADS);
+    double2 *res = stockham(a, b, n, lines, n, twl, false, threadIdx.x, FFT_THREADS);
     for (int t = threadIdx.x; t < lines * nxh; t += FFT_THREADS) {
         const int line = t / nxh, m = t - line * nxh;
         const int r = blockIdx.x * lines + line;
@@ -101,19 +109,23 @@ __global__ __launch_bounds__(FFT_THREADS) void zfft_kernel(Dims d, C *hat, const
     const int nz = d.nz, nxh = d.nx / 2 + 1;
     const int tiles = (nxh + tx - 1) / tx;
     const int j = blockIdx.x / tiles, kx0 = (blockIdx.x % tiles) * tx;
-    C *a = lds, *b = lds + (size_t)tx * nz;
+    // columns nz + 1 apart in LDS: the transposing loads and stores (tx columns of one kz row)
+    // fall in different banks
+    const int ls = nz + 1;
+    C *a = lds, *b = lds + (size_t)tx * ls, *twl = b + (size_t)tx * ls;
     C *plane = hat + (size_t)nxh * nz * j;
+    stage_twiddles(twl, tw, nz);
     for (int t = threadIdx.x; t < tx * nz; t += FFT_THREADS) {
         const int c = t % tx, kz = t / tx;
         const int kx = kx0 + c;
-        a[c * nz + kz] = kx < nxh ? plane[(size_t)kx + (size_t)nxh * kz] : Cx<C>::mk(0, 0);
+        a[c * ls + kz] = kx < nxh ? plane[(size_t)kx + (size_t)nxh * kz] : Cx<C>::mk(0, 0);
     }
     __syncthreads();
-    C *res = stockham(a, b, nz, tx, tw, inverse != 0, threadIdx.x, FFT_THREADS);
+    C *res = stockham(a, b, nz, tx, ls, twl, inverse != 0, threadIdx.x, FFT_THREADS);
     for (int t = threadIdx.x; t < tx * nz; t += FFT_THREADS) {
         const int c = t % tx, kz = t / tx;
         const int kx = kx0 + c;
-        if (kx < nxh) plane[(size_t)kx + (size_t)nxh * kz] = res[c * nz + kz];
+        if (kx < nxh) plane[(size_t)kx + (size_t)nxh * kz] = res[c * ls + kz];
     }
 }
 
@@ -199,7 +211,8 @@ __global__ __launch_bounds__(FFT_THREADS) void ixfft_kernel(Dims d, const double
     double2 *lds = reinterpret_cast<double2 *>(lds_raw);
     const int n = d.nx, nxh = n / 2 + 1;
     const int nrows = d.ny * d.nz;
-    double2 *a = lds, *b = lds + (size_t)lines * n;
+    double2 *a = lds, *b = lds + (size_t)lines * n, *twl = b + (size_t)lines * n;
+    stage_twiddles(twl, tw, n);
     for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
         const int line = t / n, m = t - line * n;
         const int r = blockIdx.x * lines + line;
@@ -213,7 +226,7 @@ __global__ __launch_bounds__(FFT_THREADS) void ixfft_kernel(Dims d, const double
         a[t] = v;
     }
     __syncthreads();
-    double2 *res = stockham(a, b, n, lines, tw, true, threadIdx.x, FFT_THREADS);
+    double2 *res = stockham(a, b, n, lines, n, twl, true, threadIdx.x, FFT_THREADS);
     for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
         const int line = t / n, i = t - line * n;
         const int r = blockIdx.x * lines + line;
@@ -261,10 +274,27 @@ static int rows_per_block(int n) {
 }
 
 static int z_tile(int nz) {
-    int tx = 2048 / nz;                  // 2 buffers x tx x nz x 16 B <= 64 KB
+    int tx = 2048 / nz;                  // 2 buffers x tx x (nz + 1) x 16 B, about 64 KB
     if (tx < 1) tx = 1;
     if (tx > 16) tx = 16;
     return tx;
+}
+
+// dynamic LDS of the transforms: two ping-pong buffers of `lines` lines (stride ls) and the twiddles
+template <typename C>
+static size_t fft_lds(int lines, int ls, int n) {
+    return (2 * (size_t)lines * ls + (size_t)(n / 2)) * sizeof(C);
+}
+
+// allow a z-transform kernel more than the default 64 KB of dynamic LDS (once per kernel)
+template <typename C>
+static hipError_t zfft_allow(size_t bytes) {
+    static size_t allowed = 65536;
+    if (bytes <= allowed) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&zfft_kernel<C>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) allowed = bytes;
+    return e;
 }
 
 hipError_t launch_fft_forward(mdx_handle *h, double dtg) {
@@ -272,7 +302,7 @@ hipError_t launch_fft_forward(mdx_handle *h, double dtg) {
     const Dims d = h->d;
     const int lines = rows_per_block(d.nx);
     const int nrows = d.ny * d.nz;
-    const size_t shx = 2 * (size_t)lines * d.nx * sizeof(double2);
+    const size_t shx = fft_lds<double2>(lines, d.nx, d.nx);
     if (h->stretched)
         hipLaunchKernelGGL(rhs_xfft_kernel<true>, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx,
                            h->stream, g, h->us, h->vs, h->ws, dtg, (const double2 *)h->tw_x, lines,
@@ -282,7 +312,9 @@ hipError_t launch_fft_forward(mdx_handle *h, double dtg) {
                            h->stream, g, h->us, h->vs, h->ws, dtg, (const double2 *)h->tw_x, lines,
                            (double2 *)h->hat);
     const int tx = z_tile(d.nz), nxh = d.nx / 2 + 1;
-    const size_t shz = 2 * (size_t)tx * d.nz * sizeof(double2);
+    const size_t shz = fft_lds<double2>(tx, d.nz + 1, d.nz);
+    hipError_t e = zfft_allow<double2>(shz);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(zfft_kernel<double2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
                        (double2 *)h->hat, (const double2 *)h->tw_z, tx, 0);
     return hipGetLastError();
@@ -300,12 +332,14 @@ hipError_t launch_fft_backward_correct(mdx_handle *h, double dtg) {
     GridK g = grid_k(h);
     const Dims d = h->d;
     const int tx = z_tile(d.nz), nxh = d.nx / 2 + 1;
-    const size_t shz = 2 * (size_t)tx * d.nz * sizeof(double2);
+    const size_t shz = fft_lds<double2>(tx, d.nz + 1, d.nz);
+    hipError_t e = zfft_allow<double2>(shz);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(zfft_kernel<double2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
                        (double2 *)h->hat, (const double2 *)h->tw_z, tx, 1);
     const int lines = rows_per_block(d.nx);
     const int nrows = d.ny * d.nz;
-    const size_t shx = 2 * (size_t)lines * d.nx * sizeof(double2);
+    const size_t shx = fft_lds<double2>(lines, d.nx, d.nx);
     const double scale = 1.0 / (double)(d.nx * d.nz);
     hipLaunchKernelGGL(ixfft_kernel, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx, h->stream, d,
                        (const double2 *)h->hat, (const double2 *)h->tw_x, lines, scale, h->fft_rhs);
@@ -441,14 +475,15 @@ __global__ __launch_bounds__(FFT_THREADS) void xfft32_kernel(Dims d, const doubl
     float2 *lds = reinterpret_cast<float2 *>(lds_raw);
     const int n = d.nx, nxh = n / 2 + 1;
     const int nrows = d.ny * d.nz;
-    float2 *a = lds, *b = lds + (size_t)lines * n;
+    float2 *a = lds, *b = lds + (size_t)lines * n, *twl = b + (size_t)lines * n;
+    stage_twiddles(twl, tw, n);
     for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
         const int line = t / n, i = t - line * n;
         const int r = blockIdx.x * lines + line;
         a[t] = make_float2(r < nrows ? (float)in[(size_t)n * r + i] : 0.f, 0.f);
     }
     __syncthreads();
-    float2 *res = stockham(a, b, n, lines, tw, false, threadIdx.x, FFT_THREADS);
+    float2 *res = stockham(a, b, n, lines, n, twl, false, threadIdx.x, FFT_THREADS);
     for (int t = threadIdx.x; t < lines * nxh; t += FFT_THREADS) {
         const int line = t / nxh, m = t - line * nxh;
         const int r = blockIdx.x * lines + line;
@@ -531,7 +566,8 @@ __global__ __launch_bounds__(FFT_THREADS) void ixfft32_add_kernel(Dims d, const 
     float2 *lds = reinterpret_cast<float2 *>(lds_raw);
     const int n = d.nx, nxh = n / 2 + 1;
     const int nrows = d.ny * d.nz;
-    float2 *a = lds, *b = lds + (size_t)lines * n;
+    float2 *a = lds, *b = lds + (size_t)lines * n, *twl = b + (size_t)lines * n;
+    stage_twiddles(twl, tw, n);
     for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
         const int line = t / n, m = t - line * n;
         const int r = blockIdx.x * lines + line;
@@ -545,7 +581,7 @@ __global__ __launch_bounds__(FFT_THREADS) void ixfft32_add_kernel(Dims d, const 
         a[t] = v;
     }
     __syncthreads();
-    float2 *res = stockham(a, b, n, lines, tw, true, threadIdx.x, FFT_THREADS);
+    float2 *res = stockham(a, b, n, lines, n, twl, true, threadIdx.x, FFT_THREADS);
     for (int t = threadIdx.x; t < lines * n; t += FFT_THREADS) {
         const int line = t / n, i = t - line * n;
         const int r = blockIdx.x * lines + line;
@@ -589,11 +625,13 @@ hipError_t launch_fft32_solve_add(mdx_handle *h) {
                        h->arow, 1.0 / (double)(d.nx * d.nz), h->fft_mean + d.ny);
     hipLaunchKernelGGL(fft_add_mean_kernel, dim3(4096), dim3(256), 0, h->stream, d, h->fft_mean + d.ny, h->fft_phi);
     const int lines = rows_per_block(d.nx), nrows = d.ny * d.nz, nxh = d.nx / 2 + 1;
-    const size_t shx = 2 * (size_t)lines * d.nx * sizeof(float2);
+    const size_t shx = fft_lds<float2>(lines, d.nx, d.nx);
     hipLaunchKernelGGL(xfft32_kernel, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx, h->stream, d,
                        h->fft_res, (const float2 *)h->tw_x32, lines, (float2 *)h->hat32);
     const int tx = z_tile(d.nz);
-    const size_t shz = 2 * (size_t)tx * d.nz * sizeof(float2);
+    const size_t shz = fft_lds<float2>(tx, d.nz + 1, d.nz);
+    hipError_t ez = zfft_allow<float2>(shz);
+    if (ez != hipSuccess) return ez;
     hipLaunchKernelGGL(zfft_kernel<float2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
                        (float2 *)h->hat32, (const float2 *)h->tw_z32, tx, 0);
     const int nlines = nxh * d.nz;
