@@ -9,7 +9,7 @@
 //   4. inverse z-FFT                                                         hat -> hat
 //   5. inverse real x-FFT with the 1/(nx nz) scaling per row                 hat -> phi
 //   6. projection u = (u* - dtg grad phi)/(1 + dtg coef), p += phi
-// Transforms are radix-2 Stockham (self-sorting) in LDS with a precomputed FP64 twiddle table;
+// Transforms are radix-4 Stockham (self-sorting) in LDS with a precomputed FP64 twiddle table;
 // hat keeps the reference's half-spectrum layout plane_hat(nxh, nz, ny), kx fastest.
 #include "mdx_internal.h"
 
@@ -26,26 +26,56 @@ template <> struct Cx<float2> {
     static __device__ inline float2 mk(float a, float b) { return make_float2(a, b); }
 };
 
-// One radix-2 Stockham pass sequence over LINES lines of length n held in LDS (a, b ping-pong),
-// line q at q * ls.  Returns the buffer holding the result.  tw[m] = exp(-2 pi i m / n), m < n/2,
-// staged in LDS by the caller (stage_twiddles).
+// Stockham (self-sorting) transform of LINES lines of length n = 2^p held in LDS (a, b ping-pong),
+// line q at q * ls: one radix-2 pass when p is odd, then radix-4 passes.  A pass combining groups
+// of ns points reads r inputs n/r apart and writes them r-interleaved with twiddles
+// w^(q kk n / (r ns)), kk = position in the group.  Returns the buffer holding the result.
+// tw[m] = exp(-2 pi i m / n), m < n/2, staged in LDS by the caller (stage_twiddles); larger
+// exponents use w^(m + n/2) = -w^m.
 template <typename C>
 __device__ inline C *stockham(C *a, C *b, int n, int lines, int ls, const C *tw, bool inverse, int tid,
                               int nthr) {
     using R = typename Cx<C>::R;
-    const int half = n >> 1;
-    for (int ns = 1; ns < n; ns <<= 1) {
-        const int stride = n / (2 * ns);
+    const int half = n >> 1, quarter = n >> 2;
+    int ns = 1;
+    if (__builtin_ctz(n) & 1) {   // radix-2 pass
         for (int t = tid; t < half * lines; t += nthr) {
             const int line = t / half, jj = t - line * half;
-            const int kk = jj & (ns - 1);
             const C x0 = a[line * ls + jj], x1 = a[line * ls + jj + half];
-            C w = tw[kk * stride];
-            if (inverse) w.y = -w.y;
-            const R tr = x1.x * w.x - x1.y * w.y, ti = x1.x * w.y + x1.y * w.x;
-            const int o = ((jj - kk) << 1) + kk;
-            b[line * ls + o] = Cx<C>::mk(x0.x + tr, x0.y + ti);
-            b[line * ls + o + ns] = Cx<C>::mk(x0.x - tr, x0.y - ti);
+            b[line * ls + 2 * jj] = Cx<C>::mk(x0.x + x1.x, x0.y + x1.y);
+            b[line * ls + 2 * jj + 1] = Cx<C>::mk(x0.x - x1.x, x0.y - x1.y);
+        }
+        __syncthreads();
+        C *t2 = a;
+        a = b;
+        b = t2;
+        ns = 2;
+    }
+    auto twid = [&](int m) {   // w^m for 0 <= m < n
+        C w = m < half ? tw[m] : Cx<C>::mk(-tw[m - half].x, -tw[m - half].y);
+        if (inverse) w.y = -w.y;
+        return w;
+    };
+    for (; ns < n; ns <<= 2) {
+        const int stride = n / (4 * ns);
+        for (int t = tid; t < quarter * lines; t += nthr) {
+            const int line = t / quarter, jj = t - line * quarter;
+            const int kk = jj & (ns - 1);
+            const C *src = a + line * ls + jj;
+            const C x0 = src[0], x1 = src[quarter], x2 = src[2 * quarter], x3 = src[3 * quarter];
+            const C w1 = twid(kk * stride), w2 = twid(2 * kk * stride), w3 = twid(3 * kk * stride);
+            const R y1r = x1.x * w1.x - x1.y * w1.y, y1i = x1.x * w1.y + x1.y * w1.x;
+            const R y2r = x2.x * w2.x - x2.y * w2.y, y2i = x2.x * w2.y + x2.y * w2.x;
+            const R y3r = x3.x * w3.x - x3.y * w3.y, y3i = x3.x * w3.y + x3.y * w3.x;
+            // 4-point DFT; the forward transform multiplies by (-i)^(pq), the inverse by i^(pq)
+            const R s02r = x0.x + y2r, s02i = x0.y + y2i, d02r = x0.x - y2r, d02i = x0.y - y2i;
+            const R s13r = y1r + y3r, s13i = y1i + y3i, d13r = y1r - y3r, d13i = y1i - y3i;
+            const R jr = inverse ? -d13i : d13i, ji = inverse ? d13r : -d13r;   // -i (or +i) * d13
+            C *dst = b + line * ls + ((jj - kk) << 2) + kk;
+            dst[0] = Cx<C>::mk(s02r + s13r, s02i + s13i);
+            dst[ns] = Cx<C>::mk(d02r + jr, d02i + ji);
+            dst[2 * ns] = Cx<C>::mk(s02r - s13r, s02i - s13i);
+            dst[3 * ns] = Cx<C>::mk(d02r - jr, d02i - ji);
         }
         __syncthreads();
         C *t2 = a;
@@ -268,7 +298,7 @@ __global__ __launch_bounds__(256) void fft_correct_kernel(GridK g, const double 
 }
 
 static int rows_per_block(int n) {
-    int lines = (2 * FFT_THREADS) / n;   // n/2 butterflies per line, FFT_THREADS threads
+    int lines = (4 * FFT_THREADS) / n;   // n/4 radix-4 butterflies per line, FFT_THREADS threads
     if (lines < 1) lines = 1;
     return lines;
 }
