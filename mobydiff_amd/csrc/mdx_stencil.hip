@@ -318,30 +318,24 @@ __global__ void bc_fft_y(Dims d, double *un, double *vn, double *wn, double *us,
     ws[d.g(i, 0, k)] = -ws[d.g(i, 1, k)];
 }
 
+// x-periodic ghost columns: one thread per (row (j,k), field, side) copy.  The copies of a row are
+// a stride of nx+2 apart, so each touches its own cache line; spreading them over threads keeps
+// many of these latency-bound accesses in flight.
 __global__ void bc_fft_x(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws,
                          double *pn) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int k = blockIdx.y;
-    if (j > d.ny + 1 || k > d.nz + 1) return;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nrow = (long)(d.ny + 2) * (d.nz + 2);
+    if (q >= 14 * nrow) return;
+    const int task = (int)(q % 14), side = task & 1, field = task >> 1;
+    const long row = q / 14;
+    const int j = (int)(row % (d.ny + 2)), k = (int)(row / (d.ny + 2));
+    double *f = field == 0 ? un : field == 1 ? wn : field == 2 ? us : field == 3 ? ws
+              : field == 4 ? vn : field == 5 ? vs : pn;
+    if (field >= 4 && j < 1) return;                     // v: rows 1..ny+1
+    if (field == 6 && j > d.ny) return;                  // p: rows 1..ny
     const int nx = d.nx;
-    un[d.g(0, j, k)] = un[d.g(nx, j, k)];
-    wn[d.g(0, j, k)] = wn[d.g(nx, j, k)];
-    un[d.g(nx + 1, j, k)] = un[d.g(1, j, k)];
-    wn[d.g(nx + 1, j, k)] = wn[d.g(1, j, k)];
-    us[d.g(0, j, k)] = us[d.g(nx, j, k)];
-    ws[d.g(0, j, k)] = ws[d.g(nx, j, k)];
-    us[d.g(nx + 1, j, k)] = us[d.g(1, j, k)];
-    ws[d.g(nx + 1, j, k)] = ws[d.g(1, j, k)];
-    if (j >= 1) {
-        vn[d.g(0, j, k)] = vn[d.g(nx, j, k)];
-        vn[d.g(nx + 1, j, k)] = vn[d.g(1, j, k)];
-        vs[d.g(0, j, k)] = vs[d.g(nx, j, k)];
-        vs[d.g(nx + 1, j, k)] = vs[d.g(1, j, k)];
-    }
-    if (j >= 1 && j <= d.ny) {
-        pn[d.g(0, j, k)] = pn[d.g(nx, j, k)];
-        pn[d.g(nx + 1, j, k)] = pn[d.g(1, j, k)];
-    }
+    if (side == 0) f[d.g(0, j, k)] = f[d.g(nx, j, k)];
+    else f[d.g(nx + 1, j, k)] = f[d.g(1, j, k)];
 }
 
 __global__ void bc_fft_z(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws,
@@ -374,7 +368,8 @@ hipError_t launch_apply_bc_fft(mdx_handle *h) {
     const Dims d = h->d;
     hipLaunchKernelGGL(bc_fft_y, dim3((d.nx + 2 + 63) / 64, d.nz + 2), dim3(64), 0, h->stream, d,
                        h->un, h->vn, h->wn, h->us, h->vs, h->ws);
-    hipLaunchKernelGGL(bc_fft_x, dim3((d.ny + 2 + 63) / 64, d.nz + 2), dim3(64), 0, h->stream, d,
+    const long nx_tasks = 14L * (d.ny + 2) * (d.nz + 2);
+    hipLaunchKernelGGL(bc_fft_x, dim3((unsigned)((nx_tasks + 255) / 256)), dim3(256), 0, h->stream, d,
                        h->un, h->vn, h->wn, h->us, h->vs, h->ws, h->pn);
     hipLaunchKernelGGL(bc_fft_z, dim3((d.nx + 2 + 63) / 64, d.ny + 2), dim3(64), 0, h->stream, d,
                        h->un, h->vn, h->wn, h->us, h->vs, h->ws, h->pn);
@@ -588,16 +583,17 @@ __global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, cons
     const Dims d = g.d;
     __shared__ double s1[SUM_THREADS], s2[SUM_THREADS];
     double a = 0.0, b = 0.0;
-    const size_t n = (size_t)d.nx * d.ny * d.nz;
-    for (size_t q = (size_t)blockIdx.x * SUM_THREADS + threadIdx.x; q < n;
-         q += (size_t)SUM_BLOCKS * SUM_THREADS) {
-        int i = (int)(q % d.nx) + 1;
-        int j = (int)((q / d.nx) % d.ny) + 1;
-        int k = (int)(q / ((size_t)d.nx * d.ny)) + 1;
-        double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
-        size_t c = d.g(i, j, k);
-        a += u[c] * hj;
-        b += (cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0) * hj;
+    // workgroup b sums the x rows r = b, b + SUM_BLOCKS, ... (r = (j-1) + ny (k-1)), threads along x
+    const int nrows = d.ny * d.nz;
+    for (int r = blockIdx.x; r < nrows; r += SUM_BLOCKS) {
+        const int j = r % d.ny + 1, k = r / d.ny + 1;
+        const double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
+        const size_t row = d.g(0, j, k);
+        for (int i = threadIdx.x + 1; i <= d.nx; i += SUM_THREADS) {
+            const size_t c = row + (size_t)i;
+            a += u[c] * hj;
+            b += (cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0) * hj;
+        }
     }
     s1[threadIdx.x] = a;
     s2[threadIdx.x] = b;
