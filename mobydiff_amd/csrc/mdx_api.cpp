@@ -225,6 +225,8 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
                   "FFT pressure solver: nx and nz must be powers of two <= 1024");
         MDX_CHECK(cfg->precision != MDX_PREC_MIXED || cfg->tol > 0.0, MDX_ERR_ARG,
                   "mixed-precision FFT solve: tol > 0 required (||div||_inf target of the defect correction)");
+        MDX_CHECK(cfg->precision != MDX_PREC_MIXED || ny <= 1024, MDX_ERR_UNSUPPORTED,
+                  "mixed-precision FFT solve: ny <= 1024 (the FP64 mean-mode solve stages its line in LDS)");
     } else if (cfg->solver == MDX_SOLVER_SOLA || cfg->solver == MDX_SOLVER_SOR) {
         MDX_CHECK(nx % 2 == 0 && nz % 2 == 0, MDX_ERR_ARG,
                   "red-black pressure solver requires even nx and nz for periodic coloring");
@@ -357,7 +359,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             const std::vector<float> cp32 = to32(cp), di32 = to32(di), ar32 = to32(ar), twx32 = to32(twx),
                                      twz32 = to32(twz);
             if ((e = dalloc(&h->fft_phi, h->ni)) != hipSuccess || (e = dalloc(&h->fft_res, h->ni)) != hipSuccess ||
-                (e = dalloc(&h->fft_mean, 2 * (size_t)ny)) != hipSuccess ||
+                (e = dalloc(&h->fft_mean, 17 * (size_t)ny)) != hipSuccess ||
                 (e = dalloc(&h->hat32, 2 * nh)) != hipSuccess || (e = dalloc(&h->cp32, nh)) != hipSuccess ||
                 (e = dalloc(&h->den32, nh)) != hipSuccess || (e = dalloc(&h->arow32, ny)) != hipSuccess ||
                 (e = dalloc(&h->tw_x32, nx)) != hipSuccess || (e = dalloc(&h->tw_z32, nz)) != hipSuccess)

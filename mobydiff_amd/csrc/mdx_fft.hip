@@ -410,41 +410,67 @@ __global__ __launch_bounds__(256) void fft_div_rhs_kernel(GridK g, const double 
 }
 
 // sums[j-1] = sum over the (x, z) plane j of rhs (one workgroup per plane, fixed order)
-__global__ __launch_bounds__(256) void fft_plane_sum_kernel(Dims d, const double *__restrict__ rhs, double *sums) {
+// partial sums of the (nx, nz) planes: workgroup (plane j, chunk c) sums chunk c of PLANE_CHUNKS
+// equal parts of plane j into part[j * PLANE_CHUNKS + c]; consumers fold a plane's chunks in order
+static constexpr int PLANE_CHUNKS = 16;
+__global__ __launch_bounds__(256) void fft_plane_sum_kernel(Dims d, const double *__restrict__ rhs, double *part) {
     __shared__ double s[256];
     const long plane = (long)d.nx * d.nz;
-    const double *p = rhs + plane * blockIdx.x;
+    const int j = blockIdx.x / PLANE_CHUNKS, c = blockIdx.x % PLANE_CHUNKS;
+    const long lo = plane * c / PLANE_CHUNKS, hi = plane * (c + 1) / PLANE_CHUNKS;
+    const double *p = rhs + plane * j;
     double a = 0.0;
-    for (long q = threadIdx.x; q < plane; q += 256) a += p[q];
+    for (long q = lo + threadIdx.x; q < hi; q += 256) a += p[q];
     s[threadIdx.x] = a;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-// the kx = kz = 0 line of tridiag_kernel in FP64 on the plane sums; mean[j-1] = x_j / (nx nz)
-__global__ void fft_mean_mode_kernel(Dims d, const double *__restrict__ sums, const double *__restrict__ cp,
-                                     const double *__restrict__ den_inv, const double *__restrict__ arow,
-                                     double scale, double *mean) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ inline double plane_sum(const double *part, int j) {
+    double a = 0.0;
+    for (int c = 0; c < PLANE_CHUNKS; ++c) a += part[j * PLANE_CHUNKS + c];
+    return a;
+}
+
+// the kx = kz = 0 line of tridiag_kernel in FP64 on the plane sums; mean[j-1] = x_j / (nx nz).
+// The operands are staged in LDS first, so the sequential recurrences do not wait on global memory.
+static constexpr int MEAN_MAX_NY = 1024;   // 40 KB of LDS
+__global__ __launch_bounds__(256) void fft_mean_mode_kernel(Dims d, const double *__restrict__ part,
+                                                            const double *__restrict__ cp,
+                                                            const double *__restrict__ den_inv,
+                                                            const double *__restrict__ arow, double scale,
+                                                            double *mean) {
+    __shared__ double ss[MEAN_MAX_NY], sa[MEAN_MAX_NY], sd[MEAN_MAX_NY], sc[MEAN_MAX_NY], sm[MEAN_MAX_NY];
     const int ny = d.ny;
     const size_t cs = (size_t)(d.nx / 2 + 1);   // stride between j of line (0, 0) in cp / den_inv
-    double dp = 0.0;
-    mean[0] = 0.0;
-    for (int jj = 2; jj <= ny; ++jj) {
-        dp = (sums[jj - 1] - arow[jj - 1] * dp) * den_inv[cs * (jj - 1)];
-        mean[jj - 1] = dp;
+    for (int q = threadIdx.x; q < ny; q += 256) {
+        ss[q] = plane_sum(part, q);
+        sa[q] = arow[q];
+        sd[q] = den_inv[cs * q];
+        sc[q] = cp[cs * q];
     }
-    double nxt = dp;
-    mean[ny - 1] = nxt * scale;
-    for (int jj = ny - 1; jj >= 1; --jj) {
-        nxt = mean[jj - 1] - cp[cs * (jj - 1)] * nxt;
-        mean[jj - 1] = nxt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double dp = 0.0;
+        sm[0] = 0.0;
+        for (int jj = 2; jj <= ny; ++jj) {
+            dp = (ss[jj - 1] - sa[jj - 1] * dp) * sd[jj - 1];
+            sm[jj - 1] = dp;
+        }
+        double nxt = dp;
+        sm[ny - 1] = nxt * scale;
+        for (int jj = ny - 1; jj >= 1; --jj) {
+            nxt = sm[jj - 1] - sc[jj - 1] * nxt;
+            sm[jj - 1] = nxt;
+        }
+        for (int jj = 1; jj < ny; ++jj) sm[jj - 1] *= scale;
     }
-    for (int jj = 1; jj < ny; ++jj) mean[jj - 1] *= scale;
+    __syncthreads();
+    for (int q = threadIdx.x; q < ny; q += 256) mean[q] = sm[q];
 }
 
 // phi += mean[j] everywhere in plane j
@@ -457,9 +483,9 @@ __global__ void fft_add_mean_kernel(Dims d, const double *__restrict__ mean, dou
 // phi -= (plane-1 sum of phi) / (nx nz): re-imposes the pin phi_hat(y_1) = 0 exactly (the FP32
 // inverse transforms leave plane means of the order of their rounding, and a constant is invisible
 // to the residual); velocities see only differences of phi and do not change
-__global__ void fft_pin_kernel(Dims d, const double *__restrict__ sums, double *__restrict__ phi) {
+__global__ void fft_pin_kernel(Dims d, const double *__restrict__ part, double *__restrict__ phi) {
     const long n = (long)d.nx * d.ny * d.nz;
-    const double c = sums[0] / (double)((long)d.nx * d.nz);
+    const double c = plane_sum(part, 0) / (double)((long)d.nx * d.nz);
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) phi[q] -= c;
 }
 
@@ -650,10 +676,13 @@ hipError_t launch_fft_residual(mdx_handle *h) {
 // Thomas), the other modes by FP32 transforms and Thomas; both corrections added to h->fft_phi
 hipError_t launch_fft32_solve_add(mdx_handle *h) {
     const Dims d = h->d;
-    hipLaunchKernelGGL(fft_plane_sum_kernel, dim3(d.ny), dim3(256), 0, h->stream, d, h->fft_res, h->fft_mean);
-    hipLaunchKernelGGL(fft_mean_mode_kernel, dim3(1), dim3(64), 0, h->stream, d, h->fft_mean, h->cp, h->den_inv,
-                       h->arow, 1.0 / (double)(d.nx * d.nz), h->fft_mean + d.ny);
-    hipLaunchKernelGGL(fft_add_mean_kernel, dim3(4096), dim3(256), 0, h->stream, d, h->fft_mean + d.ny, h->fft_phi);
+    if (d.ny > MEAN_MAX_NY) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fft_plane_sum_kernel, dim3(d.ny * PLANE_CHUNKS), dim3(256), 0, h->stream, d, h->fft_res,
+                       h->fft_mean);
+    double *mean = h->fft_mean + (size_t)d.ny * PLANE_CHUNKS;
+    hipLaunchKernelGGL(fft_mean_mode_kernel, dim3(1), dim3(256), 0, h->stream, d, h->fft_mean, h->cp, h->den_inv,
+                       h->arow, 1.0 / (double)(d.nx * d.nz), mean);
+    hipLaunchKernelGGL(fft_add_mean_kernel, dim3(4096), dim3(256), 0, h->stream, d, mean, h->fft_phi);
     const int lines = rows_per_block(d.nx), nrows = d.ny * d.nz, nxh = d.nx / 2 + 1;
     const size_t shx = fft_lds<float2>(lines, d.nx, d.nx);
     hipLaunchKernelGGL(xfft32_kernel, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx, h->stream, d,
@@ -677,7 +706,7 @@ hipError_t launch_fft32_solve_add(mdx_handle *h) {
 
 hipError_t launch_fft_pin(mdx_handle *h) {
     const Dims d = h->d;
-    hipLaunchKernelGGL(fft_plane_sum_kernel, dim3(1), dim3(256), 0, h->stream, d, h->fft_phi, h->fft_mean);
+    hipLaunchKernelGGL(fft_plane_sum_kernel, dim3(PLANE_CHUNKS), dim3(256), 0, h->stream, d, h->fft_phi, h->fft_mean);
     hipLaunchKernelGGL(fft_pin_kernel, dim3(4096), dim3(256), 0, h->stream, d, h->fft_mean, h->fft_phi);
     return hipGetLastError();
 }

@@ -134,7 +134,7 @@ struct mdx_handle {
     double *fft_rhs = nullptr, *hat = nullptr, *cp = nullptr, *den_inv = nullptr, *arow = nullptr;
     double *tw_x = nullptr, *tw_z = nullptr;
     // mixed-precision FFT solve: FP64 rhs in fft_rhs, FP64 phi and residual, FP32 spectrum and tables
-    double *fft_phi = nullptr, *fft_res = nullptr, *fft_mean = nullptr;   // fft_mean: ny sums + ny mean phi
+    double *fft_phi = nullptr, *fft_res = nullptr, *fft_mean = nullptr;   // fft_mean: 16 ny partial plane sums + ny mean phi
     float *hat32 = nullptr, *cp32 = nullptr, *den32 = nullptr, *arow32 = nullptr, *tw_x32 = nullptr, *tw_z32 = nullptr;
 
     // reductions
