@@ -831,9 +831,10 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
 // A check k forms Phi += e and the FP64 residual r_k; the FP64 iterate changes only there, so the
 // next batch of FP32 sweeps can be queued before r_k is read: if r_k turns out converged, that
 // batch's correction is simply never applied, and Phi, the iteration count and every check are
-// what the synchronous loop gives.  The batch is queued early only when the residuals seen so far
-// predict (geometric decay) that check k is still at least SPEC_MARGIN times above the tolerance,
-// so a wasted batch is rare while the GPU does not idle through the host's read of the residual.
+// what the synchronous loop gives.  The batch is queued early at the first check of a solve, and
+// later only when the residuals seen so far predict (geometric decay) that check k is still at
+// least SPEC_MARGIN times above the tolerance, so a wasted batch is rare while the GPU does not
+// idle through the host's read of the residual.
 // On the LDS-resident solver a batch ends with the next check itself (its defect epilogue).
 static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
@@ -871,7 +872,9 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     if ((rc = enqueue_check(slot))) return rc;
     for (;;) {
         const int m = std::min(c.inner_iters, c.max_iter - it);
-        bool spec = false;
+        // the first check of a solve is taken to be unconverged (a fresh right-hand side); later
+        // ones are predicted from the residuals seen so far
+        bool spec = m > 0 && checks == 0;
         if (m > 0 && r1 > 0.0) {
             const double pred = r2 > 0.0 ? r1 * std::min(1.0, r1 / r2) : r1 / SPEC_FIRST;
             spec = pred > SPEC_MARGIN * c.tol;
