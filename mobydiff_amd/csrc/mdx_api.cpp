@@ -202,9 +202,14 @@ static void free_all(mdx_handle *h) {
         if (p) (void)hipFree(p);
     if (h->h_pinned) (void)hipHostFree(h->h_pinned);
     if (h->timing) {
-        for (auto &e : h->ev) (void)hipEventDestroy(e);
+        for (auto &e : h->ev)
+            if (e) (void)hipEventDestroy(e);
         for (auto &q : h->ev_sw)
-            for (auto &e : q) (void)hipEventDestroy(e);
+            for (auto &e : q)
+                if (e) (void)hipEventDestroy(e);
+        for (auto &q : h->ev_ph)
+            for (auto &e : q)
+                if (e) (void)hipEventDestroy(e);
     }
     for (auto &e : h->ev_max)
         if (e) (void)hipEventDestroy(e);
@@ -377,6 +382,9 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         for (auto &ev : h->ev)
             if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "events");
         for (auto &q : h->ev_sw)
+            for (auto &ev : q)
+                if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "events");
+        for (auto &q : h->ev_ph)
             for (auto &ev : q)
                 if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "events");
     }
@@ -697,7 +705,28 @@ static int ensure_mu(mdx_handle *h, double dtg) {
 }
 
 static void ev_record(mdx_handle *h, int q) {
-    if (h->timing) (void)hipEventRecord(h->ev[q], h->stream);
+    if (!h->timing) return;
+    (void)hipEventRecord(q < 5 ? h->ev_ph[h->ph_cur][q] : h->ev[q], h->stream);
+}
+
+// add the phase times of completed stages (`wait`: of all recorded stages) to the counters
+static void phase_account(mdx_handle *h, bool wait) {
+    while (h->timing && h->ph_count > 0) {
+        hipEvent_t *e = h->ev_ph[h->ph_head];
+        if (wait) (void)hipEventSynchronize(e[4]);
+        else if (hipEventQuery(e[4]) != hipSuccess) break;
+        float a = 0, b = 0, c = 0, d = 0;
+        (void)hipEventElapsedTime(&a, e[0], e[1]);
+        (void)hipEventElapsedTime(&b, e[1], e[2]);
+        (void)hipEventElapsedTime(&c, e[2], e[3]);
+        (void)hipEventElapsedTime(&d, e[3], e[4]);
+        h->times.stencil_ms += a + d;
+        h->times.poisson_ms += b;
+        h->times.projection_ms += c;
+        h->times.stages += 1;
+        h->ph_head = (h->ph_head + 1) % mdx_handle::PHQ;
+        h->ph_count -= 1;
+    }
 }
 
 // A batch of m red+black iterations.  The launch sequence of a batch never changes for a given
@@ -1221,6 +1250,10 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
     int rc;
     for (int s = 0; s < 3; ++s) {
         const double dta = dt * RK_ALPHA[s], dtb = dt * RK_BETA[s], dtg = dt * RK_GAMMA[s];
+        if (h->timing) {   // this stage's event set in the ring (the oldest is folded if it is full)
+            if (h->ph_count == mdx_handle::PHQ) phase_account(h, true);
+            h->ph_cur = (h->ph_head + h->ph_count) % mdx_handle::PHQ;
+        }
         ev_record(h, 0);
         if ((rc = mdx_momentum(h, dta, dtb, dtg))) return rc;
         if (h->cfg.flowrate && (rc = mdx_flowrate_shift(h, dtg, nullptr))) return rc;
@@ -1235,16 +1268,9 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
         ev_record(h, 4);
         if (stats) stats->stage[s] = st;
         if (h->timing) {
-            MDX_HIP(hipEventSynchronize(h->ev[4]));
-            float a = 0, b = 0, c = 0, d = 0;
-            (void)hipEventElapsedTime(&a, h->ev[0], h->ev[1]);
-            (void)hipEventElapsedTime(&b, h->ev[1], h->ev[2]);
-            (void)hipEventElapsedTime(&c, h->ev[2], h->ev[3]);
-            (void)hipEventElapsedTime(&d, h->ev[3], h->ev[4]);
-            h->times.stencil_ms += a + d;
-            h->times.poisson_ms += b;
-            h->times.projection_ms += c;
-            h->times.stages += 1;
+            h->ph_count += 1;
+            h->ph_cur = mdx_handle::PHQ;   // later phase events outside a step go to the scratch set
+            phase_account(h, false);
         }
     }
     if (stats && h->cfg.flowrate) {
@@ -1298,6 +1324,7 @@ extern "C" int mdx_step_batch(mdx_handle *const *hs, int32_t n, double dt, mdx_s
 extern "C" int mdx_get_phase_times(mdx_handle *h, mdx_phase_times *t) {
     MDX_CHECK(h && t, MDX_ERR_ARG, "null argument");
     sweep_account(h, true);   // batches still queued (a discarded speculative one included)
+    phase_account(h, true);
     *t = h->times;
     return MDX_OK;
 }
@@ -1305,6 +1332,7 @@ extern "C" int mdx_get_phase_times(mdx_handle *h, mdx_phase_times *t) {
 extern "C" int mdx_reset_phase_times(mdx_handle *h) {
     MDX_CHECK(h, MDX_ERR_ARG, "null handle");
     sweep_account(h, true);
+    phase_account(h, true);
     h->times = mdx_phase_times{};
     return MDX_OK;
 }

@@ -147,12 +147,18 @@ struct mdx_handle {
 
     // timing
     bool timing = false;
-    hipEvent_t ev[10];
+    hipEvent_t ev[10] = {};         // [8], [9]: the learned guess
+    // per-stage phase events (0 stage start, 1 Poisson start, 2 projection start, 3 projection end,
+    // 4 stage end) in a ring of PHQ stages, added to `times` once complete (no stall per stage);
+    // set PHQ is scratch for phase events recorded outside mdx_step
+    static constexpr int PHQ = 8;
+    hipEvent_t ev_ph[PHQ + 1][5] = {};
+    int ph_head = 0, ph_count = 0, ph_cur = PHQ;
     mdx_phase_times times{};
     // sweep batches not yet added to the counters: start/end events, iterations and launches
     // (a speculative batch can be in flight while the previous one is accounted)
     static constexpr int SW_Q = 4;
-    hipEvent_t ev_sw[SW_Q][2];
+    hipEvent_t ev_sw[SW_Q][2] = {};
     int sw_its[SW_Q] = {}, sw_launches[SW_Q] = {};
     int sw_head = 0, sw_count = 0;
     hipEvent_t ev_max[2] = {nullptr, nullptr};   // residual-max copies of the mixed solve's checks
