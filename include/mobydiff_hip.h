@@ -102,7 +102,9 @@ typedef struct {
     double projection_ms;    /* velocity/pressure correction */
     double guess_ms;         /* learned initial guess (network inference) */
     int64_t stages;
-    double sweep_ms;         /* red-black sweep kernels alone (event-bracketed batches) */
+    double sweep_ms;         /* red-black sweep kernels alone, over a sample of the batches (every
+                                fourth batch is event-bracketed); the fields below count the same
+                                sample */
     int64_t sweep_launches;  /* kernel launches in those batches: one per colour (streaming), one
                                 per red+black iteration (fused), or one per batch (LDS-resident) */
     int64_t sweep_cells;     /* colour passes over the grid in those batches times nx ny nz, i.e.

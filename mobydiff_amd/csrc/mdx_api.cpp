@@ -1,4 +1,7 @@
 // mdx_api.cpp -- the C ABI of libmobydiff_hip: device state, solver dispatch and host-side setup.
+#include <immintrin.h>
+
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -211,8 +214,6 @@ static void free_all(mdx_handle *h) {
             for (auto &e : q)
                 if (e) (void)hipEventDestroy(e);
     }
-    for (auto &e : h->ev_max)
-        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -313,6 +314,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         }
         h->lds_sor_mode = v;
     }
+    if (const char *lc = std::getenv("MOBYDIFF_LDS_COOP")) h->rb_coop = lc[0] == '1';
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
     if (const char *sf = std::getenv("MOBYDIFF_SWEEP_FUSED")) h->sweep_fused = sf[0] == '1';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
@@ -325,7 +327,10 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         }
         h->sweep_kz = v;
     }
-    if ((e = hipHostMalloc((void **)&h->h_pinned, 64 * sizeof(double))) != hipSuccess) return fail(e, "pinned buffer");
+    // coherent (fine-grained): the solver's checks write their results here while later work runs
+    if ((e = hipHostMalloc((void **)&h->h_pinned, 64 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent)) !=
+        hipSuccess)
+        return fail(e, "pinned buffer");
     if ((e = hipHostGetDevicePointer((void **)&h->d_pinned, h->h_pinned, 0)) != hipSuccess)
         return fail(e, "pinned buffer device address");
 
@@ -388,8 +393,6 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             for (auto &ev : q)
                 if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(e, "events");
     }
-    for (auto &ev : h->ev_max)
-        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return fail(e, "events");
     if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "synchronize");
     *out = h;
     return MDX_OK;
@@ -757,7 +760,9 @@ static void sweep_account(mdx_handle *h, bool wait = false) {
         if (wait) (void)hipEventSynchronize(h->ev_sw[q][1]);
         else if (hipEventQuery(h->ev_sw[q][1]) != hipSuccess) break;
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, h->ev_sw[q][0], h->ev_sw[q][1]) == hipSuccess) {
+        if (h->sw_void[q]) {
+            h->sw_void[q] = false;   // a batch that found its input converged: nothing to count
+        } else if (hipEventElapsedTime(&ms, h->ev_sw[q][0], h->ev_sw[q][1]) == hipSuccess) {
             h->times.sweep_ms += ms;
             h->times.sweep_launches += h->sw_launches[q];
             h->times.sweep_cells += 2 * (int64_t)h->sw_its[q] * (int64_t)h->ni;
@@ -768,8 +773,8 @@ static void sweep_account(mdx_handle *h, bool wait = false) {
 }
 
 static int sweep_batch(mdx_handle *h, bool fp32, int m, const RbDefect *df = nullptr) {
-    int qs = -1;   // accounting queue entry of this batch
-    if (h->timing) {
+    int qs = -1;   // accounting queue entry of this batch (every SW_EVERY-th batch is timed)
+    if (h->timing && h->sw_seen++ % mdx_handle::SW_EVERY == 0) {
         if (h->sw_count == mdx_handle::SW_Q) sweep_account(h, true);
         qs = (h->sw_head + h->sw_count) % mdx_handle::SW_Q;
         MDX_HIP(hipEventRecord(h->ev_sw[qs][0], h->stream));
@@ -781,11 +786,11 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m, const RbDefect *df = nul
         hipError_t e = launch_rb_lds(h, fp32, m, fp32 ? 1 : 0, df);
         launches = 1;
         if (e != hipSuccess) {
-            // the runtime refused the cooperative launch (nothing ran): use the streaming sweeps
+            // the runtime refused the launch (nothing ran): use the streaming sweeps
             (void)hipGetLastError();
             launches = 0;
             h->rb_state = -1;
-            h->rb_note = std::string("streaming sweeps: cooperative launch refused (") + hipGetErrorString(e) + ")";
+            h->rb_note = std::string("streaming sweeps: LDS-resident launch refused (") + hipGetErrorString(e) + ")";
             if (fp32) MDX_HIP(launch_zero32(h, h->e32, h->ni));   // the streaming sweeps start from e = 0
         }
     }
@@ -813,8 +818,10 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m, const RbDefect *df = nul
     }
     if (!launches) launches = (h->sweep_fused ? 1 : 2) * m;   // streaming: one launch per colour
     // a batch that should have ended with the LDS defect epilogue but ran on the streaming sweeps
-    // (cooperative launch refused): the separate defect kernel forms the same check
-    if (df && !lds_for(h, fp32)) MDX_HIP(launch_defect64(h, df->phi_in, h->e32, df->phi_out, h->r32, df->host_slot));
+    // (launch refused): the separate defect kernel forms the same check
+    if (df && !lds_for(h, fp32))
+        MDX_HIP(launch_defect64(h, df->phi_in, h->e32, df->phi_out, h->r32, df->host_slot, df->seq));
+    h->sw_last = qs;
     if (qs >= 0) {
         MDX_HIP(hipEventRecord(h->ev_sw[qs][1], h->stream));
         h->sw_its[qs] = m;
@@ -855,44 +862,74 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
     return MDX_OK;
 }
 
+// Wait for the residual check that posts sequence number `seq` to the pinned words [w, w + 3):
+// [max, error word, seq].  The host polls the sequence word (no event between the kernels); now
+// and then it queries the stream, so a stream that finished or failed without posting ends the
+// wait with an error instead of spinning.
+static int wait_check(mdx_handle *h, int w, unsigned long long seq, double *r) {
+    volatile unsigned long long *sq = reinterpret_cast<volatile unsigned long long *>(h->h_pinned + w + 2);
+    for (unsigned spins = 1; *sq != seq; ++spins) {
+        if ((spins & 255) == 0) {
+            const hipError_t q = hipStreamQuery(h->stream);
+            if (q == hipErrorNotReady) continue;
+            if (*sq == seq) break;
+            if (q != hipSuccess) {
+                mdx_set_error(std::string("SOR check: ") + hipGetErrorString(q));
+                return MDX_ERR_HIP;
+            }
+            mdx_set_error("SOR check: the stream finished without posting its residual");
+            return MDX_ERR_HIP;
+        }
+        _mm_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    *r = h->h_pinned[w];
+    int err;
+    std::memcpy(&err, &h->h_pinned[w + 1], sizeof err);
+    if (err) {
+        mdx_set_error("LDS red-black solver: a neighbour wait exceeded its spin limit");
+        return MDX_ERR_HIP;
+    }
+    return MDX_OK;
+}
+
 // FP32 red-black sweeps on the defect equation inside an FP64 defect-correction loop.
 //
 // A check k forms Phi += e and the FP64 residual r_k; the FP64 iterate changes only there, so the
 // next batch of FP32 sweeps can be queued before r_k is read: if r_k turns out converged, that
 // batch's correction is simply never applied, and Phi, the iteration count and every check are
-// what the synchronous loop gives.  The batch is queued early at the first check of a solve, and
-// later only when the residuals seen so far predict (geometric decay) that check k is still at
-// least SPEC_MARGIN times above the tolerance, so a wasted batch is rare while the GPU does not
-// idle through the host's read of the residual.
-// On the LDS-resident solver a batch ends with the next check itself (its defect epilogue).
+// what the synchronous loop gives.
+// On the LDS-resident solver a batch ends with the next check itself (its defect epilogue), and a
+// batch queued ahead reads r_k on the GPU and does nothing when it is converged, so the next batch
+// is always queued before r_k is read and the GPU never waits for the host inside a solve.  On the
+// streaming sweeps a wasted batch costs its full time: it is queued early at the first check of a
+// solve, and later only when the residuals seen so far predict (geometric decay) that check k is
+// still at least SPEC_MARGIN times above the tolerance.
 static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
-    constexpr int PIN = 16;   // pinned-buffer words of the two check slots: [PIN + 4 * slot]
+    constexpr int PIN = 16;   // pinned words of the two check slots: [PIN + 4 * slot]
     const mdx_config &c = h->cfg;
     const bool lds = lds_for(h, true);
     double r = 0.0, r1 = -1.0, r2 = -1.0;   // r_k, r_{k-1}, r_{k-2}
     int it = 0, checks = 0, slot = 0;
+    unsigned long long seq[2] = {0, 0};     // the check each slot waits for
     MDX_HIP(launch_zero32(h, h->e32, h->ni));
-    // a check's kernels write [max, LDS error word] straight into pinned slot sl; an event after
-    // them tells the host the words are there
     auto slot_dev = [&](int sl) { return h->d_pinned + PIN + 4 * sl; };
-    auto post = [&](int sl) -> int {
-        MDX_HIP(hipEventRecord(h->ev_max[sl], h->stream));
+    auto enqueue_check = [&](int sl) -> int {   // separate defect kernel: Phi += e, r32, max
+        seq[sl] = ++h->chk_seq;
+        MDX_HIP(launch_defect64(h, h->phi, h->e32, h->phi2, h->r32, slot_dev(sl), seq[sl]));
+        std::swap(h->phi, h->phi2);
         return MDX_OK;
     };
-    auto enqueue_check = [&](int sl) -> int {   // separate defect kernel: Phi += e, r32, max
-        MDX_HIP(launch_reset_max(h));
-        MDX_HIP(launch_defect64(h, h->phi, h->e32, h->phi2, h->r32, slot_dev(sl)));
-        std::swap(h->phi, h->phi2);
-        return post(sl);
-    };
-    auto enqueue_batch = [&](int m, int sl) -> int {   // m iterations; on LDS also the next check
+    // m iterations; on LDS also the next check (`ahead`: queued before the previous check is read)
+    auto enqueue_batch = [&](int m, int sl, bool ahead) -> int {
         if (lds) {
-            const RbDefect df{h->phi, h->phi2, slot_dev(sl)};
+            seq[sl] = ++h->chk_seq;
+            const RbDefect df{h->phi, h->phi2, slot_dev(sl), seq[sl], ahead ? 1 : 0, c.tol};
             int rc_ = sweep_batch(h, true, m, &df);
             if (rc_) return rc_;
             std::swap(h->phi, h->phi2);
-            return post(sl);
+            return MDX_OK;
         }
         MDX_HIP(launch_zero32(h, h->e32, h->ni));
         return sweep_batch(h, true, m);
@@ -903,29 +940,26 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
         const int m = std::min(c.inner_iters, c.max_iter - it);
         // the first check of a solve is taken to be unconverged (a fresh right-hand side); later
         // ones are predicted from the residuals seen so far
-        bool spec = m > 0 && checks == 0;
-        if (m > 0 && r1 > 0.0) {
+        bool spec = m > 0 && (lds || checks == 0);
+        if (!lds && m > 0 && r1 > 0.0) {
             const double pred = r2 > 0.0 ? r1 * std::min(1.0, r1 / r2) : r1 / SPEC_FIRST;
             spec = pred > SPEC_MARGIN * c.tol;
         }
-        if (spec && (rc = enqueue_batch(m, slot ^ 1))) return rc;
-        MDX_HIP(hipEventSynchronize(h->ev_max[slot]));
-        r = h->h_pinned[PIN + 4 * slot];
-        int err;
-        std::memcpy(&err, &h->h_pinned[PIN + 4 * slot + 1], sizeof err);
-        if (err) {
-            mdx_set_error("LDS red-black solver: a neighbour wait exceeded its spin limit");
-            return MDX_ERR_HIP;
-        }
+        if (spec && (rc = enqueue_batch(m, slot ^ 1, true))) return rc;
+        if ((rc = wait_check(h, PIN + 4 * slot, seq[slot], &r))) return rc;
+        const bool done = !(r >= c.tol) || it >= c.max_iter;   // NaN stops too and is reported
+        // a batch queued ahead of a converged check did nothing on the LDS solver: not counted
+        if (done && spec && lds && lds_for(h, true) && h->sw_last >= 0) h->sw_void[h->sw_last] = true;
         sweep_account(h);
         if (checks == 0) st->div0_inf = r;
         ++checks;
-        if (!(r >= c.tol) || it >= c.max_iter) {
-            // a speculative batch is discarded: on LDS its check wrote the other Phi buffer
+        if (done) {
+            // a batch queued ahead is discarded: on LDS its check would have written the other
+            // Phi buffer
             if (spec && lds) std::swap(h->phi, h->phi2);
             break;
         }
-        if (!spec && (rc = enqueue_batch(m, slot ^ 1))) return rc;
+        if (!spec && (rc = enqueue_batch(m, slot ^ 1, false))) return rc;
         it += m;
         r2 = r1;
         r1 = r;
@@ -1283,7 +1317,7 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
 // seeds) stepped concurrently: one host thread per handle drives that handle's own stream, so the
 // GPU overlaps the instances' kernels and each instance's residual checks wait only on its own
 // work.  Each instance computes exactly what mdx_step computes for it alone.  The LDS-resident
-// sweeps need the whole GPU for one cooperative launch and are switched off for batched handles.
+// sweeps need every CU for one launch and are switched off for batched handles.
 extern "C" int mdx_step_batch(mdx_handle *const *hs, int32_t n, double dt, mdx_step_stats *stats) {
     MDX_CHECK(hs && n > 0, MDX_ERR_ARG, "mdx_step_batch: need n > 0 handles");
     for (int q = 0; q < n; ++q) {

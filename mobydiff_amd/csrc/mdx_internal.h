@@ -156,13 +156,19 @@ struct mdx_handle {
     int ph_head = 0, ph_count = 0, ph_cur = PHQ;
     mdx_phase_times times{};
     // sweep batches not yet added to the counters: start/end events, iterations and launches
-    // (a speculative batch can be in flight while the previous one is accounted)
-    static constexpr int SW_Q = 4;
+    // (a speculative batch can be in flight while the previous one is accounted).  Every
+    // SW_EVERY-th batch is timed: the two events around a batch hold the stream for several
+    // microseconds, so timing every batch would slow the solve it measures.
+    static constexpr int SW_Q = 4, SW_EVERY = 4;
+    long long sw_seen = 0;   // batches so far
+    int sw_last = -1;        // accounting entry of the last batch (-1: not timed)
     hipEvent_t ev_sw[SW_Q][2] = {};
     int sw_its[SW_Q] = {}, sw_launches[SW_Q] = {};
     int sw_head = 0, sw_count = 0;
-    hipEvent_t ev_max[2] = {nullptr, nullptr};   // residual-max copies of the mixed solve's checks
-                                                 // (two in flight with speculative batches)
+    bool sw_void[SW_Q] = {};   // a batch that found its input converged and did nothing
+    // sequence number of the last residual check posted to a pinned slot: a check writes
+    // [max, error word] and then its number, which the host polls for
+    unsigned long long chk_seq = 0;
 
     // captured sweep batches: [0] FP64, [1] FP32
     struct GraphSlot {
@@ -182,6 +188,9 @@ struct mdx_handle {
     // LDS-resident red-black solver (MOBYDIFF_LDS_SOR): 0 off, 1 FP32 sweep batches (the default),
     // 2 FP32 and FP64 batches; plans per precision and the tagged face buffers
     int lds_sor_mode = 1;
+    // launch the LDS-resident kernel through hipLaunchCooperativeKernel (MOBYDIFF_LDS_COOP=1)
+    // instead of an ordinary launch of a grid that the plan already sized to be co-resident
+    bool rb_coop = false;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
     mdx::RbPlanOut rb_plan64{}, rb_plan32{};
     void *rb_faces = nullptr;      // tagged face words of the LDS-resident sweeps
@@ -223,8 +232,9 @@ hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int us
                                 const double *hx_u, double ymid);
 hipError_t launch_reset_max(mdx_handle *h);
 // max_part[0..n) -> red_max[0]; with host_slot (device address of pinned words) also the max and
-// the LDS-solver error word to host_slot[0..1]
-hipError_t launch_max_final(mdx_handle *h, unsigned nparts, double *host_slot = nullptr);
+// the LDS-solver error word to host_slot[0..1], then `seq` to host_slot[2]
+hipError_t launch_max_final(mdx_handle *h, unsigned nparts, double *host_slot = nullptr,
+                            unsigned long long seq = 0);
 static constexpr unsigned MAX_PARTS = 1u << 17;
 
 // red-black solver
@@ -235,7 +245,7 @@ hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega);
 hipError_t launch_sor_residual64(mdx_handle *h, const double *phi);
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg);
 hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32,
-                           double *host_slot = nullptr);
+                           double *host_slot = nullptr, unsigned long long seq = 0);
 hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega);
 hipError_t launch_sor_iters64(mdx_handle *h, int m, double omega);
 hipError_t launch_sor_iters32(mdx_handle *h, int m, float omega);
@@ -248,7 +258,12 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);
 struct RbDefect {
     const double *phi_in;
     double *phi_out;
-    double *host_slot;   // device address of the pinned words receiving [max, error word]
+    double *host_slot;   // device address of the pinned words receiving [max, error word, seq]
+    unsigned long long seq;
+    // skip: the batch does nothing when the previous check (red_max[0]) is not >= skip_tol, the
+    // solve's own stopping test, so a batch queued before that check was read costs one launch
+    int skip;
+    double skip_tol;
 };
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df = nullptr);
 size_t rb_faces_bytes(const RbPlanOut &p, bool fp32);

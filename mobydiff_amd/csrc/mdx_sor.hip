@@ -170,8 +170,9 @@ struct SumPhi {
 // After u = u* (pressure_redblack.f90:64-84): Dirichlet v on the walls, rhs = div(u), Phi = 0.
 template <bool STRETCH>
 __global__ __launch_bounds__(256) void sor_prepare_kernel(GridK g, double vbot, double vtop,
-                                                          double *vn, const double *__restrict__ un,
-                                                          const double *__restrict__ wn, double *rhs,
+                                                          const double *__restrict__ us,
+                                                          const double *__restrict__ vs,
+                                                          const double *__restrict__ ws, double *rhs,
                                                           double *phi, double idx, double idy, double idz) {
     const Dims d = g.d;
     int tx_, ty_, tz_;
@@ -181,40 +182,78 @@ __global__ __launch_bounds__(256) void sor_prepare_kernel(GridK g, double vbot, 
     const int k = tz_ + 1;
     if (i > d.nx || j > d.ny) return;
     const bool per2 = g.per[1];
-    if (!per2) {
-        if (j == 1) vn[d.g(i, 1, k)] = vbot;
-        if (j == d.ny) vn[d.g(i, d.ny + 1, k)] = vtop;
-    }
     const int ip = i == d.nx ? 1 : i + 1, kp = k == d.nz ? 1 : k + 1;
     const int jp = (per2 && j == d.ny) ? 1 : j + 1;
     const size_t c = d.g(i, j, k);
-    const double dvy = STRETCH ? (vn[d.g(i, jp, k)] - vn[c]) * g.ihc[j] : (vn[d.g(i, jp, k)] - vn[c]) * idy;
-    rhs[d.in(i, j, k)] = (un[d.g(ip, j, k)] - un[c]) * idx + dvy + (wn[d.g(i, j, kp)] - wn[c]) * idz;
+    // v of u* with the wall values in place (the projection writes them into vn)
+    const double v0 = (!per2 && j == 1) ? vbot : vs[c];
+    const double v1 = (!per2 && j == d.ny) ? vtop : vs[d.g(i, jp, k)];
+    const double dvy = STRETCH ? (v1 - v0) * g.ihc[j] : (v1 - v0) * idy;
+    rhs[d.in(i, j, k)] = (us[d.g(ip, j, k)] - us[c]) * idx + dvy + (ws[d.g(i, j, kp)] - ws[c]) * idz;
     phi[d.in(i, j, k)] = 0.0;
+}
+
+// un, vn, wn on the ghost shell: the values of u*, with the top wall value of v (walled y) on the
+// interior of the j = ny + 1 plane; the boundary conditions that follow the projection overwrite
+// the faces, and edges and corners keep the u* values
+__global__ __launch_bounds__(256) void proj_ghost_kernel(Dims d, int walled_y, double vtop,
+                                                         const double *__restrict__ us,
+                                                         const double *__restrict__ vs,
+                                                         const double *__restrict__ ws, double *un, double *vn,
+                                                         double *wn) {
+    const long long sz = d.nz + 2;
+    const long long pa = (long long)d.sx * d.sy, pb = (long long)d.sx * d.nz, pc = (long long)d.ny * d.nz;
+    const long long n = 2 * (pa + pb + pc);
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        long long q = t;
+        if (q < 2 * pa) {            // the k = 0 and k = nz + 1 planes
+            k = q >= pa ? (int)(sz - 1) : 0;
+            q %= pa;
+            i = (int)(q % d.sx);
+            j = (int)(q / d.sx);
+        } else if ((q -= 2 * pa) < 2 * pb) {   // the j = 0 and j = ny + 1 rows of the inner planes
+            j = q >= pb ? d.sy - 1 : 0;
+            q %= pb;
+            i = (int)(q % d.sx);
+            k = (int)(q / d.sx) + 1;
+        } else {                     // the i = 0 and i = nx + 1 cells of the inner rows
+            q -= 2 * pb;
+            i = q >= pc ? d.sx - 1 : 0;
+            q %= pc;
+            j = (int)(q % d.ny) + 1;
+            k = (int)(q / d.ny) + 1;
+        }
+        const size_t c = d.g(i, j, k);
+        un[c] = us[c];
+        wn[c] = ws[c];
+        const bool top = walled_y && j == d.ny + 1 && i >= 1 && i <= d.nx && k >= 1 && k <= d.nz;
+        vn[c] = top ? vtop : vs[c];
+    }
 }
 
 hipError_t launch_sor_prepare(mdx_handle *h) {
     GridK g = grid_k(h);
-    const size_t bytes = h->ng * sizeof(double);
-    hipError_t e;
-    if ((e = hipMemcpyAsync(h->un, h->us, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(h->vn, h->vs, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(h->wn, h->ws, bytes, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) return e;
     dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
     const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
     if (h->stretched)
-        hipLaunchKernelGGL(sor_prepare_kernel<true>, grd, blk, 0, h->stream, g, vb, vt, h->vn, h->un, h->wn,
+        hipLaunchKernelGGL(sor_prepare_kernel<true>, grd, blk, 0, h->stream, g, vb, vt, h->us, h->vs, h->ws,
                            h->rhs, h->phi, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz);
     else
-        hipLaunchKernelGGL(sor_prepare_kernel<false>, grd, blk, 0, h->stream, g, vb, vt, h->vn, h->un, h->wn,
+        hipLaunchKernelGGL(sor_prepare_kernel<false>, grd, blk, 0, h->stream, g, vb, vt, h->us, h->vs, h->ws,
                            h->rhs, h->phi, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz);
     return hipGetLastError();
 }
 
+// un = u* - mu grad Phi (likewise v, w) and pn += Phi / dt on the interior, reading u* directly;
+// on a walled y the bottom v face keeps its wall value
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu, const double *__restrict__ phi,
-                                                        double *un, double *vn, double *wn, double *pn,
-                                                        double idx, double idy, double idz, double idt) {
+                                                        const double *__restrict__ us,
+                                                        const double *__restrict__ vs,
+                                                        const double *__restrict__ ws, double vbot, double *un,
+                                                        double *vn, double *wn, double *pn, double idx, double idy,
+                                                        double idz, double idt) {
     const Dims d = g.d;
     int tx_, ty_, tz_;
     if (!xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_)) return;
@@ -229,12 +268,14 @@ __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu,
     const double mu_u = IBM ? mu.u[mu.at(i, j, k)] : 1.0;
     const double mu_v = IBM ? mu.v[mu.at(i, j, k)] : 1.0;
     const double mu_w = IBM ? mu.w[mu.at(i, j, k)] : 1.0;
-    un[c] = un[c] + mu_u * (phi[d.in(im, j, k)] - p0) * idx;
-    wn[c] = wn[c] + mu_w * (phi[d.in(i, j, km)] - p0) * idz;
+    un[c] = us[c] + mu_u * (phi[d.in(im, j, k)] - p0) * idx;
+    wn[c] = ws[c] + mu_w * (phi[d.in(i, j, km)] - p0) * idz;
     if (j >= 2 || per2) {
         const int jm = j == 1 ? d.ny : j - 1;
         const double gy = STRETCH ? (phi[d.in(i, jm, k)] - p0) * g.ihv[j] : (phi[d.in(i, jm, k)] - p0) * idy;
-        vn[c] = vn[c] + mu_v * gy;
+        vn[c] = vs[c] + mu_v * gy;
+    } else {
+        vn[c] = vbot;
     }
     pn[c] = pn[c] + p0 * idt;
 }
@@ -244,11 +285,15 @@ hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg) {
     MuK<double> mu = mu_k<double>(h, h->mu64);
     dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
     const double a = 1.0 / h->dx, b = 1.0 / h->dy, c = 1.0 / h->dz, t = 1.0 / dtg;
-#define MDX_APPLY(S, I) \
-    hipLaunchKernelGGL((sor_apply_kernel<S, I>), grd, blk, 0, h->stream, g, mu, phi, h->un, h->vn, h->wn, h->pn, a, b, c, t)
+    const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
+#define MDX_APPLY(S, I)                                                                                     \
+    hipLaunchKernelGGL((sor_apply_kernel<S, I>), grd, blk, 0, h->stream, g, mu, phi, h->us, h->vs, h->ws, vb, \
+                       h->un, h->vn, h->wn, h->pn, a, b, c, t)
     if (h->stretched) { if (h->ibm_on) MDX_APPLY(true, true); else MDX_APPLY(true, false); }
     else { if (h->ibm_on) MDX_APPLY(false, true); else MDX_APPLY(false, false); }
 #undef MDX_APPLY
+    hipLaunchKernelGGL(proj_ghost_kernel, dim3(512), dim3(256), 0, h->stream, h->d, h->cfg.periodic[1] ? 0 : 1, vt,
+                       h->us, h->vs, h->ws, h->un, h->vn, h->wn);
     return hipGetLastError();
 }
 
@@ -724,7 +769,7 @@ __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, 
 }
 
 hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32,
-                           double *host_slot) {
+                           double *host_slot, unsigned long long seq) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
@@ -736,7 +781,7 @@ hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, 
     else { if (h->ibm_on) MDX_DF(false, true); else MDX_DF(false, false); }
 #undef MDX_DF
     hipError_t err = hipGetLastError();
-    return err != hipSuccess ? err : launch_max_final(h, grd.x, host_slot);
+    return err != hipSuccess ? err : launch_max_final(h, grd.x, host_slot, seq);
 }
 
 template <bool STRETCH, bool IBM>

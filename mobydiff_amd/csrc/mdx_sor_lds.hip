@@ -18,9 +18,9 @@
 // data itself until every word it needs shows the expected tag: one store and one load per
 // hand-off, with no acknowledgement wait and no separate counter.  Faces are double-buffered by
 // iteration parity; a producer cannot overwrite a buffer before its consumers have read it,
-// because it first needs their next layers.  All workgroups must be co-resident: the launch is
-// cooperative (the runtime refuses a grid that does not fit) and every wait is bounded (a spin
-// limit sets an error flag and the kernel runs to completion).
+// because it first needs their next layers.  All workgroups must be co-resident: the plan never
+// holds more workgroups than the CUs take at once (runtime occupancy), and every wait is bounded
+// (a spin limit sets an error flag and the kernel runs to completion).
 //
 // The arithmetic per cell is sor_residual_core(), the same expressions the streaming sweeps use,
 // and the relaxation factor -omega/denom is formed from the same values, so the iterates are
@@ -74,9 +74,14 @@ struct RbArgs {
     double *phi_out, *max_part;
     float *r_out;
     // workgroup 0 folds the workgroups' maxima into red_max[0] and the host slot, with the error
-    // word red_max[1]
+    // word red_max[1], then writes seq to host_slot[2]
     unsigned long long *red_max;
     double *host_slot;
+    unsigned long long seq;
+    // skip: return at once when the previous check, red_max[0], is not >= skip_tol (converged, or
+    // not a number): a batch queued ahead of the host's read of that check then costs one launch
+    int skip;
+    double skip_tol;
 };
 
 // a value as tagged 64-bit words: the tag in the high half, 32 payload bits in the low half
@@ -140,6 +145,12 @@ __host__ __device__ inline int rb_ring_total(int bx, int by, int bz) { return by
 
 template <typename T, bool IBM, int MAXP>
 __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
+    if (a.skip) {
+        // every workgroup reads the same word: the previous kernel's result, which this launch's
+        // workgroup 0 overwrites only after all workgroups have published their epilogue maxima
+        const unsigned long long rb = __hip_atomic_load(&a.red_max[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(__longlong_as_double((long long)rb) >= a.skip_tol)) return;
+    }
     extern __shared__ unsigned char smem_raw[];
     using TW = Tagged<T>;
     constexpr int NW = TW::NW;
@@ -606,6 +617,8 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         a.red_max[0] = (unsigned long long)__double_as_longlong(m);
         a.host_slot[0] = m;
         a.host_slot[1] = __longlong_as_double((long long)errw);
+        __threadfence_system();
+        reinterpret_cast<volatile unsigned long long *>(a.host_slot)[2] = a.seq;
     }
 }
 
@@ -661,7 +674,7 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
     const int nx = h->d.nx, ny = h->d.ny, nz = h->d.nz;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) { why = "no device properties"; return false; }
-    if (!prop.cooperativeLaunch) { why = "device reports no cooperative launch"; return false; }
+    if (h->rb_coop && !prop.cooperativeLaunch) { why = "device reports no cooperative launch"; return false; }
     const int cus = prop.multiProcessorCount;
     const size_t lds_max = prop.maxSharedMemoryPerMultiProcessor > prop.sharedMemPerBlock
                                ? prop.maxSharedMemoryPerMultiProcessor : prop.sharedMemPerBlock;
@@ -731,6 +744,9 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     a.r_out = h->r32;
     a.red_max = h->red_max;
     a.host_slot = df ? df->host_slot : nullptr;
+    a.seq = df ? df->seq : 0;
+    a.skip = df ? df->skip : 0;
+    a.skip_tol = df ? df->skip_tol : 0.0;
     a.d = h->d;
     a.px = pl.px; a.py = pl.py; a.pz = pl.pz;
     a.bx = pl.bx; a.by = pl.by; a.bz = pl.bz;
@@ -769,8 +785,15 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds);
     if (e != hipSuccess) return e;
     void *args[] = {&a};
-    return hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args, (unsigned)pl.lds,
-                                      h->stream);
+    // The plan holds at most as many workgroups as the CUs take at once (runtime occupancy), and a
+    // dispatch waits for nothing but earlier work on this stream, so every workgroup becomes
+    // resident with an ordinary launch; the cooperative launch gives the same guarantee formally
+    // at about 30 us of extra dispatch time per launch.  Every neighbour wait is bounded and
+    // raises the error word, so a workgroup that never arrived ends the batch with an error.
+    if (h->rb_coop)
+        return hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args,
+                                          (unsigned)pl.lds, h->stream);
+    return hipLaunchKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args, (size_t)pl.lds, h->stream);
 }
 
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df) {

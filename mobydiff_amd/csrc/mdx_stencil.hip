@@ -481,7 +481,8 @@ hipError_t launch_apply_bc_redblack(mdx_handle *h) {
 __device__ inline double nanmax(double a, double b) { return (a != a || a > b) ? a : b; }
 
 __global__ __launch_bounds__(1024) void max_final_kernel(const double *__restrict__ part, unsigned n,
-                                                         unsigned long long *out, double *host_slot) {
+                                                         unsigned long long *out, double *host_slot,
+                                                         unsigned long long seq) {
     __shared__ double w[16];
     double m = 0.0;
     for (unsigned q = threadIdx.x; q < n; q += 1024) m = nanmax(m, part[q]);
@@ -492,17 +493,19 @@ __global__ __launch_bounds__(1024) void max_final_kernel(const double *__restric
         for (int q = 1; q < 16; ++q) m = nanmax(m, w[q]);
         m = nanmax(w[0], m);
         out[0] = (unsigned long long)__double_as_longlong(m);
-        if (host_slot) {   // the result and the error word straight into pinned host memory
-            host_slot[0] = m;
+        if (host_slot) {   // the result and the error word straight into pinned host memory,
+            host_slot[0] = m;  // then the sequence number the host polls for
             host_slot[1] = __longlong_as_double((long long)out[1]);
+            __threadfence_system();
+            reinterpret_cast<volatile unsigned long long *>(host_slot)[2] = seq;
         }
     }
 }
 
-hipError_t launch_max_final(mdx_handle *h, unsigned nparts, double *host_slot) {
+hipError_t launch_max_final(mdx_handle *h, unsigned nparts, double *host_slot, unsigned long long seq) {
     if (nparts > MAX_PARTS) return hipErrorInvalidValue;
     hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(1024), 0, h->stream, h->max_part, nparts, h->red_max,
-                       host_slot);
+                       host_slot, seq);
     return hipGetLastError();
 }
 
