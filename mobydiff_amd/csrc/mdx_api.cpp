@@ -315,6 +315,16 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         h->lds_sor_mode = v;
     }
     if (const char *lc = std::getenv("MOBYDIFF_LDS_COOP")) h->rb_coop = lc[0] == '1';
+    if (const char *lf = std::getenv("MOBYDIFF_LDS_FORM")) {
+        if (std::strcmp(lf, "pairs") == 0) h->rb_form = 1;
+        else if (std::strcmp(lf, "columns") == 0) h->rb_form = 2;
+        else if (std::strcmp(lf, "halves") != 0) {
+            mdx_set_error("mdx_create: MOBYDIFF_LDS_FORM must be halves, columns or pairs");
+            free_all(h);
+            delete h;
+            return MDX_ERR_ARG;
+        }
+    }
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
     if (const char *sf = std::getenv("MOBYDIFF_SWEEP_FUSED")) h->sweep_fused = sf[0] == '1';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
@@ -665,8 +675,8 @@ static int rb_prepare(mdx_handle *h) {
     }
     auto desc = [](const RbPlanOut &p) {
         return std::to_string(p.px) + "x" + std::to_string(p.py) + "x" + std::to_string(p.pz) + " workgroups of " +
-               std::to_string(p.bx) + "x" + std::to_string(p.by) + "x" + std::to_string(p.bz) + " cells, LDS " +
-               std::to_string(p.lds) + " B";
+               std::to_string(p.bx) + "x" + std::to_string(p.by) + "x" + std::to_string(p.bz) + " cells (" +
+               (p.maxp == -1 ? "a z-column per thread" : (p.maxp == -2 ? "half a z-column per thread" : "x-pairs")) + "), LDS " + std::to_string(p.lds) + " B";
     };
     h->rb_note = "LDS-resident sweeps (two-deep halo, one exchange per iteration): fp32 " + desc(p32);
     if (h->lds_sor_mode == 2) h->rb_note += "; fp64 " + desc(p64);

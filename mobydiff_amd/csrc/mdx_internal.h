@@ -191,6 +191,10 @@ struct mdx_handle {
     // launch the LDS-resident kernel through hipLaunchCooperativeKernel (MOBYDIFF_LDS_COOP=1)
     // instead of an ordinary launch of a grid that the plan already sized to be co-resident
     bool rb_coop = false;
+    // thread layout of the LDS-resident kernel (MOBYDIFF_LDS_FORM): 0 half a z-column per thread,
+    // in workgroups of 1024 threads, where the subdomain and its face ring fit; 1 x-pairs anywhere
+    // in the tile; 2 one z-column per thread
+    int rb_form = 0;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
     mdx::RbPlanOut rb_plan64{}, rb_plan32{};
     void *rb_faces = nullptr;      // tagged face words of the LDS-resident sweeps

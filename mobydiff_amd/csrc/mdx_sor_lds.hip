@@ -34,6 +34,11 @@ static constexpr int RB_HE = 12;     // exchanged values per thread (a subdomain
 static constexpr int RB_MAXR = 3;    // colour-0 ring cells per thread
 static constexpr int RB_G = 3;       // cells relaxed per group (loads issued together)
 static constexpr int RB_H = 2;       // halo depth
+// column forms: cells per thread and colour, for a whole column (bz <= 2 * RB_ZC1 - 2) and for a half
+// column (bz <= 4 * RB_ZC2 - 4)
+static constexpr int RB_ZC1 = 13, RB_ZC2 = 7;
+// plan codes (in place of the pairs per thread) of the column forms
+static constexpr int RB_COLS = -1, RB_COLS2 = -2;
 static constexpr int RB_XSTRIDE = RB_HE * RB_THREADS;   // exchange entries per workgroup and buffer
 // Fixed LDS geometry: a tile row holds RB_TXP values and a plane RB_TYP rows, whatever the
 // subdomain (bx + 4 <= RB_TXP, by + 4 <= RB_TYP), so every stencil and coefficient-plane offset is
@@ -143,8 +148,22 @@ __host__ __device__ inline int rb_xchg_total(int bx, int by, int bz) {
 // colour-0 ring cells: on each face ring, half of by*bz (x faces), bx*bz (y) or bx*by (z) cells
 __host__ __device__ inline int rb_ring_total(int bx, int by, int bz) { return by * bz + bx * bz + bx * by; }
 
-template <typename T, bool IBM, int MAXP>
-__global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
+// COLS: column form.  A thread owns one column (i, j) of the subdomain or of the face ring and
+// marches it in z, relaxing every other cell; the face coefficients of the column stay in
+// registers (z-invariant tables) and the z neighbour between two cells of one colour is loaded
+// once, so a cell costs six LDS loads and one store instead of thirteen loads and a store.
+// Otherwise (pair form) a thread owns MAXP x-pairs and up to RB_MAXR ring cells anywhere in the
+// tile and reads their coefficients from an LDS table; the pair form also relaxes the deep box's
+// colour 0 while the halo is in flight.
+// FORM: 0 pair form, 1 column form, 2 column form with two threads per column (the lower and the
+// upper half of its z range) in workgroups of twice the threads.
+template <typename T, bool IBM, int MAXP, int FORM>
+__global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
+    // threads per workgroup, exchange entries per thread (the face buffer layout is the same for
+    // every form) and, in the column forms, cells per thread and colour
+    constexpr bool COLS = FORM > 0;
+    constexpr int NT = FORM == 2 ? 2 * RB_THREADS : RB_THREADS, HE = RB_XSTRIDE / NT;
+    constexpr int ZC = FORM == 2 ? RB_ZC2 : RB_ZC1;
     if (a.skip) {
         // every workgroup reads the same word: the previous kernel's result, which this launch's
         // workgroup 0 overwrites only after all workgroups have published their epilogue maxima
@@ -176,7 +195,8 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     const bool per_y = a.per_y != 0;
     // LDS index of local cell (i, j, k); the subdomain is 1..b, the halo -1..0 and b+1..b+2
     auto L = [&](int i, int j, int k) { return (i + 1) + RB_TXP * (j + 1) + RB_PLANE * (k + 1); };
-    auto wrap = [](int g, int n) { return ((g - 1) % n + n) % n + 1; };
+    // periodic image in 1..n of a coordinate within one period of the domain (-n < g <= 2n)
+    auto wrap = [](int g, int n) { return g < 1 ? g + n : (g > n ? g - n : g); };
     // the workgroup in direction (dx, dy, dz), or -1 beyond a wall
     auto neighbour = [&](int dx, int dy, int dz) {
         int ny_ = wy + dy;
@@ -188,23 +208,25 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     };
     auto wall_bits = [&](int gj) { return (!per_y && gj == d.ny ? 1 : 0) | (!per_y && gj == 1 ? 2 : 0); };
 
-    // ---- load the tile (subdomain and halo) from the global solution
-    for (int q = t; q < tx * ty * tz; q += RB_THREADS) {
-        const int li = q % tx - 1, lj = (q / tx) % ty - 1, lk = q / (tx * ty) - 1;
+    // ---- load the tile (subdomain and halo) from the global solution, in LDS order over the fixed
+    // row and plane (constant divisors; values past the tile extent in a row or plane are zero)
+    for (int q = t; q < RB_PLANE * tz; q += NT) {
+        const int xi = q % RB_TXP, yi = (q / RB_TXP) % RB_TYP;
+        const int li = xi - 1, lj = yi - 1, lk = q / RB_PLANE - 1;
         T v = T(0);
-        if (!a.zero_init) {
+        if (!a.zero_init && xi < tx && yi < ty) {
             const int gi = wrap(ox + li, d.nx), gk = wrap(oz + lk, d.nz);
             int gj = oy + lj;
             if (per_y) gj = wrap(gj, d.ny);
             if (gj >= 1 && gj <= d.ny) v = a.x[d.in(gi, gj, gk)];
         }
-        X[L(li, lj, lk)] = v;
+        X[q] = v;
     }
     // ---- face coefficient table (one x-y plane, ring included): mui, muip, mvj, mvjp, mw, -omega/denom
     T qconst = T(0);
-    if (IBM) {
+    if (IBM && !COLS) {
         const int sx = d.sx;
-        for (int q0 = t; q0 < (bx + 2) * (by + 2); q0 += RB_THREADS) {
+        for (int q0 = t; q0 < (bx + 2) * (by + 2); q0 += NT) {
             const int li = q0 % (bx + 2), lj = q0 / (bx + 2), q = li + cw * lj;
             const int gi = wrap(ox + li, d.nx);
             int gj = oy + lj;
@@ -275,14 +297,15 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         }
     };
     auto pack = [](int c, int ti, int walls, int neg) { return c | (ti << 16) | (walls << 28) | (neg << 30); };
-    int pw[MAXP];
-    T rh[MAXP][2];
+    constexpr int NP = COLS ? 1 : MAXP, NR = COLS ? 1 : RB_MAXR;
+    int pw[NP];
+    T rh[NP][2];
 #pragma unroll
-    for (int m = 0; m < MAXP; ++m) {
-        const int p = t + m * RB_THREADS;
+    for (int m = 0; m < NP; ++m) {
+        const int p = t + m * NT;
         pw[m] = pack(dummy, 0, 0, 0);
         rh[m][0] = rh[m][1] = T(0);
-        if (p < npairs) {
+        if (!COLS && p < npairs) {
             int pi, j, k;
             pair_of(p, pi, j, k);
             const int i0 = 2 * pi + 1;
@@ -297,13 +320,18 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     // ---- colour-0 cells of the face ring owned by neighbours, recomputed here.  Face f has fixed
     // coordinate 0 or b+1 along its axis A and spans 1..bU x 1..bV (U = y for x faces, else x);
     // with bU even each U-line holds bU/2 colour-0 cells.
-    int rw[RB_MAXR];   // packed as the pair words
-    T rr[RB_MAXR];
-    {
+    int rw[NR];   // packed as the pair words
+    T rr[NR];
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        rw[m] = pack(dummy, 0, 0, 0);
+        rr[m] = T(0);
+    }
+    if (!COLS) {
         const int nr = rb_ring_total(bx, by, bz);
 #pragma unroll
-        for (int m = 0; m < RB_MAXR; ++m) {
-            int e = t + m * RB_THREADS;
+        for (int m = 0; m < NR; ++m) {
+            int e = t + m * NT;
             rw[m] = pack(dummy, 0, 0, 0);
             rr[m] = T(0);
             if (e < nr) {
@@ -334,46 +362,141 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
             }
         }
     }
-    // ---- exchange lists: entry e = t + jj * RB_THREADS of the 18 regions.  Publish: the LDS cell
+    // ---- column form: the column of this thread.  Threads 0 .. bx*by-1 own the subdomain's
+    // columns and march z from 0 to bz+1 in colour 0 (the two end cells are the z faces of the
+    // ring) and from 1 to bz in colour 1; the next 2*by threads own the x faces of the ring and the
+    // next 2*bx its y faces (colour 0 only, z from 1 to bz; none beyond a wall).  Cells past a
+    // column's end, and every cell of a thread without a column, go to the dummy cell.
+    int cbase = dummy, cstep0 = 0, cstep1 = 0;   // first colour-0 / colour-1 cell and steps (0: dummy)
+    int cfirst1 = dummy, n0 = 0, n1 = 0;
+    bool cwn = false, cws = false;
+    T cmui = 1, cmuip = 1, cmvj = 1, cmvjp = 1, cmw = 1, cq = qconst;
+    T r0[COLS ? ZC : 1], r1[COLS ? ZC : 1];
+    if (COLS) {
+        // FORM 2: threads tc and tc + RB_THREADS share column tc, the lower and the upper half of its
+        // z range
+        const int tc = t % RB_THREADS, half = t / RB_THREADS;
+        const int nin = bx * by, nxf = 2 * by, nyf = 2 * bx;
+        int cli = 0, clj = 0, kz0 = 0, kz1 = -1, kc0 = 1, kc1 = bz;   // colour-0 and colour-1 z ranges
+        bool inner = false;
+        if (tc < nin) {
+            cli = tc % bx + 1; clj = tc / bx + 1; kz0 = 0; kz1 = bz + 1; inner = true;
+        } else if (tc < nin + nxf) {
+            const int e = tc - nin;
+            cli = e < by ? 0 : bx + 1; clj = e % by + 1; kz0 = 1; kz1 = bz;
+        } else if (tc < nin + nxf + nyf) {
+            const int e = tc - nin - nxf;
+            clj = e < bx ? 0 : by + 1; cli = e % bx + 1;
+            if (neighbour(0, clj == 0 ? -1 : 1, 0) >= 0) { kz0 = 1; kz1 = bz; }
+        }
+        if (FORM == 2) {
+            const int h0 = (kz1 - kz0 + 2) / 2, h1 = bz / 2;   // positions in the lower halves
+            if (half == 0) { kz1 = kz0 + h0 - 1; kc1 = h1; }
+            else { kz0 += h0; kc0 = h1 + 1; }
+        }
+        if (kz1 >= kz0) {
+            const int gi = wrap(ox + cli, d.nx);
+            int gj = oy + clj;
+            if (per_y) gj = wrap(gj, d.ny);
+            const int par = (ox + cli + oy + clj + oz) & 1;   // colour c relaxes (par + lk + c) odd
+            const int k0 = kz0 + (((par + kz0) & 1) ? 0 : 1);
+            n0 = k0 <= kz1 ? (kz1 - k0) / 2 + 1 : 0;
+            cbase = L(cli, clj, k0);
+            cstep0 = 2 * RB_PLANE;
+#pragma unroll
+            for (int n = 0; n < ZC; ++n)
+                r0[n] = n < n0 ? a.rhs[d.in(gi, gj, wrap(oz + k0 + 2 * n, d.nz))] : T(0);
+            if (inner) {
+                const int k1 = kc0 + (((par + kc0) & 1) ? 1 : 0);   // colour 1: (par + lk) even
+                n1 = k1 <= kc1 ? (kc1 - k1) / 2 + 1 : 0;
+                cfirst1 = L(cli, clj, k1);
+                cstep1 = 2 * RB_PLANE;
+#pragma unroll
+                for (int n = 0; n < ZC; ++n) r1[n] = n < n1 ? a.rhs[d.in(gi, gj, oz + k1 + 2 * n)] : T(0);
+            }
+            const int wb = wall_bits(gj);
+            cwn = (wb & 1) != 0;
+            cws = (wb & 2) != 0;
+            if (IBM) {   // the face coefficients of the column, as the streaming sweeps form them
+                const int sx = d.sx, gip = gi == d.nx ? 1 : gi + 1, gjp = (per_y && gj == d.ny) ? 1 : gj + 1;
+                cmui = a.mu_u[(size_t)gi + (size_t)sx * gj];
+                cmuip = a.mu_u[(size_t)gip + (size_t)sx * gj];
+                cmvj = a.mu_v[(size_t)gi + (size_t)sx * gj];
+                cmvjp = a.mu_v[(size_t)gi + (size_t)sx * gjp];
+                cmw = a.mu_w[(size_t)gi + (size_t)sx * gj];
+                const T denom = (cmui + cmuip) * a.s.idx2 + (cmw + cmw) * a.s.idz2 + (cmvj + cmvjp) * a.s.idy2;
+                cq = -a.omega / denom;
+            }
+        }
+    }
+    // ---- exchange lists: entry e = t + jj * NT of the 18 regions.  Publish: the LDS cell
     // this workgroup sends for its neighbour in direction D.  Pull: the region that the neighbour
     // in direction -D sent for this workgroup: LDS destination (bits 0-19) and D (bits 20-24),
     // the sender's id coming from the neighbour table.
     __shared__ int nbr_from[19];   // the workgroup that sends direction q's region to this one
+    // region table: first entry and extents of the 18 regions
+    __shared__ int rg_off[19], rg_ext[18][3];
     if (t < 18) {
         int dx, dy, dz;
         rb_dir(t, dx, dy, dz);
         nbr_from[t] = neighbour(-dx, -dy, -dz);
+        rb_extent(t, bx, by, bz, rg_ext[t][0], rg_ext[t][1], rg_ext[t][2]);
     }
     if (t == 18) nbr_from[18] = w;
+    __syncthreads();
+    if (t == 0) {
+        int off = 0;
+        for (int q = 0; q < 18; ++q) {
+            rg_off[q] = off;
+            off += rg_ext[q][0] * rg_ext[q][1] * rg_ext[q][2];
+        }
+        rg_off[18] = off;
+    }
+    __syncthreads();
     const int total = a.total;
-    // Unused entries publish the dummy cell; an entry with no sender (beyond a wall, or past the
-    // end of the list) polls this thread's own just-published word, which carries the tag, and
-    // lands in the dummy cell: the exchange needs no branches either.  Direction code 18 = self.
-    int xpub[RB_HE], xdst[RB_HE];
+    // Unused entries publish the dummy cell (pair form); an entry with no sender (beyond a wall, or
+    // past the end of the list) does not wait (bit 25 of xdst clear), reads this thread's own slot
+    // (direction code 18 = self) and lands in the dummy cell.
+    int xpub[HE], xdst[HE];
 #pragma unroll
-    for (int jj = 0; jj < RB_HE; ++jj) {
-        const int e = t + jj * RB_THREADS;
-        xpub[jj] = dummy; xdst[jj] = dummy | (18 << 20);
+    for (int jj = 0; jj < HE; ++jj) {
+        const int e = t + jj * NT;
+        xpub[jj] = dummy | (1 << 20); xdst[jj] = dummy | (18 << 20);
         if (e < total) {
-            int q = 0, off = 0, ex = 0, ey = 0, ez = 0;
-            for (; q < 18; ++q) {
-                rb_extent(q, bx, by, bz, ex, ey, ez);
-                if (e < off + ex * ey * ez) break;
-                off += ex * ey * ez;
-            }
+            int q = 0;
+#pragma unroll
+            for (int u = 1; u < 18; ++u) q += e >= rg_off[u] ? 1 : 0;
+            const int off = rg_off[q], ex = rg_ext[q][0], ey = rg_ext[q][1];
             int dx, dy, dz;
             rb_dir(q, dx, dy, dz);
-            const int h = q < 6 ? RB_H : 1, r = e - off;
-            const int rx = r % ex, ry = (r / ex) % ey, rz = r / (ex * ey);
-            // sender side: the layer next to its face D
-            auto snd = [&](int dd, int rr_, int bb) { return dd > 0 ? bb - h + 1 + rr_ : 1 + rr_; };
-            xpub[jj] = L(snd(dx, rx, bx), snd(dy, ry, by), snd(dz, rz, bz));
-            // receiver side: the sender is the neighbour in direction -D; its layer lands in the halo
-            if (neighbour(-dx, -dy, -dz) >= 0) {
-                auto rcv = [&](int dd, int rr_, int bb) {
-                    return dd < 0 ? bb + 1 + rr_ : (dd > 0 ? 1 - h + rr_ : 1 + rr_);
-                };
-                xdst[jj] = L(rcv(dx, rx, bx), rcv(dy, ry, by), rcv(dz, rz, bz)) | (q << 20);
+            const int h = q < 6 ? RB_H : 1;
+            // A region lists its colour-0 cells first, then its colour-1 cells (colour 0: odd global
+            // coordinate sum), so each colour can be published as soon as it is relaxed with whole
+            // lines of consecutive entries.  Cells 2k and 2k+1 of the region's natural order are
+            // neighbours along its first extent above one, of opposite colours; entry k of a colour
+            // half is the one of the two with that colour, on the sending and the receiving side alike.
+            const int half = (rg_off[q + 1] - off) / 2, rp = e - off, col = rp >= half ? 1 : 0;
+            const int k2 = 2 * (rp - col * half);
+            auto cell = [&](int r, auto side, int &cx, int &cy, int &cz) {
+                cx = side(dx, r % ex, bx, h);
+                cy = side(dy, (r / ex) % ey, by, h);
+                cz = side(dz, r / (ex * ey), bz, h);
+            };
+            // sender side: the layer next to its face D; receiver side: the sender is the neighbour in
+            // direction -D, and its layer lands in the halo
+            auto snd = [](int dd, int rr_, int bb, int hh) { return dd > 0 ? bb - hh + 1 + rr_ : 1 + rr_; };
+            auto rcv = [](int dd, int rr_, int bb, int hh) {
+                return dd < 0 ? bb + 1 + rr_ : (dd > 0 ? 1 - hh + rr_ : 1 + rr_);
+            };
+            auto colour = [&](int cx, int cy, int cz) { return ((ox + cx + oy + cy + oz + cz) & 1) ? 0 : 1; };
+            int cx, cy, cz;
+            cell(k2, snd, cx, cy, cz);
+            if (colour(cx, cy, cz) != col) cell(k2 + 1, snd, cx, cy, cz);
+            xpub[jj] = L(cx, cy, cz) | (col << 20);
+            if (nbr_from[q] >= 0) {
+                cell(k2, rcv, cx, cy, cz);
+                if (colour(cx, cy, cz) != col) cell(k2 + 1, rcv, cx, cy, cz);
+                xdst[jj] = L(cx, cy, cz) | (q << 20) | (1 << 25);
             }
         }
     }
@@ -397,24 +520,73 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         return p0 + r * q;
     };
     auto partner = [](int pwd) { return ((pwd >> 30) & 1) ? -1 : 1; };
+    // column form: the cell at LDS index c with its z neighbours already in registers
+    auto relax_col = [&](int c, T zlo, T zhi, T rhs) -> T {
+        T denom;
+        const T p0 = X[c];
+        const T r = sor_residual_core<T, false>(p0, X[c + 1], X[c - 1], X[c + RB_TXP], X[c - RB_TXP], zhi, zlo, cmui,
+                                                cmuip, cmvj, cmvjp, cmw, cmw, cwn, cws, rhs, a.s, T(0), T(0), denom);
+        return p0 + r * cq;
+    };
+    // one colour along this thread's column: n cells from LDS index c0 in steps of two planes; the
+    // upper z neighbour of a cell is the lower one of the next
+    auto march = [&](int c0, int step, int n, const T *rhs) {
+        int c = c0;
+        T zlo = X[c - RB_PLANE];
+#pragma unroll
+        for (int q = 0; q < ZC; ++q) {
+            const int cc = q < n ? c : dummy;
+            const T zhi = X[cc + RB_PLANE];
+            const T v = relax_col(cc, zlo, zhi, rhs[q]);
+            X[cc] = v;
+            zlo = zhi;
+            c += step;
+            if ((q % RB_G) == RB_G - 1) __builtin_amdgcn_sched_barrier(0);
+        }
+    };
 
     unsigned long long *F = a.faces;
+    // publish the layer cells of colour `col`, tagged with iteration s, into buffer s & 1
+    auto publish = [&](int s, int col) {
+        const unsigned tag = a.tag0 + (unsigned)s + 1u;
+        const size_t mybase = ((size_t)(w * 2 + (s & 1)) * RB_XSTRIDE) * NW;
+#pragma unroll
+        for (int jj = 0; jj < HE; ++jj) {
+            if (((xpub[jj] >> 20) & 1) != col) continue;
+            unsigned long long wv[NW];
+            TW::pack(X[xpub[jj] & 0xfffff], tag, wv);
+            unsigned long long *dst = F + mybase + (size_t)(t + jj * NT) * NW;
+#pragma unroll
+            for (int q = 0; q < NW; ++q)
+                __hip_atomic_store(dst + q, wv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    };
     for (int s = 0; s < a.iters; ++s) {
         // The packed words are loop-invariant; left visible, the compiler would keep every address
         // derived from them (13 per cell and colour) live across the loop and run out of registers.
         // An empty asm that claims to redefine them makes the few address adds happen per use.
 #pragma unroll
-        for (int m = 0; m < MAXP; ++m) asm volatile("" : "+v"(pw[m]));
+        for (int m = 0; m < NP; ++m) asm volatile("" : "+v"(pw[m]));
 #pragma unroll
-        for (int m = 0; m < RB_MAXR; ++m) asm volatile("" : "+v"(rw[m]));
+        for (int m = 0; m < NR; ++m) asm volatile("" : "+v"(rw[m]));
 #pragma unroll
-        for (int jj = 0; jj < RB_HE; ++jj) asm volatile("" : "+v"(xpub[jj]), "+v"(xdst[jj]));
+        for (int jj = 0; jj < HE; ++jj) asm volatile("" : "+v"(xpub[jj]), "+v"(xdst[jj]));
         // ---- colour 0 on the subdomain and the face ring.  Cells of one colour never neighbour
         // each other, so each is stored as soon as it is formed; groups of RB_G cells with a
         // scheduling barrier between them bound the loads in flight (registers).  After the first
         // iteration the deep groups (g < a.md) were relaxed while the halo was in flight.
+        if (COLS) {
+            // column data: loop-invariant, kept from being hoisted as per-cell addresses (as above)
+            asm volatile("" : "+v"(cbase), "+v"(cfirst1));
+            if (n0 > 0) march(cbase, cstep0, n0, r0);
+            __syncthreads();
+            if (s < a.iters - 1 || a.defect) publish(s, 0);
+            if (n1 > 0) march(cfirst1, cstep1, n1, r1);
+            __syncthreads();
+        }
 #pragma unroll
-        for (int g = 0; g < MAXP; g += RB_G) {
+        for (int g = 0; g < NP; g += RB_G) {
+            if (COLS) break;
             if (s > 0 && g < a.md) continue;
             T nv[RB_G];
 #pragma unroll
@@ -423,17 +595,19 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
             for (int u = 0; u < RB_G; ++u) X[pw[g + u] & 0xffff] = nv[u];
             __builtin_amdgcn_sched_barrier(0);
         }
-        {
-            T nv[RB_MAXR];
+        if (!COLS) {
+            T nv[NR];
 #pragma unroll
-            for (int m = 0; m < RB_MAXR; ++m) nv[m] = relax(rw[m], 0, rr[m]);
+            for (int m = 0; m < NR; ++m) nv[m] = relax(rw[m], 0, rr[m]);
 #pragma unroll
-            for (int m = 0; m < RB_MAXR; ++m) X[rw[m] & 0xffff] = nv[m];
+            for (int m = 0; m < NR; ++m) X[rw[m] & 0xffff] = nv[m];
+            __syncthreads();
+            if (s < a.iters - 1 || a.defect) publish(s, 0);
         }
-        __syncthreads();
         // ---- colour 1 on the subdomain: its neighbours are subdomain or ring cells, all updated
 #pragma unroll
-        for (int g = 0; g < MAXP; g += RB_G) {
+        for (int g = 0; g < NP; g += RB_G) {
+            if (COLS) break;
             T nv[RB_G];
 #pragma unroll
             for (int u = 0; u < RB_G; ++u) nv[u] = relax(pw[g + u], partner(pw[g + u]), rh[g + u][1]);
@@ -441,30 +615,21 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
             for (int u = 0; u < RB_G; ++u) X[(pw[g + u] & 0xffff) + partner(pw[g + u])] = nv[u];
             __builtin_amdgcn_sched_barrier(0);
         }
-        __syncthreads();
+        if (!COLS) __syncthreads();
         // the last iteration needs no exchange, unless the defect epilogue reads the final halo
         if (s == a.iters - 1 && !a.defect) break;
 
-        // ---- publish this subdomain's layers, tagged with this iteration
+        // ---- publish this subdomain's colour-1 layer cells (colour 0 went out after its pass)
         const int buf = s & 1;
         const unsigned tag = a.tag0 + (unsigned)s + 1u;
-        const size_t mybase = ((size_t)(w * 2 + buf) * RB_XSTRIDE) * NW;
-#pragma unroll
-        for (int jj = 0; jj < RB_HE; ++jj) {
-            unsigned long long wv[NW];
-            TW::pack(X[xpub[jj]], tag, wv);
-            unsigned long long *dst = F + mybase + (size_t)(t + jj * RB_THREADS) * NW;
-#pragma unroll
-            for (int q = 0; q < NW; ++q)
-                __hip_atomic_store(dst + q, wv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        publish(s, 1);
         // ---- the next iteration's colour 0 on the deep groups, while the neighbours' layers travel:
         // those cells read only subdomain cells of colour 1, final for this iteration; the barrier
         // lets the publish read its layers first
-        if (s < a.iters - 1) {
+        if (!COLS && s < a.iters - 1) {
             __syncthreads();
 #pragma unroll
-            for (int g = 0; g < MAXP; g += RB_G) {
+            for (int g = 0; g < NP; g += RB_G) {
                 if (g >= a.md) continue;
                 T nv[RB_G];
 #pragma unroll
@@ -478,20 +643,21 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
         // source word of entry jj: the sender's buffer `buf`, entry e, in words
         auto src_of = [&](int jj) {
             const int sender = nbr_from[(xdst[jj] >> 20) & 31];
-            return ((size_t)(sender * 2 + buf) * RB_XSTRIDE + (size_t)(t + jj * RB_THREADS)) * NW;
+            return ((size_t)(sender * 2 + buf) * RB_XSTRIDE + (size_t)(t + jj * NT)) * NW;
         };
-        unsigned long long got[RB_HE][NW];
+        unsigned long long got[HE][NW];
 #pragma unroll
-        for (int jj = 0; jj < RB_HE; ++jj)
+        for (int jj = 0; jj < HE; ++jj)
 #pragma unroll
             for (int q = 0; q < NW; ++q)
                 got[jj][q] = __hip_atomic_load(F + src_of(jj) + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (int spins = 0; !failed; ++spins) {
             bool ready = true;
 #pragma unroll
-            for (int jj = 0; jj < RB_HE; ++jj)
+            for (int jj = 0; jj < HE; ++jj)
 #pragma unroll
-                for (int q = 0; q < NW; ++q) ready = ready && (unsigned)(got[jj][q] >> 32) == tag;
+                for (int q = 0; q < NW; ++q)
+                    ready = ready && (!((xdst[jj] >> 25) & 1) || (unsigned)(got[jj][q] >> 32) == tag);
             if (ready) break;
             if (spins >= (1 << 20)) {   // a neighbour never published: flag it and run to the end
                 __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -500,21 +666,21 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
             }
             __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-            for (int jj = 0; jj < RB_HE; ++jj)
+            for (int jj = 0; jj < HE; ++jj)
 #pragma unroll
                 for (int q = 0; q < NW; ++q)
-                    if ((unsigned)(got[jj][q] >> 32) != tag)
+                    if (((xdst[jj] >> 25) & 1) && (unsigned)(got[jj][q] >> 32) != tag)
                         got[jj][q] = __hip_atomic_load(F + src_of(jj) + q, __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT);
         }
 #pragma unroll
-        for (int jj = 0; jj < RB_HE; ++jj)
+        for (int jj = 0; jj < HE; ++jj)
             X[xdst[jj] & 0xfffff] = TW::unpack(got[jj]);
         __syncthreads();
     }
     if (!a.defect) {
         // ---- write the subdomain back
-        for (int q = t; q < bx * by * bz; q += RB_THREADS) {
+        for (int q = t; q < bx * by * bz; q += NT) {
             const int i = q % bx + 1, j = (q / bx) % by + 1, k = q / (bx * by) + 1;
             a.x[d.in(ox + i, oy + j, oz + k)] = X[L(i, j, k)];
         }
@@ -575,7 +741,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     // ---- the maximum over workgroups without fences: every workgroup publishes its partial as two
     // tagged words (tag0 + iters + 1, unique to this launch) after the face buffers; workgroup 0
     // polls them all (bounded) and writes the result to red_max[0] and the host slot
-    __shared__ double wred[RB_THREADS / 64];
+    __shared__ double wred[NT / 64];
     for (int off = 32; off > 0; off >>= 1) mx = nanmax_any(mx, __shfl_xor(mx, off, 64));
     if ((t & 63) == 0) wred[t >> 6] = mx;
     __syncthreads();
@@ -583,7 +749,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     unsigned long long *MP = a.faces + (size_t)nwg * 2 * RB_XSTRIDE * NW;   // 2 words per workgroup
     if (t == 0) {
         double m = wred[0];
-        for (int q = 1; q < RB_THREADS / 64; ++q) m = nanmax_any(m, wred[q]);
+        for (int q = 1; q < NT / 64; ++q) m = nanmax_any(m, wred[q]);
         unsigned long long wv[2];
         Tagged<double>::pack(m, mtag, wv);
         __hip_atomic_store(MP + 2 * w, wv[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -591,7 +757,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     }
     if (blockIdx.x != 0) return;
     double m = 0.0;
-    for (int q = t; q < nwg; q += RB_THREADS) {
+    for (int q = t; q < nwg; q += NT) {
         unsigned long long wv[2];
         for (int spins = 0;; ++spins) {
             wv[0] = __hip_atomic_load(MP + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -611,7 +777,7 @@ __global__ __launch_bounds__(RB_THREADS) void rb_lds_kernel(RbArgs<T> a) {
     if ((t & 63) == 0) wred[t >> 6] = m;
     __syncthreads();
     if (t == 0) {
-        for (int q = 1; q < RB_THREADS / 64; ++q) m = nanmax_any(m, wred[q]);
+        for (int q = 1; q < NT / 64; ++q) m = nanmax_any(m, wred[q]);
         m = nanmax_any(wred[0], m);
         const unsigned long long errw = __hip_atomic_load(&a.red_max[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         a.red_max[0] = (unsigned long long)__double_as_longlong(m);
@@ -630,15 +796,17 @@ static size_t rb_lds_bytes(int bx, int by, int bz, size_t tsize, bool ibm) {
     return ((size_t)RB_PLANE * tz + 2 * RB_PLANE + 3 + (ibm ? 6 * (size_t)RB_PLANE : 0)) * tsize;
 }
 
-template <typename T, bool IBM, int MAXP>
+template <typename T, bool IBM, int MAXP, int FORM = 0>
 static const void *rb_fn() {
-    return reinterpret_cast<const void *>(&rb_lds_kernel<T, IBM, MAXP>);
+    return reinterpret_cast<const void *>(&rb_lds_kernel<T, IBM, MAXP, FORM>);
 }
 
-// pairs per thread: multiples of the group size RB_G
+// pairs per thread: multiples of the group size RB_G; RB_COLS: the column form
 template <typename T, bool IBM>
 static const void *rb_fn_for(int maxp) {
     switch (maxp) {
+    case RB_COLS: return rb_fn<T, IBM, RB_G, 1>();
+    case RB_COLS2: return rb_fn<T, IBM, RB_G, 2>();
     case 3: return rb_fn<T, IBM, 3>();
     case 6: return rb_fn<T, IBM, 6>();
     case 9: return rb_fn<T, IBM, 9>();
@@ -646,6 +814,9 @@ static const void *rb_fn_for(int maxp) {
     default: return rb_fn<T, IBM, 15>();
     }
 }
+
+// threads per workgroup of a plan's kernel form
+static int rb_threads(int maxp) { return maxp == RB_COLS2 ? 2 * RB_THREADS : RB_THREADS; }
 
 static int rb_maxp_class(int need) {
     const int c = (need + RB_G - 1) / RB_G * RB_G;
@@ -661,7 +832,7 @@ static int rb_occupancy(bool fp32, bool ibm, int maxp, size_t lds) {
         return 0;
     }
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, RB_THREADS, lds) != hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, rb_threads(maxp), lds) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
@@ -697,7 +868,11 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
                 // padding included) < 2^16, coefficient-table index < 2^12
                 if (bx + 2 * RB_H > RB_TXP || by + 2 * RB_H > RB_TYP) continue;
                 if ((size_t)RB_PLANE * (bz + 2 * RB_H) + 2 * RB_PLANE + 3 >= 65536 || RB_PLANE >= 4096) continue;
-                const int maxp = rb_maxp_class((bx / 2 * by * bz + RB_THREADS - 1) / RB_THREADS);
+                // the column form where a thread per column (subdomain and face ring) fits
+                const bool colfit = bx * by + 2 * (bx + by) <= RB_THREADS && bx >= 4 && by >= 4 && bz >= 4;
+                int maxp = rb_maxp_class((bx / 2 * by * bz + RB_THREADS - 1) / RB_THREADS);
+                if (colfit && h->rb_form == 0 && bz <= 4 * RB_ZC2 - 4) maxp = RB_COLS2;
+                if (colfit && h->rb_form == 2 && bz <= 2 * RB_ZC1 - 2) maxp = RB_COLS;
                 if (!maxp) continue;
                 if (rb_ring_total(bx, by, bz) > RB_MAXR * RB_THREADS) continue;
                 const int total = rb_xchg_total(bx, by, bz);
@@ -791,9 +966,10 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     // at about 30 us of extra dispatch time per launch.  Every neighbour wait is bounded and
     // raises the error word, so a workgroup that never arrived ends the batch with an error.
     if (h->rb_coop)
-        return hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args,
+        return hipLaunchCooperativeKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(rb_threads(pl.maxp)), args,
                                           (unsigned)pl.lds, h->stream);
-    return hipLaunchKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(RB_THREADS), args, (size_t)pl.lds, h->stream);
+    return hipLaunchKernel(fn, dim3(pl.px * pl.py * pl.pz), dim3(rb_threads(pl.maxp)), args, (size_t)pl.lds,
+                           h->stream);
 }
 
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df) {
