@@ -198,7 +198,7 @@ static void free_all(mdx_handle *h) {
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
                     h->red_part, h->red_max, h->max_part, h->rb_faces, h->phi_hist, h->res64,
                     h->fft_phi, h->fft_res, h->fft_mean, h->hat32, h->cp32, h->den32, h->arow32, h->tw_x32,
-                    h->tw_z32};
+                    h->tw_z32, h->rb_xl32, h->rb_xl64};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -688,6 +688,14 @@ static int rb_prepare(mdx_handle *h) {
     MDX_HIP(hipMalloc(&h->rb_faces, need));
     MDX_HIP(hipMemset(h->rb_faces, 0, need));   // tag 0 is never published
     h->rb_faces_bytes = need;
+    for (int **xl : {&h->rb_xl32, &h->rb_xl64})
+        if (*xl) { (void)hipFree(*xl); *xl = nullptr; }
+    MDX_HIP(hipMalloc(&h->rb_xl32, rb_xlist_bytes(p32)));
+    MDX_HIP(rb_build_xlists(h, p32, h->rb_xl32));
+    if (h->lds_sor_mode == 2) {
+        MDX_HIP(hipMalloc(&h->rb_xl64, rb_xlist_bytes(p64)));
+        MDX_HIP(rb_build_xlists(h, p64, h->rb_xl64));
+    }
     h->rb_tag = 0;
     h->rb_plan64 = p64;
     h->rb_plan32 = p32;

@@ -198,6 +198,7 @@ struct mdx_handle {
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
     mdx::RbPlanOut rb_plan64{}, rb_plan32{};
     void *rb_faces = nullptr;      // tagged face words of the LDS-resident sweeps
+    int *rb_xl32 = nullptr, *rb_xl64 = nullptr;   // exchange lists of the two plans
     size_t rb_faces_bytes = 0;
     int *rb_err = nullptr;         // points into red_max[1]
     unsigned long long rb_tag = 0; // half-sweeps published so far (the tag of the last one)
@@ -271,6 +272,8 @@ struct RbDefect {
 };
 hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, const RbDefect *df = nullptr);
 size_t rb_faces_bytes(const RbPlanOut &p, bool fp32);
+size_t rb_xlist_bytes(const RbPlanOut &p);
+hipError_t rb_build_xlists(mdx_handle *h, const RbPlanOut &p, int *xl);
 
 // FFT path
 hipError_t launch_fft_forward(mdx_handle *h, double dtg);

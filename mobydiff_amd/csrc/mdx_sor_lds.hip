@@ -64,6 +64,7 @@ struct RbArgs {
     int md;                  // leading pair entries of every thread that lie in the deep box
     unsigned long long *faces;   // [wg][buffer][exchange entry][words per value], tagged words
     int total;               // exchange entries in use per workgroup (faces two deep, then edges)
+    const int *xl;           // exchange lists of the plan: [wg][entry] publish words, then pull words
     unsigned tag0;           // iteration s of this launch publishes tag tag0 + s + 1 (never 0)
     int *err;
     // Defect epilogue (the FP32 batches of the mixed solve): after the last iteration and one more
@@ -429,76 +430,20 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
             }
         }
     }
-    // ---- exchange lists: entry e = t + jj * NT of the 18 regions.  Publish: the LDS cell
-    // this workgroup sends for its neighbour in direction D.  Pull: the region that the neighbour
-    // in direction -D sent for this workgroup: LDS destination (bits 0-19) and D (bits 20-24),
-    // the sender's id coming from the neighbour table.
+    // ---- exchange lists (built once per plan by rb_xlist_kernel): entry e = t + jj * NT
     __shared__ int nbr_from[19];   // the workgroup that sends direction q's region to this one
-    // region table: first entry and extents of the 18 regions
-    __shared__ int rg_off[19], rg_ext[18][3];
     if (t < 18) {
         int dx, dy, dz;
         rb_dir(t, dx, dy, dz);
         nbr_from[t] = neighbour(-dx, -dy, -dz);
-        rb_extent(t, bx, by, bz, rg_ext[t][0], rg_ext[t][1], rg_ext[t][2]);
     }
     if (t == 18) nbr_from[18] = w;
-    __syncthreads();
-    if (t == 0) {
-        int off = 0;
-        for (int q = 0; q < 18; ++q) {
-            rg_off[q] = off;
-            off += rg_ext[q][0] * rg_ext[q][1] * rg_ext[q][2];
-        }
-        rg_off[18] = off;
-    }
-    __syncthreads();
-    const int total = a.total;
-    // Unused entries publish the dummy cell (pair form); an entry with no sender (beyond a wall, or
-    // past the end of the list) does not wait (bit 25 of xdst clear), reads this thread's own slot
-    // (direction code 18 = self) and lands in the dummy cell.
     int xpub[HE], xdst[HE];
 #pragma unroll
     for (int jj = 0; jj < HE; ++jj) {
-        const int e = t + jj * NT;
-        xpub[jj] = dummy | (1 << 20); xdst[jj] = dummy | (18 << 20);
-        if (e < total) {
-            int q = 0;
-#pragma unroll
-            for (int u = 1; u < 18; ++u) q += e >= rg_off[u] ? 1 : 0;
-            const int off = rg_off[q], ex = rg_ext[q][0], ey = rg_ext[q][1];
-            int dx, dy, dz;
-            rb_dir(q, dx, dy, dz);
-            const int h = q < 6 ? RB_H : 1;
-            // A region lists its colour-0 cells first, then its colour-1 cells (colour 0: odd global
-            // coordinate sum), so each colour can be published as soon as it is relaxed with whole
-            // lines of consecutive entries.  Cells 2k and 2k+1 of the region's natural order are
-            // neighbours along its first extent above one, of opposite colours; entry k of a colour
-            // half is the one of the two with that colour, on the sending and the receiving side alike.
-            const int half = (rg_off[q + 1] - off) / 2, rp = e - off, col = rp >= half ? 1 : 0;
-            const int k2 = 2 * (rp - col * half);
-            auto cell = [&](int r, auto side, int &cx, int &cy, int &cz) {
-                cx = side(dx, r % ex, bx, h);
-                cy = side(dy, (r / ex) % ey, by, h);
-                cz = side(dz, r / (ex * ey), bz, h);
-            };
-            // sender side: the layer next to its face D; receiver side: the sender is the neighbour in
-            // direction -D, and its layer lands in the halo
-            auto snd = [](int dd, int rr_, int bb, int hh) { return dd > 0 ? bb - hh + 1 + rr_ : 1 + rr_; };
-            auto rcv = [](int dd, int rr_, int bb, int hh) {
-                return dd < 0 ? bb + 1 + rr_ : (dd > 0 ? 1 - hh + rr_ : 1 + rr_);
-            };
-            auto colour = [&](int cx, int cy, int cz) { return ((ox + cx + oy + cy + oz + cz) & 1) ? 0 : 1; };
-            int cx, cy, cz;
-            cell(k2, snd, cx, cy, cz);
-            if (colour(cx, cy, cz) != col) cell(k2 + 1, snd, cx, cy, cz);
-            xpub[jj] = L(cx, cy, cz) | (col << 20);
-            if (nbr_from[q] >= 0) {
-                cell(k2, rcv, cx, cy, cz);
-                if (colour(cx, cy, cz) != col) cell(k2 + 1, rcv, cx, cy, cz);
-                xdst[jj] = L(cx, cy, cz) | (q << 20) | (1 << 25);
-            }
-        }
+        const size_t e = (size_t)t + (size_t)jj * NT;
+        xpub[jj] = a.xl[(size_t)w * RB_XSTRIDE + e];
+        xdst[jj] = a.xl[((size_t)nwg + w) * RB_XSTRIDE + e];
     }
     bool failed = false;
     __syncthreads();
@@ -550,11 +495,17 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
     auto publish = [&](int s, int col) {
         const unsigned tag = a.tag0 + (unsigned)s + 1u;
         const size_t mybase = ((size_t)(w * 2 + (s & 1)) * RB_XSTRIDE) * NW;
+        // every layer value first (one LDS wait), then the stores of this colour's entries
+        T val[HE];
+#pragma unroll
+        for (int jj = 0; jj < HE; ++jj) val[jj] = X[xpub[jj] & 0xfffff];
+#pragma unroll
+        for (int jj = 0; jj < HE; ++jj) asm volatile("" : "+v"(val[jj]));
 #pragma unroll
         for (int jj = 0; jj < HE; ++jj) {
             if (((xpub[jj] >> 20) & 1) != col) continue;
             unsigned long long wv[NW];
-            TW::pack(X[xpub[jj] & 0xfffff], tag, wv);
+            TW::pack(val[jj], tag, wv);
             unsigned long long *dst = F + mybase + (size_t)(t + jj * NT) * NW;
 #pragma unroll
             for (int q = 0; q < NW; ++q)
@@ -788,6 +739,74 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
     }
 }
 
+// Exchange lists of workgroup w (subdomain id), entry e of the 18 regions (faces two layers deep,
+// then edges one cell thick; entry e = t + jj * threads of thread t).  Publish word: the LDS cell
+// this workgroup sends for its neighbour in direction D (bits 0-19) and its colour (bit 20).  Pull
+// word: the halo cell where the region that the neighbour in direction -D sent lands (bits 0-19),
+// D (bits 20-24) and whether to wait for it (bit 25).  A region lists its colour-0 cells first,
+// then its colour-1 cells (colour 0: odd global coordinate sum), so each colour can be published
+// as soon as it is relaxed, with whole lines of consecutive entries: cells 2k and 2k+1 of the
+// region's natural order are neighbours along its first extent above one, of opposite colours,
+// and entry k of a colour half is the one of the two with that colour, on both sides alike.
+// Unused entries publish the dummy cell; an entry with no sender (beyond a wall, or past the end
+// of the list) does not wait, reads this workgroup's own slot (direction 18) and lands in the
+// dummy cell.  Built once per plan; the batch kernel only loads it.
+struct RbGeom {
+    int px, py, pz, bx, by, bz, per_y, total, dummy;
+};
+
+__global__ __launch_bounds__(256) void rb_xlist_kernel(RbGeom g, int *xl) {
+    const int w = blockIdx.y, nwg = g.px * g.py * g.pz;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= RB_XSTRIDE) return;
+    const int bx = g.bx, by = g.by, bz = g.bz;
+    const int wx = w % g.px, wy = (w / g.px) % g.py, wz = w / (g.px * g.py);
+    const int ox = wx * bx, oy = wy * by, oz = wz * bz;
+    auto L = [](int i, int j, int k) { return (i + 1) + RB_TXP * (j + 1) + RB_PLANE * (k + 1); };
+    auto has_sender = [&](int dx, int dy, int dz) {   // the neighbour in direction -D exists
+        (void)dx;
+        (void)dz;
+        const int ny_ = wy - dy;
+        return g.per_y || (ny_ >= 0 && ny_ < g.py);
+    };
+    int xpub = g.dummy | (1 << 20), xdst = g.dummy | (18 << 20);
+    if (e < g.total) {
+        int q = 0, off = 0, ex = 0, ey = 0, ez = 0;
+        for (; q < 18; ++q) {
+            rb_extent(q, bx, by, bz, ex, ey, ez);
+            if (e < off + ex * ey * ez) break;
+            off += ex * ey * ez;
+        }
+        int dx, dy, dz;
+        rb_dir(q, dx, dy, dz);
+        const int h = q < 6 ? RB_H : 1;
+        const int half = ex * ey * ez / 2, rp = e - off, col = rp >= half ? 1 : 0;
+        const int k2 = 2 * (rp - col * half);
+        auto cell = [&](int r, auto side, int &cx, int &cy, int &cz) {
+            cx = side(dx, r % ex, bx, h);
+            cy = side(dy, (r / ex) % ey, by, h);
+            cz = side(dz, r / (ex * ey), bz, h);
+        };
+        // sender side: the layer next to its face D; receiver side: the halo beyond face -D
+        auto snd = [](int dd, int rr_, int bb, int hh) { return dd > 0 ? bb - hh + 1 + rr_ : 1 + rr_; };
+        auto rcv = [](int dd, int rr_, int bb, int hh) {
+            return dd < 0 ? bb + 1 + rr_ : (dd > 0 ? 1 - hh + rr_ : 1 + rr_);
+        };
+        auto colour = [&](int cx, int cy, int cz) { return ((ox + cx + oy + cy + oz + cz) & 1) ? 0 : 1; };
+        int cx, cy, cz;
+        cell(k2, snd, cx, cy, cz);
+        if (colour(cx, cy, cz) != col) cell(k2 + 1, snd, cx, cy, cz);
+        xpub = L(cx, cy, cz) | (col << 20);
+        if (has_sender(dx, dy, dz)) {
+            cell(k2, rcv, cx, cy, cz);
+            if (colour(cx, cy, cz) != col) cell(k2 + 1, rcv, cx, cy, cz);
+            xdst = L(cx, cy, cz) | (q << 20) | (1 << 25);
+        }
+    }
+    xl[(size_t)w * RB_XSTRIDE + e] = xpub;
+    xl[((size_t)nwg + w) * RB_XSTRIDE + e] = xdst;
+}
+
 // ------------------------------------------------------------------------------- host side
 static size_t rb_lds_bytes(int bx, int by, int bz, size_t tsize, bool ibm) {
     (void)bx;
@@ -894,6 +913,17 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
     return found;
 }
 
+// the exchange lists of plan p into xl (2 * workgroups * RB_XSTRIDE ints), on the handle's stream
+hipError_t rb_build_xlists(mdx_handle *h, const RbPlanOut &p, int *xl) {
+    const int nwg = p.px * p.py * p.pz;
+    const RbGeom g{p.px, p.py, p.pz, p.bx, p.by, p.bz, h->cfg.periodic[1] ? 1 : 0, p.total,
+                   RB_PLANE * (p.bz + 2 * RB_H) + RB_PLANE + 1};
+    hipLaunchKernelGGL(rb_xlist_kernel, dim3((RB_XSTRIDE + 255) / 256, nwg), dim3(256), 0, h->stream, g, xl);
+    return hipGetLastError();
+}
+
+size_t rb_xlist_bytes(const RbPlanOut &p) { return 2 * (size_t)p.px * p.py * p.pz * RB_XSTRIDE * sizeof(int); }
+
 size_t rb_faces_bytes(const RbPlanOut &p, bool fp32) {
     const size_t nwg = (size_t)p.px * p.py * p.pz;
     return nwg * 2 * (size_t)RB_XSTRIDE * 8 * (fp32 ? 1 : 2) + nwg * 16;   // faces, then the maxima
@@ -946,6 +976,7 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     }
     a.faces = reinterpret_cast<unsigned long long *>(h->rb_faces);
     a.total = pl.total;
+    a.xl = sizeof(T) == 4 ? h->rb_xl32 : h->rb_xl64;
     // tags of this launch: rb_tag + 1 .. rb_tag + iters + 1 (the last for the maxima); before the
     // 32-bit tag would wrap to 0 the face buffer is cleared and the count restarts
     if (h->rb_tag + (unsigned long long)iters + 2ull > 0xffffffffull) {
