@@ -931,10 +931,15 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
     double r = 0.0, r1 = -1.0, r2 = -1.0;   // r_k, r_{k-1}, r_{k-2}
     int it = 0, checks = 0, slot = 0;
     unsigned long long seq[2] = {0, 0};     // the check each slot waits for
-    MDX_HIP(launch_zero32(h, h->e32, h->ni));
     auto slot_dev = [&](int sl) { return h->d_pinned + PIN + 4 * sl; };
-    auto enqueue_check = [&](int sl) -> int {   // separate defect kernel: Phi += e, r32, max
+    // separate defect kernel: Phi += e, r32, max; the first check of a solve has no e (Phi += 0.0,
+    // in place)
+    auto enqueue_check = [&](int sl, bool first) -> int {
         seq[sl] = ++h->chk_seq;
+        if (first) {
+            MDX_HIP(launch_defect64(h, h->phi, nullptr, h->phi, h->r32, slot_dev(sl), seq[sl]));
+            return MDX_OK;
+        }
         MDX_HIP(launch_defect64(h, h->phi, h->e32, h->phi2, h->r32, slot_dev(sl), seq[sl]));
         std::swap(h->phi, h->phi2);
         return MDX_OK;
@@ -953,7 +958,7 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
         return sweep_batch(h, true, m);
     };
     int rc;
-    if ((rc = enqueue_check(slot))) return rc;
+    if ((rc = enqueue_check(slot, true))) return rc;
     for (;;) {
         const int m = std::min(c.inner_iters, c.max_iter - it);
         // the first check of a solve is taken to be unconverged (a fresh right-hand side); later
@@ -981,7 +986,7 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
         it += m;
         r2 = r1;
         r1 = r;
-        if (!lds && (rc = enqueue_check(slot ^ 1))) return rc;
+        if (!lds && (rc = enqueue_check(slot ^ 1, false))) return rc;
         slot ^= 1;
     }
     st->iterations = it;

@@ -156,13 +156,16 @@ struct PlainPhi {
 };
 
 // Phi64 + e32, the corrected iterate of the defect-correction loop
+// Phi_in + e of one cell; without e (the first check of a solve, e = 0) Phi_in + 0.0, the same
+// value (signed zeros included)
+template <bool E>
 struct SumPhi {
     const double *p;
     const float *e;
     Dims d;
     __device__ inline double operator()(int i, int j, int k) const {
         size_t q = d.in(i, j, k);
-        return p[q] + (double)e[q];
+        return p[q] + (E ? (double)e[q] : 0.0);
     }
 };
 
@@ -742,11 +745,12 @@ hipError_t launch_sor_residual64(mdx_handle *h, const double *phi) { return laun
 
 // ------------------------------------------------------------------------------- mixed precision
 // One FP64 defect-correction step: Phi_out = Phi_in + e; r = rhs - L_mu(Phi_out) in FP64;
-// r32 = (float) r for the next FP32 inner solve; max |r| for the stopping test.
-template <bool STRETCH, bool IBM>
+// r32 = (float) r for the next FP32 inner solve; max |r| for the stopping test.  Without e (E
+// false: e = 0, the first check of a solve) Phi_out may be Phi_in itself: every read adds 0.0,
+// so a neighbour read before or after its own cell's rewrite gives the same value.
+template <bool STRETCH, bool IBM, bool E>
 __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
-                                                       const double *__restrict__ rhs,
-                                                       const double *__restrict__ phi_in,
+                                                       const double *__restrict__ rhs, const double *phi_in,
                                                        const float *__restrict__ e, double *phi_out,
                                                        float *r32, double *out) {
     const Dims d = g.d;
@@ -757,7 +761,7 @@ __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, 
     const int k = tz_ + 1;
     double m = 0.0;
     if (ok && i <= d.nx && j <= d.ny) {
-        SumPhi P{phi_in, e, d};
+        SumPhi<E> P{phi_in, e, d};
         double denom;
         const size_t q = d.in(i, j, k);
         const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[q], i, j, k, denom);
@@ -774,11 +778,13 @@ hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, 
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
     dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
-#define MDX_DF(S, I)                                                                                       \
-    hipLaunchKernelGGL((defect64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi_in, e, phi_out, \
+#define MDX_DF(S, I, E)                                                                                       \
+    hipLaunchKernelGGL((defect64_kernel<S, I, E>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi_in, e, phi_out, \
                        r32, h->max_part)
-    if (h->stretched) { if (h->ibm_on) MDX_DF(true, true); else MDX_DF(true, false); }
-    else { if (h->ibm_on) MDX_DF(false, true); else MDX_DF(false, false); }
+#define MDX_DFE(S, I) do { if (e) MDX_DF(S, I, true); else MDX_DF(S, I, false); } while (0)
+    if (h->stretched) { if (h->ibm_on) MDX_DFE(true, true); else MDX_DFE(true, false); }
+    else { if (h->ibm_on) MDX_DFE(false, true); else MDX_DFE(false, false); }
+#undef MDX_DFE
 #undef MDX_DF
     hipError_t err = hipGetLastError();
     return err != hipSuccess ? err : launch_max_final(h, grd.x, host_slot, seq);
