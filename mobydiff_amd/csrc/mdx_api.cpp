@@ -315,6 +315,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         h->lds_sor_mode = v;
     }
     if (const char *lc = std::getenv("MOBYDIFF_LDS_COOP")) h->rb_coop = lc[0] == '1';
+    if (const char *nm = std::getenv("MOBYDIFF_NET_MARGIN")) h->net_margin = std::atof(nm);
     if (const char *lf = std::getenv("MOBYDIFF_LDS_FORM")) {
         if (std::strcmp(lf, "pairs") == 0) h->rb_form = 1;
         else if (std::strcmp(lf, "columns") == 0) h->rb_form = 2;
@@ -1023,7 +1024,7 @@ static void net_outcome(mdx_handle *h, int s, int iterations) {
 //   Phi = 0 (the previous pressure: the plain method, and the start without a history);
 //   Phi_0 = fac * Phi of stage s at the previous step (MDX_GUESS_PHI_PREV and MDX_GUESS_LEARNED);
 //   Phi_0 + the network's correction, predicted from (rhs - L_mu Phi_0, Phi_0) (MDX_GUESS_LEARNED),
-//   taken when its FP64 residual max-norm is below NET_MARGIN times that of Phi_0.
+//   taken when its FP64 residual max-norm is below net_margin times that of Phi_0.
 // In learned mode a start other than the network's is a fallback (stats.fallback).  With capture
 // on, the network input is built for every solve so the converged Phi can be stored as a training
 // pair.
@@ -1094,13 +1095,12 @@ static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, 
     }
     st->guess_div_inf = rg;
     // the network's start is taken only when it clearly beats the other start (residual below
-    // NET_MARGIN times it): a marginally smaller start residual saves no sweeps but costs an
+    // net_margin times it): a marginally smaller start residual saves no sweeps but costs an
     // inference per solve.  A start not taken counts as a loss; a start taken is judged by the
     // sweeps its solve needs (net_outcome).
-    constexpr double NET_MARGIN = 0.8;
     const double other = prev_start || !have_prev ? rp : -1.0;
     // the chosen start is copied into phi (captured sweep graphs hold that buffer's address)
-    if (other < 0.0 || rg < NET_MARGIN * other) {
+    if (other < 0.0 || rg < h->net_margin * other) {
         MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
         *r_first = rg;
         h->net_started[s] = true;

@@ -216,6 +216,9 @@ struct mdx_handle {
     // skip, whether the current solve starts from the network's guess, and the sweeps of the last
     // solve that did not (the yardstick of a network start)
     int net_losses[3] = {0, 0, 0}, net_skip[3] = {0, 0, 0};
+    // the network's start is taken when its residual max-norm is below net_margin times the other
+    // start's (MOBYDIFF_NET_MARGIN)
+    double net_margin = 0.8;
     bool net_started[3] = {false, false, false};
     int base_its[3] = {0, 0, 0};
     long proj_count = 0;
