@@ -130,8 +130,9 @@ def flat_fft_production(M, steps, precision=None):
     return {"workload": "flat channel 256x192x256 (beta=1.5, Re_b=5600, dt=5e-4), FFT+tridiagonal Poisson, " + label,
             "value": round(cells * steps / el / 1e6, 3), "unit": "Mcell-steps/s", "steps": steps,
             "ms_per_step": round(el / steps * 1e3, 3),
-            "phase_ms_per_step": {"stencil": round(t.stencil_ms / steps, 3), "poisson": round(t.poisson_ms / steps, 3),
-                                  "projection": round(t.projection_ms / steps, 3)},
+            "phase_ms_per_step": {"stencil": round(t.stencil_ms * 3.0 / max(t.stages, 1), 3),
+                                  "poisson": round(t.poisson_ms * 3.0 / max(t.stages, 1), 3),
+                                  "projection": round(t.projection_ms * 3.0 / max(t.stages, 1), 3)},
             "div_inf_final": div,
             "step_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(gbs / HBM_PEAK_GBS, 4),
@@ -237,13 +238,24 @@ def main():
     prevphi = fresh(M.PREC_MIXED, M.GUESS_PHI_PREV, prime=True)
     el_q, its_q, div_q, t_q, fb_q = run_steps(prevphi, args.steps)
     prevphi.close()
+    network = None
+    if learned:
+        # MFMA utilisation of the network: device time of one forward pass (HIP events on its stream)
+        ms_f, fl = net.time_forward(10)
+        ach = fl / (ms_f * 1e-3) / 1e12
+        network = {"kernel": "U-Net forward pass (BF16 MFMA, FP32 accumulate), one 192x128x96 input",
+                   "bound": "mfma", "forward_ms": round(ms_f, 3), "flops": fl, "achieved": round(ach, 1),
+                   "peak": 2500.0, "unit": "TFLOP/s", "frac": round(ach / 2500.0, 4)}
     cells = NX * NY * NZ
     rate = lambda el: cells * args.steps / el / 1e6   # noqa: E731
     per_stage = lambda its: sum(map(sum, its)) / (3.0 * args.steps)   # noqa: E731
 
     def split(t, k):
-        return {"stencil": round(t.stencil_ms / k, 3), "poisson": round(t.poisson_ms / k, 3),
-                "projection": round(t.projection_ms / k, 3), "guess": round(t.guess_ms / k, 3)}
+        # stencil / Poisson / projection come from a sample of timed stages (t.stages of them):
+        # three stages make a step; the network's time covers every start
+        st = 3.0 / max(t.stages, 1)
+        return {"stencil": round(t.stencil_ms * st, 3), "poisson": round(t.poisson_ms * st, 3),
+                "projection": round(t.projection_ms * st, 3), "guess": round(t.guess_ms / k, 3)}
 
     def side(el, its, div, t):
         return {"value": round(rate(el), 3), "ms_per_step": round(el / args.steps * 1e3, 3),
@@ -268,6 +280,7 @@ def main():
         "div_inf_final": div_h,
         "fallback_rate": round(fb_h / (3.0 * args.steps), 4) if learned else None,
         "training": training,
+        "network": network,
         "plain_fp64": dict(side(el_p, its_p, div_p, t_p), roofline=roofline(t_p, "fp64", info_p)),
         "mixed_no_guess": side(el_m, its_m, div_m, t_m),
         "mixed_previous_phi_guess": dict(side(el_q, its_q, div_q, t_q),

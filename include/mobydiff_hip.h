@@ -96,6 +96,8 @@ typedef struct {
     double ub;               /* bulk velocity after the step (flowrate on) */
 } mdx_step_stats;
 
+/* stencil, Poisson and projection times cover a sample of the stages mdx_step runs (every fourth
+ * stage is event-bracketed) and `stages` counts that sample; guess_ms covers every network start */
 typedef struct {
     double stencil_ms;       /* momentum + flow-rate shift + boundary conditions */
     double poisson_ms;       /* divergence RHS + pressure solve (FFT/tridiag or SOR sweeps + checks) */
@@ -208,6 +210,9 @@ int mdx_net_replay_reserve(mdx_net *net, int32_t capacity);
 int mdx_net_replay_count(mdx_net *net, int32_t *count);
 int mdx_net_train(mdx_net *net, int32_t steps, int32_t batch, float lr, uint64_t seed, float *mean_loss);
 int mdx_net_eval_replay(mdx_net *net, int32_t count, float *mean_loss);
+/* device time of one forward pass over replay sample 0 (HIP events around reps passes) and the
+ * MFMA flops of one pass (as the convolution kernels issue them) */
+int mdx_net_time_forward(mdx_net *net, int32_t reps, float *ms_per_forward, double *flops);
 /* attach (or detach with net = NULL) a network to a solver handle: cfg.guess = MDX_GUESS_LEARNED
  * uses it for the initial guess; capture != 0 appends a pair per converged SOR solve */
 int mdx_attach_net(mdx_handle *h, mdx_net *net, int32_t capture);

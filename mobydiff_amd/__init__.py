@@ -367,3 +367,9 @@ class Net:
         loss = C.c_float()
         _check(lib().mdx_net_eval_replay(self.h, C.c_int32(count), C.byref(loss)))
         return loss.value
+
+    def time_forward(self, reps=10):
+        """(ms per forward pass on the device, MFMA flops of one pass)"""
+        ms, fl = C.c_float(), C.c_double()
+        _check(lib().mdx_net_time_forward(self.h, C.c_int32(reps), C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value

@@ -727,7 +727,7 @@ static int ensure_mu(mdx_handle *h, double dtg) {
 }
 
 static void ev_record(mdx_handle *h, int q) {
-    if (!h->timing) return;
+    if (!h->timing || (q < 5 && h->ph_skip)) return;
     (void)hipEventRecord(q < 5 ? h->ev_ph[h->ph_cur][q] : h->ev[q], h->stream);
 }
 
@@ -1307,7 +1307,11 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
     int rc;
     for (int s = 0; s < 3; ++s) {
         const double dta = dt * RK_ALPHA[s], dtb = dt * RK_BETA[s], dtg = dt * RK_GAMMA[s];
-        if (h->timing) {   // this stage's event set in the ring (the oldest is folded if it is full)
+        // every PH_EVERY-th stage is timed, in an event set of the ring (the oldest is folded if
+        // it is full); the events held the stream for microseconds each
+        const bool timed = h->timing && h->ph_seen++ % mdx_handle::PH_EVERY == 0;
+        h->ph_skip = h->timing && !timed;
+        if (timed) {
             if (h->ph_count == mdx_handle::PHQ) phase_account(h, true);
             h->ph_cur = (h->ph_head + h->ph_count) % mdx_handle::PHQ;
         }
@@ -1324,7 +1328,8 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
         if ((rc = post_bc(h))) return rc;
         ev_record(h, 4);
         if (stats) stats->stage[s] = st;
-        if (h->timing) {
+        h->ph_skip = false;
+        if (timed) {
             h->ph_count += 1;
             h->ph_cur = mdx_handle::PHQ;   // later phase events outside a step go to the scratch set
             phase_account(h, false);

@@ -151,7 +151,9 @@ struct mdx_handle {
     // per-stage phase events (0 stage start, 1 Poisson start, 2 projection start, 3 projection end,
     // 4 stage end) in a ring of PHQ stages, added to `times` once complete (no stall per stage);
     // set PHQ is scratch for phase events recorded outside mdx_step
-    static constexpr int PHQ = 8;
+    static constexpr int PHQ = 8, PH_EVERY = 4;
+    long long ph_seen = 0;   // stages stepped so far
+    bool ph_skip = false;    // this stage is not timed
     hipEvent_t ev_ph[PHQ + 1][5] = {};
     int ph_head = 0, ph_count = 0, ph_cur = PHQ;
     mdx_phase_times times{};
