@@ -210,7 +210,7 @@ int mdx_net_replay_reserve(mdx_net *net, int32_t capacity);
 int mdx_net_replay_count(mdx_net *net, int32_t *count);
 int mdx_net_train(mdx_net *net, int32_t steps, int32_t batch, float lr, uint64_t seed, float *mean_loss);
 int mdx_net_eval_replay(mdx_net *net, int32_t count, float *mean_loss);
-/* device time of one forward pass over replay sample 0 (HIP events around reps passes) and the
+/* device time of one forward pass over replay sample 0, or the last input (HIP events around reps passes), and the
  * MFMA flops of one pass (as the convolution kernels issue them) */
 int mdx_net_time_forward(mdx_net *net, int32_t reps, float *ms_per_forward, double *flops);
 /* attach (or detach with net = NULL) a network to a solver handle: cfg.guess = MDX_GUESS_LEARNED

@@ -58,23 +58,34 @@ __global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__
         }
         return a;
     };
-    // operands are software-pipelined one K chunk ahead (the global fetches of chunk kc+1 overlap
-    // the MFMAs of chunk kc); the MFMA sequence is unchanged
-    bf16x8 anext = load_a(0), bnext[NT];
+    // operands are software-pipelined PD K chunks ahead in a ring of registers (a chunk holds only
+    // NT MFMAs, too little work to cover one global fetch); the MFMA sequence is unchanged
+    constexpr int PD = NT >= 4 ? 2 : 4;
+    bf16x8 aq[PD], bq[PD][NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) bnext[nt] = wp[(long)nt * 64 + lane];
-    for (int kc = 0; kc < KC; ++kc) {
-        const bf16x8 a = anext;
-        bf16x8 b[NT];
+    for (int p = 0; p < PD; ++p) {
+        aq[p] = p < KC ? load_a(p) : zero8;
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) b[nt] = bnext[nt];
-        if (kc + 1 < KC) {
-            anext = load_a(kc + 1);
+        for (int nt = 0; nt < NT; ++nt) bq[p][nt] = p < KC ? wp[((long)p * NT + nt) * 64 + lane] : zero8;
+    }
+    for (int kc0 = 0; kc0 < KC; kc0 += PD) {
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) bnext[nt] = wp[((long)(kc + 1) * NT + nt) * 64 + lane];
+        for (int p = 0; p < PD; ++p) {
+            const int kc = kc0 + p;
+            if (kc >= KC) break;
+            const bf16x8 a = aq[p];
+            bf16x8 b[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) b[nt] = bq[p][nt];
+            if (kc + PD < KC) {
+                aq[p] = load_a(kc + PD);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) bq[p][nt] = wp[((long)(kc + PD) * NT + nt) * 64 + lane];
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[nt], 0, 0, 0);
         }
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[nt], 0, 0, 0);
     }
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -132,29 +143,39 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[m][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 zero8 = {};
-    // weight fragments are software-pipelined one K chunk ahead: with few output tiles per wave
-    // (COUT 16: one) the global fetch of chunk kc+1 overlaps the MFMAs of chunk kc
-    bf16x8 bnext[NT];
+    // weight fragments are software-pipelined PD K chunks ahead in a ring of registers: with few
+    // output tiles per wave (COUT 16: one) a chunk's MFMAs cannot cover one global fetch; the
+    // unrolled group also issues the LDS reads of PD chunks together.  MFMA order unchanged.
+    constexpr int PD = NT >= 4 ? 2 : 4;
+    bf16x8 bq[PD][NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) bnext[nt] = wp[(long)nt * 64 + lane];
-    for (int kc = 0; kc < KC; ++kc) {
-        const int k0 = kc * 32 + g * 8;
-        const bool valid = k0 < K;
-        const int tap = valid ? k0 / CIN : 0, c0 = k0 - tap * CIN;
-        const int base = ((tap / 9) * CHY + wave + (tap / 3) % 3) * CHX + r + tap % 3;
-        bf16x8 b[NT];
+    for (int p = 0; p < PD; ++p)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) b[nt] = bnext[nt];
-        if (kc + 1 < KC) {
+        for (int nt = 0; nt < NT; ++nt) bq[p][nt] = p < KC ? wp[((long)p * NT + nt) * 64 + lane] : zero8;
+    for (int kc0 = 0; kc0 < KC; kc0 += PD) {
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) bnext[nt] = wp[((long)(kc + 1) * NT + nt) * 64 + lane];
-        }
+        for (int p = 0; p < PD; ++p) {
+            const int kc = kc0 + p;
+            if (kc >= KC) break;
+            const int k0 = kc * 32 + g * 8;
+            const bool valid = k0 < K;
+            const int tap = valid ? k0 / CIN : 0, c0 = k0 - tap * CIN;
+            const int base = ((tap / 9) * CHY + wave + (tap / 3) % 3) * CHX + r + tap % 3;
+            bf16x8 b[NT];
 #pragma unroll
-        for (int m = 0; m < TZ; ++m) {
-            const bf16x8 a = valid ? *reinterpret_cast<const bf16x8 *>(&tile[(base + m * CHX * CHY) * CS + c0]) : zero8;
+            for (int nt = 0; nt < NT; ++nt) b[nt] = bq[p][nt];
+            if (kc + PD < KC) {
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[m][nt], 0, 0, 0);
+                for (int nt = 0; nt < NT; ++nt) bq[p][nt] = wp[((long)(kc + PD) * NT + nt) * 64 + lane];
+            }
+#pragma unroll
+            for (int m = 0; m < TZ; ++m) {
+                const bf16x8 a =
+                    valid ? *reinterpret_cast<const bf16x8 *>(&tile[(base + m * CHX * CHY) * CS + c0]) : zero8;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[m][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[m][nt], 0, 0, 0);
+            }
         }
     }
     const int y = y0 + wave;

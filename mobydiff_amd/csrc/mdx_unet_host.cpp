@@ -259,15 +259,15 @@ extern "C" int mdx_net_eval_replay(mdx_net *n, int32_t count, float *mean_loss) 
 }
 
 // Device time of one forward pass (HIP events on the network's stream around `reps` passes over
-// replay sample 0, after one untimed pass) and the MFMA work of one pass: 2 * voxels * 27 * cin *
+// replay sample 0, or the last input without a replay buffer, after one untimed pass) and the MFMA work of one pass: 2 * voxels * 27 * cin *
 // cout per convolution as the kernels issue it (the 2-channel input padded to 16), plus the head.
 extern "C" int mdx_net_time_forward(mdx_net *n, int32_t reps, float *ms_per_forward, double *flops) {
-    NET_CHECK(n && ms_per_forward && flops && reps > 0 && n->count > 0, "mdx_net_time_forward: bad argument");
+    NET_CHECK(n && ms_per_forward && flops && reps > 0, "mdx_net_time_forward: bad argument");
     const double v0 = (double)n->V[0], v1 = (double)n->V[1], v2 = (double)n->V[2];
     const double convs = v0 * (16 * 16 + 16 * 16 + 48 * 16 + 16 * 16) + v1 * (16 * 32 + 32 * 32 + 96 * 32 + 32 * 32) +
                          v2 * (32 * 64 + 64 * 64);
     *flops = 2.0 * 27.0 * convs + 2.0 * v0 * 16.0;
-    MDX_HIP(net_load_replay(n, 0));
+    if (n->count > 0) MDX_HIP(net_load_replay(n, 0));   // else the input of the last forward pass
     MDX_HIP(net_forward(n));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MDX_HIP(hipEventCreate(&e0));
