@@ -409,6 +409,52 @@ def test_lds_fp32_batches_equal_streaming_at_production_shape(monkeypatch):
         assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
+@pytest.mark.parametrize("form", ["halves", "columns", "pairs"])
+def test_lds_kernel_forms_equal_streaming_at_production_shape(form, monkeypatch):
+    """Each thread layout of the LDS-resident kernel (half a z-column per thread, a whole column,
+    x-pairs with an LDS coefficient table) at 192 x 128 x 96, mixed precision, a fixed 40
+    iterations, including the kernel's defect epilogue: bit-identical to the streaming sweeps, and
+    the plan reports the form asked for."""
+    g, coef = _wavy(192, 128, 96)
+    rng = np.random.default_rng(25)
+    f = O.random_fields(g, rng, amp=0.05)
+    outs, infos = [], []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("MOBYDIFF_LDS_SOR", mode)
+        monkeypatch.setenv("MOBYDIFF_LDS_FORM", form)
+        ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-300, max_iter=40, check_every=20,
+                      precision=M.PREC_MIXED, inner_iters=20)
+        st = ch.pressure_projection(0.02)
+        assert st.iterations == 40
+        outs.append(ch.download(("un", "vn", "wn", "pn")))
+        infos.append(ch.info())
+        ch.close()
+    want = {"halves": "(half a z-column per thread)", "columns": "(a z-column per thread)", "pairs": "(x-pairs)"}[form]
+    assert want in infos[0], infos[0]
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_lds_kernel_cooperative_launch_equals_ordinary(monkeypatch):
+    """The LDS-resident kernel launched cooperatively (MOBYDIFF_LDS_COOP=1) computes exactly what
+    the ordinary launch of the same co-resident grid computes."""
+    g, coef = _wavy(96, 64, 48)
+    rng = np.random.default_rng(26)
+    f = O.random_fields(g, rng, amp=0.05)
+    outs = []
+    for coop in ("1", "0"):
+        monkeypatch.setenv("MOBYDIFF_LDS_COOP", coop)
+        ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=20,
+                      precision=M.PREC_MIXED, inner_iters=20)
+        st = ch.pressure_projection(0.02)
+        assert st.lds_sweeps == 1 and st.converged == 1
+        outs.append((st.iterations, ch.download(("un", "vn", "wn", "pn"))))
+        ch.close()
+    assert outs[0][0] == outs[1][0]
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(outs[0][1][k], outs[1][1][k]), k
+
+
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
 def test_lds_sweeps_equal_streaming_sweeps(precision, monkeypatch):
     g, coef = _wavy(48, 32, 24)
