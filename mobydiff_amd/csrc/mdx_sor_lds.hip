@@ -881,8 +881,6 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
                 if (nz % pz || (nz / pz) % 2) continue;
                 const int bx = nx / px, by = ny / py, bz = nz / pz;
                 const int nwg = px * py * pz;
-                const size_t lds = rb_lds_bytes(bx, by, bz, ts, h->ibm_on);
-                if (lds > lds_max) continue;
                 // the fixed LDS row and plane; packed per-cell words: LDS index (dummy cell and its
                 // padding included) < 2^16, coefficient-table index < 2^12
                 if (bx + 2 * RB_H > RB_TXP || by + 2 * RB_H > RB_TYP) continue;
@@ -893,6 +891,9 @@ bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why) {
                 if (colfit && h->rb_form == 0 && bz <= 4 * RB_ZC2 - 4) maxp = RB_COLS2;
                 if (colfit && h->rb_form == 2 && bz <= 2 * RB_ZC1 - 2) maxp = RB_COLS;
                 if (!maxp) continue;
+                // the column forms keep the face coefficients in registers: no LDS table
+                const size_t lds = rb_lds_bytes(bx, by, bz, ts, h->ibm_on && maxp > 0);
+                if (lds > lds_max) continue;
                 if (rb_ring_total(bx, by, bz) > RB_MAXR * RB_THREADS) continue;
                 const int total = rb_xchg_total(bx, by, bz);
                 if (total > RB_HE * RB_THREADS) continue;
