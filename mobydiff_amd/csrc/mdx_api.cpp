@@ -881,6 +881,9 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
     return MDX_OK;
 }
 
+// pinned words of the mixed solve's two check slots: [CHECK_PIN + 4 * slot] = max, error word, seq
+static constexpr int CHECK_PIN = 16;
+
 // Wait for the residual check that posts sequence number `seq` to the pinned words [w, w + 3):
 // [max, error word, seq].  The host polls the sequence word (no event between the kernels); now
 // and then it queries the stream, so a stream that finished or failed without posting ends the
@@ -924,9 +927,11 @@ static int wait_check(mdx_handle *h, int w, unsigned long long seq, double *r) {
 // streaming sweeps a wasted batch costs its full time: it is queued early at the first check of a
 // solve, and later only when the residuals seen so far predict (geometric decay) that check k is
 // still at least SPEC_MARGIN times above the tolerance.
-static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
+// first_seq != 0: the first check is already queued in slot 0 with that sequence number (the fused
+// start kernel)
+static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
-    constexpr int PIN = 16;   // pinned words of the two check slots: [PIN + 4 * slot]
+    constexpr int PIN = CHECK_PIN;   // pinned words of the two check slots: [PIN + 4 * slot]
     const mdx_config &c = h->cfg;
     const bool lds = lds_for(h, true);
     double r = 0.0, r1 = -1.0, r2 = -1.0;   // r_k, r_{k-1}, r_{k-2}
@@ -959,7 +964,8 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st) {
         return sweep_batch(h, true, m);
     };
     int rc;
-    if ((rc = enqueue_check(slot, true))) return rc;
+    if (first_seq) seq[slot] = first_seq;
+    else if ((rc = enqueue_check(slot, true))) return rc;
     for (;;) {
         const int m = std::min(c.inner_iters, c.max_iter - it);
         // the first check of a solve is taken to be unconverged (a fresh right-hand side); later
@@ -1028,30 +1034,51 @@ static void net_outcome(mdx_handle *h, int s, int iterations) {
 // In learned mode a start other than the network's is a fallback (stats.fallback).  With capture
 // on, the network input is built for every solve so the converged Phi can be stored as a training
 // pair.
-static int initial_guess(mdx_handle *h, int s, double dtg, mdx_solve_stats *st, double *r_first) {
+// The start of stage s's solve, decided before any kernel runs: `simple` when no network input is
+// formed (no learned start to evaluate this time, no capture), the start then being fac * prev
+// (prev non-null) or zero.  Updates the gate's state and the solve's guess flags.
+struct GuessPlan {
+    bool simple, learned, build_input, have_prev, use_prev;
+    const double *prev;
+    double fac;
+};
+static GuessPlan guess_plan(mdx_handle *h, int s, double dtg, mdx_solve_stats *st) {
+    GuessPlan p{};
     mdx_net *n = h->net;
-    const bool have_prev = h->hist_dtg[s] > 0.0;
-    const double *prev = have_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
-    const double fac = have_prev ? dtg / h->hist_dtg[s] : 0.0;
+    p.have_prev = h->hist_dtg[s] > 0.0;
+    p.fac = p.have_prev ? dtg / h->hist_dtg[s] : 0.0;
     const int mode = h->cfg.guess;
-    bool learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0;
+    p.learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0;
     h->net_started[s] = false;
     // gate: after the network's start has lost twice in a row for this stage, it is evaluated
     // again only after 4, 16, 64, then every NET_SKIP_MAX skipped solves (previous-Phi start then)
-    if (learned && h->net_skip[s] > 0) {
+    if (p.learned && h->net_skip[s] > 0) {
         h->net_skip[s] -= 1;
-        learned = false;
+        p.learned = false;
         st->guess_skipped = 1;
         st->fallback = 1;
     }
-    const bool use_prev = have_prev && (mode == MDX_GUESS_PHI_PREV || mode == MDX_GUESS_LEARNED);
-    const bool build_input = n && (learned || h->net_capture);
-    if (!learned && !build_input) {
+    p.use_prev = p.have_prev && (mode == MDX_GUESS_PHI_PREV || mode == MDX_GUESS_LEARNED);
+    p.build_input = n && (p.learned || h->net_capture);
+    p.simple = !p.learned && !p.build_input;
+    p.prev = p.use_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
+    return p;
+}
+
+// The initial guess after sor_prepare (rhs, Phi = 0): the previous-Phi copy of a simple start, or
+// the network's input, its start and the gate's choice.
+static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, mdx_solve_stats *st, double *r_first) {
+    mdx_net *n = h->net;
+    const bool have_prev = gp.have_prev, learned = gp.learned, use_prev = gp.use_prev;
+    const double *prev = have_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
+    const double fac = gp.fac;
+    if (gp.simple) {
         // previous-Phi start (or the plain one without a history); the solver's first residual
         // check evaluates it, so no candidate residual is formed here
         if (use_prev) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));
         return MDX_OK;
     }
+    (void)dtg;
     int rc;
     ev_record(h, 8);
     // Phi_0 in phi2 and its residual field (the network's input)
@@ -1193,11 +1220,19 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         int rc = ensure_mu(h, dtg);
         if (rc) return rc;
         if ((rc = rb_prepare(h))) return rc;
-        MDX_HIP(launch_sor_prepare(h));
         double r_first = -1.0;
         const int stage = (int)(h->proj_count % 3);
-        if ((rc = initial_guess(h, stage, dtg, &st, &r_first))) return rc;
-        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st) : sor_solve64(h, &st, r_first);
+        const GuessPlan gp = guess_plan(h, stage, dtg, &st);
+        unsigned long long first_seq = 0;
+        if (gp.simple && c.precision == MDX_PREC_MIXED) {
+            // rhs, the start and the solve's first check in one kernel
+            first_seq = ++h->chk_seq;
+            MDX_HIP(launch_sor_start(h, gp.prev, gp.fac, h->d_pinned + CHECK_PIN, first_seq));
+        } else {
+            MDX_HIP(launch_sor_prepare(h));
+            if ((rc = initial_guess(h, stage, dtg, gp, &st, &r_first))) return rc;
+        }
+        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq) : sor_solve64(h, &st, r_first);
         if (rc) return rc;
         if (st.fallback == 0 && r_first >= 0.0) st.div0_inf = r_first;
         net_outcome(h, stage, st.iterations);

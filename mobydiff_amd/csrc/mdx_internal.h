@@ -251,6 +251,9 @@ static constexpr unsigned MAX_PARTS = 1u << 17;
 hipError_t launch_sola_sweep(mdx_handle *h, int parity, double sor, double dtg);
 hipError_t launch_build_mu(mdx_handle *h, double dtg);
 hipError_t launch_sor_prepare(mdx_handle *h);
+// sor_prepare + Phi = fac * prev (prev null: 0) + the first defect check of a mixed solve, fused
+hipError_t launch_sor_start(mdx_handle *h, const double *prev, double fac, double *host_slot,
+                            unsigned long long seq);
 hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega);
 hipError_t launch_sor_residual64(mdx_handle *h, const double *phi);
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg);

@@ -196,6 +196,67 @@ __global__ __launch_bounds__(256) void sor_prepare_kernel(GridK g, double vbot, 
     phi[d.in(i, j, k)] = 0.0;
 }
 
+// The start of a mixed-precision solve in one pass (what sor_prepare, the previous-Phi copy and the
+// first defect check compute, with the same arithmetic): rhs = div u* (wall values of v), Phi =
+// fac * Phi_prev + 0.0 (or 0.0), r32 = rhs - L_mu(Phi) with every Phi operand formed the same way
+// from Phi_prev, and the block maxima of |r|.
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(256) void sor_start_kernel(GridK g, MuK<double> mu, SorConst<double> s, double vbot,
+                                                        double vtop, const double *__restrict__ us,
+                                                        const double *__restrict__ vs,
+                                                        const double *__restrict__ ws,
+                                                        const double *__restrict__ prev, double fac, double *rhs,
+                                                        double *phi, float *r32, double *out, double idx,
+                                                        double idy, double idz) {
+    const Dims d = g.d;
+    int tx_ = 0, ty_ = 0, tz_ = 0;
+    const bool ok = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);   // padding groups report 0
+    const int i = tx_ * 64 + threadIdx.x + 1;
+    const int j = ty_ * 4 + threadIdx.y + 1;
+    const int k = tz_ + 1;
+    double m = 0.0;
+    if (ok && i <= d.nx && j <= d.ny) {
+        const bool per2 = g.per[1];
+        const int ip = i == d.nx ? 1 : i + 1, kp = k == d.nz ? 1 : k + 1;
+        const int jp = (per2 && j == d.ny) ? 1 : j + 1;
+        const size_t c = d.g(i, j, k), q = d.in(i, j, k);
+        const double v0 = (!per2 && j == 1) ? vbot : vs[c];
+        const double v1 = (!per2 && j == d.ny) ? vtop : vs[d.g(i, jp, k)];
+        const double dvy = STRETCH ? (v1 - v0) * g.ihc[j] : (v1 - v0) * idy;
+        const double rc = (us[d.g(ip, j, k)] - us[c]) * idx + dvy + (ws[d.g(i, j, kp)] - ws[c]) * idz;
+        struct StartPhi {
+            const double *p;
+            double f;
+            Dims d;
+            __device__ double operator()(int a, int b, int e) const { return (p ? f * p[d.in(a, b, e)] : 0.0) + 0.0; }
+        } P{prev, fac, d};
+        double denom;
+        const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rc, i, j, k, denom);
+        rhs[q] = rc;
+        phi[q] = P(i, j, k);
+        r32[q] = (float)r;
+        m = fabs(r);
+    }
+    block_max_part(m, out);
+}
+
+hipError_t launch_sor_start(mdx_handle *h, const double *prev, double fac, double *host_slot,
+                            unsigned long long seq) {
+    GridK g = grid_k(h);
+    MuK<double> mu = mu_k<double>(h, h->mu64);
+    SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
+    const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
+#define MDX_ST(S, I)                                                                                     \
+    hipLaunchKernelGGL((sor_start_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, vb, vt, h->us, h->vs, h->ws, \
+                       prev, fac, h->rhs, h->phi, h->r32, h->max_part, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz)
+    if (h->stretched) { if (h->ibm_on) MDX_ST(true, true); else MDX_ST(true, false); }
+    else { if (h->ibm_on) MDX_ST(false, true); else MDX_ST(false, false); }
+#undef MDX_ST
+    hipError_t err = hipGetLastError();
+    return err != hipSuccess ? err : launch_max_final(h, grd.x, host_slot, seq);
+}
+
 // un, vn, wn on the ghost shell: the values of u*, with the top wall value of v (walled y) on the
 // interior of the j = ny + 1 plane; the boundary conditions that follow the projection overwrite
 // the faces, and edges and corners keep the u* values
