@@ -588,14 +588,47 @@ __global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, cons
     double a = 0.0, b = 0.0;
     // workgroup b sums the x rows r = b, b + SUM_BLOCKS, ... (r = (j-1) + ny (k-1)), threads along x
     const int nrows = d.ny * d.nz;
-    for (int r = blockIdx.x; r < nrows; r += SUM_BLOCKS) {
-        const int j = r % d.ny + 1, k = r / d.ny + 1;
-        const double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
-        const size_t row = d.g(0, j, k);
-        for (int i = threadIdx.x + 1; i <= d.nx; i += SUM_THREADS) {
-            const size_t c = row + (size_t)i;
-            a += u[c] * hj;
-            b += (cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0) * hj;
+    auto term_b = [&](size_t c, double hj) { return (cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0) * hj; };
+    if (d.nx <= SUM_THREADS) {
+        // one value per row and thread: four rows' loads are issued together, then added in row
+        // order (the same sums as the loop below)
+        const int i = threadIdx.x + 1;
+        if (i <= d.nx) {
+            int r = blockIdx.x;
+            for (; r + 3 * SUM_BLOCKS < nrows; r += 4 * SUM_BLOCKS) {
+                double uv[4], bv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rr = r + q * SUM_BLOCKS, j = rr % d.ny + 1, k = rr / d.ny + 1;
+                    const double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
+                    const size_t c = d.g(0, j, k) + (size_t)i;
+                    uv[q] = u[c] * hj;
+                    bv[q] = term_b(c, hj);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    a += uv[q];
+                    b += bv[q];
+                }
+            }
+            for (; r < nrows; r += SUM_BLOCKS) {
+                const int j = r % d.ny + 1, k = r / d.ny + 1;
+                const double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
+                const size_t c = d.g(0, j, k) + (size_t)i;
+                a += u[c] * hj;
+                b += term_b(c, hj);
+            }
+        }
+    } else {
+        for (int r = blockIdx.x; r < nrows; r += SUM_BLOCKS) {
+            const int j = r % d.ny + 1, k = r / d.ny + 1;
+            const double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
+            const size_t row = d.g(0, j, k);
+            for (int i = threadIdx.x + 1; i <= d.nx; i += SUM_THREADS) {
+                const size_t c = row + (size_t)i;
+                a += u[c] * hj;
+                b += term_b(c, hj);
+            }
         }
     }
     s1[threadIdx.x] = a;
