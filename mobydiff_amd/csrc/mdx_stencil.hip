@@ -163,11 +163,14 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
                                 idz, idx2, idy2, idz2, ire);
 }
 
-// LDS-tiled momentum: a 64 x 8 workgroup owns a 64 x 8 column of cells and marches MKZ planes in z.
+// LDS-tiled momentum: a 64 x TY workgroup owns a 64 x TY column of cells and marches MKZ planes in z.
 // The u, v, w planes k-1, k, k+1 (with a one-cell x-y halo) and the p plane k live in LDS as a
 // rolling window, so each value is fetched from global memory about 1.3 (x-y halo) x (MKZ+2)/MKZ
 // (z halo) times instead of once per stencil use (about 30 loads per cell in the direct kernel).
-static constexpr int MTX = 64, MTY = 8, MTW = MTX + 2, MTH = MTY + 2, MPL = MTW * MTH, MKZ = 8;
+// TY = 8 rows per workgroup, or 4 on grids with fewer than 2048 such workgroups (measured: 54 against
+// 61 us at 192 x 128 x 96, where 576 workgroups of 512 threads leave a thin last wave; 304 against
+// 312 us at 256 x 192 x 256, where the smaller halo of 8 rows wins)
+static constexpr int MTX = 64, MTW = MTX + 2, MKZ = 8;
 
 struct MomTile {
     const double *su, *sv, *sw, *sp;   // LDS planes
@@ -180,8 +183,8 @@ struct MomTile {
     __device__ inline double p(int a, int b, int c) const { return c == 0 ? sp[t + a + MTW * b] : pkm; }
 };
 
-template <bool STRETCH, bool IBM>
-__global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
+template <bool STRETCH, bool IBM, int TY>
+__global__ __launch_bounds__(MTX * TY) void momentum_tiled_kernel(
     GridK g, const double *__restrict__ un, const double *__restrict__ vn,
     const double *__restrict__ wn, const double *__restrict__ pn, double *__restrict__ us,
     double *__restrict__ vs, double *__restrict__ ws, double *__restrict__ oru,
@@ -189,12 +192,13 @@ __global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
     const double *__restrict__ cv, const double *__restrict__ cw, int coef_kstride, double dta, double dtb,
     double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
     double ire) {
+    constexpr int MPL = MTW * (TY + 2);   // window plane: the tile and its x-y halo
     __shared__ double su[3 * MPL], sv[3 * MPL], sw[3 * MPL], sp[MPL];
     const Dims d = g.d;
     const int nseg = (d.nz + MKZ - 1) / MKZ;
     int tx, ty, tz;
-    if (!xcd_tile(d.nx, d.ny, nseg, MTX, MTY, tx, ty, tz)) return;   // uniform per workgroup
-    const int i0 = tx * MTX + 1, j0 = ty * MTY + 1, k0 = tz * MKZ + 1;
+    if (!xcd_tile(d.nx, d.ny, nseg, MTX, TY, tx, ty, tz)) return;   // uniform per workgroup
+    const int i0 = tx * MTX + 1, j0 = ty * TY + 1, k0 = tz * MKZ + 1;
     const int k1 = k0 + MKZ - 1 < d.nz ? k0 + MKZ - 1 : d.nz;
     const int tid = threadIdx.x + MTX * threadIdx.y;
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
     const int t = (threadIdx.x + 1) + MTW * (threadIdx.y + 1);
     // plane k of u, v, w into window slot `slot`; halo cells beyond the ghost layer are not read
     auto load_uvw = [&](int slot, int k) {
-        for (int q = tid; q < MPL; q += MTX * MTY) {
+        for (int q = tid; q < MPL; q += MTX * TY) {
             const int gi = i0 - 1 + q % MTW, gj = j0 - 1 + q / MTW;
             if (gi <= d.nx + 1 && gj <= d.ny + 1) {
                 const size_t x = d.g(gi, gj, k);
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(MTX * MTY) void momentum_tiled_kernel(
     load_uvw(0, k0 - 1);
     load_uvw(1, k0);
     load_uvw(2, k0 + 1);
-    constexpr int NT_ = MTX * MTY, PER = (MPL + NT_ - 1) / NT_;   // window values per thread
+    constexpr int NT_ = MTX * TY, PER = (MPL + NT_ - 1) / NT_;   // window values per thread
     for (int q = tid; q < MPL; q += NT_) {
         const int gi = i0 - 1 + q % MTW, gj = j0 - 1 + q / MTW;
         if (gi <= d.nx + 1 && gj <= d.ny + 1) sp[q] = pn[d.g(gi, gj, k0)];
@@ -279,11 +283,15 @@ hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
     const double dx = h->dx, dy = h->dy, dz = h->dz;
     const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
     const bool tiled = h->tiled_momentum;
-    dim3 blk = tiled ? dim3(MTX, MTY, 1) : dim3(BX, BY, 1);
-    dim3 grd(tiled ? xcd_grid(h->d.nx, h->d.ny, (h->d.nz + MKZ - 1) / MKZ, MTX, MTY)
-                   : xcd_grid(h->d.nx, h->d.ny, h->d.nz, BX, BY));
+    const int nseg = (h->d.nz + MKZ - 1) / MKZ;
+    const long wg8 = (long)((h->d.nx + MTX - 1) / MTX) * ((h->d.ny + 7) / 8) * nseg;
+    const int ty = wg8 < 2048 ? 4 : 8;
+    dim3 blk = tiled ? dim3(MTX, ty, 1) : dim3(BX, BY, 1);
+    dim3 grd(tiled ? xcd_grid(h->d.nx, h->d.ny, nseg, MTX, ty) : xcd_grid(h->d.nx, h->d.ny, h->d.nz, BX, BY));
 #define MDX_MOM(S, I)                                                                        \
-    hipLaunchKernelGGL((tiled ? momentum_tiled_kernel<S, I> : momentum_kernel<S, I>), grd, blk, 0, h->stream, \
+    hipLaunchKernelGGL((tiled ? (ty == 4 ? momentum_tiled_kernel<S, I, 4> : momentum_tiled_kernel<S, I, 8>)      \
+                              : momentum_kernel<S, I>),                                                        \
+                       grd, blk, 0, h->stream,                                                               \
                        g, h->un, h->vn, h->wn,                                                   \
                        h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw, h->cu, h->cv, h->cw,  \
                        h->coef_plane ? 0 : 1, dta, dtb, dtg, 1.0 / dx, 1.0 / dy, 1.0 / dz, 1.0 / dx2, 1.0 / dy2,      \
