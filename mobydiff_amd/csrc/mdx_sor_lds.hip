@@ -3,8 +3,9 @@
 // The streaming sweeps (mdx_sor.hip) read and write the whole grid once per colour; on the wavy
 // channel (2.4 M cells) that is a few microseconds of HBM/MALL traffic, but every launch also pays
 // dispatch latency.  Here every workgroup owns one subdomain for a whole batch of iterations: its
-// Phi tile lives in LDS with a halo two cells deep, its right-hand side in registers, and its face
-// coefficients in an LDS table (the IBM geometry is z-invariant, so one x-y plane of mu suffices).
+// Phi tile lives in LDS with a halo two cells deep, and its right-hand side and face coefficients
+// (the IBM geometry is z-invariant, so one x-y plane of mu suffices) in registers of the threads
+// that own its z-columns (the pair form keeps the coefficients in an LDS table).
 //
 // Temporal blocking: an iteration relaxes colour 0 on the subdomain AND on the one-cell ring of
 // face cells around it (recomputing what the neighbours compute for those cells, from the same old
