@@ -86,7 +86,7 @@ def roofline(times, kind, info=""):
     lds = info.startswith("LDS-resident") and (kind == "fp32" or "; fp64 " in info)
     if lds:
         kz = "lds"
-        label = f"LDS-resident red-black SOR batch ({kind}, {iters:g} iterations per launch)"
+        label = f"LDS-resident red-black SOR batch ({kind}, {iters:.1f} iterations per launch on average)"
     elif iters > 0.75:
         kz = "fused"
         label = f"red-black SOR iteration ({kind}, both colours in one launch)"

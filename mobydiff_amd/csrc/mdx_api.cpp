@@ -316,6 +316,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     }
     if (const char *lc = std::getenv("MOBYDIFF_LDS_COOP")) h->rb_coop = lc[0] == '1';
     if (const char *nm = std::getenv("MOBYDIFF_NET_MARGIN")) h->net_margin = std::atof(nm);
+    if (const char *mm = std::getenv("MOBYDIFF_MERGE_MAX")) h->merge_max = std::max(1, std::atoi(mm));
     if (const char *lf = std::getenv("MOBYDIFF_LDS_FORM")) {
         if (std::strcmp(lf, "pairs") == 0) h->rb_form = 1;
         else if (std::strcmp(lf, "columns") == 0) h->rb_form = 2;
@@ -342,6 +343,9 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if ((e = hipHostMalloc((void **)&h->h_pinned, 64 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent)) !=
         hipSuccess)
         return fail(e, "pinned buffer");
+    // the check slots' sequence words must not hold a number this handle will wait for: a block
+    // freed by an earlier handle can come back with its last sequence numbers in it
+    std::memset(h->h_pinned, 0, 64 * sizeof(double));
     if ((e = hipHostGetDevicePointer((void **)&h->d_pinned, h->h_pinned, 0)) != hipSuccess)
         return fail(e, "pinned buffer device address");
 
@@ -814,7 +818,7 @@ static int sweep_batch(mdx_handle *h, bool fp32, int m, const RbDefect *df = nul
         }
     }
     if (lds_for(h, fp32)) {
-    } else if (h->use_graphs && m >= 2) {
+    } else if (h->use_graphs && m >= 2 && m == (fp32 ? h->cfg.inner_iters : h->cfg.check_every)) {
         const void *iterate = fp32 ? (const void *)h->e32 : (const void *)h->phi;
         if (!slot.exec || slot.m != m || slot.ibm != h->ibm_on || slot.iterate != iterate) {
             if (slot.exec) { (void)hipGraphExecDestroy(slot.exec); slot.exec = nullptr; }
@@ -931,6 +935,7 @@ static int wait_check(mdx_handle *h, int w, unsigned long long seq, double *r) {
 // start kernel)
 static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
+    const int MERGE_MAX = h->merge_max;   // at most this many inner_iters batches run as one
     constexpr int PIN = CHECK_PIN;   // pinned words of the two check slots: [PIN + 4 * slot]
     const mdx_config &c = h->cfg;
     const bool lds = lds_for(h, true);
@@ -966,8 +971,23 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long lon
     int rc;
     if (first_seq) seq[slot] = first_seq;
     else if ((rc = enqueue_check(slot, true))) return rc;
+    int m_a = 0, m_b = 0;   // iterations between the checks of r2 and r1, and of r1 and the pending one
     for (;;) {
-        const int m = std::min(c.inner_iters, c.max_iter - it);
+        // Batch size: inner_iters, or two or three times that while the residual decay seen so far
+        // (per iteration, from r2 to r1) predicts that many more checks would still fail after the
+        // pending one: checks that cannot stop the solve are skipped, saving their launches and
+        // FP64 corrections (the FP32 inner solve then simply runs longer on the same defect
+        // equation).  The decision uses only residuals, so every path takes the same batches.
+        int m = c.inner_iters;
+        if (r1 > 0.0 && r2 > 0.0 && r1 < r2 && m_a > 0) {
+            const double q = std::pow(r1 / r2, 1.0 / m_a);   // decay per iteration
+            const double rp = r1 * std::pow(q, m_b);           // predicted pending residual
+            if (rp > c.tol) {
+                const int blocks = (int)(std::log(c.tol / rp) / std::log(q) / c.inner_iters);
+                if (blocks >= 3 && MERGE_MAX > 1) m = c.inner_iters * std::min(MERGE_MAX, blocks - 1);
+            }
+        }
+        m = std::min(m, c.max_iter - it);
         // the first check of a solve is taken to be unconverged (a fresh right-hand side); later
         // ones are predicted from the residuals seen so far
         bool spec = m > 0 && (lds || checks == 0);
@@ -993,6 +1013,8 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long lon
         it += m;
         r2 = r1;
         r1 = r;
+        m_a = m_b;
+        m_b = m;
         if (!lds && (rc = enqueue_check(slot ^ 1, false))) return rc;
         slot ^= 1;
     }

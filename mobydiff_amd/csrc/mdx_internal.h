@@ -221,6 +221,9 @@ struct mdx_handle {
     // the network's start is taken when its residual max-norm is below net_margin times the other
     // start's (MOBYDIFF_NET_MARGIN)
     double net_margin = 0.8;
+    // mixed solve: at most this many inner_iters batches run as one while the checks between them
+    // are predicted to fail (MOBYDIFF_MERGE_MAX; 1 keeps every check)
+    int merge_max = 3;
     bool net_started[3] = {false, false, false};
     int base_its[3] = {0, 0, 0};
     long proj_count = 0;
