@@ -61,14 +61,23 @@ def run_steps(ch, k):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep_traffic_wavy_192x128x96.json")
 
 
-def measured_traffic(kind, kz):
+def measured_traffic(kind, kz, iters=None):
+    """Measured HBM bytes per dispatch.  An LDS-resident dispatch was measured at a fixed batch size
+    (`iterations_per_dispatch`, every check kept) and is scaled to `iters`, the timed region's mean
+    iterations per launch (merged batches make launches of 20 to 60 iterations)."""
     try:
         with open(TRAFFIC_FILE) as fh:
             table = json.load(fh)
     except (OSError, ValueError):
         return None
     entry = table.get("kernels", {}).get(f"{kind}_{kz}" if kz in ("fused", "lds") else f"{kind}_kz{kz}")
-    return None if entry is None else entry.get("bytes_per_dispatch")
+    if entry is None:
+        return None
+    b = entry.get("bytes_per_dispatch")
+    per = entry.get("iterations_per_dispatch")
+    if b is not None and per and iters:
+        b = round(b * iters / per)
+    return b
 
 
 def roofline(times, kind, info=""):
@@ -95,7 +104,7 @@ def roofline(times, kind, info=""):
         label = f"red-black SOR sweep ({kind}, one colour)"
     return {"kernel": label, "bound": "hbm", "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(kind, kz), "avg_launch_us": round(avg_s * 1e6, 2),
+            "traffic": measured_traffic(kind, kz, iters if kz == "lds" else None), "avg_launch_us": round(avg_s * 1e6, 2),
             "bytes_per_launch": bytes_per_launch}
 
 
