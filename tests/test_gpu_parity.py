@@ -455,6 +455,33 @@ def test_lds_kernel_cooperative_launch_equals_ordinary(monkeypatch):
         assert np.array_equal(outs[0][1][k], outs[1][1][k]), k
 
 
+@pytest.mark.parametrize("lds", ["1", "0"])
+def test_merged_batches_skip_only_failing_checks(lds, monkeypatch):
+    """Mixed SOR with merged batches (the default, up to three inner batches while the residual decay
+    predicts their checks would fail) against every check kept (MOBYDIFF_MERGE_MAX=1): both reach
+    the tolerance, the merged solve needs no more iterations than one extra inner batch and far
+    fewer checks, and its fields stay within the tolerance's reach of the unmerged ones."""
+    monkeypatch.setenv("MOBYDIFF_LDS_SOR", lds)
+    g, coef = _wavy(96, 64, 48)
+    rng = np.random.default_rng(27)
+    f = O.random_fields(g, rng, amp=0.05)
+    res = []
+    for mm in ("1", "3"):
+        monkeypatch.setenv("MOBYDIFF_MERGE_MAX", mm)
+        ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=20,
+                      precision=M.PREC_MIXED, inner_iters=20)
+        st = ch.pressure_projection(0.02)
+        res.append((st, ch.download(("un", "vn", "wn"))))
+        ch.close()
+    (s1, f1), (s3, f3) = res
+    assert s1.converged == 1 and s3.converged == 1
+    assert s1.div_inf < 1e-8 and s3.div_inf < 1e-8
+    assert s1.iterations <= s3.iterations <= s1.iterations + 20
+    assert s3.checks < 0.6 * s1.checks, (s1.checks, s3.checks)
+    for k in ("un", "vn", "wn"):
+        assert np.max(np.abs(f1[k] - f3[k])) < 1e-6, k
+
+
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
 def test_lds_sweeps_equal_streaming_sweeps(precision, monkeypatch):
     g, coef = _wavy(48, 32, 24)
