@@ -176,6 +176,9 @@ def main():
     ap.add_argument("--inner", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=200)
     ap.add_argument("--no-learned", action="store_true", help="headline without the learned guess")
+    ap.add_argument("--net-weights", default="",
+                    help="U-Net parameters written by mobydiff_amd.train_guess (.npz) instead of training "
+                         "on the warm-up steps")
     ap.add_argument("--flat-steps", type=int, default=10, help="steps of the flat FFT production line (0: skip)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once (the known-size kernel that "
@@ -205,7 +208,18 @@ def main():
     learned = not args.no_learned
     training = None
     net = None
-    if learned:
+    pretrained = learned and bool(args.net_weights)
+    if pretrained:
+        # parameters trained beforehand on other wavy channels (mobydiff_amd.train_guess)
+        from mobydiff_amd import train_guess
+        net = M.Net(NX, NY, NZ, seed=0)
+        meta = train_guess.load(args.net_weights, net)
+        training = dict(meta, weights=os.path.basename(args.net_weights),
+                        source="(div u*, previous-step Phi) -> Phi pairs from the mixed SOR solves of "
+                               f"{meta['runs']} wavy channels (A uniform in {meta['A']}, seeds "
+                               f"{meta['seeds'][0]}..{meta['seeds'][1]}), trained on the GPU by this library")
+        head.attach_net(net, capture=False)
+    elif learned:
         # the network learns from the solver's own converged solves during the warm-up steps
         net = M.Net(NX, NY, NZ, seed=0)
         net.replay_reserve(3 * args.warmup)
@@ -213,7 +227,9 @@ def main():
     for _ in range(args.warmup):
         head.step(DT)
     state = head.download()
-    if learned:
+    if pretrained:
+        head.set_guess(M.GUESS_LEARNED)
+    elif learned:
         t0 = time.perf_counter()
         pairs = net.replay_count()
         l0 = net.eval_replay(pairs)

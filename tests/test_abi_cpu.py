@@ -70,3 +70,33 @@ def test_null_arguments():
     assert M.lib().mdx_create(None, None) == 1
     assert M.lib().mdx_momentum(None, M.C.c_double(0), M.C.c_double(0), M.C.c_double(0)) == 1
     assert M.lib().mdx_destroy(None) == 0
+
+
+def test_guess_weights_round_trip(tmp_path):
+    """mobydiff_amd.train_guess writes plain arrays (no pickled objects) and load() restores the
+    parameters and c0 exactly, refusing a parameter count that does not match the network."""
+    import numpy as np
+    from mobydiff_amd import train_guess
+
+    class FakeNet:
+        def __init__(self, n):
+            self.nparam, self.p, self.c0 = n, None, None
+
+        def get_params(self):
+            return np.arange(self.nparam, dtype=np.float32) * 0.5, 1.25
+
+        def set_params(self, p, c0):
+            self.p, self.c0 = p, c0
+
+    src = FakeNet(10)
+    path = str(tmp_path / "w.npz")
+    train_guess.save(path, src, {"runs": 2, "pairs": 12})
+    dst = FakeNet(10)
+    meta = train_guess.load(path, dst)
+    assert meta == {"runs": 2, "pairs": 12}
+    assert dst.c0 == 1.25 and np.array_equal(dst.p, np.arange(10, dtype=np.float32) * 0.5)
+    with np.load(path, allow_pickle=False) as z:
+        assert set(z.files) == {"params", "c0", "meta"}
+    import pytest
+    with pytest.raises(ValueError):
+        train_guess.load(path, FakeNet(11))
