@@ -108,6 +108,18 @@ def roofline(times, kind, info=""):
             "bytes_per_launch": bytes_per_launch}
 
 
+def momentum_roofline(ch, cells, ibm, dt, workload):
+    """The momentum kernel alone: device time from HIP events on the channel's stream around 20
+    launches of the second RK3 stage's predictor (mdx_time_momentum; the RK3 history is restored).
+    Canonical algorithmic traffic 128 B per cell with IBM (10 FP64 reads + 6 writes), 104 B without."""
+    ms = ch.time_momentum(50.0 / 120.0 * dt, -34.0 / 120.0 * dt, 16.0 / 120.0 * dt, reps=20)
+    b = (128.0 if ibm else 104.0) * cells
+    ach = b / (ms * 1e-3) / 1e9
+    return {"kernel": "momentum_tiled_kernel", "workload": workload, "bound": "hbm",
+            "avg_launch_us": round(ms * 1e3, 2), "bytes_per_launch": b, "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
 def flat_fft_production(M, steps, precision=None):
     """Secondary line: the flat channel at production size, 256 x 192 x 256, tanh-stretched y
     (beta 1.5), 4 pi x 2 x 2 pi, Re_b = 5600, constant flow rate, dt = 5e-4, FFT + tridiagonal
@@ -132,8 +144,9 @@ def flat_fft_production(M, steps, precision=None):
     el = time.perf_counter() - t0
     t = ch.phase_times()
     div = ch.div_norm()
-    ch.close()
     cells = nx * ny * nz
+    mom = momentum_roofline(ch, cells, False, 5e-4, "flat channel 256x192x256") if precision is None else None
+    ch.close()
     gbs = 768.0 * cells * steps / el / 1e9
     label = "FP64" if precision in (None, M.PREC_FP64) else "FP32 transforms/Thomas in FP64 defect correction to 1e-8"
     return {"workload": "flat channel 256x192x256 (beta=1.5, Re_b=5600, dt=5e-4), FFT+tridiagonal Poisson, " + label,
@@ -145,7 +158,8 @@ def flat_fft_production(M, steps, precision=None):
             "div_inf_final": div,
             "step_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(gbs / HBM_PEAK_GBS, 4),
-                              "bytes_per_step": 768.0 * cells}}
+                              "bytes_per_step": 768.0 * cells},
+            "stencil_roofline": mom}
 
 
 def cpu_baseline(f_host, coef, cores, steps):
@@ -314,6 +328,7 @@ def main():
         "speedup_vs_plain_fp64": round(v_h / v_p, 3),
         "roofline": roofline(t_h, "fp32", head.info()),
     }
+    out["stencil_roofline"] = momentum_roofline(head, cells, True, DT, "wavy channel 192x128x96 (IBM)")
     if args.flat_steps > 0:
         out["flat_fft_production"] = flat_fft_production(M, args.flat_steps)
         out["flat_fft_production_mixed"] = flat_fft_production(M, args.flat_steps, M.PREC_MIXED)

@@ -175,6 +175,11 @@ int mdx_step_batch(mdx_handle *const *hs, int32_t n, double dt, mdx_step_stats *
 /* per-phase GPU time accumulated since the last reset (cfg.timing must be set) */
 int mdx_get_phase_times(mdx_handle *h, mdx_phase_times *t);
 int mdx_reset_phase_times(mdx_handle *h);
+/* device time of the momentum kernel alone: reps launches of mdx_momentum(dta, dtb, dtg) from the
+ * current fields between HIP events on the handle's stream (after one untimed launch); the old
+ * right-hand sides are restored afterwards, the predictor fields hold the last launch's result.
+ * Allocates a temporary copy of the old right-hand sides.  No counterpart in the reference. */
+int mdx_time_momentum(mdx_handle *h, double dta, double dtb, double dtg, int32_t reps, float *ms_per_launch);
 
 /* wall-normal coordinates of cell centres and v-faces (box coordinates, length ny+2 each) */
 int mdx_get_y_coordinates(mdx_handle *h, double *yc, double *yv);

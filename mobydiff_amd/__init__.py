@@ -253,6 +253,13 @@ class Channel:
     def synchronize(self):
         _check(lib().mdx_synchronize(self.h))
 
+    def time_momentum(self, dta, dtb, dtg, reps=20):
+        """device ms per launch of the momentum kernel (mdx_time_momentum; RK3 history kept)"""
+        ms = C.c_float()
+        _check(lib().mdx_time_momentum(self.h, C.c_double(dta), C.c_double(dtb), C.c_double(dtg), C.c_int32(reps),
+                                       C.byref(ms)))
+        return ms.value
+
     def phase_times(self):
         t = PhaseTimes()
         _check(lib().mdx_get_phase_times(self.h, C.byref(t)))

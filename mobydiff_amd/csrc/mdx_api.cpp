@@ -627,6 +627,40 @@ extern "C" int mdx_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
     return MDX_OK;
 }
 
+// Device time of the momentum kernel: `reps` launches of mdx_momentum(dta, dtb, dtg) from the
+// current fields between two HIP events on the handle's stream, after one untimed launch.  The old
+// right-hand sides are saved before and restored after, so the solver's RK3 history is unchanged;
+// the predictor fields hold the last launch's result.
+extern "C" int mdx_time_momentum(mdx_handle *h, double dta, double dtb, double dtg, int32_t reps,
+                                 float *ms_per_launch) {
+    MDX_CHECK(h && ms_per_launch && reps > 0, MDX_ERR_ARG, "mdx_time_momentum: bad argument");
+    const size_t bytes = h->ni * sizeof(double);
+    double *save = nullptr;
+    MDX_HIP(hipMalloc(&save, 3 * bytes));
+    double *const old[3] = {h->oru, h->orv, h->orw};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < 3 && e == hipSuccess; ++c)
+        e = hipMemcpyAsync(save + c * h->ni, old[c], bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = launch_momentum(h, dta, dtb, dtg);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_momentum(h, dta, dtb, dtg);
+    if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+    for (int c = 0; c < 3 && e == hipSuccess; ++c)
+        e = hipMemcpyAsync(old[c], save + c * h->ni, bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(save);
+    MDX_HIP(e);
+    *ms_per_launch = ms / (float)reps;
+    return MDX_OK;
+}
+
 extern "C" int mdx_apply_bc(mdx_handle *h) {
     MDX_CHECK(h, MDX_ERR_ARG, "null handle");
     MDX_HIP(launch_apply_bc_fft(h));

@@ -500,3 +500,23 @@ def test_lds_sweeps_equal_streaming_sweeps(precision, monkeypatch):
     assert stats[0].iterations == stats[1].iterations and stats[0].converged == 1
     for k in ("un", "vn", "wn", "pn"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+@pytest.mark.gpu
+def test_time_momentum_keeps_rk3_history():
+    """mdx_time_momentum (the bench's momentum-kernel timing) restores the old right-hand sides:
+    a step after it is bit-identical to a step without it."""
+    g, coef = _wavy(48, 32, 24)
+    f = O.random_fields(g, np.random.default_rng(31), amp=0.05)
+    res = []
+    for timed in (False, True):
+        ch = _channel(g, f, coef, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8, check_every=20)
+        ch.step(5e-4)
+        if timed:
+            ms = ch.time_momentum(50 / 120 * 5e-4, -34 / 120 * 5e-4, 16 / 120 * 5e-4, reps=3)
+            assert ms > 0.0
+        ch.step(5e-4)
+        res.append(ch.download(("un", "vn", "wn", "pn")))
+        ch.close()
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(res[0][k], res[1][k]), k
