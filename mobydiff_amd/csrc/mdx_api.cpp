@@ -966,13 +966,15 @@ static int wait_check(mdx_handle *h, int w, unsigned long long seq, double *r) {
 // solve, and later only when the residuals seen so far predict (geometric decay) that check k is
 // still at least SPEC_MARGIN times above the tolerance.
 // first_seq != 0: the first check is already queued in slot 0 with that sequence number (the fused
-// start kernel)
-static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq) {
+// start kernel).  stage: the RK stage whose previous solve's iteration count serves as the merge
+// prior.
+static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq, int stage) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
     const int MERGE_MAX = h->merge_max;   // at most this many inner_iters batches run as one
     constexpr int PIN = CHECK_PIN;   // pinned words of the two check slots: [PIN + 4 * slot]
     const mdx_config &c = h->cfg;
     const bool lds = lds_for(h, true);
+    const int prior = h->mix_prev_its[stage];
     double r = 0.0, r1 = -1.0, r2 = -1.0;   // r_k, r_{k-1}, r_{k-2}
     int it = 0, checks = 0, slot = 0;
     unsigned long long seq[2] = {0, 0};     // the check each slot waits for
@@ -1011,7 +1013,9 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long lon
         // (per iteration, from r2 to r1) predicts that many more checks would still fail after the
         // pending one: checks that cannot stop the solve are skipped, saving their launches and
         // FP64 corrections (the FP32 inner solve then simply runs longer on the same defect
-        // equation).  The decision uses only residuals, so every path takes the same batches.
+        // equation).  Before two residuals have been read, the iterations of the previous solve of
+        // this stage stand in for the prediction.  The decision uses only residuals and iteration
+        // counts, so every path takes the same batches.
         int m = c.inner_iters;
         if (r1 > 0.0 && r2 > 0.0 && r1 < r2 && m_a > 0) {
             const double q = std::pow(r1 / r2, 1.0 / m_a);   // decay per iteration
@@ -1020,6 +1024,9 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long lon
                 const int blocks = (int)(std::log(c.tol / rp) / std::log(q) / c.inner_iters);
                 if (blocks >= 3 && MERGE_MAX > 1) m = c.inner_iters * std::min(MERGE_MAX, blocks - 1);
             }
+        } else if (r2 < 0.0 && prior > it && MERGE_MAX > 1) {
+            const int blocks = (prior - it) / c.inner_iters;   // after the pending check
+            if (blocks >= 3) m = c.inner_iters * std::min(MERGE_MAX, blocks - 1);
         }
         m = std::min(m, c.max_iter - it);
         // the first check of a solve is taken to be unconverged (a fresh right-hand side); later
@@ -1057,6 +1064,7 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long lon
     st->div_inf = r;
     st->converged = r < c.tol;
     st->lds_sweeps = lds;
+    h->mix_prev_its[stage] = st->converged ? it : 0;
     return MDX_OK;
 }
 
@@ -1288,7 +1296,7 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
             MDX_HIP(launch_sor_prepare(h));
             if ((rc = initial_guess(h, stage, dtg, gp, &st, &r_first))) return rc;
         }
-        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq) : sor_solve64(h, &st, r_first);
+        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage) : sor_solve64(h, &st, r_first);
         if (rc) return rc;
         if (st.fallback == 0 && r_first >= 0.0) st.div0_inf = r_first;
         net_outcome(h, stage, st.iterations);
