@@ -223,7 +223,7 @@ struct mdx_handle {
     double net_margin = 0.8;
     // mixed solve: at most this many inner_iters batches run as one while the checks between them
     // are predicted to fail (MOBYDIFF_MERGE_MAX; 1 keeps every check)
-    int merge_max = 3;
+    int merge_max = 8;
     // iterations of the last converged mixed solve of each RK stage (the merge prior; 0: none)
     int mix_prev_its[3] = {0, 0, 0};
     bool net_started[3] = {false, false, false};
