@@ -457,8 +457,8 @@ def test_lds_kernel_cooperative_launch_equals_ordinary(monkeypatch):
 
 @pytest.mark.parametrize("lds", ["1", "0"])
 def test_merged_batches_skip_only_failing_checks(lds, monkeypatch):
-    """Mixed SOR with merged batches (the default, up to three inner batches while the residual decay
-    predicts their checks would fail) against every check kept (MOBYDIFF_MERGE_MAX=1): both reach
+    """Mixed SOR with merged batches (up to three inner batches while the residual decay predicts
+    their checks would fail) against every check kept (MOBYDIFF_MERGE_MAX=1): both reach
     the tolerance, the merged solve needs no more iterations than one extra inner batch and far
     fewer checks, and its fields stay within the tolerance's reach of the unmerged ones."""
     monkeypatch.setenv("MOBYDIFF_LDS_SOR", lds)
