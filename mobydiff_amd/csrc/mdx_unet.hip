@@ -108,11 +108,13 @@ __global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__
 // memory about 2.5 times instead of once per tap (27), and each wave reuses one weight fragment
 // for TZ output rows.  Wave w owns output row y = y0 + w, its TZ M-tiles are z = z0 .. z0+TZ-1, and
 // a tile row is 16 consecutive x voxels.  A voxel's channels sit CIN + 8 elements apart in LDS so
-// the 16 lanes of a row read disjoint bank groups.  TZ = 4 (2 from 48 channels up, to stay within
-// 64 KB of LDS).  Needs W % 16 == 0, H % 4 == 0, D % TZ == 0.
+// the 16 lanes of a row read disjoint bank groups.  TZ = 4 below 48 channels, 2 at 48 and 64, 1 at
+// 96 (measured fastest of TZ 1/2/4 per layer; the halo tile in dynamic LDS: 31 KB at 16 channels,
+// 52 KB at 32, 48 KB at 48, 62 KB at 64, 67 KB at 96).  Needs W % 16 == 0, H % 4 == 0, D % TZ == 0.
 static constexpr int CTX = 16, CTY = 4, CHX = CTX + 2, CHY = CTY + 2;
 template <int CIN> struct ConvTile {
-    static constexpr int TZ = CIN >= 48 ? 2 : 4, HV = CHX * CHY * (TZ + 2), CS = CIN + 8;
+    static constexpr int TZ = CIN >= 96 ? 1 : CIN >= 48 ? 2 : 4, HV = CHX * CHY * (TZ + 2), CS = CIN + 8;
+    static constexpr size_t BYTES = (size_t)HV * CS * sizeof(bf16);
 };
 
 template <int CIN, int COUT, bool RELU>
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
                                                           int W, int H, int D) {
     constexpr int K = 27 * CIN, KC = (K + 31) / 32, NT = COUT / 16, U = CIN / 8;
     constexpr int TZ = ConvTile<CIN>::TZ, HV = ConvTile<CIN>::HV, CS = ConvTile<CIN>::CS;
-    __shared__ __attribute__((aligned(16))) bf16 tile[HV * CS];
+    extern __shared__ __attribute__((aligned(16))) bf16 tile[];   // HV * CS values
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ntx = W / CTX, nty = H / CTY;
     const int x0 = (int)(blockIdx.x % ntx) * CTX, y0 = (int)((blockIdx.x / ntx) % nty) * CTY;
@@ -571,12 +573,16 @@ static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in
     if (n->tiled_conv && cin == CI && cout == CO && relu == R && W % CTX == 0 && H % CTY == 0 &&                   \
         D % ConvTile<CI>::TZ == 0) {                                                                               \
         const unsigned tiles = (unsigned)((W / CTX) * (H / CTY) * (D / ConvTile<CI>::TZ));                          \
-        hipLaunchKernelGGL((conv3_tiled_kernel<CI, CO, R>), dim3(tiles), dim3(256), 0, n->stream, in,               \
-                           reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);                              \
+        static const hipError_t attr = ConvTile<CI>::BYTES <= 65536 ? hipSuccess :                                 \
+            hipFuncSetAttribute(reinterpret_cast<const void *>(&conv3_tiled_kernel<CI, CO, R>),                     \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)ConvTile<CI>::BYTES);              \
+        if (attr != hipSuccess) return attr;                                                                       \
+        hipLaunchKernelGGL((conv3_tiled_kernel<CI, CO, R>), dim3(tiles), dim3(256), ConvTile<CI>::BYTES,           \
+                           n->stream, in, reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);               \
         return hipGetLastError();                                                                                  \
     }
     TILED_CASE(16, 16, true) TILED_CASE(16, 32, true) TILED_CASE(32, 32, true) TILED_CASE(32, 64, true)
-    TILED_CASE(48, 16, true) TILED_CASE(64, 64, true) TILED_CASE(16, 16, false) TILED_CASE(32, 16, false)
+    TILED_CASE(48, 16, true) TILED_CASE(64, 64, true) TILED_CASE(96, 32, true) TILED_CASE(16, 16, false) TILED_CASE(32, 16, false)
     TILED_CASE(32, 32, false) TILED_CASE(32, 96, false) TILED_CASE(16, 48, false) TILED_CASE(64, 32, false)
     TILED_CASE(64, 64, false)
 #undef TILED_CASE
