@@ -228,3 +228,19 @@ def test_previous_phi_guess_in_the_solver():
         ch.close()
     print("sweeps per solve: plain start", its[M.GUESS_PREVIOUS], "previous Phi", its[M.GUESS_PHI_PREV])
     assert sum(its[M.GUESS_PHI_PREV]) < sum(its[M.GUESS_PREVIOUS]), its
+
+
+def test_train_guess_script_round_trip(tmp_path):
+    """mobydiff_amd.train_guess end to end at its workload size, cut to one channel, two steps and
+    two Adam steps: it captures three pairs per step, trains, and writes parameters that load into
+    a fresh network unchanged (with c0 > 0, so the learned start can be used)."""
+    from mobydiff_amd import train_guess
+    out = str(tmp_path / "w.npz")
+    assert train_guess.main(["--runs", "1", "--steps", "2", "--every", "1", "--adam-steps", "2", "--out", out]) == 0
+    net = M.Net(train_guess.NX, train_guess.NY, train_guess.NZ, seed=5)
+    meta = train_guess.load(out, net)
+    assert meta["pairs"] == 6 and meta["adam_steps"] == 2
+    p, c0 = net.get_params()
+    with np.load(out, allow_pickle=False) as z:
+        assert np.array_equal(p, z["params"]) and c0 == float(z["c0"]) and c0 > 0.0
+    net.close()
