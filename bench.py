@@ -62,9 +62,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep_traffic_wavy_192x128x96.jso
 
 
 def measured_traffic(kind, kz, iters=None):
-    """Measured HBM bytes per dispatch.  An LDS-resident dispatch was measured at a fixed batch size
-    (`iterations_per_dispatch`, every check kept) and is scaled to `iters`, the timed region's mean
-    iterations per launch (merged batches make launches of 20 to 60 iterations)."""
+    """Measured HBM bytes per dispatch.  An LDS-resident dispatch is evaluated at `iters`, the
+    sampled mean iterations per launch, from the affine fit of its counters over merged dispatches
+    (or, without a fit, scaled from the fixed 20-iteration measurement)."""
     try:
         with open(TRAFFIC_FILE) as fh:
             table = json.load(fh)
@@ -75,6 +75,10 @@ def measured_traffic(kind, kz, iters=None):
         return None
     b = entry.get("bytes_per_dispatch")
     per = entry.get("iterations_per_dispatch")
+    if iters and "bytes_per_iteration" in entry:
+        # affine fit over merged dispatches of 20 to 160 iterations: the defect epilogue and the
+        # first loads are paid once per dispatch, the halo exchange once per iteration
+        return round(entry["fixed_bytes"] + entry["bytes_per_iteration"] * iters)
     if b is not None and per and iters:
         b = round(b * iters / per)
     return b
