@@ -81,7 +81,9 @@ typedef struct {
 typedef struct {
     int32_t iterations;      /* full red+black iterations (SOLA: niter) */
     int32_t converged;
-    int32_t fallback;        /* learned guess rejected (initial residual worse than the previous-step guess) */
+    int32_t fallback;        /* learned mode: the solve started from the previous pressure (Phi = 0), the
+                              * network's start being rejected (residual not below 0.8x) or skipped by
+                              * the gate */
     int32_t checks;          /* residual evaluations */
     double div0_inf;         /* ||div||_inf of the initial guess */
     double div_inf;          /* ||div||_inf after the solve (from the final residual) */

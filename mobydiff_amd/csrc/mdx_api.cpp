@@ -2,6 +2,7 @@
 #include <immintrin.h>
 
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -966,15 +967,15 @@ static int wait_check(mdx_handle *h, int w, unsigned long long seq, double *r) {
 // solve, and later only when the residuals seen so far predict (geometric decay) that check k is
 // still at least SPEC_MARGIN times above the tolerance.
 // first_seq != 0: the first check is already queued in slot 0 with that sequence number (the fused
-// start kernel).  stage: the RK stage whose previous solve's iteration count serves as the merge
-// prior.
-static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq, int stage) {
+// start kernel).  key: the RK stage + 3 * (start from Phi_0): the previous solve with the same key
+// gives the merge prior.
+static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq, int key) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
     const int MERGE_MAX = h->merge_max;   // at most this many inner_iters batches run as one
     constexpr int PIN = CHECK_PIN;   // pinned words of the two check slots: [PIN + 4 * slot]
     const mdx_config &c = h->cfg;
     const bool lds = lds_for(h, true);
-    const int prior = h->mix_prev_its[stage];
+    const int prior = h->mix_prev_its[key];
     double r = 0.0, r1 = -1.0, r2 = -1.0;   // r_k, r_{k-1}, r_{k-2}
     int it = 0, checks = 0, slot = 0;
     unsigned long long seq[2] = {0, 0};     // the check each slot waits for
@@ -1064,7 +1065,7 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long lon
     st->div_inf = r;
     st->converged = r < c.tol;
     st->lds_sweeps = lds;
-    h->mix_prev_its[stage] = st->converged ? it : 0;
+    h->mix_prev_its[key] = st->converged ? it : 0;
     return MDX_OK;
 }
 
@@ -1076,28 +1077,29 @@ static void net_loss(mdx_handle *h, int s) {
     if (h->net_losses[s] >= 2) h->net_skip[s] = std::min(NET_SKIP_MAX, 1 << std::min(2 * (h->net_losses[s] - 1), 8));
 }
 
-// After the solve of stage s: a network start wins when its solve needed at most NET_SWEEPS of the
-// sweeps of the last solve of this stage that started otherwise; that solve's count is the
-// yardstick for the next network start.
-static void net_outcome(mdx_handle *h, int s, int iterations) {
-    constexpr double NET_SWEEPS = 0.9;
+// After the solve of stage s: a network start wins when the network's evaluation and the solve it
+// started took at most NET_TIME of the host time of the last solve of this stage that started
+// otherwise (each measured from the plan of the start to the converged check, so both include the
+// stage's queued work once): sweeps saved count only when they pay for the inference.
+static void net_outcome(mdx_handle *h, int s, double seconds) {
+    constexpr double NET_TIME = 0.9;
     if (!h->net_started[s]) {
-        h->base_its[s] = iterations;
+        h->base_sec[s] = seconds;
         return;
     }
-    if (h->base_its[s] > 0 && iterations <= NET_SWEEPS * h->base_its[s]) h->net_losses[s] = 0;
+    if (h->base_sec[s] > 0.0 && seconds <= NET_TIME * h->base_sec[s]) h->net_losses[s] = 0;
     else net_loss(h, s);
 }
 
 // Initial guess of the SOR solve for stage s; the solve itself, and its stopping rule, are the
 // same whatever starts it.  Starts:
 //   Phi = 0 (the previous pressure: the plain method, and the start without a history);
-//   Phi_0 = fac * Phi of stage s at the previous step (MDX_GUESS_PHI_PREV and MDX_GUESS_LEARNED);
+//   Phi_0 = fac * Phi of stage s at the previous step (MDX_GUESS_PHI_PREV);
 //   Phi_0 + the network's correction, predicted from (rhs - L_mu Phi_0, Phi_0) (MDX_GUESS_LEARNED),
-//   taken when its FP64 residual max-norm is below net_margin times that of Phi_0.
-// In learned mode a start other than the network's is a fallback (stats.fallback).  With capture
-// on, the network input is built for every solve so the converged Phi can be stored as a training
-// pair.
+//   taken when its FP64 residual max-norm is below net_margin times that of Phi = 0.
+// In learned mode a start other than the network's is a fallback (stats.fallback) to Phi = 0.
+// With capture on, the network input is built for every solve so the converged Phi can be stored
+// as a training pair.
 // The start of stage s's solve, decided before any kernel runs: `simple` when no network input is
 // formed (no learned start to evaluate this time, no capture), the start then being fac * prev
 // (prev non-null) or zero.  Updates the gate's state and the solve's guess flags.
@@ -1122,7 +1124,9 @@ static GuessPlan guess_plan(mdx_handle *h, int s, double dtg, mdx_solve_stats *s
         st->guess_skipped = 1;
         st->fallback = 1;
     }
-    p.use_prev = p.have_prev && (mode == MDX_GUESS_PHI_PREV || mode == MDX_GUESS_LEARNED);
+    // the learned mode falls back to the plain start (Phi = 0: the previous pressure); Phi_0 serves
+    // it only as the network's input and the base of its correction
+    p.use_prev = p.have_prev && mode == MDX_GUESS_PHI_PREV;
     p.build_input = n && (p.learned || h->net_capture);
     p.simple = !p.learned && !p.build_input;
     p.prev = p.use_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
@@ -1172,8 +1176,8 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
         MDX_HIP(hipEventSynchronize(h->ev[9]));
         if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
     }
-    // rp is the residual of the start the solver would otherwise take: Phi_0 with a history (in
-    // the previous-Phi and learned modes), else the previous pressure (Phi = 0)
+    // rp is the residual of Phi_0 (with a history), else of the previous pressure (Phi = 0); the
+    // learned mode's fallback is Phi = 0 also with a history (rz below)
     const bool prev_start = use_prev;
     if (!learned) {   // capture only
         if (prev_start) {
@@ -1185,11 +1189,18 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
         return MDX_OK;
     }
     st->guess_div_inf = rg;
+    // the fallback start Phi = 0 with a history: its residual, phi being zero here
+    double rz = -1.0;
+    if (have_prev && !prev_start) {
+        MDX_HIP(launch_reset_max(h));
+        MDX_HIP(launch_sor_residual64(h, h->phi));
+        if ((rc = read_max(h, &rz))) return rc;
+    }
     // the network's start is taken only when it clearly beats the other start (residual below
     // net_margin times it): a marginally smaller start residual saves no sweeps but costs an
     // inference per solve.  A start not taken counts as a loss; a start taken is judged by the
     // sweeps its solve needs (net_outcome).
-    const double other = prev_start || !have_prev ? rp : -1.0;
+    const double other = prev_start || !have_prev ? rp : rz;
     // the chosen start is copied into phi (captured sweep graphs hold that buffer's address)
     if (other < 0.0 || rg < h->net_margin * other) {
         MDX_HIP(hipMemcpyAsync(h->phi, h->phi2, h->ni * 8, hipMemcpyDeviceToDevice, h->stream));
@@ -1200,7 +1211,7 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
     net_loss(h, s);
     st->fallback = 1;
     if (prev_start) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));   // phi2 holds the network's
-    *r_first = rp;
+    *r_first = other;
     return MDX_OK;
 }
 
@@ -1286,6 +1297,7 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         if ((rc = rb_prepare(h))) return rc;
         double r_first = -1.0;
         const int stage = (int)(h->proj_count % 3);
+        const auto t_plan = std::chrono::steady_clock::now();
         const GuessPlan gp = guess_plan(h, stage, dtg, &st);
         unsigned long long first_seq = 0;
         if (gp.simple && c.precision == MDX_PREC_MIXED) {
@@ -1296,10 +1308,10 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
             MDX_HIP(launch_sor_prepare(h));
             if ((rc = initial_guess(h, stage, dtg, gp, &st, &r_first))) return rc;
         }
-        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage) : sor_solve64(h, &st, r_first);
+        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage + (gp.use_prev ? 3 : 0)) : sor_solve64(h, &st, r_first);
         if (rc) return rc;
         if (st.fallback == 0 && r_first >= 0.0) st.div0_inf = r_first;
-        net_outcome(h, stage, st.iterations);
+        net_outcome(h, stage, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_plan).count());
         if (h->net && h->net_capture && st.converged && (rc = capture_pair(h, stage, dtg))) return rc;
         // the converged Phi becomes the history of this stage for the next step
         MDX_HIP(hipMemcpyAsync(h->phi_hist + (size_t)stage * h->ni, h->phi, h->ni * 8, hipMemcpyDeviceToDevice,
@@ -1324,7 +1336,8 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
 extern "C" int mdx_set_guess(mdx_handle *h, int32_t guess) {
     MDX_CHECK(h, MDX_ERR_ARG, "null handle");
     for (int q = 0; q < 3; ++q) {
-        h->net_losses[q] = h->net_skip[q] = h->base_its[q] = 0;
+        h->net_losses[q] = h->net_skip[q] = 0;
+        h->base_sec[q] = 0.0;
         h->net_started[q] = false;
     }
     MDX_CHECK(guess == MDX_GUESS_PREVIOUS || guess == MDX_GUESS_LEARNED || guess == MDX_GUESS_PHI_PREV,

@@ -224,10 +224,11 @@ struct mdx_handle {
     // mixed solve: at most this many inner_iters batches run as one while the checks between them
     // are predicted to fail (MOBYDIFF_MERGE_MAX; 1 keeps every check)
     int merge_max = 8;
-    // iterations of the last converged mixed solve of each RK stage (the merge prior; 0: none)
-    int mix_prev_its[3] = {0, 0, 0};
+    // iterations of the last converged mixed solve of each RK stage and start kind (the merge
+    // prior; index stage + 3 * (start from Phi_0); 0: none)
+    int mix_prev_its[6] = {0, 0, 0, 0, 0, 0};
     bool net_started[3] = {false, false, false};
-    int base_its[3] = {0, 0, 0};
+    double base_sec[3] = {0.0, 0.0, 0.0};   // host time of the last solve of each stage not started by the network
     long proj_count = 0;
 };
 
