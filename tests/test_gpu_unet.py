@@ -244,3 +244,38 @@ def test_train_guess_script_round_trip(tmp_path):
     with np.load(out, allow_pickle=False) as z:
         assert np.array_equal(p, z["params"]) and c0 == float(z["c0"]) and c0 > 0.0
     net.close()
+
+
+def test_learned_fallback_is_the_plain_start(monkeypatch):
+    """With the network's start never taken (MOBYDIFF_NET_MARGIN=0), every solve of the learned mode
+    falls back to the previous pressure (Phi = 0), the plain mode's start: the fields after several
+    steps are bit-identical to the plain mode's, and every solve reports the fallback."""
+    monkeypatch.setenv("MOBYDIFF_NET_MARGIN", "0")
+    nx, ny, nz, dt = 64, 32, 32, 5e-4
+    geo = M.wavy_geometry(A=0.1)
+    fields, fb = [], []
+    for learned in (False, True):
+        cfg = M.channel_config(nx, ny, nz, 4.0, 2.2, 2.0, 2000.0, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8,
+                               check_every=10, precision=M.PREC_MIXED, inner_iters=10, flowrate=True,
+                               flow_height=2.0)
+        ch = M.Channel(cfg)
+        ch.compute_ibm(geo)
+        ch.init_fields(seed=0, amp=0.05, geo=geo, dt=dt)
+        net = None
+        if learned:
+            net = M.Net(nx, ny, nz, seed=4)
+            p, _ = net.get_params()
+            net.set_params(p, c0=1.0)
+            ch.attach_net(net, capture=False)
+            ch.set_guess(M.GUESS_LEARNED)
+        for _ in range(4):
+            st = ch.step(dt)
+            if learned:
+                fb.extend(st.stage[q].fallback for q in range(3))
+        fields.append(ch.download(("un", "vn", "wn", "pn")))
+        ch.close()
+        if net is not None:
+            net.close()
+    assert fb == [1] * 12, fb
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(fields[0][k], fields[1][k]), k
