@@ -44,15 +44,17 @@ def make_channel(M, precision, check_every, inner):
 def run_steps(ch, k):
     ch.synchronize()
     ch.reset_phase_times()
-    its, fb = [], 0
+    its, fb, started = [], 0, []
     t0 = time.perf_counter()
     for _ in range(k):
         st = ch.step(DT)
         its.append([st.stage[q].iterations for q in range(3)])
         fb += sum(st.stage[q].fallback for q in range(3))
+        started.append([not st.stage[q].fallback for q in range(3)])
     ch.synchronize()
     el = time.perf_counter() - t0
     div = ch.div_norm()
+    run_steps.started = started   # which solves the network started (learned mode)
     return el, its, div, ch.phase_times(), fb
 
 
@@ -122,6 +124,17 @@ def momentum_roofline(ch, cells, ibm, dt, workload):
     return {"kernel": "momentum_tiled_kernel", "workload": workload, "bound": "hbm",
             "avg_launch_us": round(ms * 1e3, 2), "bytes_per_launch": b, "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
+def network_gain(its, started, its_plain):
+    """Sweeps of the solves the network started against the same step and stage of the run from the
+    plain start (mixed_no_guess, from the same state): the guess's gain where it was used."""
+    pairs = [(its[k][q], its_plain[k][q]) for k in range(len(its)) for q in range(3) if started[k][q]]
+    if not pairs:
+        return {"count": 0}
+    a, b = sum(p[0] for p in pairs), sum(p[1] for p in pairs)
+    return {"count": len(pairs), "iterations": a / len(pairs), "plain_start_iterations": b / len(pairs),
+            "sweeps_saved": round(1.0 - a / b, 4)}
 
 
 def flat_fft_production(M, steps, precision=None):
@@ -271,6 +284,7 @@ def main():
         return ch
 
     el_h, its_h, div_h, t_h, fb_h = run_steps(head, args.steps)
+    started_h = run_steps.started
     plain = fresh(M.PREC_FP64)
     el_p, its_p, div_p, t_p, _ = run_steps(plain, args.steps)
     info_p = plain.info()
@@ -322,6 +336,7 @@ def main():
         "sor_iterations_per_stage": per_stage(its_h),
         "div_inf_final": div_h,
         "fallback_rate": round(fb_h / (3.0 * args.steps), 4) if learned else None,
+        "network_started_solves": network_gain(its_h, started_h, its_m) if learned else None,
         "training": training,
         "network": network,
         "plain_fp64": dict(side(el_p, its_p, div_p, t_p), roofline=roofline(t_p, "fp64", info_p)),
