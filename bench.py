@@ -180,7 +180,8 @@ def flat_fft_production(M, steps, precision=None):
 
 
 def cpu_baseline(f_host, coef, cores, steps):
-    """The oracle (C restatement of the reference CPU path, FP64, OpenMP) on the same state."""
+    """The oracle (C restatement of the reference CPU path, FP64, OpenMP) on the same state.  Returns
+    the baseline entry and the oracle's fields after the steps (the reference of the accuracy check)."""
     from oracle import oracle as O
     g = O.Grid(NX, NY, NZ, lx=LX, ly=LY, lz=LZ, re=RE, forcing=(0.0, 0.0, 0.0), periodic=(1, 0, 1))
     f = {k: v.copy() for k, v in f_host.items()}
@@ -190,10 +191,23 @@ def cpu_baseline(f_host, coef, cores, steps):
         its.append([s[0] for s in O.rk3_step(g, f, coef, DT, solver=2, omega=OMEGA, tol=TOL, max_iter=200000,
                                               check_every=20, flowrate=True, ub_target=1.0, height=2.0)])
     el = time.perf_counter() - t0
-    return {"value": round(NX * NY * NZ * steps / el / 1e6, 3), "unit": "Mcell-steps/s", "cores": cores,
-            "kind": "port", "sample": f"{steps} RK3 step(s) of the same 192x128x96 wavy workload from the "
-            f"post-warm-up state, FP64 SOR to 1e-8 (iterations per stage {its})",
-            "seconds": round(el, 3)}
+    entry = {"value": round(NX * NY * NZ * steps / el / 1e6, 3), "unit": "Mcell-steps/s", "cores": cores,
+             "kind": "port", "sample": f"{steps} RK3 steps of the same 192x128x96 wavy workload from the "
+             f"post-warm-up state, FP64 SOR to 1e-8 (iterations per stage {its})",
+             "seconds": round(el, 3)}
+    return entry, f
+
+
+def field_errors(out, ref):
+    """rel-L2 of u, v, w and of p with its mean removed (the Neumann solve fixes p up to a constant)
+    over the interior cells."""
+    import numpy as np
+    sl = (slice(1, -1),) * 3
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))   # noqa: E731
+    e = {k[0]: rel(out[k][sl], ref[k][sl]) for k in ("un", "vn", "wn")}
+    p, pr = out["pn"][sl], ref["pn"][sl]
+    e["p"] = rel(p - p.mean(), pr - pr.mean())
+    return e
 
 
 def main():
@@ -201,7 +215,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="oracle steps timed as the CPU baseline; the GPU paths run the same steps from the "
+                         "same state for the accuracy check")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-every", type=int, default=20)
     ap.add_argument("--inner", type=int, default=20)
@@ -274,10 +290,24 @@ def main():
         head.attach_net(net, capture=False)
         head.set_guess(M.GUESS_LEARNED)
 
-    def fresh(precision, guess=M.GUESS_PREVIOUS, prime=False):
-        ch = make_channel(M, precision, args.check_every, args.inner)
+    def fresh(precision, guess=M.GUESS_PREVIOUS, prime=False, lds_fp64=False, with_net=False):
+        # MOBYDIFF_LDS_SOR is read when a handle is created: 2 puts the FP64 batches on the
+        # LDS-resident kernel too
+        saved = os.environ.get("MOBYDIFF_LDS_SOR")
+        if lds_fp64:
+            os.environ["MOBYDIFF_LDS_SOR"] = "2"
+        try:
+            ch = make_channel(M, precision, args.check_every, args.inner)
+        finally:
+            if lds_fp64:
+                if saved is None:
+                    del os.environ["MOBYDIFF_LDS_SOR"]
+                else:
+                    os.environ["MOBYDIFF_LDS_SOR"] = saved
         ch.set_ibm(*coef)
         ch.upload(**state)
+        if with_net:
+            ch.attach_net(net, capture=False)
         ch.set_guess(guess)
         if prime:
             ch.step(DT)   # fills the per-stage history the guess uses
@@ -289,6 +319,10 @@ def main():
     el_p, its_p, div_p, t_p, _ = run_steps(plain, args.steps)
     info_p = plain.info()
     plain.close()
+    plain_lds = fresh(M.PREC_FP64, lds_fp64=True)
+    el_pl, its_pl, div_pl, t_pl, _ = run_steps(plain_lds, args.steps)
+    info_pl = plain_lds.info()
+    plain_lds.close()
     mixed = fresh(M.PREC_MIXED)
     el_m, its_m, div_m, t_m, _ = run_steps(mixed, args.steps)
     mixed.close()
@@ -319,7 +353,7 @@ def main():
                 "phase_ms_per_step": split(t, args.steps), "sor_iterations_per_stage": per_stage(its),
                 "div_inf_final": div}
 
-    v_h, v_p = rate(el_h), rate(el_p)
+    v_h, v_p, v_pl = rate(el_h), rate(el_p), rate(el_pl)
     out = {
         "metric": METRIC, "value": round(v_h, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el_h / args.steps * 1e3, 3),
@@ -339,12 +373,16 @@ def main():
         "network_started_solves": network_gain(its_h, started_h, its_m) if learned else None,
         "training": training,
         "network": network,
-        "plain_fp64": dict(side(el_p, its_p, div_p, t_p), roofline=roofline(t_p, "fp64", info_p)),
+        "plain_fp64": dict(side(el_p, its_p, div_p, t_p), kernel="streaming one-colour sweeps",
+                           roofline=roofline(t_p, "fp64", info_p)),
+        "plain_fp64_lds": dict(side(el_pl, its_pl, div_pl, t_pl), kernel="LDS-resident FP64 batches",
+                               sor_kernel=info_pl),
         "mixed_no_guess": side(el_m, its_m, div_m, t_m),
         "mixed_previous_phi_guess": dict(side(el_q, its_q, div_q, t_q),
                                          fallback_rate=round(fb_q / (3.0 * args.steps), 4)),
         "sweeps_saved_vs_no_guess": round(1.0 - per_stage(its_h) / per_stage(its_m), 4),
-        "speedup_vs_plain_fp64": round(v_h / v_p, 3),
+        # against the faster of the two plain FP64 kernels
+        "speedup_vs_plain_fp64": round(v_h / max(v_p, v_pl), 3),
         "roofline": roofline(t_h, "fp32", head.info()),
     }
     out["stencil_roofline"] = momentum_roofline(head, cells, True, DT, "wavy channel 192x128x96 (IBM)")
@@ -354,10 +392,28 @@ def main():
     if not args.no_cpu_baseline:
         aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         cores = int(os.environ.get("OMP_NUM_THREADS", aff))   # the oracle's OpenMP thread count
-        out["cpu_baseline"] = cpu_baseline(state, coef, cores, args.cpu_steps)
+        out["cpu_baseline"], ref = cpu_baseline(state, coef, cores, args.cpu_steps)
+        # accuracy: each GPU path runs the oracle's steps from the same state
+        paths = [("headline", M.PREC_MIXED, M.GUESS_LEARNED if learned else M.GUESS_PREVIOUS, False),
+                 ("mixed_no_guess", M.PREC_MIXED, M.GUESS_PREVIOUS, False),
+                 ("plain_fp64", M.PREC_FP64, M.GUESS_PREVIOUS, False),
+                 ("plain_fp64_lds", M.PREC_FP64, M.GUESS_PREVIOUS, True)]
+        acc = {"steps": args.cpu_steps, "reference": "the oracle (FP64 SOR to 1e-8) from the same state",
+               "metric": "rel-L2 over interior cells; p with its mean removed"}
+        for name, prec, guess, lds64 in paths:
+            ch = fresh(prec, guess, lds_fp64=lds64, with_net=learned and name == "headline")
+            for _ in range(args.cpu_steps):
+                ch.step(DT)
+            e = field_errors(ch.download(("un", "vn", "wn", "pn")), ref)
+            e["div_inf"] = ch.div_norm()
+            ch.close()
+            acc[name] = {k: float(f"{v:.3e}") for k, v in e.items()}
+        out["error_vs_fp64_oracle"] = acc
     else:
         out["cpu_baseline"] = None
     head.close()
+    if net is not None:
+        net.close()
     print(json.dumps(out))
 
 

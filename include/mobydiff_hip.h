@@ -91,6 +91,9 @@ typedef struct {
     int32_t lds_sweeps;      /* 1: sweeps ran in the LDS-resident kernel, 0: streaming kernels */
     int32_t guess_skipped;   /* learned mode: the network was not evaluated (its start lost the
                                 previous solves of this stage; it is re-tried after 2..16 solves) */
+    int32_t lds_restarts;    /* 1: a neighbour wait of the LDS-resident sweeps gave up (another stream
+                                or process held CUs); the solve was redone from Phi = 0 on the
+                                streaming sweeps and the statistics describe that solve */
 } mdx_solve_stats;
 
 typedef struct {
@@ -199,6 +202,7 @@ int mdx_get_info(mdx_handle *h, char *buf, int32_t len);
 typedef struct mdx_net mdx_net;
 int64_t mdx_net_param_count(void);
 int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, mdx_net **out);
+/* detaches the network from every handle it is attached to, then frees it */
 int mdx_net_destroy(mdx_net *net);
 /* c0: target scale (Phi = output * rms(div u*) * c0); get may pass NULL, set ignores c0 <= 0 */
 int mdx_net_get_params(mdx_net *net, float *params, double *c0);
@@ -221,7 +225,9 @@ int mdx_net_eval_replay(mdx_net *net, int32_t count, float *mean_loss);
  * MFMA flops of one pass (as the convolution kernels issue them) */
 int mdx_net_time_forward(mdx_net *net, int32_t reps, float *ms_per_forward, double *flops);
 /* attach (or detach with net = NULL) a network to a solver handle: cfg.guess = MDX_GUESS_LEARNED
- * uses it for the initial guess; capture != 0 appends a pair per converged SOR solve */
+ * uses it for the initial guess; capture != 0 appends a pair per converged SOR solve.  One network
+ * may be attached to several handles that are stepped one after another; mdx_step_batch rejects
+ * handles that share a network. */
 int mdx_attach_net(mdx_handle *h, mdx_net *net, int32_t capture);
 
 #ifdef __cplusplus

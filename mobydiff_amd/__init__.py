@@ -9,6 +9,7 @@ library is not built, or no HIP device is present, loading fails with an error.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -44,7 +45,8 @@ class Geometry(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("fallback", C.c_int32),
                 ("checks", C.c_int32), ("div0_inf", C.c_double), ("div_inf", C.c_double),
-                ("guess_div_inf", C.c_double), ("lds_sweeps", C.c_int32), ("guess_skipped", C.c_int32)]
+                ("guess_div_inf", C.c_double), ("lds_sweeps", C.c_int32), ("guess_skipped", C.c_int32),
+                ("lds_restarts", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -178,6 +180,10 @@ class Channel:
             self.h = C.c_void_p()
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime (and a profiler around it) may already be torn
+        # down: leave the device memory to the process exit instead of calling into the library
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -311,11 +317,14 @@ class Net:
         self.nparam = int(lib().mdx_net_param_count())
 
     def close(self):
+        """Destroy the network; handles it is attached to are detached by the library."""
         if self.h:
             lib().mdx_net_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

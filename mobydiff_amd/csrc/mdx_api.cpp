@@ -316,6 +316,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         h->lds_sor_mode = v;
     }
     if (const char *lc = std::getenv("MOBYDIFF_LDS_COOP")) h->rb_coop = lc[0] == '1';
+    if (const char *sl = std::getenv("MOBYDIFF_LDS_SPIN_LIMIT")) h->rb_spin_limit = std::max(0, std::atoi(sl));
     if (const char *nm = std::getenv("MOBYDIFF_NET_MARGIN")) h->net_margin = std::atof(nm);
     if (const char *mm = std::getenv("MOBYDIFF_MERGE_MAX")) h->merge_max = std::max(1, std::atoi(mm));
     if (const char *lf = std::getenv("MOBYDIFF_LDS_FORM")) {
@@ -417,6 +418,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
 extern "C" int mdx_destroy(mdx_handle *h) {
     if (!h) return MDX_OK;
     (void)hipStreamSynchronize(h->stream);
+    if (h->net) unet::net_detach(h->net, h);
     free_all(h);
     delete h;
     return MDX_OK;
@@ -674,19 +676,41 @@ extern "C" int mdx_apply_bc_redblack(mdx_handle *h) {
     return MDX_OK;
 }
 
+// Internal status (never returned through the C ABI): a neighbour wait of the LDS-resident solver
+// gave up.  The batch drained, but its iterate and residual are not trustworthy; the SOR solve that
+// saw it restarts on the streaming sweeps.
+static constexpr int MDX_LDS_TIMEOUT = 100;
+// restarts after which the LDS-resident solver stays off for the handle
+static constexpr int LDS_TIMEOUT_MAX = 3;
+
+// The wait-timeout word has been seen set: clear it on the device (it is set again only by a later
+// wait that gives up) and report the timeout once.
+static int lds_timeout_seen(mdx_handle *h) {
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    MDX_HIP(hipMemsetAsync(h->red_max + 1, 0, sizeof(unsigned long long), h->stream));
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    mdx_set_error("LDS red-black solver: a neighbour wait exceeded its spin limit");
+    return MDX_LDS_TIMEOUT;
+}
+
 // Reads the max reduction (red_max[0]) and the LDS solver's wait-timeout word (red_max[1]) in
-// one copy; a timeout is an error (the kernel drained, but its result is not trustworthy).
+// one copy.  A set timeout word is cleared and reported as MDX_LDS_TIMEOUT; the maximum itself was
+// formed by a kernel of its own and is valid.
 static int read_max(mdx_handle *h, double *out) {
     MDX_HIP(hipMemcpyAsync(h->h_pinned, h->red_max, 16, hipMemcpyDeviceToHost, h->stream));
     MDX_HIP(hipStreamSynchronize(h->stream));
     *out = h->h_pinned[0];
     int err;
     std::memcpy(&err, &h->h_pinned[1], sizeof err);
-    if (err) {
-        mdx_set_error("LDS red-black solver: a neighbour wait exceeded its spin limit");
-        return MDX_ERR_HIP;
-    }
+    if (err) return lds_timeout_seen(h);
     return MDX_OK;
+}
+
+// read_max for the norms: a timeout word left by a batch whose check was never read (discarded
+// after convergence) does not concern them
+static int read_norm(mdx_handle *h, double *out) {
+    const int rc = read_max(h, out);
+    return rc == MDX_LDS_TIMEOUT ? MDX_OK : rc;
 }
 
 // Plan and allocate the LDS-resident solver once the IBM state is known (uniform grid, z-invariant
@@ -747,14 +771,14 @@ extern "C" int mdx_div_norm(mdx_handle *h, double *div_inf) {
     MDX_CHECK(h && div_inf, MDX_ERR_ARG, "null argument");
     MDX_HIP(launch_reset_max(h));
     MDX_HIP(launch_div_max(h, h->un, h->vn, h->wn));
-    return read_max(h, div_inf);
+    return read_norm(h, div_inf);
 }
 
 extern "C" int mdx_get_cfl(mdx_handle *h, double *cfl) {
     MDX_CHECK(h && cfl, MDX_ERR_ARG, "null argument");
     MDX_HIP(launch_reset_max(h));
     MDX_HIP(launch_cfl_max(h));
-    return read_max(h, cfl);
+    return read_norm(h, cfl);
 }
 
 static int ensure_mu(mdx_handle *h, double dtg) {
@@ -809,7 +833,9 @@ static hipError_t sweep_launches(mdx_handle *h, bool fp32, int m) {
 }
 
 // whether batches of this precision run on the LDS-resident solver
-static bool lds_for(const mdx_handle *h, bool fp32) { return h->rb_state == 1 && (fp32 || h->lds_sor_mode == 2); }
+static bool lds_for(const mdx_handle *h, bool fp32) {
+    return h->rb_state == 1 && !h->lds_suspended && (fp32 || h->lds_sor_mode == 2);
+}
 
 // sweep counters: account finished batches in order (end event complete; `wait`: block for them)
 static void sweep_account(mdx_handle *h, bool wait = false) {
@@ -947,10 +973,7 @@ static int wait_check(mdx_handle *h, int w, unsigned long long seq, double *r) {
     *r = h->h_pinned[w];
     int err;
     std::memcpy(&err, &h->h_pinned[w + 1], sizeof err);
-    if (err) {
-        mdx_set_error("LDS red-black solver: a neighbour wait exceeded its spin limit");
-        return MDX_ERR_HIP;
-    }
+    if (err) return lds_timeout_seen(h);
     return MDX_OK;
 }
 
@@ -1081,10 +1104,12 @@ static void net_loss(mdx_handle *h, int s) {
 // started took at most NET_TIME of the host time of the last solve of this stage that started
 // otherwise (each measured from the plan of the start to the converged check, so both include the
 // stage's queued work once): sweeps saved count only when they pay for the inference.
-static void net_outcome(mdx_handle *h, int s, double seconds) {
+static void net_outcome(mdx_handle *h, int s, double seconds, bool evaluated) {
     constexpr double NET_TIME = 0.9;
     if (!h->net_started[s]) {
-        h->base_sec[s] = seconds;
+        // the yardstick is a solve that ran no inference: a network evaluated and then rejected by
+        // the residual test would put its inference time into the plain start's time
+        if (!evaluated) h->base_sec[s] = seconds;
         return;
     }
     if (h->base_sec[s] > 0.0 && seconds <= NET_TIME * h->base_sec[s]) h->net_losses[s] = 0;
@@ -1120,6 +1145,13 @@ static GuessPlan guess_plan(mdx_handle *h, int s, double dtg, mdx_solve_stats *s
     // again only after 4, 16, 64, then every NET_SKIP_MAX skipped solves (previous-Phi start then)
     if (p.learned && h->net_skip[s] > 0) {
         h->net_skip[s] -= 1;
+        p.learned = false;
+        st->guess_skipped = 1;
+        st->fallback = 1;
+    }
+    // a network start is judged against the host time of a plain solve of the same stage: until
+    // one has been timed, the stage runs the plain start
+    if (p.learned && h->base_sec[s] <= 0.0) {
         p.learned = false;
         st->guess_skipped = 1;
         st->fallback = 1;
@@ -1257,7 +1289,7 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         for (;;) {
             MDX_HIP(launch_reset_max(h));
             MDX_HIP(launch_fft_residual(h));
-            if ((rc = read_max(h, &r))) return rc;
+            if ((rc = read_norm(h, &r))) return rc;
             if (passes == 0) st.div0_inf = r * dtg;
             ++st.checks;
             if (!(r * dtg >= c.tol) || passes >= MAX_FFT_PASSES) break;
@@ -1306,12 +1338,38 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
             MDX_HIP(launch_sor_start(h, gp.prev, gp.fac, h->d_pinned + CHECK_PIN, first_seq));
         } else {
             MDX_HIP(launch_sor_prepare(h));
-            if ((rc = initial_guess(h, stage, dtg, gp, &st, &r_first))) return rc;
+            rc = initial_guess(h, stage, dtg, gp, &st, &r_first);
+            if (rc && rc != MDX_LDS_TIMEOUT) return rc;
         }
-        rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage + (gp.use_prev ? 3 : 0)) : sor_solve64(h, &st, r_first);
+        if (rc == MDX_OK)
+            rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage + (gp.use_prev ? 3 : 0))
+                                               : sor_solve64(h, &st, r_first);
+        if (rc == MDX_LDS_TIMEOUT) {
+            // a neighbour wait of the LDS-resident sweeps gave up (another stream or process held
+            // CUs): the solve restarts from Phi = 0 on the streaming sweeps, and the next solve
+            // tries the LDS-resident sweeps again (for good off after LDS_TIMEOUT_MAX restarts)
+            h->lds_timeouts += 1;
+            h->lds_suspended = true;
+            if (h->lds_timeouts >= LDS_TIMEOUT_MAX) {
+                h->rb_state = -1;
+                h->rb_note = "streaming sweeps: neighbour waits of the LDS-resident sweeps gave up " +
+                             std::to_string(h->lds_timeouts) + " times";
+            }
+            const int learned_mode = c.guess == MDX_GUESS_LEARNED;
+            st = mdx_solve_stats{};
+            st.lds_restarts = 1;
+            st.fallback = learned_mode;
+            h->net_started[stage] = false;
+            r_first = -1.0;
+            MDX_HIP(launch_zero64(h, h->phi, h->ni));
+            rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, 0, stage) : sor_solve64(h, &st, -1.0);
+            if (rc == MDX_LDS_TIMEOUT) rc = MDX_ERR_HIP;   // cannot come from the streaming sweeps
+        }
+        h->lds_suspended = false;
         if (rc) return rc;
         if (st.fallback == 0 && r_first >= 0.0) st.div0_inf = r_first;
-        net_outcome(h, stage, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_plan).count());
+        net_outcome(h, stage, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_plan).count(),
+                    gp.learned);
         if (h->net && h->net_capture && st.converged && (rc = capture_pair(h, stage, dtg))) return rc;
         // the converged Phi becomes the history of this stage for the next step
         MDX_HIP(hipMemcpyAsync(h->phi_hist + (size_t)stage * h->ni, h->phi, h->ni * 8, hipMemcpyDeviceToDevice,
@@ -1462,7 +1520,12 @@ extern "C" int mdx_step_batch(mdx_handle *const *hs, int32_t n, double dt, mdx_s
     MDX_CHECK(hs && n > 0, MDX_ERR_ARG, "mdx_step_batch: need n > 0 handles");
     for (int q = 0; q < n; ++q) {
         MDX_CHECK(hs[q], MDX_ERR_ARG, "mdx_step_batch: null handle");
-        for (int r = 0; r < q; ++r) MDX_CHECK(hs[r] != hs[q], MDX_ERR_ARG, "mdx_step_batch: a handle appears twice");
+        for (int r = 0; r < q; ++r) {
+            MDX_CHECK(hs[r] != hs[q], MDX_ERR_ARG, "mdx_step_batch: a handle appears twice");
+            // a network's activations, stream and replay cursor serve one solve at a time
+            MDX_CHECK(!hs[q]->net || hs[r]->net != hs[q]->net, MDX_ERR_ARG,
+                      "mdx_step_batch: two handles share one attached network");
+        }
     }
     for (int q = 0; q < n; ++q) {
         mdx_handle *h = hs[q];

@@ -198,6 +198,13 @@ struct mdx_handle {
     // in the tile; 2 one z-column per thread
     int rb_form = 0;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
+    // polls of a neighbour wait before it gives up (MOBYDIFF_LDS_SPIN_LIMIT; a test hook); a wait
+    // that gives up sets the error word, the batch drains, and the solve restarts on the
+    // streaming sweeps (lds_suspended for the rest of that solve; after LDS_TIMEOUT_MAX such
+    // restarts the LDS-resident solver stays off)
+    int rb_spin_limit = 1 << 20;
+    int lds_timeouts = 0;
+    bool lds_suspended = false;
     mdx::RbPlanOut rb_plan64{}, rb_plan32{};
     void *rb_faces = nullptr;      // tagged face words of the LDS-resident sweeps
     int *rb_xl32 = nullptr, *rb_xl64 = nullptr;   // exchange lists of the two plans

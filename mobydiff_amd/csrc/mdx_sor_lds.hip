@@ -68,6 +68,7 @@ struct RbArgs {
     const int *xl;           // exchange lists of the plan: [wg][entry] publish words, then pull words
     unsigned tag0;           // iteration s of this launch publishes tag tag0 + s + 1 (never 0)
     int *err;
+    int spin_limit;          // polls of a neighbour wait before it gives up and sets *err
     // Defect epilogue (the FP32 batches of the mixed solve): after the last iteration and one more
     // halo exchange, the corrected iterate phi_out = phi_in + e, the new defect
     // r_out = (float)(rhs64 - L_mu phi_out) and the workgroup's max |defect| (max_part[wg]) are
@@ -611,7 +612,7 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
                 for (int q = 0; q < NW; ++q)
                     ready = ready && (!((xdst[jj] >> 25) & 1) || (unsigned)(got[jj][q] >> 32) == tag);
             if (ready) break;
-            if (spins >= (1 << 20)) {   // a neighbour never published: flag it and run to the end
+            if (spins >= a.spin_limit) {   // a neighbour never published: flag it and run to the end
                 __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 failed = true;
                 break;
@@ -715,7 +716,7 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
             wv[0] = __hip_atomic_load(MP + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             wv[1] = __hip_atomic_load(MP + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if ((unsigned)(wv[0] >> 32) == mtag && (unsigned)(wv[1] >> 32) == mtag) break;
-            if (failed || spins >= (1 << 20)) {
+            if (failed || spins >= a.spin_limit) {
                 __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 failed = true;
                 break;
@@ -989,6 +990,7 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     a.tag0 = (unsigned)h->rb_tag;
     h->rb_tag += (unsigned long long)iters + 1ull;
     a.err = h->rb_err;
+    a.spin_limit = h->rb_spin_limit;
     const void *fn = h->ibm_on ? rb_fn_for<T, true>(pl.maxp) : rb_fn_for<T, false>(pl.maxp);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds);
     if (e != hipSuccess) return e;

@@ -31,6 +31,7 @@ struct mdx_net {
     // inference timing
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool tiled_conv = true;        // LDS-tiled convolutions where the grid allows (bit-identical)
+    std::vector<struct mdx_handle *> users;   // solver handles this network is attached to
 };
 
 namespace mdx {
@@ -62,6 +63,8 @@ hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi);
 hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, double fac);
 hipError_t net_load_replay(mdx_net *n, int slot);
 hipError_t net_mse_only(mdx_net *n);
+// drop handle h from the network's attached handles
+void net_detach(mdx_net *n, struct mdx_handle *h);
 hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi, const double *phi_prev,
                              double fac);
 }  // namespace unet

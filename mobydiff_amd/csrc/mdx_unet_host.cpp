@@ -123,8 +123,20 @@ extern "C" int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed,
     return MDX_OK;
 }
 
+void mdx::unet::net_detach(mdx_net *n, mdx_handle *h) {
+    n->users.erase(std::remove(n->users.begin(), n->users.end(), h), n->users.end());
+}
+
+// Destroying a network detaches it from every handle it is attached to (their learned mode then
+// falls back to the plain start).
 extern "C" int mdx_net_destroy(mdx_net *n) {
     if (!n) return MDX_OK;
+    for (mdx_handle *h : n->users) {
+        (void)hipStreamSynchronize(h->stream);
+        h->net = nullptr;
+        h->net_capture = false;
+    }
+    n->users.clear();
     (void)hipStreamSynchronize(n->stream);
     free_net(n);
     delete n;
@@ -295,7 +307,9 @@ extern "C" int mdx_attach_net(mdx_handle *h, mdx_net *n, int32_t capture) {
         NET_CHECK(h->cfg.solver == MDX_SOLVER_SOR, "mdx_attach_net: the learned guess serves the SOR solver");
         NET_CHECK(!capture || n->cap > 0, "mdx_attach_net: capture needs mdx_net_replay_reserve first");
     }
+    if (h->net) mdx::unet::net_detach(h->net, h);
     h->net = n;
     h->net_capture = n && capture;
+    if (n) n->users.push_back(h);
     return MDX_OK;
 }

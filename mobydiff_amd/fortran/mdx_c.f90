@@ -33,7 +33,7 @@ module mdx_c
     type, bind(C) :: mdx_solve_stats
         integer(c_int32_t) :: iterations, converged, fallback, checks
         real(c_double) :: div0_inf, div_inf, guess_div_inf
-        integer(c_int32_t) :: lds_sweeps, guess_skipped
+        integer(c_int32_t) :: lds_sweeps, guess_skipped, lds_restarts
     end type mdx_solve_stats
 
     ! shared run-time state of the drop-in modules

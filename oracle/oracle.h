@@ -107,6 +107,9 @@ void mdo_sor_div_rhs(const mdo_grid *g, const double *un, const double *vn, cons
 void mdo_sor_sweep(const mdo_grid *g, int color, double omega, double dtg,
                    const double *cu, const double *cv, const double *cw,
                    const double *rhs, double *phi);
+void mdo_sor_relax_cells(const mdo_grid *g, double omega, double dtg,
+                         const double *cu, const double *cv, const double *cw,
+                         const double *rhs, double *phi, const int *cells, int ncells);
 double mdo_sor_residual_inf(const mdo_grid *g, double dtg,
                             const double *cu, const double *cv, const double *cw,
                             const double *rhs, const double *phi);

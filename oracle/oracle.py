@@ -337,6 +337,40 @@ def rk3_step(g, f, coef, dt, solver=0, niter=3, omega=1.8, tol=1e-8, max_iter=10
     return [(st[q].iterations, st[q].converged, st[q].div_inf, st[q].div0_inf) for q in range(3)]
 
 
+def rk_constants():
+    """(alpha, beta, gamma) of the oracle's RK3 stages (step.f90:21-23)."""
+    L = lib()
+    return tuple(tuple((C.c_double * 3).in_dll(L, name)) for name in ("mdo_rk_alpha", "mdo_rk_beta", "mdo_rk_gamma"))
+
+
+def run_steps(g, f, coef, nsteps, dt, cflmax=0.0, dtmax=1e-3, **step_kw):
+    """The time loop of main.f90:64-97 over rk3_step: each step runs with the current dt; after it,
+    when cflmax > 0, cfl = get_cfl (step.f90:210) and dt = min(cflmax / cfl, dtmax) if cfl > 0.
+    Returns (dts, cfls, stats): the dt of each step, the CFL after it (0 when not computed) and the
+    per-step solver statistics."""
+    dts, cfls, stats = [], [], []
+    for _ in range(nsteps):
+        dts.append(dt)
+        stats.append(rk3_step(g, f, coef, dt, **step_kw))
+        cfl = 0.0
+        if cflmax > 0.0:
+            cfl = get_cfl(g, f)
+        cfls.append(cfl)
+        if cflmax > 0.0 and cfl > 0.0:
+            dt = min(cflmax / cfl, dtmax)
+    return dts, cfls, stats
+
+
+def sor_relax_cells(g, coef, omega, dtg, rhs, phi, cells):
+    """Relax the listed (i, j, k) cells (1-based) one after another in the given order, with the
+    relaxation of sor_sweep (test hook for the colouring)."""
+    s = g.struct()
+    cu, cv, cw = coef if coef is not None else (None, None, None)
+    cl = np.ascontiguousarray(cells, dtype=np.int32)
+    lib().mdo_sor_relax_cells(C.byref(s), C.c_double(omega), C.c_double(dtg), _ptr(cu), _ptr(cv), _ptr(cw),
+                              _ptr(rhs), _ptr(phi), cl.ctypes.data_as(C.POINTER(C.c_int)), C.c_int(len(cl)))
+
+
 def new_fields(g):
     f = {k: g.zeros() for k in ("un", "vn", "wn", "pn", "us", "vs", "ws")}
     for k in ("oru", "orv", "orw"):
@@ -425,3 +459,25 @@ def ref_tridiag(g, hat):
                           C.c_double(g.ly), C.c_double(g.lz), h.ctypes.data_as(_P),
                           cp.ctypes.data_as(_P), di.ctypes.data_as(_P))
     return h, cp, di
+
+
+def ref_run_redblack(g, f, coef, niter, sor, nsteps, dt, cflmax=0.0, dtmax=1e-3):
+    """The reference's own time loop (main.f90:64-97, red-black build) run by ref_bridge_rb over its
+    modules: per step three RK stages of momentum / pressure_projection / apply_bc_redblack with
+    step.f90's rk_alpha, rk_beta, rk_gamma, then the CFL-driven dt update.  Returns (dts, cfls)."""
+    per = np.array(g.periodic, np.int32)
+    bt = np.array(g.bc_type, np.int32)
+    forcing = np.array(g.forcing, np.float64)
+    if coef is None:
+        coef = (g.zeros(), g.zeros(), g.zeros())
+    dts = np.zeros(nsteps)
+    cfls = np.zeros(nsteps)
+    ref_rb().ref_run_redblack(C.c_int(g.nx), C.c_int(g.ny), C.c_int(g.nz), C.c_double(g.lx),
+                              C.c_double(g.ly), C.c_double(g.lz), C.c_double(g.re), _ptr(forcing), _ci(per),
+                              _ci(bt), _ptr(g.bc_value), C.c_int(niter), C.c_double(sor), C.c_int(nsteps),
+                              C.c_double(dt), C.c_double(cflmax), C.c_double(dtmax),
+                              _ptr(coef[0]), _ptr(coef[1]), _ptr(coef[2]),
+                              _ptr(f["un"]), _ptr(f["vn"]), _ptr(f["wn"]), _ptr(f["pn"]),
+                              _ptr(f["us"]), _ptr(f["vs"]), _ptr(f["ws"]),
+                              _ptr(f["oru"]), _ptr(f["orv"]), _ptr(f["orw"]), _ptr(dts), _ptr(cfls))
+    return list(dts), list(cfls)

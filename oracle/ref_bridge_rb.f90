@@ -3,12 +3,15 @@
 ! oracle/Makefile.  Test infrastructure only: it lets the test suite run the reference's own
 ! momentum (step.f90:27), apply_bc_redblack (boundary.f90:173), pressure_projection_redblack
 ! (pressure_redblack.f90:27), set_ibm_coeff (ibm.f90:133) and get_cfl (step.f90:210) on
-! arbitrary inputs and compare them with the C oracle.
+! arbitrary inputs and compare them with the C oracle, and run the time loop of main.f90:64-97
+! (RK3 stages with the module's own rk_alpha/rk_beta/rk_gamma, the pressure_projection dispatch
+! of pressure_solver.f90:48-63, and the CFL-driven dt update) over those procedures.
 module ref_bridge_rb
     use, intrinsic :: iso_c_binding
     use :: init, only: grid_type, field_type, init_grid, finalize_grid
     use :: boundary, only: boundary_type, init_bc, apply_bc_redblack
-    use :: step, only: momentum, get_cfl
+    use :: step, only: momentum, get_cfl, rk_alpha, rk_beta, rk_gamma
+    use :: pressure_solver, only: init_pressure_solver, pressure_projection, destroy_pressure_solver
     use :: ibmm, only: ibm_type, init_ibm, set_ibm_coeff
     use :: pressure_workspace, only: pressure_solver_type
     use :: pressure_redblack, only: pressure_projection_redblack
@@ -167,5 +170,72 @@ contains
         call set_ibm_coeff(g, ibm, ibm%coef_u, dix, diy, diz)
         coeff = ibm%coef_u
     end subroutine ref_set_ibm_coeff
+
+    ! The time loop of main.f90:64-97 in the red-black build: per step t += dt, three RK stages of
+    ! momentum -> pressure_projection -> apply_bc_redblack (main.f90:68-89, no apply_bc under
+    ! USE_REDBLACK), then get_cfl and dt = min(cflmax/cfl, dtmax) when cflmax > 0 (main.f90:91-97).
+    ! dts(n) receives the dt step n ran with, cfls(n) the CFL computed after it (0 when not due).
+    subroutine ref_run_redblack(nx, ny, nz, lx, ly, lz, re, forcing, periodic, bctype, bcvalue, &
+            niter, sor, nsteps, dt0, cflmax, dtmax, cu, cv, cw, &
+            un, vn, wn, pn, us, vs, ws, oru, orv, orw, dts, cfls) bind(C, name="ref_run_redblack")
+        integer(c_int), value :: nx, ny, nz, niter, nsteps
+        real(c_double), value :: lx, ly, lz, re, sor, dt0, cflmax, dtmax
+        real(c_double), intent(in) :: forcing(3)
+        integer(c_int), intent(in) :: periodic(3), bctype(3,0:1,0:3)
+        real(c_double), intent(in) :: bcvalue(3,0:1,0:3)
+        real(c_double), intent(in) :: cu(0:nx+1,0:ny+1,0:nz+1), cv(0:nx+1,0:ny+1,0:nz+1)
+        real(c_double), intent(in) :: cw(0:nx+1,0:ny+1,0:nz+1)
+        real(c_double), intent(inout) :: un(0:nx+1,0:ny+1,0:nz+1), vn(0:nx+1,0:ny+1,0:nz+1)
+        real(c_double), intent(inout) :: wn(0:nx+1,0:ny+1,0:nz+1), pn(0:nx+1,0:ny+1,0:nz+1)
+        real(c_double), intent(inout) :: us(0:nx+1,0:ny+1,0:nz+1), vs(0:nx+1,0:ny+1,0:nz+1)
+        real(c_double), intent(inout) :: ws(0:nx+1,0:ny+1,0:nz+1)
+        real(c_double), intent(inout) :: oru(nx,ny,nz), orv(nx,ny,nz), orw(nx,ny,nz)
+        real(c_double), intent(out) :: dts(nsteps), cfls(nsteps)
+        type(grid_type) :: g
+        type(field_type) :: f
+        type(ibm_type) :: ibm
+        type(boundary_type) :: bc
+        type(pressure_solver_type) :: ps
+        integer :: i, rkStage
+        real(c_double) :: dt_alpha, dt_beta, dt_gamma
+        call make_grid(g, nx, ny, nz, lx, ly, lz, re, forcing)
+        g%dt = dt0
+        g%cflmax = cflmax
+        g%dtmax = dtmax
+        call make_bc(bc, periodic, bctype, bcvalue)
+        call alloc_fields(f, nx, ny, nz)
+        call init_ibm(ibm, g)
+        ibm%coef_u = cu; ibm%coef_v = cv; ibm%coef_w = cw
+        f%un = un; f%vn = vn; f%wn = wn; f%pn = pn
+        f%us = us; f%vs = vs; f%ws = ws
+        f%oldrhsu = oru; f%oldrhsv = orv; f%oldrhsw = orw
+        ps%nIter = niter
+        ps%sor = sor
+        call init_pressure_solver(ps, g)
+        do i = 1, nsteps
+            g%t_current = g%t_current + g%dt
+            dts(i) = g%dt
+            do rkStage = 1, 3
+                dt_alpha = g%dt*rk_alpha(rkStage)
+                dt_beta  = g%dt*rk_beta(rkStage)
+                dt_gamma = g%dt*rk_gamma(rkStage)
+                call momentum(f, g, dt_alpha, dt_beta, dt_gamma, ibm, bc)
+                call pressure_projection(ps, f, g, dt_gamma, ibm, bc)
+                call apply_bc_redblack(f, g, bc)
+            end do
+            cfls(i) = 0.0d0
+            if (g%cflmax > 0.0d0) then
+                g%cfl = get_cfl(f, g)
+                cfls(i) = g%cfl
+            end if
+            if (g%cflmax > 0.0d0 .and. g%cfl > 0.0d0) then
+                g%dt = min(g%cflmax/g%cfl, g%dtmax)
+            end if
+        end do
+        call destroy_pressure_solver(ps)
+        un = f%un; vn = f%vn; wn = f%wn; pn = f%pn
+        us = f%us; vs = f%vs; ws = f%ws
+        oru = f%oldrhsu; orv = f%oldrhsv; orw = f%oldrhsw
+    end subroutine ref_run_redblack
 
 end module ref_bridge_rb

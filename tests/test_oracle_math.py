@@ -290,3 +290,70 @@ def test_wavy_geometry_bisection_matches_closed_form():
             else:
                 assert cv[1, j, i] == 0
     assert checked > 20
+
+
+# --------------------------------------------------------------------------- RK3 and colouring
+def test_rk3_constants_closed_form():
+    """The low-storage RK3 of step.f90:21-23: gamma = alpha + beta per stage, sum(gamma) = 1 (the
+    stages advance one full dt), and the exact rationals 8/15, 5/12, 3/4 / 0, -17/60, -5/12."""
+    from fractions import Fraction as F
+    a, b, c = O.rk_constants()
+    want_a, want_b = (F(8, 15), F(5, 12), F(3, 4)), (F(0), F(-17, 60), F(-5, 12))
+    for s in range(3):
+        assert a[s] == float(want_a[s]) and b[s] == float(want_b[s])
+        assert F(a[s]) + F(b[s]) - F(c[s]) == 0 or abs(a[s] + b[s] - c[s]) <= 2 ** -52
+        assert c[s] == float(want_a[s] + want_b[s])
+    assert abs(sum(c) - 1.0) <= 2 ** -52
+
+
+def test_rk3_third_order_convergence():
+    """The stage update of momentum (u <- u + dt alpha_s f(u) + dt beta_s f_prev, step.f90) with the
+    oracle's constants integrates y' = -y^2 (y(0) = 1, exact 1/(1+t)) to third order: halving dt
+    divides the error by 8 (a wrong constant drops the order to 1 or 2)."""
+    a, b, _ = O.rk_constants()
+
+    def integrate(n):
+        dt, y = 1.0 / n, 1.0
+        for _ in range(n):
+            fprev = 0.0
+            for s in range(3):
+                fy = -y * y
+                y = y + dt * a[s] * fy + dt * b[s] * fprev
+                fprev = fy
+        return abs(y - 0.5)
+
+    e = [integrate(n) for n in (16, 32, 64)]
+    for e0, e1 in zip(e, e[1:]):
+        assert 7.0 < e0 / e1 < 9.0, e
+
+
+def test_colouring_order_independence():
+    """Red-black colouring: the cells of one colour (i+j+k parity) do not read each other, so
+    relaxing them one by one in any order gives sor_sweep's result bit for bit.  Controls: both
+    colours together, or a wrong parity (i+j), depend on the order."""
+    rng = np.random.default_rng(17)
+    g = O.Grid(8, 6, 4, lx=1.0, ly=1.5, lz=0.5, periodic=(1, 0, 1))
+    coef = tuple(np.abs(rng.standard_normal(g.gshape)) * 20 for _ in range(3))
+    rhs = rng.standard_normal(g.ishape)
+    phi0 = rng.standard_normal(g.ishape)
+    cells = np.array([(i, j, k) for k in range(1, 5) for j in range(1, 7) for i in range(1, 9)])
+    par = cells.sum(axis=1) % 2
+
+    def relax(sel, order_seed):
+        phi = phi0.copy()
+        c = cells[sel]
+        c = c[np.random.default_rng(order_seed).permutation(len(c))]
+        O.sor_relax_cells(g, coef, 1.8, 0.01, rhs, phi, c)
+        return phi
+
+    for color in (0, 1):
+        swept = phi0.copy()
+        O.sor_sweep(g, coef, color, 1.8, 0.01, rhs, swept)
+        sel = par == (color + 1) % 2   # colour c: the cells with i + j + k + c odd
+        for seed in range(4):
+            assert np.array_equal(relax(sel, seed), swept)
+        assert not np.array_equal(swept, phi0)
+    everything = np.ones(len(cells), bool)
+    assert not np.array_equal(relax(everything, 0), relax(everything, 1))
+    wrong = (cells[:, 0] + cells[:, 1]) % 2 == 0
+    assert not np.array_equal(relax(wrong, 0), relax(wrong, 1))

@@ -111,3 +111,63 @@ def test_tridiag_bit_exact(shape):
     assert np.array_equal(cp, rcp.real) and not rcp.imag.any()
     assert np.array_equal(di, rdi.real) and not rdi.imag.any()
     assert np.array_equal(mine, ref)
+
+
+def _wavy_state(g, seed=0, amp=0.05):
+    """Physical start: two-sided wavy-wall coefficients and the oracle's seeded initial field."""
+    geo = O.Geometry(kind=1, A=0.1, lam=1.0, half_height=1.0)
+    coef = O.ibm_coefs(g, geo)
+    f = O.new_fields(g)
+    O.initial_field(g, f, seed, amp, geo)   # fills the predictor fields
+    for a, b in (("un", "us"), ("vn", "vs"), ("wn", "ws")):
+        f[a][...] = f[b]
+    O.apply_bc_redblack(g, f)
+    return f, coef
+
+
+@pytest.mark.parametrize("per", [(1, 0, 1), (1, 1, 1), (0, 0, 0)])
+@pytest.mark.parametrize("niter,sor", [(1, 1.5), (3, 1.8)])
+def test_rk3_time_loop_bit_exact(per, niter, sor):
+    """The oracle's RK3 stage composition (rk3_step, SOLA build) against the reference's own time loop
+    (main.f90:64-89 over momentum / pressure_projection / apply_bc_redblack with step.f90's RK3
+    constants), two steps from random fields, coefficients and BC tables."""
+    rng = np.random.default_rng(21)
+    g = O.Grid(8, 6, 4, lx=1.0, ly=1.5, lz=0.5, re=300.0, forcing=(0.7, 0.1, -0.2), periodic=per)
+    _bc_table(g, rng)
+    f = O.random_fields(g, rng)
+    coef = tuple(np.abs(rng.standard_normal(g.gshape)) * 20 for _ in range(3))
+    f2 = {k: v.copy() for k, v in f.items()}
+    dts, cfls, _ = O.run_steps(g, f, coef, 2, 0.01, solver=1, niter=niter, omega=sor)
+    rdts, rcfls = O.ref_run_redblack(g, f2, coef, niter, sor, 2, 0.01)
+    assert dts == rdts and cfls == rcfls == [0.0, 0.0]
+    _same(f, f2, f.keys())
+
+
+def test_rk3_time_loop_wavy_channel_bit_exact():
+    """Same on the wavy channel (IBM walls, seeded Poiseuille start), three steps."""
+    g = O.Grid(16, 12, 8, lx=4.0, ly=2.2, lz=2.0, re=2000.0, forcing=(0.0, 0.0, 0.0), periodic=(1, 0, 1))
+    f, coef = _wavy_state(g)
+    f2 = {k: v.copy() for k, v in f.items()}
+    O.run_steps(g, f, coef, 3, 5e-4, solver=1, niter=3, omega=1.8)
+    O.ref_run_redblack(g, f2, coef, 3, 1.8, 3, 5e-4)
+    _same(f, f2, f.keys())
+    assert np.abs(f["un"]).max() > 0.5   # a real flow, not a zero field
+
+
+@pytest.mark.parametrize("cflmax,dtmax", [(0.05, 1e-2), (0.5, 2e-3)])
+def test_adaptive_dt_bit_exact(cflmax, dtmax):
+    """main.f90:91-97: after each step get_cfl, then dt = min(cflmax / cfl, dtmax).  The dt sequence,
+    the CFL values and the fields equal the reference loop's bit for bit; the first parameter set
+    makes the CFL limit bind, the second the dtmax cap."""
+    g = O.Grid(16, 12, 8, lx=4.0, ly=2.2, lz=2.0, re=2000.0, forcing=(0.0, 0.0, 0.0), periodic=(1, 0, 1))
+    f, coef = _wavy_state(g, seed=3)
+    f2 = {k: v.copy() for k, v in f.items()}
+    dts, cfls, _ = O.run_steps(g, f, coef, 4, 1e-3, cflmax=cflmax, dtmax=dtmax, solver=1, niter=3, omega=1.8)
+    rdts, rcfls = O.ref_run_redblack(g, f2, coef, 3, 1.8, 4, 1e-3, cflmax=cflmax, dtmax=dtmax)
+    assert dts == rdts and cfls == rcfls
+    assert all(c > 0 for c in cfls)
+    _same(f, f2, f.keys())
+    if cflmax == 0.05:
+        assert dts[1] == cflmax / cfls[0] < dtmax   # the CFL limit sets dt
+    else:
+        assert dts[1] == dtmax                      # the cap sets dt
