@@ -159,6 +159,12 @@ int mdx_pressure_projection(mdx_handle *h, double dt_gamma, mdx_solve_stats *sta
 /* step.f90:210 get_cfl(f, g) */
 int mdx_get_cfl(mdx_handle *h, double *cfl);
 
+/* io.f90:184-222 center_vel: cell-centred velocity uc = (u(i)+u(i+1))/2, vc, wc likewise, computed
+ * on the device from the current fields and copied to the host arrays uc/vc/wc(1:nx,1:ny,1:nz)
+ * (interior layout; NULL skips an array).  The VTK writer of the driver (io.f90:224-259) formats
+ * them.  Allocates a device buffer of 3 nx ny nz doubles on its first call. */
+int mdx_center_velocity(mdx_handle *h, double *uc, double *vc, double *wc);
+
 /* ||div u||_inf over interior cells (FP64) */
 int mdx_div_norm(mdx_handle *h, double *div_inf);
 /* constant flow rate: bulk-velocity correction of u* for the stage with dt_gamma; returns du */

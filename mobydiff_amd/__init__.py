@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "csrc", "libmobydiff_hip.so")
+IOLIBPATH = os.path.join(HERE, "csrc", "libmobydiff_io.so")
 
 SOLVER_FFT, SOLVER_SOLA, SOLVER_SOR = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
@@ -91,6 +92,52 @@ def lib():
 def _check(rc):
     if rc != 0:
         raise MdxError(rc, lib().mdx_last_error().decode())
+
+
+_iolib = None
+
+
+def io_lib():
+    """Load libmobydiff_io.so (HDF5 field files, include/mobydiff_io.h); raise if it is missing."""
+    global _iolib
+    if _iolib is None:
+        if not os.path.exists(IOLIBPATH):
+            raise RuntimeError(f"{IOLIBPATH} is not built: run make -C mobydiff_amd/csrc")
+        _iolib = C.CDLL(IOLIBPATH)
+        _iolib.mdx_io_last_error.restype = C.c_char_p
+    return _iolib
+
+
+def _io_check(rc):
+    if rc != 0:
+        raise MdxError(rc, io_lib().mdx_io_last_error().decode())
+
+
+def write_field_h5(path, nx, ny, nz, lx, ly, lz, re, dt, t_current, un, vn, wn, pn):
+    """write_field's file (io.f90:105-137, field_hdf5.c:149-184): ghosted un, vn, wn, pn of shape
+    (nz+2, ny+2, nx+2) and the twelve attributes; dx = lx/nx etc. as finalize_grid sets them."""
+    dx, dy, dz = lx / nx, ly / ny, lz / nz
+    _io_check(io_lib().mdx_h5_write_field(
+        os.fsencode(path), C.c_int(nx), C.c_int(ny), C.c_int(nz), C.c_double(lx), C.c_double(ly),
+        C.c_double(lz), C.c_double(dx), C.c_double(dy), C.c_double(dz), C.c_double(re), C.c_double(dt),
+        C.c_double(t_current), _p(un), _p(vn), _p(wn), _p(pn)))
+
+
+def read_field_h5(path, nx, ny, nz):
+    """read_field (io.f90:139-166, field_hdf5.c:186-204): the four ghosted fields of the file."""
+    out = {k: np.zeros((nz + 2, ny + 2, nx + 2)) for k in ("un", "vn", "wn", "pn")}
+    _io_check(io_lib().mdx_h5_read_field(os.fsencode(path), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+                                         _p(out["un"]), _p(out["vn"]), _p(out["wn"]), _p(out["pn"])))
+    return out
+
+
+def read_field_info(path):
+    """(nx, ny, nz, dt, t_current) of a field file"""
+    n = [C.c_int32() for _ in range(3)]
+    dt, t = C.c_double(), C.c_double()
+    _io_check(io_lib().mdx_h5_read_field_info(os.fsencode(path), *[C.byref(q) for q in n], C.byref(dt),
+                                              C.byref(t)))
+    return n[0].value, n[1].value, n[2].value, dt.value, t.value
 
 
 def _p(a):
@@ -231,6 +278,24 @@ class Channel:
         v = C.c_double()
         _check(lib().mdx_get_cfl(self.h, C.byref(v)))
         return v.value
+
+    def center_velocity(self):
+        """io.f90:184-222 center_vel on the device: (uc, vc, wc), each of shape (nz, ny, nx)"""
+        uc, vc, wc = (np.zeros(self.ishape) for _ in range(3))
+        _check(lib().mdx_center_velocity(self.h, _p(uc), _p(vc), _p(wc)))
+        return uc, vc, wc
+
+    def write_field(self, path, dt, t_current):
+        """write_field (io.f90:105-137): download un, vn, wn, pn and write the HDF5 field file"""
+        f = self.download(("un", "vn", "wn", "pn"))
+        c = self.cfg
+        write_field_h5(path, c.nx, c.ny, c.nz, c.lx, c.ly, c.lz, c.re, dt, t_current,
+                       f["un"], f["vn"], f["wn"], f["pn"])
+
+    def read_field(self, path):
+        """read_field (io.f90:139-166): read un, vn, wn, pn from the file and upload them"""
+        f = read_field_h5(path, self.nx, self.ny, self.nz)
+        self.upload(**f)
 
     def div_norm(self):
         v = C.c_double()

@@ -199,7 +199,7 @@ static void free_all(mdx_handle *h) {
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
                     h->red_part, h->red_max, h->max_part, h->rb_faces, h->phi_hist, h->res64,
                     h->fft_phi, h->fft_res, h->fft_mean, h->hat32, h->cp32, h->den32, h->arow32, h->tw_x32,
-                    h->tw_z32, h->rb_xl32, h->rb_xl64};
+                    h->tw_z32, h->rb_xl32, h->rb_xl64, h->cvel};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -772,6 +772,18 @@ extern "C" int mdx_div_norm(mdx_handle *h, double *div_inf) {
     MDX_HIP(launch_reset_max(h));
     MDX_HIP(launch_div_max(h, h->un, h->vn, h->wn));
     return read_norm(h, div_inf);
+}
+
+extern "C" int mdx_center_velocity(mdx_handle *h, double *uc, double *vc, double *wc) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    if (!h->cvel) MDX_HIP(dalloc(&h->cvel, 3 * h->ni));
+    double *uc_d = h->cvel, *vc_d = h->cvel + h->ni, *wc_d = h->cvel + 2 * h->ni;
+    MDX_HIP(launch_center_vel(h, uc_d, vc_d, wc_d));
+    int r;
+    if ((r = copy_out(h, uc, uc_d, h->ni)) || (r = copy_out(h, vc, vc_d, h->ni)) || (r = copy_out(h, wc, wc_d, h->ni)))
+        return r;
+    MDX_HIP(hipStreamSynchronize(h->stream));
+    return MDX_OK;
 }
 
 extern "C" int mdx_get_cfl(mdx_handle *h, double *cfl) {

@@ -220,6 +220,7 @@ struct mdx_handle {
     // (0: no history yet) and the count of SOR projections (stage = count mod 3)
     double *phi_hist = nullptr;
     double *res64 = nullptr;       // residual field of the previous-Phi guess (network input)
+    double *cvel = nullptr;        // cell-centred velocity for output (3 interior arrays, allocated on first use)
     double hist_dtg[3] = {0.0, 0.0, 0.0};
     // learned-guess gate per stage: consecutive solves the network's start lost, solves left to
     // skip, whether the current solve starts from the network's guess, and the sweeps of the last
@@ -249,6 +250,7 @@ hipError_t launch_apply_bc_fft(mdx_handle *h);
 hipError_t launch_apply_bc_redblack(mdx_handle *h);
 hipError_t launch_div_max(mdx_handle *h, const double *u, const double *v, const double *w);
 hipError_t launch_cfl_max(mdx_handle *h);
+hipError_t launch_center_vel(mdx_handle *h, double *uc, double *vc, double *wc);
 hipError_t launch_bulk_sum(mdx_handle *h, const double *u, double dtg, double *out_dev);
 hipError_t launch_flow_shift(mdx_handle *h, double dtg, const double *du_dev);
 hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int use_geo,

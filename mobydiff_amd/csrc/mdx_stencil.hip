@@ -584,6 +584,33 @@ hipError_t launch_cfl_max(mdx_handle *h) {
     return e != hipSuccess ? e : launch_max_final(h, 1024);
 }
 
+// io.f90:184-222 center_vel: the cell-centred velocity of each interior cell, the mean of its two
+// faces in each direction (the ghost layer supplies the far face of the last cell).  Output in the
+// interior layout uc(1:nx,1:ny,1:nz).  56 B per cell: six face reads (three of them the
+// neighbour's, shared along the row or with the next row/plane) and three writes.
+__global__ __launch_bounds__(BX * BY) void center_vel_kernel(Dims d, const double *__restrict__ un,
+                                                             const double *__restrict__ vn,
+                                                             const double *__restrict__ wn, double *__restrict__ uc,
+                                                             double *__restrict__ vc, double *__restrict__ wc) {
+    int tx = 0, ty = 0, tz = 0;
+    if (!xcd_tile(d.nx, d.ny, d.nz, BX, BY, tx, ty, tz)) return;
+    const int i = tx * BX + threadIdx.x + 1;
+    const int j = ty * BY + threadIdx.y + 1;
+    const int k = tz + 1;
+    if (i > d.nx || j > d.ny) return;
+    const size_t c = d.g(i, j, k), o = d.in(i, j, k);
+    uc[o] = 0.5 * (un[c] + un[c + 1]);
+    vc[o] = 0.5 * (vn[c] + vn[d.g(i, j + 1, k)]);
+    wc[o] = 0.5 * (wn[c] + wn[d.g(i, j, k + 1)]);
+}
+
+hipError_t launch_center_vel(mdx_handle *h, double *uc, double *vc, double *wc) {
+    const unsigned nb = xcd_grid(h->d.nx, h->d.ny, h->d.nz, BX, BY);
+    hipLaunchKernelGGL(center_vel_kernel, dim3(nb), dim3(BX, BY, 1), 0, h->stream, h->d, h->un, h->vn, h->wn, uc,
+                       vc, wc);
+    return hipGetLastError();
+}
+
 // Deterministic two-pass sums over interior u points: S1 = sum u*h_j, S2 = sum mu_u*h_j.
 static constexpr int SUM_BLOCKS = 512;
 static constexpr int SUM_THREADS = 256;

@@ -7,7 +7,8 @@
 !   [pressure] solver = fft | sola | redblack | sor, tol, max_iter, check_every,
 !              precision = fp64 | mixed, inner_iters
 !   [ibm] geometry = bump | wavy | none, a, lambda, half_height, amp_x, phase_x, n_wave_x, second_order
-!   [init] profile = zero | poiseuille, seed, amplitude       [output] dump_final = <path>
+!   [init] profile = zero | poiseuille, seed, amplitude, restart = <HDF5 field file>
+!   [output] dump_final = <path>
 module config
     use, intrinsic :: iso_c_binding
     use :: init, only: grid_type, finalize_grid
@@ -198,6 +199,7 @@ contains
             case ("profile"); mdx_init_poiseuille = to_lower(unquote(v)) == "poiseuille"
             case ("seed"); call get_int(v, mdx_init_seed, ln)
             case ("amplitude", "amp"); call get_real(v, mdx_init_amp, ln)
+            case ("restart", "restart_file", "field_file"); mdx_restart_file = unquote(v)
             end select
         case ("boundary")
             call boundary_value(k, v, ln)

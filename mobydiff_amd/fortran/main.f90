@@ -1,15 +1,17 @@
 ! main.f90 -- the reference driver (main.f90:10-120) with its setup order, RK3 stage loop, CFL/dt
-! update and timing line, calling the GPU path through the drop-in modules.  Differences: the
-! pressure solver is chosen at run time ([pressure] solver) instead of by USE_REDBLACK, the step
-! loop synchronises the device before the clock stops (the library is stream-asynchronous), an
-! optional Poiseuille + perturbation initial field ([init]) replaces the zero start, and VTK/HDF5
-! output is replaced by an optional raw dump of the final fields ([output] dump_final).
+! update, VTK/HDF5 output cadence and timing line, calling the GPU path through the drop-in modules.
+! Differences: the pressure solver is chosen at run time ([pressure] solver) instead of by
+! USE_REDBLACK, the step loop synchronises the device before the clock stops (the library is
+! stream-asynchronous), an optional Poiseuille + perturbation initial field ([init] profile) or a
+! field file ([init] restart, read by io's read_field) replaces the zero start, and an optional
+! raw dump of the final fields ([output] dump_final) follows the loop.
 program main
     use, intrinsic :: iso_fortran_env, only: int64
     use, intrinsic :: iso_c_binding
     use :: init
     use :: config
     use :: boundary
+    use :: io
     use :: step
     use :: pressure_workspace
     use :: pressure_solver
@@ -31,6 +33,7 @@ program main
     type(boundary_type) :: bc
     type(pressure_solver_type) :: ps
     type(ibm_type) :: ibm
+    type(output_workspace_type) :: out
     type(mdx_geometry), target :: geo
 
     call get_command_argument(1, input_file, status=arg_status)
@@ -52,6 +55,7 @@ program main
     print *, "initialising fields..."
     call init_field(f, g)
     call enter_field_data(f, g)
+    call init_output_workspace(out, g, output_interval)
 
     print *, "initialising pressure solver..."
     call init_pressure_solver(ps, g)
@@ -74,6 +78,7 @@ program main
                            "init_fields")
         end if
     end if
+    if (len_trim(mdx_restart_file) > 0) call restart_from(f, g, trim(mdx_restart_file))
 
     sweeps = 0
     print *, "main loop starting..."
@@ -99,12 +104,15 @@ program main
             end if
         end do
 
-        need_cfl = (g%cflmax > 0.0d0) .or. (output_interval > 0 .and. mod(i, max(output_interval, 1)) == 0)
+        need_cfl = (g%cflmax > 0.0d0) .or. output_is_due(i, output_interval)
         if (need_cfl) g%cfl = get_cfl(f, g)
 
         if (g%cflmax > 0.0d0 .and. g%cfl > 0.0d0) then
             g%dt = min(g%cflmax/g%cfl, g%dtmax)
         end if
+
+        if (output_interval > 0) call maybe_write_vtk(out, f, g, i, output_interval, output_prefix)
+        if (field_interval > 0) call maybe_write_field(f, g, i, field_interval, field_prefix)
     end do
     call mdx_check(mdx_synchronize(mdx_h), "synchronize")
     call system_clock(count=loop_clock_end)
@@ -120,6 +128,7 @@ program main
 
     call destroy_pressure_solver(ps)
     call exit_ibm_data(ibm, g)
+    call destroy_output_workspace(out, g)
     call exit_field_data(f, g)
     if (len_trim(mdx_dump_final) > 0) call dump_fields(trim(mdx_dump_final))
 
@@ -155,6 +164,7 @@ contains
         write(*,'(A,1X,I0,1X,A,1X,I0,1X,A)') "output", output_interval, trim(output_prefix), field_interval, &
             trim(field_prefix)
         write(*,'(A,1X,A)') "dump_final", trim(mdx_dump_final)
+        write(*,'(A,1X,A)') "restart", trim(mdx_restart_file)
     end subroutine print_config
 
     ! raw little-endian FP64 dump: header (nx, ny, nz as int32), then un, vn, wn, pn ghosted

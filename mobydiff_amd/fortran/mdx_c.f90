@@ -45,6 +45,7 @@ module mdx_c
     integer, save :: mdx_init_seed = 0
     real(c_double), save :: mdx_init_amp = 0.0d0
     character(len=256), save :: mdx_dump_final = ""
+    character(len=256), save :: mdx_restart_file = ""   ! [init] restart: field file to start from
 
     interface
         function mdx_last_error() bind(C, name="mdx_last_error") result(p)
@@ -145,6 +146,47 @@ module mdx_c
             real(c_double), value :: amp, dt
             integer(c_int) :: rc
         end function
+        function mdx_center_velocity(h, uc, vc, wc) bind(C, name="mdx_center_velocity") result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(inout) :: uc(*), vc(*), wc(*)
+            integer(c_int) :: rc
+        end function
+        function mdx_get_y_coordinates(h, yc, yv) bind(C, name="mdx_get_y_coordinates") result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(inout) :: yc(*), yv(*)
+            integer(c_int) :: rc
+        end function
+        ! libmobydiff_io (include/mobydiff_io.h): HDF5 field files
+        function mdx_io_last_error() bind(C, name="mdx_io_last_error") result(p)
+            import :: c_ptr
+            type(c_ptr) :: p
+        end function
+        function mdx_h5_write_field(path, nx, ny, nz, lx, ly, lz, dx, dy, dz, re, dt, t_current, &
+                un, vn, wn, pn) bind(C, name="mdx_h5_write_field") result(rc)
+            import :: c_int, c_char, c_double
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), value :: nx, ny, nz
+            real(c_double), value :: lx, ly, lz, dx, dy, dz, re, dt, t_current
+            real(c_double), intent(in) :: un(*), vn(*), wn(*), pn(*)
+            integer(c_int) :: rc
+        end function
+        function mdx_h5_read_field(path, nx, ny, nz, un, vn, wn, pn) bind(C, name="mdx_h5_read_field") result(rc)
+            import :: c_int, c_char, c_double
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), value :: nx, ny, nz
+            real(c_double), intent(inout) :: un(*), vn(*), wn(*), pn(*)
+            integer(c_int) :: rc
+        end function
+        function mdx_h5_read_field_info(path, nx, ny, nz, dt, t_current) &
+                bind(C, name="mdx_h5_read_field_info") result(rc)
+            import :: c_int, c_char, c_double, c_int32_t
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int32_t), intent(out) :: nx, ny, nz
+            real(c_double), intent(out) :: dt, t_current
+            integer(c_int) :: rc
+        end function
         function c_strlen(s) bind(C, name="strlen") result(n)
             import :: c_ptr, c_size_t
             type(c_ptr), value :: s
@@ -156,17 +198,21 @@ contains
 
     function last_error_text() result(msg)
         character(len=:), allocatable :: msg
-        type(c_ptr) :: p
+        msg = c_text(mdx_last_error())
+    end function last_error_text
+
+    function c_text(p) result(msg)
+        type(c_ptr), intent(in) :: p
+        character(len=:), allocatable :: msg
         character(kind=c_char), pointer :: chars(:)
         integer :: n, i
-        p = mdx_last_error()
         n = int(c_strlen(p))
         call c_f_pointer(p, chars, [n])
         allocate(character(len=n) :: msg)
         do i = 1, n
             msg(i:i) = chars(i)
         end do
-    end function last_error_text
+    end function c_text
 
     subroutine mdx_check(rc, what)
         integer(c_int), intent(in) :: rc

@@ -914,15 +914,66 @@ double mdo_div_inf(const mdo_grid *g, const double *un, const double *vn, const 
     return m;
 }
 
-double mdo_bulk_u(const mdo_grid *g, const double *u, double height) {
-    const int nx = g->nx, ny = g->ny, nz = g->nz;
-    double s = 0.0;
-    for (int k = 1; k <= nz; ++k)
-        for (int j = 1; j <= ny; ++j) {
-            double h = g->ihc ? 1.0 / g->ihc[j] : g->dy;
-            for (int i = 1; i <= nx; ++i) s += u[GI(i, j, k)] * h;
+/* The two sums of the constant-flow-rate control, S1 = sum u h_j and S2 = sum mu_u h_j over interior
+ * u points, in one fixed order that is part of their definition (this control has no reference
+ * counterpart): 512 partial sums, partial b taking the x rows r = b, b + 512, ... (r = (j-1) +
+ * ny (k-1)) split over 256 column lanes (lane t: columns t+1, t+257, ...), each lane adding its
+ * values row by row; the lanes of a partial, and then the partials (lane t: partials t, t + 256),
+ * are folded by a pairwise tree (s[t] += s[t + w], w = 128 ... 1).  The GPU kernels add in this
+ * order too, so the control's shift is the same double on both. */
+#define FR_PARTS 512
+#define FR_LANES 256
+static void fr_fold(double *s) {
+    for (int w = FR_LANES / 2; w > 0; w >>= 1)
+        for (int t = 0; t < w; ++t) s[t] += s[t + w];
+}
+
+static void fr_sums(const mdo_grid *g, const double *u, const double *cu, double dtg, double *S1, double *S2) {
+    const int nx = g->nx, ny = g->ny, nrows = g->ny * g->nz;
+    double p1[FR_PARTS], p2[FR_PARTS];
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < FR_PARTS; ++b) {
+        double s1[FR_LANES], s2[FR_LANES];
+        for (int t = 0; t < FR_LANES; ++t) {
+            double a = 0.0, c2 = 0.0;
+            for (int r = b; r < nrows; r += FR_PARTS) {
+                const int j = r % ny + 1, k = r / ny + 1;
+                const double h = g->ihc ? 1.0 / g->ihc[j] : g->dy;
+                for (int i = t + 1; i <= nx; i += FR_LANES) {
+                    const size_t q = GI(i, j, k);
+                    a += u[q] * h;
+                    c2 += MUQ(cu, q) * h;
+                }
+            }
+            s1[t] = a;
+            s2[t] = c2;
         }
-    return s / ((double)nx * (double)nz * height);
+        fr_fold(s1);
+        fr_fold(s2);
+        p1[b] = s1[0];
+        p2[b] = s2[0];
+    }
+    double s1[FR_LANES], s2[FR_LANES];
+    for (int t = 0; t < FR_LANES; ++t) {
+        double a = 0.0, c2 = 0.0;
+        for (int q = t; q < FR_PARTS; q += FR_LANES) {
+            a += p1[q];
+            c2 += p2[q];
+        }
+        s1[t] = a;
+        s2[t] = c2;
+    }
+    fr_fold(s1);
+    fr_fold(s2);
+    *S1 = s1[0];
+    *S2 = s2[0];
+}
+
+double mdo_bulk_u(const mdo_grid *g, const double *u, double height) {
+    const double dtg = 0.0;
+    double s1, s2;
+    fr_sums(g, u, NULL, dtg, &s1, &s2);
+    return s1 / ((double)g->nx * (double)g->nz * height);
 }
 
 /* Constant flow rate: a uniform x-force increment chosen so that the predictor's bulk velocity
@@ -930,15 +981,12 @@ double mdo_bulk_u(const mdo_grid *g, const double *u, double height) {
 double mdo_flowrate_shift(const mdo_grid *g, double *us, const double *cu, double dtg,
                           double target, double height) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    double ub = mdo_bulk_u(g, us, height);
-    double sm = 0.0;
-    for (int k = 1; k <= nz; ++k)
-        for (int j = 1; j <= ny; ++j) {
-            double h = g->ihc ? 1.0 / g->ihc[j] : g->dy;
-            for (int i = 1; i <= nx; ++i) sm += MUQ(cu, GI(i, j, k)) * h;
-        }
-    sm = sm / ((double)nx * (double)nz * height);
-    double du = (target - ub) / sm;
+    double s1, s2;
+    fr_sums(g, us, cu, dtg, &s1, &s2);
+    const double norm = (double)nx * (double)nz * height;
+    const double ub = s1 / norm;
+    const double sm = s2 / norm;
+    const double du = (target - ub) / sm;
     for (int k = 1; k <= nz; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) us[GI(i, j, k)] = us[GI(i, j, k)] + du * MUQ(cu, GI(i, j, k));

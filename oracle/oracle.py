@@ -59,6 +59,7 @@ def bci(d, side, var):
 _lib = None
 _ref_rb = None
 _ref_fft = None
+_ref_io = None
 
 
 def lib():
@@ -101,6 +102,17 @@ def ref_fft():
             return None
         _ref_fft = C.CDLL(path)
     return _ref_fft
+
+
+def ref_io():
+    """Reference output module (io.f90 + field_hdf5.c), or None when it was not built."""
+    global _ref_io
+    if _ref_io is None:
+        path = os.path.join(HERE, "_ref", "libref_io.so")
+        if not os.path.exists(path):
+            return None
+        _ref_io = C.CDLL(path)
+    return _ref_io
 
 
 class Grid:
@@ -481,3 +493,38 @@ def ref_run_redblack(g, f, coef, niter, sor, nsteps, dt, cflmax=0.0, dtmax=1e-3)
                               _ptr(f["us"]), _ptr(f["vs"]), _ptr(f["ws"]),
                               _ptr(f["oru"]), _ptr(f["orv"]), _ptr(f["orw"]), _ptr(dts), _ptr(cfls))
     return list(dts), list(cfls)
+
+
+# ----------------------------------------------------------------------------- output
+def center_vel(f):
+    """io.f90:184-222: uc(i,j,k) = 0.5 (un(i,j,k) + un(i+1,j,k)), vc with j+1, wc with k+1, over
+    interior cells (arrays of shape (nz, ny, nx))."""
+    un, vn, wn = f["un"], f["vn"], f["wn"]
+    uc = 0.5 * (un[1:-1, 1:-1, 1:-1] + un[1:-1, 1:-1, 2:])
+    vc = 0.5 * (vn[1:-1, 1:-1, 1:-1] + vn[1:-1, 2:, 1:-1])
+    wc = 0.5 * (wn[1:-1, 1:-1, 1:-1] + wn[2:, 1:-1, 1:-1])
+    return uc, vc, wc
+
+
+def ref_write_vtk(g, f, dt, t_current, cfl, step, prefix):
+    """The reference's write_vtk (io.f90:168-182) on these fields: writes <prefix>_<step>.vtk."""
+    ref_io().ref_write_vtk(C.c_int(g.nx), C.c_int(g.ny), C.c_int(g.nz), C.c_double(g.lx), C.c_double(g.ly),
+                           C.c_double(g.lz), C.c_double(g.re), C.c_double(dt), C.c_double(t_current),
+                           C.c_double(cfl), _ptr(f["un"]), _ptr(f["vn"]), _ptr(f["wn"]), C.c_int(step),
+                           os.fsencode(prefix))
+
+
+def ref_write_field(g, f, dt, t_current, step, prefix):
+    """The reference's write_field (io.f90:105-137): writes <prefix>_<step>.h5 through field_hdf5.c."""
+    ref_io().ref_write_field(C.c_int(g.nx), C.c_int(g.ny), C.c_int(g.nz), C.c_double(g.lx), C.c_double(g.ly),
+                             C.c_double(g.lz), C.c_double(g.re), C.c_double(dt), C.c_double(t_current),
+                             _ptr(f["un"]), _ptr(f["vn"]), _ptr(f["wn"]), _ptr(f["pn"]), C.c_int(step),
+                             os.fsencode(prefix))
+
+
+def ref_read_field(g, path):
+    """The reference's read_field (io.f90:139-166)."""
+    out = {k: g.zeros() for k in ("un", "vn", "wn", "pn")}
+    ref_io().ref_read_field(C.c_int(g.nx), C.c_int(g.ny), C.c_int(g.nz), _ptr(out["un"]), _ptr(out["vn"]),
+                            _ptr(out["wn"]), _ptr(out["pn"]), os.fsencode(path))
+    return out
