@@ -29,6 +29,9 @@ LX, LY, LZ = 4.0, 2.2, 2.0
 RE, DT, A, AMP = 2000.0, 5e-4, 0.1, 0.05
 TOL, OMEGA = 1e-8, 1.8
 HBM_PEAK_GBS = 8000.0
+# U-Net parameters trained by mobydiff_amd.train_guess on 20,100 solver pairs of 50 other wavy
+# channels (A uniform in [0.05, 0.15], seeds 0..49); the default start of the learned guess
+WEIGHTS = os.path.join(ROOT, "mobydiff_amd", "weights", "unet_guess_s2_wavy_192x128x96.npz")
 # canonical algorithmic bytes per cell for one full red+black iteration (FP64 88 B, FP32 44 B); a
 # one-colour launch covers half of that, a fused launch all of it
 BYTES_PER_CELL_ITER = {"fp64": 88.0, "fp32": 44.0}
@@ -126,6 +129,21 @@ def momentum_roofline(ch, cells, ibm, dt, workload):
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
+def momentum_in_step(t, cells, ibm, workload):
+    """The momentum kernel inside the timed steps: HIP events on the channel's stream around its
+    launch in every fourth RK stage (mdx_phase_times.momentum_ms over `stages`).  Canonical
+    algorithmic traffic 128 B per cell with IBM, 104 B without."""
+    if t.stages == 0:
+        return None
+    avg_s = t.momentum_ms / t.stages * 1e-3
+    b = (128.0 if ibm else 104.0) * cells
+    ach = b / avg_s / 1e9
+    return {"kernel": "momentum_tiled_kernel", "workload": workload,
+            "measured": f"in the timed steps ({t.stages} sampled launches)", "bound": "hbm",
+            "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": b, "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
 def network_gain(its, started, its_plain):
     """Sweeps of the solves the network started against the same step and stage of the run from the
     plain start (mixed_no_guess, from the same state): the guess's gain where it was used."""
@@ -162,7 +180,8 @@ def flat_fft_production(M, steps, precision=None):
     t = ch.phase_times()
     div = ch.div_norm()
     cells = nx * ny * nz
-    mom = momentum_roofline(ch, cells, False, 5e-4, "flat channel 256x192x256") if precision is None else None
+    mom = momentum_in_step(t, cells, False, "flat channel 256x192x256") if precision is None else None
+    mom_rep = momentum_roofline(ch, cells, False, 5e-4, "flat channel 256x192x256") if precision is None else None
     ch.close()
     gbs = 768.0 * cells * steps / el / 1e9
     label = "FP64" if precision in (None, M.PREC_FP64) else "FP32 transforms/Thomas in FP64 defect correction to 1e-8"
@@ -176,7 +195,7 @@ def flat_fft_production(M, steps, precision=None):
             "step_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(gbs / HBM_PEAK_GBS, 4),
                               "bytes_per_step": 768.0 * cells},
-            "stencil_roofline": mom}
+            "stencil_roofline": mom, "stencil_roofline_repeated_launches": mom_rep}
 
 
 def cpu_baseline(f_host, coef, cores, steps):
@@ -223,9 +242,11 @@ def main():
     ap.add_argument("--inner", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=200)
     ap.add_argument("--no-learned", action="store_true", help="headline without the learned guess")
-    ap.add_argument("--net-weights", default="",
-                    help="U-Net parameters written by mobydiff_amd.train_guess (.npz) instead of training "
-                         "on the warm-up steps")
+    ap.add_argument("--net-weights", default=WEIGHTS if os.path.exists(WEIGHTS) else "",
+                    help="U-Net parameters written by mobydiff_amd.train_guess (.npz); empty: train on the "
+                         "warm-up steps' pairs instead")
+    ap.add_argument("--stride", type=int, default=2,
+                    help="stride of a network trained on the warm-up steps (network grid = grid / stride)")
     ap.add_argument("--flat-steps", type=int, default=10, help="steps of the flat FFT production line (0: skip)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device once (the known-size kernel that "
@@ -259,7 +280,7 @@ def main():
     if pretrained:
         # parameters trained beforehand on other wavy channels (mobydiff_amd.train_guess)
         from mobydiff_amd import train_guess
-        net = M.Net(NX, NY, NZ, seed=0)
+        net = M.Net(NX, NY, NZ, seed=0, stride=train_guess.stride_of(args.net_weights))
         meta = train_guess.load(args.net_weights, net)
         training = dict(meta, weights=os.path.basename(args.net_weights),
                         source="(div u*, previous-step Phi) -> Phi pairs from the mixed SOR solves of "
@@ -268,7 +289,7 @@ def main():
         head.attach_net(net, capture=False)
     elif learned:
         # the network learns from the solver's own converged solves during the warm-up steps
-        net = M.Net(NX, NY, NZ, seed=0)
+        net = M.Net(NX, NY, NZ, seed=0, stride=args.stride)
         net.replay_reserve(3 * args.warmup)
         head.attach_net(net, capture=True)
     for _ in range(args.warmup):
@@ -385,7 +406,11 @@ def main():
         "speedup_vs_plain_fp64": round(v_h / max(v_p, v_pl), 3),
         "roofline": roofline(t_h, "fp32", head.info()),
     }
-    out["stencil_roofline"] = momentum_roofline(head, cells, True, DT, "wavy channel 192x128x96 (IBM)")
+    out["stencil_roofline"] = momentum_in_step(t_h, cells, True, "wavy channel 192x128x96 (IBM)")
+    # the same kernel launched 20 times back to back from one state (its working set then partly
+    # served by the 256 MB MALL)
+    out["stencil_roofline_repeated_launches"] = momentum_roofline(head, cells, True, DT,
+                                                                  "wavy channel 192x128x96 (IBM)")
     if args.flat_steps > 0:
         out["flat_fft_production"] = flat_fft_production(M, args.flat_steps)
         out["flat_fft_production_mixed"] = flat_fft_production(M, args.flat_steps, M.PREC_MIXED)

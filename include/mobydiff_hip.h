@@ -118,6 +118,8 @@ typedef struct {
                                 2 x iterations x nx ny nz (a one-colour launch covers nx ny nz of it,
                                 a fused launch 2 nx ny nz, an LDS-resident batch 2 x its iterations
                                 x nx ny nz) */
+    double momentum_ms;      /* the momentum kernel alone (its launch inside the step, between events
+                                on the handle's stream), over the `stages` sample */
 } mdx_phase_times;
 
 /* error text of the last failing call on this thread */
@@ -208,6 +210,21 @@ int mdx_get_info(mdx_handle *h, char *buf, int32_t len);
 typedef struct mdx_net mdx_net;
 int64_t mdx_net_param_count(void);
 int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, mdx_net **out);
+/* a network serving the nx x ny x nz solver grid that runs on that grid coarsened by `stride`
+ * (1, 2 or 4) in every direction: its inputs are the means over stride^3 cells, its output is
+ * prolonged trilinearly (cell-centred, the outermost cells clamped to the nearest voxel) and added
+ * to Phi_0.  The network grid needs dimensions in multiples of 4 and (voxels / 64) a multiple of 64.
+ * mdx_net_create is stride 1.  The explicit-input calls (forward_host, accumulate_grad_host) take
+ * arrays of the network grid. */
+int mdx_net_create_strided(int32_t nx, int32_t ny, int32_t nz, int32_t stride, uint64_t seed, mdx_net **out);
+/* the network grid (level 0) and its stride */
+int mdx_net_grid(mdx_net *net, int32_t *w, int32_t *h, int32_t *d, int32_t *stride);
+/* the transfer operators between the solver grid (fine, FP64, interior layout) and the network grid
+ * (coarse, FP32), run by the kernels of the solver coupling with unit scale: restriction = mean over
+ * each voxel's stride^3 cells; prolongation = the trilinear interpolation the guess uses.  Allocate
+ * a temporary device copy of the fine field. */
+int mdx_net_restrict_host(mdx_net *net, const double *fine, float *coarse);
+int mdx_net_prolong_host(mdx_net *net, const float *coarse, double *fine);
 /* detaches the network from every handle it is attached to, then frees it */
 int mdx_net_destroy(mdx_net *net);
 /* c0: target scale (Phi = output * rms(div u*) * c0); get may pass NULL, set ignores c0 <= 0 */

@@ -60,7 +60,8 @@ class StepStats(C.Structure):
 class PhaseTimes(C.Structure):
     _fields_ = [("stencil_ms", C.c_double), ("poisson_ms", C.c_double),
                 ("projection_ms", C.c_double), ("guess_ms", C.c_double), ("stages", C.c_int64),
-                ("sweep_ms", C.c_double), ("sweep_launches", C.c_int64), ("sweep_cells", C.c_int64)]
+                ("sweep_ms", C.c_double), ("sweep_launches", C.c_int64), ("sweep_cells", C.c_int64),
+                ("momentum_ms", C.c_double)]
 
 
 def bci(d, side, var):
@@ -372,12 +373,16 @@ def _pf(a):
 class Net:
     """The pressure-guess U-Net (BF16/MFMA HIP kernels); see mdx_net_* in include/mobydiff_hip.h."""
 
-    def __init__(self, nx, ny, nz, seed=0):
+    def __init__(self, nx, ny, nz, seed=0, stride=1):
+        """A network serving the nx x ny x nz solver grid; it runs on that grid coarsened by
+        `stride` (1, 2 or 4): inputs restricted by the mean over stride^3 cells, the output
+        prolonged trilinearly.  `shape` is the network grid (the host-side forward/gradient calls
+        take inputs of that shape)."""
         self.h = C.c_void_p()
-        _check(lib().mdx_net_create(C.c_int32(nx), C.c_int32(ny), C.c_int32(nz), C.c_uint64(seed),
-                                    C.byref(self.h)))
-        self.shape = (nz, ny, nx)
-        self.nparam = lib().mdx_net_param_count() if False else None
+        _check(lib().mdx_net_create_strided(C.c_int32(nx), C.c_int32(ny), C.c_int32(nz), C.c_int32(stride),
+                                            C.c_uint64(seed), C.byref(self.h)))
+        self.stride = stride
+        self.shape = (nz // stride, ny // stride, nx // stride)
         lib().mdx_net_param_count.restype = C.c_int64
         self.nparam = int(lib().mdx_net_param_count())
 
@@ -394,6 +399,21 @@ class Net:
             self.close()
         except Exception:
             pass
+
+    def restrict(self, fine):
+        """solver-grid field (nz, ny, nx) float64 -> network grid, the mean over stride^3 cells"""
+        fine = np.ascontiguousarray(fine, dtype=np.float64)
+        out = np.zeros(self.shape, np.float32)
+        _check(lib().mdx_net_restrict_host(self.h, _p(fine), _pf(out)))
+        return out
+
+    def prolong(self, coarse):
+        """network-grid field -> solver grid, the trilinear interpolation of the learned guess"""
+        coarse = np.ascontiguousarray(coarse, dtype=np.float32)
+        nz, ny, nx = self.shape
+        out = np.zeros((nz * self.stride, ny * self.stride, nx * self.stride))
+        _check(lib().mdx_net_prolong_host(self.h, _pf(coarse), _p(out)))
+        return out
 
     def get_params(self):
         p = np.zeros(self.nparam, np.float32)

@@ -140,6 +140,45 @@ static void tridiag_tables(const mdx_handle *h, std::vector<double> &cp, std::ve
         }
 }
 
+// Row coefficients and modified wavenumbers of the one-wave-per-line solve (tridiag_pcr_kernel):
+// row j (0-based) reads a_j x_{j-1} + (bd_j - k) x_j + c_j x_{j+1} with k = lamx[kx] + lamz[kz];
+// a_0 = 0 and c_{ny-1} = 0 (the homogeneous Neumann rows), the same expressions as the Thomas
+// tables above.
+static void pcr_tables(const mdx_handle *h, std::vector<double> &lx, std::vector<double> &lz,
+                       std::vector<double> &a, std::vector<double> &bd, std::vector<double> &c) {
+    const int nx = h->d.nx, ny = h->d.ny, nz = h->d.nz, nxh = nx / 2 + 1;
+    const double dx = h->dx, dy = h->dy, dz = h->dz;
+    const double dyi2 = 1.0 / (dy * dy);
+    lx.assign(nxh, 0.0);
+    lz.assign(nz, 0.0);
+    for (int kx = 0; kx < nxh; ++kx) {
+        const double s = std::sin(PI * (double)kx / (double)nx);
+        lx[kx] = (4.0 / (dx * dx)) * (s * s);
+    }
+    for (int ikz = 1; ikz <= nz; ++ikz) {
+        const int kz = (ikz <= nz / 2 + 1) ? ikz - 1 : ikz - nz - 1;
+        const double sz = std::sin(PI * (double)kz / (double)nz);
+        lz[ikz - 1] = (4.0 / (dz * dz)) * (sz * sz);
+    }
+    a.assign(ny, 0.0);
+    bd.assign(ny, 0.0);
+    c.assign(ny, 0.0);
+    for (int jj = 1; jj <= ny; ++jj) {
+        double aj, cj;
+        if (!h->stretched) {
+            aj = jj > 1 ? dyi2 : 0.0;
+            cj = jj < ny ? dyi2 : 0.0;
+            bd[jj - 1] = (jj == 1 || jj == ny) ? -dyi2 : -2.0 * dyi2;
+        } else {
+            aj = jj > 1 ? h->ihv[jj] * h->ihc[jj] : 0.0;
+            cj = jj < ny ? h->ihv[jj + 1] * h->ihc[jj] : 0.0;
+            bd[jj - 1] = jj == 1 ? -cj : -(aj + cj);
+        }
+        a[jj - 1] = aj;
+        c[jj - 1] = cj;
+    }
+}
+
 static void twiddles(int n, std::vector<double> &tw) {
     tw.assign((size_t)n, 0.0);   // n/2 complex
     for (int m = 0; m < n / 2; ++m) {
@@ -199,7 +238,7 @@ static void free_all(mdx_handle *h) {
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
                     h->red_part, h->red_max, h->max_part, h->rb_faces, h->phi_hist, h->res64,
                     h->fft_phi, h->fft_res, h->fft_mean, h->hat32, h->cp32, h->den32, h->arow32, h->tw_x32,
-                    h->tw_z32, h->rb_xl32, h->rb_xl64, h->cvel};
+                    h->tw_z32, h->rb_xl32, h->rb_xl64, h->cvel, h->lamx, h->lamz, h->pcr_a, h->bdrow, h->crow};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -318,6 +357,8 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if (const char *lc = std::getenv("MOBYDIFF_LDS_COOP")) h->rb_coop = lc[0] == '1';
     if (const char *sl = std::getenv("MOBYDIFF_LDS_SPIN_LIMIT")) h->rb_spin_limit = std::max(0, std::atoi(sl));
     if (const char *nm = std::getenv("MOBYDIFF_NET_MARGIN")) h->net_margin = std::atof(nm);
+    if (const char *tg = std::getenv("MOBYDIFF_NET_TIME_GATE")) h->net_time_gate = tg[0] != '0';
+    if (const char *pt = std::getenv("MOBYDIFF_PCR_TRIDIAG")) h->pcr_tridiag = pt[0] != '0';
     if (const char *mm = std::getenv("MOBYDIFF_MERGE_MAX")) h->merge_max = std::max(1, std::atoi(mm));
     if (const char *lf = std::getenv("MOBYDIFF_LDS_FORM")) {
         if (std::strcmp(lf, "pairs") == 0) h->rb_form = 1;
@@ -375,6 +416,18 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             (e = dalloc(&h->arow, ny)) != hipSuccess || (e = dalloc(&h->tw_x, nx)) != hipSuccess ||
             (e = dalloc(&h->tw_z, nz)) != hipSuccess)
             return fail(e, "FFT workspace");
+        std::vector<double> plx, plz, pa, pbd, pc;
+        pcr_tables(h, plx, plz, pa, pbd, pc);
+        if ((e = dalloc(&h->lamx, nxh)) != hipSuccess || (e = dalloc(&h->lamz, nz)) != hipSuccess ||
+            (e = dalloc(&h->pcr_a, ny)) != hipSuccess || (e = dalloc(&h->bdrow, ny)) != hipSuccess ||
+            (e = dalloc(&h->crow, ny)) != hipSuccess)
+            return fail(e, "FFT workspace");
+        if ((e = hipMemcpy(h->lamx, plx.data(), nxh * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->lamz, plz.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->pcr_a, pa.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->bdrow, pbd.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->crow, pc.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "FFT table upload");
         if ((e = hipMemcpy(h->cp, cp.data(), nh * 8, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(h->den_inv, di.data(), nh * 8, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(h->arow, ar.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess ||
@@ -802,8 +855,9 @@ static int ensure_mu(mdx_handle *h, double dtg) {
 }
 
 static void ev_record(mdx_handle *h, int q) {
-    if (!h->timing || (q < 5 && h->ph_skip)) return;
-    (void)hipEventRecord(q < 5 ? h->ev_ph[h->ph_cur][q] : h->ev[q], h->stream);
+    const bool phase = q < mdx_handle::PH_EVENTS;
+    if (!h->timing || (phase && h->ph_skip)) return;
+    (void)hipEventRecord(phase ? h->ev_ph[h->ph_cur][q] : h->ev[q], h->stream);
 }
 
 // add the phase times of completed stages (`wait`: of all recorded stages) to the counters
@@ -812,14 +866,16 @@ static void phase_account(mdx_handle *h, bool wait) {
         hipEvent_t *e = h->ev_ph[h->ph_head];
         if (wait) (void)hipEventSynchronize(e[4]);
         else if (hipEventQuery(e[4]) != hipSuccess) break;
-        float a = 0, b = 0, c = 0, d = 0;
+        float a = 0, b = 0, c = 0, d = 0, m = 0;
         (void)hipEventElapsedTime(&a, e[0], e[1]);
         (void)hipEventElapsedTime(&b, e[1], e[2]);
         (void)hipEventElapsedTime(&c, e[2], e[3]);
         (void)hipEventElapsedTime(&d, e[3], e[4]);
+        (void)hipEventElapsedTime(&m, e[0], e[5]);
         h->times.stencil_ms += a + d;
         h->times.poisson_ms += b;
         h->times.projection_ms += c;
+        h->times.momentum_ms += m;
         h->times.stages += 1;
         h->ph_head = (h->ph_head + 1) % mdx_handle::PHQ;
         h->ph_count -= 1;
@@ -1155,7 +1211,7 @@ static GuessPlan guess_plan(mdx_handle *h, int s, double dtg, mdx_solve_stats *s
     h->net_started[s] = false;
     // gate: after the network's start has lost twice in a row for this stage, it is evaluated
     // again only after 4, 16, 64, then every NET_SKIP_MAX skipped solves (previous-Phi start then)
-    if (p.learned && h->net_skip[s] > 0) {
+    if (p.learned && h->net_time_gate && h->net_skip[s] > 0) {
         h->net_skip[s] -= 1;
         p.learned = false;
         st->guess_skipped = 1;
@@ -1163,7 +1219,7 @@ static GuessPlan guess_plan(mdx_handle *h, int s, double dtg, mdx_solve_stats *s
     }
     // a network start is judged against the host time of a plain solve of the same stage: until
     // one has been timed, the stage runs the plain start
-    if (p.learned && h->base_sec[s] <= 0.0) {
+    if (p.learned && h->net_time_gate && h->base_sec[s] <= 0.0) {
         p.learned = false;
         st->guess_skipped = 1;
         st->fallback = 1;
@@ -1499,6 +1555,7 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
         }
         ev_record(h, 0);
         if ((rc = mdx_momentum(h, dta, dtb, dtg))) return rc;
+        ev_record(h, 5);
         if (h->cfg.flowrate && (rc = mdx_flowrate_shift(h, dtg, nullptr))) return rc;
         if (h->cfg.solver == MDX_SOLVER_FFT && (rc = mdx_apply_bc(h))) return rc;
         mdx_solve_stats st{};

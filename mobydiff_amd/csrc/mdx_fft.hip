@@ -232,6 +232,119 @@ __global__ __launch_bounds__(64) void tridiag_kernel(Dims d, double2 *hat, const
     }
 }
 
+// 3'. The same tridiagonal solves by a PCR-Thomas hybrid, one wavefront per (kx,kz) line, every
+//     value of the line held in registers: the spectrum is read once and written once (16 B per
+//     cell), against twice each plus the two FP64 tables for the thread-per-line recurrence.
+//     Lane l holds rows j = l, l + 64, ..., l + 64 (M-1) (0-based) with their coefficients
+//     a_j (sub-diagonal), b_j = bd_j - k (k = lamx[kx] + lamz[kz], the modified wavenumbers of
+//     pressure_fft.f90:20-100), c_j and the complex right-hand side.  Six parallel-cyclic-reduction
+//     steps (coupling distance 1, 2, ..., 32; the neighbours' coefficients by cross-lane shuffles)
+//     leave every row coupled only to the rows 64 away, i.e. each lane holds an independent
+//     M-row tridiagonal system, which it solves by Thomas.  Rows beyond the line (ny not a
+//     multiple of 64) and neighbours beyond its ends are identity rows (b = 1, a = c = d = 0).
+//     The mean mode (kx = kz = 0) pins row 1 to zero, as the reference does (dp(1) = 0), by
+//     making it an identity row.  The system is diagonally dominant (weakly for the mean mode,
+//     an irreducible M-matrix), so neither part pivots.  Lines are assigned to the eight waves
+//     of a workgroup consecutively: for a fixed row the eight lines' values are 128 contiguous
+//     bytes of the spectrum.
+template <typename T>
+__device__ inline T shfl_any(T v, int src) {
+    return __shfl(v, src, 64);
+}
+template <>
+__device__ inline double2 shfl_any<double2>(double2 v, int src) {
+    return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+static constexpr int PCR_WAVES = 8;
+
+template <int M>
+__global__ __launch_bounds__(64 * PCR_WAVES) void tridiag_pcr_kernel(int nlines, int nxh, int ny, size_t plane,
+                                                                    double2 *hat, const double *__restrict__ lamx,
+                                                                    const double *__restrict__ lamz,
+                                                                    const double *__restrict__ arow,
+                                                                    const double *__restrict__ bdrow,
+                                                                    const double *__restrict__ crow) {
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * PCR_WAVES + (threadIdx.x >> 6);   // the wave's line
+    if (t >= nlines) return;                                     // whole waves only
+    const int kx = t % nxh, kz = t / nxh;
+    const double k = lamx[kx] + lamz[kz];
+    double a[M], b[M], c[M];
+    double2 d[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int j = lane + 64 * q;
+        if (j < ny && !(t == 0 && j == 0)) {
+            a[q] = arow[j];
+            c[q] = crow[j];
+            b[q] = bdrow[j] - k;
+            d[q] = hat[(size_t)t + plane * j];
+        } else {
+            a[q] = 0.0;
+            c[q] = 0.0;
+            b[q] = 1.0;
+            d[q] = make_double2(0.0, 0.0);
+        }
+    }
+#pragma unroll
+    for (int delta = 1; delta < 64; delta <<= 1) {
+        double rb[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) rb[q] = 1.0 / b[q];
+        // row i - delta (lane - delta of the same row set, or of the previous one) and i + delta
+        const int lo = (lane - delta) & 63, hi = (lane + delta) & 63;
+        double la[M], lc[M], lr[M], ha[M], hc[M], hr[M];
+        double2 ld[M], hd[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            la[q] = shfl_any(a[q], lo);
+            lc[q] = shfl_any(c[q], lo);
+            lr[q] = shfl_any(rb[q], lo);
+            ld[q] = shfl_any(d[q], lo);
+            ha[q] = shfl_any(a[q], hi);
+            hc[q] = shfl_any(c[q], hi);
+            hr[q] = shfl_any(rb[q], hi);
+            hd[q] = shfl_any(d[q], hi);
+        }
+        const bool wrap_lo = lane < delta, wrap_hi = lane + delta >= 64;
+#pragma unroll
+        for (int q = M - 1; q >= 0; --q) {
+            // neighbours across the lane wrap belong to the row set q - 1 (below) or q + 1 (above)
+            double na, nc, nr, pa, pc, pr;
+            double2 nd, pd;
+            if (!wrap_lo) { na = la[q]; nc = lc[q]; nr = lr[q]; nd = ld[q]; }
+            else if (q > 0) { na = la[q - 1]; nc = lc[q - 1]; nr = lr[q - 1]; nd = ld[q - 1]; }
+            else { na = 0.0; nc = 0.0; nr = 1.0; nd = make_double2(0.0, 0.0); }
+            if (!wrap_hi) { pa = ha[q]; pc = hc[q]; pr = hr[q]; pd = hd[q]; }
+            else if (q + 1 < M) { pa = ha[q + 1]; pc = hc[q + 1]; pr = hr[q + 1]; pd = hd[q + 1]; }
+            else { pa = 0.0; pc = 0.0; pr = 1.0; pd = make_double2(0.0, 0.0); }
+            const double k1 = a[q] * nr, k2 = c[q] * pr;
+            a[q] = -na * k1;
+            b[q] = b[q] - nc * k1 - pa * k2;
+            c[q] = -pc * k2;
+            d[q] = make_double2(d[q].x - nd.x * k1 - pd.x * k2, d[q].y - nd.y * k1 - pd.y * k2);
+        }
+    }
+    // Thomas over the lane's rows (row q couples to rows q - 1 and q + 1 only)
+#pragma unroll
+    for (int q = 1; q < M; ++q) {
+        const double w = a[q] / b[q - 1];
+        b[q] = b[q] - w * c[q - 1];
+        d[q] = make_double2(d[q].x - w * d[q - 1].x, d[q].y - w * d[q - 1].y);
+    }
+    double2 x[M];
+    x[M - 1] = make_double2(d[M - 1].x / b[M - 1], d[M - 1].y / b[M - 1]);
+#pragma unroll
+    for (int q = M - 2; q >= 0; --q)
+        x[q] = make_double2((d[q].x - c[q] * x[q + 1].x) / b[q], (d[q].y - c[q] * x[q + 1].y) / b[q]);
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int j = lane + 64 * q;
+        if (j < ny) hat[(size_t)t + plane * j] = x[q];
+    }
+}
+
 // 5. c2r along x: the real parts of the kx=0 and kx=nx/2 coefficients, Hermitian extension of the
 //    rest, inverse transform, times 1/(nx nz); phi in rhs layout (nx, nz, ny).
 __global__ __launch_bounds__(FFT_THREADS) void ixfft_kernel(Dims d, const double2 *__restrict__ hat,
@@ -352,7 +465,21 @@ hipError_t launch_fft_forward(mdx_handle *h, double dtg) {
 
 hipError_t launch_tridiag(mdx_handle *h) {
     const Dims d = h->d;
-    const int nlines = (d.nx / 2 + 1) * d.nz;
+    const int nxh = d.nx / 2 + 1, nlines = nxh * d.nz;
+    const int m = (d.ny + 63) / 64;
+    if (h->pcr_tridiag && m <= 4) {
+        const size_t plane = (size_t)nxh * d.nz;
+        const dim3 grid((nlines + PCR_WAVES - 1) / PCR_WAVES), blk(64 * PCR_WAVES);
+        double2 *hat = (double2 *)h->hat;
+#define PCR_CASE(MM)                                                                                         \
+    case MM:                                                                                                 \
+        hipLaunchKernelGGL(tridiag_pcr_kernel<MM>, grid, blk, 0, h->stream, nlines, nxh, d.ny, plane, hat,    \
+                           h->lamx, h->lamz, h->pcr_a, h->bdrow, h->crow);                                   \
+        break;
+        switch (m) { PCR_CASE(1) PCR_CASE(2) PCR_CASE(3) PCR_CASE(4) }
+#undef PCR_CASE
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(tridiag_kernel, dim3((nlines + 63) / 64), dim3(64), 0, h->stream, d, (double2 *)h->hat,
                        h->cp, h->den_inv, h->arow);
     return hipGetLastError();

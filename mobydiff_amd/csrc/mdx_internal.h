@@ -149,12 +149,12 @@ struct mdx_handle {
     bool timing = false;
     hipEvent_t ev[10] = {};         // [8], [9]: the learned guess
     // per-stage phase events (0 stage start, 1 Poisson start, 2 projection start, 3 projection end,
-    // 4 stage end) in a ring of PHQ stages, added to `times` once complete (no stall per stage);
-    // set PHQ is scratch for phase events recorded outside mdx_step
-    static constexpr int PHQ = 8, PH_EVERY = 4;
+    // 4 stage end, 5 momentum kernel end) in a ring of PHQ stages, added to `times` once complete
+    // (no stall per stage); set PHQ is scratch for phase events recorded outside mdx_step
+    static constexpr int PHQ = 8, PH_EVERY = 4, PH_EVENTS = 6;
     long long ph_seen = 0;   // stages stepped so far
     bool ph_skip = false;    // this stage is not timed
-    hipEvent_t ev_ph[PHQ + 1][5] = {};
+    hipEvent_t ev_ph[PHQ + 1][PH_EVENTS] = {};
     int ph_head = 0, ph_count = 0, ph_cur = PHQ;
     mdx_phase_times times{};
     // sweep batches not yet added to the counters: start/end events, iterations and launches
@@ -221,6 +221,9 @@ struct mdx_handle {
     double *phi_hist = nullptr;
     double *res64 = nullptr;       // residual field of the previous-Phi guess (network input)
     double *cvel = nullptr;        // cell-centred velocity for output (3 interior arrays, allocated on first use)
+    // one-wave-per-line tridiagonal solve: modified wavenumbers (nxh, nz) and row coefficients (ny)
+    double *lamx = nullptr, *lamz = nullptr, *pcr_a = nullptr, *bdrow = nullptr, *crow = nullptr;
+    bool pcr_tridiag = true;       // MOBYDIFF_PCR_TRIDIAG=0: the thread-per-line Thomas recurrence
     double hist_dtg[3] = {0.0, 0.0, 0.0};
     // learned-guess gate per stage: consecutive solves the network's start lost, solves left to
     // skip, whether the current solve starts from the network's guess, and the sweeps of the last
@@ -229,6 +232,9 @@ struct mdx_handle {
     // the network's start is taken when its residual max-norm is below net_margin times the other
     // start's (MOBYDIFF_NET_MARGIN)
     double net_margin = 0.8;
+    // the time gate of the learned start (MOBYDIFF_NET_TIME_GATE=0 switches it off: the network is
+    // then evaluated for every solve and judged by the residual test alone)
+    bool net_time_gate = true;
     // mixed solve: at most this many inner_iters batches run as one while the checks between them
     // are predicted to fail (MOBYDIFF_MERGE_MAX; 1 keeps every check)
     int merge_max = 8;

@@ -9,8 +9,14 @@
 typedef __bf16 bf16;
 
 struct mdx_net {
-    int W[3], H[3], D[3];          // per level: full, /2, /4
+    int W[3], H[3], D[3];          // per level of the network grid: level 0, /2, /4
     long V[3];
+    // the solver grid the network serves: level 0 is that grid coarsened by `stride` in every
+    // direction (inputs restricted by the mean over stride^3 cells, the output prolonged
+    // trilinearly); stride 1 runs the network on the solver grid itself
+    int stride = 1;
+    int FW = 0, FH = 0, FD = 0;
+    long FV = 0;
     int64_t nparam = 0;
     int64_t woff[11], boff[11];    // flat FP32 parameter layout: per layer W(tap,cin,cout) then b(cout)
     hipStream_t stream = nullptr;
@@ -62,6 +68,8 @@ hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *
 hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi);
 hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, double fac);
 hipError_t net_load_replay(mdx_net *n, int slot);
+hipError_t net_restrict(mdx_net *n, const double *fine, float *coarse);
+hipError_t net_prolong(mdx_net *n, const float *coarse, double *fine);
 hipError_t net_mse_only(mdx_net *n);
 // drop handle h from the network's attached handles
 void net_detach(mdx_net *n, struct mdx_handle *h);

@@ -269,20 +269,30 @@ __global__ __launch_bounds__(256) void conv3_wgrad_kernel(const bf16 *__restrict
     }
 }
 
-// bias gradient: db[n] = sum_v dY[v][n]
+// bias gradient: db[n] = sum_v dY[v][n].  The COUT channels of a voxel are G = COUT/8 16-byte
+// vectors; thread t reads vectors q = t, t + stride, ... of the flat array (consecutive lanes,
+// consecutive 16 bytes: whole cache lines per wave), and since the stride is a multiple of G a thread
+// always sees the same channel group q % G.  The lanes of one group are then folded by shuffles
+// (offsets G .. 32) and one atomic addition per channel and wave is left.
 template <int COUT>
 __global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__ dy, float *__restrict__ db, long V) {
-    float s[COUT];
+    constexpr int G = COUT / 8;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const long nvec = V * G;
+    const bf16x8 *__restrict__ d8 = reinterpret_cast<const bf16x8 *>(dy);
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < nvec; q += (long)gridDim.x * 256) {
+        const bf16x8 a = d8[q];
 #pragma unroll
-    for (int n = 0; n < COUT; ++n) s[n] = 0.f;
-    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long)gridDim.x * 256)
+        for (int e = 0; e < 8; ++e) s[e] += (float)a[e];
+    }
 #pragma unroll
-        for (int n = 0; n < COUT; ++n) s[n] += (float)dy[v * COUT + n];
+    for (int e = 0; e < 8; ++e)
+        for (int off = 32; off >= G; off >>= 1) s[e] += __shfl_xor(s[e], off, 64);
+    const int lane = threadIdx.x & 63;
+    if (lane < G) {
+        const int g = (int)(((long)blockIdx.x * 256 + threadIdx.x) % G);
 #pragma unroll
-    for (int n = 0; n < COUT; ++n) {
-        float t = s[n];
-        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&db[n], t);
+        for (int e = 0; e < 8; ++e) atomicAdd(&db[g * 8 + e], s[e]);
     }
 }
 
@@ -522,6 +532,92 @@ __global__ void make_target_kernel(const double *__restrict__ phi, const double 
         t[v] = (float)((phi[v] - (phi_prev ? fac * phi_prev[v] : 0.0)) * s);
 }
 
+// Strided coupling (network grid = solver grid / s).  Network voxel (x, y, z) covers the cells
+// (s x + a, s y + b, s z + c), 0 <= a, b, c < s, of the solver's interior layout (fine index
+// i + FW (j + FH k), 0-based).
+struct FineGrid {
+    int W, H, D;      // network grid
+    int s;            // stride
+    int FW, FH, FD;   // solver grid
+};
+
+// network input: the mean over each voxel's s^3 cells of r0 / s_r and Phi_0 / s_p
+__global__ void restrict_input_kernel(FineGrid f, const double *__restrict__ r0, const double *__restrict__ phi0,
+                                      const double *__restrict__ stats, bf16 *__restrict__ x) {
+    const double ir = 1.0 / stats[0], ip = phi0 ? 1.0 / stats[1] : 0.0;
+    const double w = 1.0 / (double)(f.s * f.s * f.s);
+    const long V = (long)f.W * f.H * f.D;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        const int vx = (int)(v % f.W), vy = (int)((v / f.W) % f.H), vz = (int)(v / ((long)f.W * f.H));
+        double sr = 0.0, sp = 0.0;
+        for (int c = 0; c < f.s; ++c)
+            for (int b = 0; b < f.s; ++b) {
+                const long row = (long)f.FW * ((long)(f.s * vy + b) + (long)f.FH * (f.s * vz + c)) + (long)f.s * vx;
+                for (int a = 0; a < f.s; ++a) {
+                    sr += r0[row + a];
+                    if (phi0) sp += phi0[row + a];
+                }
+            }
+        bf16x8 lo = {}, hi = {};
+        lo[0] = (bf16)(float)(sr * w * ir);
+        lo[1] = (bf16)(float)(sp * w * ip);
+        *reinterpret_cast<bf16x8 *>(x + v * 16) = lo;
+        *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = hi;
+    }
+}
+
+// coarse cell-centred coordinate of fine cell i: (i + 1/2)/s - 1/2 = i0 + t, i0 clamped to the grid
+// (the outermost fine cells take the nearest voxel's value)
+__device__ inline void prolong_axis(int i, int s, int n, int &i0, int &i1, double &t) {
+    const double xc = ((double)i + 0.5) / (double)s - 0.5;
+    double fl = floor(xc);
+    t = xc - fl;
+    i0 = (int)fl;
+    i1 = i0 + 1;
+    if (i0 < 0) { i0 = 0; t = 0.0; }
+    if (i1 > n - 1) { i1 = n - 1; }
+}
+
+// Phi guess on the solver grid = Phi_0 + (trilinear prolongation of out) * s_r * c0
+__global__ void prolong_guess_kernel(FineGrid f, const float *__restrict__ out, const double *__restrict__ stats,
+                                     double c0, const double *phi0, double *phi) {
+    const double sc = stats[0] * c0;
+    const long FV = (long)f.FW * f.FH * f.FD;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < FV; v += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(v % f.FW), j = (int)((v / f.FW) % f.FH), k = (int)(v / ((long)f.FW * f.FH));
+        int x0, x1, y0, y1, z0, z1;
+        double tx, ty, tz;
+        prolong_axis(i, f.s, f.W, x0, x1, tx);
+        prolong_axis(j, f.s, f.H, y0, y1, ty);
+        prolong_axis(k, f.s, f.D, z0, z1, tz);
+        auto at = [&](int x, int y, int z) { return (double)out[((long)z * f.H + y) * f.W + x]; };
+        const double c00 = at(x0, y0, z0) * (1.0 - tx) + at(x1, y0, z0) * tx;
+        const double c10 = at(x0, y1, z0) * (1.0 - tx) + at(x1, y1, z0) * tx;
+        const double c01 = at(x0, y0, z1) * (1.0 - tx) + at(x1, y0, z1) * tx;
+        const double c11 = at(x0, y1, z1) * (1.0 - tx) + at(x1, y1, z1) * tx;
+        const double val = (c00 * (1.0 - ty) + c10 * ty) * (1.0 - tz) + (c01 * (1.0 - ty) + c11 * ty) * tz;
+        phi[v] = (phi0 ? phi0[v] : 0.0) + val * sc;
+    }
+}
+
+// training target at the network grid: the mean over each voxel's s^3 cells of
+// (Phi - fac Phi_prev) / (s_r c0)
+__global__ void restrict_target_kernel(FineGrid f, const double *__restrict__ phi, const double *__restrict__ phi_prev,
+                                       double fac, const double *__restrict__ stats, double c0, float *__restrict__ t) {
+    const double sc = 1.0 / (stats[0] * c0) / (double)(f.s * f.s * f.s);
+    const long V = (long)f.W * f.H * f.D;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        const int vx = (int)(v % f.W), vy = (int)((v / f.W) % f.H), vz = (int)(v / ((long)f.W * f.H));
+        double acc = 0.0;
+        for (int c = 0; c < f.s; ++c)
+            for (int b = 0; b < f.s; ++b) {
+                const long row = (long)f.FW * ((long)(f.s * vy + b) + (long)f.FH * (f.s * vz + c)) + (long)f.s * vx;
+                for (int a = 0; a < f.s; ++a) acc += phi[row + a] - (phi_prev ? fac * phi_prev[row + a] : 0.0);
+            }
+        t[v] = (float)(acc * sc);
+    }
+}
+
 // replay buffer: store / load the 2 used input channels and the target
 __global__ void pack2_kernel(const bf16 *__restrict__ x, bf16 *__restrict__ x2, long V) {
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
@@ -615,9 +711,9 @@ static hipError_t wgrad(mdx_net *n, int l, const bf16 *in, const bf16 *dy) {
     return hipErrorInvalidValue;
 #undef WG_CASE
     const long V = n->V[lvl];
-    if (L.cout == 16) hipLaunchKernelGGL(bias_grad_kernel<16>, dim3(nred(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
-    else if (L.cout == 32) hipLaunchKernelGGL(bias_grad_kernel<32>, dim3(nred(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
-    else hipLaunchKernelGGL(bias_grad_kernel<64>, dim3(nred(V)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    if (L.cout == 16) hipLaunchKernelGGL(bias_grad_kernel<16>, dim3(nred(V * 2)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    else if (L.cout == 32) hipLaunchKernelGGL(bias_grad_kernel<32>, dim3(nred(V * 4)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
+    else hipLaunchKernelGGL(bias_grad_kernel<64>, dim3(nred(V * 8)), dim3(256), 0, n->stream, dy, dBs(n, l), V);
     return hipGetLastError();
 }
 
@@ -739,8 +835,13 @@ hipError_t net_adam(mdx_net *n, float lr) {
 }
 
 // ------------------------------------------------------------------------------- solver coupling
+static FineGrid fine_grid(const mdx_net *n) {
+    return FineGrid{n->W[0], n->H[0], n->D[0], n->stride, n->FW, n->FH, n->FD};
+}
+
+// rms over the solver grid's interior cells
 static hipError_t rms_into(mdx_net *n, const double *a, const double *b, double fac, double *stat) {
-    const long V = n->V[0];
+    const long V = n->FV;
     hipLaunchKernelGGL(input_stats_partial, dim3(512), dim3(256), 0, n->stream, V, a, b, fac, n->part);
     hipLaunchKernelGGL(input_stats_final, dim3(1), dim3(64), 0, n->stream, n->part, 512, (double)V, stat);
     return hipGetLastError();
@@ -752,15 +853,23 @@ hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *
     const long V = n->V[0];
     TRY(rms_into(n, r0, nullptr, 0.0, n->stats));
     if (phi0) TRY(rms_into(n, phi0, nullptr, 0.0, n->stats + 1));
-    hipLaunchKernelGGL(make_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, V, r0, phi0, n->stats, n->x_in);
+    if (n->stride == 1)
+        hipLaunchKernelGGL(make_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, V, r0, phi0, n->stats, n->x_in);
+    else
+        hipLaunchKernelGGL(restrict_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, fine_grid(n), r0, phi0,
+                           n->stats, n->x_in);
     return hipGetLastError();
 }
 
 hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi) {
     hipError_t e;
     TRY(net_forward(n));
-    hipLaunchKernelGGL(guess_out_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->stats, n->c0,
-                       phi0, phi, n->V[0]);
+    if (n->stride == 1)
+        hipLaunchKernelGGL(guess_out_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->stats, n->c0,
+                           phi0, phi, n->V[0]);
+    else
+        hipLaunchKernelGGL(prolong_guess_kernel, dim3(nblk(n->FV)), dim3(256), 0, n->stream, fine_grid(n), n->out,
+                           n->stats, n->c0, phi0, phi);
     return hipGetLastError();
 }
 
@@ -769,8 +878,12 @@ hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, do
     const long V = n->V[0];
     const int slot = n->next;
     hipLaunchKernelGGL(pack2_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, n->x_in, n->rb_x + (size_t)slot * 2 * V, V);
-    hipLaunchKernelGGL(make_target_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, phi, phi_prev, fac, n->stats,
-                       n->c0, n->rb_t + (size_t)slot * V, V);
+    if (n->stride == 1)
+        hipLaunchKernelGGL(make_target_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, phi, phi_prev, fac, n->stats,
+                           n->c0, n->rb_t + (size_t)slot * V, V);
+    else
+        hipLaunchKernelGGL(restrict_target_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, fine_grid(n), phi, phi_prev,
+                           fac, n->stats, n->c0, n->rb_t + (size_t)slot * V);
     n->next = (n->next + 1) % n->cap;
     if (n->count < n->cap) n->count += 1;
     return hipGetLastError();
@@ -785,11 +898,30 @@ hipError_t net_mse_only(mdx_net *n) {
     return hipGetLastError();
 }
 
-// root mean square of an interior FP64 field (deterministic), into stats[3]
 // rms of the first training target's numerator, Phi - fac * Phi_prev, into stats[3]
 hipError_t net_rms_to_stats3(mdx_net *n, const mdx_handle *h, const double *phi, const double *phi_prev, double fac) {
     (void)h;
     return rms_into(n, phi, phi_prev, fac, n->stats + 3);
+}
+
+// the transfer operators alone on device buffers: coarse = restriction of fine (mean over
+// stride^3 cells), fine = trilinear prolongation of coarse (stats[6] holds the unit scale)
+hipError_t net_restrict(mdx_net *n, const double *fine, float *coarse) {
+    const double one = 1.0;
+    hipError_t e = hipMemcpyAsync(n->stats + 6, &one, 8, hipMemcpyHostToDevice, n->stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(restrict_target_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, fine_grid(n), fine, nullptr,
+                       0.0, n->stats + 6, 1.0, coarse);
+    return hipGetLastError();
+}
+
+hipError_t net_prolong(mdx_net *n, const float *coarse, double *fine) {
+    const double one = 1.0;
+    hipError_t e = hipMemcpyAsync(n->stats + 6, &one, 8, hipMemcpyHostToDevice, n->stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(prolong_guess_kernel, dim3(nblk(n->FV)), dim3(256), 0, n->stream, fine_grid(n), coarse,
+                       n->stats + 6, 1.0, nullptr, fine);
+    return hipGetLastError();
 }
 
 hipError_t net_load_replay(mdx_net *n, int slot) {

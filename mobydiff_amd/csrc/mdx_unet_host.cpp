@@ -44,17 +44,25 @@ static hipError_t nalloc(mdx_net *n, T **p, size_t count) {
     return e;
 }
 
-extern "C" int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, mdx_net **out) {
+extern "C" int mdx_net_create_strided(int32_t nx, int32_t ny, int32_t nz, int32_t stride, uint64_t seed,
+                                      mdx_net **out) {
     NET_CHECK(out, "mdx_net_create: null output");
     *out = nullptr;
-    // every level's voxel count must be a multiple of 64 (one conv workgroup = 64 voxels)
-    NET_CHECK(nx % 64 == 0 && ny % 4 == 0 && nz % 4 == 0 && nx >= 64 &&
-                  ((long)(nx / 4) * (ny / 4) * (nz / 4)) % 64 == 0,
-              "mdx_net_create: needs nx a multiple of 64, ny and nz multiples of 4, and (nx ny nz / 64) a "
-              "multiple of 64");
+    NET_CHECK(stride == 1 || stride == 2 || stride == 4, "mdx_net_create: the stride must be 1, 2 or 4");
+    NET_CHECK(nx > 0 && ny > 0 && nz > 0 && nx % stride == 0 && ny % stride == 0 && nz % stride == 0,
+              "mdx_net_create: the grid must divide by the stride");
+    const int w0 = nx / stride, h0 = ny / stride, d0 = nz / stride;
+    // two 2x2x2 poolings, and every level's voxel count a multiple of 64 (one conv workgroup = 64
+    // voxels; the coarsest level's count divides the others')
+    NET_CHECK(w0 % 4 == 0 && h0 % 4 == 0 && d0 % 4 == 0 && ((long)(w0 / 4) * (h0 / 4) * (d0 / 4)) % 64 == 0,
+              "mdx_net_create: needs the network grid (solver grid / stride) in multiples of 4 with (its voxel "
+              "count / 64) a multiple of 64");
     mdx_net *n = new mdx_net();
     if (const char *t = std::getenv("MOBYDIFF_NO_TILED_CONV")) n->tiled_conv = !(t[0] == '1');
-    n->W[0] = nx; n->H[0] = ny; n->D[0] = nz;
+    n->stride = stride;
+    n->FW = nx; n->FH = ny; n->FD = nz;
+    n->FV = (long)nx * ny * nz;
+    n->W[0] = w0; n->H[0] = h0; n->D[0] = d0;
     for (int q = 1; q < 3; ++q) { n->W[q] = n->W[q - 1] / 2; n->H[q] = n->H[q - 1] / 2; n->D[q] = n->D[q - 1] / 2; }
     for (int q = 0; q < 3; ++q) n->V[q] = (long)n->W[q] * n->H[q] * n->D[q];
     n->nparam = mdx_net_param_count();
@@ -83,7 +91,7 @@ extern "C" int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed,
         if ((e = nalloc(n, &n->wp[l], (size_t)KC * (L.cout / 16) * 512)) != hipSuccess) return fail(e);
         if (L.cin >= 16 && (e = nalloc(n, &n->wd[l], (size_t)KCd * (L.cin / 16) * 512)) != hipSuccess) return fail(e);
     }
-    // activations and their gradients (bf16, channels-last)
+    // activations and their gradients (bf16, channels-last), at the network grid
     const long V0 = n->V[0], V1 = n->V[1], V2 = n->V[2];
     struct { bf16 **p; size_t n; } bufs[] = {
         {&n->x_in, (size_t)V0 * 16}, {&n->a[0], (size_t)V0 * 16}, {&n->a[1], (size_t)V0 * 16},
@@ -120,6 +128,19 @@ extern "C" int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed,
     if ((e = repack_all(n)) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(n->stream)) != hipSuccess) return fail(e);
     *out = n;
+    return MDX_OK;
+}
+
+extern "C" int mdx_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, mdx_net **out) {
+    return mdx_net_create_strided(nx, ny, nz, 1, seed, out);
+}
+
+extern "C" int mdx_net_grid(mdx_net *n, int32_t *w, int32_t *h, int32_t *d, int32_t *stride) {
+    NET_CHECK(n, "mdx_net_grid: null network");
+    if (w) *w = n->W[0];
+    if (h) *h = n->H[0];
+    if (d) *d = n->D[0];
+    if (stride) *stride = n->stride;
     return MDX_OK;
 }
 
@@ -203,6 +224,38 @@ extern "C" int mdx_net_adam_step(mdx_net *n, float lr) {
     MDX_HIP(net_adam(n, lr));
     MDX_HIP(hipStreamSynchronize(n->stream));
     return MDX_OK;
+}
+
+// the strided network's transfer operators on host arrays (the device kernels of the solver
+// coupling, with unit scale): restriction of a solver-grid field, prolongation of a network-grid one
+static int transfer_host(mdx_net *n, const double *fine_in, const float *coarse_in, float *coarse_out,
+                         double *fine_out) {
+    double *tmp = nullptr;
+    MDX_HIP(hipMalloc((void **)&tmp, (size_t)n->FV * 8));
+    hipError_t e = hipSuccess;
+    if (fine_in) {
+        e = hipMemcpyAsync(tmp, fine_in, (size_t)n->FV * 8, hipMemcpyHostToDevice, n->stream);
+        if (e == hipSuccess) e = net_restrict(n, tmp, n->target);
+        if (e == hipSuccess) e = hipMemcpyAsync(coarse_out, n->target, (size_t)n->V[0] * 4, hipMemcpyDeviceToHost, n->stream);
+    } else {
+        e = hipMemcpyAsync(n->out, coarse_in, (size_t)n->V[0] * 4, hipMemcpyHostToDevice, n->stream);
+        if (e == hipSuccess) e = net_prolong(n, n->out, tmp);
+        if (e == hipSuccess) e = hipMemcpyAsync(fine_out, tmp, (size_t)n->FV * 8, hipMemcpyDeviceToHost, n->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+    (void)hipFree(tmp);
+    MDX_HIP(e);
+    return MDX_OK;
+}
+
+extern "C" int mdx_net_restrict_host(mdx_net *n, const double *fine, float *coarse) {
+    NET_CHECK(n && fine && coarse, "mdx_net_restrict_host: null argument");
+    return transfer_host(n, fine, nullptr, coarse, nullptr);
+}
+
+extern "C" int mdx_net_prolong_host(mdx_net *n, const float *coarse, double *fine) {
+    NET_CHECK(n && coarse && fine, "mdx_net_prolong_host: null argument");
+    return transfer_host(n, nullptr, coarse, nullptr, fine);
 }
 
 // ----------------------------------------------------------------------------------- replay / training
@@ -302,8 +355,8 @@ extern "C" int mdx_net_time_forward(mdx_net *n, int32_t reps, float *ms_per_forw
 extern "C" int mdx_attach_net(mdx_handle *h, mdx_net *n, int32_t capture) {
     if (!h) { mdx_set_error("mdx_attach_net: null handle"); return MDX_ERR_ARG; }
     if (n) {
-        NET_CHECK(n->W[0] == h->d.nx && n->H[0] == h->d.ny && n->D[0] == h->d.nz,
-                  "mdx_attach_net: network grid differs from the solver grid");
+        NET_CHECK(n->FW == h->d.nx && n->FH == h->d.ny && n->FD == h->d.nz,
+                  "mdx_attach_net: the network serves another solver grid");
         NET_CHECK(h->cfg.solver == MDX_SOLVER_SOR, "mdx_attach_net: the learned guess serves the SOR solver");
         NET_CHECK(!capture || n->cap > 0, "mdx_attach_net: capture needs mdx_net_replay_reserve first");
     }

@@ -5,11 +5,16 @@ constant flow rate, dt = 5e-4, walls at y = +-(1 - A cos(2 pi x)) with A drawn u
 [amin, amax], Poiseuille start with a 0.05 perturbation from seed `seed0 + run`) runs `steps` RK3
 steps of the mixed-precision SOR solver (tolerance 1e-8, previous-step Phi guess); every `every`-th
 step its three solves are captured into the network's replay buffer on the GPU as
-(network input, converged Phi) pairs.  Then `adam_steps` Adam updates (batch 4, lr 1e-3) on pairs
-drawn uniformly from the buffer.  The parameters (FP32) and the target scale c0 are written to an
-.npz file with plain arrays only (numpy.load with allow_pickle=False reads it).
+(network input, converged Phi) pairs.  Then `adam_steps` Adam updates (batch 4, lr 1e-3, MSE) on
+pairs drawn uniformly from the buffer.  The parameters (FP32), the target scale c0 and the stride
+are written to an .npz file with plain arrays only (numpy.load with allow_pickle=False reads it).
 
-    python -m mobydiff_amd.train_guess --out unet_wavy_192x128x96.npz
+The defaults are the recipe of the learned-guess workload: 50 channels (seeds 0..49, A uniform in
+[0.05, 0.15]), 134 steps each with every solve captured: 20,100 pairs.  With the network on the
+solver grid coarsened by 2 (stride 2, its levels 96 x 64 x 48, 48 x 32 x 24, 24 x 16 x 12) a pair
+takes 2.4 MB of HBM, so all of them stay resident (47 GB).
+
+    python -m mobydiff_amd.train_guess --out unet_guess_s2_wavy_192x128x96.npz
 
 The network, its input and target, the capture and the Adam kernels are the library's own
 (mdx_net_*, mdx_attach_net); this script only drives them.
@@ -41,28 +46,39 @@ def wavy_channel(M, A, seed):
 
 def save(path, net, meta):
     p, c0 = net.get_params()
-    np.savez(path, params=p.astype(np.float32), c0=np.float64(c0), meta=np.array(json.dumps(meta)))
+    np.savez(path, params=p.astype(np.float32), c0=np.float64(c0), stride=np.int32(net.stride),
+             meta=np.array(json.dumps(meta)))
+
+
+def stride_of(path):
+    """The stride of the network an .npz written by save() was trained with (1 for older files)."""
+    with np.load(path, allow_pickle=False) as z:
+        return int(z["stride"]) if "stride" in z.files else 1
 
 
 def load(path, net):
     """Parameters and c0 from an .npz written by save() into `net`; returns the metadata dict."""
     with np.load(path, allow_pickle=False) as z:
         p, c0, meta = z["params"], float(z["c0"]), json.loads(str(z["meta"]))
+        stride = int(z["stride"]) if "stride" in z.files else 1
     if p.size != net.nparam:
         raise ValueError(f"{path}: {p.size} parameters, the network has {net.nparam}")
+    if stride != net.stride:
+        raise ValueError(f"{path}: trained with stride {stride}, the network has stride {net.stride}")
     net.set_params(np.ascontiguousarray(p, dtype=np.float32), c0)
     return meta
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--runs", type=int, default=24)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--every", type=int, default=2)
+    ap.add_argument("--runs", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=134)
+    ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--amin", type=float, default=0.05)
     ap.add_argument("--amax", type=float, default=0.15)
-    ap.add_argument("--seed0", type=int, default=1)
-    ap.add_argument("--adam-steps", type=int, default=2000)
+    ap.add_argument("--seed0", type=int, default=0)
+    ap.add_argument("--stride", type=int, default=2, help="network grid = solver grid / stride (1, 2 or 4)")
+    ap.add_argument("--adam-steps", type=int, default=10000)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--init-seed", type=int, default=0)
@@ -73,7 +89,7 @@ def main(argv=None):
 
     rng = np.random.default_rng(args.seed0)
     amps = rng.uniform(args.amin, args.amax, args.runs)
-    net = M.Net(NX, NY, NZ, seed=args.init_seed)
+    net = M.Net(NX, NY, NZ, seed=args.init_seed, stride=args.stride)
     per_run = 3 * ((args.steps + args.every - 1) // args.every)
     net.replay_reserve(args.runs * per_run)
     t0 = time.perf_counter()
@@ -90,19 +106,20 @@ def main(argv=None):
     pairs = net.replay_count()
     t1 = time.perf_counter()
     l0 = net.eval_replay(min(pairs, 64))
-    chunk, done, hist = 100, 0, []
+    chunk, done, hist = 500, 0, []
     while done < args.adam_steps:
         n = min(chunk, args.adam_steps - done)
         hist.append(round(float(net.train(n, batch=args.batch, lr=args.lr, seed=done)), 6))
         done += n
-        print(f"adam {done}/{args.adam_steps}: mean loss {hist[-1]:.5f} ({time.perf_counter() - t1:.1f} s)",
+        print(f"adam {done}/{args.adam_steps}: mean loss over the last {n} steps {hist[-1]:.5f} "
+              f"({time.perf_counter() - t1:.1f} s)",
               flush=True)
     l1 = net.eval_replay(min(pairs, 64))
     meta = {"runs": args.runs, "steps": args.steps, "every": args.every, "A": [args.amin, args.amax],
             "seeds": [args.seed0, args.seed0 + args.runs - 1], "pairs": pairs, "adam_steps": args.adam_steps,
             "batch": args.batch, "lr": args.lr, "mse_first64_before": float(l0), "mse_first64_after": float(l1),
-            "loss_per_100_steps": hist, "capture_seconds": round(t1 - t0, 1),
-            "train_seconds": round(time.perf_counter() - t1, 1), "grid": [NX, NY, NZ]}
+            "loss_per_500_steps": hist, "capture_seconds": round(t1 - t0, 1),
+            "train_seconds": round(time.perf_counter() - t1, 1), "grid": [NX, NY, NZ], "stride": args.stride}
     save(args.out, net, meta)
     print(json.dumps(meta), flush=True)
     net.close()

@@ -129,3 +129,43 @@ def adam_step(params, grads, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         out_p[k] = params[k] - lr * mh / (torch.sqrt(vh) + eps)
         out_m[k], out_v[k] = mk, vk
     return out_p, out_m, out_v
+
+
+# ----------------------------------------------------------------------------- strided coupling
+# A network with stride s runs on the solver grid coarsened by s in every direction.  Its input is
+# the restriction of the solver's fields (mean over each voxel's s^3 cells), its output reaches the
+# solver grid by trilinear interpolation between voxel centres; a cell beyond the outermost centre
+# takes that voxel's value.  Here both operators are written as separable matrices (numpy, FP64).
+def restrict_mean(fine, s):
+    """(FZ, FY, FX) -> (FZ/s, FY/s, FX/s): the mean of each s x s x s block"""
+    import numpy as np
+    fz, fy, fx = fine.shape
+    return np.asarray(fine, dtype=np.float64).reshape(fz // s, s, fy // s, s, fx // s, s).mean(axis=(1, 3, 5))
+
+
+def prolong_matrix(nf, s):
+    """(nf x nf/s) interpolation matrix of one axis: fine cell i sits at coarse coordinate
+    (i + 1/2)/s - 1/2; linear weights between the two nearest voxel centres, clamped at the ends"""
+    import numpy as np
+    nc = nf // s
+    P = np.zeros((nf, nc))
+    for i in range(nf):
+        xc = (i + 0.5) / s - 0.5
+        i0 = int(np.floor(xc))
+        t = xc - i0
+        if i0 < 0:
+            P[i, 0] = 1.0
+        elif i0 + 1 > nc - 1:
+            P[i, nc - 1] = 1.0
+        else:
+            P[i, i0] += 1.0 - t
+            P[i, i0 + 1] += t
+    return P
+
+
+def prolong_trilinear(coarse, s):
+    """(CZ, CY, CX) -> (s CZ, s CY, s CX) by the tensor product of the three axes' matrices"""
+    import numpy as np
+    cz, cy, cx = coarse.shape
+    Pz, Py, Px = prolong_matrix(cz * s, s), prolong_matrix(cy * s, s), prolong_matrix(cx * s, s)
+    return np.einsum("ia,jb,kc,abc->ijk", Pz, Py, Px, np.asarray(coarse, dtype=np.float64), optimize=True)

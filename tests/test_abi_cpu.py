@@ -79,8 +79,8 @@ def test_guess_weights_round_trip(tmp_path):
     from mobydiff_amd import train_guess
 
     class FakeNet:
-        def __init__(self, n):
-            self.nparam, self.p, self.c0 = n, None, None
+        def __init__(self, n, stride=2):
+            self.nparam, self.p, self.c0, self.stride = n, None, None, stride
 
         def get_params(self):
             return np.arange(self.nparam, dtype=np.float32) * 0.5, 1.25
@@ -96,7 +96,10 @@ def test_guess_weights_round_trip(tmp_path):
     assert meta == {"runs": 2, "pairs": 12}
     assert dst.c0 == 1.25 and np.array_equal(dst.p, np.arange(10, dtype=np.float32) * 0.5)
     with np.load(path, allow_pickle=False) as z:
-        assert set(z.files) == {"params", "c0", "meta"}
+        assert set(z.files) == {"params", "c0", "stride", "meta"}
+    assert train_guess.stride_of(path) == 2
     import pytest
     with pytest.raises(ValueError):
         train_guess.load(path, FakeNet(11))
+    with pytest.raises(ValueError, match="stride"):
+        train_guess.load(path, FakeNet(10, stride=1))

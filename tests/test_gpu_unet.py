@@ -237,7 +237,8 @@ def test_train_guess_script_round_trip(tmp_path):
     from mobydiff_amd import train_guess
     out = str(tmp_path / "w.npz")
     assert train_guess.main(["--runs", "1", "--steps", "2", "--every", "1", "--adam-steps", "2", "--out", out]) == 0
-    net = M.Net(train_guess.NX, train_guess.NY, train_guess.NZ, seed=5)
+    assert train_guess.stride_of(out) == 2
+    net = M.Net(train_guess.NX, train_guess.NY, train_guess.NZ, seed=5, stride=2)
     meta = train_guess.load(out, net)
     assert meta["pairs"] == 6 and meta["adam_steps"] == 2
     p, c0 = net.get_params()
@@ -279,3 +280,54 @@ def test_learned_fallback_is_the_plain_start(monkeypatch):
     assert fb == [1] * 12, fb
     for k in ("un", "vn", "wn", "pn"):
         assert np.array_equal(fields[0][k], fields[1][k]), k
+
+
+@pytest.mark.parametrize("s", [2, 4])
+def test_strided_transfer_operators_match_oracle(s):
+    """The strided network's restriction (mean over s^3 cells) and trilinear prolongation, run by
+    the kernels of the solver coupling, against the oracle's separable-matrix restatement."""
+    n = 64
+    net = M.Net(n, n, n, seed=0, stride=s)
+    assert net.shape == (n // s,) * 3
+    rng = np.random.default_rng(s)
+    fine = rng.standard_normal((n, n, n))
+    r = net.restrict(fine)
+    assert np.allclose(r, R.restrict_mean(fine, s), rtol=1e-6, atol=1e-7)
+    coarse = rng.standard_normal(net.shape).astype(np.float32)
+    p = net.prolong(coarse)
+    ref = R.prolong_trilinear(coarse, s)
+    assert np.abs(p - ref).max() <= 1e-12 * np.abs(ref).max()
+    net.close()
+
+
+def test_strided_learned_guess_in_the_solver():
+    """A stride-2 network (its levels at 32x16x16, 16x8x8, 8x4x4 for a 64x32x32 channel): pairs
+    captured from the solver, trained, then the learned start inside the solver meets the
+    tolerance on every solve."""
+    nx, ny, nz, dt = 64, 32, 32, 5e-4
+    geo = M.wavy_geometry(A=0.1)
+    cfg = M.channel_config(nx, ny, nz, 4.0, 2.2, 2.0, 2000.0, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8,
+                           check_every=10, precision=M.PREC_MIXED, inner_iters=10, flowrate=True,
+                           flow_height=2.0)
+    ch = M.Channel(cfg)
+    ch.compute_ibm(geo)
+    ch.init_fields(seed=0, amp=0.05, geo=geo, dt=dt)
+    net = M.Net(nx, ny, nz, seed=0, stride=2)
+    net.replay_reserve(64)
+    ch.attach_net(net, capture=True)
+    for _ in range(12):
+        ch.step(dt)
+    assert net.replay_count() == 36
+    l0 = net.eval_replay(8)
+    net.train(150, batch=4, lr=1e-3)
+    l1 = net.eval_replay(8)
+    assert l1 < 0.5 * l0, (l0, l1)
+    ch.attach_net(net, capture=False)
+    ch.set_guess(M.GUESS_LEARNED)
+    for _ in range(3):
+        st = ch.step(dt)
+        for q in range(3):
+            assert st.stage[q].converged == 1 and st.stage[q].div_inf < 1e-8
+    assert ch.div_norm() < 1.5e-8
+    ch.close()
+    net.close()
