@@ -29,6 +29,9 @@ LX, LY, LZ = 4.0, 2.2, 2.0
 RE, DT, A, AMP = 2000.0, 5e-4, 0.1, 0.05
 TOL, OMEGA = 1e-8, 1.8
 HBM_PEAK_GBS = 8000.0
+# LDS read bandwidth of the chip: 256 CUs x 128 B per clock x 2.4 GHz (CDNA LDS: 32 banks x 4 B,
+# dual-issue of 64-lane reads per clock; 160 KB per CU on CDNA 4)
+LDS_PEAK_GBS = 256 * 128 * 2.4
 # U-Net parameters trained by mobydiff_amd.train_guess on 20,100 solver pairs of 50 other wavy
 # channels (A uniform in [0.05, 0.15], seeds 0..49); the default start of the learned guess
 WEIGHTS = os.path.join(ROOT, "mobydiff_amd", "weights", "unet_guess_s2_wavy_192x128x96.npz")
@@ -89,32 +92,83 @@ def measured_traffic(kind, kz, iters=None):
     return b
 
 
+def lds_plan(info):
+    """(px, py, pz, bx, by, bz) of the LDS-resident solver's FP32 plan, from its info line
+    ("... fp32 8x8x4 workgroups of 24x16x24 cells ...")."""
+    import re
+    m = re.search(r"fp32 (\d+)x(\d+)x(\d+) workgroups of (\d+)x(\d+)x(\d+) cells", info)
+    return tuple(int(v) for v in m.groups()) if m else None
+
+
+def lds_bytes_per_iteration(plan, elem=4):
+    """LDS bytes one red+black iteration of the LDS-resident solver moves, over all workgroups:
+    every relaxed cell takes six LDS loads and one store (the column forms: the z neighbour shared
+    by two cells of one colour is loaded once); colour 0 relaxes the subdomain's half and the
+    colour-0 cells of the face ring, colour 1 the subdomain's other half; and every exchanged value
+    is read from LDS once to be published and written once into the receiver's halo (faces two
+    layers deep, edges one cell thick; none across the walls)."""
+    px, py, pz, bx, by, bz = plan
+    total = 0
+    for wy in range(py):
+        has = {(0, -1, 0): wy > 0, (0, 1, 0): wy < py - 1}   # y neighbours (x, z periodic)
+        nb = lambda dx, dy, dz: has.get((0, dy, 0), True) if dy else True   # noqa: E731
+        ring = (by * bz) * (nb(-1, 0, 0) + nb(1, 0, 0)) // 2 + (bx * bz) * (nb(0, -1, 0) + nb(0, 1, 0)) // 2 \
+            + (bx * by) * (nb(0, 0, -1) + nb(0, 0, 1)) // 2
+        relaxed = bx * by * bz + ring
+        faces = 2 * (by * bz * (nb(-1, 0, 0) + nb(1, 0, 0)) + bx * bz * (nb(0, -1, 0) + nb(0, 1, 0))
+                     + bx * by * (nb(0, 0, -1) + nb(0, 0, 1)))
+        edges = sum(bz for sx in (-1, 1) for sy in (-1, 1) if nb(0, sy, 0)) \
+            + sum(by for sx in (-1, 1) for sz in (-1, 1)) + sum(bx for sy in (-1, 1) for sz in (-1, 1) if nb(0, sy, 0))
+        total += px * pz * (7 * elem * relaxed + 2 * elem * (faces + edges))
+    return total
+
+
 def roofline(times, kind, info=""):
-    """Roofline entry of the sweep kernel: canonical bytes of the red+black iterations one launch
-    performs (44 B/cell per iteration in FP32, 88 in FP64) over the launch's event-timed duration.
-    An LDS-resident launch performs a whole batch of iterations from LDS, so its figure can exceed
-    the HBM peak: that is the point of the temporal blocking."""
+    """Roofline entry of the red-black sweep kernel over its event-timed launches.
+    * LDS-resident batches (rb_lds_kernel): the iterations run from LDS, so the bound is LDS
+      bandwidth: LDS bytes per launch (lds_bytes_per_iteration x the launch's iterations) over the
+      launch time, against the LDS peak.  The HBM side is the counter-measured traffic per launch
+      (FETCH_SIZE + WRITE_SIZE, scaled by a known-size copy) over the same time.
+    * streaming one-colour sweeps: canonical bytes (44 B per cell and iteration in FP32, 88 in FP64,
+      half per colour) over the launch time against the HBM peak; at 192x128x96 the working set sits
+      in the 256 MB MALL, so the counter-measured HBM bytes are reported beside it and the fraction
+      is taken of those."""
     if times.sweep_launches == 0:
         return None
     avg_s = times.sweep_ms / times.sweep_launches * 1e-3
     cells_per_launch = times.sweep_cells / times.sweep_launches
-    bytes_per_launch = BYTES_PER_CELL_ITER[kind] / 2.0 * cells_per_launch
-    ach = bytes_per_launch / avg_s / 1e9
+    canonical = BYTES_PER_CELL_ITER[kind] / 2.0 * cells_per_launch
     iters = cells_per_launch / (2.0 * NX * NY * NZ)
     lds = info.startswith("LDS-resident") and (kind == "fp32" or "; fp64 " in info)
     if lds:
-        kz = "lds"
-        label = f"LDS-resident red-black SOR batch ({kind}, {iters:.1f} iterations per launch on average)"
-    elif iters > 0.75:
-        kz = "fused"
-        label = f"red-black SOR iteration ({kind}, both colours in one launch)"
+        plan = lds_plan(info)
+        lds_b = lds_bytes_per_iteration(plan) * iters if plan else None
+        traffic = measured_traffic(kind, "lds", iters)
+        out = {"kernel": f"rb_lds_kernel: LDS-resident red-black SOR batch ({kind}, {iters:.1f} iterations per "
+                         "launch on average)",
+               "bound": "lds", "unit": "GB/s", "peak": LDS_PEAK_GBS,
+               "achieved": round(lds_b / avg_s / 1e9, 1) if lds_b else None,
+               "frac": round(lds_b / avg_s / 1e9 / LDS_PEAK_GBS, 4) if lds_b else None,
+               "lds_bytes_per_launch": lds_b, "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2),
+               "us_per_iteration": round(avg_s * 1e6 / iters, 3) if iters else None,
+               "canonical_bytes_if_streamed": canonical}
+        if traffic:
+            out["hbm"] = {"achieved": round(traffic / avg_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                          "frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4), "source": "rocprofv3 counters"}
+        return out
+    if iters > 0.75:
+        kz, label = "fused", f"red-black SOR iteration ({kind}, both colours in one launch)"
     else:
         kz = os.environ.get("MOBYDIFF_SWEEP_KZ", "4")
-        label = f"red-black SOR sweep ({kind}, one colour)"
-    return {"kernel": label, "bound": "hbm", "achieved": round(ach, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(kind, kz, iters if kz == "lds" else None), "avg_launch_us": round(avg_s * 1e6, 2),
-            "bytes_per_launch": bytes_per_launch}
+        label = f"sor_sweep_zm_kernel: red-black SOR sweep ({kind}, one colour)"
+    traffic = measured_traffic(kind, kz, None)
+    out = {"kernel": label, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS, "bytes_per_launch": canonical,
+           "canonical_gbs": round(canonical / avg_s / 1e9, 1), "traffic": traffic,
+           "avg_launch_us": round(avg_s * 1e6, 2)}
+    if traffic:
+        out["achieved"] = round(traffic / avg_s / 1e9, 1)
+        out["frac"] = round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+    return out
 
 
 def momentum_roofline(ch, cells, ibm, dt, workload):
