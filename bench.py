@@ -29,9 +29,10 @@ LX, LY, LZ = 4.0, 2.2, 2.0
 RE, DT, A, AMP = 2000.0, 5e-4, 0.1, 0.05
 TOL, OMEGA = 1e-8, 1.8
 HBM_PEAK_GBS = 8000.0
-# LDS read bandwidth of the chip: 256 CUs x 128 B per clock x 2.4 GHz (CDNA LDS: 32 banks x 4 B,
-# dual-issue of 64-lane reads per clock; 160 KB per CU on CDNA 4)
-LDS_PEAK_GBS = 256 * 128 * 2.4
+# LDS read bandwidth of the chip: 256 CUs x 256 B per clock x 2.4 GHz (CDNA 4: 160 KB of LDS per CU
+# with twice CDNA 3's 128 B per clock of read bandwidth); mdx_lds_read_rate measures the sustained
+# rate on the device, reported beside it
+LDS_PEAK_GBS = 256 * 256 * 2.4
 # U-Net parameters trained by mobydiff_amd.train_guess on 20,100 solver pairs of 50 other wavy
 # channels (A uniform in [0.05, 0.15], seeds 0..49); the default start of the learned guess
 WEIGHTS = os.path.join(ROOT, "mobydiff_amd", "weights", "unet_guess_s2_wavy_192x128x96.npz")
@@ -460,6 +461,10 @@ def main():
         "speedup_vs_plain_fp64": round(v_h / max(v_p, v_pl), 3),
         "roofline": roofline(t_h, "fp32", head.info()),
     }
+    if out["roofline"] and out["roofline"].get("bound") == "lds":
+        rate, per_clk = M.lds_read_rate()
+        out["roofline"]["lds_read_probe"] = {"achieved": round(rate / 1e9, 1), "bytes_per_clock_per_cu": round(per_clk, 1),
+                                             "source": "mdx_lds_read_rate: 16-byte LDS reads, 8 workgroups per CU"}
     out["stencil_roofline"] = momentum_in_step(t_h, cells, True, "wavy channel 192x128x96 (IBM)")
     # the same kernel launched 20 times back to back from one state (its working set then partly
     # served by the 256 MB MALL)

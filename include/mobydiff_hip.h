@@ -194,6 +194,11 @@ int mdx_reset_phase_times(mdx_handle *h);
  * Allocates a temporary copy of the old right-hand sides.  No counterpart in the reference. */
 int mdx_time_momentum(mdx_handle *h, double dta, double dtb, double dtg, int32_t reps, float *ms_per_launch);
 
+/* sustained LDS read bandwidth of the current device (16-byte reads, 8 workgroups per CU), the
+ * ceiling the LDS-resident sweeps are measured against; bytes per clock and CU optional.  Allocates
+ * a small device buffer.  No counterpart in the reference. */
+int mdx_lds_read_rate(double *bytes_per_s, double *bytes_per_clock_per_cu);
+
 /* wall-normal coordinates of cell centres and v-faces (box coordinates, length ny+2 each) */
 int mdx_get_y_coordinates(mdx_handle *h, double *yc, double *yv);
 

@@ -95,6 +95,13 @@ def _check(rc):
         raise MdxError(rc, lib().mdx_last_error().decode())
 
 
+def lds_read_rate():
+    """(bytes/s, bytes per clock and CU) of the device's sustained LDS reads (mdx_lds_read_rate)"""
+    b, c = C.c_double(), C.c_double()
+    _check(lib().mdx_lds_read_rate(C.byref(b), C.byref(c)))
+    return b.value, c.value
+
+
 _iolib = None
 
 

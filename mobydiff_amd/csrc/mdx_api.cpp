@@ -140,11 +140,11 @@ static void tridiag_tables(const mdx_handle *h, std::vector<double> &cp, std::ve
         }
 }
 
-// Row coefficients and modified wavenumbers of the one-wave-per-line solve (tridiag_pcr_kernel):
+// Row coefficients and modified wavenumbers of the one-wave-per-line solve (tridiag_wave_kernel):
 // row j (0-based) reads a_j x_{j-1} + (bd_j - k) x_j + c_j x_{j+1} with k = lamx[kx] + lamz[kz];
 // a_0 = 0 and c_{ny-1} = 0 (the homogeneous Neumann rows), the same expressions as the Thomas
 // tables above.
-static void pcr_tables(const mdx_handle *h, std::vector<double> &lx, std::vector<double> &lz,
+static void tridiag_row_tables(const mdx_handle *h, std::vector<double> &lx, std::vector<double> &lz,
                        std::vector<double> &a, std::vector<double> &bd, std::vector<double> &c) {
     const int nx = h->d.nx, ny = h->d.ny, nz = h->d.nz, nxh = nx / 2 + 1;
     const double dx = h->dx, dy = h->dy, dz = h->dz;
@@ -238,7 +238,8 @@ static void free_all(mdx_handle *h) {
                     h->mu32[2], h->fft_rhs, h->hat, h->cp, h->den_inv, h->arow, h->tw_x, h->tw_z,
                     h->red_part, h->red_max, h->max_part, h->rb_faces, h->phi_hist, h->res64,
                     h->fft_phi, h->fft_res, h->fft_mean, h->hat32, h->cp32, h->den32, h->arow32, h->tw_x32,
-                    h->tw_z32, h->rb_xl32, h->rb_xl64, h->cvel, h->lamx, h->lamz, h->pcr_a, h->bdrow, h->crow};
+                    h->tw_z32, h->rb_xl32, h->rb_xl64, h->cvel, h->lamx, h->lamz, h->tw_a, h->bdrow, h->crow, h->lamx32, h->lamz32,
+                    h->tw_a32, h->bdrow32, h->crow32};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -358,7 +359,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if (const char *sl = std::getenv("MOBYDIFF_LDS_SPIN_LIMIT")) h->rb_spin_limit = std::max(0, std::atoi(sl));
     if (const char *nm = std::getenv("MOBYDIFF_NET_MARGIN")) h->net_margin = std::atof(nm);
     if (const char *tg = std::getenv("MOBYDIFF_NET_TIME_GATE")) h->net_time_gate = tg[0] != '0';
-    if (const char *pt = std::getenv("MOBYDIFF_PCR_TRIDIAG")) h->pcr_tridiag = pt[0] != '0';
+    if (const char *wt = std::getenv("MOBYDIFF_WAVE_TRIDIAG")) h->wave_tridiag = wt[0] != '0';
     if (const char *mm = std::getenv("MOBYDIFF_MERGE_MAX")) h->merge_max = std::max(1, std::atoi(mm));
     if (const char *lf = std::getenv("MOBYDIFF_LDS_FORM")) {
         if (std::strcmp(lf, "pairs") == 0) h->rb_form = 1;
@@ -417,14 +418,14 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
             (e = dalloc(&h->tw_z, nz)) != hipSuccess)
             return fail(e, "FFT workspace");
         std::vector<double> plx, plz, pa, pbd, pc;
-        pcr_tables(h, plx, plz, pa, pbd, pc);
+        tridiag_row_tables(h, plx, plz, pa, pbd, pc);
         if ((e = dalloc(&h->lamx, nxh)) != hipSuccess || (e = dalloc(&h->lamz, nz)) != hipSuccess ||
-            (e = dalloc(&h->pcr_a, ny)) != hipSuccess || (e = dalloc(&h->bdrow, ny)) != hipSuccess ||
+            (e = dalloc(&h->tw_a, ny)) != hipSuccess || (e = dalloc(&h->bdrow, ny)) != hipSuccess ||
             (e = dalloc(&h->crow, ny)) != hipSuccess)
             return fail(e, "FFT workspace");
         if ((e = hipMemcpy(h->lamx, plx.data(), nxh * 8, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(h->lamz, plz.data(), nz * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(h->pcr_a, pa.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->tw_a, pa.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(h->bdrow, pbd.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(h->crow, pc.data(), ny * 8, hipMemcpyHostToDevice)) != hipSuccess)
             return fail(e, "FFT table upload");
@@ -449,6 +450,18 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
                 (e = hipMemcpy(h->arow32, ar32.data(), ny * 4, hipMemcpyHostToDevice)) != hipSuccess ||
                 (e = hipMemcpy(h->tw_x32, twx32.data(), nx * 4, hipMemcpyHostToDevice)) != hipSuccess ||
                 (e = hipMemcpy(h->tw_z32, twz32.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess)
+                return fail(e, "mixed FFT table upload");
+            const std::vector<float> plx32 = to32(plx), plz32 = to32(plz), pa32 = to32(pa), pbd32 = to32(pbd),
+                                     pc32 = to32(pc);
+            if ((e = dalloc(&h->lamx32, nxh)) != hipSuccess || (e = dalloc(&h->lamz32, nz)) != hipSuccess ||
+                (e = dalloc(&h->tw_a32, ny)) != hipSuccess || (e = dalloc(&h->bdrow32, ny)) != hipSuccess ||
+                (e = dalloc(&h->crow32, ny)) != hipSuccess)
+                return fail(e, "mixed FFT workspace");
+            if ((e = hipMemcpy(h->lamx32, plx32.data(), nxh * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(h->lamz32, plz32.data(), nz * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(h->tw_a32, pa32.data(), ny * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(h->bdrow32, pbd32.data(), ny * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(h->crow32, pc32.data(), ny * 4, hipMemcpyHostToDevice)) != hipSuccess)
                 return fail(e, "mixed FFT table upload");
         }
     }

@@ -232,116 +232,194 @@ __global__ __launch_bounds__(64) void tridiag_kernel(Dims d, double2 *hat, const
     }
 }
 
-// 3'. The same tridiagonal solves by a PCR-Thomas hybrid, one wavefront per (kx,kz) line, every
-//     value of the line held in registers: the spectrum is read once and written once (16 B per
-//     cell), against twice each plus the two FP64 tables for the thread-per-line recurrence.
-//     Lane l holds rows j = l, l + 64, ..., l + 64 (M-1) (0-based) with their coefficients
+// 3'. The same tridiagonal solves with one wavefront per (kx,kz) line, every value of the line in
+//     registers: the spectrum is read once and written once (16 B per cell), against twice each
+//     plus the two FP64 tables for the thread-per-line recurrence.  Partitioned Thomas: lane p
+//     holds the M consecutive rows p M .. p M + M - 1 (0-based; rows past ny are identity rows) with
 //     a_j (sub-diagonal), b_j = bd_j - k (k = lamx[kx] + lamz[kz], the modified wavenumbers of
-//     pressure_fft.f90:20-100), c_j and the complex right-hand side.  Six parallel-cyclic-reduction
-//     steps (coupling distance 1, 2, ..., 32; the neighbours' coefficients by cross-lane shuffles)
-//     leave every row coupled only to the rows 64 away, i.e. each lane holds an independent
-//     M-row tridiagonal system, which it solves by Thomas.  Rows beyond the line (ny not a
-//     multiple of 64) and neighbours beyond its ends are identity rows (b = 1, a = c = d = 0).
+//     pressure_fft.f90:20-100), c_j and the complex right-hand side.
+//       1. Each lane eliminates its M - 1 interior rows by Thomas, for three right-hand sides: the
+//          data, and the couplings to the last row of the lane below and to its own last row, so
+//          that x_i = y_i + v_i x_(last row of lane p - 1) + w_i x_(last row of lane p).
+//       2. Substituting these into the lanes' last rows (with the next lane's first interior row)
+//          leaves a tridiagonal system of 64 rows in the lanes' last unknowns, the Schur
+//          complement of the interior rows; every lane runs its Thomas recurrence in step, reading
+//          row p's coefficients from lane p (readlane), and lane p keeps what it needs.
+//       3. Each lane back-substitutes its interior rows.
+//     A Schur complement of a diagonally dominant matrix is diagonally dominant, so no step
+//     pivots and the result is as accurate as the serial recurrence (parallel cyclic reduction
+//     over the lanes, tried first, left residuals 100-1000x larger on the tanh-stretched grid).
 //     The mean mode (kx = kz = 0) pins row 1 to zero, as the reference does (dp(1) = 0), by
-//     making it an identity row.  The system is diagonally dominant (weakly for the mean mode,
-//     an irreducible M-matrix), so neither part pivots.  Lines are assigned to the eight waves
-//     of a workgroup consecutively: for a fixed row the eight lines' values are 128 contiguous
-//     bytes of the spectrum.
+//     making it an identity row.  Lines go to the eight waves of a workgroup consecutively: for
+//     a fixed row the eight lines' values are 128 contiguous bytes of the spectrum.
+__device__ inline double readlane_any(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline float readlane_any(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 template <typename T>
-__device__ inline T shfl_any(T v, int src) {
-    return __shfl(v, src, 64);
-}
-template <>
-__device__ inline double2 shfl_any<double2>(double2 v, int src) {
-    return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+__device__ inline T shfl_dn1(T v) {   // the value of lane + 1 (lane 63 reads lane 0's; unused there)
+    return __shfl(v, (threadIdx.x + 1) & 63, 64);
 }
 
-static constexpr int PCR_WAVES = 8;
+static constexpr int TW_WAVES = 8;
 
-template <int M>
-__global__ __launch_bounds__(64 * PCR_WAVES) void tridiag_pcr_kernel(int nlines, int nxh, int ny, size_t plane,
-                                                                    double2 *hat, const double *__restrict__ lamx,
-                                                                    const double *__restrict__ lamz,
-                                                                    const double *__restrict__ arow,
-                                                                    const double *__restrict__ bdrow,
-                                                                    const double *__restrict__ crow) {
+// C: double2 (the FP64 solve; the mean mode pins row 1) or float2 (the FP32 defect solves; the
+// mean mode is solved apart in FP64, so its line is zeroed, MEAN_ZERO)
+template <typename C, int M, bool MEAN_ZERO>
+__global__ __launch_bounds__(64 * TW_WAVES) void tridiag_wave_kernel(int nlines, int nxh, int ny, size_t plane, C *hat,
+                                                                    const typename Cx<C>::R *__restrict__ lamx,
+                                                                    const typename Cx<C>::R *__restrict__ lamz,
+                                                                    const typename Cx<C>::R *__restrict__ arow,
+                                                                    const typename Cx<C>::R *__restrict__ bdrow,
+                                                                    const typename Cx<C>::R *__restrict__ crow) {
+    using R = typename Cx<C>::R;
+    constexpr int K = M - 1;   // interior rows per lane
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * PCR_WAVES + (threadIdx.x >> 6);   // the wave's line
-    if (t >= nlines) return;                                     // whole waves only
+    const int t = blockIdx.x * TW_WAVES + (threadIdx.x >> 6);   // the wave's line
+    if (t >= nlines) return;                                    // whole waves only
+    if (MEAN_ZERO && t == 0) {
+        for (int j = lane; j < ny; j += 64) hat[plane * j] = Cx<C>::mk(R(0), R(0));
+        return;
+    }
     const int kx = t % nxh, kz = t / nxh;
-    const double k = lamx[kx] + lamz[kz];
-    double a[M], b[M], c[M];
-    double2 d[M];
+    const R k = lamx[kx] + lamz[kz];
+    R a[M], b[M], c[M];
+    C d[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-        const int j = lane + 64 * q;
+        const int j = lane * M + q;
         if (j < ny && !(t == 0 && j == 0)) {
             a[q] = arow[j];
             c[q] = crow[j];
             b[q] = bdrow[j] - k;
             d[q] = hat[(size_t)t + plane * j];
         } else {
-            a[q] = 0.0;
-            c[q] = 0.0;
-            b[q] = 1.0;
-            d[q] = make_double2(0.0, 0.0);
+            a[q] = R(0);
+            c[q] = R(0);
+            b[q] = R(1);
+            d[q] = Cx<C>::mk(R(0), R(0));
         }
     }
+    // 1. interior rows 0 .. K-1: y (data), v (coupling a_0 to the lane below), w (coupling
+    //    c_(K-1) to the lane's last row)
+    C y[K > 0 ? K : 1];
+    R v[K > 0 ? K : 1], w[K > 0 ? K : 1];
+    if (K > 0) {
+        R cp[K > 0 ? K : 1];
 #pragma unroll
-    for (int delta = 1; delta < 64; delta <<= 1) {
-        double rb[M];
-#pragma unroll
-        for (int q = 0; q < M; ++q) rb[q] = 1.0 / b[q];
-        // row i - delta (lane - delta of the same row set, or of the previous one) and i + delta
-        const int lo = (lane - delta) & 63, hi = (lane + delta) & 63;
-        double la[M], lc[M], lr[M], ha[M], hc[M], hr[M];
-        double2 ld[M], hd[M];
-#pragma unroll
-        for (int q = 0; q < M; ++q) {
-            la[q] = shfl_any(a[q], lo);
-            lc[q] = shfl_any(c[q], lo);
-            lr[q] = shfl_any(rb[q], lo);
-            ld[q] = shfl_any(d[q], lo);
-            ha[q] = shfl_any(a[q], hi);
-            hc[q] = shfl_any(c[q], hi);
-            hr[q] = shfl_any(rb[q], hi);
-            hd[q] = shfl_any(d[q], hi);
+        for (int i = 0; i < K; ++i) {
+            const R ai = i > 0 ? a[i] : R(0);
+            const R den = i > 0 ? b[i] - ai * cp[i - 1] : b[i];
+            const R rden = R(1) / den;
+            cp[i] = (i < K - 1 ? c[i] : R(0)) * rden;
+            const R rv = i == 0 ? -a[0] : R(0), rw = i == K - 1 ? -c[K - 1] : R(0);
+            if (i == 0) {
+                y[i] = Cx<C>::mk(d[i].x * rden, d[i].y * rden);
+                v[i] = rv * rden;
+                w[i] = rw * rden;
+            } else {
+                y[i] = Cx<C>::mk((d[i].x - ai * y[i - 1].x) * rden, (d[i].y - ai * y[i - 1].y) * rden);
+                v[i] = (rv - ai * v[i - 1]) * rden;
+                w[i] = (rw - ai * w[i - 1]) * rden;
+            }
         }
-        const bool wrap_lo = lane < delta, wrap_hi = lane + delta >= 64;
 #pragma unroll
-        for (int q = M - 1; q >= 0; --q) {
-            // neighbours across the lane wrap belong to the row set q - 1 (below) or q + 1 (above)
-            double na, nc, nr, pa, pc, pr;
-            double2 nd, pd;
-            if (!wrap_lo) { na = la[q]; nc = lc[q]; nr = lr[q]; nd = ld[q]; }
-            else if (q > 0) { na = la[q - 1]; nc = lc[q - 1]; nr = lr[q - 1]; nd = ld[q - 1]; }
-            else { na = 0.0; nc = 0.0; nr = 1.0; nd = make_double2(0.0, 0.0); }
-            if (!wrap_hi) { pa = ha[q]; pc = hc[q]; pr = hr[q]; pd = hd[q]; }
-            else if (q + 1 < M) { pa = ha[q + 1]; pc = hc[q + 1]; pr = hr[q + 1]; pd = hd[q + 1]; }
-            else { pa = 0.0; pc = 0.0; pr = 1.0; pd = make_double2(0.0, 0.0); }
-            const double k1 = a[q] * nr, k2 = c[q] * pr;
-            a[q] = -na * k1;
-            b[q] = b[q] - nc * k1 - pa * k2;
-            c[q] = -pc * k2;
-            d[q] = make_double2(d[q].x - nd.x * k1 - pd.x * k2, d[q].y - nd.y * k1 - pd.y * k2);
+        for (int i = K - 2; i >= 0; --i) {
+            y[i] = Cx<C>::mk(y[i].x - cp[i] * y[i + 1].x, y[i].y - cp[i] * y[i + 1].y);
+            v[i] = v[i] - cp[i] * v[i + 1];
+            w[i] = w[i] - cp[i] * w[i + 1];
         }
     }
-    // Thomas over the lane's rows (row q couples to rows q - 1 and q + 1 only)
-#pragma unroll
-    for (int q = 1; q < M; ++q) {
-        const double w = a[q] / b[q - 1];
-        b[q] = b[q] - w * c[q - 1];
-        d[q] = make_double2(d[q].x - w * d[q - 1].x, d[q].y - w * d[q - 1].y);
+    // 2. the reduced row of the lane's last unknown: ra x_(p-1) + rb x_p + rc x_(p+1) = rd
+    const R ae = a[M - 1], ce = c[M - 1];
+    R ra, rb = b[M - 1], rc;
+    C rd = d[M - 1];
+    {
+        R vn, wn;
+        C yn;
+        if (K > 0) {
+            vn = shfl_dn1(v[0]);
+            wn = shfl_dn1(w[0]);
+            yn = Cx<C>::mk(shfl_dn1(y[0].x), shfl_dn1(y[0].y));
+        } else {
+            vn = R(1);   // the next lane's "interior" is its last row itself
+            wn = R(0);
+            yn = Cx<C>::mk(R(0), R(0));
+        }
+        if (lane == 63) { vn = R(0); wn = R(0); yn = Cx<C>::mk(R(0), R(0)); }   // c of the last row is 0
+        if (K > 0) {
+            ra = ae * v[K - 1];
+            rb = rb + ae * w[K - 1];
+            rd = Cx<C>::mk(rd.x - ae * y[K - 1].x, rd.y - ae * y[K - 1].y);
+        } else {
+            ra = ae;
+        }
+        if (K > 0) {
+            rb = rb + ce * vn;
+            rc = ce * wn;
+            rd = Cx<C>::mk(rd.x - ce * yn.x, rd.y - ce * yn.y);
+        } else {
+            rc = lane == 63 ? R(0) : ce;
+        }
     }
-    double2 x[M];
-    x[M - 1] = make_double2(d[M - 1].x / b[M - 1], d[M - 1].y / b[M - 1]);
+    // Thomas over the 64 reduced rows, every lane in step; lane p keeps cp_p and dp_p
+    R cpl = R(0);
+    C dpl = Cx<C>::mk(R(0), R(0));
+    {
+        R cprev = R(0);
+        C dprev = Cx<C>::mk(R(0), R(0));
+        for (int p = 0; p < 64; ++p) {
+            const R rap = readlane_any(ra, p), rbp = readlane_any(rb, p), rcp = readlane_any(rc, p);
+            const R rdx = readlane_any(rd.x, p), rdy = readlane_any(rd.y, p);
+            const R rden = R(1) / (rbp - rap * cprev);
+            cprev = rcp * rden;
+            dprev = Cx<C>::mk((rdx - rap * dprev.x) * rden, (rdy - rap * dprev.y) * rden);
+            if (lane == p) {
+                cpl = cprev;
+                dpl = dprev;
+            }
+        }
+    }
+    C xe = Cx<C>::mk(R(0), R(0)), xb = Cx<C>::mk(R(0), R(0));   // x of this lane's last row and the one below
+    {
+        C xn = Cx<C>::mk(R(0), R(0));
+        for (int p = 63; p >= 0; --p) {
+            const R cpp = readlane_any(cpl, p), dpx = readlane_any(dpl.x, p), dpy = readlane_any(dpl.y, p);
+            xn = Cx<C>::mk(dpx - cpp * xn.x, dpy - cpp * xn.y);
+            if (lane == p) xe = xn;
+            if (lane == p + 1) xb = xn;
+        }
+    }
+    // 3. interior rows, then the stores
 #pragma unroll
-    for (int q = M - 2; q >= 0; --q)
-        x[q] = make_double2((d[q].x - c[q] * x[q + 1].x) / b[q], (d[q].y - c[q] * x[q + 1].y) / b[q]);
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-        const int j = lane + 64 * q;
-        if (j < ny) hat[(size_t)t + plane * j] = x[q];
+    for (int i = 0; i < K; ++i) {
+        const C xi = Cx<C>::mk(y[i].x + v[i] * xb.x + w[i] * xe.x, y[i].y + v[i] * xb.y + w[i] * xe.y);
+        const int j = lane * M + i;
+        if (j < ny) hat[(size_t)t + plane * j] = xi;
+    }
+    const int je = lane * M + M - 1;
+    if (je < ny) hat[(size_t)t + plane * je] = xe;
+}
+
+// launches the one-wave-per-line solve for ny <= 256 (false: the caller runs the Thomas kernel)
+template <typename C, bool MEAN_ZERO>
+static bool launch_tridiag_wave(hipStream_t st, const Dims &d, C *hat, const typename Cx<C>::R *lamx,
+                                const typename Cx<C>::R *lamz, const typename Cx<C>::R *a,
+                                const typename Cx<C>::R *bd, const typename Cx<C>::R *c) {
+    const int nxh = d.nx / 2 + 1, nlines = nxh * d.nz, m = (d.ny + 63) / 64;
+    const size_t plane = (size_t)nxh * d.nz;
+    const dim3 grid((nlines + TW_WAVES - 1) / TW_WAVES), blk(64 * TW_WAVES);
+    switch (m) {
+    case 1: hipLaunchKernelGGL((tridiag_wave_kernel<C, 1, MEAN_ZERO>), grid, blk, 0, st, nlines, nxh, d.ny, plane, hat, lamx, lamz, a, bd, c); return true;
+    case 2: hipLaunchKernelGGL((tridiag_wave_kernel<C, 2, MEAN_ZERO>), grid, blk, 0, st, nlines, nxh, d.ny, plane, hat, lamx, lamz, a, bd, c); return true;
+    case 3: hipLaunchKernelGGL((tridiag_wave_kernel<C, 3, MEAN_ZERO>), grid, blk, 0, st, nlines, nxh, d.ny, plane, hat, lamx, lamz, a, bd, c); return true;
+    case 4: hipLaunchKernelGGL((tridiag_wave_kernel<C, 4, MEAN_ZERO>), grid, blk, 0, st, nlines, nxh, d.ny, plane, hat, lamx, lamz, a, bd, c); return true;
+    default: return false;
     }
 }
 
@@ -465,21 +543,10 @@ hipError_t launch_fft_forward(mdx_handle *h, double dtg) {
 
 hipError_t launch_tridiag(mdx_handle *h) {
     const Dims d = h->d;
-    const int nxh = d.nx / 2 + 1, nlines = nxh * d.nz;
-    const int m = (d.ny + 63) / 64;
-    if (h->pcr_tridiag && m <= 4) {
-        const size_t plane = (size_t)nxh * d.nz;
-        const dim3 grid((nlines + PCR_WAVES - 1) / PCR_WAVES), blk(64 * PCR_WAVES);
-        double2 *hat = (double2 *)h->hat;
-#define PCR_CASE(MM)                                                                                         \
-    case MM:                                                                                                 \
-        hipLaunchKernelGGL(tridiag_pcr_kernel<MM>, grid, blk, 0, h->stream, nlines, nxh, d.ny, plane, hat,    \
-                           h->lamx, h->lamz, h->pcr_a, h->bdrow, h->crow);                                   \
-        break;
-        switch (m) { PCR_CASE(1) PCR_CASE(2) PCR_CASE(3) PCR_CASE(4) }
-#undef PCR_CASE
+    const int nlines = (d.nx / 2 + 1) * d.nz;
+    if (h->wave_tridiag && launch_tridiag_wave<double2, false>(h->stream, d, (double2 *)h->hat, h->lamx, h->lamz, h->tw_a,
+                                                     h->bdrow, h->crow))
         return hipGetLastError();
-    }
     hipLaunchKernelGGL(tridiag_kernel, dim3((nlines + 63) / 64), dim3(64), 0, h->stream, d, (double2 *)h->hat,
                        h->cp, h->den_inv, h->arow);
     return hipGetLastError();
@@ -821,8 +888,10 @@ hipError_t launch_fft32_solve_add(mdx_handle *h) {
     hipLaunchKernelGGL(zfft_kernel<float2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
                        (float2 *)h->hat32, (const float2 *)h->tw_z32, tx, 0);
     const int nlines = nxh * d.nz;
-    hipLaunchKernelGGL(tridiag32_kernel, dim3((nlines + 63) / 64), dim3(64), 0, h->stream, d, (float2 *)h->hat32,
-                       h->cp32, h->den32, h->arow32);
+    if (!(h->wave_tridiag && launch_tridiag_wave<float2, true>(h->stream, d, (float2 *)h->hat32, h->lamx32, h->lamz32,
+                                                     h->tw_a32, h->bdrow32, h->crow32)))
+        hipLaunchKernelGGL(tridiag32_kernel, dim3((nlines + 63) / 64), dim3(64), 0, h->stream, d, (float2 *)h->hat32,
+                           h->cp32, h->den32, h->arow32);
     hipLaunchKernelGGL(zfft_kernel<float2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
                        (float2 *)h->hat32, (const float2 *)h->tw_z32, tx, 1);
     hipLaunchKernelGGL(ixfft32_add_kernel, dim3((nrows + lines - 1) / lines), dim3(FFT_THREADS), shx, h->stream, d,

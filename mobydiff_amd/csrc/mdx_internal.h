@@ -222,8 +222,9 @@ struct mdx_handle {
     double *res64 = nullptr;       // residual field of the previous-Phi guess (network input)
     double *cvel = nullptr;        // cell-centred velocity for output (3 interior arrays, allocated on first use)
     // one-wave-per-line tridiagonal solve: modified wavenumbers (nxh, nz) and row coefficients (ny)
-    double *lamx = nullptr, *lamz = nullptr, *pcr_a = nullptr, *bdrow = nullptr, *crow = nullptr;
-    bool pcr_tridiag = true;       // MOBYDIFF_PCR_TRIDIAG=0: the thread-per-line Thomas recurrence
+    double *lamx = nullptr, *lamz = nullptr, *tw_a = nullptr, *bdrow = nullptr, *crow = nullptr;
+    float *lamx32 = nullptr, *lamz32 = nullptr, *tw_a32 = nullptr, *bdrow32 = nullptr, *crow32 = nullptr;
+    bool wave_tridiag = true;      // MOBYDIFF_WAVE_TRIDIAG=0: the thread-per-line Thomas recurrence
     double hist_dtg[3] = {0.0, 0.0, 0.0};
     // learned-guess gate per stage: consecutive solves the network's start lost, solves left to
     // skip, whether the current solve starts from the network's guess, and the sweeps of the last

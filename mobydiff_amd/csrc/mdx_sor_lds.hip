@@ -1015,3 +1015,66 @@ hipError_t launch_rb_lds(mdx_handle *h, bool fp32, int iters, int zero_init, con
 }
 
 }  // namespace mdx
+
+// ------------------------------------------------------------------------------- LDS read rate
+// The chip's sustained LDS read bandwidth, the ceiling of the LDS-resident sweeps: every thread of
+// 8 workgroups per CU reads 16-byte vectors of a 64 KB LDS array in a rotating, bank-conflict-free
+// pattern (index masked into the array) and folds them, so nothing is optimised away.
+namespace mdx {
+static constexpr int LDSR_VEC = 4096, LDSR_ITERS = 4096;
+__global__ __launch_bounds__(256) void lds_read_rate_kernel(float *out) {
+    __shared__ float4 buf[LDSR_VEC];
+    for (int q = threadIdx.x; q < LDSR_VEC; q += 256) buf[q] = make_float4((float)q, (float)q + 1.f, 0.f, 1.f);
+    __syncthreads();
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int idx = threadIdx.x;
+#pragma unroll 8
+    for (int i = 0; i < LDSR_ITERS; ++i) {
+        const float4 v = buf[idx & (LDSR_VEC - 1)];
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+        idx += 256;
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = acc.x + acc.y + acc.z + acc.w;
+}
+}  // namespace mdx
+
+extern "C" int mdx_lds_read_rate(double *bytes_per_s, double *bytes_per_clock_per_cu) {
+    using namespace mdx;
+    if (!bytes_per_s) {
+        mdx_set_error("mdx_lds_read_rate: null argument");
+        return MDX_ERR_ARG;
+    }
+    int dev = 0;
+    MDX_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    MDX_HIP(hipGetDeviceProperties(&p, dev));
+    const int blocks = p.multiProcessorCount * 8;
+    float *out = nullptr;
+    MDX_HIP(hipMalloc(&out, (size_t)blocks * 256 * sizeof(float)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(lds_read_rate_kernel, dim3(blocks), dim3(256), 0, nullptr, out);   // warm-up
+        e = hipEventRecord(e0, nullptr);
+    }
+    for (int r = 0; r < 10 && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(lds_read_rate_kernel, dim3(blocks), dim3(256), 0, nullptr, out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    MDX_HIP(e);
+    const double bytes = 10.0 * blocks * 256.0 * LDSR_ITERS * 16.0;
+    *bytes_per_s = bytes / (ms * 1e-3);
+    if (bytes_per_clock_per_cu) *bytes_per_clock_per_cu = *bytes_per_s / (p.clockRate * 1e3) / p.multiProcessorCount;
+    return MDX_OK;
+}

@@ -133,7 +133,7 @@ dump_final = final.bin
     g = O.Grid(NX, NY, NZ, lx=1.0, ly=1.0, lz=1.0, re=100.0)
     O.ref_write_vtk(g, f, 1e-4, 0.0 + 1e-4 + 1e-4, 0.0, 2, str(tmp_path / "ref"))
     theirs = (tmp_path / "ref_2.vtk").read_bytes()
-    assert len(ours.splitlines()) == 13 + NX * NY * NZ
+    assert len(ours.splitlines()) == 12 + NX * NY * NZ
     assert ours == theirs
 
 
