@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # with twice CDNA 3's 128 B per clock of read bandwidth); mdx_lds_read_rate measures the sustained
 # rate on the device, reported beside it
 LDS_PEAK_GBS = 256 * 256 * 2.4
-# U-Net parameters trained by mobydiff_amd.train_guess on 20,100 solver pairs of 50 other wavy
-# channels (A uniform in [0.05, 0.15], seeds 0..49); the default start of the learned guess
-WEIGHTS = os.path.join(ROOT, "mobydiff_amd", "weights", "unet_guess_s2_wavy_192x128x96.npz")
+# U-Net parameters (stride 4) trained by mobydiff_amd.train_guess on 20,100 solver pairs of 50 wavy
+# channels (A uniform in [0.05, 0.15], seeds 0..49); the default network of the learned guess
+WEIGHTS = os.path.join(ROOT, "mobydiff_amd", "weights", "unet_guess_s4_wavy_192x128x96.npz")
 # canonical algorithmic bytes per cell for one full red+black iteration (FP64 88 B, FP32 44 B); a
 # one-colour launch covers half of that, a fused launch all of it
 BYTES_PER_CELL_ITER = {"fp64": 88.0, "fp32": 44.0}
@@ -300,7 +300,7 @@ def main():
     ap.add_argument("--net-weights", default=WEIGHTS if os.path.exists(WEIGHTS) else "",
                     help="U-Net parameters written by mobydiff_amd.train_guess (.npz); empty: train on the "
                          "warm-up steps' pairs instead")
-    ap.add_argument("--stride", type=int, default=2,
+    ap.add_argument("--stride", type=int, default=4,
                     help="stride of a network trained on the warm-up steps (network grid = grid / stride)")
     ap.add_argument("--flat-steps", type=int, default=10, help="steps of the flat FFT production line (0: skip)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,

@@ -359,6 +359,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if (const char *sl = std::getenv("MOBYDIFF_LDS_SPIN_LIMIT")) h->rb_spin_limit = std::max(0, std::atoi(sl));
     if (const char *nm = std::getenv("MOBYDIFF_NET_MARGIN")) h->net_margin = std::atof(nm);
     if (const char *tg = std::getenv("MOBYDIFF_NET_TIME_GATE")) h->net_time_gate = tg[0] != '0';
+    if (const char *nf = std::getenv("MOBYDIFF_NET_FALLBACK")) h->net_fallback_prev = std::strcmp(nf, "prev") == 0;
     if (const char *wt = std::getenv("MOBYDIFF_WAVE_TRIDIAG")) h->wave_tridiag = wt[0] != '0';
     if (const char *mm = std::getenv("MOBYDIFF_MERGE_MAX")) h->merge_max = std::max(1, std::atoi(mm));
     if (const char *lf = std::getenv("MOBYDIFF_LDS_FORM")) {
@@ -1029,6 +1030,8 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
 
 // pinned words of the mixed solve's two check slots: [CHECK_PIN + 4 * slot] = max, error word, seq
 static constexpr int CHECK_PIN = 16;
+// pinned words of the learned start's comparison: max |rhs| (the residual of Phi = 0)
+static constexpr int RZ_PIN = 40;
 
 // Wait for the residual check that posts sequence number `seq` to the pinned words [w, w + 3):
 // [max, error word, seq].  The host polls the sequence word (no event between the kernels); now
@@ -1071,8 +1074,8 @@ static int wait_check(mdx_handle *h, int w, unsigned long long seq, double *r) {
 // solve, and later only when the residuals seen so far predict (geometric decay) that check k is
 // still at least SPEC_MARGIN times above the tolerance.
 // first_seq != 0: the first check is already queued in slot 0 with that sequence number (the fused
-// start kernel).  key: the RK stage + 3 * (start from Phi_0): the previous solve with the same key
-// gives the merge prior.
+// start kernel).  key: the RK stage + 3 * the kind of start (0 Phi = 0, 1 Phi_0, 2 the network's):
+// the previous solve with the same key gives the merge prior (none for network starts).
 static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq, int key) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
     const int MERGE_MAX = h->merge_max;   // at most this many inner_iters batches run as one
@@ -1129,7 +1132,8 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long lon
                 const int blocks = (int)(std::log(c.tol / rp) / std::log(q) / c.inner_iters);
                 if (blocks >= 3 && MERGE_MAX > 1) m = c.inner_iters * std::min(MERGE_MAX, blocks - 1);
             }
-        } else if (r2 < 0.0 && prior > it && MERGE_MAX > 1) {
+        } else if (r2 < 0.0 && prior > it && MERGE_MAX > 1 && key < 6) {
+            // (no prior for network starts: their solves vary too much from one step to the next)
             const int blocks = (prior - it) / c.inner_iters;   // after the pending check
             if (blocks >= 3) m = c.inner_iters * std::min(MERGE_MAX, blocks - 1);
         }
@@ -1220,7 +1224,9 @@ static GuessPlan guess_plan(mdx_handle *h, int s, double dtg, mdx_solve_stats *s
     p.have_prev = h->hist_dtg[s] > 0.0;
     p.fac = p.have_prev ? dtg / h->hist_dtg[s] : 0.0;
     const int mode = h->cfg.guess;
-    p.learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0;
+    // the network completes Phi_0: without a history of the stage there is nothing to complete,
+    // and the solve starts from the previous pressure
+    p.learned = n && mode == MDX_GUESS_LEARNED && n->c0 > 0.0 && p.have_prev;
     h->net_started[s] = false;
     // gate: after the network's start has lost twice in a row for this stage, it is evaluated
     // again only after 4, 16, 64, then every NET_SKIP_MAX skipped solves (previous-Phi start then)
@@ -1237,9 +1243,12 @@ static GuessPlan guess_plan(mdx_handle *h, int s, double dtg, mdx_solve_stats *s
         st->guess_skipped = 1;
         st->fallback = 1;
     }
-    // the learned mode falls back to the plain start (Phi = 0: the previous pressure); Phi_0 serves
-    // it only as the network's input and the base of its correction
-    p.use_prev = p.have_prev && mode == MDX_GUESS_PHI_PREV;
+    // the learned mode falls back to the plain start (Phi = 0: the previous pressure), where Phi_0
+    // serves only as the network's input and the base of its correction, or with
+    // net_fallback_prev to the previous-step guess Phi_0
+    p.use_prev = p.have_prev && (mode == MDX_GUESS_PHI_PREV || (mode == MDX_GUESS_LEARNED && h->net_fallback_prev));
+    // in the learned mode every solve the network does not start is a fallback
+    if (n && mode == MDX_GUESS_LEARNED && !p.learned) st->fallback = 1;
     p.build_input = n && (p.learned || h->net_capture);
     p.simple = !p.learned && !p.build_input;
     p.prev = p.use_prev ? h->phi_hist + (size_t)s * h->ni : nullptr;
@@ -1264,10 +1273,19 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
     ev_record(h, 8);
     // Phi_0 in phi2 and its residual field (the network's input)
     if (have_prev) MDX_HIP(launch_scale_copy64(h, h->phi2, prev, fac, h->ni));
+    // the residual of Phi_0 (the network's input field and its max-norm rp); rp goes to pinned
+    // host memory behind the network's kernels and is read after the stream's next synchronisation
+    constexpr int RP_PIN = 32;
     double rp = -1.0;
     MDX_HIP(launch_reset_max(h));
     MDX_HIP(launch_residual_field64(h, have_prev ? h->phi2 : h->phi, h->res64));
-    if ((rc = read_max(h, &rp))) return rc;
+    MDX_HIP(hipMemcpyAsync(h->h_pinned + RP_PIN, h->red_max, 16, hipMemcpyDeviceToHost, h->stream));
+    auto take_rp = [&]() -> int {
+        rp = h->h_pinned[RP_PIN];
+        int err;
+        std::memcpy(&err, &h->h_pinned[RP_PIN + 1], sizeof err);
+        return err ? lds_timeout_seen(h) : MDX_OK;
+    };
     {
         hipStream_t saved = n->stream;
         n->stream = h->stream;
@@ -1281,8 +1299,11 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
     if (learned) {
         MDX_HIP(launch_reset_max(h));
         MDX_HIP(launch_sor_residual64(h, h->phi2));
-        if ((rc = read_max(h, &rg))) return rc;
+        if ((rc = read_max(h, &rg))) return rc;   // synchronises the stream: rp is there too
+    } else {
+        MDX_HIP(hipStreamSynchronize(h->stream));
     }
+    if ((rc = take_rp())) return rc;
     ev_record(h, 9);
     if (h->timing && learned) {
         float ms = 0.f;
@@ -1290,7 +1311,7 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
         if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
     }
     // rp is the residual of Phi_0 (with a history), else of the previous pressure (Phi = 0); the
-    // learned mode's fallback is Phi = 0 also with a history (rz below)
+    // learned mode falls back to Phi = 0 (rz below), or with net_fallback_prev to Phi_0
     const bool prev_start = use_prev;
     if (!learned) {   // capture only
         if (prev_start) {
@@ -1302,13 +1323,10 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
         return MDX_OK;
     }
     st->guess_div_inf = rg;
-    // the fallback start Phi = 0 with a history: its residual, phi being zero here
+    // the fallback start Phi = 0 with a history: its residual, max |rhs| (sor_prepare wrote it to
+    // the pinned slot; the stream has synchronised since)
     double rz = -1.0;
-    if (have_prev && !prev_start) {
-        MDX_HIP(launch_reset_max(h));
-        MDX_HIP(launch_sor_residual64(h, h->phi));
-        if ((rc = read_max(h, &rz))) return rc;
-    }
+    if (have_prev && !prev_start) rz = h->h_pinned[RZ_PIN];
     // the network's start is taken only when it clearly beats the other start (residual below
     // net_margin times it): a marginally smaller start residual saves no sweeps but costs an
     // inference per solve.  A start not taken counts as a loss; a start taken is judged by the
@@ -1418,13 +1436,19 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
             first_seq = ++h->chk_seq;
             MDX_HIP(launch_sor_start(h, gp.prev, gp.fac, h->d_pinned + CHECK_PIN, first_seq));
         } else {
-            MDX_HIP(launch_sor_prepare(h));
+            // the learned start falling back to Phi = 0 compares against that start's residual,
+            // max |rhs|, formed here on the way
+            const bool want_rz = gp.learned && gp.have_prev && !gp.use_prev;
+            MDX_HIP(launch_sor_prepare(h, want_rz ? h->d_pinned + RZ_PIN : nullptr, 0));
             rc = initial_guess(h, stage, dtg, gp, &st, &r_first);
             if (rc && rc != MDX_LDS_TIMEOUT) return rc;
         }
-        if (rc == MDX_OK)
-            rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage + (gp.use_prev ? 3 : 0))
+        if (rc == MDX_OK) {
+            // the merge prior is the previous solve of this stage from the same kind of start
+            const int kind = h->net_started[stage] ? 2 : (gp.use_prev ? 1 : 0);
+            rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage + 3 * kind)
                                                : sor_solve64(h, &st, r_first);
+        }
         if (rc == MDX_LDS_TIMEOUT) {
             // a neighbour wait of the LDS-resident sweeps gave up (another stream or process held
             // CUs): the solve restarts from Phi = 0 on the streaming sweeps, and the next solve

@@ -234,39 +234,48 @@ __global__ __launch_bounds__(64) void tridiag_kernel(Dims d, double2 *hat, const
 
 // 3'. The same tridiagonal solves with one wavefront per (kx,kz) line, every value of the line in
 //     registers: the spectrum is read once and written once (16 B per cell), against twice each
-//     plus the two FP64 tables for the thread-per-line recurrence.  Partitioned Thomas: lane p
-//     holds the M consecutive rows p M .. p M + M - 1 (0-based; rows past ny are identity rows) with
-//     a_j (sub-diagonal), b_j = bd_j - k (k = lamx[kx] + lamz[kz], the modified wavenumbers of
-//     pressure_fft.f90:20-100), c_j and the complex right-hand side.
-//       1. Each lane eliminates its M - 1 interior rows by Thomas, for three right-hand sides: the
-//          data, and the couplings to the last row of the lane below and to its own last row, so
-//          that x_i = y_i + v_i x_(last row of lane p - 1) + w_i x_(last row of lane p).
-//       2. Substituting these into the lanes' last rows (with the next lane's first interior row)
-//          leaves a tridiagonal system of 64 rows in the lanes' last unknowns, the Schur
-//          complement of the interior rows; every lane runs its Thomas recurrence in step, reading
-//          row p's coefficients from lane p (readlane), and lane p keeps what it needs.
-//       3. Each lane back-substitutes its interior rows.
-//     A Schur complement of a diagonally dominant matrix is diagonally dominant, so no step
-//     pivots and the result is as accurate as the serial recurrence (parallel cyclic reduction
-//     over the lanes, tried first, left residuals 100-1000x larger on the tanh-stretched grid).
-//     The mean mode (kx = kz = 0) pins row 1 to zero, as the reference does (dp(1) = 0), by
-//     making it an identity row.  Lines go to the eight waves of a workgroup consecutively: for
-//     a fixed row the eight lines' values are 128 contiguous bytes of the spectrum.
-__device__ inline double readlane_any(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ inline float readlane_any(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
+//     plus the two FP64 tables for the thread-per-line recurrence.  Lane l holds rows
+//     j = l, l + 64, ..., l + 64 (M-1) (0-based) with a_j (sub-diagonal), b_j = bd_j - k
+//     (k = lamx[kx] + lamz[kz], the modified wavenumbers of pressure_fft.f90:20-100), c_j and the
+//     complex right-hand side; rows past ny, and neighbours past the ends, are identity rows.
+//       * Parallel cyclic reduction, six steps (coupling distance 1, 2, ..., 32, the neighbours'
+//         coefficients by cross-lane shuffles), leaves every row coupled only to the rows 64 away:
+//         each lane then holds an independent M-row system, solved by Thomas.
+//       * One step of iterative refinement in the same precision: the residual d - A x with the
+//         original rows (neighbours x_(j-1), x_(j+1) by shuffles) goes through the same reduction,
+//         whose multipliers and Thomas factors were kept in registers, and the correction is added.
+//     Reduction alone left residuals 100-1000x larger than the serial recurrence's on the
+//     tanh-stretched grid (weakly dominant low-wavenumber rows); with the refinement step they are
+//     within 2x of it.  Every lane works through the whole solve (no serial chain across lanes:
+//     a line's serial recurrence in one wave would cost 64 lanes' worth of FP64 issue per step).
+//     The mean mode (kx = kz = 0) pins row 1 to zero, as the reference does (dp(1) = 0), by making
+//     it an identity row.  Lines go to the eight waves of a workgroup consecutively: for a fixed row
+//     the eight lines' values are 128 contiguous bytes of the spectrum.
 template <typename T>
-__device__ inline T shfl_dn1(T v) {   // the value of lane + 1 (lane 63 reads lane 0's; unused there)
-    return __shfl(v, (threadIdx.x + 1) & 63, 64);
+__device__ inline T shfl_any(T v, int src) {
+    return __shfl(v, src, 64);
+}
+template <>
+__device__ inline double2 shfl_any<double2>(double2 v, int src) {
+    return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+template <>
+__device__ inline float2 shfl_any<float2>(float2 v, int src) {
+    return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
-static constexpr int TW_WAVES = 8;
+static constexpr int TW_WAVES = 8, TW_STEPS = 6;
+
+// the value of row i - delta (lo) or i + delta (hi) for row set q, from the shuffled copies of all
+// row sets (rows across the lane wrap belong to set q - 1 / q + 1; beyond the ends: `edge`)
+template <int M, typename V>
+__device__ inline V pick_lo(const V (&sh)[M], int q, bool wrap, V edge) {
+    return !wrap ? sh[q] : (q > 0 ? sh[q > 0 ? q - 1 : 0] : edge);
+}
+template <int M, typename V>
+__device__ inline V pick_hi(const V (&sh)[M], int q, bool wrap, V edge) {
+    return !wrap ? sh[q] : (q + 1 < M ? sh[q + 1 < M ? q + 1 : q] : edge);
+}
 
 // C: double2 (the FP64 solve; the mean mode pins row 1) or float2 (the FP32 defect solves; the
 // mean mode is solved apart in FP64, so its line is zeroed, MEAN_ZERO)
@@ -278,132 +287,138 @@ __global__ __launch_bounds__(64 * TW_WAVES) void tridiag_wave_kernel(int nlines,
                                                                     const typename Cx<C>::R *__restrict__ bdrow,
                                                                     const typename Cx<C>::R *__restrict__ crow) {
     using R = typename Cx<C>::R;
-    constexpr int K = M - 1;   // interior rows per lane
     const int lane = threadIdx.x & 63;
     const int t = blockIdx.x * TW_WAVES + (threadIdx.x >> 6);   // the wave's line
     if (t >= nlines) return;                                    // whole waves only
+    const C zero2 = Cx<C>::mk(R(0), R(0));
     if (MEAN_ZERO && t == 0) {
-        for (int j = lane; j < ny; j += 64) hat[plane * j] = Cx<C>::mk(R(0), R(0));
+        for (int j = lane; j < ny; j += 64) hat[plane * j] = zero2;
         return;
     }
     const int kx = t % nxh, kz = t / nxh;
     const R k = lamx[kx] + lamz[kz];
-    R a[M], b[M], c[M];
-    C d[M];
+    // the original rows (kept for the residual) and the right-hand side
+    R a0[M], b0[M], c0[M];
+    C d0[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-        const int j = lane * M + q;
+        const int j = lane + 64 * q;
         if (j < ny && !(t == 0 && j == 0)) {
-            a[q] = arow[j];
-            c[q] = crow[j];
-            b[q] = bdrow[j] - k;
-            d[q] = hat[(size_t)t + plane * j];
+            a0[q] = arow[j];
+            c0[q] = crow[j];
+            b0[q] = bdrow[j] - k;
+            d0[q] = hat[(size_t)t + plane * j];
         } else {
-            a[q] = R(0);
-            c[q] = R(0);
-            b[q] = R(1);
-            d[q] = Cx<C>::mk(R(0), R(0));
+            a0[q] = R(0);
+            c0[q] = R(0);
+            b0[q] = R(1);
+            d0[q] = zero2;
         }
     }
-    // 1. interior rows 0 .. K-1: y (data), v (coupling a_0 to the lane below), w (coupling
-    //    c_(K-1) to the lane's last row)
-    C y[K > 0 ? K : 1];
-    R v[K > 0 ? K : 1], w[K > 0 ? K : 1];
-    if (K > 0) {
-        R cp[K > 0 ? K : 1];
+    // reduction of the rows, keeping the multipliers of every step
+    R a[M], b[M], c[M], k1s[TW_STEPS][M], k2s[TW_STEPS][M];
 #pragma unroll
-        for (int i = 0; i < K; ++i) {
-            const R ai = i > 0 ? a[i] : R(0);
-            const R den = i > 0 ? b[i] - ai * cp[i - 1] : b[i];
-            const R rden = R(1) / den;
-            cp[i] = (i < K - 1 ? c[i] : R(0)) * rden;
-            const R rv = i == 0 ? -a[0] : R(0), rw = i == K - 1 ? -c[K - 1] : R(0);
-            if (i == 0) {
-                y[i] = Cx<C>::mk(d[i].x * rden, d[i].y * rden);
-                v[i] = rv * rden;
-                w[i] = rw * rden;
-            } else {
-                y[i] = Cx<C>::mk((d[i].x - ai * y[i - 1].x) * rden, (d[i].y - ai * y[i - 1].y) * rden);
-                v[i] = (rv - ai * v[i - 1]) * rden;
-                w[i] = (rw - ai * w[i - 1]) * rden;
+    for (int q = 0; q < M; ++q) {
+        a[q] = a0[q];
+        b[q] = b0[q];
+        c[q] = c0[q];
+    }
+#pragma unroll
+    for (int s = 0; s < TW_STEPS; ++s) {
+        const int delta = 1 << s;
+        const int lo = (lane - delta) & 63, hi = (lane + delta) & 63;
+        const bool wlo = lane < delta, whi = lane + delta >= 64;
+        R rb[M], la[M], lc[M], lr[M], ha[M], hc[M], hr[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) rb[q] = R(1) / b[q];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            la[q] = shfl_any(a[q], lo);
+            lc[q] = shfl_any(c[q], lo);
+            lr[q] = shfl_any(rb[q], lo);
+            ha[q] = shfl_any(a[q], hi);
+            hc[q] = shfl_any(c[q], hi);
+            hr[q] = shfl_any(rb[q], hi);
+        }
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const R na = pick_lo<M>(la, q, wlo, R(0)), nc = pick_lo<M>(lc, q, wlo, R(0)),
+                    nr = pick_lo<M>(lr, q, wlo, R(1));
+            const R pa = pick_hi<M>(ha, q, whi, R(0)), pc = pick_hi<M>(hc, q, whi, R(0)),
+                    pr = pick_hi<M>(hr, q, whi, R(1));
+            const R k1 = a[q] * nr, k2 = c[q] * pr;
+            k1s[s][q] = k1;
+            k2s[s][q] = k2;
+            a[q] = -na * k1;
+            b[q] = b[q] - nc * k1 - pa * k2;
+            c[q] = -pc * k2;
+        }
+    }
+    // Thomas factors of the lane's M-row system (row q couples to rows q - 1 and q + 1 only)
+    R cp[M], rden[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const R den = q > 0 ? b[q] - a[q] * cp[q - 1] : b[q];
+        rden[q] = R(1) / den;
+        cp[q] = c[q] * rden[q];
+    }
+    // a right-hand side through the kept reduction and Thomas factors
+    auto solve = [&](C (&d)[M]) {
+#pragma unroll
+        for (int s = 0; s < TW_STEPS; ++s) {
+            const int delta = 1 << s;
+            const int lo = (lane - delta) & 63, hi = (lane + delta) & 63;
+            const bool wlo = lane < delta, whi = lane + delta >= 64;
+            C ld[M], hd[M];
+#pragma unroll
+            for (int q = 0; q < M; ++q) {
+                ld[q] = shfl_any(d[q], lo);
+                hd[q] = shfl_any(d[q], hi);
+            }
+#pragma unroll
+            for (int q = 0; q < M; ++q) {
+                const C nd = pick_lo<M>(ld, q, wlo, zero2), pd = pick_hi<M>(hd, q, whi, zero2);
+                d[q] = Cx<C>::mk(d[q].x - nd.x * k1s[s][q] - pd.x * k2s[s][q],
+                                 d[q].y - nd.y * k1s[s][q] - pd.y * k2s[s][q]);
             }
         }
 #pragma unroll
-        for (int i = K - 2; i >= 0; --i) {
-            y[i] = Cx<C>::mk(y[i].x - cp[i] * y[i + 1].x, y[i].y - cp[i] * y[i + 1].y);
-            v[i] = v[i] - cp[i] * v[i + 1];
-            w[i] = w[i] - cp[i] * w[i + 1];
+        for (int q = 0; q < M; ++q) {
+            if (q > 0) d[q] = Cx<C>::mk(d[q].x - a[q] * d[q - 1].x, d[q].y - a[q] * d[q - 1].y);
+            d[q] = Cx<C>::mk(d[q].x * rden[q], d[q].y * rden[q]);
         }
-    }
-    // 2. the reduced row of the lane's last unknown: ra x_(p-1) + rb x_p + rc x_(p+1) = rd
-    const R ae = a[M - 1], ce = c[M - 1];
-    R ra, rb = b[M - 1], rc;
-    C rd = d[M - 1];
-    {
-        R vn, wn;
-        C yn;
-        if (K > 0) {
-            vn = shfl_dn1(v[0]);
-            wn = shfl_dn1(w[0]);
-            yn = Cx<C>::mk(shfl_dn1(y[0].x), shfl_dn1(y[0].y));
-        } else {
-            vn = R(1);   // the next lane's "interior" is its last row itself
-            wn = R(0);
-            yn = Cx<C>::mk(R(0), R(0));
-        }
-        if (lane == 63) { vn = R(0); wn = R(0); yn = Cx<C>::mk(R(0), R(0)); }   // c of the last row is 0
-        if (K > 0) {
-            ra = ae * v[K - 1];
-            rb = rb + ae * w[K - 1];
-            rd = Cx<C>::mk(rd.x - ae * y[K - 1].x, rd.y - ae * y[K - 1].y);
-        } else {
-            ra = ae;
-        }
-        if (K > 0) {
-            rb = rb + ce * vn;
-            rc = ce * wn;
-            rd = Cx<C>::mk(rd.x - ce * yn.x, rd.y - ce * yn.y);
-        } else {
-            rc = lane == 63 ? R(0) : ce;
-        }
-    }
-    // Thomas over the 64 reduced rows, every lane in step; lane p keeps cp_p and dp_p
-    R cpl = R(0);
-    C dpl = Cx<C>::mk(R(0), R(0));
-    {
-        R cprev = R(0);
-        C dprev = Cx<C>::mk(R(0), R(0));
-        for (int p = 0; p < 64; ++p) {
-            const R rap = readlane_any(ra, p), rbp = readlane_any(rb, p), rcp = readlane_any(rc, p);
-            const R rdx = readlane_any(rd.x, p), rdy = readlane_any(rd.y, p);
-            const R rden = R(1) / (rbp - rap * cprev);
-            cprev = rcp * rden;
-            dprev = Cx<C>::mk((rdx - rap * dprev.x) * rden, (rdy - rap * dprev.y) * rden);
-            if (lane == p) {
-                cpl = cprev;
-                dpl = dprev;
-            }
-        }
-    }
-    C xe = Cx<C>::mk(R(0), R(0)), xb = Cx<C>::mk(R(0), R(0));   // x of this lane's last row and the one below
-    {
-        C xn = Cx<C>::mk(R(0), R(0));
-        for (int p = 63; p >= 0; --p) {
-            const R cpp = readlane_any(cpl, p), dpx = readlane_any(dpl.x, p), dpy = readlane_any(dpl.y, p);
-            xn = Cx<C>::mk(dpx - cpp * xn.x, dpy - cpp * xn.y);
-            if (lane == p) xe = xn;
-            if (lane == p + 1) xb = xn;
-        }
-    }
-    // 3. interior rows, then the stores
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-        const C xi = Cx<C>::mk(y[i].x + v[i] * xb.x + w[i] * xe.x, y[i].y + v[i] * xb.y + w[i] * xe.y);
-        const int j = lane * M + i;
-        if (j < ny) hat[(size_t)t + plane * j] = xi;
+        for (int q = M - 2; q >= 0; --q) d[q] = Cx<C>::mk(d[q].x - cp[q] * d[q + 1].x, d[q].y - cp[q] * d[q + 1].y);
+    };
+    C x[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) x[q] = d0[q];
+    solve(x);
+    // refinement: r = d - A x with the original rows, solved the same way, added
+    {
+        const int lo = (lane - 1) & 63, hi = (lane + 1) & 63;
+        const bool wlo = lane < 1, whi = lane + 1 >= 64;
+        C lx[M], hx[M], r[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            lx[q] = shfl_any(x[q], lo);
+            hx[q] = shfl_any(x[q], hi);
+        }
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const C xm = pick_lo<M>(lx, q, wlo, zero2), xp = pick_hi<M>(hx, q, whi, zero2);
+            r[q] = Cx<C>::mk(d0[q].x - (a0[q] * xm.x + b0[q] * x[q].x + c0[q] * xp.x),
+                             d0[q].y - (a0[q] * xm.y + b0[q] * x[q].y + c0[q] * xp.y));
+        }
+        solve(r);
+#pragma unroll
+        for (int q = 0; q < M; ++q) x[q] = Cx<C>::mk(x[q].x + r[q].x, x[q].y + r[q].y);
     }
-    const int je = lane * M + M - 1;
-    if (je < ny) hat[(size_t)t + plane * je] = xe;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int j = lane + 64 * q;
+        if (j < ny) hat[(size_t)t + plane * j] = x[q];
+    }
 }
 
 // launches the one-wave-per-line solve for ny <= 256 (false: the caller runs the Thomas kernel)

@@ -236,12 +236,18 @@ struct mdx_handle {
     // the time gate of the learned start (MOBYDIFF_NET_TIME_GATE=0 switches it off: the network is
     // then evaluated for every solve and judged by the residual test alone)
     bool net_time_gate = true;
+    // the learned mode's start when the network's is not taken: Phi = 0, the previous pressure
+    // (false, the default), or Phi_0, the previous-step guess (MOBYDIFF_NET_FALLBACK=prev).  Warm
+    // starts carried from step to step gather slowly converging error: from Phi_0 the solves of the
+    // 192x128x96 channel went from 40 to 170 iterations within 20 steps, while the plain start
+    // holds its count; plain solves between network starts keep the network's input fresh.
+    bool net_fallback_prev = false;
     // mixed solve: at most this many inner_iters batches run as one while the checks between them
     // are predicted to fail (MOBYDIFF_MERGE_MAX; 1 keeps every check)
     int merge_max = 8;
     // iterations of the last converged mixed solve of each RK stage and start kind (the merge
     // prior; index stage + 3 * (start from Phi_0); 0: none)
-    int mix_prev_its[6] = {0, 0, 0, 0, 0, 0};
+    int mix_prev_its[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // [stage + 3 * start kind (0 plain, 1 Phi_0, 2 network)]
     bool net_started[3] = {false, false, false};
     double base_sec[3] = {0.0, 0.0, 0.0};   // host time of the last solve of each stage not started by the network
     long proj_count = 0;
@@ -272,7 +278,7 @@ static constexpr unsigned MAX_PARTS = 1u << 17;
 // red-black solver
 hipError_t launch_sola_sweep(mdx_handle *h, int parity, double sor, double dtg);
 hipError_t launch_build_mu(mdx_handle *h, double dtg);
-hipError_t launch_sor_prepare(mdx_handle *h);
+hipError_t launch_sor_prepare(mdx_handle *h, double *rhs_max_slot = nullptr, unsigned long long seq = 0);
 // sor_prepare + Phi = fac * prev (prev null: 0) + the first defect check of a mixed solve, fused
 hipError_t launch_sor_start(mdx_handle *h, const double *prev, double fac, double *host_slot,
                             unsigned long long seq);

@@ -171,29 +171,38 @@ struct SumPhi {
 
 // ------------------------------------------------------------------------------- prepare / apply
 // After u = u* (pressure_redblack.f90:64-84): Dirichlet v on the walls, rhs = div(u), Phi = 0.
+// With part != nullptr the block maxima of |rhs| go to part: the residual max-norm of the start
+// Phi = 0, whose residual is the right-hand side itself.
 template <bool STRETCH>
 __global__ __launch_bounds__(256) void sor_prepare_kernel(GridK g, double vbot, double vtop,
                                                           const double *__restrict__ us,
                                                           const double *__restrict__ vs,
                                                           const double *__restrict__ ws, double *rhs,
-                                                          double *phi, double idx, double idy, double idz) {
+                                                          double *phi, double idx, double idy, double idz,
+                                                          double *part) {
     const Dims d = g.d;
     int tx_, ty_, tz_;
-    if (!xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_)) return;
+    const bool tile = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);
+    if (!tile && !part) return;
     const int i = tx_ * 64 + threadIdx.x + 1;
     const int j = ty_ * 4 + threadIdx.y + 1;
     const int k = tz_ + 1;
-    if (i > d.nx || j > d.ny) return;
-    const bool per2 = g.per[1];
-    const int ip = i == d.nx ? 1 : i + 1, kp = k == d.nz ? 1 : k + 1;
-    const int jp = (per2 && j == d.ny) ? 1 : j + 1;
-    const size_t c = d.g(i, j, k);
-    // v of u* with the wall values in place (the projection writes them into vn)
-    const double v0 = (!per2 && j == 1) ? vbot : vs[c];
-    const double v1 = (!per2 && j == d.ny) ? vtop : vs[d.g(i, jp, k)];
-    const double dvy = STRETCH ? (v1 - v0) * g.ihc[j] : (v1 - v0) * idy;
-    rhs[d.in(i, j, k)] = (us[d.g(ip, j, k)] - us[c]) * idx + dvy + (ws[d.g(i, j, kp)] - ws[c]) * idz;
-    phi[d.in(i, j, k)] = 0.0;
+    double m = 0.0;
+    if (tile && i <= d.nx && j <= d.ny) {
+        const bool per2 = g.per[1];
+        const int ip = i == d.nx ? 1 : i + 1, kp = k == d.nz ? 1 : k + 1;
+        const int jp = (per2 && j == d.ny) ? 1 : j + 1;
+        const size_t c = d.g(i, j, k);
+        // v of u* with the wall values in place (the projection writes them into vn)
+        const double v0 = (!per2 && j == 1) ? vbot : vs[c];
+        const double v1 = (!per2 && j == d.ny) ? vtop : vs[d.g(i, jp, k)];
+        const double dvy = STRETCH ? (v1 - v0) * g.ihc[j] : (v1 - v0) * idy;
+        const double r = (us[d.g(ip, j, k)] - us[c]) * idx + dvy + (ws[d.g(i, j, kp)] - ws[c]) * idz;
+        rhs[d.in(i, j, k)] = r;
+        phi[d.in(i, j, k)] = 0.0;
+        m = fabs(r);
+    }
+    if (part) block_max_part(m, part);
 }
 
 // The start of a mixed-precision solve in one pass (what sor_prepare, the previous-Phi copy and the
@@ -296,17 +305,22 @@ __global__ __launch_bounds__(256) void proj_ghost_kernel(Dims d, int walled_y, d
     }
 }
 
-hipError_t launch_sor_prepare(mdx_handle *h) {
+// rhs and Phi = 0; with rhs_max_slot != nullptr also max |rhs| (the residual of Phi = 0) into that
+// pinned host slot (value, error word, then seq), read after the stream's next synchronisation
+hipError_t launch_sor_prepare(mdx_handle *h, double *rhs_max_slot, unsigned long long seq) {
     GridK g = grid_k(h);
     dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
     const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
+    double *part = rhs_max_slot ? h->max_part : nullptr;
     if (h->stretched)
         hipLaunchKernelGGL(sor_prepare_kernel<true>, grd, blk, 0, h->stream, g, vb, vt, h->us, h->vs, h->ws,
-                           h->rhs, h->phi, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz);
+                           h->rhs, h->phi, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz, part);
     else
         hipLaunchKernelGGL(sor_prepare_kernel<false>, grd, blk, 0, h->stream, g, vb, vt, h->us, h->vs, h->ws,
-                           h->rhs, h->phi, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz);
-    return hipGetLastError();
+                           h->rhs, h->phi, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !rhs_max_slot) return e;
+    return launch_max_final(h, grd.x, rhs_max_slot, seq);
 }
 
 // un = u* - mu grad Phi (likewise v, w) and pn += Phi / dt on the interior, reading u* directly;

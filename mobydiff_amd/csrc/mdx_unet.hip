@@ -508,12 +508,16 @@ __global__ __launch_bounds__(256) void input_stats_partial(long V, const double 
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-__global__ void input_stats_final(const double *part, int nb, double V, double *stat) {
-    if (threadIdx.x != 0) return;
+// one wave folds the partial sums: lane l adds partials l, l + 64, ... in order, then a shuffle
+// tree (a fixed order, so the statistics repeat exactly)
+__global__ __launch_bounds__(64) void input_stats_final(const double *part, int nb, double V, double *stat) {
     double a = 0;
-    for (int q = 0; q < nb; ++q) a += part[q];
-    const double r = sqrt(a / V);
-    stat[0] = r > 1e-300 ? r : 1.0;
+    for (int q = threadIdx.x; q < nb; q += 64) a += part[q];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (threadIdx.x == 0) {
+        const double r = sqrt(a / V);
+        stat[0] = r > 1e-300 ? r : 1.0;
+    }
 }
 
 // Phi guess (FP64, interior layout) = Phi_0 + out * s_r * c0 (in place when phi == phi0)

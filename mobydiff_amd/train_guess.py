@@ -10,11 +10,11 @@ pairs drawn uniformly from the buffer.  The parameters (FP32), the target scale 
 are written to an .npz file with plain arrays only (numpy.load with allow_pickle=False reads it).
 
 The defaults are the recipe of the learned-guess workload: 50 channels (seeds 0..49, A uniform in
-[0.05, 0.15]), 134 steps each with every solve captured: 20,100 pairs.  With the network on the
-solver grid coarsened by 2 (stride 2, its levels 96 x 64 x 48, 48 x 32 x 24, 24 x 16 x 12) a pair
-takes 2.4 MB of HBM, so all of them stay resident (47 GB).
+[0.05, 0.15]), 134 steps each with every solve captured: 20,100 pairs.  The network runs on the
+solver grid coarsened by 4 (stride 4: its levels 48 x 32 x 24, 24 x 16 x 12, 12 x 8 x 6), so a
+pair takes 0.3 MB of HBM and all of them stay resident (6 GB; 47 GB at stride 2).
 
-    python -m mobydiff_amd.train_guess --out unet_guess_s2_wavy_192x128x96.npz
+    python -m mobydiff_amd.train_guess --out unet_guess_s4_wavy_192x128x96.npz
 
 The network, its input and target, the capture and the Adam kernels are the library's own
 (mdx_net_*, mdx_attach_net); this script only drives them.
@@ -77,7 +77,7 @@ def main(argv=None):
     ap.add_argument("--amin", type=float, default=0.05)
     ap.add_argument("--amax", type=float, default=0.15)
     ap.add_argument("--seed0", type=int, default=0)
-    ap.add_argument("--stride", type=int, default=2, help="network grid = solver grid / stride (1, 2 or 4)")
+    ap.add_argument("--stride", type=int, default=4, help="network grid = solver grid / stride (1, 2 or 4)")
     ap.add_argument("--adam-steps", type=int, default=10000)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)

@@ -2,7 +2,7 @@
 seed 0, Re_b = 2000, constant flow rate, dt = 5e-4), FP32 red-black sweeps in FP64 defect
 correction to ||div||_inf < 1e-8.
 
-The network is the stride-2 U-Net trained by mobydiff_amd.train_guess on 20,100 pairs of 50 other
+The network is the stride-4 U-Net trained by mobydiff_amd.train_guess on 20,100 pairs of 50 other
 wavy channels (mobydiff_amd/weights/).  From one state (six steps after the seeded start) the same
 steps run twice: from the plain start (the previous pressure, Phi = 0) and with the learned
 guess, whose start the solver takes whenever its residual beats the plain start's (the time gate
@@ -21,7 +21,7 @@ import mobydiff_amd as M
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-WEIGHTS = os.path.join(ROOT, "mobydiff_amd", "weights", "unet_guess_s2_wavy_192x128x96.npz")
+WEIGHTS = os.path.join(ROOT, "mobydiff_amd", "weights", "unet_guess_s4_wavy_192x128x96.npz")
 NX, NY, NZ, DT = 192, 128, 96, 5e-4
 
 

@@ -237,8 +237,8 @@ def test_train_guess_script_round_trip(tmp_path):
     from mobydiff_amd import train_guess
     out = str(tmp_path / "w.npz")
     assert train_guess.main(["--runs", "1", "--steps", "2", "--every", "1", "--adam-steps", "2", "--out", out]) == 0
-    assert train_guess.stride_of(out) == 2
-    net = M.Net(train_guess.NX, train_guess.NY, train_guess.NZ, seed=5, stride=2)
+    assert train_guess.stride_of(out) == 4
+    net = M.Net(train_guess.NX, train_guess.NY, train_guess.NZ, seed=5, stride=4)
     meta = train_guess.load(out, net)
     assert meta["pairs"] == 6 and meta["adam_steps"] == 2
     p, c0 = net.get_params()
@@ -247,18 +247,23 @@ def test_train_guess_script_round_trip(tmp_path):
     net.close()
 
 
-def test_learned_fallback_is_the_plain_start(monkeypatch):
+@pytest.mark.parametrize("fallback", ["prev", "zero"])
+def test_learned_fallback_is_the_other_start(monkeypatch, fallback):
     """With the network's start never taken (MOBYDIFF_NET_MARGIN=0), every solve of the learned mode
-    falls back to the previous pressure (Phi = 0), the plain mode's start: the fields after several
-    steps are bit-identical to the plain mode's, and every solve reports the fallback."""
+    falls back: to the previous pressure (Phi = 0, the plain mode's start; the default) or, with
+    MOBYDIFF_NET_FALLBACK=prev, to the previous-step guess Phi_0 (the previous-Phi mode's start).
+    The fields after several steps are bit-identical to that mode's, and every solve reports the
+    fallback."""
     monkeypatch.setenv("MOBYDIFF_NET_MARGIN", "0")
+    monkeypatch.setenv("MOBYDIFF_NET_FALLBACK", fallback)
+    other = M.GUESS_PHI_PREV if fallback == "prev" else M.GUESS_PREVIOUS
     nx, ny, nz, dt = 64, 32, 32, 5e-4
     geo = M.wavy_geometry(A=0.1)
     fields, fb = [], []
     for learned in (False, True):
         cfg = M.channel_config(nx, ny, nz, 4.0, 2.2, 2.0, 2000.0, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8,
                                check_every=10, precision=M.PREC_MIXED, inner_iters=10, flowrate=True,
-                               flow_height=2.0)
+                               flow_height=2.0, guess=other)
         ch = M.Channel(cfg)
         ch.compute_ibm(geo)
         ch.init_fields(seed=0, amp=0.05, geo=geo, dt=dt)
