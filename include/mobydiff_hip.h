@@ -169,6 +169,20 @@ int mdx_center_velocity(mdx_handle *h, double *uc, double *vc, double *wc);
 
 /* ||div u||_inf over interior cells (FP64) */
 int mdx_div_norm(mdx_handle *h, double *div_inf);
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): mdx_div_norm without a
+ * handle and without any allocation.  un/vn/wn are device arrays of (nz+2) x (ny+2) x (nx+2)
+ * doubles in the handle's layout (one ghost layer per side, x fastest); *div_inf is a device
+ * double that receives ||div u||_inf over the interior cells, bit for bit what mdx_div_norm
+ * returns for the same fields, after the work queued on `stream` (NULL: the null stream) is done.
+ * periodic[3] flags x, y, z as in mdx_config.  The per-workgroup partial maxima live in a
+ * file-scope device array of the library (131072 doubles, 1 MiB, per GPU, held until the process
+ * ends), so grids needing more than 131072 workgroups of 64 x 4 cells (about 33.5 M cells) are
+ * rejected with MDX_ERR_ARG before anything is launched.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+int mdx_div_norm_noalloc(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz, const int32_t *periodic,
+                         const double *un, const double *vn, const double *wn, double *div_inf, void *stream);
 /* constant flow rate: bulk-velocity correction of u* for the stage with dt_gamma; returns du */
 int mdx_flowrate_shift(mdx_handle *h, double dt_gamma, double *du);
 /* bulk velocity of u over the channel height */

@@ -1,6 +1,8 @@
 // mdx_stencil.hip -- momentum predictor, ghost-cell kernels, reductions, flow-rate control and
 // the initial field, for gfx950.  FP64 throughout; compiled with -ffp-contract=off so every
 // expression rounds exactly as the reference's (step.f90, boundary.f90) does on the CPU.
+#include <mutex>
+
 #include "mdx_internal.h"
 
 namespace mdx {
@@ -787,4 +789,70 @@ hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int us
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------- no-allocation norm
+// mdx_div_norm_noalloc keeps the per-workgroup partial maxima of div_max_kernel in this file-scope
+// device array instead of a handle's buffer: one copy per GPU, held until the process ends.  Host
+// code never passes the array's name to a kernel (in host code it is only a stand-in symbol); it
+// passes the device address hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) double scratch[MAX_PARTS];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// the device address of `scratch` on device dev (looked up once per device, with dev current)
+static hipError_t scratch_address(int dev, double **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
 }  // namespace mdx
+
+#define NOALLOC_CHECK(cond, code, msg) \
+    do {                               \
+        if (!(cond)) {                 \
+            mdx_set_error(msg);        \
+            return code;               \
+        }                              \
+    } while (0)
+
+extern "C" int mdx_div_norm_noalloc(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz,
+                                    const int32_t *periodic, const double *un, const double *vn, const double *wn,
+                                    double *div_inf, void *stream) {
+    using namespace mdx;
+    NOALLOC_CHECK(nx > 0 && ny > 0 && nz > 0 && lx > 0.0 && ly > 0.0 && lz > 0.0, MDX_ERR_ARG,
+              "mdx_div_norm_noalloc: grid sizes and lengths must be positive");
+    NOALLOC_CHECK(periodic && un && vn && wn && div_inf, MDX_ERR_ARG, "mdx_div_norm_noalloc: null argument");
+    const unsigned nb = xcd_grid(nx, ny, nz, BX, BY);
+    NOALLOC_CHECK((double)nx * ny * nz < 2.0e9 && nb <= MAX_PARTS, MDX_ERR_ARG,
+              "mdx_div_norm_noalloc: grid too large for the static partial-max buffer (" + std::to_string(MAX_PARTS) +
+                  " workgroups of 64 x 4 cells)");
+    int dev = 0;
+    MDX_HIP(hipGetDevice(&dev));
+    NOALLOC_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, MDX_ERR_UNSUPPORTED, "mdx_div_norm_noalloc: device index too large");
+    double *part = nullptr;
+    MDX_HIP(scratch_address(dev, &part));
+    GridK g{};
+    g.d = Dims{nx, ny, nz, nx + 2, ny + 2};
+    g.dx = lx / (double)nx;   // the spacings mdx_create derives (init.f90:48-55)
+    g.dy = ly / (double)ny;
+    g.dz = lz / (double)nz;
+    for (int q = 0; q < 3; ++q) g.per[q] = periodic[q] ? 1 : 0;
+    g.ihc = g.ihv = nullptr;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(div_max_kernel<false>, dim3(nb), dim3(BX, BY, 1), 0, st, g, un, vn, wn, 1.0 / g.dx, 1.0 / g.dy,
+                       1.0 / g.dz, part);
+    MDX_HIP(hipGetLastError());
+    // the fold writes the maximum's bits as one 64-bit word: the double itself
+    hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(1024), 0, st, part, nb,
+                       reinterpret_cast<unsigned long long *>(div_inf), nullptr, 0ull);
+    MDX_HIP(hipGetLastError());
+    return MDX_OK;
+}
