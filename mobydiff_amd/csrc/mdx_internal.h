@@ -197,6 +197,9 @@ struct mdx_handle {
     // in workgroups of 1024 threads, where the subdomain and its face ring fit; 1 x-pairs anywhere
     // in the tile; 2 one z-column per thread
     int rb_form = 0;
+    // half-column form: the interior columns relax the halo-free part of each colour while the
+    // exchange is in flight (MOBYDIFF_LDS_OVERLAP=0 keeps the plain colour-by-colour order)
+    bool rb_overlap = true;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
     // polls of a neighbour wait before it gives up (MOBYDIFF_LDS_SPIN_LIMIT; a test hook); a wait
     // that gives up sets the error word, the batch drains, and the solve restarts on the

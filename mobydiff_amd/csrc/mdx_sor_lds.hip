@@ -12,7 +12,10 @@
 // values, so the results are identical), then colour 1 on the subdomain, whose neighbours are then
 // all up to date.  One halo exchange per iteration therefore suffices instead of one per colour.
 // The ring cells read the second halo layer and the halo edge lines, so a workgroup exchanges with
-// up to 18 neighbours: 6 faces two layers deep and 12 edges one cell thick.
+// up to 18 neighbours: 6 faces two layers deep and 12 edges one cell thick.  Only the colour-1
+// cells of those regions travel: the ring's colour-0 cells are the only colour-0 halo cells any
+// relaxation reads, and this workgroup forms them itself (the recomputation above), so half of the
+// halo is exchanged.
 //
 // Exchange: every published word carries the number of the iteration it belongs to (a 32-bit
 // payload and a 32-bit tag in one 64-bit agent-scope atomic store), and a consumer polls the face
@@ -31,7 +34,7 @@
 namespace mdx {
 
 static constexpr int RB_THREADS = 512;
-static constexpr int RB_HE = 12;     // exchanged values per thread (a subdomain's halo <= RB_HE * RB_THREADS)
+static constexpr int RB_HE = 6;      // exchanged values per thread (a subdomain's colour-1 halo <= RB_HE * RB_THREADS)
 static constexpr int RB_MAXR = 3;    // colour-0 ring cells per thread
 static constexpr int RB_G = 3;       // cells relaxed per group (loads issued together)
 static constexpr int RB_H = 2;       // halo depth
@@ -90,6 +93,8 @@ struct RbArgs {
     // not a number): a batch queued ahead of the host's read of that check then costs one launch
     int skip;
     double skip_tol;
+    // half-column form: overlap the interior columns' halo-free cells with the exchange
+    int ovl;
 };
 
 // a value as tagged 64-bit words: the tag in the high half, 32 payload bits in the low half
@@ -138,12 +143,14 @@ __host__ __device__ inline void rb_extent(int q, int bx, int by, int bz, int &ex
     ez = dz ? h : bz;
 }
 
+// exchanged values per workgroup: the colour-1 half of the 18 regions (every region holds an even
+// number of cells, as every subdomain extent is even)
 __host__ __device__ inline int rb_xchg_total(int bx, int by, int bz) {
     int t = 0;
     for (int q = 0; q < 18; ++q) {
         int ex, ey, ez;
         rb_extent(q, bx, by, bz, ex, ey, ez);
-        t += ex * ey * ez;
+        t += ex * ey * ez / 2;
     }
     return t;
 }
@@ -365,33 +372,70 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
             }
         }
     }
-    // ---- column form: the column of this thread.  Threads 0 .. bx*by-1 own the subdomain's
-    // columns and march z from 0 to bz+1 in colour 0 (the two end cells are the z faces of the
-    // ring) and from 1 to bz in colour 1; the next 2*by threads own the x faces of the ring and the
-    // next 2*bx its y faces (colour 0 only, z from 1 to bz; none beyond a wall).  Cells past a
-    // column's end, and every cell of a thread without a column, go to the dummy cell.
-    int cbase = dummy, cstep0 = 0, cstep1 = 0;   // first colour-0 / colour-1 cell and steps (0: dummy)
-    int cfirst1 = dummy, n0 = 0, n1 = 0;
+    // ---- column form: the column of this thread.  The subdomain's columns march z from 0 to bz+1
+    // in colour 0 (the two end cells are the z faces of the ring) and from 1 to bz in colour 1;
+    // after them come the columns of the ring's x faces (2*by) and y faces (2*bx), colour 0 only, z
+    // from 1 to bz, none beyond a wall.  FORM 2 orders the subdomain's columns interior first:
+    // those at least 3 cells from every x and y face (i in 3..bx-2, j in 3..by-2), padded to whole
+    // wavefronts where the threads allow, then the frame of the two outer rings.
+    // The cells of one colour in a thread's range are its end cell, next to the z face the range
+    // ends at (the lowest; the highest in FORM 2's upper halves), and the rest, marched upward.  In
+    // a wavefront of interior columns only (csplit), the rest of each colour reads no halo value
+    // and sends none (colour 0 is at least 2 cells from every face, colour 1 at least 3), so it
+    // relaxes while the halo exchange is in flight: colour 1's after the publish, the next
+    // iteration's colour 0 after that.  Cells past a range's end, and every cell of a thread
+    // without a column, go to the dummy cell.
+    int cb0 = dummy, cr0 = dummy, nr0 = 0;   // colour 0: end cell, first of the rest, their count
+    int cb1 = dummy, cr1 = dummy, nr1 = 0;   // colour 1
+    bool csplit = false;
     bool cwn = false, cws = false;
     T cmui = 1, cmuip = 1, cmvj = 1, cmvjp = 1, cmw = 1, cq = qconst;
-    T r0[COLS ? ZC : 1], r1[COLS ? ZC : 1];
+    T rb0 = T(0), rb1 = T(0);
+    T r0[COLS ? ZC - 1 : 1], r1[COLS ? ZC - 1 : 1];
+#pragma unroll
+    for (int n = 0; n < (COLS ? ZC - 1 : 1); ++n) r0[n] = r1[n] = T(0);
     if (COLS) {
         // FORM 2: threads tc and tc + RB_THREADS share column tc, the lower and the upper half of its
         // z range
         const int tc = t % RB_THREADS, half = t / RB_THREADS;
         const int nin = bx * by, nxf = 2 * by, nyf = 2 * bx;
+        int nI = 0, nIp = 0;   // interior columns, and the threads they take with their padding
+        if (FORM == 2 && bx > 4 && by > 4) {
+            nI = (bx - 4) * (by - 4);
+            nIp = (nI + 63) / 64 * 64;
+            if (nIp + (nin - nI) + nxf + nyf > RB_THREADS) nIp = nI;
+        }
+        const int nsub = nIp + nin - nI;   // threads of the subdomain's columns, padding included
         int cli = 0, clj = 0, kz0 = 0, kz1 = -1, kc0 = 1, kc1 = bz;   // colour-0 and colour-1 z ranges
         bool inner = false;
-        if (tc < nin) {
-            cli = tc % bx + 1; clj = tc / bx + 1; kz0 = 0; kz1 = bz + 1; inner = true;
-        } else if (tc < nin + nxf) {
-            const int e = tc - nin;
+        if (tc < nsub) {
+            if (tc < nI) {
+                cli = 3 + tc % (bx - 4); clj = 3 + tc / (bx - 4); inner = true;
+            } else if (tc >= nIp) {
+                const int f = tc - nIp;
+                if (nI == 0) {   // natural order
+                    cli = f % bx + 1; clj = f / bx + 1;
+                } else if (f < 4 * bx) {   // rows 1, 2, by-1, by
+                    const int r = f / bx;
+                    cli = f % bx + 1; clj = r < 2 ? r + 1 : by - 3 + r;
+                } else {   // columns 1, 2, bx-1, bx of rows 3 .. by-2
+                    const int g2 = f - 4 * bx, c = g2 % 4;
+                    clj = 3 + g2 / 4; cli = c < 2 ? c + 1 : bx - 3 + c;
+                }
+                inner = true;
+            }
+            if (inner) { kz0 = 0; kz1 = bz + 1; }
+        } else if (tc < nsub + nxf) {
+            const int e = tc - nsub;
             cli = e < by ? 0 : bx + 1; clj = e % by + 1; kz0 = 1; kz1 = bz;
-        } else if (tc < nin + nxf + nyf) {
-            const int e = tc - nin - nxf;
+        } else if (tc < nsub + nxf + nyf) {
+            const int e = tc - nsub - nxf;
             clj = e < bx ? 0 : by + 1; cli = e % bx + 1;
             if (neighbour(0, clj == 0 ? -1 : 1, 0) >= 0) { kz0 = 1; kz1 = bz; }
         }
+        // the wavefront's threads tc are 64 consecutive ones: all interior (or padding)
+        csplit = FORM == 2 && a.ovl && ((tc >> 6) + 1) * 64 <= nIp;
+        const bool top = FORM == 2 && half == 1;   // the end cell is the highest of the range
         if (FORM == 2) {
             const int h0 = (kz1 - kz0 + 2) / 2, h1 = bz / 2;   // positions in the lower halves
             if (half == 0) { kz1 = kz0 + h0 - 1; kc1 = h1; }
@@ -402,20 +446,31 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
             int gj = oy + clj;
             if (per_y) gj = wrap(gj, d.ny);
             const int par = (ox + cli + oy + clj + oz) & 1;   // colour c relaxes (par + lk + c) odd
+            // colour 0: cells k0, k0 + 2, ..., n0 of them
             const int k0 = kz0 + (((par + kz0) & 1) ? 0 : 1);
-            n0 = k0 <= kz1 ? (kz1 - k0) / 2 + 1 : 0;
-            cbase = L(cli, clj, k0);
-            cstep0 = 2 * RB_PLANE;
+            const int n0 = k0 <= kz1 ? (kz1 - k0) / 2 + 1 : 0;
+            if (n0 > 0) {
+                const int kb = top ? k0 + 2 * (n0 - 1) : k0, kr = top ? k0 : k0 + 2;
+                cb0 = L(cli, clj, kb);
+                rb0 = a.rhs[d.in(gi, gj, wrap(oz + kb, d.nz))];
+                nr0 = n0 - 1;
+                if (nr0 > 0) cr0 = L(cli, clj, kr);
 #pragma unroll
-            for (int n = 0; n < ZC; ++n)
-                r0[n] = n < n0 ? a.rhs[d.in(gi, gj, wrap(oz + k0 + 2 * n, d.nz))] : T(0);
+                for (int n = 0; n < ZC - 1; ++n)
+                    r0[n] = n < nr0 ? a.rhs[d.in(gi, gj, wrap(oz + kr + 2 * n, d.nz))] : T(0);
+            }
             if (inner) {
                 const int k1 = kc0 + (((par + kc0) & 1) ? 1 : 0);   // colour 1: (par + lk) even
-                n1 = k1 <= kc1 ? (kc1 - k1) / 2 + 1 : 0;
-                cfirst1 = L(cli, clj, k1);
-                cstep1 = 2 * RB_PLANE;
+                const int n1 = k1 <= kc1 ? (kc1 - k1) / 2 + 1 : 0;
+                if (n1 > 0) {
+                    const int kb = top ? k1 + 2 * (n1 - 1) : k1, kr = top ? k1 : k1 + 2;
+                    cb1 = L(cli, clj, kb);
+                    rb1 = a.rhs[d.in(gi, gj, oz + kb)];
+                    nr1 = n1 - 1;
+                    if (nr1 > 0) cr1 = L(cli, clj, kr);
 #pragma unroll
-                for (int n = 0; n < ZC; ++n) r1[n] = n < n1 ? a.rhs[d.in(gi, gj, oz + k1 + 2 * n)] : T(0);
+                    for (int n = 0; n < ZC - 1; ++n) r1[n] = n < nr1 ? a.rhs[d.in(gi, gj, oz + kr + 2 * n)] : T(0);
+                }
             }
             const int wb = wall_bits(gj);
             cwn = (wb & 1) != 0;
@@ -475,29 +530,34 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
                                                 cmuip, cmvj, cmvjp, cmw, cmw, cwn, cws, rhs, a.s, T(0), T(0), denom);
         return p0 + r * cq;
     };
-    // one colour along this thread's column: n cells from LDS index c0 in steps of two planes; the
-    // upper z neighbour of a cell is the lower one of the next
-    auto march = [&](int c0, int step, int n, const T *rhs) {
+    // the rest of one colour along this thread's column: n cells upward from LDS index c0, two
+    // planes apart; the upper z neighbour of a cell is the lower one of the next
+    auto march = [&](int c0, int n, const T *rhs) {
         int c = c0;
         T zlo = X[c - RB_PLANE];
 #pragma unroll
-        for (int q = 0; q < ZC; ++q) {
+        for (int q = 0; q < ZC - 1; ++q) {
             const int cc = q < n ? c : dummy;
             const T zhi = X[cc + RB_PLANE];
             const T v = relax_col(cc, zlo, zhi, rhs[q]);
             X[cc] = v;
             zlo = zhi;
-            c += step;
+            c += 2 * RB_PLANE;
             if ((q % RB_G) == RB_G - 1) __builtin_amdgcn_sched_barrier(0);
         }
     };
+    // the end cell of one colour
+    auto relax_end = [&](int c, T rhs) {
+        const T v = relax_col(c, X[c - RB_PLANE], X[c + RB_PLANE], rhs);
+        X[c] = v;
+    };
 
     unsigned long long *F = a.faces;
-    // publish the layer cells of colour `col`, tagged with iteration s, into buffer s & 1
-    auto publish = [&](int s, int col) {
+    // publish the colour-1 layer cells, tagged with iteration s, into buffer s & 1
+    auto publish = [&](int s) {
         const unsigned tag = a.tag0 + (unsigned)s + 1u;
         const size_t mybase = ((size_t)(w * 2 + (s & 1)) * RB_XSTRIDE) * NW;
-        // every layer value first (one LDS wait), then the stores of this colour's entries
+        // every layer value first (one LDS wait), then the stores
         T val[HE];
 #pragma unroll
         for (int jj = 0; jj < HE; ++jj) val[jj] = X[xpub[jj] & 0xfffff];
@@ -505,7 +565,6 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
         for (int jj = 0; jj < HE; ++jj) asm volatile("" : "+v"(val[jj]));
 #pragma unroll
         for (int jj = 0; jj < HE; ++jj) {
-            if (((xpub[jj] >> 20) & 1) != col) continue;
             unsigned long long wv[NW];
             TW::pack(val[jj], tag, wv);
             unsigned long long *dst = F + mybase + (size_t)(t + jj * NT) * NW;
@@ -530,11 +589,14 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
         // iteration the deep groups (g < a.md) were relaxed while the halo was in flight.
         if (COLS) {
             // column data: loop-invariant, kept from being hoisted as per-cell addresses (as above)
-            asm volatile("" : "+v"(cbase), "+v"(cfirst1));
-            if (n0 > 0) march(cbase, cstep0, n0, r0);
+            asm volatile("" : "+v"(cb0), "+v"(cr0), "+v"(cb1), "+v"(cr1));
+            // colour 0: the end cells, and the rest where the last exchange's wait has not relaxed it
+            relax_end(cb0, rb0);
+            if (!csplit || s == 0) march(cr0, nr0, r0);
             __syncthreads();
-            if (s < a.iters - 1 || a.defect) publish(s, 0);
-            if (n1 > 0) march(cfirst1, cstep1, n1, r1);
+            // colour 1: the end cells, and the rest where it is not left for the exchange's flight
+            relax_end(cb1, rb1);
+            if (!csplit) march(cr1, nr1, r1);
             __syncthreads();
         }
 #pragma unroll
@@ -555,7 +617,6 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
 #pragma unroll
             for (int m = 0; m < NR; ++m) X[rw[m] & 0xffff] = nv[m];
             __syncthreads();
-            if (s < a.iters - 1 || a.defect) publish(s, 0);
         }
         // ---- colour 1 on the subdomain: its neighbours are subdomain or ring cells, all updated
 #pragma unroll
@@ -570,12 +631,29 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
         }
         if (!COLS) __syncthreads();
         // the last iteration needs no exchange, unless the defect epilogue reads the final halo
-        if (s == a.iters - 1 && !a.defect) break;
+        if (s == a.iters - 1 && !a.defect) {
+            if (COLS && a.ovl) {   // the rest of colour 1 in the split wavefronts, before the write-back
+                if (csplit) march(cr1, nr1, r1);
+                __syncthreads();
+            }
+            break;
+        }
 
-        // ---- publish this subdomain's colour-1 layer cells (colour 0 went out after its pass)
+        // ---- publish this subdomain's colour-1 layer cells
         const int buf = s & 1;
         const unsigned tag = a.tag0 + (unsigned)s + 1u;
-        publish(s, 1);
+        publish(s);
+        // ---- column form, split wavefronts: the rest of colour 1, then (after every colour-1 cell
+        // it reads is final) the next iteration's rest of colour 0, while the layers travel.  The
+        // publish reads only colour-1 layer cells, which neither writes, and the halo the pull
+        // writes is read by neither.
+        if (COLS && a.ovl) {
+            if (csplit) march(cr1, nr1, r1);
+            if (s < a.iters - 1) {
+                __syncthreads();
+                if (csplit) march(cr0, nr0, r0);
+            }
+        }
         // ---- the next iteration's colour 0 on the deep groups, while the neighbours' layers travel:
         // those cells read only subdomain cells of colour 1, final for this iteration; the barrier
         // lets the publish read its layers first
@@ -741,15 +819,14 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
     }
 }
 
-// Exchange lists of workgroup w (subdomain id), entry e of the 18 regions (faces two layers deep,
-// then edges one cell thick; entry e = t + jj * threads of thread t).  Publish word: the LDS cell
-// this workgroup sends for its neighbour in direction D (bits 0-19) and its colour (bit 20).  Pull
-// word: the halo cell where the region that the neighbour in direction -D sent lands (bits 0-19),
-// D (bits 20-24) and whether to wait for it (bit 25).  A region lists its colour-0 cells first,
-// then its colour-1 cells (colour 0: odd global coordinate sum), so each colour can be published
-// as soon as it is relaxed, with whole lines of consecutive entries: cells 2k and 2k+1 of the
-// region's natural order are neighbours along its first extent above one, of opposite colours,
-// and entry k of a colour half is the one of the two with that colour, on both sides alike.
+// Exchange lists of workgroup w (subdomain id), entry e of the colour-1 cells of the 18 regions
+// (faces two layers deep, then edges one cell thick; entry e = t + jj * threads of thread t).
+// Publish word: the LDS cell this workgroup sends for its neighbour in direction D (bits 0-19).
+// Pull word: the halo cell where the region that the neighbour in direction -D sent lands (bits
+// 0-19), D (bits 20-24) and whether to wait for it (bit 25).  Colour 1 has an even global
+// coordinate sum; cells 2k and 2k+1 of a region's natural order are neighbours along its first
+// extent above one, of opposite colours, and entry k of the region is the one of the two of
+// colour 1, on both sides alike.
 // Unused entries publish the dummy cell; an entry with no sender (beyond a wall, or past the end
 // of the list) does not wait, reads this workgroup's own slot (direction 18) and lands in the
 // dummy cell.  Built once per plan; the batch kernel only loads it.
@@ -771,19 +848,18 @@ __global__ __launch_bounds__(256) void rb_xlist_kernel(RbGeom g, int *xl) {
         const int ny_ = wy - dy;
         return g.per_y || (ny_ >= 0 && ny_ < g.py);
     };
-    int xpub = g.dummy | (1 << 20), xdst = g.dummy | (18 << 20);
+    int xpub = g.dummy, xdst = g.dummy | (18 << 20);
     if (e < g.total) {
         int q = 0, off = 0, ex = 0, ey = 0, ez = 0;
         for (; q < 18; ++q) {
             rb_extent(q, bx, by, bz, ex, ey, ez);
-            if (e < off + ex * ey * ez) break;
-            off += ex * ey * ez;
+            if (e < off + ex * ey * ez / 2) break;
+            off += ex * ey * ez / 2;
         }
         int dx, dy, dz;
         rb_dir(q, dx, dy, dz);
         const int h = q < 6 ? RB_H : 1;
-        const int half = ex * ey * ez / 2, rp = e - off, col = rp >= half ? 1 : 0;
-        const int k2 = 2 * (rp - col * half);
+        const int col = 1, k2 = 2 * (e - off);
         auto cell = [&](int r, auto side, int &cx, int &cy, int &cz) {
             cx = side(dx, r % ex, bx, h);
             cy = side(dy, (r / ex) % ey, by, h);
@@ -798,7 +874,7 @@ __global__ __launch_bounds__(256) void rb_xlist_kernel(RbGeom g, int *xl) {
         int cx, cy, cz;
         cell(k2, snd, cx, cy, cz);
         if (colour(cx, cy, cz) != col) cell(k2 + 1, snd, cx, cy, cz);
-        xpub = L(cx, cy, cz) | (col << 20);
+        xpub = L(cx, cy, cz);
         if (has_sender(dx, dy, dz)) {
             cell(k2, rcv, cx, cy, cz);
             if (colour(cx, cy, cz) != col) cell(k2 + 1, rcv, cx, cy, cz);
@@ -955,6 +1031,7 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
     a.seq = df ? df->seq : 0;
     a.skip = df ? df->skip : 0;
     a.skip_tol = df ? df->skip_tol : 0.0;
+    a.ovl = h->rb_overlap ? 1 : 0;
     a.d = h->d;
     a.px = pl.px; a.py = pl.py; a.pz = pl.pz;
     a.bx = pl.bx; a.by = pl.by; a.bz = pl.bz;

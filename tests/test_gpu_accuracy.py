@@ -6,8 +6,8 @@ state (four steps after the seeded start) each path runs three RK3 steps and is 
 field with the oracle's three steps from the same state:
 
   * plain FP64 (streaming sweeps, and the LDS-resident FP64 batches): round-off only, rel-L2 <= 1e-10;
-  * FP32 sweeps in FP64 defect correction, without and with the learned initial guess:
-    u, v, w within rel-L2 1e-6;
+  * FP32 sweeps in FP64 defect correction, without and with the learned initial guess (a network
+    trained on this run's warm-up pairs, and the shipped stride-4 network): u, v, w within rel-L2 1e-6;
   * p is compared with its mean removed (a Neumann solve fixes p up to a constant).  Two solves that
     each stop at ||div||_inf < 1e-8 differ in p by about the distance of either from the converged
     solution, so the bar for p is set by the oracle's own distance from a solve converged to 1e-12:
@@ -27,6 +27,8 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 NX, NY, NZ = 192, 128, 96
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mobydiff_amd", "weights",
+                       "unet_guess_s4_wavy_192x128x96.npz")
 DT, TOL, STEPS, WARM = 5e-4, 1e-8, 3, 4
 SL = (slice(1, -1),) * 3
 
@@ -133,12 +135,25 @@ def test_mixed_matches_oracle(case):
     assert div < TOL
 
 
-def test_learned_matches_oracle(case):
-    # the network's start is taken whenever it is evaluated (margin test off)
-    out, stats, div, _ = _run(case["state"], case["coef"], M.PREC_MIXED, guess=M.GUESS_LEARNED, net=case["net"],
-                              env={"MOBYDIFF_NET_MARGIN": "1e30"})
+@pytest.mark.parametrize("which", ["warm-up pairs", "shipped"])
+def test_learned_matches_oracle(case, which):
+    # the network's start is taken whenever it is evaluated (margin test off): the network trained
+    # on this run's warm-up pairs, and the stride-4 network trained on 50 other wavy channels
+    net = case["net"]
+    if which == "shipped":
+        from mobydiff_amd import train_guess
+        if not os.path.exists(WEIGHTS):
+            pytest.skip("trained U-Net parameters not present")
+        net = M.Net(NX, NY, NZ, seed=0, stride=train_guess.stride_of(WEIGHTS))
+        train_guess.load(WEIGHTS, net)
+    try:
+        out, stats, div, _ = _run(case["state"], case["coef"], M.PREC_MIXED, guess=M.GUESS_LEARNED, net=net,
+                                  env={"MOBYDIFF_NET_MARGIN": "1e30", "MOBYDIFF_NET_TIME_GATE": "0"})
+    finally:
+        if net is not case["net"]:
+            net.close()
     e = _errors(out, case["ref"])
-    _report("learned guess + mixed", e, div, stats)
+    _report(f"learned guess ({which}) + mixed", e, div, stats)
     started = sum(1 for s in stats for q in range(3) if not s.stage[q].fallback)
     print(f"  network-started solves {started} of {3 * STEPS}", flush=True)
     assert started >= 1
