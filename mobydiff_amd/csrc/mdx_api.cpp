@@ -1031,8 +1031,11 @@ static int sor_solve64(mdx_handle *h, mdx_solve_stats *st, double r_first) {
 
 // pinned words of the mixed solve's two check slots: [CHECK_PIN + 4 * slot] = max, error word, seq
 static constexpr int CHECK_PIN = 16;
-// pinned words of the learned start's comparison: max |rhs| (the residual of Phi = 0)
+// pinned words of the learned start's comparison, each [max, error word, seq]: the residual of
+// Phi_0 (RP), max |rhs| (RZ: the residual of Phi = 0) and the learned start's first check (SEL)
+static constexpr int RP_PIN = 32;
 static constexpr int RZ_PIN = 40;
+static constexpr int SEL_PIN = 48;
 
 // Wait for the residual check that posts sequence number `seq` to the pinned words [w, w + 3):
 // [max, error word, seq].  The host polls the sequence word (no event between the kernels); now
@@ -1276,7 +1279,6 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
     if (have_prev) MDX_HIP(launch_scale_copy64(h, h->phi2, prev, fac, h->ni));
     // the residual of Phi_0 (the network's input field and its max-norm rp); rp goes to pinned
     // host memory behind the network's kernels and is read after the stream's next synchronisation
-    constexpr int RP_PIN = 32;
     double rp = -1.0;
     MDX_HIP(launch_reset_max(h));
     MDX_HIP(launch_residual_field64(h, have_prev ? h->phi2 : h->phi, h->res64));
@@ -1344,6 +1346,59 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
     st->fallback = 1;
     if (prev_start) MDX_HIP(launch_scale_copy64(h, h->phi, prev, fac, h->ni));   // phi2 holds the network's
     *r_first = other;
+    return MDX_OK;
+}
+
+// The learned start of a mixed-precision solve: formed in phi, its first check and the choice
+// against the fallback start (Phi = 0, or Phi_0 with net_fallback_prev) made on the device by
+// select_start_kernel, so the only host round trip is the wait for that first check, which the
+// solve needs anyway (no residual pass of its own, no copy of the start).  *first_seq: the
+// sequence number of the first check (slot 0); resolve_learned_start then reads which start won.
+static int learned_start_mixed(mdx_handle *h, int s, const GuessPlan &gp, unsigned long long *first_seq) {
+    mdx_net *n = h->net;
+    const double *prev = h->phi_hist + (size_t)s * h->ni;   // a learned start has a history
+    ev_record(h, 8);
+    // Phi_0 in phi2, its residual (the network's input field) in res64, max-norm to the RP slot
+    MDX_HIP(launch_scale_copy64(h, h->phi2, prev, gp.fac, h->ni));
+    MDX_HIP(launch_residual_field64(h, h->phi2, h->res64, h->d_pinned + RP_PIN));
+    {
+        hipStream_t saved = n->stream;
+        n->stream = h->stream;
+        hipError_t e = unet::net_input_from_solver(n, h, h->res64, h->phi2);
+        if (e == hipSuccess) e = unet::net_guess_to_phi(n, h->phi2, h->phi);
+        n->stream = saved;
+        MDX_HIP(e);
+    }
+    ev_record(h, 9);
+    MDX_HIP(launch_defect64(h, h->phi, nullptr, h->phi, h->r32, h->d_pinned + SEL_PIN, 0));
+    const unsigned long long seq = ++h->chk_seq;
+    const bool prev_fb = gp.use_prev;
+    MDX_HIP(launch_select_start(h, h->net_margin, h->d_pinned + SEL_PIN, h->d_pinned + (prev_fb ? RP_PIN : RZ_PIN),
+                                prev_fb ? h->phi2 : nullptr, h->res64, h->d_pinned + CHECK_PIN, seq));
+    *first_seq = seq;
+    return MDX_OK;
+}
+
+// Which start the device chose in learned_start_mixed: the same comparison on the same pinned
+// values, read once its first check has posted.  The choice sets the kind of start, and with it
+// the solve's batch schedule, before the first batch is queued: a fallback solve is the other
+// start's solve exactly.
+static int resolve_learned_start(mdx_handle *h, int s, const GuessPlan &gp, mdx_solve_stats *st, double *r_first) {
+    const double rg = h->h_pinned[SEL_PIN], other = h->h_pinned[gp.use_prev ? RP_PIN : RZ_PIN];
+    st->guess_div_inf = rg;
+    if (rg < h->net_margin * other) {
+        h->net_started[s] = true;
+        *r_first = rg;
+    } else {
+        net_loss(h, s);
+        st->fallback = 1;
+        *r_first = other;
+    }
+    if (h->timing) {
+        float ms = 0.f;
+        MDX_HIP(hipEventSynchronize(h->ev[9]));
+        if (hipEventElapsedTime(&ms, h->ev[8], h->ev[9]) == hipSuccess) h->times.guess_ms += ms;
+    }
     return MDX_OK;
 }
 
@@ -1431,6 +1486,8 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         const int stage = (int)(h->proj_count % 3);
         const auto t_plan = std::chrono::steady_clock::now();
         const GuessPlan gp = guess_plan(h, stage, dtg, &st);
+        // a mixed-precision learned start is chosen on the device (no host round trip)
+        const bool dev_select = gp.learned && c.precision == MDX_PREC_MIXED;
         unsigned long long first_seq = 0;
         if (gp.simple && c.precision == MDX_PREC_MIXED) {
             // rhs, the start and the solve's first check in one kernel
@@ -1441,7 +1498,14 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
             // max |rhs|, formed here on the way
             const bool want_rz = gp.learned && gp.have_prev && !gp.use_prev;
             MDX_HIP(launch_sor_prepare(h, want_rz ? h->d_pinned + RZ_PIN : nullptr, 0));
-            rc = initial_guess(h, stage, dtg, gp, &st, &r_first);
+            if (dev_select) {
+                rc = learned_start_mixed(h, stage, gp, &first_seq);
+                double r0 = 0.0;
+                if (rc == MDX_OK) rc = wait_check(h, CHECK_PIN, first_seq, &r0);
+                if (rc == MDX_OK) rc = resolve_learned_start(h, stage, gp, &st, &r_first);
+            } else {
+                rc = initial_guess(h, stage, dtg, gp, &st, &r_first);
+            }
             if (rc && rc != MDX_LDS_TIMEOUT) return rc;
         }
         if (rc == MDX_OK) {

@@ -801,8 +801,9 @@ __global__ __launch_bounds__(256) void sor_residual64_kernel(GridK g, MuK<double
     block_max_part(m, out);
 }
 
-// max |rhs - L_mu(phi)| into red_max; with field != nullptr the residual itself is stored too
-hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *field) {
+// max |rhs - L_mu(phi)| into red_max (and host_slot); with field != nullptr the residual itself is
+// stored too
+hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *field, double *host_slot) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
@@ -813,10 +814,57 @@ hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *fie
     else { if (h->ibm_on) MDX_RS(false, true); else MDX_RS(false, false); }
 #undef MDX_RS
     hipError_t e = hipGetLastError();
-    return e != hipSuccess ? e : launch_max_final(h, grd.x);
+    return e != hipSuccess ? e : launch_max_final(h, grd.x, host_slot, 0);
 }
 
-hipError_t launch_sor_residual64(mdx_handle *h, const double *phi) { return launch_residual_field64(h, phi, nullptr); }
+hipError_t launch_sor_residual64(mdx_handle *h, const double *phi) {
+    return launch_residual_field64(h, phi, nullptr, nullptr);
+}
+
+// The start of a mixed solve whose first check ran on the learned start (phi, its defect r32 and
+// maximum rg = rg_slot[0]): kept when rg < margin * other (other_slot[0], the fallback start's
+// residual maximum; a NaN keeps nothing), else replaced by the fallback: Phi = 0 with the defect
+// rhs, or (PREV) Phi_0 = phi0 with the defect res0, each the value the first check forms for that
+// start (r32 = (float) of the FP64 defect).  Workgroup 0 posts the chosen maximum as the check: to
+// red_max[0], which a batch queued ahead reads, and with the error word and seq to host_slot.
+// The host repeats the comparison on the same pinned values to learn which start ran.
+template <bool PREV>
+__global__ __launch_bounds__(256) void select_start_kernel(size_t n, double margin, const double *rg_slot,
+                                                           const double *other_slot, double *phi, float *r32,
+                                                           const double *__restrict__ fb_defect,
+                                                           const double *__restrict__ phi0,
+                                                           unsigned long long *red_max, double *host_slot,
+                                                           unsigned long long seq) {
+    const double rg = rg_slot[0], other = other_slot[0];
+    const bool take = rg < margin * other;
+    if (!take) {
+        for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+            phi[q] = PREV ? phi0[q] : 0.0;
+            r32[q] = (float)fb_defect[q];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double v = take ? rg : other;
+        red_max[0] = (unsigned long long)__double_as_longlong(v);
+        host_slot[0] = v;
+        host_slot[1] = __longlong_as_double((long long)red_max[1]);
+        __threadfence_system();
+        reinterpret_cast<volatile unsigned long long *>(host_slot)[2] = seq;
+    }
+}
+
+hipError_t launch_select_start(mdx_handle *h, double margin, const double *rg_slot, const double *other_slot,
+                               const double *phi0, const double *res0, double *host_slot, unsigned long long seq) {
+    const size_t n = h->ni;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 1023) / 1024, 4096);
+    if (phi0)
+        hipLaunchKernelGGL(select_start_kernel<true>, dim3(blocks), dim3(256), 0, h->stream, n, margin, rg_slot,
+                           other_slot, h->phi, h->r32, res0, phi0, h->red_max, host_slot, seq);
+    else
+        hipLaunchKernelGGL(select_start_kernel<false>, dim3(blocks), dim3(256), 0, h->stream, n, margin, rg_slot,
+                           other_slot, h->phi, h->r32, h->rhs, nullptr, h->red_max, host_slot, seq);
+    return hipGetLastError();
+}
 
 // ------------------------------------------------------------------------------- mixed precision
 // One FP64 defect-correction step: Phi_out = Phi_in + e; r = rhs - L_mu(Phi_out) in FP64;

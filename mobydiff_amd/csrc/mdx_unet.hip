@@ -520,6 +520,45 @@ __global__ __launch_bounds__(64) void input_stats_final(const double *part, int 
     }
 }
 
+// both input statistics in one pass: rms of a into stat[0], of b into stat[1], each summed in the
+// order input_stats_partial / input_stats_final use for one array (the same values)
+__global__ __launch_bounds__(256) void input_stats2_partial(long V, const double *__restrict__ a,
+                                                            const double *__restrict__ b, double *part) {
+    __shared__ double sa[256], sb[256];
+    double acc_a = 0, acc_b = 0;
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < V; v += (long)gridDim.x * 256) {
+        const double ta = a[v], tb = b[v];
+        acc_a += ta * ta;
+        acc_b += tb * tb;
+    }
+    sa[threadIdx.x] = acc_a;
+    sb[threadIdx.x] = acc_b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sa[0];
+        part[gridDim.x + blockIdx.x] = sb[0];
+    }
+}
+
+// wave w folds the partials of array w as input_stats_final does
+__global__ __launch_bounds__(128) void input_stats2_final(const double *part, int nb, double V, double *stat) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    double a = 0;
+    for (int q = l; q < nb; q += 64) a += part[w * nb + q];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (l == 0) {
+        const double r = sqrt(a / V);
+        stat[w] = r > 1e-300 ? r : 1.0;
+    }
+}
+
 // Phi guess (FP64, interior layout) = Phi_0 + out * s_r * c0 (in place when phi == phi0)
 __global__ void guess_out_kernel(const float *__restrict__ out, const double *__restrict__ stats, double c0,
                                  const double *phi0, double *phi, long V) {
@@ -855,8 +894,13 @@ hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *
     (void)h;
     hipError_t e;
     const long V = n->V[0];
-    TRY(rms_into(n, r0, nullptr, 0.0, n->stats));
-    if (phi0) TRY(rms_into(n, phi0, nullptr, 0.0, n->stats + 1));
+    if (phi0) {   // rms of r0 and of Phi_0 into stats[0..1]
+        hipLaunchKernelGGL(input_stats2_partial, dim3(512), dim3(256), 0, n->stream, n->FV, r0, phi0, n->part);
+        hipLaunchKernelGGL(input_stats2_final, dim3(1), dim3(128), 0, n->stream, n->part, 512, (double)n->FV,
+                           n->stats);
+    } else {
+        TRY(rms_into(n, r0, nullptr, 0.0, n->stats));
+    }
     if (n->stride == 1)
         hipLaunchKernelGGL(make_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, V, r0, phi0, n->stats, n->x_in);
     else
