@@ -37,6 +37,7 @@ struct mdx_net {
     // inference timing
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool tiled_conv = true;        // LDS-tiled convolutions where the grid allows (bit-identical)
+    bool split_conv = true;        // split-K convolutions on small grids the tiled kernel cannot take
     std::vector<struct mdx_handle *> users;   // solver handles this network is attached to
 };
 

@@ -101,6 +101,92 @@ __global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__
     }
 }
 
+// ------------------------------------------------------------------------------- conv forward, split K
+// The same GEMM for grids too small to fill the chip (the coarse levels of the strided network:
+// 12 x 8 x 6 voxels are 36 M-tiles): S waves share one 16-voxel M-tile, wave w taking the K chunks
+// w, w + S, w + 2S, ... (the direct kernel's chunks and weight fragments), and their partial
+// accumulators are summed in LDS in wave order, then bias and ReLU as above.  A wave then waits
+// for KC / S global fetches instead of KC.  Needs V % 16 == 0.
+template <int COUT> struct SplitK {
+    static constexpr int NT = COUT / 16, S = NT >= 3 ? 8 : 16;
+};
+template <int CIN, int COUT, bool RELU>
+__global__ __launch_bounds__(64 * SplitK<COUT>::S) void conv3_splitk_kernel(const bf16 *__restrict__ in,
+                                                                            const bf16x8 *__restrict__ wp,
+                                                                            const float *__restrict__ bias,
+                                                                            bf16 *__restrict__ out, int W, int H,
+                                                                            int D) {
+    constexpr int K = 27 * CIN, KC = (K + 31) / 32, NT = SplitK<COUT>::NT, S = SplitK<COUT>::S;
+    constexpr int NJ = (KC + S - 1) / S, PD = 2;
+    __shared__ float red[S][NT * 4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long v0 = (long)blockIdx.x * 16;
+    const int r = lane & 15, g = lane >> 4;
+    const long v = v0 + r;
+    const int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long)W * H));
+    f32x4 acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero8 = {};
+    auto load_a = [&](int kc) {
+        const int k0 = kc * 32 + g * 8;
+        bf16x8 a = zero8;
+        if (k0 < K) {
+            const int tap = k0 / CIN, c0 = k0 - tap * CIN;
+            const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
+            if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
+                a = *reinterpret_cast<const bf16x8 *>(in + (((long)zz * H + yy) * W + xx) * CIN + c0);
+        }
+        return a;
+    };
+    bf16x8 aq[PD], bq[PD][NT];
+#pragma unroll
+    for (int p = 0; p < PD; ++p) {
+        const int kc = wave + p * S;
+        aq[p] = kc < KC ? load_a(kc) : zero8;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bq[p][nt] = kc < KC ? wp[((long)kc * NT + nt) * 64 + lane] : zero8;
+    }
+    for (int j0 = 0; j0 < NJ; j0 += PD) {
+#pragma unroll
+        for (int p = 0; p < PD; ++p) {
+            const int j = j0 + p, kc = wave + j * S;
+            if (j >= NJ) break;
+            const bf16x8 a = aq[p];
+            bf16x8 b[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) b[nt] = bq[p][nt];
+            // chunks past K are zero operands, not a branch around the MFMAs: kc depends on the
+            // wave, so such a branch would be a masked region, and MFMA does not honour the mask
+            const int kn = kc + PD * S;
+            if (j + PD < NJ) {
+                aq[p] = kn < KC ? load_a(kn) : zero8;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) bq[p][nt] = kn < KC ? wp[((long)kn * NT + nt) * 64 + lane] : zero8;
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[nt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) red[wave][nt * 4 + i][lane] = acc[nt][i];
+    __syncthreads();
+    // output element e = voxel vo of the tile, channel ch; MFMA lane (ch & 15) + 16 (vo / 4), row vo % 4
+    for (int e = threadIdx.x; e < 16 * COUT; e += 64 * S) {
+        const int vo = e / COUT, ch = e - vo * COUT;
+        const int q = (ch >> 4) * 4 + (vo & 3), ln = (ch & 15) + 16 * (vo >> 2);
+        float sum = red[0][q][ln];
+#pragma unroll
+        for (int w = 1; w < S; ++w) sum += red[w][q][ln];
+        float val = sum + (bias ? bias[ch] : 0.f);
+        if (RELU) val = val > 0.f ? val : 0.f;
+        out[(v0 + vo) * COUT + ch] = (bf16)val;
+    }
+}
+
 // ------------------------------------------------------------------------------- conv forward, LDS-tiled
 // Same GEMM, same weight fragments and the same K order and chunking as conv3_fwd_kernel (so the
 // outputs are bit-identical), but a workgroup computes a 16 x 4 x TZ output tile and first stages
@@ -725,6 +811,21 @@ static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in
     TILED_CASE(32, 32, false) TILED_CASE(32, 96, false) TILED_CASE(16, 48, false) TILED_CASE(64, 32, false)
     TILED_CASE(64, 64, false)
 #undef TILED_CASE
+    // grids the tiled kernel cannot take and too small to fill the chip: split K
+    const long mt = n->V[lvl] / 16;   // 16-voxel M-tiles
+#define SPLIT_CASE(CI, CO, R)                                                                                      \
+    if (n->split_conv && cin == CI && cout == CO && relu == R &&                                               \
+        !(W % CTX == 0 && H % CTY == 0 && D % ConvTile<CI>::TZ == 0) &&                                           \
+        n->V[lvl] % 16 == 0 && mt < 1024) {                                                                        \
+        hipLaunchKernelGGL((conv3_splitk_kernel<CI, CO, R>), dim3((unsigned)mt), dim3(64 * SplitK<CO>::S), 0,       \
+                           n->stream, in, reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);               \
+        return hipGetLastError();                                                                                  \
+    }
+    SPLIT_CASE(16, 16, true) SPLIT_CASE(16, 32, true) SPLIT_CASE(32, 32, true) SPLIT_CASE(32, 64, true)
+    SPLIT_CASE(64, 64, true) SPLIT_CASE(96, 32, true) SPLIT_CASE(48, 16, true)
+    SPLIT_CASE(16, 16, false) SPLIT_CASE(32, 16, false) SPLIT_CASE(32, 32, false) SPLIT_CASE(64, 32, false)
+    SPLIT_CASE(64, 64, false) SPLIT_CASE(32, 96, false) SPLIT_CASE(16, 48, false)
+#undef SPLIT_CASE
 #define CONV_CASE(CI, CO, R)                                                                                       \
     if (cin == CI && cout == CO && relu == R) {                                                                    \
         hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, R>), dim3(blocks), dim3(256), 0, n->stream, in,                \

@@ -59,6 +59,7 @@ extern "C" int mdx_net_create_strided(int32_t nx, int32_t ny, int32_t nz, int32_
               "count / 64) a multiple of 64");
     mdx_net *n = new mdx_net();
     if (const char *t = std::getenv("MOBYDIFF_NO_TILED_CONV")) n->tiled_conv = !(t[0] == '1');
+    if (const char *t = std::getenv("MOBYDIFF_NO_SPLIT_CONV")) n->split_conv = !(t[0] == '1');
     n->stride = stride;
     n->FW = nx; n->FH = ny; n->FD = nz;
     n->FV = (long)nx * ny * nz;

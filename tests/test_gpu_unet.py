@@ -80,6 +80,36 @@ def test_tiled_convolutions_match_direct_kernels(monkeypatch):
     assert _rel(grads[1], grads[0]) < 1e-5
 
 
+def test_split_k_convolutions_match_direct_kernels(monkeypatch):
+    """On the coarse levels of the stride-4 network (24 x 16 x 12 and 12 x 8 x 6 voxels, which the
+    tiled kernel cannot take) the split-K convolutions sum the direct kernels' K chunks in another
+    order: forward outputs and gradients agree to rounding."""
+    rng = np.random.default_rng(7)
+    n0 = (48, 32, 24)
+    x2 = rng.standard_normal(n0[::-1] + (2,)).astype(np.float32)
+    target = rng.standard_normal(n0[::-1]).astype(np.float32)
+    outs, grads = [], []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MOBYDIFF_NO_SPLIT_CONV", flag)
+        net = M.Net(192, 128, 96, seed=3, stride=4)
+        p, _ = net.get_params()
+        outs.append(net.forward(x2))
+        net.zero_grads()
+        net.accumulate_grad(x2, target)
+        grads.append(net.get_grads())
+        net.close()
+    t = torch.zeros(1, 16, *n0[::-1])
+    t[0, 0] = torch.as_tensor(x2[..., 0])
+    t[0, 1] = torch.as_tensor(x2[..., 1])
+    ref = R.forward_bf16(R.flat_to_params(p), t)[0, 0].numpy()
+    print(f"split-K against direct: forward rel {_rel(outs[1], outs[0]):.2e}, gradients rel "
+          f"{_rel(grads[1], grads[0]):.2e}; against the reference: direct {_rel(outs[0], ref):.2e}, "
+          f"split-K {_rel(outs[1], ref):.2e}", flush=True)
+    assert _rel(outs[0], ref) < 1e-2 and _rel(outs[1], ref) < 1e-2
+    assert _rel(outs[1], outs[0]) < 1e-2
+    assert _rel(grads[1], grads[0]) < 1e-2
+
+
 def test_gradients_match_autograd():
     net = M.Net(NX, NY, NZ, seed=2)
     p, _ = net.get_params()
