@@ -249,8 +249,10 @@ __global__ __launch_bounds__(64) void tridiag_kernel(Dims d, double2 *hat, const
 //     within 2x of it.  Every lane works through the whole solve (no serial chain across lanes:
 //     a line's serial recurrence in one wave would cost 64 lanes' worth of FP64 issue per step).
 //     The mean mode (kx = kz = 0) pins row 1 to zero, as the reference does (dp(1) = 0), by making
-//     it an identity row.  Lines go to the eight waves of a workgroup consecutively: for a fixed row
-//     the eight lines' values are 128 contiguous bytes of the spectrum.
+//     it an identity row.  A workgroup takes 16 consecutive lines and stages them through LDS: for
+//     a fixed row the 16 lines' values are contiguous in the spectrum (256 B in FP64), so the loads
+//     and stores move whole cache lines, and each wave then reads its line's rows from LDS
+//     (rows of a line contiguous, padded by one value so the transposed copy is conflict-free).
 template <typename T>
 __device__ inline T shfl_any(T v, int src) {
     return __shfl(v, src, 64);
@@ -264,7 +266,8 @@ __device__ inline float2 shfl_any<float2>(float2 v, int src) {
     return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
-static constexpr int TW_WAVES = 8, TW_STEPS = 6;
+// a workgroup of TW_WAVES waves solves TW_LINES consecutive lines, staged through LDS
+static constexpr int TW_WAVES = 4, TW_LINES = 16, TW_STEPS = 6;
 
 // the value of row i - delta (lo) or i + delta (hi) for row set q, from the shuffled copies of all
 // row sets (rows across the lane wrap belong to set q - 1 / q + 1; beyond the ends: `edge`)
@@ -287,137 +290,163 @@ __global__ __launch_bounds__(64 * TW_WAVES) void tridiag_wave_kernel(int nlines,
                                                                     const typename Cx<C>::R *__restrict__ bdrow,
                                                                     const typename Cx<C>::R *__restrict__ crow) {
     using R = typename Cx<C>::R;
-    const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * TW_WAVES + (threadIdx.x >> 6);   // the wave's line
-    if (t >= nlines) return;                                    // whole waves only
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *tile = reinterpret_cast<C *>(lds_raw);   // [TW_LINES][ny + 1]: line-major, rows contiguous
+    const int ldt = ny + 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t0 = blockIdx.x * TW_LINES;
     const C zero2 = Cx<C>::mk(R(0), R(0));
-    if (MEAN_ZERO && t == 0) {
-        for (int j = lane; j < ny; j += 64) hat[plane * j] = zero2;
-        return;
+    // ---- the TW_LINES lines of this workgroup into LDS: for each row, the lines' values are
+    // consecutive in the spectrum (TW_LINES * sizeof(C) contiguous bytes per row)
+    {
+        const int tt = threadIdx.x % TW_LINES, jj = threadIdx.x / TW_LINES;
+        const int t = t0 + tt;
+        for (int j = jj; j < ny; j += 64 * TW_WAVES / TW_LINES)
+            tile[tt * ldt + j] = t < nlines ? hat[(size_t)t + plane * j] : zero2;
     }
-    const int kx = t % nxh, kz = t / nxh;
-    const R k = lamx[kx] + lamz[kz];
-    // the original rows (kept for the residual) and the right-hand side
-    R a0[M], b0[M], c0[M];
-    C d0[M];
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-        const int j = lane + 64 * q;
-        if (j < ny && !(t == 0 && j == 0)) {
-            a0[q] = arow[j];
-            c0[q] = crow[j];
-            b0[q] = bdrow[j] - k;
-            d0[q] = hat[(size_t)t + plane * j];
-        } else {
-            a0[q] = R(0);
-            c0[q] = R(0);
-            b0[q] = R(1);
-            d0[q] = zero2;
+    __syncthreads();
+    // ---- every wave solves its lines from LDS (lane l: rows l, l + 64, ...)
+    for (int l = wave; l < TW_LINES; l += TW_WAVES) {
+        const int t = t0 + l;
+        if (t >= nlines) break;   // whole waves
+        C *line = tile + (size_t)l * ldt;
+        if (MEAN_ZERO && t == 0) {
+            for (int j = lane; j < ny; j += 64) line[j] = zero2;
+            continue;
         }
-    }
-    // reduction of the rows, keeping the multipliers of every step
-    R a[M], b[M], c[M], k1s[TW_STEPS][M], k2s[TW_STEPS][M];
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-        a[q] = a0[q];
-        b[q] = b0[q];
-        c[q] = c0[q];
-    }
-#pragma unroll
-    for (int s = 0; s < TW_STEPS; ++s) {
-        const int delta = 1 << s;
-        const int lo = (lane - delta) & 63, hi = (lane + delta) & 63;
-        const bool wlo = lane < delta, whi = lane + delta >= 64;
-        R rb[M], la[M], lc[M], lr[M], ha[M], hc[M], hr[M];
-#pragma unroll
-        for (int q = 0; q < M; ++q) rb[q] = R(1) / b[q];
+        const int kx = t % nxh, kz = t / nxh;
+        const R k = lamx[kx] + lamz[kz];
+        // the original rows (kept for the residual) and the right-hand side
+        R a0[M], b0[M], c0[M];
+        C d0[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) {
-            la[q] = shfl_any(a[q], lo);
-            lc[q] = shfl_any(c[q], lo);
-            lr[q] = shfl_any(rb[q], lo);
-            ha[q] = shfl_any(a[q], hi);
-            hc[q] = shfl_any(c[q], hi);
-            hr[q] = shfl_any(rb[q], hi);
+            const int j = lane + 64 * q;
+            if (j < ny && !(t == 0 && j == 0)) {
+                a0[q] = arow[j];
+                c0[q] = crow[j];
+                b0[q] = bdrow[j] - k;
+                d0[q] = line[j];
+            } else {
+                a0[q] = R(0);
+                c0[q] = R(0);
+                b0[q] = R(1);
+                d0[q] = zero2;
+            }
         }
+        // reduction of the rows, keeping the multipliers of every step
+        R a[M], b[M], c[M], k1s[TW_STEPS][M], k2s[TW_STEPS][M];
 #pragma unroll
         for (int q = 0; q < M; ++q) {
-            const R na = pick_lo<M>(la, q, wlo, R(0)), nc = pick_lo<M>(lc, q, wlo, R(0)),
-                    nr = pick_lo<M>(lr, q, wlo, R(1));
-            const R pa = pick_hi<M>(ha, q, whi, R(0)), pc = pick_hi<M>(hc, q, whi, R(0)),
-                    pr = pick_hi<M>(hr, q, whi, R(1));
-            const R k1 = a[q] * nr, k2 = c[q] * pr;
-            k1s[s][q] = k1;
-            k2s[s][q] = k2;
-            a[q] = -na * k1;
-            b[q] = b[q] - nc * k1 - pa * k2;
-            c[q] = -pc * k2;
+            a[q] = a0[q];
+            b[q] = b0[q];
+            c[q] = c0[q];
         }
-    }
-    // Thomas factors of the lane's M-row system (row q couples to rows q - 1 and q + 1 only)
-    R cp[M], rden[M];
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-        const R den = q > 0 ? b[q] - a[q] * cp[q - 1] : b[q];
-        rden[q] = R(1) / den;
-        cp[q] = c[q] * rden[q];
-    }
-    // a right-hand side through the kept reduction and Thomas factors
-    auto solve = [&](C (&d)[M]) {
 #pragma unroll
         for (int s = 0; s < TW_STEPS; ++s) {
             const int delta = 1 << s;
             const int lo = (lane - delta) & 63, hi = (lane + delta) & 63;
             const bool wlo = lane < delta, whi = lane + delta >= 64;
-            C ld[M], hd[M];
+            R rb[M], la[M], lc[M], lr[M], ha[M], hc[M], hr[M];
+#pragma unroll
+            for (int q = 0; q < M; ++q) rb[q] = R(1) / b[q];
 #pragma unroll
             for (int q = 0; q < M; ++q) {
-                ld[q] = shfl_any(d[q], lo);
-                hd[q] = shfl_any(d[q], hi);
+                la[q] = shfl_any(a[q], lo);
+                lc[q] = shfl_any(c[q], lo);
+                lr[q] = shfl_any(rb[q], lo);
+                ha[q] = shfl_any(a[q], hi);
+                hc[q] = shfl_any(c[q], hi);
+                hr[q] = shfl_any(rb[q], hi);
             }
 #pragma unroll
             for (int q = 0; q < M; ++q) {
-                const C nd = pick_lo<M>(ld, q, wlo, zero2), pd = pick_hi<M>(hd, q, whi, zero2);
-                d[q] = Cx<C>::mk(d[q].x - nd.x * k1s[s][q] - pd.x * k2s[s][q],
-                                 d[q].y - nd.y * k1s[s][q] - pd.y * k2s[s][q]);
+                const R na = pick_lo<M>(la, q, wlo, R(0)), nc = pick_lo<M>(lc, q, wlo, R(0)),
+                        nr = pick_lo<M>(lr, q, wlo, R(1));
+                const R pa = pick_hi<M>(ha, q, whi, R(0)), pc = pick_hi<M>(hc, q, whi, R(0)),
+                        pr = pick_hi<M>(hr, q, whi, R(1));
+                const R k1 = a[q] * nr, k2 = c[q] * pr;
+                k1s[s][q] = k1;
+                k2s[s][q] = k2;
+                a[q] = -na * k1;
+                b[q] = b[q] - nc * k1 - pa * k2;
+                c[q] = -pc * k2;
             }
         }
+        // Thomas factors of the lane's M-row system (row q couples to rows q - 1 and q + 1 only)
+        R cp[M], rden[M];
 #pragma unroll
         for (int q = 0; q < M; ++q) {
-            if (q > 0) d[q] = Cx<C>::mk(d[q].x - a[q] * d[q - 1].x, d[q].y - a[q] * d[q - 1].y);
-            d[q] = Cx<C>::mk(d[q].x * rden[q], d[q].y * rden[q]);
+            const R den = q > 0 ? b[q] - a[q] * cp[q - 1] : b[q];
+            rden[q] = R(1) / den;
+            cp[q] = c[q] * rden[q];
         }
+        // a right-hand side through the kept reduction and Thomas factors
+        auto solve = [&](C (&d)[M]) {
 #pragma unroll
-        for (int q = M - 2; q >= 0; --q) d[q] = Cx<C>::mk(d[q].x - cp[q] * d[q + 1].x, d[q].y - cp[q] * d[q + 1].y);
-    };
-    C x[M];
+            for (int s = 0; s < TW_STEPS; ++s) {
+                const int delta = 1 << s;
+                const int lo = (lane - delta) & 63, hi = (lane + delta) & 63;
+                const bool wlo = lane < delta, whi = lane + delta >= 64;
+                C ld[M], hd[M];
 #pragma unroll
-    for (int q = 0; q < M; ++q) x[q] = d0[q];
-    solve(x);
-    // refinement: r = d - A x with the original rows, solved the same way, added
-    {
-        const int lo = (lane - 1) & 63, hi = (lane + 1) & 63;
-        const bool wlo = lane < 1, whi = lane + 1 >= 64;
-        C lx[M], hx[M], r[M];
+                for (int q = 0; q < M; ++q) {
+                    ld[q] = shfl_any(d[q], lo);
+                    hd[q] = shfl_any(d[q], hi);
+                }
+#pragma unroll
+                for (int q = 0; q < M; ++q) {
+                    const C nd = pick_lo<M>(ld, q, wlo, zero2), pd = pick_hi<M>(hd, q, whi, zero2);
+                    d[q] = Cx<C>::mk(d[q].x - nd.x * k1s[s][q] - pd.x * k2s[s][q],
+                                     d[q].y - nd.y * k1s[s][q] - pd.y * k2s[s][q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < M; ++q) {
+                if (q > 0) d[q] = Cx<C>::mk(d[q].x - a[q] * d[q - 1].x, d[q].y - a[q] * d[q - 1].y);
+                d[q] = Cx<C>::mk(d[q].x * rden[q], d[q].y * rden[q]);
+            }
+#pragma unroll
+            for (int q = M - 2; q >= 0; --q)
+                d[q] = Cx<C>::mk(d[q].x - cp[q] * d[q + 1].x, d[q].y - cp[q] * d[q + 1].y);
+        };
+        C x[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) x[q] = d0[q];
+        solve(x);
+        // refinement: r = d - A x with the original rows, solved the same way, added
+        {
+            const int lo = (lane - 1) & 63, hi = (lane + 1) & 63;
+            const bool wlo = lane < 1, whi = lane + 1 >= 64;
+            C lx[M], hx[M], r[M];
+#pragma unroll
+            for (int q = 0; q < M; ++q) {
+                lx[q] = shfl_any(x[q], lo);
+                hx[q] = shfl_any(x[q], hi);
+            }
+#pragma unroll
+            for (int q = 0; q < M; ++q) {
+                const C xm = pick_lo<M>(lx, q, wlo, zero2), xp = pick_hi<M>(hx, q, whi, zero2);
+                r[q] = Cx<C>::mk(d0[q].x - (a0[q] * xm.x + b0[q] * x[q].x + c0[q] * xp.x),
+                                 d0[q].y - (a0[q] * xm.y + b0[q] * x[q].y + c0[q] * xp.y));
+            }
+            solve(r);
+#pragma unroll
+            for (int q = 0; q < M; ++q) x[q] = Cx<C>::mk(x[q].x + r[q].x, x[q].y + r[q].y);
+        }
 #pragma unroll
         for (int q = 0; q < M; ++q) {
-            lx[q] = shfl_any(x[q], lo);
-            hx[q] = shfl_any(x[q], hi);
+            const int j = lane + 64 * q;
+            if (j < ny) line[j] = x[q];
         }
-#pragma unroll
-        for (int q = 0; q < M; ++q) {
-            const C xm = pick_lo<M>(lx, q, wlo, zero2), xp = pick_hi<M>(hx, q, whi, zero2);
-            r[q] = Cx<C>::mk(d0[q].x - (a0[q] * xm.x + b0[q] * x[q].x + c0[q] * xp.x),
-                             d0[q].y - (a0[q] * xm.y + b0[q] * x[q].y + c0[q] * xp.y));
-        }
-        solve(r);
-#pragma unroll
-        for (int q = 0; q < M; ++q) x[q] = Cx<C>::mk(x[q].x + r[q].x, x[q].y + r[q].y);
     }
-#pragma unroll
-    for (int q = 0; q < M; ++q) {
-        const int j = lane + 64 * q;
-        if (j < ny) hat[(size_t)t + plane * j] = x[q];
+    __syncthreads();
+    // ---- back to the spectrum, as loaded
+    {
+        const int tt = threadIdx.x % TW_LINES, jj = threadIdx.x / TW_LINES;
+        const int t = t0 + tt;
+        if (t < nlines)
+            for (int j = jj; j < ny; j += 64 * TW_WAVES / TW_LINES) hat[(size_t)t + plane * j] = tile[tt * ldt + j];
     }
 }
 
@@ -428,14 +457,23 @@ static bool launch_tridiag_wave(hipStream_t st, const Dims &d, C *hat, const typ
                                 const typename Cx<C>::R *bd, const typename Cx<C>::R *c) {
     const int nxh = d.nx / 2 + 1, nlines = nxh * d.nz, m = (d.ny + 63) / 64;
     const size_t plane = (size_t)nxh * d.nz;
-    const dim3 grid((nlines + TW_WAVES - 1) / TW_WAVES), blk(64 * TW_WAVES);
+    const dim3 grid((nlines + TW_LINES - 1) / TW_LINES), blk(64 * TW_WAVES);
+    const size_t lds = (size_t)TW_LINES * (d.ny + 1) * sizeof(C);   // <= 66 KB (ny <= 256)
+#define TW_CASE(MM)                                                                                            \
+    case MM: {                                                                                                 \
+        static const hipError_t attr = hipFuncSetAttribute(                                                    \
+            reinterpret_cast<const void *>(&tridiag_wave_kernel<C, MM, MEAN_ZERO>),                            \
+            hipFuncAttributeMaxDynamicSharedMemorySize, 16 * 257 * (int)sizeof(C));                            \
+        if (attr != hipSuccess) return false;                                                                  \
+        hipLaunchKernelGGL((tridiag_wave_kernel<C, MM, MEAN_ZERO>), grid, blk, lds, st, nlines, nxh, d.ny, plane, \
+                           hat, lamx, lamz, a, bd, c);                                                         \
+        return true;                                                                                           \
+    }
     switch (m) {
-    case 1: hipLaunchKernelGGL((tridiag_wave_kernel<C, 1, MEAN_ZERO>), grid, blk, 0, st, nlines, nxh, d.ny, plane, hat, lamx, lamz, a, bd, c); return true;
-    case 2: hipLaunchKernelGGL((tridiag_wave_kernel<C, 2, MEAN_ZERO>), grid, blk, 0, st, nlines, nxh, d.ny, plane, hat, lamx, lamz, a, bd, c); return true;
-    case 3: hipLaunchKernelGGL((tridiag_wave_kernel<C, 3, MEAN_ZERO>), grid, blk, 0, st, nlines, nxh, d.ny, plane, hat, lamx, lamz, a, bd, c); return true;
-    case 4: hipLaunchKernelGGL((tridiag_wave_kernel<C, 4, MEAN_ZERO>), grid, blk, 0, st, nlines, nxh, d.ny, plane, hat, lamx, lamz, a, bd, c); return true;
+    TW_CASE(1) TW_CASE(2) TW_CASE(3) TW_CASE(4)
     default: return false;
     }
+#undef TW_CASE
 }
 
 // 5. c2r along x: the real parts of the kx=0 and kx=nx/2 coefficients, Hermitian extension of the
