@@ -157,7 +157,8 @@ __global__ __launch_bounds__(64 * SplitK<COUT>::S) void conv3_splitk_kernel(cons
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) b[nt] = bq[p][nt];
             // chunks past K are zero operands, not a branch around the MFMAs: kc depends on the
-            // wave, so such a branch would be a masked region, and MFMA does not honour the mask
+            // wave, so such a branch is an execution-masked region, and a version written that way
+            // gave wrong sums
             const int kn = kc + PD * S;
             if (j + PD < NJ) {
                 aq[p] = kn < KC ? load_a(kn) : zero8;
