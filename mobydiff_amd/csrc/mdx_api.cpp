@@ -373,6 +373,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         }
     }
     if (const char *lo = std::getenv("MOBYDIFF_LDS_OVERLAP")) h->rb_overlap = lo[0] != '0';
+    if (const char *mk = std::getenv("MOBYDIFF_MOM_KZ")) h->mom_kz = std::max(1, std::atoi(mk));
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
     if (const char *sf = std::getenv("MOBYDIFF_SWEEP_FUSED")) h->sweep_fused = sf[0] == '1';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {

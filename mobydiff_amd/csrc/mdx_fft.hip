@@ -508,6 +508,9 @@ __global__ __launch_bounds__(FFT_THREADS) void ixfft_kernel(Dims d, const double
     }
 }
 
+// workgroup shape of the k-marching fft_div_rhs_kernel: 64 x FC_Y cells, FC_KZ planes
+static constexpr int FC_Y = 4, FC_KZ = 16;
+
 // 6. pressure_solver.f90:150-169
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(256) void fft_correct_kernel(GridK g, const double *__restrict__ phi,
@@ -637,22 +640,38 @@ hipError_t launch_fft_backward_correct(mdx_handle *h, double dtg) {
 // system is singular and pinned by phi_hat(y_1) = 0 (pressure_fft.f90:92-96, 119-121), is solved in
 // FP64 from the plane sums of r; the other modes go through the FP32 transforms and Thomas solves.
 
-// rhs(i,k,j) of rhs_xfft_kernel, in FP64, layout (nx, nz, ny)
+// rhs(i,k,j) of rhs_xfft_kernel, in FP64, layout (nx, nz, ny).  A 64 x FC_Y workgroup owns rows
+// j0 .. j0+FC_Y-1 of a 64-cell x segment and marches FC_KZ planes: w*(k+1) becomes w*(k) of the
+// next plane in a register and u*(i+1) comes from the lane above (lane 63 reads it), so w* and u*
+// are fetched about once; v*(j+1) is the row above's v*(j) (a cache hit).
 template <bool STRETCH>
-__global__ __launch_bounds__(256) void fft_div_rhs_kernel(GridK g, const double *__restrict__ us,
-                                                          const double *__restrict__ vs,
-                                                          const double *__restrict__ ws, double dtg,
-                                                          double *__restrict__ rhs) {
+__global__ __launch_bounds__(64 * FC_Y) void fft_div_rhs_kernel(GridK g, const double *__restrict__ us,
+                                                               const double *__restrict__ vs,
+                                                               const double *__restrict__ ws, double dtg,
+                                                               double *__restrict__ rhs) {
     const Dims d = g.d;
-    const long n = (long)d.nx * d.ny * d.nz;
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(q % d.nx) + 1;
-        const long r = q / d.nx;
-        const int k = (int)(r % d.nz) + 1, j = (int)(r / d.nz) + 1;
-        const double dv = STRETCH ? (vs[d.g(i, j + 1, k)] - vs[d.g(i, j, k)]) * g.ihc[j]
-                                  : (vs[d.g(i, j + 1, k)] - vs[d.g(i, j, k)]) / g.dy;
-        rhs[q] = ((us[d.g(i + 1, j, k)] - us[d.g(i, j, k)]) / g.dx + dv +
-                  (ws[d.g(i, j, k + 1)] - ws[d.g(i, j, k)]) / g.dz) / dtg;
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * 64 + lane + 1;
+    const int j = blockIdx.y * FC_Y + threadIdx.y + 1;
+    const int k0 = blockIdx.z * FC_KZ + 1, k1 = k0 + FC_KZ - 1 < d.nz ? k0 + FC_KZ - 1 : d.nz;
+    const bool act = i <= d.nx && j <= d.ny;
+    double wk = act ? ws[d.g(i, j, k0)] : 0.0;
+    for (int k = k0; k <= k1; ++k) {
+        double uc = 0.0, up = 0.0, vj = 0.0, vjp = 0.0, wkp = 0.0;
+        if (act) {
+            uc = us[d.g(i, j, k)];
+            vj = vs[d.g(i, j, k)];
+            vjp = vs[d.g(i, j + 1, k)];
+            wkp = ws[d.g(i, j, k + 1)];
+        }
+        up = __shfl_down(uc, 1, 64);
+        if (act && (lane == 63 || i == d.nx)) up = us[d.g(i + 1, j, k)];
+        if (act) {
+            const double dv = STRETCH ? (vjp - vj) * g.ihc[j] : (vjp - vj) / g.dy;
+            rhs[(size_t)(i - 1) + (size_t)d.nx * ((size_t)(k - 1) + (size_t)d.nz * (size_t)(j - 1))] =
+                ((up - uc) / g.dx + dv + (wkp - wk) / g.dz) / dtg;
+        }
+        wk = wkp;
     }
 }
 
@@ -895,12 +914,12 @@ __global__ __launch_bounds__(FFT_THREADS) void ixfft32_add_kernel(Dims d, const 
 // the FP64 right-hand side -> h->fft_rhs, phi = 0
 hipError_t launch_fft_mixed_setup(mdx_handle *h, double dtg) {
     GridK g = grid_k(h);
+    const Dims d = h->d;
+    dim3 blk(64, FC_Y), grd((d.nx + 63) / 64, (d.ny + FC_Y - 1) / FC_Y, (d.nz + FC_KZ - 1) / FC_KZ);
     if (h->stretched)
-        hipLaunchKernelGGL(fft_div_rhs_kernel<true>, dim3(4096), dim3(256), 0, h->stream, g, h->us, h->vs, h->ws, dtg,
-                           h->fft_rhs);
+        hipLaunchKernelGGL(fft_div_rhs_kernel<true>, grd, blk, 0, h->stream, g, h->us, h->vs, h->ws, dtg, h->fft_rhs);
     else
-        hipLaunchKernelGGL(fft_div_rhs_kernel<false>, dim3(4096), dim3(256), 0, h->stream, g, h->us, h->vs, h->ws,
-                           dtg, h->fft_rhs);
+        hipLaunchKernelGGL(fft_div_rhs_kernel<false>, grd, blk, 0, h->stream, g, h->us, h->vs, h->ws, dtg, h->fft_rhs);
     hipError_t e = hipGetLastError();
     return e != hipSuccess ? e : launch_zero64(h, h->fft_phi, h->ni);
 }

@@ -200,6 +200,8 @@ struct mdx_handle {
     // half-column form: the interior columns relax the halo-free part of each colour while the
     // exchange is in flight (MOBYDIFF_LDS_OVERLAP=0 keeps the plain colour-by-colour order)
     bool rb_overlap = true;
+    // z-planes each workgroup of the tiled momentum kernel marches (MOBYDIFF_MOM_KZ)
+    int mom_kz = 8;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
     // polls of a neighbour wait before it gives up (MOBYDIFF_LDS_SPIN_LIMIT; a test hook); a wait
     // that gives up sets the error word, the batch drains, and the solve restarts on the

@@ -149,7 +149,7 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
     double *__restrict__ orv, double *__restrict__ orw, const double *__restrict__ cu,
     const double *__restrict__ cv, const double *__restrict__ cw, int coef_kstride, double dta, double dtb,
     double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
-    double ire) {
+    double ire, int /* planes per workgroup: the tiled kernel's */) {
     const Dims d = g.d;
     int tx, ty, tz;
     if (!xcd_tile(d.nx, d.ny, d.nz, BX, BY, tx, ty, tz)) return;
@@ -193,15 +193,15 @@ __global__ __launch_bounds__(MTX * TY) void momentum_tiled_kernel(
     double *__restrict__ orv, double *__restrict__ orw, const double *__restrict__ cu,
     const double *__restrict__ cv, const double *__restrict__ cw, int coef_kstride, double dta, double dtb,
     double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
-    double ire) {
+    double ire, int mkz) {
     constexpr int MPL = MTW * (TY + 2);   // window plane: the tile and its x-y halo
     __shared__ double su[3 * MPL], sv[3 * MPL], sw[3 * MPL], sp[MPL];
     const Dims d = g.d;
-    const int nseg = (d.nz + MKZ - 1) / MKZ;
+    const int nseg = (d.nz + mkz - 1) / mkz;
     int tx, ty, tz;
     if (!xcd_tile(d.nx, d.ny, nseg, MTX, TY, tx, ty, tz)) return;   // uniform per workgroup
-    const int i0 = tx * MTX + 1, j0 = ty * TY + 1, k0 = tz * MKZ + 1;
-    const int k1 = k0 + MKZ - 1 < d.nz ? k0 + MKZ - 1 : d.nz;
+    const int i0 = tx * MTX + 1, j0 = ty * TY + 1, k0 = tz * mkz + 1;
+    const int k1 = k0 + mkz - 1 < d.nz ? k0 + mkz - 1 : d.nz;
     const int tid = threadIdx.x + MTX * threadIdx.y;
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     const bool active = i <= d.nx && j <= d.ny;
@@ -285,7 +285,8 @@ hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
     const double dx = h->dx, dy = h->dy, dz = h->dz;
     const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
     const bool tiled = h->tiled_momentum;
-    const int nseg = (h->d.nz + MKZ - 1) / MKZ;
+    const int mkz = h->mom_kz;
+    const int nseg = (h->d.nz + mkz - 1) / mkz;
     const long wg8 = (long)((h->d.nx + MTX - 1) / MTX) * ((h->d.ny + 7) / 8) * nseg;
     const int ty = wg8 < 2048 ? 4 : 8;
     dim3 blk = tiled ? dim3(MTX, ty, 1) : dim3(BX, BY, 1);
@@ -297,7 +298,7 @@ hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
                        g, h->un, h->vn, h->wn,                                                   \
                        h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw, h->cu, h->cv, h->cw,  \
                        h->coef_plane ? 0 : 1, dta, dtb, dtg, 1.0 / dx, 1.0 / dy, 1.0 / dz, 1.0 / dx2, 1.0 / dy2,      \
-                       1.0 / dz2, 1.0 / h->cfg.re)
+                       1.0 / dz2, 1.0 / h->cfg.re, mkz)
     if (h->stretched) {
         if (h->ibm_on) MDX_MOM(true, true); else MDX_MOM(true, false);
     } else {
