@@ -21,6 +21,7 @@ The network, its input and target, the capture and the Adam kernels are the libr
 """
 import argparse
 import json
+import math
 import sys
 import time
 
@@ -81,6 +82,9 @@ def main(argv=None):
     ap.add_argument("--adam-steps", type=int, default=10000)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-end", type=float, default=None,
+                    help="cosine decay of the learning rate from --lr to this value over the Adam steps "
+                         "(per chunk of 500 steps; default: constant --lr)")
     ap.add_argument("--init-seed", type=int, default=0)
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
@@ -109,7 +113,10 @@ def main(argv=None):
     chunk, done, hist = 500, 0, []
     while done < args.adam_steps:
         n = min(chunk, args.adam_steps - done)
-        hist.append(round(float(net.train(n, batch=args.batch, lr=args.lr, seed=done)), 6))
+        lr = args.lr
+        if args.lr_end is not None:
+            lr = args.lr_end + (args.lr - args.lr_end) * 0.5 * (1.0 + math.cos(math.pi * done / args.adam_steps))
+        hist.append(round(float(net.train(n, batch=args.batch, lr=lr, seed=done)), 6))
         done += n
         print(f"adam {done}/{args.adam_steps}: mean loss over the last {n} steps {hist[-1]:.5f} "
               f"({time.perf_counter() - t1:.1f} s)",
@@ -117,7 +124,8 @@ def main(argv=None):
     l1 = net.eval_replay(min(pairs, 64))
     meta = {"runs": args.runs, "steps": args.steps, "every": args.every, "A": [args.amin, args.amax],
             "seeds": [args.seed0, args.seed0 + args.runs - 1], "pairs": pairs, "adam_steps": args.adam_steps,
-            "batch": args.batch, "lr": args.lr, "mse_first64_before": float(l0), "mse_first64_after": float(l1),
+            "batch": args.batch, "lr": args.lr, "lr_end": args.lr_end, "mse_first64_before": float(l0),
+            "mse_first64_after": float(l1),
             "loss_per_500_steps": hist, "capture_seconds": round(t1 - t0, 1),
             "train_seconds": round(time.perf_counter() - t1, 1), "grid": [NX, NY, NZ], "stride": args.stride}
     save(args.out, net, meta)
