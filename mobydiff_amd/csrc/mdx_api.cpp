@@ -1359,13 +1359,13 @@ static int learned_start_mixed(mdx_handle *h, int s, const GuessPlan &gp, unsign
     mdx_net *n = h->net;
     const double *prev = h->phi_hist + (size_t)s * h->ni;   // a learned start has a history
     ev_record(h, 8);
-    // Phi_0 in phi2, its residual (the network's input field) in res64, max-norm to the RP slot
-    MDX_HIP(launch_scale_copy64(h, h->phi2, prev, gp.fac, h->ni));
-    MDX_HIP(launch_residual_field64(h, h->phi2, h->res64, h->d_pinned + RP_PIN));
+    // Phi_0 in phi2, its residual (the network's input field) in res64, max-norm to the RP slot,
+    // and the network's input statistics, in one pass
+    MDX_HIP(launch_phi0_residual(h, prev, gp.fac, h->phi2, h->res64, h->d_pinned + RP_PIN, n->stats));
     {
         hipStream_t saved = n->stream;
         n->stream = h->stream;
-        hipError_t e = unet::net_input_from_solver(n, h, h->res64, h->phi2);
+        hipError_t e = unet::net_input_from_solver(n, h, h->res64, h->phi2, true);
         if (e == hipSuccess) e = unet::net_guess_to_phi(n, h->phi2, h->phi);
         n->stream = saved;
         MDX_HIP(e);

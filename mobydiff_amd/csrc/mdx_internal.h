@@ -299,6 +299,8 @@ hipError_t launch_zero32(mdx_handle *h, float *p, size_t n);
 hipError_t launch_zero64(mdx_handle *h, double *p, size_t n);
 hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, double fac, size_t n);
 hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *field, double *host_slot = nullptr);
+hipError_t launch_phi0_residual(mdx_handle *h, const double *prev, double fac, double *phi0, double *res,
+                                double *host_slot, double *stat);
 hipError_t launch_select_start(mdx_handle *h, double margin, const double *rg_slot, const double *other_slot,
                                const double *phi0, const double *res0, double *host_slot, unsigned long long seq);
 bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);

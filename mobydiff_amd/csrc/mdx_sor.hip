@@ -821,6 +821,108 @@ hipError_t launch_sor_residual64(mdx_handle *h, const double *phi) {
     return launch_residual_field64(h, phi, nullptr, nullptr);
 }
 
+// fac * Phi of one cell: the value the previous-Phi copy (scale_copy_kernel) stores
+struct ScaledPhi {
+    const double *p;
+    double fac;
+    Dims d;
+    __device__ inline double operator()(int i, int j, int k) const { return fac * p[d.in(i, j, k)]; }
+};
+
+// The learned start's input pass in one sweep: Phi_0 = fac * prev into phi0 and its residual
+// r0 = rhs - L_mu Phi_0 into res (the values the copy and the residual kernel store), and per
+// workgroup the maximum |r0| (part[b]) and the sums of r0^2 (part[nb + b]) and Phi_0^2
+// (part[2 nb + b]): the network's input statistics.
+template <bool STRETCH, bool IBM>
+__global__ __launch_bounds__(256) void phi0_residual_kernel(GridK g, MuK<double> mu, SorConst<double> s,
+                                                            const double *__restrict__ rhs,
+                                                            const double *__restrict__ prev, double fac,
+                                                            double *__restrict__ phi0, double *__restrict__ res,
+                                                            double *part) {
+    __shared__ double ssr[256], ssp[256];
+    const Dims d = g.d;
+    int tx_ = 0, ty_ = 0, tz_ = 0;
+    const bool ok = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);   // padding groups report 0
+    const int i = tx_ * 64 + threadIdx.x + 1;
+    const int j = ty_ * 4 + threadIdx.y + 1;
+    const int k = tz_ + 1;
+    double m = 0.0, sr = 0.0, sp = 0.0;
+    if (ok && i <= d.nx && j <= d.ny) {
+        const ScaledPhi P{prev, fac, d};
+        const size_t q = d.in(i, j, k);
+        double denom;
+        const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[q], i, j, k, denom);
+        const double p0 = P(i, j, k);
+        phi0[q] = p0;
+        res[q] = r;
+        m = fabs(r);
+        sr = r * r;
+        sp = p0 * p0;
+    }
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    ssr[t] = sr;
+    ssp[t] = sp;
+    block_max_part(m, part);   // synchronises the workgroup
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            ssr[t] += ssr[t + w];
+            ssp[t] += ssp[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[gridDim.x + blockIdx.x] = ssr[0];
+        part[2 * gridDim.x + blockIdx.x] = ssp[0];
+    }
+}
+
+// the two sums of phi0_residual_kernel's nb workgroups, folded in a fixed order: stat[0] =
+// rms(r0), stat[1] = rms(Phi_0) over n cells (1 where a field is zero)
+__global__ __launch_bounds__(256) void phi0_stats_kernel(const double *part, unsigned nb, double n, double *stat) {
+    __shared__ double sa[256], sb[256];
+    double a = 0.0, b = 0.0;
+    for (unsigned q = threadIdx.x; q < nb; q += 256) {
+        a += part[nb + q];
+        b += part[2 * nb + q];
+    }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double ra = sqrt(sa[0] / n), rb = sqrt(sb[0] / n);
+        stat[0] = ra > 1e-300 ? ra : 1.0;
+        stat[1] = rb > 1e-300 ? rb : 1.0;
+    }
+}
+
+hipError_t launch_phi0_residual(mdx_handle *h, const double *prev, double fac, double *phi0, double *res,
+                                double *host_slot, double *stat) {
+    GridK g = grid_k(h);
+    MuK<double> mu = mu_k<double>(h, h->mu64);
+    SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
+    if (3 * (size_t)grd.x > MAX_PARTS) return hipErrorInvalidValue;
+#define MDX_P0(S, I)                                                                                              \
+    hipLaunchKernelGGL((phi0_residual_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, prev, fac, phi0, res, \
+                       h->max_part)
+    if (h->stretched) { if (h->ibm_on) MDX_P0(true, true); else MDX_P0(true, false); }
+    else { if (h->ibm_on) MDX_P0(false, true); else MDX_P0(false, false); }
+#undef MDX_P0
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(phi0_stats_kernel, dim3(1), dim3(256), 0, h->stream, h->max_part, grd.x,
+                       (double)h->d.nx * h->d.ny * h->d.nz, stat);
+    e = hipGetLastError();
+    return e != hipSuccess ? e : launch_max_final(h, grd.x, host_slot, 0);
+}
+
 // The start of a mixed solve whose first check ran on the learned start (phi, its defect r32 and
 // maximum rg = rg_slot[0]): kept when rg < margin * other (other_slot[0], the fallback start's
 // residual maximum; a NaN keeps nothing), else replaced by the fallback: Phi = 0 with the defect

@@ -165,14 +165,15 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
                                 idz, idx2, idy2, idz2, ire);
 }
 
-// LDS-tiled momentum: a 64 x TY workgroup owns a 64 x TY column of cells and marches MKZ planes in z.
-// The u, v, w planes k-1, k, k+1 (with a one-cell x-y halo) and the p plane k live in LDS as a
-// rolling window, so each value is fetched from global memory about 1.3 (x-y halo) x (MKZ+2)/MKZ
-// (z halo) times instead of once per stencil use (about 30 loads per cell in the direct kernel).
+// LDS-tiled momentum: a 64 x TY workgroup owns a 64 x TY column of cells and marches mkz planes in z
+// (8 by default; 4, 16 and 32 measured slower).  The u, v, w planes k-1, k, k+1 (with a one-cell
+// x-y halo) and the p plane k live in LDS as a rolling window, so each value is fetched from global
+// memory about 1.3 (x-y halo) x (mkz+2)/mkz (z halo) times instead of once per stencil use (about
+// 30 loads per cell in the direct kernel).
 // TY = 8 rows per workgroup, or 4 on grids with fewer than 2048 such workgroups (measured: 54 against
 // 61 us at 192 x 128 x 96, where 576 workgroups of 512 threads leave a thin last wave; 304 against
 // 312 us at 256 x 192 x 256, where the smaller halo of 8 rows wins)
-static constexpr int MTX = 64, MTW = MTX + 2, MKZ = 8;
+static constexpr int MTX = 64, MTW = MTX + 2;
 
 struct MomTile {
     const double *su, *sv, *sw, *sp;   // LDS planes

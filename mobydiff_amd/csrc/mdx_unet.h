@@ -65,7 +65,8 @@ hipError_t net_adam(mdx_net *n, float lr);
 // Inputs: ch0 = r0 / s_r with r0 = rhs - L_mu(Phi_0) computed in FP64 by the solver, ch1 = Phi_0 /
 // s_p.  The network predicts the remaining correction: Phi = Phi_0 + out * s_r * c0, so the
 // training target is (Phi - Phi_0) / (s_r c0).
-hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0);
+hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0,
+                                 bool stats_ready = false);
 hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi);
 hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, double fac);
 hipError_t net_load_replay(mdx_net *n, int slot);

@@ -992,11 +992,14 @@ static hipError_t rms_into(mdx_net *n, const double *a, const double *b, double 
     return hipGetLastError();
 }
 
-hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0) {
+// stats_ready: stats[0..1] already hold rms(r0) and rms(Phi_0) (the solver's input pass)
+hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0,
+                                 bool stats_ready) {
     (void)h;
     hipError_t e;
     const long V = n->V[0];
-    if (phi0) {   // rms of r0 and of Phi_0 into stats[0..1]
+    if (stats_ready) {
+    } else if (phi0) {   // rms of r0 and of Phi_0 into stats[0..1]
         hipLaunchKernelGGL(input_stats2_partial, dim3(512), dim3(256), 0, n->stream, n->FV, r0, phi0, n->part);
         hipLaunchKernelGGL(input_stats2_final, dim3(1), dim3(128), 0, n->stream, n->part, 512, (double)n->FV,
                            n->stats);
