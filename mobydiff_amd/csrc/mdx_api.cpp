@@ -361,6 +361,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if (const char *tg = std::getenv("MOBYDIFF_NET_TIME_GATE")) h->net_time_gate = tg[0] != '0';
     if (const char *nf = std::getenv("MOBYDIFF_NET_FALLBACK")) h->net_fallback_prev = std::strcmp(nf, "prev") == 0;
     if (const char *wt = std::getenv("MOBYDIFF_WAVE_TRIDIAG")) h->wave_tridiag = wt[0] != '0';
+    if (const char *pt = std::getenv("MOBYDIFF_PART_TRIDIAG")) h->part_tridiag = pt[0] != '0';
     if (const char *mm = std::getenv("MOBYDIFF_MERGE_MAX")) h->merge_max = std::max(1, std::atoi(mm));
     if (const char *lf = std::getenv("MOBYDIFF_LDS_FORM")) {
         if (std::strcmp(lf, "pairs") == 0) h->rb_form = 1;

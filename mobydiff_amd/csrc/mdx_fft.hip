@@ -450,6 +450,186 @@ __global__ __launch_bounds__(64 * TW_WAVES) void tridiag_wave_kernel(int nlines,
     }
 }
 
+// 3''. The same solves as a partitioned Thomas recurrence, for ny >= 64.  A workgroup of 256 threads
+//     stages 16 consecutive lines through LDS (as above) and gives each line 16 threads, one per
+//     segment of M consecutive rows (rows past ny are identity rows).  A segment's interior rows
+//     (1 .. M-2) are eliminated by the Thomas recurrence with three right-hand sides: the data, and
+//     the couplings to the segment's first row F (through a_1) and last row E (through c_(M-2)), so
+//     every interior value is p + q F + s E.  Substituted into the first and last rows, that leaves
+//     a tridiagonal system in F_0, E_0, F_1, E_1, ... (32 unknowns per line), which one thread per
+//     line solves by the Thomas recurrence; the segments then form their interior values.  The
+//     serial chain per line is 2 (M - 2) + 64 steps instead of 2 ny, the work about twice the plain
+//     recurrence's, and the data stay in LDS: the kernel moves the spectrum once each way.  The
+//     mean mode pins row 1 as the wave solve does; MEAN_ZERO zeroes that line.
+static constexpr int TP_LINES = 16, TP_SEGS = 16;
+template <typename C, int M, bool MEAN_ZERO>
+__global__ __launch_bounds__(TP_LINES * TP_SEGS) void tridiag_part_kernel(int nlines, int nxh, int ny, size_t plane,
+                                                                        C *hat,
+                                                                        const typename Cx<C>::R *__restrict__ lamx,
+                                                                        const typename Cx<C>::R *__restrict__ lamz,
+                                                                        const typename Cx<C>::R *__restrict__ arow,
+                                                                        const typename Cx<C>::R *__restrict__ bdrow,
+                                                                        const typename Cx<C>::R *__restrict__ crow) {
+    using R = typename Cx<C>::R;
+    constexpr int NR = 2 * TP_SEGS;   // reduced rows per line
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C *tile = reinterpret_cast<C *>(lds_raw);   // [TP_LINES][ny + 1]
+    __shared__ R ra[TP_LINES][NR], rb[TP_LINES][NR], rc[TP_LINES][NR];
+    __shared__ C rd[TP_LINES][NR];
+    const int ldt = ny + 1;
+    const int tt = threadIdx.x % TP_LINES, sg = threadIdx.x / TP_LINES;
+    const int t = blockIdx.x * TP_LINES + tt;
+    const C zero2 = Cx<C>::mk(R(0), R(0));
+    for (int j = sg; j < ny; j += TP_SEGS) tile[tt * ldt + j] = t < nlines ? hat[(size_t)t + plane * j] : zero2;
+    __syncthreads();
+    C *line = tile + (size_t)tt * ldt;
+    R k = R(0);
+    if (t < nlines) k = lamx[t % nxh] + lamz[t / nxh];
+    const bool ident_line = t >= nlines || (MEAN_ZERO && t == 0);
+    const int j0 = sg * M;
+    // row r of the segment: (a, b, c); the data row is line[j0 + r]
+    auto coef = [&](int r, R &a, R &b, R &c) {
+        const int j = j0 + r;
+        if (!ident_line && j < ny && !(t == 0 && j == 0)) {
+            a = arow[j];
+            b = bdrow[j] - k;
+            c = crow[j];
+        } else {
+            a = R(0);
+            b = R(1);
+            c = R(0);
+        }
+    };
+    auto data = [&](int r) {
+        const int j = j0 + r;
+        return (!ident_line && j < ny && !(t == 0 && j == 0)) ? line[j] : zero2;
+    };
+    // ---- interior rows 1 .. M-2: forward elimination with the three right-hand sides; the data's
+    // eliminated values go back into LDS, the couplings' stay in registers
+    R cp[M], qq[M], ss[M];
+#pragma unroll
+    for (int r = 0; r < M; ++r) cp[r] = qq[r] = ss[r] = R(0);
+    R a0, b0, c0, aL, bL, cL;
+    coef(0, a0, b0, c0);
+    coef(M - 1, aL, bL, cL);
+    const C d0 = data(0), dL = data(M - 1);
+    C pprev = zero2;
+#pragma unroll
+    for (int r = 1; r <= M - 2; ++r) {
+        R a, b, c;
+        coef(r, a, b, c);
+        const C d = data(r);
+        const R den = r > 1 ? b - a * cp[r - 1] : b;
+        const R rden = R(1) / den;
+        cp[r] = r < M - 2 ? c * rden : R(0);
+        const C p = r > 1 ? Cx<C>::mk((d.x - a * pprev.x) * rden, (d.y - a * pprev.y) * rden)
+                          : Cx<C>::mk(d.x * rden, d.y * rden);
+        qq[r] = (r == 1 ? -a : -a * qq[r - 1]) * rden;
+        ss[r] = ((r == M - 2 ? -c : R(0)) - (r > 1 ? a * ss[r - 1] : R(0))) * rden;
+        if (j0 + r < ny) line[j0 + r] = p;
+        pprev = p;
+    }
+    // back substitution of the three right-hand sides
+    C pnext = zero2;
+#pragma unroll
+    for (int r = M - 2; r >= 1; --r) {
+        C p = j0 + r < ny ? line[j0 + r] : zero2;
+        if (r < M - 2) {
+            p = Cx<C>::mk(p.x - cp[r] * pnext.x, p.y - cp[r] * pnext.y);
+            qq[r] -= cp[r] * qq[r + 1];
+            ss[r] -= cp[r] * ss[r + 1];
+        }
+        if (j0 + r < ny) line[j0 + r] = p;
+        pnext = p;
+    }
+    // ---- the segment's two rows of the reduced system: F (row 0) and E (row M-1)
+    {
+        const C p1 = M > 2 ? (j0 + 1 < ny ? line[j0 + 1] : zero2) : zero2;
+        const C pl = M > 2 ? (j0 + M - 2 < ny ? line[j0 + M - 2] : zero2) : zero2;
+        if (M > 2) {
+            ra[tt][2 * sg] = a0;
+            rb[tt][2 * sg] = b0 + c0 * qq[1];
+            rc[tt][2 * sg] = c0 * ss[1];
+            rd[tt][2 * sg] = Cx<C>::mk(d0.x - c0 * p1.x, d0.y - c0 * p1.y);
+            ra[tt][2 * sg + 1] = aL * qq[M - 2];
+            rb[tt][2 * sg + 1] = bL + aL * ss[M - 2];
+            rc[tt][2 * sg + 1] = cL;
+            rd[tt][2 * sg + 1] = Cx<C>::mk(dL.x - aL * pl.x, dL.y - aL * pl.y);
+        } else {
+            ra[tt][2 * sg] = a0;
+            rb[tt][2 * sg] = b0;
+            rc[tt][2 * sg] = c0;
+            rd[tt][2 * sg] = d0;
+            ra[tt][2 * sg + 1] = aL;
+            rb[tt][2 * sg + 1] = bL;
+            rc[tt][2 * sg + 1] = cL;
+            rd[tt][2 * sg + 1] = dL;
+        }
+    }
+    __syncthreads();
+    // ---- the reduced system of each line, one thread per line (Thomas; solution into rd)
+    if (sg == 0) {
+        R cprev = R(0);
+        C dprev = zero2;
+        for (int r = 0; r < NR; ++r) {
+            const R a = ra[tt][r], den = rb[tt][r] - a * cprev, rden = R(1) / den;
+            const C d = rd[tt][r];
+            cprev = rc[tt][r] * rden;
+            dprev = Cx<C>::mk((d.x - a * dprev.x) * rden, (d.y - a * dprev.y) * rden);
+            rc[tt][r] = cprev;
+            rd[tt][r] = dprev;
+        }
+        C xn = rd[tt][NR - 1];
+        for (int r = NR - 2; r >= 0; --r) {
+            const C d = rd[tt][r];
+            xn = Cx<C>::mk(d.x - rc[tt][r] * xn.x, d.y - rc[tt][r] * xn.y);
+            rd[tt][r] = xn;
+        }
+    }
+    __syncthreads();
+    // ---- every row of the segment from F and E
+    {
+        const C F = rd[tt][2 * sg], E = rd[tt][2 * sg + 1];
+        if (j0 < ny) line[j0] = F;
+#pragma unroll
+        for (int r = 1; r <= M - 2; ++r) {
+            if (j0 + r < ny) {
+                const C p = line[j0 + r];
+                line[j0 + r] = Cx<C>::mk(p.x + qq[r] * F.x + ss[r] * E.x, p.y + qq[r] * F.y + ss[r] * E.y);
+            }
+        }
+        if (j0 + M - 1 < ny) line[j0 + M - 1] = E;
+    }
+    __syncthreads();
+    if (t < nlines)
+        for (int j = sg; j < ny; j += TP_SEGS) hat[(size_t)t + plane * j] = tile[tt * ldt + j];
+}
+
+// launches the partitioned solve for 64 <= ny <= 256 (false: the caller takes another solver)
+template <typename C, bool MEAN_ZERO>
+static bool launch_tridiag_part(hipStream_t st, const Dims &d, C *hat, const typename Cx<C>::R *lamx,
+                                const typename Cx<C>::R *lamz, const typename Cx<C>::R *a,
+                                const typename Cx<C>::R *bd, const typename Cx<C>::R *c) {
+    if (d.ny < 64 || d.ny > 256) return false;
+    const int nxh = d.nx / 2 + 1, nlines = nxh * d.nz, m = (d.ny + TP_SEGS - 1) / TP_SEGS;
+    const size_t plane = (size_t)nxh * d.nz;
+    const dim3 grid((nlines + TP_LINES - 1) / TP_LINES), blk(TP_LINES * TP_SEGS);
+    const size_t lds = (size_t)TP_LINES * (d.ny + 1) * sizeof(C);   // <= 66 KB
+#define TP_CASE(MM)                                                                                            \
+    if (m <= MM) {                                                                                             \
+        static const hipError_t attr = hipFuncSetAttribute(                                                    \
+            reinterpret_cast<const void *>(&tridiag_part_kernel<C, MM, MEAN_ZERO>),                            \
+            hipFuncAttributeMaxDynamicSharedMemorySize, 16 * 257 * (int)sizeof(C));                            \
+        if (attr != hipSuccess) return false;                                                                  \
+        hipLaunchKernelGGL((tridiag_part_kernel<C, MM, MEAN_ZERO>), grid, blk, lds, st, nlines, nxh, d.ny, plane, \
+                           hat, lamx, lamz, a, bd, c);                                                         \
+        return true;                                                                                           \
+    }
+    TP_CASE(4) TP_CASE(8) TP_CASE(12) TP_CASE(16)
+#undef TP_CASE
+    return false;
+}
+
 // launches the one-wave-per-line solve for ny <= 256 (false: the caller runs the Thomas kernel)
 template <typename C, bool MEAN_ZERO>
 static bool launch_tridiag_wave(hipStream_t st, const Dims &d, C *hat, const typename Cx<C>::R *lamx,
@@ -600,6 +780,10 @@ hipError_t launch_fft_forward(mdx_handle *h, double dtg) {
 hipError_t launch_tridiag(mdx_handle *h) {
     const Dims d = h->d;
     const int nlines = (d.nx / 2 + 1) * d.nz;
+    if (h->wave_tridiag && h->part_tridiag &&
+        launch_tridiag_part<double2, false>(h->stream, d, (double2 *)h->hat, h->lamx, h->lamz, h->tw_a, h->bdrow,
+                                            h->crow))
+        return hipGetLastError();
     if (h->wave_tridiag && launch_tridiag_wave<double2, false>(h->stream, d, (double2 *)h->hat, h->lamx, h->lamz, h->tw_a,
                                                      h->bdrow, h->crow))
         return hipGetLastError();
@@ -960,7 +1144,10 @@ hipError_t launch_fft32_solve_add(mdx_handle *h) {
     hipLaunchKernelGGL(zfft_kernel<float2>, dim3(d.ny * ((nxh + tx - 1) / tx)), dim3(FFT_THREADS), shz, h->stream, d,
                        (float2 *)h->hat32, (const float2 *)h->tw_z32, tx, 0);
     const int nlines = nxh * d.nz;
-    if (!(h->wave_tridiag && launch_tridiag_wave<float2, true>(h->stream, d, (float2 *)h->hat32, h->lamx32, h->lamz32,
+    if (!(h->wave_tridiag && h->part_tridiag &&
+          launch_tridiag_part<float2, true>(h->stream, d, (float2 *)h->hat32, h->lamx32, h->lamz32, h->tw_a32,
+                                            h->bdrow32, h->crow32)) &&
+        !(h->wave_tridiag && launch_tridiag_wave<float2, true>(h->stream, d, (float2 *)h->hat32, h->lamx32, h->lamz32,
                                                      h->tw_a32, h->bdrow32, h->crow32)))
         hipLaunchKernelGGL(tridiag32_kernel, dim3((nlines + 63) / 64), dim3(64), 0, h->stream, d, (float2 *)h->hat32,
                            h->cp32, h->den32, h->arow32);

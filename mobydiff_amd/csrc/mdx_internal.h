@@ -230,6 +230,8 @@ struct mdx_handle {
     double *lamx = nullptr, *lamz = nullptr, *tw_a = nullptr, *bdrow = nullptr, *crow = nullptr;
     float *lamx32 = nullptr, *lamz32 = nullptr, *tw_a32 = nullptr, *bdrow32 = nullptr, *crow32 = nullptr;
     bool wave_tridiag = true;      // MOBYDIFF_WAVE_TRIDIAG=0: the thread-per-line Thomas recurrence
+    bool part_tridiag = true;      // MOBYDIFF_PART_TRIDIAG=0: the wave-per-line PCR solve instead of the
+                                   // partitioned Thomas one (64 <= ny <= 256)
     double hist_dtg[3] = {0.0, 0.0, 0.0};
     // learned-guess gate per stage: consecutive solves the network's start lost, solves left to
     // skip, whether the current solve starts from the network's guess, and the sweeps of the last
