@@ -1607,8 +1607,13 @@ static const double RK_ALPHA[3] = {64.0 / 120.0, 50.0 / 120.0, 90.0 / 120.0};   
 static const double RK_BETA[3] = {0.0, -34.0 / 120.0, -50.0 / 120.0};
 static const double RK_GAMMA[3] = {64.0 / 120.0, 16.0 / 120.0, 40.0 / 120.0};
 
+// after the projection: the current fields are the only arrays it wrote
 static int post_bc(mdx_handle *h) {
-    return h->cfg.solver == MDX_SOLVER_FFT ? mdx_apply_bc(h) : mdx_apply_bc_redblack(h);
+    if (h->cfg.solver == MDX_SOLVER_FFT) {
+        MDX_HIP(launch_apply_bc_fft(h, BC_N));
+        return MDX_OK;
+    }
+    return mdx_apply_bc_redblack(h);
 }
 
 extern "C" int mdx_init_fields(mdx_handle *h, uint32_t seed, double amp, const mdx_geometry *geo, double dt) {
@@ -1661,7 +1666,9 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
         if ((rc = mdx_momentum(h, dta, dtb, dtg))) return rc;
         ev_record(h, 5);
         if (h->cfg.flowrate && (rc = mdx_flowrate_shift(h, dtg, nullptr))) return rc;
-        if (h->cfg.solver == MDX_SOLVER_FFT && (rc = mdx_apply_bc(h))) return rc;
+        // the predictor's arrays are the only ones written since the last call (the current
+        // fields' ghosts are those of the previous stage's projection or of the upload)
+        if (h->cfg.solver == MDX_SOLVER_FFT) MDX_HIP(launch_apply_bc_fft(h, BC_S));
         mdx_solve_stats st{};
         h->proj_count = s;   // RK stage of this projection (initial-guess history)
         if ((rc = mdx_pressure_projection(h, dtg, &st))) {

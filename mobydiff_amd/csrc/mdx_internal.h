@@ -266,7 +266,9 @@ GridK grid_k(const mdx_handle *h);
 BcK bc_k(const mdx_handle *h);
 
 hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg);
-hipError_t launch_apply_bc_fft(mdx_handle *h);
+// array sets of the FFT build's apply_bc: the current fields and the predictor's
+enum { BC_N = 1, BC_S = 2, BC_ALL = 3 };
+hipError_t launch_apply_bc_fft(mdx_handle *h, int fields = BC_ALL);
 hipError_t launch_apply_bc_redblack(mdx_handle *h);
 hipError_t launch_div_max(mdx_handle *h, const double *u, const double *v, const double *w);
 hipError_t launch_cfl_max(mdx_handle *h);

@@ -310,31 +310,39 @@ hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
 }
 
 // ------------------------------------------------------------------------------- apply_bc (FFT build)
-// boundary.f90:23-170: y no-slip walls (edges included), then x- and z-periodic copies.
-__global__ void bc_fft_y(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws) {
+// boundary.f90:23-170: y no-slip walls (edges included), then x- and z-periodic copies.  `fields`
+// selects the arrays: BC_N (un, vn, wn, pn), BC_S (us, vs, ws) or both (the reference's apply_bc).
+// Every ghost value is a function of its own array's interior, so refreshing only the arrays
+// written since the last call leaves every ghost as the full call would.
+__global__ void bc_fft_y(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws,
+                         int fields) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int k = blockIdx.y;
     if (i > d.nx + 1 || k > d.nz + 1) return;
     const int ny = d.ny;
-    un[d.g(i, ny + 1, k)] = -un[d.g(i, ny, k)];
-    vn[d.g(i, ny + 1, k)] = 0.0;
-    wn[d.g(i, ny + 1, k)] = -wn[d.g(i, ny, k)];
-    un[d.g(i, 0, k)] = -un[d.g(i, 1, k)];
-    vn[d.g(i, 1, k)] = 0.0;
-    wn[d.g(i, 0, k)] = -wn[d.g(i, 1, k)];
-    us[d.g(i, ny + 1, k)] = -us[d.g(i, ny, k)];
-    vs[d.g(i, ny + 1, k)] = 0.0;
-    ws[d.g(i, ny + 1, k)] = -ws[d.g(i, ny, k)];
-    us[d.g(i, 0, k)] = -us[d.g(i, 1, k)];
-    vs[d.g(i, 1, k)] = 0.0;
-    ws[d.g(i, 0, k)] = -ws[d.g(i, 1, k)];
+    if (fields & BC_N) {
+        un[d.g(i, ny + 1, k)] = -un[d.g(i, ny, k)];
+        vn[d.g(i, ny + 1, k)] = 0.0;
+        wn[d.g(i, ny + 1, k)] = -wn[d.g(i, ny, k)];
+        un[d.g(i, 0, k)] = -un[d.g(i, 1, k)];
+        vn[d.g(i, 1, k)] = 0.0;
+        wn[d.g(i, 0, k)] = -wn[d.g(i, 1, k)];
+    }
+    if (fields & BC_S) {
+        us[d.g(i, ny + 1, k)] = -us[d.g(i, ny, k)];
+        vs[d.g(i, ny + 1, k)] = 0.0;
+        ws[d.g(i, ny + 1, k)] = -ws[d.g(i, ny, k)];
+        us[d.g(i, 0, k)] = -us[d.g(i, 1, k)];
+        vs[d.g(i, 1, k)] = 0.0;
+        ws[d.g(i, 0, k)] = -ws[d.g(i, 1, k)];
+    }
 }
 
 // x-periodic ghost columns: one thread per (row (j,k), field, side) copy.  The copies of a row are
 // a stride of nx+2 apart, so each touches its own cache line; spreading them over threads keeps
 // many of these latency-bound accesses in flight.
 __global__ void bc_fft_x(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws,
-                         double *pn) {
+                         double *pn, int fields) {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nrow = (long)(d.ny + 2) * (d.nz + 2);
     if (q >= 14 * nrow) return;
@@ -343,6 +351,8 @@ __global__ void bc_fft_x(Dims d, double *un, double *vn, double *wn, double *us,
     const int j = (int)(row % (d.ny + 2)), k = (int)(row / (d.ny + 2));
     double *f = field == 0 ? un : field == 1 ? wn : field == 2 ? us : field == 3 ? ws
               : field == 4 ? vn : field == 5 ? vs : pn;
+    const bool s_field = field == 2 || field == 3 || field == 5;
+    if (!(fields & (s_field ? BC_S : BC_N))) return;
     if (field >= 4 && j < 1) return;                     // v: rows 1..ny+1
     if (field == 6 && j > d.ny) return;                  // p: rows 1..ny
     const int nx = d.nx;
@@ -351,40 +361,48 @@ __global__ void bc_fft_x(Dims d, double *un, double *vn, double *wn, double *us,
 }
 
 __global__ void bc_fft_z(Dims d, double *un, double *vn, double *wn, double *us, double *vs, double *ws,
-                         double *pn) {
+                         double *pn, int fields) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
     if (i > d.nx + 1 || j > d.ny + 1) return;
     const int nz = d.nz;
-    un[d.g(i, j, 0)] = un[d.g(i, j, nz)];
-    wn[d.g(i, j, 0)] = wn[d.g(i, j, nz)];
-    un[d.g(i, j, nz + 1)] = un[d.g(i, j, 1)];
-    wn[d.g(i, j, nz + 1)] = wn[d.g(i, j, 1)];
-    us[d.g(i, j, 0)] = us[d.g(i, j, nz)];
-    ws[d.g(i, j, 0)] = ws[d.g(i, j, nz)];
-    us[d.g(i, j, nz + 1)] = us[d.g(i, j, 1)];
-    ws[d.g(i, j, nz + 1)] = ws[d.g(i, j, 1)];
-    if (j >= 1) {
-        vn[d.g(i, j, 0)] = vn[d.g(i, j, nz)];
-        vn[d.g(i, j, nz + 1)] = vn[d.g(i, j, 1)];
-        vs[d.g(i, j, 0)] = vs[d.g(i, j, nz)];
-        vs[d.g(i, j, nz + 1)] = vs[d.g(i, j, 1)];
+    if (fields & BC_N) {
+        un[d.g(i, j, 0)] = un[d.g(i, j, nz)];
+        wn[d.g(i, j, 0)] = wn[d.g(i, j, nz)];
+        un[d.g(i, j, nz + 1)] = un[d.g(i, j, 1)];
+        wn[d.g(i, j, nz + 1)] = wn[d.g(i, j, 1)];
     }
-    if (j >= 1 && j <= d.ny) {
+    if (fields & BC_S) {
+        us[d.g(i, j, 0)] = us[d.g(i, j, nz)];
+        ws[d.g(i, j, 0)] = ws[d.g(i, j, nz)];
+        us[d.g(i, j, nz + 1)] = us[d.g(i, j, 1)];
+        ws[d.g(i, j, nz + 1)] = ws[d.g(i, j, 1)];
+    }
+    if (j >= 1) {
+        if (fields & BC_N) {
+            vn[d.g(i, j, 0)] = vn[d.g(i, j, nz)];
+            vn[d.g(i, j, nz + 1)] = vn[d.g(i, j, 1)];
+        }
+        if (fields & BC_S) {
+            vs[d.g(i, j, 0)] = vs[d.g(i, j, nz)];
+            vs[d.g(i, j, nz + 1)] = vs[d.g(i, j, 1)];
+        }
+    }
+    if ((fields & BC_N) && j >= 1 && j <= d.ny) {
         pn[d.g(i, j, 0)] = pn[d.g(i, j, nz)];
         pn[d.g(i, j, nz + 1)] = pn[d.g(i, j, 1)];
     }
 }
 
-hipError_t launch_apply_bc_fft(mdx_handle *h) {
+hipError_t launch_apply_bc_fft(mdx_handle *h, int fields) {
     const Dims d = h->d;
     hipLaunchKernelGGL(bc_fft_y, dim3((d.nx + 2 + 63) / 64, d.nz + 2), dim3(64), 0, h->stream, d,
-                       h->un, h->vn, h->wn, h->us, h->vs, h->ws);
+                       h->un, h->vn, h->wn, h->us, h->vs, h->ws, fields);
     const long nx_tasks = 14L * (d.ny + 2) * (d.nz + 2);
     hipLaunchKernelGGL(bc_fft_x, dim3((unsigned)((nx_tasks + 255) / 256)), dim3(256), 0, h->stream, d,
-                       h->un, h->vn, h->wn, h->us, h->vs, h->ws, h->pn);
+                       h->un, h->vn, h->wn, h->us, h->vs, h->ws, h->pn, fields);
     hipLaunchKernelGGL(bc_fft_z, dim3((d.nx + 2 + 63) / 64, d.ny + 2), dim3(64), 0, h->stream, d,
-                       h->un, h->vn, h->wn, h->us, h->vs, h->ws, h->pn);
+                       h->un, h->vn, h->wn, h->us, h->vs, h->ws, h->pn, fields);
     return hipGetLastError();
 }
 
@@ -629,15 +647,16 @@ __global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, cons
     const int nrows = d.ny * d.nz;
     auto term_b = [&](size_t c, double hj) { return (cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0) * hj; };
     if (d.nx <= SUM_THREADS) {
-        // one value per row and thread: four rows' loads are issued together, then added in row
-        // order (the same sums as the loop below)
+        // one value per row and thread: SUM_ROWS rows' loads are issued together, then added in
+        // row order (the same sums as the loop below)
+        constexpr int SUM_ROWS = 8;
         const int i = threadIdx.x + 1;
         if (i <= d.nx) {
             int r = blockIdx.x;
-            for (; r + 3 * SUM_BLOCKS < nrows; r += 4 * SUM_BLOCKS) {
-                double uv[4], bv[4];
+            for (; r + (SUM_ROWS - 1) * SUM_BLOCKS < nrows; r += SUM_ROWS * SUM_BLOCKS) {
+                double uv[SUM_ROWS], bv[SUM_ROWS];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < SUM_ROWS; ++q) {
                     const int rr = r + q * SUM_BLOCKS, j = rr % d.ny + 1, k = rr / d.ny + 1;
                     const double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
                     const size_t c = d.g(0, j, k) + (size_t)i;
@@ -645,7 +664,7 @@ __global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, cons
                     bv[q] = term_b(c, hj);
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
+                for (int q = 0; q < SUM_ROWS; ++q) {
                     a += uv[q];
                     b += bv[q];
                 }
