@@ -671,19 +671,26 @@ struct FineGrid {
     int FW, FH, FD;   // solver grid
 };
 
-// network input: the mean over each voxel's s^3 cells of r0 / s_r and Phi_0 / s_p
+// network input: the mean over each voxel's s^3 cells of r0 / s_r and Phi_0 / s_p.  S = s (2 or 4)
+// as a constant: the s^3 loads of a voxel are unrolled and issued together; the sums keep the loop
+// order.
+template <int S>
 __global__ void restrict_input_kernel(FineGrid f, const double *__restrict__ r0, const double *__restrict__ phi0,
                                       const double *__restrict__ stats, bf16 *__restrict__ x) {
+    constexpr int s = S;
     const double ir = 1.0 / stats[0], ip = phi0 ? 1.0 / stats[1] : 0.0;
-    const double w = 1.0 / (double)(f.s * f.s * f.s);
+    const double w = 1.0 / (double)(s * s * s);
     const long V = (long)f.W * f.H * f.D;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
         const int vx = (int)(v % f.W), vy = (int)((v / f.W) % f.H), vz = (int)(v / ((long)f.W * f.H));
         double sr = 0.0, sp = 0.0;
-        for (int c = 0; c < f.s; ++c)
-            for (int b = 0; b < f.s; ++b) {
-                const long row = (long)f.FW * ((long)(f.s * vy + b) + (long)f.FH * (f.s * vz + c)) + (long)f.s * vx;
-                for (int a = 0; a < f.s; ++a) {
+#pragma unroll
+        for (int c = 0; c < s; ++c)
+#pragma unroll
+            for (int b = 0; b < s; ++b) {
+                const long row = (long)f.FW * ((long)(s * vy + b) + (long)f.FH * (s * vz + c)) + (long)s * vx;
+#pragma unroll
+                for (int a = 0; a < s; ++a) {
                     sr += r0[row + a];
                     if (phi0) sp += phi0[row + a];
                 }
@@ -698,7 +705,7 @@ __global__ void restrict_input_kernel(FineGrid f, const double *__restrict__ r0,
 
 // coarse cell-centred coordinate of fine cell i: (i + 1/2)/s - 1/2 = i0 + t, i0 clamped to the grid
 // (the outermost fine cells take the nearest voxel's value)
-__device__ inline void prolong_axis(int i, int s, int n, int &i0, int &i1, double &t) {
+__device__ __forceinline__ void prolong_axis(int i, int s, int n, int &i0, int &i1, double &t) {
     const double xc = ((double)i + 0.5) / (double)s - 0.5;
     double fl = floor(xc);
     t = xc - fl;
@@ -708,18 +715,21 @@ __device__ inline void prolong_axis(int i, int s, int n, int &i0, int &i1, doubl
     if (i1 > n - 1) { i1 = n - 1; }
 }
 
-// Phi guess on the solver grid = Phi_0 + (trilinear prolongation of out) * s_r * c0
+// Phi guess on the solver grid = Phi_0 + (trilinear prolongation of out) * s_r * c0.  S = s (2 or
+// 4) as a constant: the division by s is then by a power-of-two constant, the same value.
+template <int S>
 __global__ void prolong_guess_kernel(FineGrid f, const float *__restrict__ out, const double *__restrict__ stats,
                                      double c0, const double *phi0, double *phi) {
+    constexpr int s = S;
     const double sc = stats[0] * c0;
     const long FV = (long)f.FW * f.FH * f.FD;
     for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < FV; v += (long)gridDim.x * blockDim.x) {
         const int i = (int)(v % f.FW), j = (int)((v / f.FW) % f.FH), k = (int)(v / ((long)f.FW * f.FH));
         int x0, x1, y0, y1, z0, z1;
         double tx, ty, tz;
-        prolong_axis(i, f.s, f.W, x0, x1, tx);
-        prolong_axis(j, f.s, f.H, y0, y1, ty);
-        prolong_axis(k, f.s, f.D, z0, z1, tz);
+        prolong_axis(i, s, f.W, x0, x1, tx);
+        prolong_axis(j, s, f.H, y0, y1, ty);
+        prolong_axis(k, s, f.D, z0, z1, tz);
         auto at = [&](int x, int y, int z) { return (double)out[((long)z * f.H + y) * f.W + x]; };
         const double c00 = at(x0, y0, z0) * (1.0 - tx) + at(x1, y0, z0) * tx;
         const double c10 = at(x0, y1, z0) * (1.0 - tx) + at(x1, y1, z0) * tx;
@@ -993,6 +1003,15 @@ static hipError_t rms_into(mdx_net *n, const double *a, const double *b, double 
 }
 
 // stats_ready: stats[0..1] already hold rms(r0) and rms(Phi_0) (the solver's input pass)
+// the prolongation at the network's stride (a compile-time constant for 2 and 4)
+static void launch_prolong(mdx_net *n, const float *coarse, const double *stats, double c0, const double *phi0,
+                           double *phi) {
+    const FineGrid f = fine_grid(n);
+    const dim3 grd(nblk(n->FV)), blk(256);
+    if (f.s == 4) hipLaunchKernelGGL(prolong_guess_kernel<4>, grd, blk, 0, n->stream, f, coarse, stats, c0, phi0, phi);
+    else hipLaunchKernelGGL(prolong_guess_kernel<2>, grd, blk, 0, n->stream, f, coarse, stats, c0, phi0, phi);   // stride 2
+}
+
 hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0,
                                  bool stats_ready) {
     (void)h;
@@ -1008,8 +1027,11 @@ hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *
     }
     if (n->stride == 1)
         hipLaunchKernelGGL(make_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, V, r0, phi0, n->stats, n->x_in);
-    else
-        hipLaunchKernelGGL(restrict_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, fine_grid(n), r0, phi0,
+    else if (n->stride == 4)
+        hipLaunchKernelGGL(restrict_input_kernel<4>, dim3(nblk(V)), dim3(256), 0, n->stream, fine_grid(n), r0, phi0,
+                           n->stats, n->x_in);
+    else   // stride 2 (the strides are 1, 2 and 4)
+        hipLaunchKernelGGL(restrict_input_kernel<2>, dim3(nblk(V)), dim3(256), 0, n->stream, fine_grid(n), r0, phi0,
                            n->stats, n->x_in);
     return hipGetLastError();
 }
@@ -1021,8 +1043,7 @@ hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi) {
         hipLaunchKernelGGL(guess_out_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->stats, n->c0,
                            phi0, phi, n->V[0]);
     else
-        hipLaunchKernelGGL(prolong_guess_kernel, dim3(nblk(n->FV)), dim3(256), 0, n->stream, fine_grid(n), n->out,
-                           n->stats, n->c0, phi0, phi);
+        launch_prolong(n, n->out, n->stats, n->c0, phi0, phi);
     return hipGetLastError();
 }
 
@@ -1072,8 +1093,7 @@ hipError_t net_prolong(mdx_net *n, const float *coarse, double *fine) {
     const double one = 1.0;
     hipError_t e = hipMemcpyAsync(n->stats + 6, &one, 8, hipMemcpyHostToDevice, n->stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(prolong_guess_kernel, dim3(nblk(n->FV)), dim3(256), 0, n->stream, fine_grid(n), coarse,
-                       n->stats + 6, 1.0, nullptr, fine);
+    launch_prolong(n, coarse, n->stats + 6, 1.0, nullptr, fine);
     return hipGetLastError();
 }
 
