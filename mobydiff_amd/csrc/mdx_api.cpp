@@ -314,6 +314,11 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     h->ni = (size_t)nx * ny * nz;
     h->stretched = cfg->stretch_beta > 0.0;
     if (const char *ng = std::getenv("MOBYDIFF_NO_GRAPHS")) h->use_graphs = !(ng[0] == '1');
+    h->inner_omega = h->cfg.omega;
+    if (const char *io = std::getenv("MOBYDIFF_INNER_OMEGA")) {
+        const double v = std::atof(io);
+        if (v > 0.0 && v < 2.0) h->inner_omega = v;
+    }
     y_metrics(h);
 
     auto fail = [&](hipError_t e, const char *what) {
@@ -904,11 +909,11 @@ static void phase_account(mdx_handle *h, bool wait) {
 // tables are rewritten in place, and a batch is captured again if the iterate buffer changed.
 static hipError_t sweep_launches(mdx_handle *h, bool fp32, int m) {
     hipError_t e = hipSuccess;
-    if (h->sweep_fused) return fp32 ? launch_sor_iters32(h, m, (float)h->cfg.omega) : launch_sor_iters64(h, m, h->cfg.omega);
+    if (h->sweep_fused) return fp32 ? launch_sor_iters32(h, m, (float)h->inner_omega) : launch_sor_iters64(h, m, h->cfg.omega);
     for (int q = 0; q < m && e == hipSuccess; ++q) {
         if (fp32) {
-            if ((e = launch_sor_sweep32(h, 0, (float)h->cfg.omega)) == hipSuccess)
-                e = launch_sor_sweep32(h, 1, (float)h->cfg.omega);
+            if ((e = launch_sor_sweep32(h, 0, (float)h->inner_omega)) == hipSuccess)
+                e = launch_sor_sweep32(h, 1, (float)h->inner_omega);
         } else {
             if ((e = launch_sor_sweep64(h, 0, h->cfg.omega)) == hipSuccess)
                 e = launch_sor_sweep64(h, 1, h->cfg.omega);

@@ -200,6 +200,9 @@ struct mdx_handle {
     // half-column form: the interior columns relax the halo-free part of each colour while the
     // exchange is in flight (MOBYDIFF_LDS_OVERLAP=0 keeps the plain colour-by-colour order)
     bool rb_overlap = true;
+    // relaxation factor of the FP32 inner sweeps of the mixed solve (the configured omega unless
+    // MOBYDIFF_INNER_OMEGA sets another); the FP64 sweeps always use the configured one
+    double inner_omega = 1.0;
     // z-planes each workgroup of the tiled momentum kernel marches (MOBYDIFF_MOM_KZ)
     int mom_kz = 8;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable

@@ -169,6 +169,26 @@ struct SumPhi {
     }
 };
 
+// ------------------------------------------------------------------------------- z-marched residuals
+// The FP64 residual kernels give a thread the column (i, j) of a 64 x 4 tile over RZ planes: the
+// loads of its planes do not depend on the arithmetic and are issued together, Phi at k - 1 and
+// k + 1 of consecutive cells are the same values (carried in registers), and a z-invariant
+// coefficient table is read once per column.  Every cell's operands are the ones cell_residual
+// forms, so each residual is that function's value.
+static constexpr int RZ = 4;
+
+__host__ inline unsigned rz_grid(const Dims &d) { return xcd_grid(d.nx, d.ny, (d.nz + RZ - 1) / RZ, 64, 4); }
+
+// the column of this thread: (i, j) and its first plane k0; false for padding threads
+__device__ inline bool rz_column(const Dims &d, int &i, int &j, int &k0) {
+    int tx_ = 0, ty_ = 0, tz_ = 0;
+    if (!xcd_tile(d.nx, d.ny, (d.nz + RZ - 1) / RZ, 64, 4, tx_, ty_, tz_)) return false;
+    i = tx_ * 64 + threadIdx.x + 1;
+    j = ty_ * 4 + threadIdx.y + 1;
+    k0 = tz_ * RZ + 1;
+    return i <= d.nx && j <= d.ny;
+}
+
 // ------------------------------------------------------------------------------- prepare / apply
 // After u = u* (pressure_redblack.f90:64-84): Dirichlet v on the walls, rhs = div(u), Phi = 0.
 // With part != nullptr the block maxima of |rhs| go to part: the residual max-norm of the start
@@ -268,16 +288,14 @@ hipError_t launch_sor_start(mdx_handle *h, const double *prev, double fac, doubl
 
 // un, vn, wn on the ghost shell: the values of u*, with the top wall value of v (walled y) on the
 // interior of the j = ny + 1 plane; the boundary conditions that follow the projection overwrite
-// the faces, and edges and corners keep the u* values
-__global__ __launch_bounds__(256) void proj_ghost_kernel(Dims d, int walled_y, double vtop,
-                                                         const double *__restrict__ us,
-                                                         const double *__restrict__ vs,
-                                                         const double *__restrict__ ws, double *un, double *vn,
-                                                         double *wn) {
+// the faces, and edges and corners keep the u* values.  Thread `first` of `stride` threads.
+__device__ inline void proj_ghost(const Dims &d, int walled_y, double vtop, const double *__restrict__ us,
+                                  const double *__restrict__ vs, const double *__restrict__ ws, double *un,
+                                  double *vn, double *wn, long long first, long long stride) {
     const long long sz = d.nz + 2;
     const long long pa = (long long)d.sx * d.sy, pb = (long long)d.sx * d.nz, pc = (long long)d.ny * d.nz;
     const long long n = 2 * (pa + pb + pc);
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = first; t < n; t += stride) {
         int i, j, k;
         long long q = t;
         if (q < 2 * pa) {            // the k = 0 and k = nz + 1 planes
@@ -323,55 +341,96 @@ hipError_t launch_sor_prepare(mdx_handle *h, double *rhs_max_slot, unsigned long
     return launch_max_final(h, grd.x, rhs_max_slot, seq);
 }
 
+static constexpr unsigned PROJ_GHOST_BLOCKS = 256;
+
 // un = u* - mu grad Phi (likewise v, w) and pn += Phi / dt on the interior, reading u* directly;
-// on a walled y the bottom v face keeps its wall value
+// on a walled y the bottom v face keeps its wall value.  A thread takes the column (i, j) of a
+// 64 x 4 tile over RZ planes (the residual kernels' layout): its loads are issued before its
+// stores, Phi at k - 1 is the previous cell's Phi, and a z-invariant coefficient table is read once.
+// The `tiles` workgroups of the tiles are followed by PROJ_GHOST_BLOCKS that fill the ghost shell
+// (proj_ghost: cells no tile writes, read from cells no tile reads).
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu, const double *__restrict__ phi,
                                                         const double *__restrict__ us,
                                                         const double *__restrict__ vs,
-                                                        const double *__restrict__ ws, double vbot, double *un,
-                                                        double *vn, double *wn, double *pn, double idx, double idy,
-                                                        double idz, double idt) {
+                                                        const double *__restrict__ ws, double vbot,
+                                                        double *__restrict__ un, double *__restrict__ vn,
+                                                        double *__restrict__ wn, double *__restrict__ pn, double idx,
+                                                        double idy, double idz, double idt, unsigned tiles,
+                                                        double vtop) {
     const Dims d = g.d;
-    int tx_, ty_, tz_;
-    if (!xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_)) return;
-    const int i = tx_ * 64 + threadIdx.x + 1;
-    const int j = ty_ * 4 + threadIdx.y + 1;
-    const int k = tz_ + 1;
-    if (i > d.nx || j > d.ny) return;
-    const bool per2 = g.per[1];
-    const int im = i == 1 ? d.nx : i - 1, km = k == 1 ? d.nz : k - 1;
-    const size_t c = d.g(i, j, k);
-    const double p0 = phi[d.in(i, j, k)];
-    const double mu_u = IBM ? mu.u[mu.at(i, j, k)] : 1.0;
-    const double mu_v = IBM ? mu.v[mu.at(i, j, k)] : 1.0;
-    const double mu_w = IBM ? mu.w[mu.at(i, j, k)] : 1.0;
-    un[c] = us[c] + mu_u * (phi[d.in(im, j, k)] - p0) * idx;
-    wn[c] = ws[c] + mu_w * (phi[d.in(i, j, km)] - p0) * idz;
-    if (j >= 2 || per2) {
-        const int jm = j == 1 ? d.ny : j - 1;
-        const double gy = STRETCH ? (phi[d.in(i, jm, k)] - p0) * g.ihv[j] : (phi[d.in(i, jm, k)] - p0) * idy;
-        vn[c] = vs[c] + mu_v * gy;
-    } else {
-        vn[c] = vbot;
+    if (blockIdx.x >= tiles) {   // the workgroups past the tiles: un, vn, wn on the ghost shell
+        proj_ghost(d, g.per[1] ? 0 : 1, vtop, us, vs, ws, un, vn, wn,
+                   (long long)(blockIdx.x - tiles) * 256 + threadIdx.x + 64 * threadIdx.y,
+                   (long long)(gridDim.x - tiles) * 256);
+        return;
     }
-    pn[c] = pn[c] + p0 * idt;
+    int i = 0, j = 0, k0 = 0;
+    if (!rz_column(d, i, j, k0)) return;
+    const bool per2 = g.per[1];
+    const bool vwall = !(j >= 2 || per2);
+    const int im = i == 1 ? d.nx : i - 1, jm = j == 1 ? d.ny : j - 1;
+    const int n = d.nz - k0 + 1 < RZ ? d.nz - k0 + 1 : RZ;
+    double pz[RZ + 1], pxm[RZ], pym[RZ], uo[RZ], vo[RZ], wo[RZ], po[RZ];
+    pz[0] = phi[d.in(i, j, k0 == 1 ? d.nz : k0 - 1)];
+#pragma unroll
+    for (int u = 0; u < RZ; ++u) {
+        if (u < n) {
+            const int k = k0 + u;
+            const size_t c = d.g(i, j, k);
+            pz[u + 1] = phi[d.in(i, j, k)];
+            pxm[u] = phi[d.in(im, j, k)];
+            pym[u] = vwall ? 0.0 : phi[d.in(i, jm, k)];
+            uo[u] = us[c];
+            vo[u] = vwall ? 0.0 : vs[c];
+            wo[u] = ws[c];
+            po[u] = pn[c];
+        }
+    }
+    double mu_u = 1.0, mu_v = 1.0, mu_w = 1.0;
+    if (IBM && mu.plane) {
+        mu_u = mu.u[mu.at(i, j, 1)];
+        mu_v = mu.v[mu.at(i, j, 1)];
+        mu_w = mu.w[mu.at(i, j, 1)];
+    }
+    const double ihv = STRETCH ? g.ihv[j] : 0.0;
+#pragma unroll
+    for (int u = 0; u < RZ; ++u) {
+        if (u < n) {
+            const int k = k0 + u;
+            const size_t c = d.g(i, j, k);
+            if (IBM && !mu.plane) {
+                mu_u = mu.u[mu.at(i, j, k)];
+                mu_v = mu.v[mu.at(i, j, k)];
+                mu_w = mu.w[mu.at(i, j, k)];
+            }
+            const double p0 = pz[u + 1];
+            un[c] = uo[u] + mu_u * (pxm[u] - p0) * idx;
+            wn[c] = wo[u] + mu_w * (pz[u] - p0) * idz;
+            if (!vwall) {
+                const double gy = STRETCH ? (pym[u] - p0) * ihv : (pym[u] - p0) * idy;
+                vn[c] = vo[u] + mu_v * gy;
+            } else {
+                vn[c] = vbot;
+            }
+            pn[c] = po[u] + p0 * idt;
+        }
+    }
 }
 
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
-    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
+    const unsigned tiles = rz_grid(h->d);
+    dim3 blk(64, 4), grd(tiles + PROJ_GHOST_BLOCKS);
     const double a = 1.0 / h->dx, b = 1.0 / h->dy, c = 1.0 / h->dz, t = 1.0 / dtg;
     const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
 #define MDX_APPLY(S, I)                                                                                     \
     hipLaunchKernelGGL((sor_apply_kernel<S, I>), grd, blk, 0, h->stream, g, mu, phi, h->us, h->vs, h->ws, vb, \
-                       h->un, h->vn, h->wn, h->pn, a, b, c, t)
+                       h->un, h->vn, h->wn, h->pn, a, b, c, t, tiles, vt)
     if (h->stretched) { if (h->ibm_on) MDX_APPLY(true, true); else MDX_APPLY(true, false); }
     else { if (h->ibm_on) MDX_APPLY(false, true); else MDX_APPLY(false, false); }
 #undef MDX_APPLY
-    hipLaunchKernelGGL(proj_ghost_kernel, dim3(512), dim3(256), 0, h->stream, h->d, h->cfg.periodic[1] ? 0 : 1, vt,
-                       h->us, h->vs, h->ws, h->un, h->vn, h->wn);
     return hipGetLastError();
 }
 
@@ -779,24 +838,85 @@ hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega) {
     return hipGetLastError();
 }
 
+// residuals r[u] = rhs - L_mu(Phi) of cells (i, j, k0 + u), u < n = min(RZ, nz - k0 + 1), and
+// Phi of those cells in p[u]
+template <bool STRETCH, bool IBM, typename PhiAt>
+__device__ inline int residual_column(const GridK &g, const MuK<double> &mu, const SorConst<double> &s, PhiAt P,
+                                      const double *__restrict__ rhs, int i, int j, int k0, double (&r)[RZ],
+                                      double (&p)[RZ]) {
+    const Dims d = g.d;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const bool per2 = g.per[1];
+    const int ip = i == nx ? 1 : i + 1, im = i == 1 ? nx : i - 1;
+    int jp = j + 1, jm = j - 1;
+    if (per2) {
+        if (jp > ny) jp = 1;
+        if (jm < 1) jm = ny;
+    }
+    const bool wn_ = !per2 && j == ny, ws_ = !per2 && j == 1;
+    const int jn = wn_ ? j : jp, js = ws_ ? j : jm;
+    double an = 0, as = 0;
+    if (STRETCH) {
+        an = g.ihv[j + 1] * g.ihc[j];
+        as = g.ihv[j] * g.ihc[j];
+    }
+    const int n = nz - k0 + 1 < RZ ? nz - k0 + 1 : RZ;
+    double pz[RZ + 2];   // Phi at k0 - 1 .. k0 + n (periodic in z), of this column
+    pz[0] = P(i, j, k0 == 1 ? nz : k0 - 1);
+#pragma unroll
+    for (int u = 0; u <= RZ; ++u) {
+        const int k = k0 + u;
+        pz[u + 1] = u <= n ? P(i, j, k > nz ? k - nz : k) : 0.0;
+    }
+    double mui = 1, muip = 1, mvj = 1, mvjp = 1, mwc = 1;
+    if (IBM && mu.plane) {   // z-invariant table: the column's six face coefficients (w's twice)
+        mui = mu.u[mu.at(i, j, 1)];
+        muip = mu.u[mu.at(ip, j, 1)];
+        mvj = mu.v[mu.at(i, j, 1)];
+        mvjp = mu.v[mu.at(i, jp, 1)];
+        mwc = mu.w[mu.at(i, j, 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < RZ; ++u) {
+        r[u] = 0.0;
+        p[u] = pz[u + 1];
+        if (u < n) {
+            const int k = k0 + u, kp = k == nz ? 1 : k + 1;
+            double mwk = mwc, mwkp = mwc;
+            if (IBM && !mu.plane) {
+                mui = mu.u[mu.at(i, j, k)];
+                muip = mu.u[mu.at(ip, j, k)];
+                mvj = mu.v[mu.at(i, j, k)];
+                mvjp = mu.v[mu.at(i, jp, k)];
+                mwk = mu.w[mu.at(i, j, k)];
+                mwkp = mu.w[mu.at(i, j, kp)];
+            }
+            double denom;
+            r[u] = sor_residual_core<double, STRETCH>(pz[u + 1], P(ip, j, k), P(im, j, k), P(i, jn, k), P(i, js, k),
+                                                     pz[u + 2], pz[u], mui, muip, mvj, mvjp, mwk, mwkp, wn_, ws_,
+                                                     rhs[d.in(i, j, k)], s, an, as, denom);
+        }
+    }
+    return n;
+}
+
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(256) void sor_residual64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
                                                              const double *__restrict__ rhs,
                                                              const double *__restrict__ phi, double *out,
                                                              double *__restrict__ field) {
     const Dims d = g.d;
-    int tx_ = 0, ty_ = 0, tz_ = 0;
-    const bool ok = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);   // padding groups report 0
-    const int i = tx_ * 64 + threadIdx.x + 1;
-    const int j = ty_ * 4 + threadIdx.y + 1;
-    const int k = tz_ + 1;
+    int i = 0, j = 0, k0 = 0;
     double m = 0.0;
-    if (ok && i <= d.nx && j <= d.ny) {
-        PlainPhi<double> P{phi, d};
-        double denom;
-        const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[d.in(i, j, k)], i, j, k, denom);
-        if (field) field[d.in(i, j, k)] = r;
-        m = fabs(r);
+    if (rz_column(d, i, j, k0)) {   // padding threads report 0
+        double r[RZ], p[RZ];
+        const int n = residual_column<STRETCH, IBM>(g, mu, s, PlainPhi<double>{phi, d}, rhs, i, j, k0, r, p);
+#pragma unroll
+        for (int u = 0; u < RZ; ++u)
+            if (u < n) {
+                if (field) field[d.in(i, j, k0 + u)] = r[u];
+                m = nanmax_any(m, fabs(r[u]));
+            }
     }
     block_max_part(m, out);
 }
@@ -807,7 +927,7 @@ hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *fie
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
-    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
+    dim3 blk(64, 4), grd(rz_grid(h->d));
 #define MDX_RS(S, I) \
     hipLaunchKernelGGL((sor_residual64_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi, h->max_part, field)
     if (h->stretched) { if (h->ibm_on) MDX_RS(true, true); else MDX_RS(true, false); }
@@ -839,64 +959,81 @@ __global__ __launch_bounds__(256) void phi0_residual_kernel(GridK g, MuK<double>
                                                             const double *__restrict__ prev, double fac,
                                                             double *__restrict__ phi0, double *__restrict__ res,
                                                             double *part) {
-    __shared__ double ssr[256], ssp[256];
+    __shared__ double red[3][4];
     const Dims d = g.d;
-    int tx_ = 0, ty_ = 0, tz_ = 0;
-    const bool ok = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);   // padding groups report 0
-    const int i = tx_ * 64 + threadIdx.x + 1;
-    const int j = ty_ * 4 + threadIdx.y + 1;
-    const int k = tz_ + 1;
+    int i = 0, j = 0, k0 = 0;
     double m = 0.0, sr = 0.0, sp = 0.0;
-    if (ok && i <= d.nx && j <= d.ny) {
-        const ScaledPhi P{prev, fac, d};
-        const size_t q = d.in(i, j, k);
-        double denom;
-        const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[q], i, j, k, denom);
-        const double p0 = P(i, j, k);
-        phi0[q] = p0;
-        res[q] = r;
-        m = fabs(r);
-        sr = r * r;
-        sp = p0 * p0;
+    if (rz_column(d, i, j, k0)) {   // padding threads report 0
+        double r[RZ], p[RZ];
+        const int n = residual_column<STRETCH, IBM>(g, mu, s, ScaledPhi{prev, fac, d}, rhs, i, j, k0, r, p);
+#pragma unroll
+        for (int u = 0; u < RZ; ++u)
+            if (u < n) {
+                const size_t q = d.in(i, j, k0 + u);
+                phi0[q] = p[u];
+                res[q] = r[u];
+                m = nanmax_any(m, fabs(r[u]));
+                sr += r[u] * r[u];
+                sp += p[u] * p[u];
+            }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m = nanmax_any(m, __shfl_xor(m, off, 64));
+        sr += __shfl_xor(sr, off, 64);
+        sp += __shfl_xor(sp, off, 64);
     }
     const int t = threadIdx.x + 64 * threadIdx.y;
-    ssr[t] = sr;
-    ssp[t] = sp;
-    block_max_part(m, part);   // synchronises the workgroup
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) {
-            ssr[t] += ssr[t + w];
-            ssp[t] += ssp[t + w];
-        }
-        __syncthreads();
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = m;
+        red[1][t >> 6] = sr;
+        red[2][t >> 6] = sp;
     }
+    __syncthreads();
     if (t == 0) {
-        part[gridDim.x + blockIdx.x] = ssr[0];
-        part[2 * gridDim.x + blockIdx.x] = ssp[0];
+        const unsigned nb = gridDim.x, b = blockIdx.x;
+        part[b] = nanmax_any(nanmax_any(red[0][0], red[0][1]), nanmax_any(red[0][2], red[0][3]));
+        part[nb + b] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        part[2 * nb + b] = (red[2][0] + red[2][1]) + (red[2][2] + red[2][3]);
     }
 }
 
-// the two sums of phi0_residual_kernel's nb workgroups, folded in a fixed order: stat[0] =
-// rms(r0), stat[1] = rms(Phi_0) over n cells (1 where a field is zero)
-__global__ __launch_bounds__(256) void phi0_stats_kernel(const double *part, unsigned nb, double n, double *stat) {
-    __shared__ double sa[256], sb[256];
+// the two sums of phi0_residual_kernel's nb workgroups (1024 threads, the loads of a thread
+// issued together): stat[0] = rms(r0), stat[1] = rms(Phi_0) over n cells (1 where a field is zero)
+__global__ __launch_bounds__(1024) void phi0_stats_kernel(const double *part, unsigned nb, double n, double *stat) {
+    __shared__ double sa[16], sb[16];
+    constexpr unsigned B = 8;
     double a = 0.0, b = 0.0;
-    for (unsigned q = threadIdx.x; q < nb; q += 256) {
-        a += part[nb + q];
-        b += part[2 * nb + q];
-    }
-    sa[threadIdx.x] = a;
-    sb[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            sa[threadIdx.x] += sa[threadIdx.x + w];
-            sb[threadIdx.x] += sb[threadIdx.x + w];
+    for (unsigned q0 = threadIdx.x; q0 < nb; q0 += 1024 * B) {
+        double va[B], vb[B];
+#pragma unroll
+        for (unsigned u = 0; u < B; ++u) {
+            const unsigned q = q0 + 1024 * u;
+            va[u] = q < nb ? part[nb + q] : 0.0;
+            vb[u] = q < nb ? part[2 * nb + q] : 0.0;
         }
-        __syncthreads();
+#pragma unroll
+        for (unsigned u = 0; u < B; ++u) {
+            a += va[u];
+            b += vb[u];
+        }
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sa[threadIdx.x >> 6] = a;
+        sb[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const double ra = sqrt(sa[0] / n), rb = sqrt(sb[0] / n);
+        a = 0.0;
+        b = 0.0;
+        for (int w = 0; w < 16; ++w) {
+            a += sa[w];
+            b += sb[w];
+        }
+        const double ra = sqrt(a / n), rb = sqrt(b / n);
         stat[0] = ra > 1e-300 ? ra : 1.0;
         stat[1] = rb > 1e-300 ? rb : 1.0;
     }
@@ -907,7 +1044,7 @@ hipError_t launch_phi0_residual(mdx_handle *h, const double *prev, double fac, d
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
-    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
+    dim3 blk(64, 4), grd(rz_grid(h->d));
     if (3 * (size_t)grd.x > MAX_PARTS) return hipErrorInvalidValue;
 #define MDX_P0(S, I)                                                                                              \
     hipLaunchKernelGGL((phi0_residual_kernel<S, I>), grd, blk, 0, h->stream, g, mu, s, h->rhs, prev, fac, phi0, res, \
@@ -917,7 +1054,7 @@ hipError_t launch_phi0_residual(mdx_handle *h, const double *prev, double fac, d
 #undef MDX_P0
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(phi0_stats_kernel, dim3(1), dim3(256), 0, h->stream, h->max_part, grd.x,
+    hipLaunchKernelGGL(phi0_stats_kernel, dim3(1), dim3(1024), 0, h->stream, h->max_part, grd.x,
                        (double)h->d.nx * h->d.ny * h->d.nz, stat);
     e = hipGetLastError();
     return e != hipSuccess ? e : launch_max_final(h, grd.x, host_slot, 0);
@@ -972,27 +1109,28 @@ hipError_t launch_select_start(mdx_handle *h, double margin, const double *rg_sl
 // One FP64 defect-correction step: Phi_out = Phi_in + e; r = rhs - L_mu(Phi_out) in FP64;
 // r32 = (float) r for the next FP32 inner solve; max |r| for the stopping test.  Without e (E
 // false: e = 0, the first check of a solve) Phi_out may be Phi_in itself: every read adds 0.0,
-// so a neighbour read before or after its own cell's rewrite gives the same value.
-template <bool STRETCH, bool IBM, bool E>
+// so a neighbour read before or after its own cell's rewrite gives the same value.  STORE false:
+// Phi_out is Phi_in and E is false, and Phi is left as it is (Phi_in + 0.0 differs from Phi_in only
+// in the sign of a zero, which no sum with another operand keeps).
+template <bool STRETCH, bool IBM, bool E, bool STORE>
 __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, SorConst<double> s,
                                                        const double *__restrict__ rhs, const double *phi_in,
                                                        const float *__restrict__ e, double *phi_out,
                                                        float *r32, double *out) {
     const Dims d = g.d;
-    int tx_ = 0, ty_ = 0, tz_ = 0;
-    const bool ok = xcd_tile(d.nx, d.ny, d.nz, 64, 4, tx_, ty_, tz_);   // padding groups report 0
-    const int i = tx_ * 64 + threadIdx.x + 1;
-    const int j = ty_ * 4 + threadIdx.y + 1;
-    const int k = tz_ + 1;
+    int i = 0, j = 0, k0 = 0;
     double m = 0.0;
-    if (ok && i <= d.nx && j <= d.ny) {
-        SumPhi<E> P{phi_in, e, d};
-        double denom;
-        const size_t q = d.in(i, j, k);
-        const double r = cell_residual<double, STRETCH, IBM>(g, mu, s, P, rhs[q], i, j, k, denom);
-        phi_out[q] = P(i, j, k);
-        r32[q] = (float)r;
-        m = fabs(r);
+    if (rz_column(d, i, j, k0)) {   // padding threads report 0
+        double r[RZ], p[RZ];
+        const int n = residual_column<STRETCH, IBM>(g, mu, s, SumPhi<E>{phi_in, e, d}, rhs, i, j, k0, r, p);
+#pragma unroll
+        for (int u = 0; u < RZ; ++u)
+            if (u < n) {
+                const size_t q = d.in(i, j, k0 + u);
+                if (STORE) phi_out[q] = p[u];
+                r32[q] = (float)r[u];
+                m = nanmax_any(m, fabs(r[u]));
+            }
     }
     block_max_part(m, out);
 }
@@ -1002,11 +1140,16 @@ hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, 
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
-    dim3 blk(64, 4), grd(xcd_grid(h->d.nx, h->d.ny, h->d.nz, 64, 4));
-#define MDX_DF(S, I, E)                                                                                       \
-    hipLaunchKernelGGL((defect64_kernel<S, I, E>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi_in, e, phi_out, \
+    dim3 blk(64, 4), grd(rz_grid(h->d));
+#define MDX_DF(S, I, E, W)                                                                                       \
+    hipLaunchKernelGGL((defect64_kernel<S, I, E, W>), grd, blk, 0, h->stream, g, mu, s, h->rhs, phi_in, e, phi_out, \
                        r32, h->max_part)
-#define MDX_DFE(S, I) do { if (e) MDX_DF(S, I, true); else MDX_DF(S, I, false); } while (0)
+#define MDX_DFE(S, I)                                                    \
+    do {                                                                 \
+        if (e) MDX_DF(S, I, true, true);                                 \
+        else if (phi_out == phi_in) MDX_DF(S, I, false, false);          \
+        else MDX_DF(S, I, false, true);                                  \
+    } while (0)
     if (h->stretched) { if (h->ibm_on) MDX_DFE(true, true); else MDX_DFE(true, false); }
     else { if (h->ibm_on) MDX_DFE(false, true); else MDX_DFE(false, false); }
 #undef MDX_DFE

@@ -1041,7 +1041,7 @@ static hipError_t rb_launch_t(mdx_handle *h, const RbPlanOut &pl, T *x, const T 
         a.s = SorConst<T>{(T)(1.0 / (h->dx * h->dx)), (T)(1.0 / (h->dy * h->dy)), (T)(1.0 / (h->dz * h->dz))};
     else
         a.s = SorConst<T>{(T)ix2, (T)iy2, (T)iz2};
-    a.omega = (T)h->cfg.omega;
+    a.omega = (T)(sizeof(T) == 4 ? h->inner_omega : h->cfg.omega);
     a.mu_u = h->ibm_on ? mu[0] : nullptr;
     a.mu_v = h->ibm_on ? mu[1] : nullptr;
     a.mu_w = h->ibm_on ? mu[2] : nullptr;

@@ -515,7 +515,19 @@ __global__ __launch_bounds__(1024) void max_final_kernel(const double *__restric
                                                          unsigned long long seq) {
     __shared__ double w[16];
     double m = 0.0;
-    for (unsigned q = threadIdx.x; q < n; q += 1024) m = nanmax(m, part[q]);
+    // MF_BATCH partials per thread are loaded before any is folded (one memory latency per batch
+    // instead of one per partial); the maximum does not depend on the order
+    constexpr unsigned MF_BATCH = 16;
+    for (unsigned q0 = threadIdx.x; q0 < n; q0 += 1024 * MF_BATCH) {
+        double v[MF_BATCH];
+#pragma unroll
+        for (unsigned u = 0; u < MF_BATCH; ++u) {
+            const unsigned q = q0 + u * 1024;
+            v[u] = q < n ? part[q] : 0.0;
+        }
+#pragma unroll
+        for (unsigned u = 0; u < MF_BATCH; ++u) m = nanmax(m, v[u]);
+    }
     for (int off = 32; off > 0; off >>= 1) m = nanmax(m, __shfl_xor(m, off, 64));
     if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
     __syncthreads();

@@ -672,35 +672,52 @@ struct FineGrid {
 };
 
 // network input: the mean over each voxel's s^3 cells of r0 / s_r and Phi_0 / s_p.  S = s (2 or 4)
-// as a constant: the s^3 loads of a voxel are unrolled and issued together; the sums keep the loop
-// order.
+// as a constant.  A workgroup of 64 x S^2 threads takes 64 voxels of one (vy, vz) row (grid
+// ceil(W/64) x H x D): thread (x, y) sums the S cells of fine row y (b = y % S, c = y / S) of voxel
+// x, lanes reading consecutive memory, and the S^2 row sums of a voxel are added in row order.
 template <int S>
-__global__ void restrict_input_kernel(FineGrid f, const double *__restrict__ r0, const double *__restrict__ phi0,
-                                      const double *__restrict__ stats, bf16 *__restrict__ x) {
-    constexpr int s = S;
+__global__ __launch_bounds__(64 * S * S) void restrict_input_kernel(FineGrid f, const double *__restrict__ r0,
+                                                                    const double *__restrict__ phi0,
+                                                                    const double *__restrict__ stats,
+                                                                    bf16 *__restrict__ x) {
+    constexpr int s = S, R = S * S;
+    __shared__ double pr[R][64], pp[R][64];
+    const int vx = blockIdx.x * 64 + threadIdx.x, vy = blockIdx.y, vz = blockIdx.z, y = threadIdx.y;
+    double sr = 0.0, sp = 0.0;
+    if (vx < f.W) {
+        const int b = y % s, c = y / s;
+        const long row = (long)f.FW * ((long)(s * vy + b) + (long)f.FH * (s * vz + c)) + (long)s * vx;
+        double vr[s], vp[s];
+#pragma unroll
+        for (int a = 0; a < s; ++a) {
+            vr[a] = r0[row + a];
+            vp[a] = phi0 ? phi0[row + a] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < s; ++a) {
+            sr += vr[a];
+            sp += vp[a];
+        }
+    }
+    pr[y][threadIdx.x] = sr;
+    pp[y][threadIdx.x] = sp;
+    __syncthreads();
+    if (y != 0 || vx >= f.W) return;
+    sr = 0.0;
+    sp = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        sr += pr[q][threadIdx.x];
+        sp += pp[q][threadIdx.x];
+    }
     const double ir = 1.0 / stats[0], ip = phi0 ? 1.0 / stats[1] : 0.0;
     const double w = 1.0 / (double)(s * s * s);
-    const long V = (long)f.W * f.H * f.D;
-    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
-        const int vx = (int)(v % f.W), vy = (int)((v / f.W) % f.H), vz = (int)(v / ((long)f.W * f.H));
-        double sr = 0.0, sp = 0.0;
-#pragma unroll
-        for (int c = 0; c < s; ++c)
-#pragma unroll
-            for (int b = 0; b < s; ++b) {
-                const long row = (long)f.FW * ((long)(s * vy + b) + (long)f.FH * (s * vz + c)) + (long)s * vx;
-#pragma unroll
-                for (int a = 0; a < s; ++a) {
-                    sr += r0[row + a];
-                    if (phi0) sp += phi0[row + a];
-                }
-            }
-        bf16x8 lo = {}, hi = {};
-        lo[0] = (bf16)(float)(sr * w * ir);
-        lo[1] = (bf16)(float)(sp * w * ip);
-        *reinterpret_cast<bf16x8 *>(x + v * 16) = lo;
-        *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = hi;
-    }
+    const long v = ((long)vz * f.H + vy) * f.W + vx;
+    bf16x8 lo = {}, hi = {};
+    lo[0] = (bf16)(float)(sr * w * ir);
+    lo[1] = (bf16)(float)(sp * w * ip);
+    *reinterpret_cast<bf16x8 *>(x + v * 16) = lo;
+    *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = hi;
 }
 
 // coarse cell-centred coordinate of fine cell i: (i + 1/2)/s - 1/2 = i0 + t, i0 clamped to the grid
@@ -716,28 +733,29 @@ __device__ __forceinline__ void prolong_axis(int i, int s, int n, int &i0, int &
 }
 
 // Phi guess on the solver grid = Phi_0 + (trilinear prolongation of out) * s_r * c0.  S = s (2 or
-// 4) as a constant: the division by s is then by a power-of-two constant, the same value.
+// 4) as a constant: the division by s is then by a power-of-two constant, the same value.  One
+// thread per fine cell, on a grid of ceil(FW/64) x ceil(FH/4) x FD workgroups of 64 x 4.
 template <int S>
-__global__ void prolong_guess_kernel(FineGrid f, const float *__restrict__ out, const double *__restrict__ stats,
-                                     double c0, const double *phi0, double *phi) {
+__global__ __launch_bounds__(256) void prolong_guess_kernel(FineGrid f, const float *__restrict__ out,
+                                                            const double *__restrict__ stats, double c0,
+                                                            const double *phi0, double *phi) {
     constexpr int s = S;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= f.FW || j >= f.FH) return;
     const double sc = stats[0] * c0;
-    const long FV = (long)f.FW * f.FH * f.FD;
-    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < FV; v += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(v % f.FW), j = (int)((v / f.FW) % f.FH), k = (int)(v / ((long)f.FW * f.FH));
-        int x0, x1, y0, y1, z0, z1;
-        double tx, ty, tz;
-        prolong_axis(i, s, f.W, x0, x1, tx);
-        prolong_axis(j, s, f.H, y0, y1, ty);
-        prolong_axis(k, s, f.D, z0, z1, tz);
-        auto at = [&](int x, int y, int z) { return (double)out[((long)z * f.H + y) * f.W + x]; };
-        const double c00 = at(x0, y0, z0) * (1.0 - tx) + at(x1, y0, z0) * tx;
-        const double c10 = at(x0, y1, z0) * (1.0 - tx) + at(x1, y1, z0) * tx;
-        const double c01 = at(x0, y0, z1) * (1.0 - tx) + at(x1, y0, z1) * tx;
-        const double c11 = at(x0, y1, z1) * (1.0 - tx) + at(x1, y1, z1) * tx;
-        const double val = (c00 * (1.0 - ty) + c10 * ty) * (1.0 - tz) + (c01 * (1.0 - ty) + c11 * ty) * tz;
-        phi[v] = (phi0 ? phi0[v] : 0.0) + val * sc;
-    }
+    int x0, x1, y0, y1, z0, z1;
+    double tx, ty, tz;
+    prolong_axis(i, s, f.W, x0, x1, tx);
+    prolong_axis(j, s, f.H, y0, y1, ty);
+    prolong_axis(k, s, f.D, z0, z1, tz);
+    auto at = [&](int x, int y, int z) { return (double)out[(z * f.H + y) * f.W + x]; };
+    const double c00 = at(x0, y0, z0) * (1.0 - tx) + at(x1, y0, z0) * tx;
+    const double c10 = at(x0, y1, z0) * (1.0 - tx) + at(x1, y1, z0) * tx;
+    const double c01 = at(x0, y0, z1) * (1.0 - tx) + at(x1, y0, z1) * tx;
+    const double c11 = at(x0, y1, z1) * (1.0 - tx) + at(x1, y1, z1) * tx;
+    const double val = (c00 * (1.0 - ty) + c10 * ty) * (1.0 - tz) + (c01 * (1.0 - ty) + c11 * ty) * tz;
+    const long v = ((long)k * f.FH + j) * f.FW + i;
+    phi[v] = (phi0 ? phi0[v] : 0.0) + val * sc;
 }
 
 // training target at the network grid: the mean over each voxel's s^3 cells of
@@ -1007,7 +1025,7 @@ static hipError_t rms_into(mdx_net *n, const double *a, const double *b, double 
 static void launch_prolong(mdx_net *n, const float *coarse, const double *stats, double c0, const double *phi0,
                            double *phi) {
     const FineGrid f = fine_grid(n);
-    const dim3 grd(nblk(n->FV)), blk(256);
+    const dim3 grd((f.FW + 63) / 64, (f.FH + 3) / 4, f.FD), blk(64, 4);
     if (f.s == 4) hipLaunchKernelGGL(prolong_guess_kernel<4>, grd, blk, 0, n->stream, f, coarse, stats, c0, phi0, phi);
     else hipLaunchKernelGGL(prolong_guess_kernel<2>, grd, blk, 0, n->stream, f, coarse, stats, c0, phi0, phi);   // stride 2
 }
@@ -1028,11 +1046,11 @@ hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *
     if (n->stride == 1)
         hipLaunchKernelGGL(make_input_kernel, dim3(nblk(V)), dim3(256), 0, n->stream, V, r0, phi0, n->stats, n->x_in);
     else if (n->stride == 4)
-        hipLaunchKernelGGL(restrict_input_kernel<4>, dim3(nblk(V)), dim3(256), 0, n->stream, fine_grid(n), r0, phi0,
-                           n->stats, n->x_in);
+        hipLaunchKernelGGL(restrict_input_kernel<4>, dim3((n->W[0] + 63) / 64, n->H[0], n->D[0]), dim3(64, 16), 0,
+                           n->stream, fine_grid(n), r0, phi0, n->stats, n->x_in);
     else   // stride 2 (the strides are 1, 2 and 4)
-        hipLaunchKernelGGL(restrict_input_kernel<2>, dim3(nblk(V)), dim3(256), 0, n->stream, fine_grid(n), r0, phi0,
-                           n->stats, n->x_in);
+        hipLaunchKernelGGL(restrict_input_kernel<2>, dim3((n->W[0] + 63) / 64, n->H[0], n->D[0]), dim3(64, 4), 0,
+                           n->stream, fine_grid(n), r0, phi0, n->stats, n->x_in);
     return hipGetLastError();
 }
 
