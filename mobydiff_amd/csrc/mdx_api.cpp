@@ -314,6 +314,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     h->ni = (size_t)nx * ny * nz;
     h->stretched = cfg->stretch_beta > 0.0;
     if (const char *ng = std::getenv("MOBYDIFF_NO_GRAPHS")) h->use_graphs = !(ng[0] == '1');
+    if (const char *fi = std::getenv("MOBYDIFF_FUSED_INPUT")) h->fused_input = !(fi[0] == '0');
     h->inner_omega = h->cfg.omega;
     if (const char *io = std::getenv("MOBYDIFF_INNER_OMEGA")) {
         const double v = std::atof(io);
@@ -1361,14 +1362,30 @@ static int initial_guess(mdx_handle *h, int s, double dtg, const GuessPlan &gp, 
 // select_start_kernel, so the only host round trip is the wait for that first check, which the
 // solve needs anyway (no residual pass of its own, no copy of the start).  *first_seq: the
 // sequence number of the first check (slot 0); resolve_learned_start then reads which start won.
-static int learned_start_mixed(mdx_handle *h, int s, const GuessPlan &gp, unsigned long long *first_seq) {
+// fused: the right-hand side is formed here too (learned_input_kernel, with the network's input at
+// its stride); otherwise sor_prepare has formed it.
+static int learned_start_mixed(mdx_handle *h, int s, const GuessPlan &gp, bool fused, unsigned long long *first_seq) {
     mdx_net *n = h->net;
     const double *prev = h->phi_hist + (size_t)s * h->ni;   // a learned start has a history
     ev_record(h, 8);
-    // Phi_0 in phi2, its residual (the network's input field) in res64, max-norm to the RP slot,
-    // and the network's input statistics, in one pass
-    MDX_HIP(launch_phi0_residual(h, prev, gp.fac, h->phi2, h->res64, h->d_pinned + RP_PIN, n->stats));
-    {
+    if (fused) {
+        // rhs, Phi_0 and r0 (kept in phi2 / res64 only for a fallback to Phi_0), the voxel sums
+        // and the partials in one pass; then the input, its statistics and the RZ / RP maxima in
+        // one small kernel, the network, and Phi = fac * prev + guess
+        unsigned nb = 0;
+        MDX_HIP(launch_learned_input(h, prev, gp.fac, gp.use_prev ? h->phi2 : nullptr,
+                                     gp.use_prev ? h->res64 : nullptr, n->vsum, n->stride, &nb));
+        hipStream_t saved = n->stream;
+        n->stream = h->stream;
+        hipError_t e = unet::net_input_from_sums(n, h->max_part, nb, h->d_pinned + RZ_PIN, h->d_pinned + RP_PIN,
+                                                 h->red_max + 1);
+        if (e == hipSuccess) e = unet::net_guess_to_phi_scaled(n, prev, gp.fac, h->phi);
+        n->stream = saved;
+        MDX_HIP(e);
+    } else {
+        // Phi_0 in phi2, its residual (the network's input field) in res64, max-norm to the RP
+        // slot, and the network's input statistics, in one pass
+        MDX_HIP(launch_phi0_residual(h, prev, gp.fac, h->phi2, h->res64, h->d_pinned + RP_PIN, n->stats));
         hipStream_t saved = n->stream;
         n->stream = h->stream;
         hipError_t e = unet::net_input_from_solver(n, h, h->res64, h->phi2, true);
@@ -1504,9 +1521,11 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
             // the learned start falling back to Phi = 0 compares against that start's residual,
             // max |rhs|, formed here on the way
             const bool want_rz = gp.learned && gp.have_prev && !gp.use_prev;
-            MDX_HIP(launch_sor_prepare(h, want_rz ? h->d_pinned + RZ_PIN : nullptr, 0));
+            // the fused input pass also forms rhs and max |rhs|
+            const bool fused = dev_select && h->fused_input && learned_input_fits(h, h->net->stride);
+            if (!fused) MDX_HIP(launch_sor_prepare(h, want_rz ? h->d_pinned + RZ_PIN : nullptr, 0));
             if (dev_select) {
-                rc = learned_start_mixed(h, stage, gp, &first_seq);
+                rc = learned_start_mixed(h, stage, gp, fused, &first_seq);
                 double r0 = 0.0;
                 if (rc == MDX_OK) rc = wait_check(h, CHECK_PIN, first_seq, &r0);
                 if (rc == MDX_OK) rc = resolve_learned_start(h, stage, gp, &st, &r_first);

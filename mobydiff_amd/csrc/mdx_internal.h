@@ -203,6 +203,9 @@ struct mdx_handle {
     // relaxation factor of the FP32 inner sweeps of the mixed solve (the configured omega unless
     // MOBYDIFF_INNER_OMEGA sets another); the FP64 sweeps always use the configured one
     double inner_omega = 1.0;
+    // the learned start's input pass fused with the right-hand side (MOBYDIFF_FUSED_INPUT=0: the
+    // separate right-hand side, residual and restriction kernels)
+    bool fused_input = true;
     // z-planes each workgroup of the tiled momentum kernel marches (MOBYDIFF_MOM_KZ)
     int mom_kz = 8;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
@@ -308,6 +311,12 @@ hipError_t launch_scale_copy64(mdx_handle *h, double *dst, const double *src, do
 hipError_t launch_residual_field64(mdx_handle *h, const double *phi, double *field, double *host_slot = nullptr);
 hipError_t launch_phi0_residual(mdx_handle *h, const double *prev, double fac, double *phi0, double *res,
                                 double *host_slot, double *stat);
+// the learned start's input pass fused with the right-hand side (learned_input_kernel): rhs, r0 and
+// Phi_0 (into phi0 / res when given), voxel sums at the network's stride into vsum, and
+// 4 * nparts workgroup partials in max_part
+bool learned_input_fits(const mdx_handle *h, int stride);
+hipError_t launch_learned_input(mdx_handle *h, const double *prev, double fac, double *phi0, double *res,
+                                double *vsum, int stride, unsigned *nparts);
 hipError_t launch_select_start(mdx_handle *h, double margin, const double *rg_slot, const double *other_slot,
                                const double *phi0, const double *res0, double *host_slot, unsigned long long seq);
 bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);

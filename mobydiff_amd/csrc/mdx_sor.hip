@@ -838,11 +838,18 @@ hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega) {
     return hipGetLastError();
 }
 
-// residuals r[u] = rhs - L_mu(Phi) of cells (i, j, k0 + u), u < n = min(RZ, nz - k0 + 1), and
-// Phi of those cells in p[u]
+// the right-hand side of this column's cells (0 past the last plane)
+__device__ inline void rhs_column(const Dims &d, const double *__restrict__ rhs, int i, int j, int k0,
+                                  double (&rv)[RZ]) {
+#pragma unroll
+    for (int u = 0; u < RZ; ++u) rv[u] = k0 + u <= d.nz ? rhs[d.in(i, j, k0 + u)] : 0.0;
+}
+
+// residuals r[u] = rv[u] - L_mu(Phi) of cells (i, j, k0 + u), u < n = min(RZ, nz - k0 + 1)
+// (rv: the right-hand side of those cells), and Phi of those cells in p[u]
 template <bool STRETCH, bool IBM, typename PhiAt>
 __device__ inline int residual_column(const GridK &g, const MuK<double> &mu, const SorConst<double> &s, PhiAt P,
-                                      const double *__restrict__ rhs, int i, int j, int k0, double (&r)[RZ],
+                                      const double (&rv)[RZ], int i, int j, int k0, double (&r)[RZ],
                                       double (&p)[RZ]) {
     const Dims d = g.d;
     const int nx = d.nx, ny = d.ny, nz = d.nz;
@@ -894,7 +901,7 @@ __device__ inline int residual_column(const GridK &g, const MuK<double> &mu, con
             double denom;
             r[u] = sor_residual_core<double, STRETCH>(pz[u + 1], P(ip, j, k), P(im, j, k), P(i, jn, k), P(i, js, k),
                                                      pz[u + 2], pz[u], mui, muip, mvj, mvjp, mwk, mwkp, wn_, ws_,
-                                                     rhs[d.in(i, j, k)], s, an, as, denom);
+                                                     rv[u], s, an, as, denom);
         }
     }
     return n;
@@ -910,7 +917,9 @@ __global__ __launch_bounds__(256) void sor_residual64_kernel(GridK g, MuK<double
     double m = 0.0;
     if (rz_column(d, i, j, k0)) {   // padding threads report 0
         double r[RZ], p[RZ];
-        const int n = residual_column<STRETCH, IBM>(g, mu, s, PlainPhi<double>{phi, d}, rhs, i, j, k0, r, p);
+        double rv[RZ];
+        rhs_column(d, rhs, i, j, k0, rv);
+        const int n = residual_column<STRETCH, IBM>(g, mu, s, PlainPhi<double>{phi, d}, rv, i, j, k0, r, p);
 #pragma unroll
         for (int u = 0; u < RZ; ++u)
             if (u < n) {
@@ -965,7 +974,9 @@ __global__ __launch_bounds__(256) void phi0_residual_kernel(GridK g, MuK<double>
     double m = 0.0, sr = 0.0, sp = 0.0;
     if (rz_column(d, i, j, k0)) {   // padding threads report 0
         double r[RZ], p[RZ];
-        const int n = residual_column<STRETCH, IBM>(g, mu, s, ScaledPhi{prev, fac, d}, rhs, i, j, k0, r, p);
+        double rv[RZ];
+        rhs_column(d, rhs, i, j, k0, rv);
+        const int n = residual_column<STRETCH, IBM>(g, mu, s, ScaledPhi{prev, fac, d}, rv, i, j, k0, r, p);
 #pragma unroll
         for (int u = 0; u < RZ; ++u)
             if (u < n) {
@@ -1060,6 +1071,157 @@ hipError_t launch_phi0_residual(mdx_handle *h, const double *prev, double fac, d
     return e != hipSuccess ? e : launch_max_final(h, grd.x, host_slot, 0);
 }
 
+// The learned start's whole input pass in one sweep, for a network of stride S (2 or 4): rhs =
+// div u* (sor_prepare's arithmetic, wall values of v), Phi_0 = fac * prev, r0 = rhs - L_mu Phi_0
+// (cell_residual's operands), and the network's input before its scaling: the sums of r0 and of
+// Phi_0 over each voxel's S^3 cells into vsum[2 v] and vsum[2 v + 1] (voxel v = vx + W (vy + H vz)).
+// Per workgroup: part[b] = max |rhs|, part[nb + b] = max |r0|, part[2 nb + b] = sum of r0^2,
+// part[3 nb + b] = sum of Phi_0^2.  phi0 and res, when given, receive Phi_0 and r0 (the start that
+// falls back to Phi_0 copies them).  A workgroup's 64 x 4 x RZ cells are whole voxels (S divides
+// 4 and the grid's extents); a voxel's sum adds its cells plane by plane in a column, then the S
+// columns of a row (lane pairs), then the S rows, in a fixed order.
+template <bool STRETCH, bool IBM, int S>
+__global__ __launch_bounds__(256) void learned_input_kernel(GridK g, MuK<double> mu, SorConst<double> s,
+                                                            double vbot, double vtop, const double *__restrict__ us,
+                                                            const double *__restrict__ vs,
+                                                            const double *__restrict__ ws, double idx, double idy,
+                                                            double idz, const double *__restrict__ prev, double fac,
+                                                            double *__restrict__ rhs, double *__restrict__ phi0,
+                                                            double *__restrict__ res, double *__restrict__ vsum,
+                                                            int W, int H, double *part) {
+    static_assert(S == 2 || S == 4, "voxels of 2 or 4 cells");
+    constexpr int ZG = RZ / S;   // voxel planes per column
+    __shared__ double red[4][4];
+    __shared__ double vr[4][64], vp[4][64];
+    const Dims d = g.d;
+    int i = 0, j = 0, k0 = 0;
+    const bool ok = rz_column(d, i, j, k0);
+    double r[RZ] = {}, p[RZ] = {};
+    int n = 0;
+    double mrhs = 0.0, mr = 0.0, sr = 0.0, sp = 0.0;
+    if (ok) {
+        const bool per2 = g.per[1];
+        const int ip = i == d.nx ? 1 : i + 1;
+        const int jp = (per2 && j == d.ny) ? 1 : j + 1;
+        double rv[RZ];
+#pragma unroll
+        for (int u = 0; u < RZ; ++u) {
+            rv[u] = 0.0;
+            const int k = k0 + u;
+            if (k <= d.nz) {
+                const int kp = k == d.nz ? 1 : k + 1;
+                const size_t c = d.g(i, j, k);
+                const double v0 = (!per2 && j == 1) ? vbot : vs[c];
+                const double v1 = (!per2 && j == d.ny) ? vtop : vs[d.g(i, jp, k)];
+                const double dvy = STRETCH ? (v1 - v0) * g.ihc[j] : (v1 - v0) * idy;
+                rv[u] = (us[d.g(ip, j, k)] - us[c]) * idx + dvy + (ws[d.g(i, j, kp)] - ws[c]) * idz;
+            }
+        }
+        n = residual_column<STRETCH, IBM>(g, mu, s, ScaledPhi{prev, fac, d}, rv, i, j, k0, r, p);
+#pragma unroll
+        for (int u = 0; u < RZ; ++u)
+            if (u >= n) p[u] = 0.0;   // past the last plane (residual_column leaves the wrapped neighbour)
+#pragma unroll
+        for (int u = 0; u < RZ; ++u)
+            if (u < n) {
+                const size_t q = d.in(i, j, k0 + u);
+                rhs[q] = rv[u];
+                if (phi0) phi0[q] = p[u];
+                if (res) res[q] = r[u];
+                mrhs = nanmax_any(mrhs, fabs(rv[u]));
+                mr = nanmax_any(mr, fabs(r[u]));
+                sr += r[u] * r[u];
+                sp += p[u] * p[u];
+            }
+    }
+    // ---- voxel sums (threads past the grid hold zeros)
+    const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+    for (int gz = 0; gz < ZG; ++gz) {
+        double cr = 0.0, cp = 0.0;
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            cr += r[gz * S + u];
+            cp += p[gz * S + u];
+        }
+#pragma unroll
+        for (int off = 1; off < S; off <<= 1) {
+            cr += __shfl_xor(cr, off, 64);
+            cp += __shfl_xor(cp, off, 64);
+        }
+        vr[ty][tx] = cr;
+        vp[ty][tx] = cp;
+        __syncthreads();
+        if (ok && ty % S == 0 && tx % S == 0) {
+            double a = 0.0, b = 0.0;
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                a += vr[ty + q][tx];
+                b += vp[ty + q][tx];
+            }
+            const int vx = (i - 1) / S, vy = (j - 1) / S, vz = (k0 - 1) / S + gz;
+            if (k0 + gz * S <= d.nz) {
+                const size_t v = (size_t)vx + (size_t)W * ((size_t)vy + (size_t)H * vz);
+                vsum[2 * v] = a;
+                vsum[2 * v + 1] = b;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- workgroup partials
+    for (int off = 32; off > 0; off >>= 1) {
+        mrhs = nanmax_any(mrhs, __shfl_xor(mrhs, off, 64));
+        mr = nanmax_any(mr, __shfl_xor(mr, off, 64));
+        sr += __shfl_xor(sr, off, 64);
+        sp += __shfl_xor(sp, off, 64);
+    }
+    const int t = tx + 64 * ty;
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = mrhs;
+        red[1][t >> 6] = mr;
+        red[2][t >> 6] = sr;
+        red[3][t >> 6] = sp;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const unsigned nb = gridDim.x, b = blockIdx.x;
+        part[b] = nanmax_any(nanmax_any(red[0][0], red[0][1]), nanmax_any(red[0][2], red[0][3]));
+        part[nb + b] = nanmax_any(nanmax_any(red[1][0], red[1][1]), nanmax_any(red[1][2], red[1][3]));
+        part[2 * nb + b] = (red[2][0] + red[2][1]) + (red[2][2] + red[2][3]);
+        part[3 * nb + b] = (red[3][0] + red[3][1]) + (red[3][2] + red[3][3]);
+    }
+}
+
+// whether learned_input_kernel serves this grid at stride s: voxels of whole workgroup tiles
+bool learned_input_fits(const mdx_handle *h, int stride) {
+    const Dims &d = h->d;
+    return (stride == 2 || stride == 4) && RZ % stride == 0 && 4 % stride == 0 && d.nx % stride == 0 &&
+           d.ny % stride == 0 && d.nz % stride == 0 && 4 * (size_t)rz_grid(d) <= MAX_PARTS;
+}
+
+hipError_t launch_learned_input(mdx_handle *h, const double *prev, double fac, double *phi0, double *res,
+                                double *vsum, int stride, unsigned *nparts) {
+    if (!learned_input_fits(h, stride)) return hipErrorInvalidValue;
+    GridK g = grid_k(h);
+    MuK<double> mu = mu_k<double>(h, h->mu64);
+    SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
+    const unsigned nb = rz_grid(h->d);
+    *nparts = nb;
+    dim3 blk(64, 4), grd(nb);
+    const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
+    const int W = h->d.nx / stride, H = h->d.ny / stride;
+#define MDX_LI(S, I, Q)                                                                                          \
+    hipLaunchKernelGGL((learned_input_kernel<S, I, Q>), grd, blk, 0, h->stream, g, mu, s, vb, vt, h->us, h->vs, \
+                       h->ws, 1.0 / h->dx, 1.0 / h->dy, 1.0 / h->dz, prev, fac, h->rhs, phi0, res, vsum, W, H,   \
+                       h->max_part)
+#define MDX_LIQ(S, I) do { if (stride == 4) MDX_LI(S, I, 4); else MDX_LI(S, I, 2); } while (0)
+    if (h->stretched) { if (h->ibm_on) MDX_LIQ(true, true); else MDX_LIQ(true, false); }
+    else { if (h->ibm_on) MDX_LIQ(false, true); else MDX_LIQ(false, false); }
+#undef MDX_LIQ
+#undef MDX_LI
+    return hipGetLastError();
+}
+
 // The start of a mixed solve whose first check ran on the learned start (phi, its defect r32 and
 // maximum rg = rg_slot[0]): kept when rg < margin * other (other_slot[0], the fallback start's
 // residual maximum; a NaN keeps nothing), else replaced by the fallback: Phi = 0 with the defect
@@ -1122,7 +1284,9 @@ __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, 
     double m = 0.0;
     if (rz_column(d, i, j, k0)) {   // padding threads report 0
         double r[RZ], p[RZ];
-        const int n = residual_column<STRETCH, IBM>(g, mu, s, SumPhi<E>{phi_in, e, d}, rhs, i, j, k0, r, p);
+        double rv[RZ];
+        rhs_column(d, rhs, i, j, k0, rv);
+        const int n = residual_column<STRETCH, IBM>(g, mu, s, SumPhi<E>{phi_in, e, d}, rv, i, j, k0, r, p);
 #pragma unroll
         for (int u = 0; u < RZ; ++u)
             if (u < n) {

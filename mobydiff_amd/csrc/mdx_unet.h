@@ -26,6 +26,7 @@ struct mdx_net {
     bf16 *g[10] = {}, *gp1 = nullptr, *gp2 = nullptr, *gcat1 = nullptr, *gcat2 = nullptr;
     float *out = nullptr, *dout = nullptr, *target = nullptr, *loss = nullptr, *x2_stage = nullptr;
     double *stats = nullptr, *part = nullptr, *h_pinned = nullptr;
+    double *vsum = nullptr;        // per voxel: the sums of r0 and of Phi_0 over its cells (fused input pass)
     std::vector<void *> allocs;
     double c0 = 0.0;               // target scale: Phi = out * rms(div u*) * c0
     int adam_t = 0;
@@ -68,6 +69,14 @@ hipError_t net_adam(mdx_net *n, float lr);
 hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0,
                                  bool stats_ready = false);
 hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi);
+// The input from the fused solver pass (launch_learned_input): stats[0..1] = rms(r0), rms(Phi_0)
+// from the nb workgroup partials in part (sums of r0^2 at 2 nb, of Phi_0^2 at 3 nb), the network
+// input from n->vsum, and the maxima |rhs| (part[0, nb)) and |r0| (part[nb, 2 nb)) into the pinned
+// slots rz_slot and rp_slot (value, the error word err[0], then 0 as the slot's sequence number).
+hipError_t net_input_from_sums(mdx_net *n, const double *part, unsigned nb, double *rz_slot, double *rp_slot,
+                               const unsigned long long *err);
+// the guess Phi = fac * prev + out * s_r * c0 (Phi_0 formed from the history, as the input pass does)
+hipError_t net_guess_to_phi_scaled(mdx_net *n, const double *prev, double fac, double *phi);
 hipError_t net_capture(mdx_net *n, const double *phi, const double *phi_prev, double fac);
 hipError_t net_load_replay(mdx_net *n, int slot);
 hipError_t net_restrict(mdx_net *n, const double *fine, float *coarse);

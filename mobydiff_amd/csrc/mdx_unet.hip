@@ -720,6 +720,95 @@ __global__ __launch_bounds__(64 * S * S) void restrict_input_kernel(FineGrid f, 
     *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = hi;
 }
 
+// The network input from the fused solver pass: every workgroup folds the nb partial sums of r0^2
+// and Phi_0^2 in one fixed order (so every workgroup holds the same statistics), then scales the
+// voxel sums of its voxels: x = (bf16)(float)(sum * w / s), w = 1 / s^3, s the rms of the field.
+// Workgroup 0 also stores stats[0..1] and the two maxima into their pinned slots.
+__global__ __launch_bounds__(1024) void input_from_sums_kernel(long V, const double *__restrict__ vsum,
+                                                              const double *__restrict__ part, unsigned nb,
+                                                              double n_cells, double w, double *stats,
+                                                              bf16 *__restrict__ x, double *rz_slot,
+                                                              double *rp_slot, const unsigned long long *err) {
+    __shared__ double red[4][16];
+    const int t = threadIdx.x;
+    const bool first = blockIdx.x == 0;
+    constexpr unsigned B = 8;
+    double a = 0.0, b = 0.0, m0 = 0.0, m1 = 0.0;
+    for (unsigned q0 = (unsigned)t; q0 < nb; q0 += 1024 * B) {
+        double va[B], vb[B];
+#pragma unroll
+        for (unsigned u = 0; u < B; ++u) {
+            const unsigned q = q0 + 1024 * u;
+            va[u] = q < nb ? part[2 * nb + q] : 0.0;
+            vb[u] = q < nb ? part[3 * nb + q] : 0.0;
+        }
+#pragma unroll
+        for (unsigned u = 0; u < B; ++u) {
+            a += va[u];
+            b += vb[u];
+        }
+        if (first) {
+            for (unsigned u = 0; u < B; ++u) {
+                const unsigned q = q0 + 1024 * u;
+                if (q < nb) {
+                    m0 = nanmax_any(m0, part[q]);
+                    m1 = nanmax_any(m1, part[nb + q]);
+                }
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        m0 = nanmax_any(m0, __shfl_xor(m0, off, 64));
+        m1 = nanmax_any(m1, __shfl_xor(m1, off, 64));
+    }
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = a;
+        red[1][t >> 6] = b;
+        red[2][t >> 6] = m0;
+        red[3][t >> 6] = m1;
+    }
+    __syncthreads();
+    a = 0.0;
+    b = 0.0;
+    for (int w = 0; w < 16; ++w) {
+        a += red[0][w];
+        b += red[1][w];
+    }
+    const double ra = sqrt(a / n_cells), rb = sqrt(b / n_cells);
+    const double sr = ra > 1e-300 ? ra : 1.0, sp = rb > 1e-300 ? rb : 1.0;
+    const double ir = 1.0 / sr, ip = 1.0 / sp;
+    for (long v = (long)blockIdx.x * 1024 + t; v < V; v += (long)gridDim.x * 1024) {
+        bf16x8 lo = {}, hi = {};
+        lo[0] = (bf16)(float)(vsum[2 * v] * w * ir);
+        lo[1] = (bf16)(float)(vsum[2 * v + 1] * w * ip);
+        *reinterpret_cast<bf16x8 *>(x + v * 16) = lo;
+        *reinterpret_cast<bf16x8 *>(x + v * 16 + 8) = hi;
+    }
+    if (first && t == 0) {
+        stats[0] = sr;
+        stats[1] = sp;
+        double mz = red[2][0], mp = red[3][0];
+        for (int w = 1; w < 16; ++w) {
+            mz = nanmax_any(mz, red[2][w]);
+            mp = nanmax_any(mp, red[3][w]);
+        }
+        const double ew = __longlong_as_double((long long)err[0]);
+        if (rz_slot) {
+            rz_slot[0] = mz;
+            rz_slot[1] = ew;
+        }
+        if (rp_slot) {
+            rp_slot[0] = mp;
+            rp_slot[1] = ew;
+        }
+        __threadfence_system();
+        if (rz_slot) reinterpret_cast<volatile unsigned long long *>(rz_slot)[2] = 0ull;
+        if (rp_slot) reinterpret_cast<volatile unsigned long long *>(rp_slot)[2] = 0ull;
+    }
+}
+
 // coarse cell-centred coordinate of fine cell i: (i + 1/2)/s - 1/2 = i0 + t, i0 clamped to the grid
 // (the outermost fine cells take the nearest voxel's value)
 __device__ __forceinline__ void prolong_axis(int i, int s, int n, int &i0, int &i1, double &t) {
@@ -732,13 +821,15 @@ __device__ __forceinline__ void prolong_axis(int i, int s, int n, int &i0, int &
     if (i1 > n - 1) { i1 = n - 1; }
 }
 
-// Phi guess on the solver grid = Phi_0 + (trilinear prolongation of out) * s_r * c0.  S = s (2 or
+// Phi guess on the solver grid = Phi_0 + (trilinear prolongation of out) * s_r * c0, Phi_0 read
+// from phi0 or formed as fac * prev.  S = s (2 or
 // 4) as a constant: the division by s is then by a power-of-two constant, the same value.  One
 // thread per fine cell, on a grid of ceil(FW/64) x ceil(FH/4) x FD workgroups of 64 x 4.
 template <int S>
 __global__ __launch_bounds__(256) void prolong_guess_kernel(FineGrid f, const float *__restrict__ out,
                                                             const double *__restrict__ stats, double c0,
-                                                            const double *phi0, double *phi) {
+                                                            const double *phi0, const double *prev, double fac,
+                                                            double *phi) {
     constexpr int s = S;
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= f.FW || j >= f.FH) return;
@@ -755,7 +846,8 @@ __global__ __launch_bounds__(256) void prolong_guess_kernel(FineGrid f, const fl
     const double c11 = at(x0, y1, z1) * (1.0 - tx) + at(x1, y1, z1) * tx;
     const double val = (c00 * (1.0 - ty) + c10 * ty) * (1.0 - tz) + (c01 * (1.0 - ty) + c11 * ty) * tz;
     const long v = ((long)k * f.FH + j) * f.FW + i;
-    phi[v] = (phi0 ? phi0[v] : 0.0) + val * sc;
+    const double base = phi0 ? phi0[v] : (prev ? fac * prev[v] : 0.0);   // Phi_0, or formed as the input pass does
+    phi[v] = base + val * sc;
 }
 
 // training target at the network grid: the mean over each voxel's s^3 cells of
@@ -1023,11 +1115,13 @@ static hipError_t rms_into(mdx_net *n, const double *a, const double *b, double 
 // stats_ready: stats[0..1] already hold rms(r0) and rms(Phi_0) (the solver's input pass)
 // the prolongation at the network's stride (a compile-time constant for 2 and 4)
 static void launch_prolong(mdx_net *n, const float *coarse, const double *stats, double c0, const double *phi0,
-                           double *phi) {
+                           double *phi, const double *prev = nullptr, double fac = 0.0) {
     const FineGrid f = fine_grid(n);
     const dim3 grd((f.FW + 63) / 64, (f.FH + 3) / 4, f.FD), blk(64, 4);
-    if (f.s == 4) hipLaunchKernelGGL(prolong_guess_kernel<4>, grd, blk, 0, n->stream, f, coarse, stats, c0, phi0, phi);
-    else hipLaunchKernelGGL(prolong_guess_kernel<2>, grd, blk, 0, n->stream, f, coarse, stats, c0, phi0, phi);   // stride 2
+    if (f.s == 4)
+        hipLaunchKernelGGL(prolong_guess_kernel<4>, grd, blk, 0, n->stream, f, coarse, stats, c0, phi0, prev, fac, phi);
+    else   // stride 2
+        hipLaunchKernelGGL(prolong_guess_kernel<2>, grd, blk, 0, n->stream, f, coarse, stats, c0, phi0, prev, fac, phi);
 }
 
 hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *r0, const double *phi0,
@@ -1062,6 +1156,25 @@ hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi) {
                            phi0, phi, n->V[0]);
     else
         launch_prolong(n, n->out, n->stats, n->c0, phi0, phi);
+    return hipGetLastError();
+}
+
+hipError_t net_guess_to_phi_scaled(mdx_net *n, const double *prev, double fac, double *phi) {
+    hipError_t e;
+    if (n->stride == 1) return hipErrorInvalidValue;
+    TRY(net_forward(n));
+    launch_prolong(n, n->out, n->stats, n->c0, nullptr, phi, prev, fac);
+    return hipGetLastError();
+}
+
+hipError_t net_input_from_sums(mdx_net *n, const double *part, unsigned nb, double *rz_slot, double *rp_slot,
+                               const unsigned long long *err) {
+    if (n->stride != 2 && n->stride != 4) return hipErrorInvalidValue;
+    const long V = n->V[0];
+    const unsigned blocks = (unsigned)((V + 1023) / 1024);
+    hipLaunchKernelGGL(input_from_sums_kernel, dim3(blocks), dim3(1024), 0, n->stream, V, n->vsum, part, nb,
+                       (double)n->FV, 1.0 / (double)(n->stride * n->stride * n->stride), n->stats, n->x_in,
+                       rz_slot, rp_slot, err);
     return hipGetLastError();
 }
 

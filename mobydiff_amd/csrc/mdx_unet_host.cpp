@@ -111,7 +111,8 @@ extern "C" int mdx_net_create_strided(int32_t nx, int32_t ny, int32_t nz, int32_
     if ((e = nalloc(n, &n->out, V0)) != hipSuccess || (e = nalloc(n, &n->dout, V0)) != hipSuccess ||
         (e = nalloc(n, &n->x2_stage, 2 * V0)) != hipSuccess ||
         (e = nalloc(n, &n->target, V0)) != hipSuccess || (e = nalloc(n, &n->loss, 4)) != hipSuccess ||
-        (e = nalloc(n, &n->stats, 8)) != hipSuccess || (e = nalloc(n, &n->part, 3 * 512)) != hipSuccess)
+        (e = nalloc(n, &n->stats, 8)) != hipSuccess || (e = nalloc(n, &n->part, 3 * 512)) != hipSuccess ||
+        (e = nalloc(n, &n->vsum, 2 * V0)) != hipSuccess)
         return fail(e);
     if ((e = hipHostMalloc((void **)&n->h_pinned, 64 * sizeof(double))) != hipSuccess) return fail(e);
     // He-normal initialisation (fan-in), zero biases, deterministic in the seed
