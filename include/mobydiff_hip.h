@@ -194,6 +194,11 @@ int mdx_init_fields(mdx_handle *h, uint32_t seed, double amp, const mdx_geometry
 int mdx_set_guess(mdx_handle *h, int32_t guess);
 /* one RK3 step (main.f90:66-89): 3 x {momentum, [flow rate], [apply_bc], projection, bc} */
 int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats);
+/* whether mdx_step reports the bulk velocity in stats->ub (flowrate on; the default 1).  The report
+ * waits for the step's last kernel before mdx_step returns; with 0, stats->ub is NaN and mdx_step
+ * returns once the step is queued, so the next step's kernels queue behind it without a gap.  No
+ * counterpart in the reference, whose time loop computes no bulk velocity (main.f90:66-104). */
+int mdx_set_step_report(mdx_handle *h, int32_t bulk_velocity);
 
 /* one RK3 step of n independent channels, driven concurrently (one host thread and stream per
  * handle); stats may be NULL or hold n entries.  No counterpart in the reference. */

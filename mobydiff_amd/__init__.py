@@ -329,6 +329,12 @@ class Channel:
         _check(lib().mdx_step(self.h, C.c_double(dt), C.byref(st)))
         return st
 
+    def set_step_report(self, bulk_velocity):
+        """Whether step() reports the bulk velocity in StepStats.ub (flow-rate control on; the
+        default).  The report waits for the step's last kernel; without it, ub is NaN and step()
+        returns once the step is queued (mdx_set_step_report)."""
+        _check(lib().mdx_set_step_report(self.h, C.c_int32(1 if bulk_velocity else 0)))
+
     def synchronize(self):
         _check(lib().mdx_synchronize(self.h))
 

@@ -1394,10 +1394,11 @@ static int learned_start_mixed(mdx_handle *h, int s, const GuessPlan &gp, bool f
         MDX_HIP(e);
     }
     ev_record(h, 9);
-    MDX_HIP(launch_defect64(h, h->phi, nullptr, h->phi, h->r32, h->d_pinned + SEL_PIN, 0));
+    unsigned nb = 0;   // the check's workgroup maxima, folded by the select kernel
+    MDX_HIP(launch_defect64(h, h->phi, nullptr, h->phi, h->r32, nullptr, 0, &nb));
     const unsigned long long seq = ++h->chk_seq;
     const bool prev_fb = gp.use_prev;
-    MDX_HIP(launch_select_start(h, h->net_margin, h->d_pinned + SEL_PIN, h->d_pinned + (prev_fb ? RP_PIN : RZ_PIN),
+    MDX_HIP(launch_select_start(h, h->net_margin, nb, h->d_pinned + SEL_PIN, h->d_pinned + (prev_fb ? RP_PIN : RZ_PIN),
                                 prev_fb ? h->phi2 : nullptr, h->res64, h->d_pinned + CHECK_PIN, seq));
     *first_seq = seq;
     return MDX_OK;
@@ -1567,13 +1568,12 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         net_outcome(h, stage, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_plan).count(),
                     gp.learned);
         if (h->net && h->net_capture && st.converged && (rc = capture_pair(h, stage, dtg))) return rc;
-        // the converged Phi becomes the history of this stage for the next step
-        MDX_HIP(hipMemcpyAsync(h->phi_hist + (size_t)stage * h->ni, h->phi, h->ni * 8, hipMemcpyDeviceToDevice,
-                               h->stream));
+        // the converged Phi becomes the history of this stage for the next step (copied by the
+        // projection kernel, which reads every Phi anyway)
         h->hist_dtg[stage] = dtg;
         h->proj_count += 1;
         ev_record(h, 2);
-        MDX_HIP(launch_sor_apply(h, h->phi, dtg));
+        MDX_HIP(launch_sor_apply(h, h->phi, dtg, h->phi_hist + (size_t)stage * h->ni));
         ev_record(h, 3);
         // Reaching max_iter is reported through stats.converged (the caller decides); a
         // non-finite residual is an error.
@@ -1606,8 +1606,7 @@ extern "C" int mdx_flowrate_shift(mdx_handle *h, double dtg, double *du) {
     MDX_CHECK(h, MDX_ERR_ARG, "null handle");
     MDX_CHECK(h->cfg.flow_height > 0.0, MDX_ERR_ARG, "flow_height must be positive");
     double *dev = h->red_part + 2048;
-    MDX_HIP(launch_bulk_sum(h, h->us, dtg, dev));
-    MDX_HIP(launch_flow_shift(h, dtg, dev));
+    MDX_HIP(launch_flowrate(h, dtg, dev));
     if (du) {
         MDX_HIP(hipMemcpyAsync(h->h_pinned, dev, 3 * 8, hipMemcpyDeviceToHost, h->stream));
         MDX_HIP(hipStreamSynchronize(h->stream));
@@ -1710,8 +1709,15 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
         }
     }
     if (stats && h->cfg.flowrate) {
-        if ((rc = mdx_bulk_velocity(h, &stats->ub))) return rc;
+        if (!h->step_ub) stats->ub = std::nan("");
+        else if ((rc = mdx_bulk_velocity(h, &stats->ub))) return rc;
     }
+    return MDX_OK;
+}
+
+extern "C" int mdx_set_step_report(mdx_handle *h, int32_t bulk_velocity) {
+    MDX_CHECK(h, MDX_ERR_ARG, "null handle");
+    h->step_ub = bulk_velocity != 0;
     return MDX_OK;
 }
 

@@ -206,6 +206,8 @@ struct mdx_handle {
     // the learned start's input pass fused with the right-hand side (MOBYDIFF_FUSED_INPUT=0: the
     // separate right-hand side, residual and restriction kernels)
     bool fused_input = true;
+    // mdx_step reports the bulk velocity (a wait for the step's end); mdx_set_step_report
+    bool step_ub = true;
     // z-planes each workgroup of the tiled momentum kernel marches (MOBYDIFF_MOM_KZ)
     int mom_kz = 8;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
@@ -280,7 +282,9 @@ hipError_t launch_div_max(mdx_handle *h, const double *u, const double *v, const
 hipError_t launch_cfl_max(mdx_handle *h);
 hipError_t launch_center_vel(mdx_handle *h, double *uc, double *vc, double *wc);
 hipError_t launch_bulk_sum(mdx_handle *h, const double *u, double dtg, double *out_dev);
-hipError_t launch_flow_shift(mdx_handle *h, double dtg, const double *du_dev);
+// constant flow rate: the bulk sums of u*, and the shift u* += du * mu_u (every workgroup of the
+// second launch folds the sums itself); ub, mean mu_u, du to out_dev
+hipError_t launch_flowrate(mdx_handle *h, double dtg, double *out_dev);
 hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int use_geo,
                                 const double *hx_u, double ymid);
 hipError_t launch_reset_max(mdx_handle *h);
@@ -299,9 +303,12 @@ hipError_t launch_sor_start(mdx_handle *h, const double *prev, double fac, doubl
                             unsigned long long seq);
 hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega);
 hipError_t launch_sor_residual64(mdx_handle *h, const double *phi);
-hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg);
+// the projection; with hist != nullptr Phi is also copied there (the stage's history)
+hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg, double *hist = nullptr);
+// parts_only != nullptr: the workgroup maxima stay in max_part (their count to *parts_only) and
+// no fold follows (the caller's next kernel folds them)
 hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32,
-                           double *host_slot = nullptr, unsigned long long seq = 0);
+                           double *host_slot = nullptr, unsigned long long seq = 0, unsigned *parts_only = nullptr);
 hipError_t launch_sor_sweep32(mdx_handle *h, int color, float omega);
 hipError_t launch_sor_iters64(mdx_handle *h, int m, double omega);
 hipError_t launch_sor_iters32(mdx_handle *h, int m, float omega);
@@ -317,7 +324,9 @@ hipError_t launch_phi0_residual(mdx_handle *h, const double *prev, double fac, d
 bool learned_input_fits(const mdx_handle *h, int stride);
 hipError_t launch_learned_input(mdx_handle *h, const double *prev, double fac, double *phi0, double *res,
                                 double *vsum, int stride, unsigned *nparts);
-hipError_t launch_select_start(mdx_handle *h, double margin, const double *rg_slot, const double *other_slot,
+// the learned start's choice (select_start_kernel): rg folded from the nb maxima of its first
+// check in max_part, posted to rg_slot
+hipError_t launch_select_start(mdx_handle *h, double margin, unsigned nb, double *rg_slot, const double *other_slot,
                                const double *phi0, const double *res0, double *host_slot, unsigned long long seq);
 bool rb_plan(const mdx_handle *h, bool fp32, RbPlanOut &out, std::string &why);
 // defect epilogue of an FP32 LDS batch: phi_out = phi_in + e, r32 = defect, max_part = its maxima

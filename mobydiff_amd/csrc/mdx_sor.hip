@@ -348,7 +348,8 @@ static constexpr unsigned PROJ_GHOST_BLOCKS = 256;
 // 64 x 4 tile over RZ planes (the residual kernels' layout): its loads are issued before its
 // stores, Phi at k - 1 is the previous cell's Phi, and a z-invariant coefficient table is read once.
 // The `tiles` workgroups of the tiles are followed by PROJ_GHOST_BLOCKS that fill the ghost shell
-// (proj_ghost: cells no tile writes, read from cells no tile reads).
+// (proj_ghost: cells no tile writes, read from cells no tile reads).  hist, when given, receives
+// a copy of Phi.
 template <bool STRETCH, bool IBM>
 __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu, const double *__restrict__ phi,
                                                         const double *__restrict__ us,
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu,
                                                         double *__restrict__ un, double *__restrict__ vn,
                                                         double *__restrict__ wn, double *__restrict__ pn, double idx,
                                                         double idy, double idz, double idt, unsigned tiles,
-                                                        double vtop) {
+                                                        double vtop, double *__restrict__ hist) {
     const Dims d = g.d;
     if (blockIdx.x >= tiles) {   // the workgroups past the tiles: un, vn, wn on the ghost shell
         proj_ghost(d, g.per[1] ? 0 : 1, vtop, us, vs, ws, un, vn, wn,
@@ -405,6 +406,7 @@ __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu,
                 mu_w = mu.w[mu.at(i, j, k)];
             }
             const double p0 = pz[u + 1];
+            if (hist) hist[d.in(i, j, k)] = p0;
             un[c] = uo[u] + mu_u * (pxm[u] - p0) * idx;
             wn[c] = wo[u] + mu_w * (pz[u] - p0) * idz;
             if (!vwall) {
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu,
     }
 }
 
-hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg) {
+hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg, double *hist) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     const unsigned tiles = rz_grid(h->d);
@@ -427,7 +429,7 @@ hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg) {
     const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
 #define MDX_APPLY(S, I)                                                                                     \
     hipLaunchKernelGGL((sor_apply_kernel<S, I>), grd, blk, 0, h->stream, g, mu, phi, h->us, h->vs, h->ws, vb, \
-                       h->un, h->vn, h->wn, h->pn, a, b, c, t, tiles, vt)
+                       h->un, h->vn, h->wn, h->pn, a, b, c, t, tiles, vt, hist)
     if (h->stretched) { if (h->ibm_on) MDX_APPLY(true, true); else MDX_APPLY(true, false); }
     else { if (h->ibm_on) MDX_APPLY(false, true); else MDX_APPLY(false, false); }
 #undef MDX_APPLY
@@ -1222,21 +1224,42 @@ hipError_t launch_learned_input(mdx_handle *h, const double *prev, double fac, d
     return hipGetLastError();
 }
 
-// The start of a mixed solve whose first check ran on the learned start (phi, its defect r32 and
-// maximum rg = rg_slot[0]): kept when rg < margin * other (other_slot[0], the fallback start's
-// residual maximum; a NaN keeps nothing), else replaced by the fallback: Phi = 0 with the defect
-// rhs, or (PREV) Phi_0 = phi0 with the defect res0, each the value the first check forms for that
-// start (r32 = (float) of the FP64 defect).  Workgroup 0 posts the chosen maximum as the check: to
-// red_max[0], which a batch queued ahead reads, and with the error word and seq to host_slot.
-// The host repeats the comparison on the same pinned values to learn which start ran.
+// The start of a mixed solve whose first check ran on the learned start (phi, its defect r32, and
+// the nb workgroup maxima of that check in part): every workgroup folds the maxima into rg (the
+// same value in each), and the start is kept when rg < margin * other (other_slot[0], the fallback
+// start's residual maximum; a NaN keeps nothing), else replaced by the fallback: Phi = 0 with the
+// defect rhs, or (PREV) Phi_0 = phi0 with the defect res0, each the value the first check forms for
+// that start (r32 = (float) of the FP64 defect).  Workgroup 0 stores rg with the error word to
+// rg_slot, and posts the chosen maximum as the check: to red_max[0], which a batch queued ahead
+// reads, and with the error word and seq to host_slot.  The host repeats the comparison on the
+// same pinned values to learn which start ran.
 template <bool PREV>
-__global__ __launch_bounds__(256) void select_start_kernel(size_t n, double margin, const double *rg_slot,
-                                                           const double *other_slot, double *phi, float *r32,
+__global__ __launch_bounds__(256) void select_start_kernel(size_t n, double margin, const double *__restrict__ part,
+                                                           unsigned nb, double *rg_slot, const double *other_slot,
+                                                           double *phi, float *r32,
                                                            const double *__restrict__ fb_defect,
                                                            const double *__restrict__ phi0,
                                                            unsigned long long *red_max, double *host_slot,
                                                            unsigned long long seq) {
-    const double rg = rg_slot[0], other = other_slot[0];
+    __shared__ double wred[4];
+    const int t = threadIdx.x;
+    constexpr unsigned B = 8;
+    double m = 0.0;
+    for (unsigned q0 = (unsigned)t; q0 < nb; q0 += 256 * B) {
+        double v[B];
+#pragma unroll
+        for (unsigned u = 0; u < B; ++u) {
+            const unsigned q = q0 + 256 * u;
+            v[u] = q < nb ? part[q] : 0.0;
+        }
+#pragma unroll
+        for (unsigned u = 0; u < B; ++u) m = nanmax_any(m, v[u]);
+    }
+    for (int off = 32; off > 0; off >>= 1) m = nanmax_any(m, __shfl_xor(m, off, 64));
+    if ((t & 63) == 0) wred[t >> 6] = m;
+    __syncthreads();
+    const double rg = nanmax_any(nanmax_any(wred[0], wred[1]), nanmax_any(wred[2], wred[3]));
+    const double other = other_slot[0];
     const bool take = rg < margin * other;
     if (!take) {
         for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
@@ -1246,24 +1269,29 @@ __global__ __launch_bounds__(256) void select_start_kernel(size_t n, double marg
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const double v = take ? rg : other;
+        const double ew = __longlong_as_double((long long)red_max[1]);
+        rg_slot[0] = rg;
+        rg_slot[1] = ew;
         red_max[0] = (unsigned long long)__double_as_longlong(v);
         host_slot[0] = v;
-        host_slot[1] = __longlong_as_double((long long)red_max[1]);
+        host_slot[1] = ew;
         __threadfence_system();
         reinterpret_cast<volatile unsigned long long *>(host_slot)[2] = seq;
     }
 }
 
-hipError_t launch_select_start(mdx_handle *h, double margin, const double *rg_slot, const double *other_slot,
+hipError_t launch_select_start(mdx_handle *h, double margin, unsigned nb, double *rg_slot, const double *other_slot,
                                const double *phi0, const double *res0, double *host_slot, unsigned long long seq) {
     const size_t n = h->ni;
-    const unsigned blocks = (unsigned)std::min<size_t>((n + 1023) / 1024, 4096);
+    // 256 workgroups: each folds the nb maxima (from the L2) and rewrites n / 256 cells on a fallback
+    const unsigned blocks = 256;
+    if (nb > MAX_PARTS) return hipErrorInvalidValue;
     if (phi0)
-        hipLaunchKernelGGL(select_start_kernel<true>, dim3(blocks), dim3(256), 0, h->stream, n, margin, rg_slot,
-                           other_slot, h->phi, h->r32, res0, phi0, h->red_max, host_slot, seq);
+        hipLaunchKernelGGL(select_start_kernel<true>, dim3(blocks), dim3(256), 0, h->stream, n, margin, h->max_part,
+                           nb, rg_slot, other_slot, h->phi, h->r32, res0, phi0, h->red_max, host_slot, seq);
     else
-        hipLaunchKernelGGL(select_start_kernel<false>, dim3(blocks), dim3(256), 0, h->stream, n, margin, rg_slot,
-                           other_slot, h->phi, h->r32, h->rhs, nullptr, h->red_max, host_slot, seq);
+        hipLaunchKernelGGL(select_start_kernel<false>, dim3(blocks), dim3(256), 0, h->stream, n, margin, h->max_part,
+                           nb, rg_slot, other_slot, h->phi, h->r32, h->rhs, nullptr, h->red_max, host_slot, seq);
     return hipGetLastError();
 }
 
@@ -1300,7 +1328,7 @@ __global__ __launch_bounds__(256) void defect64_kernel(GridK g, MuK<double> mu, 
 }
 
 hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32,
-                           double *host_slot, unsigned long long seq) {
+                           double *host_slot, unsigned long long seq, unsigned *parts_only) {
     GridK g = grid_k(h);
     MuK<double> mu = mu_k<double>(h, h->mu64);
     SorConst<double> s{1.0 / h->dx * (1.0 / h->dx), 1.0 / h->dy * (1.0 / h->dy), 1.0 / h->dz * (1.0 / h->dz)};
@@ -1319,6 +1347,10 @@ hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, 
 #undef MDX_DFE
 #undef MDX_DF
     hipError_t err = hipGetLastError();
+    if (parts_only) {   // the caller folds the grd.x workgroup maxima itself
+        *parts_only = grd.x;
+        return err;
+    }
     return err != hipSuccess ? err : launch_max_final(h, grd.x, host_slot, seq);
 }
 

@@ -650,14 +650,17 @@ static constexpr int SUM_BLOCKS = 512;
 static constexpr int SUM_THREADS = 256;
 
 __global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, const double *__restrict__ u,
-                                                                   const double *__restrict__ cu, double dtg,
-                                                                   double *part) {
+                                                                   const double *__restrict__ cu, int cplane,
+                                                                   double dtg, double *part) {
     const Dims d = g.d;
     __shared__ double s1[SUM_THREADS], s2[SUM_THREADS];
     double a = 0.0, b = 0.0;
     // workgroup b sums the x rows r = b, b + SUM_BLOCKS, ... (r = (j-1) + ny (k-1)), threads along x
     const int nrows = d.ny * d.nz;
+    // mu_u of a cell from its coefficient; a z-invariant coefficient array is read at k = 1 (the
+    // same value, with 1/nz of the traffic)
     auto term_b = [&](size_t c, double hj) { return (cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0) * hj; };
+    auto cidx = [&](int i, int j, int k) { return d.g(0, j, cplane ? 1 : k) + (size_t)i; };
     if (d.nx <= SUM_THREADS) {
         // one value per row and thread: SUM_ROWS rows' loads are issued together, then added in
         // row order (the same sums as the loop below)
@@ -673,7 +676,7 @@ __global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, cons
                     const double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
                     const size_t c = d.g(0, j, k) + (size_t)i;
                     uv[q] = u[c] * hj;
-                    bv[q] = term_b(c, hj);
+                    bv[q] = term_b(cidx(i, j, k), hj);
                 }
 #pragma unroll
                 for (int q = 0; q < SUM_ROWS; ++q) {
@@ -686,7 +689,7 @@ __global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, cons
                 const double hj = g.ihc ? 1.0 / g.ihc[j] : g.dy;
                 const size_t c = d.g(0, j, k) + (size_t)i;
                 a += u[c] * hj;
-                b += term_b(c, hj);
+                b += term_b(cidx(i, j, k), hj);
             }
         }
     } else {
@@ -697,7 +700,7 @@ __global__ __launch_bounds__(SUM_THREADS) void bulk_partial_kernel(GridK g, cons
             for (int i = threadIdx.x + 1; i <= d.nx; i += SUM_THREADS) {
                 const size_t c = row + (size_t)i;
                 a += u[c] * hj;
-                b += term_b(c, hj);
+                b += term_b(cidx(i, j, k), hj);
             }
         }
     }
@@ -748,32 +751,83 @@ __global__ __launch_bounds__(SUM_THREADS) void bulk_final_kernel(const double *p
 hipError_t launch_bulk_sum(mdx_handle *h, const double *u, double dtg, double *out_dev) {
     GridK g = grid_k(h);
     hipLaunchKernelGGL(bulk_partial_kernel, dim3(SUM_BLOCKS), dim3(SUM_THREADS), 0, h->stream, g, u,
-                       h->ibm_on ? h->cu : nullptr, dtg, h->red_part);
+                       h->ibm_on ? h->cu : nullptr, h->coef_plane ? 1 : 0, dtg, h->red_part);
     double norm = (double)h->d.nx * (double)h->d.nz * h->cfg.flow_height;
     hipLaunchKernelGGL(bulk_final_kernel, dim3(1), dim3(SUM_THREADS), 0, h->stream, h->red_part, norm,
                        h->cfg.ub_target, out_dev);
     return hipGetLastError();
 }
 
-// u* += du * mu_u on interior u points; du read from device memory (no host round trip)
-__global__ __launch_bounds__(BX * BY) void flow_shift_kernel(GridK g, double *us, const double *cu,
-                                                             double dtg, const double *du_dev) {
+// Constant flow rate, second half: every workgroup folds the SUM_BLOCKS partial sums of
+// bulk_partial_kernel as bulk_final_kernel does (so every workgroup holds the same du; workgroup 0
+// also stores ub, mean mu_u and du to out), then u* += du * mu_u on its cells: a 64 x 4 tile of
+// columns over FS_KZ planes, mu_u formed once per column where the coefficients are z-invariant
+// (cplane), else per cell: u* + du * (1 / (1 + dt_gamma c_u)) as the oracle's shift forms it.
+static constexpr int FS_KZ = 16;
+__global__ __launch_bounds__(SUM_THREADS) void flow_shift_fold_kernel(GridK g, double *us, const double *cu,
+                                                                      int cplane, double dtg,
+                                                                      const double *__restrict__ part, double norm,
+                                                                      double target, double *out) {
+    static_assert(SUM_THREADS == 256, "the fold's lanes are the 64 x 4 threads");
+    __shared__ double s1[SUM_THREADS], s2[SUM_THREADS];
+    const int t = threadIdx.x + 64 * threadIdx.y;
+    double a = 0.0, b = 0.0;
+    for (int q = t; q < SUM_BLOCKS; q += SUM_THREADS) {
+        a += part[2 * q];
+        b += part[2 * q + 1];
+    }
+    s1[t] = a;
+    s2[t] = b;
+    __syncthreads();
+    for (int w = SUM_THREADS / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            s1[t] += s1[t + w];
+            s2[t] += s2[t + w];
+        }
+        __syncthreads();
+    }
+    const double ub = s1[0] / norm;
+    const double sm = s2[0] / norm;
+    const double du = (target - ub) / sm;
+    if (blockIdx.x == 0 && t == 0) {
+        out[0] = ub;
+        out[1] = sm;
+        out[2] = du;
+    }
     const Dims d = g.d;
-    const int i = blockIdx.x * BX + threadIdx.x + 1;
-    const int j = blockIdx.y * BY + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
+    int tx_, ty_, tz_;
+    if (!xcd_tile(d.nx, d.ny, (d.nz + FS_KZ - 1) / FS_KZ, 64, 4, tx_, ty_, tz_)) return;
+    const int i = tx_ * 64 + threadIdx.x + 1, j = ty_ * 4 + threadIdx.y + 1, k0 = tz_ * FS_KZ + 1;
     if (i > d.nx || j > d.ny) return;
-    const double du = du_dev[2];
-    size_t c = d.g(i, j, k);
-    double mu = cu ? 1.0 / (1.0 + dtg * cu[c]) : 1.0;
-    us[c] = us[c] + du * mu;
+    const double mc = (cu && cplane) ? 1.0 / (1.0 + dtg * cu[d.g(i, j, 1)]) : 1.0;
+    double uo[FS_KZ], mk[FS_KZ];
+#pragma unroll
+    for (int u = 0; u < FS_KZ; ++u) {
+        const int k = k0 + u;
+        uo[u] = 0.0;
+        mk[u] = mc;
+        if (k <= d.nz) {
+            const size_t c = d.g(i, j, k);
+            uo[u] = us[c];
+            if (cu && !cplane) mk[u] = 1.0 / (1.0 + dtg * cu[c]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < FS_KZ; ++u)
+        if (k0 + u <= d.nz) us[d.g(i, j, k0 + u)] = uo[u] + du * mk[u];
 }
 
-hipError_t launch_flow_shift(mdx_handle *h, double dtg, const double *du_dev) {
+// the flow-rate correction of u* in two launches: the partial sums, then the fold and the shift
+hipError_t launch_flowrate(mdx_handle *h, double dtg, double *out_dev) {
     GridK g = grid_k(h);
-    dim3 blk(BX, BY, 1), grd((h->d.nx + BX - 1) / BX, (h->d.ny + BY - 1) / BY, h->d.nz);
-    hipLaunchKernelGGL(flow_shift_kernel, grd, blk, 0, h->stream, g, h->us, h->ibm_on ? h->cu : nullptr,
-                       dtg, du_dev);
+    const double *cu = h->ibm_on ? h->cu : nullptr;
+    const int cplane = h->coef_plane ? 1 : 0;
+    hipLaunchKernelGGL(bulk_partial_kernel, dim3(SUM_BLOCKS), dim3(SUM_THREADS), 0, h->stream, g, h->us, cu, cplane,
+                       dtg, h->red_part);
+    const double norm = (double)h->d.nx * (double)h->d.nz * h->cfg.flow_height;
+    const unsigned grd = xcd_grid(h->d.nx, h->d.ny, (h->d.nz + FS_KZ - 1) / FS_KZ, 64, 4);
+    hipLaunchKernelGGL(flow_shift_fold_kernel, dim3(grd), dim3(64, 4), 0, h->stream, g, h->us, cu, cplane, dtg,
+                       h->red_part, norm, h->cfg.ub_target, out_dev);
     return hipGetLastError();
 }
 
