@@ -718,13 +718,20 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
         return;
     }
     // ---- defect epilogue: the halo holds the neighbours' final e (exchanged above).  A thread
-    // owns a column (i, j) of the subdomain and marches k, keeping Phi = phi_in + e of planes
-    // k-1, k, k+1 of its column and loading the plane after next ahead; the face coefficients of
-    // a column do not change with k (z-invariant tables).  The operands handed to
-    // sor_residual_core are those cell_residual forms, so the defect equals the separate kernel's.
+    // owns a column (i, j) of the subdomain, or half of its planes where the workgroup has the
+    // threads (EP = 2), and marches k, keeping Phi = phi_in + e of planes k-1, k, k+1 of its column;
+    // the face coefficients of a column do not change with k (z-invariant tables).  The pointers
+    // are declared unaliased so the loads of the next plane may issue before this plane's stores.
+    // The operands handed to sor_residual_core are those cell_residual forms, so the defect equals
+    // the separate kernel's.
     double mx = 0.0;
-    if (t < bx * by) {
-        const int li = t % bx + 1, lj = t / bx + 1;
+    const int ncol = bx * by;
+    const int EP = NT >= 2 * ncol ? 2 : 1;
+    if (t < EP * ncol) {
+        const int col = t % ncol, part = t / ncol;
+        const int li = col % bx + 1, lj = col / bx + 1;
+        const int hz = (bz + EP - 1) / EP;
+        const int lk0 = 1 + part * hz, lk1 = lk0 + hz - 1 < bz ? lk0 + hz - 1 : bz;
         const int i = ox + li, j = oy + lj, nx = d.nx, ny = d.ny, nz = d.nz;
         const bool per2 = per_y;
         const int ip = i == nx ? 1 : i + 1, im = i == 1 ? nx : i - 1;
@@ -746,13 +753,17 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
             mwkp = mwk;   // z-invariant table
         }
         const double *__restrict__ pin = a.phi_in;
+        const double *__restrict__ r64 = a.rhs64;
+        double *__restrict__ pout = a.phi_out;
+        float *__restrict__ rout = a.r_out;
         auto Pc = [&](int gi, int gj, int gk, int xi, int xj, int xk) {   // phi_in + e of one cell
             return pin[d.in(gi, gj, gk)] + (double)X[L(xi, xj, xk)];
         };
         auto kw = [&](int k) { return k < 1 ? k + nz : (k > nz ? k - nz : k); };
-        // z neighbours of the first plane (k = oz + 1): its lower neighbour wraps periodically
-        double pm = Pc(i, j, kw(oz), li, lj, 0), pc = Pc(i, j, oz + 1, li, lj, 1);
-        for (int lk = 1; lk <= bz; ++lk) {
+        // z neighbours of the first plane: below the subdomain the lower neighbour wraps periodically
+        double pm = Pc(i, j, kw(oz + lk0 - 1), li, lj, lk0 - 1), pc = Pc(i, j, oz + lk0, li, lj, lk0);
+#pragma unroll 2
+        for (int lk = lk0; lk <= lk1; ++lk) {
             const int k = oz + lk;
             const double pn = Pc(i, j, kw(k + 1), li, lj, lk + 1);
             const double pxp = Pc(ip, j, k, li + 1, lj, lk), pxm = Pc(im, j, k, li - 1, lj, lk);
@@ -760,10 +771,10 @@ __global__ __launch_bounds__(FORM == 2 ? 2 * RB_THREADS : RB_THREADS) void rb_ld
             const size_t c = d.in(i, j, k);
             double denom;
             const double r = sor_residual_core<double, false>(pc, pxp, pxm, pyp, pym, pn, pm, mui, muip, mvj, mvjp,
-                                                              mwk, mwkp, wn_, ws_, a.rhs64[c], a.s64, 0.0, 0.0,
+                                                              mwk, mwkp, wn_, ws_, r64[c], a.s64, 0.0, 0.0,
                                                               denom);
-            a.phi_out[c] = pc;
-            a.r_out[c] = (float)r;
+            pout[c] = pc;
+            rout[c] = (float)r;
             mx = nanmax_any(mx, fabs(r));
             pm = pc;
             pc = pn;
