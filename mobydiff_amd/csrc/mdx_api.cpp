@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -315,6 +316,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     h->stretched = cfg->stretch_beta > 0.0;
     if (const char *ng = std::getenv("MOBYDIFF_NO_GRAPHS")) h->use_graphs = !(ng[0] == '1');
     if (const char *fi = std::getenv("MOBYDIFF_FUSED_INPUT")) h->fused_input = !(fi[0] == '0');
+    if (const char *dc = std::getenv("MOBYDIFF_DEFER_CHOICE")) h->defer_choice = !(dc[0] == '0');
     h->inner_omega = h->cfg.omega;
     if (const char *io = std::getenv("MOBYDIFF_INNER_OMEGA")) {
         const double v = std::atof(io);
@@ -1088,7 +1090,10 @@ static int wait_check(mdx_handle *h, int w, unsigned long long seq, double *r) {
 // first_seq != 0: the first check is already queued in slot 0 with that sequence number (the fused
 // start kernel).  key: the RK stage + 3 * the kind of start (0 Phi = 0, 1 Phi_0, 2 the network's):
 // the previous solve with the same key gives the merge prior (none for network starts).
-static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq, int key) {
+// on_first: called once the first check has been read (a learned start whose choice was deferred:
+// it learns which start ran and may change the key).
+static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long long first_seq, int key,
+                           const std::function<int(int &)> &on_first = nullptr) {
     constexpr double SPEC_MARGIN = 4.0, SPEC_FIRST = 1e3;
     const int MERGE_MAX = h->merge_max;   // at most this many inner_iters batches run as one
     constexpr int PIN = CHECK_PIN;   // pinned words of the two check slots: [PIN + 4 * slot]
@@ -1163,7 +1168,10 @@ static int sor_solve_mixed(mdx_handle *h, mdx_solve_stats *st, unsigned long lon
         // a batch queued ahead of a converged check did nothing on the LDS solver: not counted
         if (done && spec && lds && lds_for(h, true) && h->sw_last >= 0) h->sw_void[h->sw_last] = true;
         sweep_account(h);
-        if (checks == 0) st->div0_inf = r;
+        if (checks == 0) {
+            st->div0_inf = r;
+            if (on_first && (rc = on_first(key))) return rc;
+        }
         ++checks;
         if (done) {
             // a batch queued ahead is discarded: on LDS its check would have written the other
@@ -1514,6 +1522,7 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         // a mixed-precision learned start is chosen on the device (no host round trip)
         const bool dev_select = gp.learned && c.precision == MDX_PREC_MIXED;
         unsigned long long first_seq = 0;
+        bool defer = false;
         if (gp.simple && c.precision == MDX_PREC_MIXED) {
             // rhs, the start and the solve's first check in one kernel
             first_seq = ++h->chk_seq;
@@ -1527,9 +1536,15 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
             if (!fused) MDX_HIP(launch_sor_prepare(h, want_rz ? h->d_pinned + RZ_PIN : nullptr, 0));
             if (dev_select) {
                 rc = learned_start_mixed(h, stage, gp, fused, &first_seq);
+                // After a network-started solve of this stage the network's start is expected
+                // again: the solve queues its first batch (inner_iters, the network start's first
+                // batch) behind the choice and learns which start ran from the first check.  Else
+                // the choice is read first, so the batches are those of the start that runs (a
+                // fallback solve is then the other mode's solve exactly).
+                defer = rc == MDX_OK && h->defer_choice && h->net_prev_started[stage] && lds_for(h, true);
                 double r0 = 0.0;
-                if (rc == MDX_OK) rc = wait_check(h, CHECK_PIN, first_seq, &r0);
-                if (rc == MDX_OK) rc = resolve_learned_start(h, stage, gp, &st, &r_first);
+                if (rc == MDX_OK && !defer) rc = wait_check(h, CHECK_PIN, first_seq, &r0);
+                if (rc == MDX_OK && !defer) rc = resolve_learned_start(h, stage, gp, &st, &r_first);
             } else {
                 rc = initial_guess(h, stage, dtg, gp, &st, &r_first);
             }
@@ -1537,8 +1552,15 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         }
         if (rc == MDX_OK) {
             // the merge prior is the previous solve of this stage from the same kind of start
-            const int kind = h->net_started[stage] ? 2 : (gp.use_prev ? 1 : 0);
-            rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage + 3 * kind)
+            const int kind = (defer || h->net_started[stage]) ? 2 : (gp.use_prev ? 1 : 0);
+            std::function<int(int &)> on_first;
+            if (defer)
+                on_first = [&](int &key) {
+                    const int r_ = resolve_learned_start(h, stage, gp, &st, &r_first);
+                    if (!h->net_started[stage]) key = stage + 3 * (gp.use_prev ? 1 : 0);
+                    return r_;
+                };
+            rc = c.precision == MDX_PREC_MIXED ? sor_solve_mixed(h, &st, first_seq, stage + 3 * kind, on_first)
                                                : sor_solve64(h, &st, r_first);
         }
         if (rc == MDX_LDS_TIMEOUT) {
@@ -1567,6 +1589,7 @@ extern "C" int mdx_pressure_projection(mdx_handle *h, double dtg, mdx_solve_stat
         if (st.fallback == 0 && r_first >= 0.0) st.div0_inf = r_first;
         net_outcome(h, stage, std::chrono::duration<double>(std::chrono::steady_clock::now() - t_plan).count(),
                     gp.learned);
+        h->net_prev_started[stage] = h->net_started[stage];
         if (h->net && h->net_capture && st.converged && (rc = capture_pair(h, stage, dtg))) return rc;
         // the converged Phi becomes the history of this stage for the next step (copied by the
         // projection kernel, which reads every Phi anyway)
@@ -1592,7 +1615,7 @@ extern "C" int mdx_set_guess(mdx_handle *h, int32_t guess) {
     for (int q = 0; q < 3; ++q) {
         h->net_losses[q] = h->net_skip[q] = 0;
         h->base_sec[q] = 0.0;
-        h->net_started[q] = false;
+        h->net_started[q] = h->net_prev_started[q] = false;
     }
     MDX_CHECK(guess == MDX_GUESS_PREVIOUS || guess == MDX_GUESS_LEARNED || guess == MDX_GUESS_PHI_PREV,
               MDX_ERR_ARG, "mdx_set_guess: unknown guess mode");

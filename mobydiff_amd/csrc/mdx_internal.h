@@ -264,6 +264,10 @@ struct mdx_handle {
     // prior; index stage + 3 * (start from Phi_0); 0: none)
     int mix_prev_its[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // [stage + 3 * start kind (0 plain, 1 Phi_0, 2 network)]
     bool net_started[3] = {false, false, false};
+    // the previous solve of each stage was network-started: the next learned start queues its first
+    // batch before the device's choice has been read (MOBYDIFF_DEFER_CHOICE=0: it always waits)
+    bool net_prev_started[3] = {false, false, false};
+    bool defer_choice = true;
     double base_sec[3] = {0.0, 0.0, 0.0};   // host time of the last solve of each stage not started by the network
     long proj_count = 0;
 };
