@@ -39,6 +39,7 @@ struct mdx_net {
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool tiled_conv = true;        // LDS-tiled convolutions where the grid allows (bit-identical)
     bool split_conv = true;        // split-K convolutions on small grids the tiled kernel cannot take
+    bool fuse_cat = true;          // inference: decoder convolutions read the up-sampling and skip in place
     std::vector<struct mdx_handle *> users;   // solver handles this network is attached to
 };
 
@@ -58,7 +59,8 @@ static constexpr int NL = 11;
 
 hipError_t repack_all(mdx_net *n);
 hipError_t upload_input2(mdx_net *n, const float *x2);
-hipError_t net_forward(mdx_net *n);
+// infer: no backward pass follows (decoder concats are not materialised)
+hipError_t net_forward(mdx_net *n, bool infer = false);
 hipError_t net_backward(mdx_net *n, float scale);
 hipError_t net_adam(mdx_net *n, float lr);
 // solver coupling.  The solve starts from Phi_0 = fac * Phi_prev (the converged correction of the

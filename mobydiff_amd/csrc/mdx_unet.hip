@@ -107,11 +107,22 @@ __global__ __launch_bounds__(256) void conv3_fwd_kernel(const bf16 *__restrict__
 // w, w + S, w + 2S, ... (the direct kernel's chunks and weight fragments), and their partial
 // accumulators are summed in LDS in wave order, then bias and ReLU as above.  A wave then waits
 // for KC / S global fetches instead of KC.  Needs V % 16 == 0.
+// The input of a decoder convolution may come unmaterialised (up != nullptr): channels [0, cu) are
+// the nearest up-sampling of `up` (a grid of half the extents), channels [cu, cin) are `in` (the
+// skip tensor, cin - cu channels): the values upcat_kernel would have stored.
+__device__ __forceinline__ const bf16 *conv_in_at(const bf16 *in, const bf16 *up, int cu, int cin, int W, int H, int x,
+                                                  int y, int z, int c0) {
+    if (!up) return in + (((long)z * H + y) * W + x) * cin + c0;
+    if (c0 < cu) return up + (((long)(z >> 1) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * cu + c0;
+    return in + (((long)z * H + y) * W + x) * (cin - cu) + (c0 - cu);
+}
+
 template <int COUT> struct SplitK {
     static constexpr int NT = COUT / 16, S = NT >= 3 ? 8 : 16;
 };
 template <int CIN, int COUT, bool RELU>
 __global__ __launch_bounds__(64 * SplitK<COUT>::S) void conv3_splitk_kernel(const bf16 *__restrict__ in,
+                                                                            const bf16 *__restrict__ up, int cu,
                                                                             const bf16x8 *__restrict__ wp,
                                                                             const float *__restrict__ bias,
                                                                             bf16 *__restrict__ out, int W, int H,
@@ -135,7 +146,7 @@ __global__ __launch_bounds__(64 * SplitK<COUT>::S) void conv3_splitk_kernel(cons
             const int tap = k0 / CIN, c0 = k0 - tap * CIN;
             const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
             if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
-                a = *reinterpret_cast<const bf16x8 *>(in + (((long)zz * H + yy) * W + xx) * CIN + c0);
+                a = *reinterpret_cast<const bf16x8 *>(conv_in_at(in, up, cu, CIN, W, H, xx, yy, zz, c0));
         }
         return a;
     };
@@ -205,7 +216,8 @@ template <int CIN> struct ConvTile {
 };
 
 template <int CIN, int COUT, bool RELU>
-__global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict__ in, const bf16x8 *__restrict__ wp,
+__global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ up,
+                                                          int cu, const bf16x8 *__restrict__ wp,
                                                           const float *__restrict__ bias, bf16 *__restrict__ out,
                                                           int W, int H, int D) {
     constexpr int K = 27 * CIN, KC = (K + 31) / 32, NT = COUT / 16, U = CIN / 8;
@@ -221,7 +233,7 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
         const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
         bf16x8 a = {};
         if (x >= 0 && x < W && y >= 0 && y < H && z >= 0 && z < D)
-            a = *reinterpret_cast<const bf16x8 *>(in + (((long)z * H + y) * W + x) * CIN + u * 8);
+            a = *reinterpret_cast<const bf16x8 *>(conv_in_at(in, up, cu, CIN, W, H, x, y, z, u * 8));
         *reinterpret_cast<bf16x8 *>(&tile[v * CS + u * 8]) = a;
     }
     __syncthreads();
@@ -911,8 +923,10 @@ static float *dWt(mdx_net *n, int l) { return n->grads + n->woff[l]; }
 static float *dBs(mdx_net *n, int l) { return n->grads + n->boff[l]; }
 
 // 3x3x3 conv (implicit GEMM on MFMA): forward (relu, bias) or input gradient (plain)
+// up != nullptr: the input's channels [0, cu) are the nearest up-sampling of `up` and the rest
+// `in` (conv_in_at); only the tiled and split-K kernels take such an input
 static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in, const bf16 *wp, const float *bias,
-                        bf16 *out, int lvl) {
+                        bf16 *out, int lvl, const bf16 *up = nullptr, int cu = 0) {
     const int W = n->W[lvl], H = n->H[lvl], D = n->D[lvl];
     const unsigned blocks = (unsigned)(n->V[lvl] / 64);
 #define TILED_CASE(CI, CO, R)                                                                                      \
@@ -924,7 +938,7 @@ static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)ConvTile<CI>::BYTES);              \
         if (attr != hipSuccess) return attr;                                                                       \
         hipLaunchKernelGGL((conv3_tiled_kernel<CI, CO, R>), dim3(tiles), dim3(256), ConvTile<CI>::BYTES,           \
-                           n->stream, in, reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);               \
+                           n->stream, in, up, cu, reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);       \
         return hipGetLastError();                                                                                  \
     }
     TILED_CASE(16, 16, true) TILED_CASE(16, 32, true) TILED_CASE(32, 32, true) TILED_CASE(32, 64, true)
@@ -939,7 +953,7 @@ static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in
         !(W % CTX == 0 && H % CTY == 0 && D % ConvTile<CI>::TZ == 0) &&                                           \
         n->V[lvl] % 16 == 0 && mt < 1024) {                                                                        \
         hipLaunchKernelGGL((conv3_splitk_kernel<CI, CO, R>), dim3((unsigned)mt), dim3(64 * SplitK<CO>::S), 0,       \
-                           n->stream, in, reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);               \
+                           n->stream, in, up, cu, reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D);       \
         return hipGetLastError();                                                                                  \
     }
     SPLIT_CASE(16, 16, true) SPLIT_CASE(16, 32, true) SPLIT_CASE(32, 32, true) SPLIT_CASE(32, 64, true)
@@ -947,6 +961,7 @@ static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in
     SPLIT_CASE(16, 16, false) SPLIT_CASE(32, 16, false) SPLIT_CASE(32, 32, false) SPLIT_CASE(64, 32, false)
     SPLIT_CASE(64, 64, false) SPLIT_CASE(32, 96, false) SPLIT_CASE(16, 48, false)
 #undef SPLIT_CASE
+    if (up) return hipErrorInvalidValue;   // the direct kernel takes a materialised input only
 #define CONV_CASE(CI, CO, R)                                                                                       \
     if (cin == CI && cout == CO && relu == R) {                                                                    \
         hipLaunchKernelGGL((conv3_fwd_kernel<CI, CO, R>), dim3(blocks), dim3(256), 0, n->stream, in,                \
@@ -1010,7 +1025,7 @@ hipError_t upload_input2(mdx_net *n, const float *x2) {
     return hipGetLastError();
 }
 
-hipError_t net_forward(mdx_net *n) {
+hipError_t net_forward(mdx_net *n, bool infer) {
     hipError_t e;
     const int W0 = n->W[0], H0 = n->H[0], D0 = n->D[0], W1 = n->W[1], H1 = n->H[1], D1 = n->D[1];
 #define TRY(x) if ((e = (x)) != hipSuccess) return e
@@ -1022,13 +1037,24 @@ hipError_t net_forward(mdx_net *n) {
     hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[2] * 4)), dim3(256), 0, n->stream, n->a[3], n->p2, 32, W1, H1, D1);
     TRY(conv3(n, 32, 64, true, n->p2, n->wp[4], Bs(n, 4), n->a[4], 2));
     TRY(conv3(n, 64, 64, true, n->a[4], n->wp[5], Bs(n, 5), n->a[5], 2));
-    hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[1] * 12)), dim3(256), 0, n->stream, n->a[5], n->a[3], n->cat2, 64, 32,
-                       W1, H1, D1);
-    TRY(conv3(n, 96, 32, true, n->cat2, n->wp[6], Bs(n, 6), n->a[6], 1));
+    // decoder inputs: inference reads the up-sampling and the skip tensor where they lie (the
+    // convolution's loader forms the concat); training keeps the concat for the backward pass
+    const bool fuse = infer && n->fuse_cat;
+    e = fuse ? conv3(n, 96, 32, true, n->a[3], n->wp[6], Bs(n, 6), n->a[6], 1, n->a[5], 64) : hipErrorInvalidValue;
+    if (e == hipErrorInvalidValue) {
+        hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[1] * 12)), dim3(256), 0, n->stream, n->a[5], n->a[3], n->cat2,
+                           64, 32, W1, H1, D1);
+        e = conv3(n, 96, 32, true, n->cat2, n->wp[6], Bs(n, 6), n->a[6], 1);
+    }
+    TRY(e);
     TRY(conv3(n, 32, 32, true, n->a[6], n->wp[7], Bs(n, 7), n->a[7], 1));
-    hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[0] * 6)), dim3(256), 0, n->stream, n->a[7], n->a[1], n->cat1, 32, 16,
-                       W0, H0, D0);
-    TRY(conv3(n, 48, 16, true, n->cat1, n->wp[8], Bs(n, 8), n->a[8], 0));
+    e = fuse ? conv3(n, 48, 16, true, n->a[1], n->wp[8], Bs(n, 8), n->a[8], 0, n->a[7], 32) : hipErrorInvalidValue;
+    if (e == hipErrorInvalidValue) {
+        hipLaunchKernelGGL(upcat_kernel, dim3(nblk(n->V[0] * 6)), dim3(256), 0, n->stream, n->a[7], n->a[1], n->cat1,
+                           32, 16, W0, H0, D0);
+        e = conv3(n, 48, 16, true, n->cat1, n->wp[8], Bs(n, 8), n->a[8], 0);
+    }
+    TRY(e);
     TRY(conv3(n, 16, 16, true, n->a[8], n->wp[9], Bs(n, 9), n->a[9], 0));
     hipLaunchKernelGGL(head_fwd_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->a[9], Wt(n, 10), Bs(n, 10),
                        n->out, n->V[0]);
@@ -1150,7 +1176,7 @@ hipError_t net_input_from_solver(mdx_net *n, const mdx_handle *h, const double *
 
 hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi) {
     hipError_t e;
-    TRY(net_forward(n));
+    TRY(net_forward(n, true));
     if (n->stride == 1)
         hipLaunchKernelGGL(guess_out_kernel, dim3(nblk(n->V[0])), dim3(256), 0, n->stream, n->out, n->stats, n->c0,
                            phi0, phi, n->V[0]);
@@ -1162,7 +1188,7 @@ hipError_t net_guess_to_phi(mdx_net *n, const double *phi0, double *phi) {
 hipError_t net_guess_to_phi_scaled(mdx_net *n, const double *prev, double fac, double *phi) {
     hipError_t e;
     if (n->stride == 1) return hipErrorInvalidValue;
-    TRY(net_forward(n));
+    TRY(net_forward(n, true));
     launch_prolong(n, n->out, n->stats, n->c0, nullptr, phi, prev, fac);
     return hipGetLastError();
 }

@@ -60,6 +60,7 @@ extern "C" int mdx_net_create_strided(int32_t nx, int32_t ny, int32_t nz, int32_
     mdx_net *n = new mdx_net();
     if (const char *t = std::getenv("MOBYDIFF_NO_TILED_CONV")) n->tiled_conv = !(t[0] == '1');
     if (const char *t = std::getenv("MOBYDIFF_NO_SPLIT_CONV")) n->split_conv = !(t[0] == '1');
+    if (const char *t = std::getenv("MOBYDIFF_NO_CAT_FUSION")) n->fuse_cat = !(t[0] == '1');
     n->stride = stride;
     n->FW = nx; n->FH = ny; n->FD = nz;
     n->FV = (long)nx * ny * nz;
@@ -200,7 +201,7 @@ extern "C" int mdx_net_zero_grads(mdx_net *n) {
 extern "C" int mdx_net_forward_host(mdx_net *n, const float *x2, float *out) {
     NET_CHECK(n && x2 && out, "mdx_net_forward_host: null argument");
     MDX_HIP(upload_input2(n, x2));
-    MDX_HIP(net_forward(n));
+    MDX_HIP(net_forward(n, true));
     MDX_HIP(hipMemcpyAsync(out, n->out, n->V[0] * 4, hipMemcpyDeviceToHost, n->stream));
     MDX_HIP(hipStreamSynchronize(n->stream));
     return MDX_OK;
@@ -335,12 +336,12 @@ extern "C" int mdx_net_time_forward(mdx_net *n, int32_t reps, float *ms_per_forw
                          v2 * (32 * 64 + 64 * 64);
     *flops = 2.0 * 27.0 * convs + 2.0 * v0 * 16.0;
     if (n->count > 0) MDX_HIP(net_load_replay(n, 0));   // else the input of the last forward pass
-    MDX_HIP(net_forward(n));
+    MDX_HIP(net_forward(n, true));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MDX_HIP(hipEventCreate(&e0));
     hipError_t e = hipEventCreate(&e1);
     if (e == hipSuccess) e = hipEventRecord(e0, n->stream);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = net_forward(n);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = net_forward(n, true);
     if (e == hipSuccess) e = hipEventRecord(e1, n->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.f;

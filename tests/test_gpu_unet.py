@@ -366,3 +366,22 @@ def test_strided_learned_guess_in_the_solver():
     assert ch.div_norm() < 1.5e-8
     ch.close()
     net.close()
+
+
+@pytest.mark.parametrize("stride", [1, 4])
+def test_inference_concat_read_in_place(stride, monkeypatch):
+    """In inference the decoder convolutions read the up-sampled coarse tensor and the skip tensor
+    where they lie (the loader forms the concat) instead of a materialised concat: the forward
+    output is bit-identical to the run that stores the concat first (MOBYDIFF_NO_CAT_FUSION=1)."""
+    rng = np.random.default_rng(13)
+    grid = (NX, NY, NZ) if stride == 1 else (192, 128, 96)
+    n0 = tuple(g // stride for g in grid)
+    x2 = rng.standard_normal(n0[::-1] + (2,)).astype(np.float32)
+    outs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MOBYDIFF_NO_CAT_FUSION", flag)
+        net = M.Net(*grid, seed=17, stride=stride)
+        outs.append(net.forward(x2))
+        net.close()
+    assert np.isfinite(outs[0]).all()
+    assert np.array_equal(outs[0], outs[1])
