@@ -107,7 +107,8 @@ typedef struct {
     double stencil_ms;       /* momentum + flow-rate shift + boundary conditions */
     double poisson_ms;       /* divergence RHS + initial guess + pressure solve (FFT/tridiag or SOR sweeps + checks) */
     double projection_ms;    /* velocity/pressure correction */
-    double guess_ms;         /* learned initial guess (network input, inference, start residual; part of poisson_ms) */
+    double guess_ms;         /* learned initial guess (network input with the divergence RHS pass, inference,
+                                prolongation; part of poisson_ms) */
     int64_t stages;
     double sweep_ms;         /* red-black sweep kernels alone, over a sample of the batches (every
                                 fourth batch is event-bracketed); the fields below count the same
