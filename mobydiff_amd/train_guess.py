@@ -18,6 +18,16 @@ pair takes 0.3 MB of HBM and all of them stay resident (6 GB; 47 GB at stride 2)
 
 The network, its input and target, the capture and the Adam kernels are the library's own
 (mdx_net_*, mdx_attach_net); this script only drives them.
+
+Calibration of the output scale (`--calibrate WEIGHTS`, no training): the network is trained for
+the mean-square error of its correction, but the SOR sweeps that follow remove the rough part of
+an error quickly and its smooth part slowly, so the correction's best amplitude for the solve is
+not the MSE-optimal one.  On held-out channels (seeds the training did not use, A across the
+range) the script runs the learned start on every solve with the correction scaled by each
+factor of `--scales` and keeps the factor with the fewest SOR iterations per solve; the weights
+are written again with c0 times that factor, the sweep recorded in the metadata.
+
+    python -m mobydiff_amd.train_guess --calibrate unet_guess_s4_wavy_192x128x96.npz --out calibrated.npz
 """
 import argparse
 import json
@@ -70,6 +80,58 @@ def load(path, net):
     return meta
 
 
+def calibrate(M, path, out, scales, channels, warmup, steps):
+    """The output-scale sweep described in the module docstring; returns the metadata written."""
+    import os
+    with np.load(path, allow_pickle=False) as z:
+        stride = int(z["stride"]) if "stride" in z.files else 1
+    net = M.Net(NX, NY, NZ, seed=0, stride=stride)
+    meta = load(path, net)
+    p, c0 = net.get_params()
+    env = {"MOBYDIFF_NET_MARGIN": "1e30", "MOBYDIFF_NET_TIME_GATE": "0"}   # the network starts every solve
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    results = {f"{s:g}": [] for s in scales}
+    try:
+        for A, seed in channels:
+            ch = wavy_channel(M, A, seed)
+            geo = M.wavy_geometry(A=A)
+            coef = ch.compute_ibm(geo)
+            for _ in range(warmup):
+                ch.step(DT)
+            state = ch.download()
+            ch.close()
+            for s in scales:
+                net.set_params(p, c0 * s)
+                run = wavy_channel(M, A, seed)
+                run.set_ibm(*coef)
+                run.upload(**state)
+                run.attach_net(net, capture=False)
+                run.set_guess(M.GUESS_LEARNED)
+                run.step(DT)   # the per-stage history the network's input needs
+                its = [run.step(DT).stage[q].iterations for _ in range(steps) for q in range(3)]
+                run.attach_net(None)
+                run.close()
+                results[f"{s:g}"].append(float(np.mean(its)))
+                print(f"A={A} seed={seed} scale {s:g}: {np.mean(its):.1f} iterations per solve", flush=True)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    mean = {k: float(np.mean(v)) for k, v in results.items()}
+    # fewest iterations; among equals the factor nearest 1
+    best = min(scales, key=lambda s: (round(mean[f"{s:g}"], 6), abs(s - 1.0)))
+    net.set_params(p, c0 * best)
+    meta = dict(meta, c0_calibration={"scale": best, "c0_trained": c0, "channels": [list(c) for c in channels],
+                                      "warmup_steps": warmup, "steps": steps,
+                                      "mean_iterations_per_solve": mean})
+    save(out, net, meta)
+    net.close()
+    return meta
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--runs", type=int, default=50)
@@ -87,9 +149,23 @@ def main(argv=None):
                          "(per chunk of 500 steps; default: constant --lr)")
     ap.add_argument("--init-seed", type=int, default=0)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--calibrate", default=None, metavar="WEIGHTS",
+                    help="calibrate the output scale of trained weights instead of training")
+    ap.add_argument("--scales", default="0.6,0.7,0.75,0.8,0.85,0.9,1.0")
+    ap.add_argument("--cal-channels", default="0.06:50,0.10:51,0.14:52",
+                    help="held-out channels A:seed of the calibration")
+    ap.add_argument("--cal-warmup", type=int, default=10)
+    ap.add_argument("--cal-steps", type=int, default=12)
     args = ap.parse_args(argv)
 
     import mobydiff_amd as M
+
+    if args.calibrate:
+        scales = [float(x) for x in args.scales.split(",")]
+        channels = [(float(a), int(s)) for a, s in (c.split(":") for c in args.cal_channels.split(","))]
+        meta = calibrate(M, args.calibrate, args.out, scales, channels, args.cal_warmup, args.cal_steps)
+        print(json.dumps(meta["c0_calibration"]), flush=True)
+        return 0
 
     rng = np.random.default_rng(args.seed0)
     amps = rng.uniform(args.amin, args.amax, args.runs)

@@ -5,6 +5,8 @@ Bar (written per test): forward within 1e-2 relative of the bf16-emulating refer
 gradients within 5e-2 relative (L2, per layer) of fp32 autograd on the same bf16-rounded inputs;
 Adam bit-for-bit-close (1e-6) to the closed form applied to the GPU's own gradients.
 """
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -275,6 +277,35 @@ def test_train_guess_script_round_trip(tmp_path):
     with np.load(out, allow_pickle=False) as z:
         assert np.array_equal(p, z["params"]) and c0 == float(z["c0"]) and c0 > 0.0
     net.close()
+
+
+SHIPPED = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mobydiff_amd", "weights",
+                       "unet_guess_s4_wavy_192x128x96.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(SHIPPED), reason="trained U-Net parameters not present")
+def test_train_guess_calibration_round_trip(tmp_path):
+    """mobydiff_amd.train_guess --calibrate, cut to one held-out channel, one warm-up step, two
+    measured steps and two factors: it runs the learned start at each factor, keeps the one with
+    fewer iterations per solve (ties to the factor nearest 1), and writes the same parameters with
+    c0 times that factor and the sweep in the metadata."""
+    from mobydiff_amd import train_guess
+    out = str(tmp_path / "cal.npz")
+    assert train_guess.main(["--calibrate", SHIPPED, "--out", out, "--scales", "0.5,1.0",
+                             "--cal-channels", "0.08:60", "--cal-warmup", "1", "--cal-steps", "2"]) == 0
+    with np.load(SHIPPED, allow_pickle=False) as a, np.load(out, allow_pickle=False) as b:
+        assert np.array_equal(a["params"], b["params"])
+        c0_in, c0_out = float(a["c0"]), float(b["c0"])
+    net = M.Net(train_guess.NX, train_guess.NY, train_guess.NZ, seed=0, stride=4)
+    meta = train_guess.load(out, net)
+    net.close()
+    cal = meta["c0_calibration"]
+    its = cal["mean_iterations_per_solve"]
+    print(f"calibration: {its}, scale {cal['scale']}", flush=True)
+    assert set(its) == {"0.5", "1"} and all(v > 0 for v in its.values())
+    want = 0.5 if its["0.5"] < its["1"] else 1.0
+    assert cal["scale"] == want and cal["c0_trained"] == c0_in
+    assert abs(c0_out - c0_in * want) <= 1e-15 * c0_in
 
 
 @pytest.mark.parametrize("fallback", ["prev", "zero"])
