@@ -43,10 +43,10 @@ RE, DT, AMP = 2000.0, 5e-4, 0.05
 TOL, OMEGA = 1e-8, 1.8
 
 
-def wavy_channel(M, A, seed):
+def wavy_channel(M, A, seed, inner=20):
     """A fresh wavy channel (mixed precision, previous-step Phi guess) at its seeded start."""
     cfg = M.channel_config(NX, NY, NZ, LX, LY, LZ, RE, solver=M.SOLVER_SOR, omega=OMEGA, tol=TOL,
-                           max_iter=200000, check_every=20, precision=M.PREC_MIXED, inner_iters=20,
+                           max_iter=200000, check_every=20, precision=M.PREC_MIXED, inner_iters=inner,
                            guess=M.GUESS_PHI_PREV, flowrate=True, flow_height=2.0)
     ch = M.Channel(cfg)
     geo = M.wavy_geometry(A=A)
@@ -80,14 +80,19 @@ def load(path, net):
     return meta
 
 
-def calibrate(M, path, out, scales, channels, warmup, steps):
-    """The output-scale sweep described in the module docstring; returns the metadata written."""
+def calibrate(M, path, out, scales, channels, warmup, steps, inner):
+    """The output-scale sweep described in the module docstring, with `inner` FP32 iterations per
+    defect-correction batch (the solver configuration the weights serve); the factors scale the
+    trained c0 (a file calibrated before is recalibrated from its trained c0).  Returns the
+    metadata written."""
     import os
     with np.load(path, allow_pickle=False) as z:
         stride = int(z["stride"]) if "stride" in z.files else 1
     net = M.Net(NX, NY, NZ, seed=0, stride=stride)
     meta = load(path, net)
     p, c0 = net.get_params()
+    if "c0_calibration" in meta:
+        c0 = float(meta["c0_calibration"]["c0_trained"])
     env = {"MOBYDIFF_NET_MARGIN": "1e30", "MOBYDIFF_NET_TIME_GATE": "0"}   # the network starts every solve
     saved = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
@@ -103,7 +108,7 @@ def calibrate(M, path, out, scales, channels, warmup, steps):
             ch.close()
             for s in scales:
                 net.set_params(p, c0 * s)
-                run = wavy_channel(M, A, seed)
+                run = wavy_channel(M, A, seed, inner)
                 run.set_ibm(*coef)
                 run.upload(**state)
                 run.attach_net(net, capture=False)
@@ -125,7 +130,7 @@ def calibrate(M, path, out, scales, channels, warmup, steps):
     best = min(scales, key=lambda s: (round(mean[f"{s:g}"], 6), abs(s - 1.0)))
     net.set_params(p, c0 * best)
     meta = dict(meta, c0_calibration={"scale": best, "c0_trained": c0, "channels": [list(c) for c in channels],
-                                      "warmup_steps": warmup, "steps": steps,
+                                      "warmup_steps": warmup, "steps": steps, "inner_iters": inner,
                                       "mean_iterations_per_solve": mean})
     save(out, net, meta)
     net.close()
@@ -156,6 +161,8 @@ def main(argv=None):
                     help="held-out channels A:seed of the calibration")
     ap.add_argument("--cal-warmup", type=int, default=10)
     ap.add_argument("--cal-steps", type=int, default=12)
+    ap.add_argument("--cal-inner", type=int, default=32,
+                    help="FP32 iterations per defect-correction batch of the calibration runs")
     args = ap.parse_args(argv)
 
     import mobydiff_amd as M
@@ -163,7 +170,8 @@ def main(argv=None):
     if args.calibrate:
         scales = [float(x) for x in args.scales.split(",")]
         channels = [(float(a), int(s)) for a, s in (c.split(":") for c in args.cal_channels.split(","))]
-        meta = calibrate(M, args.calibrate, args.out, scales, channels, args.cal_warmup, args.cal_steps)
+        meta = calibrate(M, args.calibrate, args.out, scales, channels, args.cal_warmup, args.cal_steps,
+                         args.cal_inner)
         print(json.dumps(meta["c0_calibration"]), flush=True)
         return 0
 

@@ -5,6 +5,7 @@ Bar (written per test): forward within 1e-2 relative of the bf16-emulating refer
 gradients within 5e-2 relative (L2, per layer) of fp32 autograd on the same bf16-rounded inputs;
 Adam bit-for-bit-close (1e-6) to the closed form applied to the GPU's own gradients.
 """
+import json
 import os
 
 import numpy as np
@@ -288,14 +289,17 @@ def test_train_guess_calibration_round_trip(tmp_path):
     """mobydiff_amd.train_guess --calibrate, cut to one held-out channel, one warm-up step, two
     measured steps and two factors: it runs the learned start at each factor, keeps the one with
     fewer iterations per solve (ties to the factor nearest 1), and writes the same parameters with
-    c0 times that factor and the sweep in the metadata."""
+    the trained c0 times that factor and the sweep in the metadata."""
     from mobydiff_amd import train_guess
     out = str(tmp_path / "cal.npz")
     assert train_guess.main(["--calibrate", SHIPPED, "--out", out, "--scales", "0.5,1.0",
                              "--cal-channels", "0.08:60", "--cal-warmup", "1", "--cal-steps", "2"]) == 0
     with np.load(SHIPPED, allow_pickle=False) as a, np.load(out, allow_pickle=False) as b:
         assert np.array_equal(a["params"], b["params"])
-        c0_in, c0_out = float(a["c0"]), float(b["c0"])
+        c0_out = float(b["c0"])
+        meta_in = json.loads(str(a["meta"]))
+        # the factors scale the trained c0 (the shipped file may be calibrated already)
+        c0_in = float(meta_in["c0_calibration"]["c0_trained"]) if "c0_calibration" in meta_in else float(a["c0"])
     net = M.Net(train_guess.NX, train_guess.NY, train_guess.NZ, seed=0, stride=4)
     meta = train_guess.load(out, net)
     net.close()
