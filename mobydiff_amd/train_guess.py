@@ -114,11 +114,12 @@ def calibrate(M, path, out, scales, channels, warmup, steps, inner):
                 run.attach_net(net, capture=False)
                 run.set_guess(M.GUESS_LEARNED)
                 run.step(DT)   # the per-stage history the network's input needs
-                its = [run.step(DT).stage[q].iterations for _ in range(steps) for q in range(3)]
+                its = np.array([[st.stage[q].iterations for q in range(3)] for st in (run.step(DT) for _ in range(steps))])
                 run.attach_net(None)
                 run.close()
                 results[f"{s:g}"].append(float(np.mean(its)))
-                print(f"A={A} seed={seed} scale {s:g}: {np.mean(its):.1f} iterations per solve", flush=True)
+                print(f"A={A} seed={seed} scale {s:g}: {np.mean(its):.1f} iterations per solve (stages "
+                      f"{', '.join(f'{v:.1f}' for v in its.mean(axis=0))})", flush=True)
     finally:
         for k, v in saved.items():
             if v is None:
