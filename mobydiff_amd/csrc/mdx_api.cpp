@@ -240,7 +240,7 @@ static void free_all(mdx_handle *h) {
                     h->red_part, h->red_max, h->max_part, h->rb_faces, h->phi_hist, h->res64,
                     h->fft_phi, h->fft_res, h->fft_mean, h->hat32, h->cp32, h->den32, h->arow32, h->tw_x32,
                     h->tw_z32, h->rb_xl32, h->rb_xl64, h->cvel, h->lamx, h->lamz, h->tw_a, h->bdrow, h->crow, h->lamx32, h->lamz32,
-                    h->tw_a32, h->bdrow32, h->crow32};
+                    h->tw_a32, h->bdrow32, h->crow32, h->fr_part};
     for (auto &s : h->graphs)
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
     for (void *p : ptrs)
@@ -384,6 +384,10 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if (const char *lo = std::getenv("MOBYDIFF_LDS_OVERLAP")) h->rb_overlap = lo[0] != '0';
     if (const char *mk = std::getenv("MOBYDIFF_MOM_KZ")) h->mom_kz = std::max(1, std::atoi(mk));
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
+    if (const char *ff = std::getenv("MOBYDIFF_FUSED_FLOWRATE")) h->fused_flowrate = ff[0] != '0';
+    if (h->cfg.flowrate && h->tiled_momentum && h->fused_flowrate &&
+        (e = dalloc(&h->fr_part, 2 * (size_t)momentum_tiles(h))) != hipSuccess)
+        return fail(e, "flow-rate partial buffer");
     if (const char *sf = std::getenv("MOBYDIFF_SWEEP_FUSED")) h->sweep_fused = sf[0] == '1';
     if (const char *kz = std::getenv("MOBYDIFF_SWEEP_KZ")) {
         const int v = std::atoi(kz);
@@ -1709,9 +1713,16 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
             h->ph_cur = (h->ph_head + h->ph_count) % mdx_handle::PHQ;
         }
         ev_record(h, 0);
-        if ((rc = mdx_momentum(h, dta, dtb, dtg))) return rc;
-        ev_record(h, 5);
-        if (h->cfg.flowrate && (rc = mdx_flowrate_shift(h, dtg, nullptr))) return rc;
+        if (h->cfg.flowrate && h->fr_part) {
+            // momentum forms the flow-rate partial sums of u*; one launch folds them and shifts u*
+            MDX_HIP(launch_momentum(h, dta, dtb, dtg, h->fr_part));
+            ev_record(h, 5);
+            MDX_HIP(launch_flow_fold(h, dtg, h->fr_part, momentum_tiles(h), h->red_part + 2048));
+        } else {
+            if ((rc = mdx_momentum(h, dta, dtb, dtg))) return rc;
+            ev_record(h, 5);
+            if (h->cfg.flowrate && (rc = mdx_flowrate_shift(h, dtg, nullptr))) return rc;
+        }
         // the predictor's arrays are the only ones written since the last call (the current
         // fields' ghosts are those of the previous stage's projection or of the upload)
         if (h->cfg.solver == MDX_SOLVER_FFT) MDX_HIP(launch_apply_bc_fft(h, BC_S));

@@ -182,6 +182,10 @@ struct mdx_handle {
     bool use_graphs = true;
     // momentum through the LDS-tiled kernel (MOBYDIFF_MOMENTUM_DIRECT=1: one thread per cell)
     bool tiled_momentum = true;
+    // in a step, the tiled momentum kernel forms the flow-rate partial sums (fr_part, 2 doubles per
+    // tile) and one launch folds them and shifts u* (MOBYDIFF_FUSED_FLOWRATE=0: the separate sum pass)
+    bool fused_flowrate = true;
+    double *fr_part = nullptr;
     // streaming red-black sweeps: z planes marched per thread (0: one cell per thread)
     int sweep_kz = 4;
     // one launch per red+black iteration (halo-recomputing tiles) instead of one per colour
@@ -277,7 +281,9 @@ namespace mdx {
 GridK grid_k(const mdx_handle *h);
 BcK bc_k(const mdx_handle *h);
 
-hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg);
+// frpart: the tiled kernel also writes the flow-rate partial sums of its momentum_tiles(h) tiles
+hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg, double *frpart = nullptr);
+int momentum_tiles(const mdx_handle *h);
 // array sets of the FFT build's apply_bc: the current fields and the predictor's
 enum { BC_N = 1, BC_S = 2, BC_ALL = 3 };
 hipError_t launch_apply_bc_fft(mdx_handle *h, int fields = BC_ALL);
@@ -289,6 +295,8 @@ hipError_t launch_bulk_sum(mdx_handle *h, const double *u, double dtg, double *o
 // constant flow rate: the bulk sums of u*, and the shift u* += du * mu_u (every workgroup of the
 // second launch folds the sums itself); ub, mean mu_u, du to out_dev
 hipError_t launch_flowrate(mdx_handle *h, double dtg, double *out_dev);
+// fold nparts (sum u* h_j, sum mu_u h_j) pairs from part, then u* += du mu_u; ub, mean mu_u, du to out_dev
+hipError_t launch_flow_fold(mdx_handle *h, double dtg, const double *part, int nparts, double *out_dev);
 hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int use_geo,
                                 const double *hx_u, double ymid);
 hipError_t launch_reset_max(mdx_handle *h);

@@ -34,8 +34,9 @@ __device__ inline MomOld mom_old(const double *__restrict__ oru, const double *_
     return m;
 }
 
+// returns the u* it stores (0 where the cell's u is a boundary value it does not update)
 template <bool STRETCH, bool IBM, class Acc>
-__device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j, int k, size_t c, size_t o,
+__device__ inline double momentum_cell(const GridK &g, const Acc &L, int i, int j, int k, size_t c, size_t o,
                                      const MomOld &m, double *__restrict__ us, double *__restrict__ vs,
                                      double *__restrict__ ws, double *__restrict__ oru, double *__restrict__ orv,
                                      double *__restrict__ orw, double dta, double dtb, double dtg, double idx,
@@ -43,6 +44,7 @@ __device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j,
     const int nx0 = g.per[0] ? 1 : 2, ny0 = g.per[1] ? 1 : 2, nz0 = g.per[2] ? 1 : 2;
     const int jp = j + 1, jm = j - 1;
     const double *ihc = g.ihc, *ihv = g.ihv;
+    double ustar = 0.0;
     if (i >= nx0) {
         const double u0 = L.u(0, 0, 0), uip = L.u(1, 0, 0), uim = L.u(-1, 0, 0);
         const double ujp = L.u(0, 1, 0), ujm = L.u(0, -1, 0);
@@ -70,6 +72,7 @@ __device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j,
         if (IBM) s = s * (1.0 / (1.0 + dtg * m.cu));
         us[c] = s;
         oru[o] = rhsu;
+        ustar = s;
     }
     if (j >= ny0) {
         const double v0 = L.v(0, 0, 0), vip = L.v(1, 0, 0), vim = L.v(-1, 0, 0);
@@ -128,6 +131,7 @@ __device__ inline void momentum_cell(const GridK &g, const Acc &L, int i, int j,
         ws[c] = s;
         orw[o] = rhsw;
     }
+    return ustar;
 }
 
 // global-memory operands of the stencil
@@ -149,7 +153,8 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
     double *__restrict__ orv, double *__restrict__ orw, const double *__restrict__ cu,
     const double *__restrict__ cv, const double *__restrict__ cw, int coef_kstride, double dta, double dtb,
     double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
-    double ire, int /* planes per workgroup: the tiled kernel's */) {
+    double ire, int /* planes per workgroup */, double * /* flow-rate partials */) {
+    // (the last two arguments serve the tiled kernel: the launcher passes both kernels one list)
     const Dims d = g.d;
     int tx, ty, tz;
     if (!xcd_tile(d.nx, d.ny, d.nz, BX, BY, tx, ty, tz)) return;
@@ -173,6 +178,9 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
 // TY = 8 rows per workgroup, or 4 on grids with fewer than 2048 such workgroups (measured: 54 against
 // 61 us at 192 x 128 x 96, where 576 workgroups of 512 threads leave a thin last wave; 304 against
 // 312 us at 256 x 192 x 256, where the smaller halo of 8 rows wins)
+// FR: the workgroup also sums the flow-rate terms of its cells, u* h_j and mu_u h_j (the sums
+// bulk_partial_kernel forms), into frpart[2 wg], frpart[2 wg + 1] for the tile wg = tx + nbx (ty +
+// nby tz), so the flow-rate correction needs no pass of its own over u*.
 static constexpr int MTX = 64, MTW = MTX + 2;
 
 struct MomTile {
@@ -186,7 +194,7 @@ struct MomTile {
     __device__ inline double p(int a, int b, int c) const { return c == 0 ? sp[t + a + MTW * b] : pkm; }
 };
 
-template <bool STRETCH, bool IBM, int TY>
+template <bool STRETCH, bool IBM, int TY, bool FR>
 __global__ __launch_bounds__(MTX * TY) void momentum_tiled_kernel(
     GridK g, const double *__restrict__ un, const double *__restrict__ vn,
     const double *__restrict__ wn, const double *__restrict__ pn, double *__restrict__ us,
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(MTX * TY) void momentum_tiled_kernel(
     double *__restrict__ orv, double *__restrict__ orw, const double *__restrict__ cu,
     const double *__restrict__ cv, const double *__restrict__ cw, int coef_kstride, double dta, double dtb,
     double dtg, double idx, double idy, double idz, double idx2, double idy2, double idz2,
-    double ire, int mkz) {
+    double ire, int mkz, double *__restrict__ frpart) {
     constexpr int MPL = MTW * (TY + 2);   // window plane: the tile and its x-y halo
     __shared__ double su[3 * MPL], sv[3 * MPL], sw[3 * MPL], sp[MPL];
     const Dims d = g.d;
@@ -231,6 +239,8 @@ __global__ __launch_bounds__(MTX * TY) void momentum_tiled_kernel(
     // the per-cell operands of plane k0 (the next plane's are fetched while a plane is computed)
     MomOld m{};
     if (active) m = mom_old<IBM>(oru, orv, orw, cu, cv, cw, d.in(i, j, k0), d.g(i, j, k0 * coef_kstride));
+    const double hj = (FR && active) ? (g.ihc ? 1.0 / g.ihc[j] : g.dy) : 0.0;
+    double fa = 0.0, fb = 0.0;
     __syncthreads();
     for (int k = k0; k <= k1; ++k) {
         // the next planes (u, v, w at k+2, p at k+1, the per-cell operands at k+1) are fetched into
@@ -257,8 +267,13 @@ __global__ __launch_bounds__(MTX * TY) void momentum_tiled_kernel(
         double pnext = 0.0;
         if (active) {
             const MomTile L{su, sv, sw, sp, {s0 * MPL, ((s0 + 1) % 3) * MPL, ((s0 + 2) % 3) * MPL}, t, pkm};
-            momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), d.in(i, j, k), m, us, vs, ws, oru, orv, orw, dta,
-                                        dtb, dtg, idx, idy, idz, idx2, idy2, idz2, ire);
+            const double uw = momentum_cell<STRETCH, IBM>(g, L, i, j, k, d.g(i, j, k), d.in(i, j, k), m, us, vs, ws,
+                                                          oru, orv, orw, dta, dtb, dtg, idx, idy, idz, idx2, idy2,
+                                                          idz2, ire);
+            if (FR) {   // x is periodic (the launcher checks), so uw is the u* of every cell
+                fa += uw * hj;
+                fb += (IBM ? 1.0 / (1.0 + dtg * m.cu) : 1.0) * hj;
+            }
             pnext = sp[t];
         }
         __syncthreads();
@@ -279,33 +294,80 @@ __global__ __launch_bounds__(MTX * TY) void momentum_tiled_kernel(
         m = mn;
         __syncthreads();
     }
+    if (FR) {
+        // a fixed order: a butterfly within each wave (every lane ends with the same sums), then
+        // the waves in index order; the window's LDS is free after the march's last barrier
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            fa += __shfl_xor(fa, o);
+            fb += __shfl_xor(fb, o);
+        }
+        constexpr int NW = MTX * TY / 64;
+        if ((tid & 63) == 0) {
+            su[tid >> 6] = fa;
+            su[NW + (tid >> 6)] = fb;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double a = 0.0, b = 0.0;
+            for (int w = 0; w < NW; ++w) {
+                a += su[w];
+                b += su[NW + w];
+            }
+            const size_t nbx = (d.nx + MTX - 1) / MTX, nby = (d.ny + TY - 1) / TY;
+            const size_t wg = (size_t)tx + nbx * ((size_t)ty + nby * (size_t)tz);
+            frpart[2 * wg] = a;
+            frpart[2 * wg + 1] = b;
+        }
+    }
 }
 
-hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg) {
+// rows per tiled workgroup (TY above)
+static int momentum_rows(const mdx_handle *h) {
+    const int nseg = (h->d.nz + h->mom_kz - 1) / h->mom_kz;
+    const long wg8 = (long)((h->d.nx + MTX - 1) / MTX) * ((h->d.ny + 7) / 8) * nseg;
+    return wg8 < 2048 ? 4 : 8;
+}
+
+int momentum_tiles(const mdx_handle *h) {
+    const int ty = momentum_rows(h);
+    return ((h->d.nx + MTX - 1) / MTX) * ((h->d.ny + ty - 1) / ty) * ((h->d.nz + h->mom_kz - 1) / h->mom_kz);
+}
+
+hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg, double *frpart) {
     GridK g = grid_k(h);
     const double dx = h->dx, dy = h->dy, dz = h->dz;
     const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
     const bool tiled = h->tiled_momentum;
     const int mkz = h->mom_kz;
     const int nseg = (h->d.nz + mkz - 1) / mkz;
-    const long wg8 = (long)((h->d.nx + MTX - 1) / MTX) * ((h->d.ny + 7) / 8) * nseg;
-    const int ty = wg8 < 2048 ? 4 : 8;
+    const int ty = momentum_rows(h);
     dim3 blk = tiled ? dim3(MTX, ty, 1) : dim3(BX, BY, 1);
     dim3 grd(tiled ? xcd_grid(h->d.nx, h->d.ny, nseg, MTX, ty) : xcd_grid(h->d.nx, h->d.ny, h->d.nz, BX, BY));
-#define MDX_MOM(S, I)                                                                        \
-    hipLaunchKernelGGL((tiled ? (ty == 4 ? momentum_tiled_kernel<S, I, 4> : momentum_tiled_kernel<S, I, 8>)      \
+    // the partials come from the tiled kernel, over a periodic x (as constant flow rate requires)
+    if (frpart && (!tiled || !h->cfg.periodic[0])) return hipErrorInvalidValue;
+#define MDX_MOM_(S, I, FR)                                                                                     \
+    hipLaunchKernelGGL((tiled ? (ty == 4 ? momentum_tiled_kernel<S, I, 4, FR> : momentum_tiled_kernel<S, I, 8, FR>) \
                               : momentum_kernel<S, I>),                                                        \
                        grd, blk, 0, h->stream,                                                               \
                        g, h->un, h->vn, h->wn,                                                   \
                        h->pn, h->us, h->vs, h->ws, h->oru, h->orv, h->orw, h->cu, h->cv, h->cw,  \
                        h->coef_plane ? 0 : 1, dta, dtb, dtg, 1.0 / dx, 1.0 / dy, 1.0 / dz, 1.0 / dx2, 1.0 / dy2,      \
-                       1.0 / dz2, 1.0 / h->cfg.re, mkz)
+                       1.0 / dz2, 1.0 / h->cfg.re, mkz, frpart)
+#define MDX_MOM(S, I)                \
+    do {                             \
+        if (frpart)                  \
+            MDX_MOM_(S, I, true);    \
+        else                         \
+            MDX_MOM_(S, I, false);   \
+    } while (0)
     if (h->stretched) {
         if (h->ibm_on) MDX_MOM(true, true); else MDX_MOM(true, false);
     } else {
         if (h->ibm_on) MDX_MOM(false, true); else MDX_MOM(false, false);
     }
 #undef MDX_MOM
+#undef MDX_MOM_
     return hipGetLastError();
 }
 
@@ -758,21 +820,22 @@ hipError_t launch_bulk_sum(mdx_handle *h, const double *u, double dtg, double *o
     return hipGetLastError();
 }
 
-// Constant flow rate, second half: every workgroup folds the SUM_BLOCKS partial sums of
-// bulk_partial_kernel as bulk_final_kernel does (so every workgroup holds the same du; workgroup 0
-// also stores ub, mean mu_u and du to out), then u* += du * mu_u on its cells: a 64 x 4 tile of
+// Constant flow rate, second half: every workgroup folds the nparts partial sums (those of
+// bulk_partial_kernel, folded as bulk_final_kernel does, or those of the momentum tiles) in the
+// same order (so every workgroup holds the same du; workgroup 0 also stores ub, mean mu_u and du
+// to out), then u* += du * mu_u on its cells: a 64 x 4 tile of
 // columns over FS_KZ planes, mu_u formed once per column where the coefficients are z-invariant
 // (cplane), else per cell: u* + du * (1 / (1 + dt_gamma c_u)) as the oracle's shift forms it.
 static constexpr int FS_KZ = 16;
 __global__ __launch_bounds__(SUM_THREADS) void flow_shift_fold_kernel(GridK g, double *us, const double *cu,
                                                                       int cplane, double dtg,
-                                                                      const double *__restrict__ part, double norm,
-                                                                      double target, double *out) {
+                                                                      const double *__restrict__ part, int nparts,
+                                                                      double norm, double target, double *out) {
     static_assert(SUM_THREADS == 256, "the fold's lanes are the 64 x 4 threads");
     __shared__ double s1[SUM_THREADS], s2[SUM_THREADS];
     const int t = threadIdx.x + 64 * threadIdx.y;
     double a = 0.0, b = 0.0;
-    for (int q = t; q < SUM_BLOCKS; q += SUM_THREADS) {
+    for (int q = t; q < nparts; q += SUM_THREADS) {
         a += part[2 * q];
         b += part[2 * q + 1];
     }
@@ -817,18 +880,24 @@ __global__ __launch_bounds__(SUM_THREADS) void flow_shift_fold_kernel(GridK g, d
         if (k0 + u <= d.nz) us[d.g(i, j, k0 + u)] = uo[u] + du * mk[u];
 }
 
-// the flow-rate correction of u* in two launches: the partial sums, then the fold and the shift
-hipError_t launch_flowrate(mdx_handle *h, double dtg, double *out_dev) {
+hipError_t launch_flow_fold(mdx_handle *h, double dtg, const double *part, int nparts, double *out_dev) {
     GridK g = grid_k(h);
     const double *cu = h->ibm_on ? h->cu : nullptr;
     const int cplane = h->coef_plane ? 1 : 0;
-    hipLaunchKernelGGL(bulk_partial_kernel, dim3(SUM_BLOCKS), dim3(SUM_THREADS), 0, h->stream, g, h->us, cu, cplane,
-                       dtg, h->red_part);
     const double norm = (double)h->d.nx * (double)h->d.nz * h->cfg.flow_height;
     const unsigned grd = xcd_grid(h->d.nx, h->d.ny, (h->d.nz + FS_KZ - 1) / FS_KZ, 64, 4);
-    hipLaunchKernelGGL(flow_shift_fold_kernel, dim3(grd), dim3(64, 4), 0, h->stream, g, h->us, cu, cplane, dtg,
-                       h->red_part, norm, h->cfg.ub_target, out_dev);
+    hipLaunchKernelGGL(flow_shift_fold_kernel, dim3(grd), dim3(64, 4), 0, h->stream, g, h->us, cu, cplane, dtg, part,
+                       nparts, norm, h->cfg.ub_target, out_dev);
     return hipGetLastError();
+}
+
+// the flow-rate correction of u* in two launches: the partial sums, then the fold and the shift
+hipError_t launch_flowrate(mdx_handle *h, double dtg, double *out_dev) {
+    GridK g = grid_k(h);
+    hipLaunchKernelGGL(bulk_partial_kernel, dim3(SUM_BLOCKS), dim3(SUM_THREADS), 0, h->stream, g, h->us,
+                       h->ibm_on ? h->cu : nullptr, h->coef_plane ? 1 : 0, dtg, h->red_part);
+    const hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_flow_fold(h, dtg, h->red_part, SUM_BLOCKS, out_dev);
 }
 
 // ------------------------------------------------------------------------------- initial field

@@ -24,6 +24,8 @@
  *   mdo_stretched_metrics   NEW: reduces to uniform dy at beta -> 0; closed form tanh map
  *   mdo_initial_field       NEW: golden RNG values (SplitMix64 known-answer vectors)
  *   mdo_flowrate_shift      NEW: bulk velocity equals the target to round-off
+ *   mdo_bulk_u_step         NEW: the step's order of the same sums; closed forms and an exactly
+ *                                rounded sum (math.fsum) on analytic and random fields
  *   mdo_rk3_step            main.f90:66-89 stage order; RK constants step.f90:21-23
  */
 #include "oracle.h"
@@ -969,6 +971,77 @@ static void fr_sums(const mdo_grid *g, const double *u, const double *cu, double
     *S2 = s2[0];
 }
 
+/* The order of the same sums inside a time step (the GPU forms them in its momentum kernel, the
+ * flow-rate control's only pass over u* there): tiles of 64 x ty columns over 8 planes (ty = 4 when
+ * fewer than 2048 tiles of 64 x 8 columns cover the grid, else 8), tile b = tx + nbx (ty_ + nby tz).
+ * In a tile the lane of column (i, j) adds its cells plane by plane; each row of 64 lanes (lanes
+ * beyond nx or ny adding nothing) is folded by a pairwise tree (s[l] += s[l + w], w = 32 ... 1) and
+ * the rows' sums are added in row order.  The tile sums are folded as the partials above (lane t:
+ * tiles t, t + 256, ...; pairwise tree over the 256 lanes). */
+#define FR_TX 64
+#define FR_TKZ 8
+static void fr_sums_step(const mdo_grid *g, const double *u, const double *cu, double dtg, double *S1, double *S2) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const int nbx = (nx + FR_TX - 1) / FR_TX, nbz = (nz + FR_TKZ - 1) / FR_TKZ;
+    const int ty = (long)nbx * ((ny + 7) / 8) * nbz < 2048 ? 4 : 8;
+    const int nby = (ny + ty - 1) / ty, nparts = nbx * nby * nbz;
+    double *p1 = (double *)malloc(sizeof(double) * 2 * (size_t)nparts), *p2 = p1 + nparts;
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nparts; ++b) {
+        const int tx = b % nbx, tr = (b / nbx) % nby, tz = b / (nbx * nby);
+        double a = 0.0, c2 = 0.0;
+        for (int r = 0; r < ty; ++r) {
+            const int j = tr * ty + r + 1;
+            double s1[FR_TX], s2[FR_TX];
+            for (int l = 0; l < FR_TX; ++l) {
+                const int i = tx * FR_TX + l + 1;
+                double x = 0.0, y = 0.0;
+                if (i <= nx && j <= ny) {
+                    const double h = g->ihc ? 1.0 / g->ihc[j] : g->dy;
+                    for (int k = tz * FR_TKZ + 1; k <= (tz + 1) * FR_TKZ && k <= nz; ++k) {
+                        const size_t q = GI(i, j, k);
+                        x += u[q] * h;
+                        y += MUQ(cu, q) * h;
+                    }
+                }
+                s1[l] = x;
+                s2[l] = y;
+            }
+            for (int w = FR_TX / 2; w > 0; w >>= 1)
+                for (int l = 0; l < w; ++l) {
+                    s1[l] += s1[l + w];
+                    s2[l] += s2[l + w];
+                }
+            a += s1[0];
+            c2 += s2[0];
+        }
+        p1[b] = a;
+        p2[b] = c2;
+    }
+    double s1[FR_LANES], s2[FR_LANES];
+    for (int t = 0; t < FR_LANES; ++t) {
+        double a = 0.0, c2 = 0.0;
+        for (int q = t; q < nparts; q += FR_LANES) {
+            a += p1[q];
+            c2 += p2[q];
+        }
+        s1[t] = a;
+        s2[t] = c2;
+    }
+    fr_fold(s1);
+    fr_fold(s2);
+    free(p1);
+    *S1 = s1[0];
+    *S2 = s2[0];
+}
+
+double mdo_bulk_u_step(const mdo_grid *g, const double *u, double height) {
+    const double dtg = 0.0;
+    double s1, s2;
+    fr_sums_step(g, u, NULL, dtg, &s1, &s2);
+    return s1 / ((double)g->nx * (double)g->nz * height);
+}
+
 double mdo_bulk_u(const mdo_grid *g, const double *u, double height) {
     const double dtg = 0.0;
     double s1, s2;
@@ -978,11 +1051,14 @@ double mdo_bulk_u(const mdo_grid *g, const double *u, double height) {
 
 /* Constant flow rate: a uniform x-force increment chosen so that the predictor's bulk velocity
  * equals the target; like the body force in the momentum equation it is damped by mu_u. */
-double mdo_flowrate_shift(const mdo_grid *g, double *us, const double *cu, double dtg,
-                          double target, double height) {
+static double flowrate_shift_with(const mdo_grid *g, double *us, const double *cu, double dtg, double target,
+                                  double height, int step_order) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     double s1, s2;
-    fr_sums(g, us, cu, dtg, &s1, &s2);
+    if (step_order)
+        fr_sums_step(g, us, cu, dtg, &s1, &s2);
+    else
+        fr_sums(g, us, cu, dtg, &s1, &s2);
     const double norm = (double)nx * (double)nz * height;
     const double ub = s1 / norm;
     const double sm = s2 / norm;
@@ -991,6 +1067,11 @@ double mdo_flowrate_shift(const mdo_grid *g, double *us, const double *cu, doubl
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) us[GI(i, j, k)] = us[GI(i, j, k)] + du * MUQ(cu, GI(i, j, k));
     return du;
+}
+
+double mdo_flowrate_shift(const mdo_grid *g, double *us, const double *cu, double dtg, double target,
+                          double height) {
+    return flowrate_shift_with(g, us, cu, dtg, target, height, 0);
 }
 
 /* ------------------------------------------------------------------ immersed boundary */
@@ -1113,7 +1194,7 @@ void mdo_rk3_step(const mdo_grid *g, const mdo_step_opts *o, double dt,
     for (int s = 0; s < 3; ++s) {
         double dta = dt * mdo_rk_alpha[s], dtb = dt * mdo_rk_beta[s], dtg = dt * mdo_rk_gamma[s];
         mdo_momentum(g, un, vn, wn, pn, us, vs, ws, oru, orv, orw, cu, cv, cw, dta, dtb, dtg);
-        if (o->flowrate) mdo_flowrate_shift(g, us, cu, dtg, o->ub_target, o->height);
+        if (o->flowrate) flowrate_shift_with(g, us, cu, dtg, o->ub_target, o->height, 1);
         if (o->solver == 0) {
             mdo_apply_bc_fft(g, un, vn, wn, us, vs, ws, pn);
             mdo_projection_fft(g, dtg, us, vs, ws, cu, cv, cw, un, vn, wn, pn);

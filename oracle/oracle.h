@@ -125,6 +125,8 @@ void mdo_projection_sor(const mdo_grid *g, double omega, double tol, int max_ite
 /* divergence norm, bulk velocity */
 double mdo_div_inf(const mdo_grid *g, const double *un, const double *vn, const double *wn);
 double mdo_bulk_u(const mdo_grid *g, const double *u, double height);
+/* the same bulk velocity in the order of the sums inside a time step (mdo_rk3_step's control) */
+double mdo_bulk_u_step(const mdo_grid *g, const double *u, double height);
 double mdo_flowrate_shift(const mdo_grid *g, double *us, const double *cu, double dtg,
                           double target, double height);
 

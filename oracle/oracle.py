@@ -72,6 +72,7 @@ def lib():
         _lib.mdo_get_cfl.restype = C.c_double
         _lib.mdo_div_inf.restype = C.c_double
         _lib.mdo_bulk_u.restype = C.c_double
+        _lib.mdo_bulk_u_step.restype = C.c_double
         _lib.mdo_flowrate_shift.restype = C.c_double
         _lib.mdo_sor_residual_inf.restype = C.c_double
         _lib.mdo_uniform01.restype = C.c_double
@@ -307,6 +308,12 @@ def div_inf(g, f):
 def bulk_u(g, u, height):
     s = g.struct()
     return lib().mdo_bulk_u(C.byref(s), _ptr(u), C.c_double(height))
+
+
+def bulk_u_step(g, u, height):
+    """bulk_u with the sums in the order the time step's flow-rate control adds them."""
+    s = g.struct()
+    return lib().mdo_bulk_u_step(C.byref(s), _ptr(u), C.c_double(height))
 
 
 def set_ibm_coeff(g, geo, dix, diy, diz):

@@ -261,6 +261,26 @@ def test_flowrate_shift_sets_bulk_exactly():
     assert du != 0
 
 
+@pytest.mark.parametrize("shape", [(8, 16, 4), (70, 13, 11), (192, 128, 96), (256, 128, 256)])
+def test_step_order_bulk_sums_equal_exact_sum(shape):
+    """The time step's order of the flow-rate sums (64-column tiles over 8 planes, 4 or 8 rows per
+    tile; the last two shapes take each row count, the second ragged tiles in x, y and z) against
+    the exactly rounded sum (math.fsum), and a uniform field against its closed form."""
+    nx, ny, nz = shape
+    g = O.Grid(nx, ny, nz, ly=2.0, beta=1.5 if nx < 100 else 0.0)
+    rng = np.random.default_rng(nx + ny + nz)
+    u = rng.standard_normal(g.gshape) + 1.0
+    h = 1.0 / g.ihc[1:-1] if g.beta else np.full(ny, g.dy)
+    terms = (u[1:-1, 1:-1, 1:-1] * h[None, :, None]).ravel()
+    exact = math.fsum(terms.tolist()) / (nx * nz * 2.0)
+    got = O.bulk_u_step(g, u, 2.0)
+    assert abs(got - exact) <= 1e-13 * max(abs(exact), 1.0), (got, exact)
+    assert abs(got - O.bulk_u(g, u, 2.0)) <= 1e-13 * max(abs(exact), 1.0)
+    # u = 1: the sum of the cell heights over each column is ly
+    one = np.ones(g.gshape)
+    assert abs(O.bulk_u_step(g, one, 2.0) - 1.0) <= 1e-13
+
+
 def test_wavy_geometry_bisection_matches_closed_form():
     """For y-neighbours the interface distance has a closed form: |y_wall(x) - y|."""
     g = O.Grid(32, 40, 2, lx=2.0, ly=2.2, lz=0.1, re=1.0)
