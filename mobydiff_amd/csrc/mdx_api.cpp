@@ -385,6 +385,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if (const char *mk = std::getenv("MOBYDIFF_MOM_KZ")) h->mom_kz = std::max(1, std::atoi(mk));
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
     if (const char *ff = std::getenv("MOBYDIFF_FUSED_FLOWRATE")) h->fused_flowrate = ff[0] != '0';
+    if (const char *fb = std::getenv("MOBYDIFF_FUSED_BC")) h->fused_bc = fb[0] != '0';
     if (h->cfg.flowrate && h->tiled_momentum && h->fused_flowrate &&
         (e = dalloc(&h->fr_part, 2 * (size_t)momentum_tiles(h))) != hipSuccess)
         return fail(e, "flow-rate partial buffer");
@@ -1728,11 +1729,17 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
         if (h->cfg.solver == MDX_SOLVER_FFT) MDX_HIP(launch_apply_bc_fft(h, BC_S));
         mdx_solve_stats st{};
         h->proj_count = s;   // RK stage of this projection (initial-guess history)
-        if ((rc = mdx_pressure_projection(h, dtg, &st))) {
+        // the fill follows the projection at once: its kernel may write the boundary conditions
+        h->step_bc = true;
+        h->apply_wrote_bc = false;
+        rc = mdx_pressure_projection(h, dtg, &st);
+        h->step_bc = false;
+        if (rc) {
             if (stats) stats->stage[s] = st;
             return rc;
         }
-        if ((rc = post_bc(h))) return rc;
+        if (h->apply_wrote_bc) h->apply_wrote_bc = false;
+        else if ((rc = post_bc(h))) return rc;
         ev_record(h, 4);
         if (stats) stats->stage[s] = st;
         h->ph_skip = false;

@@ -186,6 +186,13 @@ struct mdx_handle {
     // tile) and one launch folds them and shifts u* (MOBYDIFF_FUSED_FLOWRATE=0: the separate sum pass)
     bool fused_flowrate = true;
     double *fr_part = nullptr;
+    // inside a step (step_bc, set around its projection, whose fill follows at once) the SOR
+    // projection kernel writes the channel's boundary conditions itself (apply_writes_bc;
+    // MOBYDIFF_FUSED_BC=0: the separate fills); apply_wrote_bc: the last projection did.  The
+    // projection called on its own leaves the ghost shell as the reference's projection does.
+    bool fused_bc = true;
+    bool step_bc = false;
+    bool apply_wrote_bc = false;
     // streaming red-black sweeps: z planes marched per thread (0: one cell per thread)
     int sweep_kz = 4;
     // one launch per red+black iteration (halo-recomputing tiles) instead of one per colour
@@ -316,7 +323,9 @@ hipError_t launch_sor_start(mdx_handle *h, const double *prev, double fac, doubl
 hipError_t launch_sor_sweep64(mdx_handle *h, int color, double omega);
 hipError_t launch_sor_residual64(mdx_handle *h, const double *phi);
 // the projection; with hist != nullptr Phi is also copied there (the stage's history)
+// (inside a step: sets h->apply_wrote_bc, whether the launch also wrote the boundary conditions)
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg, double *hist = nullptr);
+bool apply_writes_bc(const mdx_handle *h);
 // parts_only != nullptr: the workgroup maxima stay in max_part (their count to *parts_only) and
 // no fold follows (the caller's next kernel folds them)
 hipError_t launch_defect64(mdx_handle *h, const double *phi_in, const float *e, double *phi_out, float *r32,

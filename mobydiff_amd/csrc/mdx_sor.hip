@@ -289,7 +289,9 @@ hipError_t launch_sor_start(mdx_handle *h, const double *prev, double fac, doubl
 // un, vn, wn on the ghost shell: the values of u*, with the top wall value of v (walled y) on the
 // interior of the j = ny + 1 plane; the boundary conditions that follow the projection overwrite
 // the faces, and edges and corners keep the u* values.  Thread `first` of `stride` threads.
-__device__ inline void proj_ghost(const Dims &d, int walled_y, double vtop, const double *__restrict__ us,
+// bc_faces: the projection's tiles write the faces the boundary conditions set (the interiors of
+// the x and z faces, u and w on the interiors of the y faces), and they are left to them.
+__device__ inline void proj_ghost(const Dims &d, int walled_y, double vtop, bool bc_faces, const double *__restrict__ us,
                                   const double *__restrict__ vs, const double *__restrict__ ws, double *un,
                                   double *vn, double *wn, long long first, long long stride) {
     const long long sz = d.nz + 2;
@@ -316,9 +318,13 @@ __device__ inline void proj_ghost(const Dims &d, int walled_y, double vtop, cons
             k = (int)(q / d.ny) + 1;
         }
         const size_t c = d.g(i, j, k);
-        un[c] = us[c];
-        wn[c] = ws[c];
-        const bool top = walled_y && j == d.ny + 1 && i >= 1 && i <= d.nx && k >= 1 && k <= d.nz;
+        const bool in_i = i >= 1 && i <= d.nx, in_j = j >= 1 && j <= d.ny, in_k = k >= 1 && k <= d.nz;
+        if (bc_faces && in_j && (in_k || in_i)) continue;   // an x or z face's interior
+        if (!(bc_faces && in_i && in_k)) {                   // not a y face's interior
+            un[c] = us[c];
+            wn[c] = ws[c];
+        }
+        const bool top = walled_y && j == d.ny + 1 && in_i && in_k;
         vn[c] = top ? vtop : vs[c];
     }
 }
@@ -349,8 +355,12 @@ static constexpr unsigned PROJ_GHOST_BLOCKS = 256;
 // stores, Phi at k - 1 is the previous cell's Phi, and a z-invariant coefficient table is read once.
 // The `tiles` workgroups of the tiles are followed by PROJ_GHOST_BLOCKS that fill the ghost shell
 // (proj_ghost: cells no tile writes, read from cells no tile reads).  hist, when given, receives
-// a copy of Phi.
-template <bool STRETCH, bool IBM>
+// a copy of Phi.  BC (x and z periodic, Dirichlet u, v, w on the y walls): the tiles also write the
+// ghost values the boundary conditions then set, each from the one interior value it depends on
+// (bc_rb_y, bc_rb_x, bc_rb_z in that order give the same values): the periodic copies of u, v, w,
+// p on the x and z faces and u, w on the y faces (2 V - the wall-adjacent value; ub2 = 2 V of u at
+// the bottom wall, ut2 at the top, wb2 and wt2 of w), so no fill follows the projection.
+template <bool STRETCH, bool IBM, bool BC>
 __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu, const double *__restrict__ phi,
                                                         const double *__restrict__ us,
                                                         const double *__restrict__ vs,
@@ -358,10 +368,11 @@ __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu,
                                                         double *__restrict__ un, double *__restrict__ vn,
                                                         double *__restrict__ wn, double *__restrict__ pn, double idx,
                                                         double idy, double idz, double idt, unsigned tiles,
-                                                        double vtop, double *__restrict__ hist) {
+                                                        double vtop, double *__restrict__ hist, double ub2,
+                                                        double ut2, double wb2, double wt2) {
     const Dims d = g.d;
     if (blockIdx.x >= tiles) {   // the workgroups past the tiles: un, vn, wn on the ghost shell
-        proj_ghost(d, g.per[1] ? 0 : 1, vtop, us, vs, ws, un, vn, wn,
+        proj_ghost(d, g.per[1] ? 0 : 1, vtop, BC, us, vs, ws, un, vn, wn,
                    (long long)(blockIdx.x - tiles) * 256 + threadIdx.x + 64 * threadIdx.y,
                    (long long)(gridDim.x - tiles) * 256);
         return;
@@ -407,17 +418,53 @@ __global__ __launch_bounds__(256) void sor_apply_kernel(GridK g, MuK<double> mu,
             }
             const double p0 = pz[u + 1];
             if (hist) hist[d.in(i, j, k)] = p0;
-            un[c] = uo[u] + mu_u * (pxm[u] - p0) * idx;
-            wn[c] = wo[u] + mu_w * (pz[u] - p0) * idz;
+            const double uv = uo[u] + mu_u * (pxm[u] - p0) * idx;
+            const double wv = wo[u] + mu_w * (pz[u] - p0) * idz;
+            double vv = vbot;
             if (!vwall) {
                 const double gy = STRETCH ? (pym[u] - p0) * ihv : (pym[u] - p0) * idy;
-                vn[c] = vo[u] + mu_v * gy;
-            } else {
-                vn[c] = vbot;
+                vv = vo[u] + mu_v * gy;
             }
-            pn[c] = po[u] + p0 * idt;
+            const double pv = po[u] + p0 * idt;
+            un[c] = uv;
+            wn[c] = wv;
+            vn[c] = vv;
+            pn[c] = pv;
+            if (BC) {
+                auto put = [&](size_t q) {
+                    un[q] = uv;
+                    vn[q] = vv;
+                    wn[q] = wv;
+                    pn[q] = pv;
+                };
+                if (i == d.nx) put(d.g(0, j, k));
+                if (i == 1) put(d.g(d.nx + 1, j, k));
+                if (k == d.nz) put(d.g(i, j, 0));
+                if (k == 1) put(d.g(i, j, d.nz + 1));
+                if (j == 1) {
+                    const size_t q = d.g(i, 0, k);
+                    un[q] = ub2 - uv;
+                    wn[q] = wb2 - wv;
+                }
+                if (j == d.ny) {
+                    const size_t q = d.g(i, d.ny + 1, k);
+                    un[q] = ut2 - uv;
+                    wn[q] = wt2 - wv;
+                }
+            }
         }
     }
+}
+
+// the boundary conditions the projection kernel writes itself inside a step: x and z periodic,
+// Dirichlet u, v, w on both y walls (the channel's), unless MOBYDIFF_FUSED_BC=0
+bool apply_writes_bc(const mdx_handle *h) {
+    const mdx_config &c = h->cfg;
+    if (!h->fused_bc || !c.periodic[0] || !c.periodic[2] || c.periodic[1]) return false;
+    for (int s = 0; s < 2; ++s)
+        for (int q = 1; q <= 3; ++q)
+            if (c.bc_type[bci(2, s, q)] != 0) return false;
+    return true;
 }
 
 hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg, double *hist) {
@@ -427,13 +474,21 @@ hipError_t launch_sor_apply(mdx_handle *h, const double *phi, double dtg, double
     dim3 blk(64, 4), grd(tiles + PROJ_GHOST_BLOCKS);
     const double a = 1.0 / h->dx, b = 1.0 / h->dy, c = 1.0 / h->dz, t = 1.0 / dtg;
     const double vb = h->cfg.bc_value[bci(2, 0, 2)], vt = h->cfg.bc_value[bci(2, 1, 2)];
-#define MDX_APPLY(S, I)                                                                                     \
-    hipLaunchKernelGGL((sor_apply_kernel<S, I>), grd, blk, 0, h->stream, g, mu, phi, h->us, h->vs, h->ws, vb, \
-                       h->un, h->vn, h->wn, h->pn, a, b, c, t, tiles, vt, hist)
+    const bool bc = h->step_bc && apply_writes_bc(h);
+    const double *bv = h->cfg.bc_value;
+    const double ub2 = 2 * bv[bci(2, 0, 1)], ut2 = 2 * bv[bci(2, 1, 1)];
+    const double wb2 = 2 * bv[bci(2, 0, 3)], wt2 = 2 * bv[bci(2, 1, 3)];
+#define MDX_APPLY_(S, I, B)                                                                                    \
+    hipLaunchKernelGGL((sor_apply_kernel<S, I, B>), grd, blk, 0, h->stream, g, mu, phi, h->us, h->vs, h->ws, vb, \
+                       h->un, h->vn, h->wn, h->pn, a, b, c, t, tiles, vt, hist, ub2, ut2, wb2, wt2)
+#define MDX_APPLY(S, I) do { if (bc) MDX_APPLY_(S, I, true); else MDX_APPLY_(S, I, false); } while (0)
     if (h->stretched) { if (h->ibm_on) MDX_APPLY(true, true); else MDX_APPLY(true, false); }
     else { if (h->ibm_on) MDX_APPLY(false, true); else MDX_APPLY(false, false); }
 #undef MDX_APPLY
-    return hipGetLastError();
+#undef MDX_APPLY_
+    const hipError_t e = hipGetLastError();
+    h->apply_wrote_bc = e == hipSuccess && bc;
+    return e;
 }
 
 // ------------------------------------------------------------------------------- z-marching sweeps
