@@ -382,6 +382,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
         }
     }
     if (const char *lo = std::getenv("MOBYDIFF_LDS_OVERLAP")) h->rb_overlap = lo[0] != '0';
+    h->mom_kz = momentum_default_planes(h->d);
     if (const char *mk = std::getenv("MOBYDIFF_MOM_KZ")) h->mom_kz = std::max(1, std::atoi(mk));
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
     if (const char *ff = std::getenv("MOBYDIFF_FUSED_FLOWRATE")) h->fused_flowrate = ff[0] != '0';

@@ -219,7 +219,8 @@ struct mdx_handle {
     bool fused_input = true;
     // mdx_step reports the bulk velocity (a wait for the step's end); mdx_set_step_report
     bool step_ub = true;
-    // z-planes each workgroup of the tiled momentum kernel marches (MOBYDIFF_MOM_KZ)
+    // z-planes each workgroup of the tiled momentum kernel marches (momentum_default_planes, 8 or
+    // 12; MOBYDIFF_MOM_KZ)
     int mom_kz = 8;
     int rb_state = 0;              // 0 not planned, 1 ready, -1 unavailable
     // polls of a neighbour wait before it gives up (MOBYDIFF_LDS_SPIN_LIMIT; a test hook); a wait
@@ -291,6 +292,7 @@ BcK bc_k(const mdx_handle *h);
 // frpart: the tiled kernel also writes the flow-rate partial sums of its momentum_tiles(h) tiles
 hipError_t launch_momentum(mdx_handle *h, double dta, double dtb, double dtg, double *frpart = nullptr);
 int momentum_tiles(const mdx_handle *h);
+int momentum_default_planes(const Dims &d);
 // array sets of the FFT build's apply_bc: the current fields and the predictor's
 enum { BC_N = 1, BC_S = 2, BC_ALL = 3 };
 hipError_t launch_apply_bc_fft(mdx_handle *h, int fields = BC_ALL);

@@ -322,11 +322,27 @@ __global__ __launch_bounds__(MTX * TY) void momentum_tiled_kernel(
     }
 }
 
-// rows per tiled workgroup (TY above)
-static int momentum_rows(const mdx_handle *h) {
-    const int nseg = (h->d.nz + h->mom_kz - 1) / h->mom_kz;
-    const long wg8 = (long)((h->d.nx + MTX - 1) / MTX) * ((h->d.ny + 7) / 8) * nseg;
+// rows per tiled workgroup (TY above) when each marches kz planes
+static int momentum_rows_for(const Dims &d, int kz) {
+    const long wg8 = (long)((d.nx + MTX - 1) / MTX) * ((d.ny + 7) / 8) * ((d.nz + kz - 1) / kz);
     return wg8 < 2048 ? 4 : 8;
+}
+
+static int momentum_rows(const mdx_handle *h) { return momentum_rows_for(h->d, h->mom_kz); }
+
+// The planes a workgroup marches unless configured: 12 where its workgroups fill their last round
+// on the device better than those of 8 planes, else 8.  The device holds 4096 / TY of them at once
+// (256 CUs, 4 waves per SIMD at up to 128 VGPRs).  At 192 x 128 x 96: 768 workgroups, one round
+// (75% of the slots), against 1152 in two (56%): 220-224 against 231-240 us per step of momentum;
+// at 256 x 192 x 256, 8 planes fill every round.  The flow-rate sums follow this tiling.
+int momentum_default_planes(const Dims &d) {
+    auto fill = [&](int kz) {
+        const int ty = momentum_rows_for(d, kz);
+        const long n = (long)((d.nx + MTX - 1) / MTX) * ((d.ny + ty - 1) / ty) * ((d.nz + kz - 1) / kz);
+        const long slots = 4096 / ty, rounds = (n + slots - 1) / slots;
+        return (double)n / (double)(rounds * slots);
+    };
+    return fill(12) > fill(8) ? 12 : 8;
 }
 
 int momentum_tiles(const mdx_handle *h) {

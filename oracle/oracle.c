@@ -972,18 +972,31 @@ static void fr_sums(const mdo_grid *g, const double *u, const double *cu, double
 }
 
 /* The order of the same sums inside a time step (the GPU forms them in its momentum kernel, the
- * flow-rate control's only pass over u* there): tiles of 64 x ty columns over 8 planes (ty = 4 when
- * fewer than 2048 tiles of 64 x 8 columns cover the grid, else 8), tile b = tx + nbx (ty_ + nby tz).
+ * flow-rate control's only pass over u* there): tiles of 64 x ty columns over kz planes, tile b =
+ * tx + nbx (ty_ + nby tz).  ty = 4 when fewer than 2048 tiles of 64 x 8 columns cover the grid,
+ * else 8; kz = 12 when its tiles fill their last round of 4096 / ty better than those of kz = 8,
+ * else 8 (the GPU's momentum tiling).
  * In a tile the lane of column (i, j) adds its cells plane by plane; each row of 64 lanes (lanes
  * beyond nx or ny adding nothing) is folded by a pairwise tree (s[l] += s[l + w], w = 32 ... 1) and
  * the rows' sums are added in row order.  The tile sums are folded as the partials above (lane t:
  * tiles t, t + 256, ...; pairwise tree over the 256 lanes). */
 #define FR_TX 64
-#define FR_TKZ 8
+static int fr_tile_rows(const mdo_grid *g, int kz) {
+    return (long)((g->nx + FR_TX - 1) / FR_TX) * ((g->ny + 7) / 8) * ((g->nz + kz - 1) / kz) < 2048 ? 4 : 8;
+}
+
+static double fr_tile_fill(const mdo_grid *g, int kz) {
+    const int ty = fr_tile_rows(g, kz);
+    const long n = (long)((g->nx + FR_TX - 1) / FR_TX) * ((g->ny + ty - 1) / ty) * ((g->nz + kz - 1) / kz);
+    const long slots = 4096 / ty, rounds = (n + slots - 1) / slots;
+    return (double)n / (double)(rounds * slots);
+}
+
 static void fr_sums_step(const mdo_grid *g, const double *u, const double *cu, double dtg, double *S1, double *S2) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
-    const int nbx = (nx + FR_TX - 1) / FR_TX, nbz = (nz + FR_TKZ - 1) / FR_TKZ;
-    const int ty = (long)nbx * ((ny + 7) / 8) * nbz < 2048 ? 4 : 8;
+    const int kz = fr_tile_fill(g, 12) > fr_tile_fill(g, 8) ? 12 : 8;
+    const int nbx = (nx + FR_TX - 1) / FR_TX, nbz = (nz + kz - 1) / kz;
+    const int ty = fr_tile_rows(g, kz);
     const int nby = (ny + ty - 1) / ty, nparts = nbx * nby * nbz;
     double *p1 = (double *)malloc(sizeof(double) * 2 * (size_t)nparts), *p2 = p1 + nparts;
 #pragma omp parallel for schedule(static)
@@ -998,7 +1011,7 @@ static void fr_sums_step(const mdo_grid *g, const double *u, const double *cu, d
                 double x = 0.0, y = 0.0;
                 if (i <= nx && j <= ny) {
                     const double h = g->ihc ? 1.0 / g->ihc[j] : g->dy;
-                    for (int k = tz * FR_TKZ + 1; k <= (tz + 1) * FR_TKZ && k <= nz; ++k) {
+                    for (int k = tz * kz + 1; k <= (tz + 1) * kz && k <= nz; ++k) {
                         const size_t q = GI(i, j, k);
                         x += u[q] * h;
                         y += MUQ(cu, q) * h;

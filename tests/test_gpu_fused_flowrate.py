@@ -10,8 +10,8 @@ fixed-sweep SOLA solve (constant flow rate needs periodic x; SOR to tolerance pe
 
 The oracle's time step adds the same sums in the same order (mdo_rk3_step), so one RK3 step with
 the fixed-sweep SOLA projection (momentum, boundary fills and SOLA being bit-exact each) equals
-the oracle's step bit for bit, with 4-row tiles (the 192x128x96 wavy channel), 8-row tiles
-(256x128x256) and a stretched grid.
+the oracle's step bit for bit, with 4-row tiles of 12 planes (the 192x128x96 wavy channel), 8-row
+tiles of 8 planes (256x128x256) and a stretched grid.
 """
 import numpy as np
 import pytest

@@ -263,9 +263,10 @@ def test_flowrate_shift_sets_bulk_exactly():
 
 @pytest.mark.parametrize("shape", [(8, 16, 4), (70, 13, 11), (192, 128, 96), (256, 128, 256)])
 def test_step_order_bulk_sums_equal_exact_sum(shape):
-    """The time step's order of the flow-rate sums (64-column tiles over 8 planes, 4 or 8 rows per
-    tile; the last two shapes take each row count, the second ragged tiles in x, y and z) against
-    the exactly rounded sum (math.fsum), and a uniform field against its closed form."""
+    """The time step's order of the flow-rate sums (64-column tiles over 8 or 12 planes, 4 or 8 rows
+    per tile; 192x128x96 takes 12 planes and 4 rows, 256x128x256 8 planes and 8 rows, 70x13x11
+    ragged tiles in x, y and z) against the exactly rounded sum (math.fsum), and a uniform field
+    against its closed form."""
     nx, ny, nz = shape
     g = O.Grid(nx, ny, nz, ly=2.0, beta=1.5 if nx < 100 else 0.0)
     rng = np.random.default_rng(nx + ny + nz)
