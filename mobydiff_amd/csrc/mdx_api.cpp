@@ -387,6 +387,7 @@ extern "C" int mdx_create(const mdx_config *cfg, mdx_handle **out) {
     if (const char *md = std::getenv("MOBYDIFF_MOMENTUM_DIRECT")) h->tiled_momentum = md[0] != '1';
     if (const char *ff = std::getenv("MOBYDIFF_FUSED_FLOWRATE")) h->fused_flowrate = ff[0] != '0';
     if (const char *fb = std::getenv("MOBYDIFF_FUSED_BC")) h->fused_bc = fb[0] != '0';
+    if (const char *fm = std::getenv("MOBYDIFF_FUSED_MU")) h->fused_mu = fm[0] != '0';
     if (h->cfg.flowrate && h->tiled_momentum && h->fused_flowrate &&
         (e = dalloc(&h->fr_part, 2 * (size_t)momentum_tiles(h))) != hipSuccess)
         return fail(e, "flow-rate partial buffer");
@@ -1716,10 +1717,16 @@ extern "C" int mdx_step(mdx_handle *h, double dt, mdx_step_stats *stats) {
         }
         ev_record(h, 0);
         if (h->cfg.flowrate && h->fr_part) {
-            // momentum forms the flow-rate partial sums of u*; one launch folds them and shifts u*
+            // momentum forms the flow-rate partial sums of u*; one launch folds them, shifts u* and
+            // builds the stage's face-coefficient tables if the solve needs new ones
             MDX_HIP(launch_momentum(h, dta, dtb, dtg, h->fr_part));
             ev_record(h, 5);
-            MDX_HIP(launch_flow_fold(h, dtg, h->fr_part, momentum_tiles(h), h->red_part + 2048));
+            // (the tables exist for the SOR solver only)
+            const bool tables = h->fused_mu && h->ibm_on && h->mu_dtg != dtg && h->cfg.solver == MDX_SOLVER_SOR &&
+                                h->mu64[0] && h->mu32[0];
+            MDX_HIP(launch_flow_fold(h, dtg, h->fr_part, momentum_tiles(h), h->red_part + 2048,
+                                     tables ? mu_build(h, dtg) : MuBuild{}));
+            if (tables) h->mu_dtg = dtg;
         } else {
             if ((rc = mdx_momentum(h, dta, dtb, dtg))) return rc;
             ev_record(h, 5);

@@ -54,6 +54,26 @@ struct MuK {
     }
 };
 
+// The tables of a stage: entry q (of n) of mu, mv, mw (FP64) and fu, fv, fw (FP32) from the
+// coefficients; n = 0: nothing to build.  build_mu_kernel and the flow-rate fold both use mu_entry.
+struct MuBuild {
+    size_t n = 0;
+    double dtg = 0.0;
+    const double *cu = nullptr, *cv = nullptr, *cw = nullptr;
+    double *mu = nullptr, *mv = nullptr, *mw = nullptr;
+    float *fu = nullptr, *fv = nullptr, *fw = nullptr;
+};
+__device__ inline void mu_entry(const MuBuild &b, size_t q) {
+    const double a = 1.0 / (1.0 + b.dtg * b.cu[q]), c = 1.0 / (1.0 + b.dtg * b.cv[q]),
+                 e = 1.0 / (1.0 + b.dtg * b.cw[q]);
+    b.mu[q] = a;
+    b.mv[q] = c;
+    b.mw[q] = e;
+    b.fu[q] = (float)a;
+    b.fv[q] = (float)c;
+    b.fw[q] = (float)e;
+}
+
 // XCD-aware tile order for the 3-D stencil kernels.  Workgroups are dispatched round-robin over the
 // 8 XCDs (blockIdx.x % 8); XCD x is given the contiguous tile range [x*per, (x+1)*per), tiles ordered
 // x fastest, then y, then z, so the neighbouring z planes a stencil reads stay in that XCD's L2.
@@ -192,6 +212,9 @@ struct mdx_handle {
     // projection called on its own leaves the ghost shell as the reference's projection does.
     bool fused_bc = true;
     bool step_bc = false;
+    // in a step with constant flow rate the fold-and-shift launch also builds the stage's face
+    // coefficient tables (MOBYDIFF_FUSED_MU=0: build_mu_kernel before the solve)
+    bool fused_mu = true;
     bool apply_wrote_bc = false;
     // streaming red-black sweeps: z planes marched per thread (0: one cell per thread)
     int sweep_kz = 4;
@@ -305,7 +328,9 @@ hipError_t launch_bulk_sum(mdx_handle *h, const double *u, double dtg, double *o
 // second launch folds the sums itself); ub, mean mu_u, du to out_dev
 hipError_t launch_flowrate(mdx_handle *h, double dtg, double *out_dev);
 // fold nparts (sum u* h_j, sum mu_u h_j) pairs from part, then u* += du mu_u; ub, mean mu_u, du to out_dev
-hipError_t launch_flow_fold(mdx_handle *h, double dtg, const double *part, int nparts, double *out_dev);
+// mb: also build those tables (mu_build), once the fold is done
+hipError_t launch_flow_fold(mdx_handle *h, double dtg, const double *part, int nparts, double *out_dev,
+                            const MuBuild &mb = MuBuild{});
 hipError_t launch_initial_field(mdx_handle *h, uint32_t seed, double amp, int use_geo,
                                 const double *hx_u, double ymid);
 hipError_t launch_reset_max(mdx_handle *h);
@@ -318,6 +343,8 @@ static constexpr unsigned MAX_PARTS = 1u << 17;
 // red-black solver
 hipError_t launch_sola_sweep(mdx_handle *h, int parity, double sor, double dtg);
 hipError_t launch_build_mu(mdx_handle *h, double dtg);
+// the stage's tables for launch_build_mu's work (n = 0 without IBM)
+MuBuild mu_build(const mdx_handle *h, double dtg);
 hipError_t launch_sor_prepare(mdx_handle *h, double *rhs_max_slot = nullptr, unsigned long long seq = 0);
 // sor_prepare + Phi = fac * prev (prev null: 0) + the first defect check of a mixed solve, fused
 hipError_t launch_sor_start(mdx_handle *h, const double *prev, double fac, double *host_slot,

@@ -110,26 +110,32 @@ hipError_t launch_sola_sweep(mdx_handle *h, int parity, double sor, double dtg) 
 }
 
 // ------------------------------------------------------------------------------- face coefficients
-__global__ void build_mu_kernel(size_t n, double dtg, const double *__restrict__ cu,
-                                const double *__restrict__ cv, const double *__restrict__ cw,
-                                double *mu, double *mv, double *mw, float *fu, float *fv, float *fw) {
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    double a = 1.0 / (1.0 + dtg * cu[q]), b = 1.0 / (1.0 + dtg * cv[q]), c = 1.0 / (1.0 + dtg * cw[q]);
-    mu[q] = a;
-    mv[q] = b;
-    mw[q] = c;
-    fu[q] = (float)a;
-    fv[q] = (float)b;
-    fw[q] = (float)c;
+__global__ void build_mu_kernel(MuBuild b) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < b.n) mu_entry(b, q);
+}
+
+MuBuild mu_build(const mdx_handle *h, double dtg) {
+    MuBuild b;
+    if (!h->ibm_on) return b;
+    b.n = h->coef_plane ? (size_t)h->d.sx * h->d.sy : h->ng;
+    b.dtg = dtg;
+    b.cu = h->cu;
+    b.cv = h->cv;
+    b.cw = h->cw;
+    b.mu = h->mu64[0];
+    b.mv = h->mu64[1];
+    b.mw = h->mu64[2];
+    b.fu = h->mu32[0];
+    b.fv = h->mu32[1];
+    b.fw = h->mu32[2];
+    return b;
 }
 
 hipError_t launch_build_mu(mdx_handle *h, double dtg) {
-    if (!h->ibm_on) return hipSuccess;
-    size_t n = h->coef_plane ? (size_t)h->d.sx * h->d.sy : h->ng;
-    hipLaunchKernelGGL(build_mu_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, n, dtg,
-                       h->cu, h->cv, h->cw, h->mu64[0], h->mu64[1], h->mu64[2], h->mu32[0], h->mu32[1],
-                       h->mu32[2]);
+    const MuBuild b = mu_build(h, dtg);
+    if (!b.n) return hipSuccess;
+    hipLaunchKernelGGL(build_mu_kernel, dim3((unsigned)((b.n + 255) / 256)), dim3(256), 0, h->stream, b);
     return hipGetLastError();
 }
 

@@ -839,14 +839,16 @@ hipError_t launch_bulk_sum(mdx_handle *h, const double *u, double dtg, double *o
 // Constant flow rate, second half: every workgroup folds the nparts partial sums (those of
 // bulk_partial_kernel, folded as bulk_final_kernel does, or those of the momentum tiles) in the
 // same order (so every workgroup holds the same du; workgroup 0 also stores ub, mean mu_u and du
-// to out), then u* += du * mu_u on its cells: a 64 x 4 tile of
+// to out), builds its share of the stage's face-coefficient tables when mb.n > 0 (grid-stride
+// over all workgroups, padding included), then u* += du * mu_u on its cells: a 64 x 4 tile of
 // columns over FS_KZ planes, mu_u formed once per column where the coefficients are z-invariant
 // (cplane), else per cell: u* + du * (1 / (1 + dt_gamma c_u)) as the oracle's shift forms it.
 static constexpr int FS_KZ = 16;
 __global__ __launch_bounds__(SUM_THREADS) void flow_shift_fold_kernel(GridK g, double *us, const double *cu,
                                                                       int cplane, double dtg,
                                                                       const double *__restrict__ part, int nparts,
-                                                                      double norm, double target, double *out) {
+                                                                      double norm, double target, double *out,
+                                                                      MuBuild mb) {
     static_assert(SUM_THREADS == 256, "the fold's lanes are the 64 x 4 threads");
     __shared__ double s1[SUM_THREADS], s2[SUM_THREADS];
     const int t = threadIdx.x + 64 * threadIdx.y;
@@ -873,6 +875,8 @@ __global__ __launch_bounds__(SUM_THREADS) void flow_shift_fold_kernel(GridK g, d
         out[1] = sm;
         out[2] = du;
     }
+    for (size_t q = (size_t)blockIdx.x * SUM_THREADS + t; q < mb.n; q += (size_t)gridDim.x * SUM_THREADS)
+        mu_entry(mb, q);
     const Dims d = g.d;
     int tx_, ty_, tz_;
     if (!xcd_tile(d.nx, d.ny, (d.nz + FS_KZ - 1) / FS_KZ, 64, 4, tx_, ty_, tz_)) return;
@@ -896,14 +900,15 @@ __global__ __launch_bounds__(SUM_THREADS) void flow_shift_fold_kernel(GridK g, d
         if (k0 + u <= d.nz) us[d.g(i, j, k0 + u)] = uo[u] + du * mk[u];
 }
 
-hipError_t launch_flow_fold(mdx_handle *h, double dtg, const double *part, int nparts, double *out_dev) {
+hipError_t launch_flow_fold(mdx_handle *h, double dtg, const double *part, int nparts, double *out_dev,
+                            const MuBuild &mb) {
     GridK g = grid_k(h);
     const double *cu = h->ibm_on ? h->cu : nullptr;
     const int cplane = h->coef_plane ? 1 : 0;
     const double norm = (double)h->d.nx * (double)h->d.nz * h->cfg.flow_height;
     const unsigned grd = xcd_grid(h->d.nx, h->d.ny, (h->d.nz + FS_KZ - 1) / FS_KZ, 64, 4);
     hipLaunchKernelGGL(flow_shift_fold_kernel, dim3(grd), dim3(64, 4), 0, h->stream, g, h->us, cu, cplane, dtg, part,
-                       nparts, norm, h->cfg.ub_target, out_dev);
+                       nparts, norm, h->cfg.ub_target, out_dev, mb);
     return hipGetLastError();
 }
 

@@ -97,3 +97,39 @@ def test_step_with_flow_rate_bit_exact_with_oracle(shape, A, beta):
         d = float(np.max(np.abs(out[k][sl] - f[k][sl])))
         print(f"  {k}: max |GPU - oracle| {d:.3e}", flush=True)
         assert np.array_equal(out[k][sl], f[k][sl]), k
+
+
+@pytest.mark.parametrize("kind", ["wavy_plane_tables", "random_full_tables"])
+def test_tables_built_in_fold_equal_separate_build(kind, monkeypatch):
+    """In a step with constant flow rate the fold-and-shift launch builds the stage's face
+    coefficient tables (mu_entry, the arithmetic of build_mu_kernel); MOBYDIFF_FUSED_MU=0 builds
+    them in their own launch.  Bit-identical fields and iterations: on the wavy channel (z-invariant
+    coefficients, one plane per table, mixed precision) and with random per-cell coefficients (whole
+    tables, FP64)."""
+    print(f"face tables in the fold: {kind}", flush=True)
+    outs = []
+    for fused in (True, False):
+        monkeypatch.setenv("MOBYDIFF_FUSED_MU", "1" if fused else "0")
+        if kind == "wavy_plane_tables":
+            nx, ny, nz, prec = 192, 128, 96, M.PREC_MIXED
+        else:
+            nx, ny, nz, prec = 32, 24, 16, M.PREC_FP64
+        cfg = M.channel_config(nx, ny, nz, 4.0, 2.2, 2.0, 2000.0, solver=M.SOLVER_SOR, omega=1.8, tol=1e-8,
+                               check_every=10, precision=prec, inner_iters=32, flowrate=True, flow_height=2.0)
+        ch = M.Channel(cfg)
+        if kind == "wavy_plane_tables":
+            geo = M.wavy_geometry(A=0.1)
+            ch.compute_ibm(geo)
+        else:
+            geo = None
+            rng = np.random.default_rng(11)
+            ch.set_ibm(*(np.abs(rng.standard_normal(ch.gshape)) * 20 for _ in range(3)))
+        ch.init_fields(seed=2, amp=0.05, geo=geo, dt=DT)
+        its = [[s.iterations for s in ch.step(DT).stage] for _ in range(2)]
+        outs.append((its, ch.download(("un", "vn", "wn", "pn"))))
+        ch.close()
+    (its_f, f), (its_s, s) = outs
+    print(f"  iterations {its_f} / {its_s}", flush=True)
+    assert its_f == its_s
+    for k in ("un", "vn", "wn", "pn"):
+        assert np.array_equal(f[k], s[k]), k
