@@ -40,6 +40,7 @@ struct mdx_net {
     bool tiled_conv = true;        // LDS-tiled convolutions where the grid allows (bit-identical)
     bool split_conv = true;        // split-K convolutions on small grids the tiled kernel cannot take
     bool fuse_cat = true;          // inference: decoder convolutions read the up-sampling and skip in place
+    bool fuse_pool = true;         // inference: the first pooling in the epilogue of the convolution before it
     std::vector<struct mdx_handle *> users;   // solver handles this network is attached to
 };
 

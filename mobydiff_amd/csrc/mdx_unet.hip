@@ -215,11 +215,15 @@ template <int CIN> struct ConvTile {
     static constexpr size_t BYTES = (size_t)HV * CS * sizeof(bf16);
 };
 
-template <int CIN, int COUT, bool RELU>
+// POOL: the workgroup also writes the 2x2x2 averages of its output tile to pool (W/2 x H/2 x D/2,
+// channels last): after the tile's outputs are stored, their bf16 values pass through LDS and one
+// thread per coarse voxel and channel adds its eight in pool_fwd_kernel's order (dz, dy, dx from
+// 0.f) and scales by 0.125, so pool holds pool_fwd_kernel's values.
+template <int CIN, int COUT, bool RELU, bool POOL = false>
 __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ up,
                                                           int cu, const bf16x8 *__restrict__ wp,
                                                           const float *__restrict__ bias, bf16 *__restrict__ out,
-                                                          int W, int H, int D) {
+                                                          int W, int H, int D, bf16 *__restrict__ pool = nullptr) {
     constexpr int K = 27 * CIN, KC = (K + 31) / 32, NT = COUT / 16, U = CIN / 8;
     constexpr int TZ = ConvTile<CIN>::TZ, HV = ConvTile<CIN>::HV, CS = ConvTile<CIN>::CS;
     extern __shared__ __attribute__((aligned(16))) bf16 tile[];   // HV * CS values
@@ -280,6 +284,7 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
         }
     }
     const int y = y0 + wave;
+    bf16 keep[POOL ? TZ : 1][4];
 #pragma unroll
     for (int m = 0; m < TZ; ++m) {
         const long row = ((long)(z0 + m) * H + y) * W + x0;
@@ -291,8 +296,34 @@ __global__ __launch_bounds__(256) void conv3_tiled_kernel(const bf16 *__restrict
             for (int i = 0; i < 4; ++i) {
                 float val = acc[m][nt][i] + bb;
                 if (RELU) val = val > 0.f ? val : 0.f;
-                out[(row + 4 * g + i) * COUT + ch] = (bf16)val;
+                const bf16 bv = (bf16)val;
+                out[(row + 4 * g + i) * COUT + ch] = bv;
+                if (POOL) keep[POOL ? m : 0][i] = bv;
             }
+        }
+    }
+    if (POOL) {
+        static_assert(!POOL || (COUT == 16 && TZ % 2 == 0 && CTX * CTY / 4 * 16 == 256),
+                      "pooling: 16 channels, whole 2x2x2 cells, one thread per coarse voxel and channel");
+        static_assert(!POOL || TZ * CTY * CTX * 16 <= HV * CS, "the pooled tile fits the input tile's LDS");
+        __syncthreads();   // every wave has read its last input operands
+#pragma unroll
+        for (int m = 0; m < TZ; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                tile[((m * CTY + wave) * CTX + 4 * g + i) * 16 + (lane & 15)] = keep[POOL ? m : 0][i];
+        __syncthreads();
+        const int ch = threadIdx.x & 15, cell = threadIdx.x >> 4;
+        const int xc = cell % (CTX / 2), yc = cell / (CTX / 2);
+        const int Wc = W / 2, Hc = H / 2;
+#pragma unroll
+        for (int zc = 0; zc < TZ / 2; ++zc) {
+            float s = 0.f;
+            for (int dz = 0; dz < 2; ++dz)
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx)
+                        s += (float)tile[(((2 * zc + dz) * CTY + 2 * yc + dy) * CTX + 2 * xc + dx) * 16 + ch];
+            pool[(((long)(z0 / 2 + zc) * Hc + y0 / 2 + yc) * Wc + x0 / 2 + xc) * 16 + ch] = (bf16)(s * 0.125f);
         }
     }
 }
@@ -925,10 +956,21 @@ static float *dBs(mdx_net *n, int l) { return n->grads + n->boff[l]; }
 // 3x3x3 conv (implicit GEMM on MFMA): forward (relu, bias) or input gradient (plain)
 // up != nullptr: the input's channels [0, cu) are the nearest up-sampling of `up` and the rest
 // `in` (conv_in_at); only the tiled and split-K kernels take such an input
+// pool != nullptr: also the output's 2x2x2 averages (the tiled 16 -> 16 ReLU convolution only;
+// hipErrorInvalidValue otherwise, before anything runs)
 static hipError_t conv3(mdx_net *n, int cin, int cout, bool relu, const bf16 *in, const bf16 *wp, const float *bias,
-                        bf16 *out, int lvl, const bf16 *up = nullptr, int cu = 0) {
+                        bf16 *out, int lvl, const bf16 *up = nullptr, int cu = 0, bf16 *pool = nullptr) {
     const int W = n->W[lvl], H = n->H[lvl], D = n->D[lvl];
     const unsigned blocks = (unsigned)(n->V[lvl] / 64);
+    if (pool) {
+        if (!(n->tiled_conv && cin == 16 && cout == 16 && relu && !up && W % CTX == 0 && H % CTY == 0 &&
+              D % ConvTile<16>::TZ == 0))
+            return hipErrorInvalidValue;
+        const unsigned tiles = (unsigned)((W / CTX) * (H / CTY) * (D / ConvTile<16>::TZ));
+        hipLaunchKernelGGL((conv3_tiled_kernel<16, 16, true, true>), dim3(tiles), dim3(256), ConvTile<16>::BYTES,
+                           n->stream, in, nullptr, 0, reinterpret_cast<const bf16x8 *>(wp), bias, out, W, H, D, pool);
+        return hipGetLastError();
+    }
 #define TILED_CASE(CI, CO, R)                                                                                      \
     if (n->tiled_conv && cin == CI && cout == CO && relu == R && W % CTX == 0 && H % CTY == 0 &&                   \
         D % ConvTile<CI>::TZ == 0) {                                                                               \
@@ -1030,8 +1072,16 @@ hipError_t net_forward(mdx_net *n, bool infer) {
     const int W0 = n->W[0], H0 = n->H[0], D0 = n->D[0], W1 = n->W[1], H1 = n->H[1], D1 = n->D[1];
 #define TRY(x) if ((e = (x)) != hipSuccess) return e
     TRY(conv3(n, 16, 16, true, n->x_in, n->wp[0], Bs(n, 0), n->a[0], 0));
-    TRY(conv3(n, 16, 16, true, n->a[0], n->wp[1], Bs(n, 1), n->a[1], 0));
-    hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[1] * 2)), dim3(256), 0, n->stream, n->a[1], n->p1, 16, W0, H0, D0);
+    // inference: the second convolution also writes the first pooling's averages
+    e = infer && n->fuse_pool ? conv3(n, 16, 16, true, n->a[0], n->wp[1], Bs(n, 1), n->a[1], 0, nullptr, 0, n->p1)
+                              : hipErrorInvalidValue;
+    if (e == hipErrorInvalidValue) {
+        TRY(conv3(n, 16, 16, true, n->a[0], n->wp[1], Bs(n, 1), n->a[1], 0));
+        hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[1] * 2)), dim3(256), 0, n->stream, n->a[1], n->p1, 16, W0,
+                           H0, D0);
+    } else {
+        TRY(e);
+    }
     TRY(conv3(n, 16, 32, true, n->p1, n->wp[2], Bs(n, 2), n->a[2], 1));
     TRY(conv3(n, 32, 32, true, n->a[2], n->wp[3], Bs(n, 3), n->a[3], 1));
     hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk(n->V[2] * 4)), dim3(256), 0, n->stream, n->a[3], n->p2, 32, W1, H1, D1);

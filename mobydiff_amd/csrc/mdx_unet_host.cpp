@@ -61,6 +61,7 @@ extern "C" int mdx_net_create_strided(int32_t nx, int32_t ny, int32_t nz, int32_
     if (const char *t = std::getenv("MOBYDIFF_NO_TILED_CONV")) n->tiled_conv = !(t[0] == '1');
     if (const char *t = std::getenv("MOBYDIFF_NO_SPLIT_CONV")) n->split_conv = !(t[0] == '1');
     if (const char *t = std::getenv("MOBYDIFF_NO_CAT_FUSION")) n->fuse_cat = !(t[0] == '1');
+    if (const char *t = std::getenv("MOBYDIFF_NO_POOL_FUSION")) n->fuse_pool = !(t[0] == '1');
     n->stride = stride;
     n->FW = nx; n->FH = ny; n->FD = nz;
     n->FV = (long)nx * ny * nz;

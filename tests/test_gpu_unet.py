@@ -420,3 +420,22 @@ def test_inference_concat_read_in_place(stride, monkeypatch):
         net.close()
     assert np.isfinite(outs[0]).all()
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("stride", [1, 4])
+def test_inference_first_pooling_in_conv_epilogue(stride, monkeypatch):
+    """In inference the convolution before the first pooling also writes the 2x2x2 averages of its
+    output (pool_fwd_kernel's order of addition): the forward output is bit-identical to the run
+    with the separate pooling launch (MOBYDIFF_NO_POOL_FUSION=1)."""
+    rng = np.random.default_rng(29)
+    grid = (NX, NY, NZ) if stride == 1 else (192, 128, 96)
+    n0 = tuple(g // stride for g in grid)
+    x2 = rng.standard_normal(n0[::-1] + (2,)).astype(np.float32)
+    outs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MOBYDIFF_NO_POOL_FUSION", flag)
+        net = M.Net(*grid, seed=23, stride=stride)
+        outs.append(net.forward(x2))
+        net.close()
+    assert np.isfinite(outs[0]).all()
+    assert np.array_equal(outs[0], outs[1])
