@@ -171,7 +171,7 @@ __global__ __launch_bounds__(BX * BY) void momentum_kernel(
 }
 
 // LDS-tiled momentum: a 64 x TY workgroup owns a 64 x TY column of cells and marches mkz planes in z
-// (8 by default; 4, 16 and 32 measured slower).  The u, v, w planes k-1, k, k+1 (with a one-cell
+// (8 or 12, momentum_default_planes; 4, 16 and 32 measured slower).  The u, v, w planes k-1, k, k+1 (with a one-cell
 // x-y halo) and the p plane k live in LDS as a rolling window, so each value is fetched from global
 // memory about 1.3 (x-y halo) x (mkz+2)/mkz (z halo) times instead of once per stencil use (about
 // 30 loads per cell in the direct kernel).
